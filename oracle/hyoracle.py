@@ -1,0 +1,325 @@
+"""ctypes front end for the FP64 CPU oracle (libhyoracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()``
+and bench.py's ``cpu_baseline`` leg, as the checker or the CPU baseline.
+The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhyoracle.so")
+
+_i32p = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+_i8p = C.POINTER(C.c_int8)
+_u64p = C.POINTER(C.c_uint64)
+_f64p = C.POINTER(C.c_double)
+
+
+class OrCode(C.Structure):
+    _fields_ = [("S", C.c_int32), ("code_id", C.c_int32), ("c64_to_sense", C.c_int8 * 64),
+                ("sense_to_c64", C.c_int8 * 64), ("aa", C.c_char * 64), ("nd", C.c_int8 * 4096),
+                ("dpos", C.c_int8 * 12288), ("dfrom", C.c_int8 * 12288), ("dto", C.c_int8 * 12288),
+                ("syn", C.c_uint8 * 4096)]
+
+
+class OrTree(C.Structure):
+    _fields_ = [("num_nodes", C.c_int32), ("root", C.c_int32), ("num_taxa", C.c_int32),
+                ("num_patterns", C.c_int32), ("S", C.c_int32), ("pad0", C.c_int32),
+                ("parent", _i32p), ("child_ptr", _i32p), ("child_idx", _i32p), ("post_order", _i32p),
+                ("leaf_taxon", _i32p), ("is_leaf", _u8p), ("states", _i8p), ("masks", _u64p),
+                ("weights", _f64p)]
+
+
+class OrParams(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("beta", C.c_double), ("theta", C.c_double * 5),
+                ("delta", C.c_double), ("psi", C.c_double)]
+
+
+class OrMixture(C.Structure):
+    _fields_ = [("n_mat", C.c_int32), ("n_class", C.c_int32), ("convention", C.c_int32), ("pad0", C.c_int32),
+                ("omegas", _f64p), ("branch_class", _i32p), ("comp_ptr", _i32p), ("comp_mat", _i32p),
+                ("comp_w", _f64p)]
+
+
+class OrMixtureOut(C.Structure):
+    _fields_ = [("log_likelihood", C.c_double), ("site_ll", _f64p), ("grad_bl", _f64p), ("grad_omega", _f64p),
+                ("grad_w", _f64p), ("grad_theta", _f64p), ("dtau", _f64p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        L.or_code_build.argtypes = [C.c_int32, C.POINTER(OrCode)]
+        L.or_mg94_Q.argtypes = [C.POINTER(OrCode), C.POINTER(OrParams), _f64p, _f64p, _f64p, _f64p]
+        L.or_mg94_decompose.argtypes = [C.c_int32, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.or_transition.argtypes = [C.c_int32, _f64p, _f64p, _f64p, C.c_double, _f64p]
+        L.or_expm_taylor.argtypes = [C.c_int32, _f64p, C.c_double, _f64p]
+        L.or_sym_eigen.argtypes = [C.c_int32, _f64p, _f64p, _f64p]
+        L.or_loglik_from_P.argtypes = [C.POINTER(OrTree), _f64p, _f64p, C.c_int32, C.c_int32, _f64p]
+        L.or_loglik_from_P.restype = C.c_double
+        L.or_mg94_loglik.argtypes = [C.POINTER(OrTree), C.POINTER(OrCode), C.POINTER(OrParams), _f64p, _f64p,
+                                     _f64p, C.c_double, C.c_int32, _f64p]
+        L.or_mg94_loglik.restype = C.c_double
+        L.or_branch_length_gradients.argtypes = [C.POINTER(OrTree), C.POINTER(OrCode), C.POINTER(OrParams),
+                                                 _f64p, _f64p, _f64p, C.c_int32, _f64p]
+        L.or_branch_length_gradients.restype = C.c_double
+        L.or_joint_frechet.argtypes = [C.POINTER(OrTree), C.POINTER(OrCode), C.POINTER(OrParams), _f64p, _f64p,
+                                       _f64p, _u8p, C.c_double, C.c_int32, _f64p, _f64p, _f64p, _f64p]
+        L.or_joint_frechet.restype = C.c_double
+        L.or_mixture_eval.argtypes = [C.POINTER(OrTree), C.POINTER(OrCode), C.POINTER(OrParams), _f64p, _f64p,
+                                      C.POINTER(OrMixture), _f64p, C.c_int32, C.c_int32, C.c_int32,
+                                      C.POINTER(OrMixtureOut)]
+        L.or_mixture_eval.restype = C.c_int32
+        L.or_neumaier_sum.argtypes = [_f64p, C.c_int64]
+        L.or_neumaier_sum.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Code:
+    def __init__(self, code_id: int = 0):
+        self.c = OrCode()
+        if lib().or_code_build(code_id, C.byref(self.c)) != 0:
+            raise ValueError(f"unknown genetic code id {code_id}")
+
+    @property
+    def S(self):
+        return self.c.S
+
+
+def params(alpha=1.0, beta=1.0, theta=(1, 1, 1, 1, 1), delta=0.0, psi=0.0):
+    p = OrParams()
+    p.alpha = alpha
+    p.beta = beta
+    for i in range(5):
+        p.theta[i] = float(theta[i])
+    p.delta = delta
+    p.psi = psi
+    return p
+
+
+class Problem:
+    """Flat tree + compressed-pattern arrays kept alive for the C oracle.
+
+    ``parent``/``children``/``post_order``/``is_leaf`` describe the tree by
+    node id; ``leaf_taxon`` maps node -> alignment row (-1 unmatched);
+    ``states``/``masks`` are [num_patterns, num_taxa].
+    """
+
+    def __init__(self, parent, children, post_order, is_leaf, leaf_taxon, states, masks, weights, S=61):
+        n = len(parent)
+        self.num_nodes = n
+        self.parent = np.ascontiguousarray(parent, dtype=np.int32)
+        cp = [0]
+        ci = []
+        for k in range(n):
+            ci.extend(children[k])
+            cp.append(len(ci))
+        self.child_ptr = np.array(cp, dtype=np.int32)
+        self.child_idx = np.array(ci if ci else [0], dtype=np.int32)
+        self.post_order = np.ascontiguousarray(post_order, dtype=np.int32)
+        self.is_leaf = np.ascontiguousarray(is_leaf, dtype=np.uint8)
+        self.leaf_taxon = np.ascontiguousarray(leaf_taxon, dtype=np.int32)
+        self.states = np.ascontiguousarray(states, dtype=np.int8)
+        self.masks = np.ascontiguousarray(masks, dtype=np.uint64)
+        self.weights = f64(weights)
+        self.root = int(np.where(self.parent < 0)[0][0])
+        self.S = S
+        t = OrTree()
+        t.num_nodes = n
+        t.root = self.root
+        t.num_patterns = int(self.states.shape[0])
+        t.num_taxa = int(self.states.shape[1]) if self.states.ndim == 2 else 0
+        t.S = S
+        t.parent = ptr(self.parent, _i32p)
+        t.child_ptr = ptr(self.child_ptr, _i32p)
+        t.child_idx = ptr(self.child_idx, _i32p)
+        t.post_order = ptr(self.post_order, _i32p)
+        t.leaf_taxon = ptr(self.leaf_taxon, _i32p)
+        t.is_leaf = ptr(self.is_leaf, _u8p)
+        t.states = ptr(self.states, _i8p)
+        t.masks = ptr(self.masks, _u64p)
+        t.weights = ptr(self.weights, _f64p)
+        self.t = t
+        self.num_patterns = t.num_patterns
+
+    @staticmethod
+    def from_refs(tree, aln):
+        """Build from oracle.data_ref TreeRef + AlignmentRef."""
+        parent = [n.parent for n in tree.nodes]
+        children = [list(n.children) for n in tree.nodes]
+        is_leaf = [1 if n.is_leaf else 0 for n in tree.nodes]
+        l2t = tree.leaf_to_taxon(aln.names)
+        return Problem(parent, children, tree.post_order, is_leaf, l2t, aln.states, aln.pmasks, aln.weights,
+                       S=aln.code.S)
+
+
+def mg94_Q(code, p, pos, pi):
+    S = code.S
+    Q = np.zeros((S, S))
+    mu = C.c_double()
+    pos = f64(pos).reshape(12)
+    pi = f64(pi)
+    lib().or_mg94_Q(C.byref(code.c), C.byref(p), ptr(pos, _f64p), ptr(pi, _f64p), ptr(Q, _f64p), C.byref(mu))
+    return Q, mu.value
+
+
+def decompose(Q, pi):
+    S = Q.shape[0]
+    ev = np.zeros(S)
+    U = np.zeros((S, S))
+    V = np.zeros((S, S))
+    Qc = f64(Q)
+    pi = f64(pi)
+    rc = lib().or_mg94_decompose(S, ptr(Qc, _f64p), ptr(pi, _f64p), ptr(ev, _f64p), ptr(U, _f64p), ptr(V, _f64p))
+    if rc != 0:
+        raise RuntimeError("eigensolver did not converge")
+    return ev, U, V
+
+
+def transition(ev, U, V, t):
+    S = len(ev)
+    P = np.zeros((S, S))
+    lib().or_transition(S, ptr(f64(ev), _f64p), ptr(f64(U), _f64p), ptr(f64(V), _f64p), float(t), ptr(P, _f64p))
+    return P
+
+
+def expm_taylor(A, t):
+    n = A.shape[0]
+    out = np.zeros((n, n))
+    Ac = f64(A)
+    lib().or_expm_taylor(n, ptr(Ac, _f64p), float(t), ptr(out, _f64p))
+    return out
+
+
+def sym_eigen(A):
+    n = A.shape[0]
+    Ac = f64(A).copy()
+    ev = np.zeros(n)
+    Z = np.zeros((n, n))
+    rc = lib().or_sym_eigen(n, ptr(Ac, _f64p), ptr(ev, _f64p), ptr(Z, _f64p))
+    if rc != 0:
+        raise RuntimeError("eigensolver did not converge")
+    return ev, Z
+
+
+def loglik_from_P(prob, P, pi, tile=0, threads=1, site=False):
+    Pc = f64(P)
+    pi = f64(pi)
+    s = np.zeros(prob.num_patterns) if site else None
+    ll = lib().or_loglik_from_P(C.byref(prob.t), ptr(Pc, _f64p), ptr(pi, _f64p), tile, threads,
+                                ptr(s, _f64p) if site else None)
+    return (ll, s) if site else ll
+
+
+def mg94_loglik(prob, code, p, pos, pi, bl, branch_scale=1.0, threads=1, site=False):
+    pos = f64(pos).reshape(12)
+    pi = f64(pi)
+    blc = f64(bl)
+    s = np.zeros(prob.num_patterns) if site else None
+    ll = lib().or_mg94_loglik(C.byref(prob.t), C.byref(code.c), C.byref(p), ptr(pos, _f64p), ptr(pi, _f64p),
+                              ptr(blc, _f64p), branch_scale, threads, ptr(s, _f64p) if site else None)
+    return (ll, s) if site else ll
+
+
+def branch_length_gradients(prob, code, p, pos, pi, bl, threads=1):
+    pos = f64(pos).reshape(12)
+    pi = f64(pi)
+    blc = f64(bl)
+    g = np.zeros(prob.num_nodes)
+    ll = lib().or_branch_length_gradients(C.byref(prob.t), C.byref(code.c), C.byref(p), ptr(pos, _f64p),
+                                          ptr(pi, _f64p), ptr(blc, _f64p), threads, ptr(g, _f64p))
+    return ll, g
+
+
+def joint_frechet(prob, code, p, pos, pi, bl, is_test=None, omega_bg=-1.0, threads=1):
+    pos = f64(pos).reshape(12)
+    pi = f64(pi)
+    blc = f64(bl)
+    g = np.zeros(prob.num_nodes)
+    go = C.c_double()
+    gob = C.c_double()
+    gt = np.zeros(5)
+    it = None if is_test is None else np.ascontiguousarray(is_test, dtype=np.uint8)
+    ll = lib().or_joint_frechet(C.byref(prob.t), C.byref(code.c), C.byref(p), ptr(pos, _f64p), ptr(pi, _f64p),
+                                ptr(blc, _f64p), ptr(it, _u8p) if it is not None else None, omega_bg, threads,
+                                ptr(g, _f64p), C.byref(go), C.byref(gob), ptr(gt, _f64p))
+    return {"log_likelihood": ll, "grad_branches": g, "grad_omega": go.value, "grad_omega_bg": gob.value,
+            "grad_theta": gt}
+
+
+class Mixture:
+    """Branch-site mixture specification mirrored by the GPU library."""
+
+    def __init__(self, omegas, branch_class, classes, convention=0):
+        # classes: list of lists of (matrix index, weight)
+        self.omegas = f64(omegas)
+        self.branch_class = np.ascontiguousarray(branch_class, dtype=np.int32)
+        cp = [0]
+        cm, cw = [], []
+        for comps in classes:
+            for m, w in comps:
+                cm.append(m)
+                cw.append(w)
+            cp.append(len(cm))
+        self.comp_ptr = np.array(cp, dtype=np.int32)
+        self.comp_mat = np.array(cm, dtype=np.int32)
+        self.comp_w = f64(cw)
+        self.convention = convention
+        m = OrMixture()
+        m.n_mat = len(self.omegas)
+        m.n_class = len(classes)
+        m.convention = convention
+        m.omegas = ptr(self.omegas, _f64p)
+        m.branch_class = ptr(self.branch_class, _i32p)
+        m.comp_ptr = ptr(self.comp_ptr, _i32p)
+        m.comp_mat = ptr(self.comp_mat, _i32p)
+        m.comp_w = ptr(self.comp_w, _f64p)
+        self.m = m
+        self.n_comp = len(cm)
+
+
+def mixture_eval(prob, code, base, pos, pi, mix, bl, want_grad=True, tile=0, threads=1):
+    pos = f64(pos).reshape(12)
+    pi = f64(pi)
+    blc = f64(bl)
+    N = prob.num_nodes
+    site = np.zeros(max(prob.num_patterns, 1))
+    gbl = np.zeros(N)
+    gom = np.zeros(len(mix.omegas))
+    gw = np.zeros(max(mix.n_comp, 1))
+    gth = np.zeros(5)
+    dtau = np.zeros(N)
+    out = OrMixtureOut()
+    out.site_ll = ptr(site, _f64p)
+    out.grad_bl = ptr(gbl, _f64p)
+    out.grad_omega = ptr(gom, _f64p)
+    out.grad_w = ptr(gw, _f64p)
+    out.grad_theta = ptr(gth, _f64p)
+    out.dtau = ptr(dtau, _f64p)
+    rc = lib().or_mixture_eval(C.byref(prob.t), C.byref(code.c), C.byref(base), ptr(pos, _f64p), ptr(pi, _f64p),
+                               C.byref(mix.m), ptr(blc, _f64p), 1 if want_grad else 0, tile, threads, C.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"or_mixture_eval failed ({rc})")
+    return {"log_likelihood": out.log_likelihood, "site_ll": site[:prob.num_patterns], "grad_bl": gbl,
+            "grad_omega": gom, "grad_w": gw[:mix.n_comp], "grad_theta": gth, "dtau": dtau}
