@@ -1,0 +1,877 @@
+// Host side of the MI355X codon likelihood engine: the C ABI declared in
+// include/hyphy_mi355x.h.  Owns device memory and one HIP stream per handle,
+// turns the caller's tree into a post-order pruning schedule, and launches the
+// kernels of hy_kernels.hip.
+#include "hyphy_mi355x.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "hy_internal.h"
+#include "hyphy/core/genetic_code.hpp"
+
+using namespace hy;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct hy_engine_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool profile = false;
+    hipEvent_t ev[9] = {};
+    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    // model frame
+    bool have_model = false;
+    int code_id = 0, S = 61;
+    DevBuf d_nd, d_dpos, d_dfrom, d_dto, d_syn, d_pos, d_pi, d_pi_custom, d_Phost;
+
+    // problem
+    bool have_problem = false;
+    int num_nodes = 0, n_ext = 0, root = -1, num_taxa = 0, P = 0, n_int = 0, n_vslot = 0, n_branch = 0;
+    std::vector<int32_t> sched, blist;
+    std::vector<int32_t> real_parent;
+    std::vector<uint16_t> codes;  // [num_taxa][P]
+    std::vector<uint64_t> amb;
+    std::vector<double> weights;
+    DevBuf d_sched, d_blist, d_tips, d_amb, d_weights;
+    int T_req = 0, T = 0, n_tiles = 0, P_pad = 0;
+
+    // mixture
+    int n_mat = 1, n_class = 1, n_comp = 1, convention = HY_TIME_RAW, max_comp = 1;
+    std::vector<int32_t> comp_ptr{0, 1}, comp_mat{0}, branch_class;
+    DevBuf d_comp_ptr, d_comp_mat;
+
+    // per-eval buffers
+    ParamLayout pl{};
+    double *h_params = nullptr;
+    size_t h_params_cap = 0;
+    double *h_res = nullptr;
+    size_t h_res_cap = 0;
+    int *h_status = nullptr;
+    DevBuf d_params, d_lam, d_U, d_V, d_mu, d_status, d_tau, d_M, d_dmu;
+    DevBuf d_Prow[2], d_Rsum[2], d_PA[2], d_PTA[2];
+    DevBuf d_D[2], d_Vp[2], d_E, d_G, d_Lroot, d_Eroot, d_site, d_Apart, d_fr, d_res;
+    int n_chunks = 1, tiles_per_chunk = 1;
+
+    size_t device_bytes = 0;
+};
+
+namespace {
+
+int fail(hy_engine h, int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h)
+        h->err = buf;
+    else
+        g_create_error = buf;
+    return code;
+}
+
+#define HIP_TRY(h, expr)                                                                          \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess) return fail(h, HY_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+void release(hy_engine h, DevBuf &b) {
+    if (b.p) {
+        (void)hipFree(b.p);
+        h->device_bytes -= b.bytes;
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int ensure(hy_engine h, DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (b.p && b.bytes >= bytes) return HY_OK;
+    release(h, b);
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(h, HY_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    h->device_bytes += bytes;
+    return HY_OK;
+}
+
+int upload(hy_engine h, DevBuf &b, const void *src, size_t bytes) {
+    int rc = ensure(h, b, bytes);
+    if (rc) return rc;
+    HIP_TRY(h, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return HY_OK;
+}
+
+#define TRY(expr)                \
+    do {                         \
+        int _rc = (expr);        \
+        if (_rc != HY_OK) return _rc; \
+    } while (0)
+
+int choose_tile(int P, int req) {
+    if (req == 16 || req == 32 || req == 64) return req;
+    const int target = 512;  // >= 2 workgroups per CU on 256 CUs
+    if ((P + 63) / 64 >= target) return 64;
+    if ((P + 31) / 32 >= target) return 32;
+    return 16;
+}
+
+void build_layout(hy_engine h) {
+    ParamLayout &pl = h->pl;
+    pl.n_nodes = h->n_ext;
+    pl.n_mat = h->n_mat;
+    pl.n_comp = h->n_comp;
+    int o = 0;
+    pl.off_bl = o;
+    o += h->n_ext;
+    pl.off_alpha = o++;
+    pl.off_omega = o;
+    o += h->n_mat;
+    pl.off_w = o;
+    o += h->n_comp;
+    pl.off_theta = o;
+    o += 5;
+    pl.off_delta = o++;
+    pl.off_psi = o++;
+    pl.total = o;
+}
+
+// (Re)allocate everything that depends on the tile width and precision.
+int prepare_buffers(hy_engine h, int prec) {
+    const int T = choose_tile(h->P, h->T_req);
+    const int ri = prec == HY_PREC_FP64 ? 1 : 0;
+    const size_t rs = prec == HY_PREC_FP64 ? 8 : 4;
+    if (T != h->T) {
+        h->T = T;
+        h->n_tiles = (h->P + T - 1) / T;
+        h->P_pad = h->n_tiles * T;
+        // tip rows: taxa + one all-ones row
+        std::vector<uint16_t> tips((size_t)(h->num_taxa + 1) * h->P_pad, TIP_ONES);
+        for (int t = 0; t < h->num_taxa; ++t)
+            std::memcpy(&tips[(size_t)t * h->P_pad], &h->codes[(size_t)t * h->P], sizeof(uint16_t) * h->P);
+        TRY(upload(h, h->d_tips, tips.data(), tips.size() * sizeof(uint16_t)));
+        std::vector<double> w(h->P_pad, 0.0);
+        std::copy(h->weights.begin(), h->weights.end(), w.begin());
+        TRY(upload(h, h->d_weights, w.data(), w.size() * sizeof(double)));
+        for (int k = 0; k < 2; ++k) {
+            release(h, h->d_D[k]);
+            release(h, h->d_Vp[k]);
+        }
+    }
+    const size_t Pp = (size_t)h->P_pad;
+    TRY(ensure(h, h->d_D[ri], (size_t)h->n_int * Pp * 64 * rs));
+    TRY(ensure(h, h->d_Vp[ri], (size_t)h->n_vslot * Pp * 64 * rs));
+    TRY(ensure(h, h->d_E, (size_t)h->n_int * Pp * 4));
+    TRY(ensure(h, h->d_G, (size_t)h->n_vslot * Pp * 4));
+    TRY(ensure(h, h->d_Lroot, Pp * 8));
+    TRY(ensure(h, h->d_Eroot, Pp * 4));
+    TRY(ensure(h, h->d_site, Pp * 8));
+    const size_t ne = (size_t)h->n_ext;
+    TRY(ensure(h, h->d_Prow[ri], ne * 4096 * rs));
+    TRY(ensure(h, h->d_PA[ri], ne * 4096 * rs));
+    TRY(ensure(h, h->d_PTA[ri], ne * 4096 * rs));
+    TRY(ensure(h, h->d_Rsum[ri], ne * 64 * rs));
+    // adjoint chunking: aim for >= 2048 workgroups
+    int nb = std::max(1, h->n_branch);
+    int nch = std::min(h->n_tiles, std::max(1, (2048 + nb - 1) / nb));
+    h->tiles_per_chunk = (h->n_tiles + nch - 1) / nch;
+    h->n_chunks = (h->n_tiles + h->tiles_per_chunk - 1) / h->tiles_per_chunk;
+    TRY(ensure(h, h->d_Apart, (size_t)nb * h->n_chunks * 4096 * 8));
+    TRY(ensure(h, h->d_fr, (size_t)nb * h->max_comp * 8 * 8));
+    const int nm = h->n_mat;
+    TRY(ensure(h, h->d_lam, (size_t)nm * 64 * 8));
+    TRY(ensure(h, h->d_U, (size_t)nm * 4096 * 8));
+    TRY(ensure(h, h->d_V, (size_t)nm * 4096 * 8));
+    TRY(ensure(h, h->d_mu, (size_t)nm * 8));
+    TRY(ensure(h, h->d_status, (size_t)nm * 4));
+    TRY(ensure(h, h->d_M, (size_t)nm * 6 * 4096 * 8));
+    TRY(ensure(h, h->d_dmu, (size_t)nm * 6 * 8));
+    TRY(ensure(h, h->d_tau, ne * 8));
+    build_layout(h);
+    TRY(ensure(h, h->d_params, (size_t)h->pl.total * 8));
+    const size_t nres = 1 + (size_t)h->num_nodes + h->n_mat + h->n_comp + 5;
+    TRY(ensure(h, h->d_res, nres * 8));
+    if (h->h_params_cap < (size_t)h->pl.total) {
+        if (h->h_params) (void)hipHostFree(h->h_params);
+        HIP_TRY(h, hipHostMalloc((void **)&h->h_params, sizeof(double) * h->pl.total, hipHostMallocDefault));
+        h->h_params_cap = h->pl.total;
+    }
+    if (h->h_res_cap < nres) {
+        if (h->h_res) (void)hipHostFree(h->h_res);
+        HIP_TRY(h, hipHostMalloc((void **)&h->h_res, sizeof(double) * nres, hipHostMallocDefault));
+        h->h_res_cap = nres;
+    }
+    if (!h->h_status) HIP_TRY(h, hipHostMalloc((void **)&h->h_status, sizeof(int) * 64, hipHostMallocDefault));
+    return HY_OK;
+}
+
+void refresh_blist_classes(hy_engine h) {
+    for (int bi = 0; bi < h->n_branch; ++bi) {
+        const int node = h->blist[bi * BR_STRIDE];
+        h->blist[bi * BR_STRIDE + 3] = h->branch_class.empty() ? 0 : h->branch_class[node];
+    }
+}
+
+template <typename R>
+int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
+    const int ri = sizeof(R) == 8 ? 1 : 0;
+    const bool grad = (flags & HY_WANT_GRAD) != 0;
+    hipStream_t s = h->stream;
+    const bool prof = h->profile;
+    if (prof) HIP_TRY(h, hipEventRecord(h->ev[0], s));
+
+    const double *params = (const double *)h->d_params.p;
+    if (!custom_pi) {
+        ModelArgs ma{};
+        ma.nd = (const int8_t *)h->d_nd.p;
+        ma.dpos = (const int8_t *)h->d_dpos.p;
+        ma.dfrom = (const int8_t *)h->d_dfrom.p;
+        ma.dto = (const int8_t *)h->d_dto.p;
+        ma.syn = (const uint8_t *)h->d_syn.p;
+        ma.pos = (const double *)h->d_pos.p;
+        ma.pi = (const double *)h->d_pi.p;
+        ma.S = h->S;
+        ma.params = params;
+        ma.pl = h->pl;
+        ma.lam = (double *)h->d_lam.p;
+        ma.U = (double *)h->d_U.p;
+        ma.V = (double *)h->d_V.p;
+        ma.mu = (double *)h->d_mu.p;
+        ma.status = (int *)h->d_status.p;
+        launch_model(ma, h->n_mat, s);
+        if (prof) HIP_TRY(h, hipEventRecord(h->ev[1], s));
+        PmatArgs<R> pa{};
+        pa.blist = (const int32_t *)h->d_blist.p;
+        pa.n_branch = h->n_branch;
+        pa.comp_ptr = (const int32_t *)h->d_comp_ptr.p;
+        pa.comp_mat = (const int32_t *)h->d_comp_mat.p;
+        pa.convention = h->convention;
+        pa.params = params;
+        pa.pl = h->pl;
+        pa.lam = (const double *)h->d_lam.p;
+        pa.U = (const double *)h->d_U.p;
+        pa.V = (const double *)h->d_V.p;
+        pa.mu = (const double *)h->d_mu.p;
+        pa.S = h->S;
+        pa.tau = (double *)h->d_tau.p;
+        pa.Prow = (R *)h->d_Prow[ri].p;
+        pa.Rsum = (R *)h->d_Rsum[ri].p;
+        pa.PA = (R *)h->d_PA[ri].p;
+        pa.PTA = (R *)h->d_PTA[ri].p;
+        if (h->n_branch > 0) launch_pmat<R>(pa, s);
+    } else {
+        if (prof) HIP_TRY(h, hipEventRecord(h->ev[1], s));
+        PmatArgs<R> pa{};
+        pa.blist = (const int32_t *)h->d_blist.p;
+        pa.n_branch = h->n_branch;
+        pa.S = h->S;
+        pa.Prow = (R *)h->d_Prow[ri].p;
+        pa.Rsum = (R *)h->d_Rsum[ri].p;
+        pa.PA = (R *)h->d_PA[ri].p;
+        pa.PTA = (R *)h->d_PTA[ri].p;
+        if (h->n_branch > 0) launch_pmat_host<R>(pa, (const double *)h->d_Phost.p, s);
+    }
+    if (prof) HIP_TRY(h, hipEventRecord(h->ev[2], s));
+
+    PruneArgs<R> pr{};
+    pr.sched = (const int32_t *)h->d_sched.p;
+    pr.n_sched = (int)(h->sched.size() / SCHED_STRIDE);
+    pr.PA = (const R *)h->d_PA[ri].p;
+    pr.PTA = (const R *)h->d_PTA[ri].p;
+    pr.Prow = (const R *)h->d_Prow[ri].p;
+    pr.Rsum = (const R *)h->d_Rsum[ri].p;
+    pr.tips = (const uint16_t *)h->d_tips.p;
+    pr.amb = (const uint64_t *)h->d_amb.p;
+    pr.P_pad = h->P_pad;
+    pr.n_tiles = h->n_tiles;
+    pr.n_patterns = h->P;
+    pr.pi = custom_pi ? (const double *)h->d_pi_custom.p : (const double *)h->d_pi.p;
+    pr.D = (R *)h->d_D[ri].p;
+    pr.E = (int32_t *)h->d_E.p;
+    pr.V = (R *)h->d_Vp[ri].p;
+    pr.G = (int32_t *)h->d_G.p;
+    pr.Lroot = (double *)h->d_Lroot.p;
+    pr.Eroot = (int32_t *)h->d_Eroot.p;
+    pr.site_ll = (double *)h->d_site.p;
+    launch_inside<R>(pr, h->T, s);
+    if (prof) HIP_TRY(h, hipEventRecord(h->ev[3], s));
+    if (grad) {
+        launch_outside<R>(pr, h->T, s);
+        if (prof) HIP_TRY(h, hipEventRecord(h->ev[4], s));
+        AdjArgs<R> aa{};
+        aa.blist = (const int32_t *)h->d_blist.p;
+        aa.n_branch = h->n_branch;
+        aa.n_chunks = h->n_chunks;
+        aa.tiles_per_chunk = h->tiles_per_chunk;
+        aa.D = pr.D;
+        aa.V = pr.V;
+        aa.E = pr.E;
+        aa.G = pr.G;
+        aa.Eroot = pr.Eroot;
+        aa.Lroot = pr.Lroot;
+        aa.weights = (const double *)h->d_weights.p;
+        aa.tips = pr.tips;
+        aa.amb = pr.amb;
+        aa.P_pad = h->P_pad;
+        aa.n_tiles = h->n_tiles;
+        aa.S = h->S;
+        aa.Apart = (double *)h->d_Apart.p;
+        if (h->n_branch > 0) launch_adjoint<R>(aa, h->T, s);
+        if (prof) HIP_TRY(h, hipEventRecord(h->ev[5], s));
+        ProjArgs pj{};
+        pj.nd = (const int8_t *)h->d_nd.p;
+        pj.dpos = (const int8_t *)h->d_dpos.p;
+        pj.dfrom = (const int8_t *)h->d_dfrom.p;
+        pj.dto = (const int8_t *)h->d_dto.p;
+        pj.syn = (const uint8_t *)h->d_syn.p;
+        pj.pos = (const double *)h->d_pos.p;
+        pj.pi = (const double *)h->d_pi.p;
+        pj.S = h->S;
+        pj.params = params;
+        pj.pl = h->pl;
+        pj.U = (const double *)h->d_U.p;
+        pj.V = (const double *)h->d_V.p;
+        pj.M = (double *)h->d_M.p;
+        pj.dmu = (double *)h->d_dmu.p;
+        launch_proj(pj, h->n_mat, s);
+        FrechetArgs fa{};
+        fa.blist = (const int32_t *)h->d_blist.p;
+        fa.n_branch = h->n_branch;
+        fa.n_chunks = h->n_chunks;
+        fa.max_comp = h->max_comp;
+        fa.comp_ptr = (const int32_t *)h->d_comp_ptr.p;
+        fa.comp_mat = (const int32_t *)h->d_comp_mat.p;
+        fa.params = params;
+        fa.pl = h->pl;
+        fa.Apart = (const double *)h->d_Apart.p;
+        fa.lam = (const double *)h->d_lam.p;
+        fa.U = (const double *)h->d_U.p;
+        fa.V = (const double *)h->d_V.p;
+        fa.M = (const double *)h->d_M.p;
+        fa.tau = (const double *)h->d_tau.p;
+        fa.out = (double *)h->d_fr.p;
+        if (h->n_branch > 0) launch_frechet(fa, s);
+        if (prof) HIP_TRY(h, hipEventRecord(h->ev[6], s));
+    } else if (prof) {
+        for (int k = 4; k <= 6; ++k) HIP_TRY(h, hipEventRecord(h->ev[k], s));
+    }
+    FinalArgs fn{};
+    fn.site_ll = (const double *)h->d_site.p;
+    fn.weights = (const double *)h->d_weights.p;
+    fn.Lroot = (const double *)h->d_Lroot.p;
+    fn.n_patterns = h->P;
+    fn.want_grad = grad ? 1 : 0;
+    fn.blist = (const int32_t *)h->d_blist.p;
+    fn.n_branch = h->n_branch;
+    fn.max_comp = h->max_comp;
+    fn.n_class = h->n_class;
+    fn.n_real_nodes = h->num_nodes;
+    fn.comp_ptr = (const int32_t *)h->d_comp_ptr.p;
+    fn.comp_mat = (const int32_t *)h->d_comp_mat.p;
+    fn.convention = h->convention;
+    fn.params = params;
+    fn.pl = h->pl;
+    fn.fr = (const double *)h->d_fr.p;
+    fn.tau = (const double *)h->d_tau.p;
+    fn.mu = (const double *)h->d_mu.p;
+    fn.dmu = (const double *)h->d_dmu.p;
+    fn.res = (double *)h->d_res.p;
+    launch_final(fn, s);
+    HIP_TRY(h, hipGetLastError());
+    if (prof) HIP_TRY(h, hipEventRecord(h->ev[7], s));
+    return HY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hy_create(int device, hy_engine *out) {
+    if (!out) return fail(nullptr, HY_ERR_ARG, "hy_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(nullptr, HY_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= n) return fail(nullptr, HY_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return fail(nullptr, HY_ERR_HIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, HY_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950 (MI355X)", device,
+                    prop.gcnArchName);
+    hy_engine h = new hy_engine_s();
+    h->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(nullptr, HY_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+    }
+    const char *pe = std::getenv("HY_PROFILE");
+    h->profile = pe && pe[0] == '1';
+    if (h->profile)
+        for (auto &ev : h->ev) (void)hipEventCreate(&ev);
+    *out = h;
+    return HY_OK;
+}
+
+void hy_destroy(hy_engine h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    DevBuf *bufs[] = {&h->d_nd, &h->d_dpos, &h->d_dfrom, &h->d_dto, &h->d_syn, &h->d_pos, &h->d_pi,
+                      &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
+                      &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
+                      &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
+                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_res};
+    for (DevBuf *b : bufs) release(h, *b);
+    for (int k = 0; k < 2; ++k) {
+        release(h, h->d_Prow[k]);
+        release(h, h->d_Rsum[k]);
+        release(h, h->d_PA[k]);
+        release(h, h->d_PTA[k]);
+        release(h, h->d_D[k]);
+        release(h, h->d_Vp[k]);
+    }
+    if (h->h_params) (void)hipHostFree(h->h_params);
+    if (h->h_res) (void)hipHostFree(h->h_res);
+    if (h->h_status) (void)hipHostFree(h->h_status);
+    if (h->profile)
+        for (auto &ev : h->ev) (void)hipEventDestroy(ev);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *hy_last_error(hy_engine h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double *pi) {
+    if (!h) return HY_ERR_ARG;
+    if (!pos_freqs || !pi) return fail(h, HY_ERR_ARG, "hy_set_model: NULL frequencies");
+    HIP_TRY(h, hipSetDevice(h->device));
+    std::shared_ptr<const hyphy::core::GeneticCode> gc;
+    try {
+        gc = hyphy::core::GeneticCode::from_id(code_id);
+    } catch (const std::exception &ex) {
+        return fail(h, HY_ERR_ARG, "%s", ex.what());
+    }
+    const int S = gc->num_sense_codons;
+    if (h->have_problem && S != h->S)
+        return fail(h, HY_ERR_ARG, "genetic code %d has %d sense codons but the problem was set with S=%d", code_id, S,
+                    h->S);
+    std::vector<int8_t> nd(4096, 0), dpos(4096 * 3, -1), dfrom(4096 * 3, -1), dto(4096 * 3, -1);
+    std::vector<uint8_t> syn(4096, 0);
+    for (int i = 0; i < S; ++i)
+        for (int j = 0; j < S; ++j) {
+            if (i == j) continue;
+            const auto &d = gc->diff_matrix[i][j];
+            const int k = i * 64 + j;
+            nd[k] = d.num_diffs;
+            syn[k] = d.synonymous ? 1 : 0;
+            for (int q = 0; q < 3; ++q) {
+                dpos[3 * k + q] = d.diff_pos[q];
+                dfrom[3 * k + q] = d.diff_from[q];
+                dto[3 * k + q] = d.diff_to[q];
+            }
+        }
+    std::vector<double> pp(64, 0.0);
+    for (int i = 0; i < S; ++i) {
+        if (!(pi[i] >= 0.0)) return fail(h, HY_ERR_ARG, "hy_set_model: pi[%d] is negative or NaN", i);
+        pp[i] = pi[i];
+    }
+    for (int k = 0; k < 12; ++k)
+        if (!(pos_freqs[k] >= 0.0)) return fail(h, HY_ERR_ARG, "hy_set_model: pos_freqs[%d] is negative or NaN", k);
+    TRY(upload(h, h->d_nd, nd.data(), nd.size()));
+    TRY(upload(h, h->d_dpos, dpos.data(), dpos.size()));
+    TRY(upload(h, h->d_dfrom, dfrom.data(), dfrom.size()));
+    TRY(upload(h, h->d_dto, dto.data(), dto.size()));
+    TRY(upload(h, h->d_syn, syn.data(), syn.size()));
+    TRY(upload(h, h->d_pos, pos_freqs, 12 * sizeof(double)));
+    TRY(upload(h, h->d_pi, pp.data(), 64 * sizeof(double)));
+    h->code_id = code_id;
+    h->S = S;
+    h->have_model = true;
+    return HY_OK;
+}
+
+int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int32_t *child_ptr,
+                   const int32_t *child_idx, const int32_t *leaf_taxon, int num_taxa, int num_patterns,
+                   const int8_t *states, const uint64_t *masks, const double *weights, int S) {
+    if (!h) return HY_ERR_ARG;
+    HIP_TRY(h, hipSetDevice(h->device));
+    if (num_nodes < 1 || !parent || !child_ptr || !leaf_taxon)
+        return fail(h, HY_ERR_ARG, "hy_set_problem: empty tree or NULL topology arrays");
+    if (num_patterns < 0 || num_taxa < 0 || S < 1 || S > 64)
+        return fail(h, HY_ERR_ARG, "hy_set_problem: bad sizes (patterns=%d taxa=%d S=%d)", num_patterns, num_taxa, S);
+    if (num_patterns > 0 && (!states || !masks || !weights))
+        return fail(h, HY_ERR_ARG, "hy_set_problem: NULL pattern arrays");
+    if (h->have_model && S != h->S) return fail(h, HY_ERR_ARG, "hy_set_problem: S=%d differs from the model's %d", S, h->S);
+    // --- topology checks ---
+    int root = -1;
+    for (int n = 0; n < num_nodes; ++n) {
+        if (parent[n] < 0) {
+            if (root >= 0) return fail(h, HY_ERR_ARG, "hy_set_problem: more than one root (%d and %d)", root, n);
+            root = n;
+        } else if (parent[n] >= num_nodes) {
+            return fail(h, HY_ERR_ARG, "hy_set_problem: parent[%d]=%d out of range", n, parent[n]);
+        }
+    }
+    if (root < 0) return fail(h, HY_ERR_ARG, "hy_set_problem: no root (parent == -1)");
+    if (child_ptr[0] != 0) return fail(h, HY_ERR_ARG, "hy_set_problem: child_ptr[0] must be 0");
+    std::vector<std::vector<int32_t>> ch(num_nodes);
+    for (int n = 0; n < num_nodes; ++n) {
+        if (child_ptr[n + 1] < child_ptr[n]) return fail(h, HY_ERR_ARG, "hy_set_problem: child_ptr not monotone");
+        for (int k = child_ptr[n]; k < child_ptr[n + 1]; ++k) {
+            const int c = child_idx[k];
+            if (c < 0 || c >= num_nodes || parent[c] != n)
+                return fail(h, HY_ERR_ARG, "hy_set_problem: child %d of node %d inconsistent with parent[]", c, n);
+            ch[n].push_back(c);
+        }
+    }
+    if (child_ptr[num_nodes] != num_nodes - 1)
+        return fail(h, HY_ERR_ARG, "hy_set_problem: %d child links for %d nodes (a tree has n-1)", child_ptr[num_nodes],
+                    num_nodes);
+    for (int n = 0; n < num_nodes; ++n)
+        if (ch[n].empty() && (leaf_taxon[n] < -1 || leaf_taxon[n] >= num_taxa))
+            return fail(h, HY_ERR_ARG, "hy_set_problem: leaf_taxon[%d]=%d out of range", n, leaf_taxon[n]);
+    // --- resolve polytomies wider than MAXC with identity edges ---
+    std::vector<int32_t> ext_parent(parent, parent + num_nodes);
+    std::vector<uint8_t> ident(num_nodes, 0);
+    for (size_t u = 0; u < ch.size(); ++u) {
+        if ((int)ch[u].size() <= MAXC) continue;
+        const int a = (int)ch.size();
+        std::vector<int32_t> rest(ch[u].begin() + (MAXC - 1), ch[u].end());
+        ch[u].resize(MAXC - 1);
+        ch[u].push_back(a);
+        ch.push_back(rest);
+        ext_parent.push_back((int32_t)u);
+        ident.push_back(1);
+        for (int c : ch[a]) ext_parent[c] = a;
+    }
+    const int n_ext = (int)ch.size();
+    // --- post-order (children in order) ---
+    std::vector<int32_t> post;
+    post.reserve(n_ext);
+    {
+        std::vector<std::pair<int, int>> st;
+        st.push_back({root, 0});
+        std::vector<uint8_t> seen(n_ext, 0);
+        while (!st.empty()) {
+            auto &[node, k] = st.back();
+            if (k == 0) {
+                if (seen[node]) return fail(h, HY_ERR_ARG, "hy_set_problem: cycle at node %d", node);
+                seen[node] = 1;
+            }
+            if (k < (int)ch[node].size()) {
+                const int c = ch[node][k++];
+                st.push_back({c, 0});
+            } else {
+                post.push_back(node);
+                st.pop_back();
+            }
+        }
+        if ((int)post.size() != n_ext) return fail(h, HY_ERR_ARG, "hy_set_problem: tree is not connected");
+    }
+    std::vector<int32_t> int_slot(n_ext, -1), vslot(n_ext, -1);
+    int n_int = 0, n_v = 0;
+    for (int u : post) {
+        if (!ch[u].empty()) int_slot[u] = n_int++;
+        if (u != root) vslot[u] = n_v++;
+    }
+    auto ref_of = [&](int c) -> int32_t {
+        if (ch[c].empty()) {
+            const int row = (c < num_nodes && leaf_taxon[c] >= 0) ? leaf_taxon[c] : num_taxa;
+            return (KIND_TIP << 28) | row;
+        }
+        return ((ident[c] ? KIND_IDENT : KIND_INT) << 28) | int_slot[c];
+    };
+    std::vector<int32_t> sched;
+    for (int u : post) {
+        if (ch[u].empty()) continue;
+        int32_t e[SCHED_STRIDE] = {0};
+        e[0] = u;
+        e[1] = int_slot[u];
+        e[2] = (int32_t)ch[u].size();
+        e[3] = (u == root) ? -1 : vslot[u];
+        e[4] = ident[u];
+        for (size_t c = 0; c < ch[u].size(); ++c) {
+            const int cn = ch[u][c];
+            e[6 + 3 * c] = cn;
+            e[7 + 3 * c] = ref_of(cn);
+            e[8 + 3 * c] = vslot[cn];
+        }
+        sched.insert(sched.end(), e, e + SCHED_STRIDE);
+    }
+    if (sched.empty()) {
+        // single-node tree: root is a leaf; prune it as an internal node with no children
+        int32_t e[SCHED_STRIDE] = {0};
+        e[0] = root;
+        e[1] = 0;
+        e[2] = 0;
+        e[3] = -1;
+        sched.insert(sched.end(), e, e + SCHED_STRIDE);
+        n_int = 1;
+        int_slot[root] = 0;
+    }
+    std::vector<int32_t> blist;
+    for (int b = 0; b < num_nodes; ++b) {
+        if (b == root) continue;
+        blist.push_back(b);
+        blist.push_back(vslot[b]);
+        blist.push_back(ref_of(b));
+        blist.push_back(0);
+    }
+    // --- tip codes ---
+    std::vector<uint16_t> codes((size_t)num_taxa * std::max(num_patterns, 0));
+    std::vector<uint64_t> amb;
+    std::unordered_map<uint64_t, int> amb_idx;
+    const uint64_t all = S >= 64 ? ~uint64_t(0) : ((uint64_t(1) << S) - 1);
+    for (int p = 0; p < num_patterns; ++p)
+        for (int t = 0; t < num_taxa; ++t) {
+            const int8_t st = states[(size_t)p * num_taxa + t];
+            const uint64_t m = masks[(size_t)p * num_taxa + t];
+            uint16_t code;
+            if (st >= 0 && st < S) {
+                code = (uint16_t)st;
+            } else if (m == 0 || (m & all) == all) {
+                code = TIP_ONES;
+            } else {
+                auto it = amb_idx.find(m & all);
+                int k;
+                if (it == amb_idx.end()) {
+                    k = (int)amb.size();
+                    if (k >= 0x7FFF) return fail(h, HY_ERR_UNSUPPORTED, "more than 32766 distinct ambiguity masks");
+                    amb_idx[m & all] = k;
+                    amb.push_back(m & all);
+                } else {
+                    k = it->second;
+                }
+                code = (uint16_t)(TIP_AMB | k);
+            }
+            codes[(size_t)t * num_patterns + p] = code;
+        }
+    for (int p = 0; p < num_patterns; ++p)
+        if (!(weights[p] >= 0.0)) return fail(h, HY_ERR_ARG, "hy_set_problem: weight[%d] negative or NaN", p);
+    if (amb.empty()) amb.push_back(0);
+    // --- commit ---
+    h->num_nodes = num_nodes;
+    h->n_ext = n_ext;
+    h->root = root;
+    h->num_taxa = num_taxa;
+    h->P = num_patterns;
+    h->S = S;
+    h->n_int = n_int;
+    h->n_vslot = std::max(n_v, 1);
+    h->n_branch = (int)(blist.size() / BR_STRIDE);
+    h->sched = std::move(sched);
+    h->blist = std::move(blist);
+    h->real_parent.assign(parent, parent + num_nodes);
+    h->codes = std::move(codes);
+    h->amb = std::move(amb);
+    h->weights.assign(weights, weights + num_patterns);
+    h->T = 0;
+    h->branch_class.assign(num_nodes, 0);
+    h->n_mat = 1;
+    h->n_class = 1;
+    h->n_comp = 1;
+    h->max_comp = 1;
+    h->convention = HY_TIME_RAW;
+    h->comp_ptr = {0, 1};
+    h->comp_mat = {0};
+    refresh_blist_classes(h);
+    TRY(upload(h, h->d_sched, h->sched.data(), h->sched.size() * 4));
+    TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), 4) * 4));
+    TRY(upload(h, h->d_amb, h->amb.data(), h->amb.size() * 8));
+    TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
+    TRY(upload(h, h->d_comp_mat, h->comp_mat.data(), h->comp_mat.size() * 4));
+    h->have_problem = true;
+    return HY_OK;
+}
+
+int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_class, const int32_t *comp_ptr,
+                   const int32_t *comp_mat, int time_convention) {
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem) return fail(h, HY_ERR_STATE, "hy_set_mixture before hy_set_problem");
+    if (n_mat < 1 || n_mat > 48 || n_class < 1 || n_class > 64 || !comp_ptr || !comp_mat)
+        return fail(h, HY_ERR_ARG, "hy_set_mixture: bad sizes (n_mat=%d n_class=%d)", n_mat, n_class);
+    if (time_convention != HY_TIME_RAW && time_convention != HY_TIME_CONV)
+        return fail(h, HY_ERR_ARG, "hy_set_mixture: unknown time convention %d", time_convention);
+    if (comp_ptr[0] != 0) return fail(h, HY_ERR_ARG, "hy_set_mixture: comp_ptr[0] must be 0");
+    int max_comp = 0;
+    for (int c = 0; c < n_class; ++c) {
+        const int k = comp_ptr[c + 1] - comp_ptr[c];
+        if (k < 1) return fail(h, HY_ERR_ARG, "hy_set_mixture: class %d has no component", c);
+        max_comp = std::max(max_comp, k);
+    }
+    const int n_comp = comp_ptr[n_class];
+    if (n_comp > 192) return fail(h, HY_ERR_ARG, "hy_set_mixture: at most 192 components");
+    for (int j = 0; j < n_comp; ++j)
+        if (comp_mat[j] < 0 || comp_mat[j] >= n_mat)
+            return fail(h, HY_ERR_ARG, "hy_set_mixture: comp_mat[%d]=%d out of range", j, comp_mat[j]);
+    std::vector<int32_t> bc(h->num_nodes, 0);
+    if (branch_class) {
+        for (int n = 0; n < h->num_nodes; ++n) {
+            if (n == h->root) continue;
+            if (branch_class[n] < 0 || branch_class[n] >= n_class)
+                return fail(h, HY_ERR_ARG, "hy_set_mixture: branch_class[%d]=%d out of range", n, branch_class[n]);
+            bc[n] = branch_class[n];
+        }
+    } else if (n_class != 1) {
+        return fail(h, HY_ERR_ARG, "hy_set_mixture: branch_class required when n_class > 1");
+    }
+    h->n_mat = n_mat;
+    h->n_class = n_class;
+    h->n_comp = n_comp;
+    h->max_comp = max_comp;
+    h->convention = time_convention;
+    h->comp_ptr.assign(comp_ptr, comp_ptr + n_class + 1);
+    h->comp_mat.assign(comp_mat, comp_mat + n_comp);
+    h->branch_class = bc;
+    refresh_blist_classes(h);
+    TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), 4) * 4));
+    TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
+    TRY(upload(h, h->d_comp_mat, h->comp_mat.data(), h->comp_mat.size() * 4));
+    return HY_OK;
+}
+
+int hy_set_tile(hy_engine h, int patterns_per_tile) {
+    if (!h) return HY_ERR_ARG;
+    if (patterns_per_tile != 0 && patterns_per_tile != 16 && patterns_per_tile != 32 && patterns_per_tile != 64)
+        return fail(h, HY_ERR_ARG, "hy_set_tile: tile must be 0, 16, 32 or 64 (got %d)", patterns_per_tile);
+    h->T_req = patterns_per_tile;
+    return HY_OK;
+}
+
+static int finish_eval(hy_engine h, bool check_status, double *out_logL, double *out_site_ll, double *out_grad_bl,
+                       double *out_grad_omega, double *out_grad_w, double *out_grad_theta, bool grad) {
+    const size_t nres = 1 + (size_t)h->num_nodes + h->n_mat + h->n_comp + 5;
+    HIP_TRY(h, hipMemcpyAsync(h->h_res, h->d_res.p, nres * 8, hipMemcpyDeviceToHost, h->stream));
+    if (check_status)
+        HIP_TRY(h, hipMemcpyAsync(h->h_status, h->d_status.p, sizeof(int) * h->n_mat, hipMemcpyDeviceToHost, h->stream));
+    if (out_site_ll && h->P > 0)
+        HIP_TRY(h, hipMemcpyAsync(out_site_ll, h->d_site.p, sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->profile) {
+        float ms;
+        for (int k = 0; k < 7; ++k) {
+            (void)hipEventElapsedTime(&ms, h->ev[k], h->ev[k + 1]);
+            h->timings[k] = ms;
+        }
+        (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[7]);
+        h->timings[7] = ms;
+    }
+    if (check_status)
+        for (int m = 0; m < h->n_mat; ++m)
+            if (h->h_status[m] < 0)
+                return fail(h, HY_ERR_NUMERIC, "symmetric eigensolver did not converge for rate matrix %d", m);
+    const double *r = h->h_res;
+    if (out_logL) *out_logL = r[0];
+    if (grad) {
+        const double *gbl = r + 1, *gom = gbl + h->num_nodes, *gw = gom + h->n_mat, *gth = gw + h->n_comp;
+        if (out_grad_bl) std::memcpy(out_grad_bl, gbl, sizeof(double) * h->num_nodes);
+        if (out_grad_omega) std::memcpy(out_grad_omega, gom, sizeof(double) * h->n_mat);
+        if (out_grad_w) std::memcpy(out_grad_w, gw, sizeof(double) * h->n_comp);
+        if (out_grad_theta) std::memcpy(out_grad_theta, gth, sizeof(double) * 5);
+    }
+    return HY_OK;
+}
+
+int hy_eval(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
+            const double *comp_w, const double *theta, double delta, double psi, double *out_logL,
+            double *out_site_ll, double *out_grad_bl, double *out_grad_omega, double *out_grad_w,
+            double *out_grad_theta) {
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "hy_eval before hy_set_problem/hy_set_model");
+    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
+        return fail(h, HY_ERR_ARG, "hy_eval: unknown precision %d", precision);
+    if (!bl || !omegas || !comp_w || !theta) return fail(h, HY_ERR_ARG, "hy_eval: NULL parameter array");
+    if (h->P == 0) {
+        if (out_logL) *out_logL = 0.0;
+        if (flags & HY_WANT_GRAD) {
+            if (out_grad_bl) std::fill(out_grad_bl, out_grad_bl + h->num_nodes, 0.0);
+            if (out_grad_omega) std::fill(out_grad_omega, out_grad_omega + h->n_mat, 0.0);
+            if (out_grad_w) std::fill(out_grad_w, out_grad_w + h->n_comp, 0.0);
+            if (out_grad_theta) std::fill(out_grad_theta, out_grad_theta + 5, 0.0);
+        }
+        return HY_OK;
+    }
+    HIP_TRY(h, hipSetDevice(h->device));
+    TRY(prepare_buffers(h, precision));
+    double *hp = h->h_params;
+    const ParamLayout &pl = h->pl;
+    for (int n = 0; n < h->n_ext; ++n) hp[pl.off_bl + n] = (n < h->num_nodes) ? bl[n] : 0.0;
+    hp[pl.off_alpha] = alpha;
+    for (int m = 0; m < h->n_mat; ++m) hp[pl.off_omega + m] = omegas[m];
+    for (int j = 0; j < h->n_comp; ++j) hp[pl.off_w + j] = comp_w[j];
+    for (int k = 0; k < 5; ++k) hp[pl.off_theta + k] = theta[k];
+    hp[pl.off_delta] = delta;
+    hp[pl.off_psi] = psi;
+    HIP_TRY(h, hipMemcpyAsync(h->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, h->stream));
+    const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
+    if (rc) return rc;
+    return finish_eval(h, true, out_logL, out_site_ll, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta,
+                       (flags & HY_WANT_GRAD) != 0);
+}
+
+int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
+                     double *out_site_ll) {
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem) return fail(h, HY_ERR_STATE, "hy_loglik_from_P before hy_set_problem");
+    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
+        return fail(h, HY_ERR_ARG, "hy_loglik_from_P: unknown precision %d", precision);
+    if (!P || !pi) return fail(h, HY_ERR_ARG, "hy_loglik_from_P: NULL P or pi");
+    if (h->P == 0) {
+        if (out_logL) *out_logL = 0.0;
+        return HY_OK;
+    }
+    HIP_TRY(h, hipSetDevice(h->device));
+    TRY(prepare_buffers(h, precision));
+    std::vector<double> pp(64, 0.0);
+    for (int i = 0; i < h->S; ++i) pp[i] = pi[i];
+    TRY(upload(h, h->d_pi_custom, pp.data(), 64 * sizeof(double)));
+    TRY(upload(h, h->d_Phost, P, sizeof(double) * (size_t)h->num_nodes * h->S * h->S));
+    const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, HY_WANT_LOGL, pp.data())
+                                             : run_eval<float>(h, HY_WANT_LOGL, pp.data());
+    if (rc) return rc;
+    return finish_eval(h, false, out_logL, out_site_ll, nullptr, nullptr, nullptr, nullptr, false);
+}
+
+int hy_last_timings(hy_engine h, double *ms8) {
+    if (!h || !ms8) return HY_ERR_ARG;
+    if (!h->profile) return fail(h, HY_ERR_STATE, "timings need HY_PROFILE=1 at hy_create");
+    std::memcpy(ms8, h->timings, sizeof(h->timings));
+    return HY_OK;
+}
+
+int hy_device_bytes(hy_engine h, size_t *out) {
+    if (!h || !out) return HY_ERR_ARG;
+    *out = h->device_bytes;
+    return HY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+// Internal interface between the host engine (hy_engine.cpp) and the HIP
+// kernels (hy_kernels.hip).  Not part of the public C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hy {
+
+// Child / branch reference encoding: (kind << 28) | index.
+constexpr int KIND_INT = 0;    // internal node, index = D slot, edge matrix P(node)
+constexpr int KIND_TIP = 1;    // leaf, index = tip row, edge matrix P(node)
+constexpr int KIND_IDENT = 2;  // internal node joined by an identity edge (resolved polytomy)
+constexpr int REF_MASK = 0x0FFFFFFF;
+constexpr uint16_t TIP_ONES = 0xFFFFu;  // all-ones partial (gap, unmatched leaf, padding)
+constexpr uint16_t TIP_AMB = 0x8000u;   // | index into the ambiguity-mask table
+
+// Schedule entry layout (int32 x SCHED_STRIDE), one per internal node in
+// post-order: [0] node, [1] D slot, [2] #children (<= 3), [3] V slot of the
+// node (-1 at the root), [4] 1 if the edge above the node is an identity
+// edge, [5] unused, then per child c: [6+3c] node id of the child (selects
+// its edge matrix), [7+3c] child reference, [8+3c] V slot of the child.
+constexpr int SCHED_STRIDE = 16;
+constexpr int MAXC = 3;
+
+// Branch list entry (int32 x 4): node, V slot, child reference, class.
+constexpr int BR_STRIDE = 4;
+
+// Per-evaluation parameter block (device, FP64), offsets in doubles.
+struct ParamLayout {
+    int n_nodes;   // extended node count
+    int n_mat, n_comp;
+    int off_bl, off_alpha, off_omega, off_w, off_theta, off_delta, off_psi, total;
+};
+
+struct ModelArgs {
+    const int8_t *nd, *dpos, *dfrom, *dto;  // [64*64], [64*64*3] x3
+    const uint8_t *syn;                     // [64*64]
+    const double *pos;                      // [12]
+    const double *pi;                       // [64], zero beyond S
+    int S;
+    const double *params;
+    ParamLayout pl;
+    double *lam;  // [n_mat][64]
+    double *U;    // [n_mat][64][64]
+    double *V;    // [n_mat][64][64]
+    double *mu;   // [n_mat]
+    int *status;  // [n_mat] sweeps used, negative = no convergence
+};
+
+template <typename R>
+struct PmatArgs {
+    const int32_t *blist;  // [n_branch][BR_STRIDE]
+    int n_branch;
+    const int32_t *comp_ptr, *comp_mat;
+    int convention;
+    const double *params;
+    ParamLayout pl;
+    const double *lam, *U, *V, *mu;
+    int S;
+    double *tau;  // [n_nodes]
+    R *Prow;      // [n_nodes][64][64]
+    R *Rsum;      // [n_nodes][64]
+    R *PA;        // [n_nodes][4][16][64]   A-operand swizzle of P
+    R *PTA;       // [n_nodes][4][16][64]   A-operand swizzle of P^T
+};
+
+template <typename R>
+struct PruneArgs {
+    const int32_t *sched;
+    int n_sched;
+    const R *PA, *PTA, *Prow, *Rsum;
+    const uint16_t *tips;  // [rows][P_pad]
+    const uint64_t *amb;
+    int P_pad, n_tiles, n_patterns;
+    const double *pi;
+    R *D;
+    int32_t *E;    // [n_int][P_pad]
+    R *V;
+    int32_t *G;    // [n_vslot][P_pad]
+    double *Lroot; // [P_pad]
+    int32_t *Eroot;
+    double *site_ll;
+};
+
+template <typename R>
+struct AdjArgs {
+    const int32_t *blist;
+    int n_branch, n_chunks, tiles_per_chunk;
+    const R *D, *V;
+    const int32_t *E, *G, *Eroot;
+    const double *Lroot, *weights;
+    const uint16_t *tips;
+    const uint64_t *amb;
+    int P_pad, n_tiles, S;
+    double *Apart;  // [n_branch][n_chunks][64][64]
+};
+
+struct ProjArgs {
+    const int8_t *nd, *dpos, *dfrom, *dto;
+    const uint8_t *syn;
+    const double *pos, *pi;
+    int S;
+    const double *params;
+    ParamLayout pl;
+    const double *U, *V;
+    double *M;    // [n_mat][6][64][64]
+    double *dmu;  // [n_mat][6]
+};
+
+struct FrechetArgs {
+    const int32_t *blist;
+    int n_branch, n_chunks, max_comp;
+    const int32_t *comp_ptr, *comp_mat;
+    const double *params;
+    ParamLayout pl;
+    const double *Apart, *lam, *U, *V, *M, *tau;
+    double *out;  // [n_branch][max_comp][8]
+};
+
+struct FinalArgs {
+    const double *site_ll, *weights;
+    const double *Lroot;
+    int n_patterns, want_grad;
+    const int32_t *blist;
+    int n_branch, max_comp, n_class, n_real_nodes;
+    const int32_t *comp_ptr, *comp_mat;
+    int convention;
+    const double *params;
+    ParamLayout pl;
+    const double *fr, *tau, *mu, *dmu;
+    double *res;  // [0] lnL, [1..n_real] grad_bl, then omega[n_mat], w[n_comp], theta[5]
+};
+
+// Launchers (hy_kernels.hip).
+void launch_model(const ModelArgs &a, int n_mat, hipStream_t s);
+template <typename R> void launch_pmat(const PmatArgs<R> &a, hipStream_t s);
+template <typename R> void launch_pmat_host(const PmatArgs<R> &a, const double *Phost, hipStream_t s);
+template <typename R> void launch_inside(const PruneArgs<R> &a, int T, hipStream_t s);
+template <typename R> void launch_outside(const PruneArgs<R> &a, int T, hipStream_t s);
+template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s);
+void launch_proj(const ProjArgs &a, int n_mat, hipStream_t s);
+void launch_frechet(const FrechetArgs &a, hipStream_t s);
+void launch_final(const FinalArgs &a, hipStream_t s);
+
+}  // namespace hy

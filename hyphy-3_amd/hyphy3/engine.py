@@ -1,0 +1,192 @@
+"""ctypes binding of the MI355X engine's C ABI (include/hyphy_mi355x.h).
+
+The shared library ``libhyphy_mi355x.so`` is built in-tree by
+``make -C hyphy-3_amd`` (or ``__graft_entry__.build()``).  Importing this
+module never falls back to a CPU implementation: a missing library or a
+failing HIP call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libhyphy_mi355x.so")
+
+HY_OK = 0
+HY_PREC_FP64 = 0
+HY_PREC_FP32 = 1
+HY_TIME_RAW = 0
+HY_TIME_CONV = 1
+HY_WANT_LOGL = 1
+HY_WANT_SITE_LL = 2
+HY_WANT_GRAD = 4
+
+EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
+           "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes")
+
+_i32p = C.POINTER(C.c_int32)
+_i8p = C.POINTER(C.c_int8)
+_u64p = C.POINTER(C.c_uint64)
+_f64p = C.POINTER(C.c_double)
+
+_lib = None
+
+
+class HyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hyphy_mi355x error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libhyphy_mi355x.so and declare its prototypes (no device work)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"MI355X engine library not built: {path} (run make -C hyphy-3_amd)")
+    L = C.CDLL(path)
+    L.hy_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    L.hy_destroy.argtypes = [C.c_void_p]
+    L.hy_destroy.restype = None
+    L.hy_last_error.argtypes = [C.c_void_p]
+    L.hy_last_error.restype = C.c_char_p
+    L.hy_set_problem.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _i32p, _i32p, C.c_int, C.c_int, _i8p, _u64p,
+                                 _f64p, C.c_int]
+    L.hy_set_model.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p]
+    L.hy_set_mixture.argtypes = [C.c_void_p, C.c_int, C.c_int, _i32p, _i32p, _i32p, C.c_int]
+    L.hy_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
+                          C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.hy_loglik_from_P.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
+    L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
+    L.hy_last_timings.argtypes = [C.c_void_p, _f64p]
+    L.hy_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Engine:
+    """One MI355X likelihood engine (one device, one stream)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.hy_create(device, C.byref(h))
+        if rc != HY_OK:
+            raise HyError(rc, self.L.hy_last_error(None).decode())
+        self.h = h
+        self.num_nodes = 0
+        self.num_patterns = 0
+        self.n_mat = 1
+        self.n_comp = 1
+
+    def _check(self, rc):
+        if rc != HY_OK:
+            raise HyError(rc, self.L.hy_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hy_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_problem(self, parent, children, leaf_taxon, states, masks, weights, S=61):
+        n = len(parent)
+        self._parent = np.ascontiguousarray(parent, dtype=np.int32)
+        cp, ci = [0], []
+        for k in range(n):
+            ci.extend(children[k])
+            cp.append(len(ci))
+        self._cp = np.array(cp, dtype=np.int32)
+        self._ci = np.array(ci if ci else [0], dtype=np.int32)
+        self._l2t = np.ascontiguousarray(leaf_taxon, dtype=np.int32)
+        st = np.ascontiguousarray(states, dtype=np.int8)
+        mk = np.ascontiguousarray(masks, dtype=np.uint64)
+        w = _f64(weights)
+        P = int(st.shape[0])
+        ntax = int(st.shape[1]) if st.ndim == 2 else 0
+        self._check(self.L.hy_set_problem(self.h, n, _p(self._parent, _i32p), _p(self._cp, _i32p),
+                                          _p(self._ci, _i32p), _p(self._l2t, _i32p), ntax, P, _p(st, _i8p),
+                                          _p(mk, _u64p), _p(w, _f64p), S))
+        self.num_nodes, self.num_patterns = n, P
+        self.n_mat, self.n_comp = 1, 1
+
+    def set_model(self, code_id, pos_freqs, pi):
+        pos = _f64(pos_freqs).reshape(12)
+        pi = _f64(pi)
+        self._check(self.L.hy_set_model(self.h, code_id, _p(pos, _f64p), _p(pi, _f64p)))
+
+    def set_mixture(self, n_mat, branch_class, classes, convention=HY_TIME_RAW):
+        """classes: list (per class) of lists of matrix indices."""
+        cp, cm = [0], []
+        for comps in classes:
+            cm.extend(int(m) for m in comps)
+            cp.append(len(cm))
+        cp = np.array(cp, dtype=np.int32)
+        cm = np.array(cm, dtype=np.int32)
+        bc = None if branch_class is None else np.ascontiguousarray(branch_class, dtype=np.int32)
+        self._check(self.L.hy_set_mixture(self.h, n_mat, len(classes), _p(bc, _i32p), _p(cp, _i32p),
+                                          _p(cm, _i32p), convention))
+        self.n_mat, self.n_comp = n_mat, len(cm)
+
+    def set_tile(self, T):
+        self._check(self.L.hy_set_tile(self.h, int(T)))
+
+    def eval(self, bl, omegas, comp_w, theta, alpha=1.0, delta=0.0, psi=0.0, precision=HY_PREC_FP64, grad=True,
+             site=False):
+        bl = _f64(bl)
+        om = _f64(np.atleast_1d(omegas))
+        cw = _f64(np.atleast_1d(comp_w))
+        th = _f64(theta)
+        ll = C.c_double()
+        out = {}
+        s = np.zeros(max(self.num_patterns, 1)) if site else None
+        gbl = np.zeros(self.num_nodes) if grad else None
+        gom = np.zeros(self.n_mat) if grad else None
+        gw = np.zeros(self.n_comp) if grad else None
+        gth = np.zeros(5) if grad else None
+        flags = HY_WANT_LOGL | (HY_WANT_SITE_LL if site else 0) | (HY_WANT_GRAD if grad else 0)
+        self._check(self.L.hy_eval(self.h, precision, flags, _p(bl, _f64p), alpha, _p(om, _f64p), _p(cw, _f64p),
+                                   _p(th, _f64p), delta, psi, C.byref(ll), _p(s, _f64p), _p(gbl, _f64p),
+                                   _p(gom, _f64p), _p(gw, _f64p), _p(gth, _f64p)))
+        out["log_likelihood"] = ll.value
+        if site:
+            out["site_ll"] = s[:self.num_patterns]
+        if grad:
+            out.update(grad_bl=gbl, grad_omega=gom, grad_w=gw, grad_theta=gth)
+        return out
+
+    def loglik_from_P(self, P, pi, precision=HY_PREC_FP64, site=False):
+        P = _f64(P)
+        pi = _f64(pi)
+        ll = C.c_double()
+        s = np.zeros(max(self.num_patterns, 1)) if site else None
+        self._check(self.L.hy_loglik_from_P(self.h, precision, _p(P, _f64p), _p(pi, _f64p), C.byref(ll),
+                                            _p(s, _f64p)))
+        return (ll.value, s[:self.num_patterns]) if site else ll.value
+
+    def last_timings(self):
+        t = np.zeros(8)
+        self._check(self.L.hy_last_timings(self.h, _p(t, _f64p)))
+        return t
+
+    def device_bytes(self):
+        v = C.c_size_t()
+        self._check(self.L.hy_device_bytes(self.h, C.byref(v)))
+        return v.value

@@ -1,0 +1,140 @@
+/*
+ * hyphy_mi355x.h -- C ABI of the MI355X MG94xREV codon likelihood engine.
+ *
+ * One handle = one GPU, one HIP stream, one host thread at a time.  All
+ * arguments are plain pointers and sizes; every function returns HY_OK (0) or
+ * a negative HY_ERR_* code and records a message readable with
+ * hy_last_error().  Host arrays are copied at call time; the caller keeps
+ * ownership of everything it passes in.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   hy_set_problem   Tree + Alignment + leaf_to_taxon arguments of every
+ *                    LikelihoodEngine entry (include/hyphy/core/likelihood.hpp:
+ *                    343-354, 766-774, 1102-1108, 1228-1236; leaf map :150-178)
+ *   hy_set_model     aln.pos_nuc_frequencies / aln.codon_frequencies_f3x4 /
+ *                    aln.code as read by MG94Matrix::update
+ *                    (include/hyphy/core/rate_matrix.hpp:165-170)
+ *   hy_set_mixture   branch-site mixture layout of BUSTED's P_mix and the
+ *                    test/background partition of the joint gradient
+ *                    (include/hyphy/analyses/busted.hpp:561-581,1191-1216;
+ *                    likelihood.hpp:1240-1275)
+ *   hy_eval          compute_mg94_log_likelihood (likelihood.hpp:257-292),
+ *                    compute_branch_length_gradients (:1102-1211),
+ *                    compute_joint_frechet_gradients (:1228-1594),
+ *                    BUSTEDAnalyzer::evaluate_log_l (busted.hpp:509-659) and
+ *                    compute_busted_branch_gradients (busted.hpp:1138-1429)
+ *                    with M = 1 synonymous class
+ *   hy_loglik_from_P compute_tree_log_likelihood_from_P (likelihood.hpp:766-806)
+ */
+#ifndef HYPHY_MI355X_H
+#define HYPHY_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HY_OK 0
+#define HY_ERR_ARG (-1)         /* invalid argument or shape                  */
+#define HY_ERR_HIP (-2)         /* HIP runtime failure                        */
+#define HY_ERR_STATE (-3)       /* call order (e.g. eval before set_problem)  */
+#define HY_ERR_NOMEM (-4)       /* device allocation failed                   */
+#define HY_ERR_UNSUPPORTED (-5) /* e.g. Wheatstone-bridged trees               */
+#define HY_ERR_NUMERIC (-6)     /* eigensolver did not converge               */
+
+/* Arithmetic of the pruning / adjoint contractions. */
+#define HY_PREC_FP64 0 /* FP64 MFMA throughout (on-device oracle path)         */
+#define HY_PREC_FP32 1 /* FP32 MFMA contractions, FP64 spectra and reductions  */
+
+/* Branch time convention (see hy_set_mixture). */
+#define HY_TIME_RAW 0  /* tau_b = bl_b                                          */
+#define HY_TIME_CONV 1 /* tau_b = bl_b * 3 / mubar_class                        */
+
+/* hy_eval flags */
+#define HY_WANT_LOGL 1u
+#define HY_WANT_SITE_LL 2u
+#define HY_WANT_GRAD 4u /* branch, omega, mixture-weight and GTR gradients     */
+
+typedef struct hy_engine_s *hy_engine;
+
+/* Create an engine on HIP device `device`. */
+int hy_create(int device, hy_engine *out);
+void hy_destroy(hy_engine h);
+/* Message of the last failing call on this handle (or of a failed hy_create
+ * when h is NULL).  Valid until the next call on the handle. */
+const char *hy_last_error(hy_engine h);
+
+/* Tree and compressed alignment.
+ *   num_nodes            nodes of the (unrooted or rooted) tree, ids 0..num_nodes-1
+ *   parent[num_nodes]    parent id, -1 at the root (exactly one root)
+ *   child_ptr[num_nodes+1], child_idx[]   children of node n in Newick order:
+ *                        child_idx[child_ptr[n] .. child_ptr[n+1]-1]
+ *   leaf_taxon[num_nodes] alignment row of each leaf, -1 when unmatched
+ *                        (an unmatched leaf contributes all-ones partials,
+ *                        likelihood.hpp:62-65); ignored for internal nodes
+ *   states[num_patterns*num_taxa]  sense-codon index or -1 (pattern-major)
+ *   masks[num_patterns*num_taxa]   compatible-codon bit masks
+ *                        (alignment.hpp:26-31, genetic_code.hpp:82-109)
+ *   weights[num_patterns] pattern multiplicities
+ *   S                    number of sense codons of the genetic code (<= 64)
+ * Re-calling replaces the problem and frees the previous device buffers. */
+int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int32_t *child_ptr,
+                   const int32_t *child_idx, const int32_t *leaf_taxon, int num_taxa, int num_patterns,
+                   const int8_t *states, const uint64_t *masks, const double *weights, int S);
+
+/* Genetic code (registry id 0..21 in the reference's order,
+ * genetic_code.hpp:274-370), CF3x4-corrected position frequencies
+ * pos_freqs[3*4] (row = codon position, column = A,C,G,T) and the codon
+ * frequencies pi[S]. */
+int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double *pi);
+
+/* Branch-site mixture layout.
+ *   n_mat rate matrices; matrix m has nonsynonymous rate omegas[m] at eval time.
+ *   branch_class[num_nodes]: class of the branch above each node (root ignored).
+ *   Class c mixes components comp_ptr[c] .. comp_ptr[c+1]-1; component j uses
+ *   matrix comp_mat[j] with weight comp_w[j] (weights given at eval time):
+ *       P_b = sum_j comp_w[j] * exp(Q_{comp_mat[j]} tau_b)
+ *   time_convention HY_TIME_RAW or HY_TIME_CONV (mubar_c = sum_j w_j mu_j).
+ * Default after hy_set_problem: one matrix, one class, one component, RAW. */
+int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_class, const int32_t *comp_ptr,
+                   const int32_t *comp_mat, int time_convention);
+
+/* Evaluate lnL (and gradients) at the given parameters.
+ *   bl[num_nodes]        branch lengths (root entry ignored)
+ *   alpha                synonymous rate (MG94Parameters::alpha)
+ *   omegas[n_mat]        nonsynonymous rates (MG94Parameters::beta per matrix)
+ *   comp_w[n_comp]       mixture weights
+ *   theta[5]             GTR exchangeabilities AC, AT, CG, CT, GT (AG == 1)
+ *   delta, psi           multi-hit rates (forward only; gradients assume 0)
+ * Outputs (NULL to skip): out_logL, out_site_ll[num_patterns] (per-pattern lnL,
+ * unweighted), out_grad_bl[num_nodes] (d lnL/d bl, root 0), out_grad_omega
+ * [n_mat], out_grad_w[n_comp], out_grad_theta[5]. */
+int hy_eval(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
+            const double *comp_w, const double *theta, double delta, double psi, double *out_logL,
+            double *out_site_ll, double *out_grad_bl, double *out_grad_omega, double *out_grad_w,
+            double *out_grad_theta);
+
+/* lnL from caller-supplied transition matrices P[num_nodes*S*S] (row-major,
+ * entry of the root ignored) and root frequencies pi[S]. */
+int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
+                     double *out_site_ll);
+
+/* Pattern-tile width used by the pruning kernels (16, 32 or 64 patterns per
+ * workgroup; 0 = automatic).  Applies from the next hy_eval. */
+int hy_set_tile(hy_engine h, int patterns_per_tile);
+
+/* Timing of the last hy_eval's device work, in milliseconds, per stage:
+ * [0] spectra, [1] P(t), [2] inside, [3] outside, [4] adjoint accumulation,
+ * [5] Frechet projection, [6] reductions, [7] total.  Only filled when the
+ * engine was created with HY_PROFILE=1 in the environment. */
+int hy_last_timings(hy_engine h, double *ms8);
+
+/* Device memory currently held by the engine, in bytes. */
+int hy_device_bytes(hy_engine h, size_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

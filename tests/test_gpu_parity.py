@@ -1,0 +1,82 @@
+"""MI355X engine vs the FP64 CPU oracle on the same seeded inputs (GPU)."""
+import numpy as np
+import pytest
+
+from helpers import (branch_lengths, grad_err, load_cd2_reduced, make_engine, rel, sim_problem)
+from oracle import hyoracle as ho
+
+pytestmark = pytest.mark.gpu
+
+FP64, FP32 = 0, 1
+GOLDEN_MG94 = -92.77744887734814
+
+
+def test_cd2_reduced_point_logl_fp64_fp32():
+    tree, aln = load_cd2_reduced()
+    eng = make_engine(tree, aln)
+    bl = branch_lengths(tree)
+    theta = [1, 1, 1, 2, 1]
+    prob = ho.Problem.from_refs(tree, aln)
+    ref = ho.mg94_loglik(prob, ho.Code(0), ho.params(1.0, 0.5, theta), aln.pos_freqs, aln.cf3x4, bl)
+    r64 = eng.eval(bl, [0.5], [1.0], theta, precision=FP64, grad=False, site=True)
+    r32 = eng.eval(bl, [0.5], [1.0], theta, precision=FP32, grad=False)
+    print("cd2_reduced lnL fp64", r64["log_likelihood"], "fp32", r32["log_likelihood"], "oracle", ref)
+    assert rel(r64["log_likelihood"], ref) < 1e-9
+    assert rel(r64["log_likelihood"], GOLDEN_MG94) < 1e-8
+    assert rel(r32["log_likelihood"], ref) < 1e-5
+
+
+def test_cd2_reduced_joint_gradients():
+    """compute_joint_frechet_gradients semantics (3/mu time, K=1) at the
+    reference test's point (tests/test_core.cpp:149-223)."""
+    tree, aln = load_cd2_reduced()
+    eng = make_engine(tree, aln)
+    eng.set_mixture(1, None, [[0]], convention=1)
+    th = [0.8, 1.2, 0.6, 1.8, 0.9]
+    bl = np.array([0.0 if n.id == tree.root else max(1e-4, n.bl) for n in tree.nodes])
+    prob = ho.Problem.from_refs(tree, aln)
+    ref = ho.joint_frechet(prob, ho.Code(0), ho.params(1.0, 0.45, th), aln.pos_freqs, aln.cf3x4, bl)
+    for prec, tl, tg in ((FP64, 1e-9, 1e-7), (FP32, 1e-5, 1e-4)):
+        r = eng.eval(bl, [0.45], [1.0], th, precision=prec)
+        print(prec, r["log_likelihood"], ref["log_likelihood"])
+        assert rel(r["log_likelihood"], ref["log_likelihood"]) < tl
+        assert grad_err(r["grad_bl"], ref["grad_branches"]) < tg
+        assert rel(r["grad_omega"][0], ref["grad_omega"]) < tg
+        assert grad_err(r["grad_theta"], ref["grad_theta"]) < tg
+
+
+@pytest.mark.parametrize("K", [1, 3])
+def test_sim16_mixture_fp64_fp32(K):
+    omegas = [0.5] if K == 1 else [0.1, 0.8, 3.0]
+    weights = [1.0] if K == 1 else [0.6, 0.35, 0.05]
+    tree, aln, sim = sim_problem(16, 500, 101, omegas=omegas, weights=weights)
+    eng = make_engine(tree, aln)
+    N = len(tree.nodes)
+    eng.set_mixture(K, np.zeros(N, dtype=np.int32), [list(range(K))], convention=0)
+    bl = branch_lengths(tree) * sim["conv"]
+    prob = ho.Problem.from_refs(tree, aln)
+    mix = ho.Mixture(omegas, np.zeros(N, dtype=np.int32), [[(k, w) for k, w in enumerate(weights)]], 0)
+    ref = ho.mixture_eval(prob, ho.Code(0), ho.params(1.0, 1.0, sim["theta"]), aln.pos_freqs, aln.cf3x4, mix, bl)
+    for prec, tl, tg in ((FP64, 1e-9, 1e-7), (FP32, 1e-5, 1e-4)):
+        r = eng.eval(bl, omegas, weights, sim["theta"], precision=prec, site=True)
+        print("K", K, "prec", prec, r["log_likelihood"], ref["log_likelihood"])
+        assert rel(r["log_likelihood"], ref["log_likelihood"]) < tl
+        assert grad_err(r["grad_bl"], ref["grad_bl"]) < tg
+        assert grad_err(r["grad_omega"], ref["grad_omega"]) < tg
+        assert grad_err(r["grad_w"], ref["grad_w"]) < tg
+        assert grad_err(r["grad_theta"], ref["grad_theta"]) < tg
+        np.testing.assert_allclose(r["site_ll"], ref["site_ll"], rtol=1e-6 if prec else 1e-11, atol=1e-9)
+
+
+@pytest.mark.parametrize("T", [16, 32, 64])
+def test_tile_widths_agree(T):
+    tree, aln, sim = sim_problem(12, 300, 7)
+    eng = make_engine(tree, aln)
+    eng.set_tile(T)
+    bl = branch_lengths(tree)
+    prob = ho.Problem.from_refs(tree, aln)
+    ll, g = ho.branch_length_gradients(prob, ho.Code(0), ho.params(1.0, 0.5, sim["theta"]), aln.pos_freqs,
+                                       aln.cf3x4, bl)
+    r = eng.eval(bl, [0.5], [1.0], sim["theta"], precision=FP64)
+    assert rel(r["log_likelihood"], ll) < 1e-10
+    assert grad_err(r["grad_bl"], g) < 1e-8
