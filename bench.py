@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""MG94xREV codon likelihood + full gradient throughput on one MI355X.
+
+Workload (default, "c2"): 64 taxa x 5,000 codons simulated under a 3-category
+branch-site omega mixture (omega = 0.1, 0.8, 3.0; weights 0.6, 0.35, 0.05),
+Yule tree from seed 202.  One step = one evaluation of lnL plus the gradient
+with respect to all 125 branch lengths, the 3 omegas, the 3 mixture weights
+and the 5 GTR exchangeabilities, including the per-evaluation eigensolves and
+P(t) builds, with the compressed patterns already resident in HBM.
+
+Prints one JSON line (rank 0).  ``value`` is FP32-MFMA evaluations/s; the FP64
+path, the FP32-vs-FP64 errors, per-stage times, the roofline of the dominant
+kernel and the CPU baseline (FP64 restatement of the reference algorithm on
+the host cores) are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hyphy-3_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "LL+gradient evals/sec (64 taxa, 5k codons) FP32-MFMA vs FP64; |ΔlogL| vs FP64"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+S_ALG = 61
+
+CONFIGS = {
+    "c1": dict(taxa=16, codons=500, omegas=[0.5], weights=[1.0], seed=101, gaps=0.0),
+    "c2": dict(taxa=64, codons=5000, omegas=[0.1, 0.8, 3.0], weights=[0.6, 0.35, 0.05], seed=202, gaps=0.0),
+    "c4": dict(taxa=512, codons=20000, omegas=[0.05, 0.4, 1.0, 3.0], weights=[0.4, 0.3, 0.2, 0.1], seed=404,
+               gaps=0.25),
+    "c5": dict(taxa=256, codons=10000, omegas=[0.1, 0.8, 3.0], weights=[0.6, 0.35, 0.05], seed=505, gaps=0.0),
+}
+STAGES = ["spectra", "pmat", "inside", "outside", "adjoint", "frechet", "final"]
+
+
+def build_problem(cfg):
+    import hyphy3
+    from hyphy3 import simulate
+    sim = simulate.simulate(cfg["taxa"], cfg["codons"], cfg["seed"], omegas=cfg["omegas"], weights=cfg["weights"],
+                            gap_fraction=cfg["gaps"])
+    aln = hyphy3.Alignment.from_text(sim["fasta"])
+    tree = hyphy3.Tree.from_newick(sim["newick"], keep_rooted=False)
+    arrays = hyphy3.problem_arrays(tree, aln)
+    tau = np.array([0.0 if n.id == tree.root_id else n.branch_length * sim["conv"] for n in tree.nodes])
+    return sim, aln, tree, arrays, tau
+
+
+def stage_work(arrays, K, b):
+    """Algorithmic flops and HBM bytes per launch of each pruning kernel."""
+    parent = arrays["parent"]
+    cp = arrays["child_ptr"]
+    n_nodes = len(parent)
+    is_leaf = np.array([cp[i + 1] == cp[i] for i in range(n_nodes)])
+    n_tax = int(is_leaf.sum())
+    n_int = n_nodes - n_tax
+    root = int(np.where(parent < 0)[0][0])
+    e_int = n_int - 1  # internal nodes other than the root
+    B = n_nodes - 1
+    P = int(len(arrays["weights"]))
+    blk = b * 64 * P  # one node's partials (64 padded states x P patterns)
+    return {
+        "inside": dict(flops=2.0 * S_ALG ** 2 * P * e_int, bytes=blk * n_int + blk * e_int + 2 * n_tax * P + 8 * n_int * P),
+        "outside": dict(flops=2.0 * S_ALG ** 2 * P * (e_int + e_int), bytes=blk * e_int + blk * e_int + blk * B + 8 * B * P),
+        "adjoint": dict(flops=2.0 * S_ALG ** 2 * P * e_int + 2.0 * S_ALG * P * n_tax, bytes=blk * B + blk * e_int + 8 * B * P),
+        "pmat": dict(flops=2.0 * S_ALG ** 3 * B * K, bytes=3 * b * 4096 * B),
+        "_meta": dict(P=P, n_int=n_int, e_int=e_int, B=B, n_tax=n_tax, root=root),
+    }
+
+
+def make_engine(arrays, K, T=0):
+    from hyphy3.engine import Engine
+    eng = Engine(0)
+    children = [list(arrays["child_idx"][arrays["child_ptr"][i]:arrays["child_ptr"][i + 1]])
+                for i in range(len(arrays["parent"]))]
+    eng.set_problem(arrays["parent"], children, arrays["leaf_taxon"], arrays["states"], arrays["masks"],
+                    arrays["weights"], S=int(arrays["S"]))
+    eng.set_model(int(arrays["code_id"]), arrays["pos_freqs"], arrays["pi"])
+    eng.set_mixture(K, np.zeros(len(arrays["parent"]), dtype=np.int32), [list(range(K))], convention=0)
+    if T:
+        eng.set_tile(T)
+    return eng
+
+
+def run_steps(eng, args_eval, precision, steps, warmup, profile):
+    for _ in range(warmup):
+        eng.eval(*args_eval, precision=precision)
+    sync()
+    stage = np.zeros(8)
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = eng.eval(*args_eval, precision=precision)
+        if profile:
+            stage += eng.last_timings()
+    sync()
+    dt = time.perf_counter() - t0
+    return dt / steps, out, stage / max(steps, 1)
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def cpu_baseline(cfg, sim, seconds):
+    """FP64 restatement of the reference algorithm (oracle) on the host cores."""
+    from oracle import data_ref as dr
+    from oracle import hyoracle as ho
+    names, seqs = dr.parse_fasta(sim["fasta"])
+    aln = dr.AlignmentRef(names, seqs)
+    tree = dr.TreeRef.from_newick(sim["newick"], keep_rooted=False)
+    prob = ho.Problem.from_refs(tree, aln)
+    K = len(cfg["omegas"])
+    N = len(tree.nodes)
+    mix = ho.Mixture(cfg["omegas"], np.zeros(N, dtype=np.int32), [[(k, w) for k, w in enumerate(cfg["weights"])]], 0)
+    bl = np.array([0.0 if n.id == tree.root else n.bl * sim["conv"] for n in tree.nodes])
+    threads = os.cpu_count() or 1
+    # BUSTED tile rule (busted.hpp:385-401): clamp(ceil(P/threads), 16, 64)
+    P = prob.num_patterns
+    tile = 64 if threads <= 1 else int(min(64, max(16, -(-P // threads))))
+    base = ho.params(1.0, 1.0, sim["theta"])
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        ho.mixture_eval(prob, ho.Code(0), base, aln.pos_freqs, aln.cf3x4, mix, bl, want_grad=True, tile=tile,
+                        threads=threads)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 200:
+            break
+    return {"value": n / el, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"full {cfg['taxa']} taxa x {cfg['codons']} codons x K={K} problem, {n} logL+gradient evals "
+                      f"in {el:.1f} s (FP64 restatement of the reference algorithm, tile {tile}, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one MI355X; multi-GPU runs are not part of it "
+                                   "(the path would shard patterns across GPUs, see DESIGN.md)"}))
+        sys.exit(2)
+    os.environ["HY_PROFILE"] = "1"
+    cfg = CONFIGS[args.config]
+    K = len(cfg["omegas"])
+    sim, aln, tree, arrays, tau = build_problem(cfg)
+    print(f"[bench] {args.config}: taxa={cfg['taxa']} codons={cfg['codons']} patterns={len(arrays['weights'])} "
+          f"nodes={len(arrays['parent'])} K={K}", file=sys.stderr, flush=True)
+    eng = make_engine(arrays, K, args.tile)
+    eval_args = (tau, cfg["omegas"], cfg["weights"], sim["theta"])
+    ms32, out32, st32 = run_steps(eng, eval_args, 1, args.steps, args.warmup, True)
+    print(f"[bench] fp32 {ms32 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
+    res = {}
+    if not args.no_fp64:
+        ms64, out64, st64 = run_steps(eng, eval_args, 0, args.steps, args.warmup, True)
+        print(f"[bench] fp64 {ms64 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
+        g32 = np.concatenate([out32["grad_bl"], out32["grad_omega"], out32["grad_w"], out32["grad_theta"]])
+        g64 = np.concatenate([out64["grad_bl"], out64["grad_omega"], out64["grad_w"], out64["grad_theta"]])
+        scale = np.abs(g64).max()
+        res.update({
+            "fp64_evals_per_s": 1.0 / ms64, "fp64_ms_per_step": ms64 * 1e3, "fp32_over_fp64": ms64 / ms32,
+            "rel_err_logl_vs_fp64": abs(out32["log_likelihood"] - out64["log_likelihood"]) / abs(out64["log_likelihood"]),
+            "abs_err_logl_vs_fp64": abs(out32["log_likelihood"] - out64["log_likelihood"]),
+            "rel_err_grad_vs_fp64_max_component": float(np.max(np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-2 * scale))),
+            "rel_err_grad_vs_fp64_norm": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+            "stage_ms_fp64": {s: float(st64[i]) for i, s in enumerate(STAGES)},
+            "logL": out64["log_likelihood"],
+        })
+    work = stage_work(arrays, K, 4)
+    stage_ms = {s: float(st32[i]) for i, s in enumerate(STAGES)}
+    cand = {s: stage_ms[s] for s in ("inside", "outside", "adjoint")}
+    dom = max(cand, key=cand.get)
+    t_dom = stage_ms[dom] * 1e-3
+    fl, by = work[dom]["flops"], work[dom]["bytes"]
+    if fl / by > PEAK_FP32_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9):
+        roof = {"bound": "mfma", "achieved": fl / t_dom / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": by / t_dom / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = {"inside": "k_inside<float>", "outside": "k_outside<float>", "adjoint": "k_adjoint<float>"}[dom]
+    roof["algorithmic_flops"] = fl
+    roof["algorithmic_bytes"] = by
+    roof["kernel_ms"] = stage_ms[dom]
+    line = {
+        "metric": METRIC, "value": 1.0 / ms32, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms32 * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.config}: {cfg['taxa']} taxa x {cfg['codons']} codons x {K} omega categories, "
+                               "logL + full gradient (branches, omegas, weights, GTR)",
+                   "taxa": cfg["taxa"], "codons": cfg["codons"], "patterns": int(len(arrays["weights"])),
+                   "categories": K, "seed": cfg["seed"], "tile": eng_tile(args.tile, len(arrays["weights"]))},
+        "roofline": roof, "stage_ms_fp32": stage_ms,
+    }
+    line.update(res)
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(cfg, sim, args.cpu_seconds)
+            line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+        except Exception as ex:  # the oracle library may not be built on a bare box
+            line["cpu_baseline"] = {"value": None, "error": str(ex)}
+    print(json.dumps(line), flush=True)
+
+
+def eng_tile(req, P):
+    if req:
+        return req
+    if -(-P // 64) >= 512:
+        return 64
+    if -(-P // 32) >= 512:
+        return 32
+    return 16
+
+
+if __name__ == "__main__":
+    main()
