@@ -47,7 +47,13 @@ struct hy_engine_s {
     // problem
     bool have_problem = false;
     int num_nodes = 0, n_ext = 0, root = -1, num_taxa = 0, P = 0, n_int = 0, n_vslot = 0, n_branch = 0;
-    std::vector<int32_t> sched, blist;
+    std::vector<int32_t> sched, osched, blist;
+    std::vector<int32_t> order, int_slot, vslot, leaf_taxon_ext;
+    std::vector<std::vector<int32_t>> ext_children;
+    std::vector<uint8_t> ext_ident;
+    int ext_root = -1, sched_key = 0;
+    DevBuf d_osched, d_Uspill[2];
+    int n_uspill = 0;
     std::vector<int32_t> real_parent;
     std::vector<uint16_t> codes;  // [num_taxa][P]
     std::vector<uint64_t> amb;
@@ -67,7 +73,8 @@ struct hy_engine_s {
     double *h_res = nullptr;
     size_t h_res_cap = 0;
     int *h_status = nullptr;
-    DevBuf d_params, d_lam, d_U, d_V, d_mu, d_status, d_tau, d_M, d_dmu;
+    DevBuf d_params, d_lam, d_U, d_V, d_mu, d_status, d_tau, d_M, d_dmu, d_Z;
+    bool warm_enable = true, warm_valid = false;
     DevBuf d_Prow[2], d_Rsum[2], d_PA[2], d_PTA[2];
     DevBuf d_D[2], d_Vp[2], d_E, d_G, d_Lroot, d_Eroot, d_site, d_Apart, d_fr, d_res;
     int n_chunks = 1, tiles_per_chunk = 1;
@@ -161,6 +168,8 @@ void build_layout(hy_engine h) {
     pl.total = o;
 }
 
+void build_schedules(hy_engine h, int T, int esz);
+
 // (Re)allocate everything that depends on the tile width and precision.
 int prepare_buffers(hy_engine h, int prec) {
     const int T = choose_tile(h->P, h->T_req);
@@ -181,11 +190,18 @@ int prepare_buffers(hy_engine h, int prec) {
         for (int k = 0; k < 2; ++k) {
             release(h, h->d_D[k]);
             release(h, h->d_Vp[k]);
+            release(h, h->d_Uspill[k]);
         }
+    }
+    if (h->sched_key != T * 16 + (int)rs) {
+        build_schedules(h, T, (int)rs);
+        TRY(upload(h, h->d_sched, h->sched.data(), h->sched.size() * 4));
+        TRY(upload(h, h->d_osched, h->osched.data(), h->osched.size() * 4));
     }
     const size_t Pp = (size_t)h->P_pad;
     TRY(ensure(h, h->d_D[ri], (size_t)h->n_int * Pp * 64 * rs));
     TRY(ensure(h, h->d_Vp[ri], (size_t)h->n_vslot * Pp * 64 * rs));
+    TRY(ensure(h, h->d_Uspill[ri], h->n_uspill ? (size_t)h->n_int * Pp * 64 * rs : 256));
     TRY(ensure(h, h->d_E, (size_t)h->n_int * Pp * 4));
     TRY(ensure(h, h->d_G, (size_t)h->n_vslot * Pp * 4));
     TRY(ensure(h, h->d_Lroot, Pp * 8));
@@ -196,9 +212,10 @@ int prepare_buffers(hy_engine h, int prec) {
     TRY(ensure(h, h->d_PA[ri], ne * 4096 * rs));
     TRY(ensure(h, h->d_PTA[ri], ne * 4096 * rs));
     TRY(ensure(h, h->d_Rsum[ri], ne * 64 * rs));
-    // adjoint chunking: aim for >= 2048 workgroups
+    // adjoint chunking: aim for >= 1024 workgroups (4 per CU), each covering a
+    // contiguous run of tiles
     int nb = std::max(1, h->n_branch);
-    int nch = std::min(h->n_tiles, std::max(1, (2048 + nb - 1) / nb));
+    int nch = std::min(h->n_tiles, std::max(1, (1024 + nb - 1) / nb));
     h->tiles_per_chunk = (h->n_tiles + nch - 1) / nch;
     h->n_chunks = (h->n_tiles + h->tiles_per_chunk - 1) / h->tiles_per_chunk;
     TRY(ensure(h, h->d_Apart, (size_t)nb * h->n_chunks * 4096 * 8));
@@ -206,6 +223,8 @@ int prepare_buffers(hy_engine h, int prec) {
     const int nm = h->n_mat;
     TRY(ensure(h, h->d_lam, (size_t)nm * 64 * 8));
     TRY(ensure(h, h->d_U, (size_t)nm * 4096 * 8));
+    if (!h->d_Z.p || h->d_Z.bytes < (size_t)nm * 4096 * 8) h->warm_valid = false;
+    TRY(ensure(h, h->d_Z, (size_t)nm * 4096 * 8));
     TRY(ensure(h, h->d_V, (size_t)nm * 4096 * 8));
     TRY(ensure(h, h->d_mu, (size_t)nm * 8));
     TRY(ensure(h, h->d_status, (size_t)nm * 4));
@@ -228,6 +247,99 @@ int prepare_buffers(hy_engine h, int prec) {
     }
     if (!h->h_status) HIP_TRY(h, hipHostMalloc((void **)&h->h_status, sizeof(int) * 64, hipHostMallocDefault));
     return HY_OK;
+}
+
+// Inside / outside schedules with LDS slot assignment for tile width T and
+// element size esz.
+void build_schedules(hy_engine h, int T, int esz) {
+    const int NSI = inside_slots(T, esz), NSO = outside_slots(T, esz);
+    const auto &ch = h->ext_children;
+    const int n_ext = (int)ch.size();
+    auto ref_of = [&](int c) -> int32_t {
+        if (ch[c].empty()) {
+            const int row = (h->leaf_taxon_ext[c] >= 0) ? h->leaf_taxon_ext[c] : h->num_taxa;
+            return (KIND_TIP << 28) | row;
+        }
+        return ((h->ext_ident[c] ? KIND_IDENT : KIND_INT) << 28) | h->int_slot[c];
+    };
+    // ---- inside: free-list LDS slots, spill to HBM when none is free ----
+    std::vector<int32_t> where(n_ext, -1);  // LDS slot of a computed internal node (-1: HBM)
+    std::vector<int> free_slots;
+    for (int s = NSI - 1; s >= 0; --s) free_slots.push_back(s);
+    h->sched.clear();
+    for (int u : h->order) {
+        int32_t e[SCHED_STRIDE] = {0};
+        e[0] = u;
+        e[1] = h->int_slot[u];
+        e[2] = (int32_t)ch[u].size();
+        for (size_t c = 0; c < ch[u].size(); ++c) {
+            const int cn = ch[u][c];
+            e[6 + 3 * c] = cn;
+            e[7 + 3 * c] = ref_of(cn);
+            e[8 + 3 * c] = ch[cn].empty() ? -1 : where[cn];
+        }
+        if (ch[u].empty()) {
+            // single-node tree: the root is a leaf whose partials are its tip
+            // indicator (one tip child on an identity edge, node id -1)
+            e[2] = 1;
+            e[6] = -1;
+            e[7] = ref_of(u);
+            e[8] = -1;
+        }
+        for (int cn : ch[u])
+            if (!ch[cn].empty() && where[cn] >= 0) free_slots.push_back(where[cn]);
+        int out = -1;
+        if (u != h->ext_root && !free_slots.empty()) {
+            out = free_slots.back();
+            free_slots.pop_back();
+        }
+        where[u] = out;
+        e[3] = out;
+        h->sched.insert(h->sched.end(), e, e + SCHED_STRIDE);
+    }
+    // ---- outside: reverse order; pending U messages in LDS slots ----
+    std::vector<int32_t> usrc(n_ext, U_SPILL);
+    usrc[h->ext_root] = U_ROOT;
+    free_slots.clear();
+    for (int s = NSO - 1; s >= 0; --s) free_slots.push_back(s);
+    h->osched.clear();
+    int spills = 0;
+    const int n_order = (int)h->order.size();
+    for (int i = n_order - 1; i >= 0; --i) {
+        const int u = h->order[i];
+        const int next = i > 0 ? h->order[i - 1] : -1;  // entry processed after u
+        int32_t e[OSCHED_STRIDE] = {0};
+        e[0] = u;
+        e[1] = h->int_slot[u];
+        e[2] = (int32_t)ch[u].size();
+        e[3] = (u == h->ext_root) ? -1 : h->vslot[u];
+        e[4] = h->ext_ident[u];
+        e[5] = usrc[u];
+        if (usrc[u] >= 0) free_slots.push_back(usrc[u]);
+        for (size_t c = 0; c < ch[u].size(); ++c) {
+            const int cn = ch[u][c];
+            e[6 + 4 * c] = cn;
+            e[7 + 4 * c] = ref_of(cn);
+            e[8 + 4 * c] = h->vslot[cn];
+            int dst = -1;
+            if (!ch[cn].empty()) {
+                if (cn == next) {
+                    dst = U_NEXT;
+                } else if (!free_slots.empty()) {
+                    dst = free_slots.back();
+                    free_slots.pop_back();
+                } else {
+                    dst = U_SPILL;
+                    ++spills;
+                }
+                usrc[cn] = dst;
+            }
+            e[9 + 4 * c] = dst;
+        }
+        h->osched.insert(h->osched.end(), e, e + OSCHED_STRIDE);
+    }
+    h->n_uspill = spills;
+    h->sched_key = T * 16 + esz;
 }
 
 void refresh_blist_classes(hy_engine h) {
@@ -262,6 +374,8 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         ma.U = (double *)h->d_U.p;
         ma.V = (double *)h->d_V.p;
         ma.mu = (double *)h->d_mu.p;
+        ma.Z = (double *)h->d_Z.p;
+        ma.warm = (h->warm_enable && h->warm_valid) ? 1 : 0;
         ma.status = (int *)h->d_status.p;
         launch_model(ma, h->n_mat, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[1], s));
@@ -301,6 +415,11 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     PruneArgs<R> pr{};
     pr.sched = (const int32_t *)h->d_sched.p;
     pr.n_sched = (int)(h->sched.size() / SCHED_STRIDE);
+    pr.osched = (const int32_t *)h->d_osched.p;
+    pr.n_osched = (int)(h->osched.size() / OSCHED_STRIDE);
+    pr.tip_rows = h->num_taxa + 1;
+    pr.S = h->S;
+    pr.Uspill = (R *)h->d_Uspill[ri].p;
     pr.PA = (const R *)h->d_PA[ri].p;
     pr.PTA = (const R *)h->d_PTA[ri].p;
     pr.Prow = (const R *)h->d_Prow[ri].p;
@@ -446,9 +565,9 @@ void hy_destroy(hy_engine h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     DevBuf *bufs[] = {&h->d_nd, &h->d_dpos, &h->d_dfrom, &h->d_dto, &h->d_syn, &h->d_pos, &h->d_pi,
-                      &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
+                      &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
-                      &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
+                      &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
                       &h->d_site, &h->d_Apart, &h->d_fr, &h->d_res};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
@@ -458,6 +577,7 @@ void hy_destroy(hy_engine h) {
         release(h, h->d_PTA[k]);
         release(h, h->d_D[k]);
         release(h, h->d_Vp[k]);
+        release(h, h->d_Uspill[k]);
     }
     if (h->h_params) (void)hipHostFree(h->h_params);
     if (h->h_res) (void)hipHostFree(h->h_res);
@@ -516,6 +636,7 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
     h->code_id = code_id;
     h->S = S;
     h->have_model = true;
+    h->warm_valid = false;
     return HY_OK;
 }
 
@@ -597,6 +718,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
         }
         if ((int)post.size() != n_ext) return fail(h, HY_ERR_ARG, "hy_set_problem: tree is not connected");
     }
+    // internal nodes get a global D index in post-order; non-root nodes a V slot
     std::vector<int32_t> int_slot(n_ext, -1), vslot(n_ext, -1);
     int n_int = 0, n_v = 0;
     for (int u : post) {
@@ -610,33 +732,37 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
         }
         return ((ident[c] ? KIND_IDENT : KIND_INT) << 28) | int_slot[c];
     };
-    std::vector<int32_t> sched;
+    // internal-node counts of subtrees (visit larger subtrees first)
+    std::vector<int> sub(n_ext, 0);
     for (int u : post) {
-        if (ch[u].empty()) continue;
-        int32_t e[SCHED_STRIDE] = {0};
-        e[0] = u;
-        e[1] = int_slot[u];
-        e[2] = (int32_t)ch[u].size();
-        e[3] = (u == root) ? -1 : vslot[u];
-        e[4] = ident[u];
-        for (size_t c = 0; c < ch[u].size(); ++c) {
-            const int cn = ch[u][c];
-            e[6 + 3 * c] = cn;
-            e[7 + 3 * c] = ref_of(cn);
-            e[8 + 3 * c] = vslot[cn];
-        }
-        sched.insert(sched.end(), e, e + SCHED_STRIDE);
+        int s = ch[u].empty() ? 0 : 1;
+        for (int c : ch[u]) s += sub[c];
+        sub[u] = s;
     }
-    if (sched.empty()) {
-        // single-node tree: root is a leaf; prune it as an internal node with no children
-        int32_t e[SCHED_STRIDE] = {0};
-        e[0] = root;
-        e[1] = 0;
-        e[2] = 0;
-        e[3] = -1;
-        sched.insert(sched.end(), e, e + SCHED_STRIDE);
+    std::vector<int32_t> order;  // internal nodes, post-order, larger subtrees first
+    {
+        std::vector<std::pair<int, int>> st{{root, 0}};
+        std::vector<std::vector<int32_t>> vis(n_ext);
+        while (!st.empty()) {
+            auto &[node, k] = st.back();
+            if (k == 0) {
+                vis[node] = ch[node];
+                std::stable_sort(vis[node].begin(), vis[node].end(), [&](int x, int y) { return sub[x] > sub[y]; });
+            }
+            if (k < (int)vis[node].size()) {
+                const int c = vis[node][k++];
+                if (!ch[c].empty()) st.push_back({c, 0});
+            } else {
+                if (!ch[node].empty()) order.push_back(node);
+                st.pop_back();
+            }
+        }
+    }
+    if (order.empty()) {
+        // single-node tree: the root is a leaf; prune it as an internal node with no children
         n_int = 1;
         int_slot[root] = 0;
+        order.push_back(root);
     }
     std::vector<int32_t> blist;
     for (int b = 0; b < num_nodes; ++b) {
@@ -688,7 +814,16 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     h->n_int = n_int;
     h->n_vslot = std::max(n_v, 1);
     h->n_branch = (int)(blist.size() / BR_STRIDE);
-    h->sched = std::move(sched);
+    h->order = std::move(order);
+    h->ext_children = ch;
+    h->ext_ident = ident;
+    h->int_slot = int_slot;
+    h->vslot = vslot;
+    h->ext_root = root;
+    h->leaf_taxon_ext.assign(n_ext, -1);
+    for (int n = 0; n < num_nodes; ++n) h->leaf_taxon_ext[n] = leaf_taxon[n];
+    h->sched.clear();
+    h->sched_key = 0;
     h->blist = std::move(blist);
     h->real_parent.assign(parent, parent + num_nodes);
     h->codes = std::move(codes);
@@ -704,7 +839,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     h->comp_ptr = {0, 1};
     h->comp_mat = {0};
     refresh_blist_classes(h);
-    TRY(upload(h, h->d_sched, h->sched.data(), h->sched.size() * 4));
+    h->warm_valid = false;
     TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), 4) * 4));
     TRY(upload(h, h->d_amb, h->amb.data(), h->amb.size() * 8));
     TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
@@ -753,6 +888,7 @@ int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_cl
     h->comp_mat.assign(comp_mat, comp_mat + n_comp);
     h->branch_class = bc;
     refresh_blist_classes(h);
+    h->warm_valid = false;
     TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), 4) * 4));
     TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
     TRY(upload(h, h->d_comp_mat, h->comp_mat.data(), h->comp_mat.size() * 4));
@@ -785,10 +921,14 @@ static int finish_eval(hy_engine h, bool check_status, double *out_logL, double 
         (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[7]);
         h->timings[7] = ms;
     }
-    if (check_status)
+    if (check_status) {
+        h->warm_valid = true;
         for (int m = 0; m < h->n_mat; ++m)
-            if (h->h_status[m] < 0)
+            if (h->h_status[m] < 0) {
+                h->warm_valid = false;
                 return fail(h, HY_ERR_NUMERIC, "symmetric eigensolver did not converge for rate matrix %d", m);
+            }
+    }
     const double *r = h->h_res;
     if (out_logL) *out_logL = r[0];
     if (grad) {
@@ -859,6 +999,20 @@ int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *
                                              : run_eval<float>(h, HY_WANT_LOGL, pp.data());
     if (rc) return rc;
     return finish_eval(h, false, out_logL, out_site_ll, nullptr, nullptr, nullptr, nullptr, false);
+}
+
+int hy_set_warm_start(hy_engine h, int enable) {
+    if (!h) return HY_ERR_ARG;
+    h->warm_enable = enable != 0;
+    if (!h->warm_enable) h->warm_valid = false;
+    return HY_OK;
+}
+
+int hy_last_eigen_status(hy_engine h, int *status, int n) {
+    if (!h || !status) return HY_ERR_ARG;
+    if (!h->h_status) return fail(h, HY_ERR_STATE, "no evaluation yet");
+    for (int m = 0; m < n && m < h->n_mat; ++m) status[m] = h->h_status[m];
+    return HY_OK;
 }
 
 int hy_last_timings(hy_engine h, double *ms8) {

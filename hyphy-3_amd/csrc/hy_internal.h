@@ -14,13 +14,36 @@ constexpr int REF_MASK = 0x0FFFFFFF;
 constexpr uint16_t TIP_ONES = 0xFFFFu;  // all-ones partial (gap, unmatched leaf, padding)
 constexpr uint16_t TIP_AMB = 0x8000u;   // | index into the ambiguity-mask table
 
-// Schedule entry layout (int32 x SCHED_STRIDE), one per internal node in
-// post-order: [0] node, [1] D slot, [2] #children (<= 3), [3] V slot of the
-// node (-1 at the root), [4] 1 if the edge above the node is an identity
-// edge, [5] unused, then per child c: [6+3c] node id of the child (selects
-// its edge matrix), [7+3c] child reference, [8+3c] V slot of the child.
+// Inside schedule entry (int32 x SCHED_STRIDE), one per internal node in a
+// post-order that visits larger subtrees first: [0] node, [1] global D index,
+// [2] #children (<= 3), [3] LDS slot receiving the node's partials (-1: HBM
+// only), [4..5] unused, then per child c: [6+3c] node id of the child
+// (selects its edge matrix), [7+3c] child reference (kind << 28 | tip row or
+// global D index), [8+3c] LDS slot holding the child's partials (-1: reload
+// from HBM).
 constexpr int SCHED_STRIDE = 16;
 constexpr int MAXC = 3;
+
+// Outside schedule entry (int32 x OSCHED_STRIDE), reverse of the inside
+// order: [0] node, [1] global D index, [2] #children, [3] V slot of the node
+// (-1 at the root), [4] 1 if the edge above the node is an identity edge,
+// [5] source of the node's U message (U_ROOT, U_SPILL or an LDS slot), then per
+// child c: [6+4c] node id, [7+4c] child reference, [8+4c] V slot of the
+// child, [9+4c] destination of the child's U (LDS slot, U_SPILL, or -1 for tips).
+constexpr int OSCHED_STRIDE = 20;
+constexpr int U_ROOT = -3;
+constexpr int U_SPILL = -2;
+constexpr int U_NEXT = -4;  // message handed to the next entry in registers
+
+// LDS slot capacities of the pruning kernels for tile width T and element
+// size esz (bytes of partials per slot = 64 * T * esz); each kernel stays
+// within the 160 KiB of LDS a gfx950 workgroup may use.
+constexpr int inside_slots(int T, int esz) { return T == 16 ? 12 : (T == 32 ? 6 : 3); }
+constexpr int outside_slots(int T, int esz) { return T == 16 ? 10 : (T == 32 ? 4 : (esz == 8 ? 1 : 2)); }
+// Tip codes of a tile kept in LDS when (taxa + 1) * T fits.
+constexpr int tip_lds_codes(int T, int esz) {
+    return T == 16 ? 12288 : (esz == 4 ? 24576 : (T == 32 ? 16384 : 8192));
+}
 
 // Branch list entry (int32 x 4): node, V slot, child reference, class.
 constexpr int BR_STRIDE = 4;
@@ -44,7 +67,9 @@ struct ModelArgs {
     double *U;    // [n_mat][64][64]
     double *V;    // [n_mat][64][64]
     double *mu;   // [n_mat]
-    int *status;  // [n_mat] sweeps used, negative = no convergence
+    double *Z;    // [n_mat][64][64] eigenvectors of the symmetrised generator (warm-start state)
+    int warm;     // 1: refine Z from the previous call
+    int *status;  // [n_mat] Jacobi sweeps, 100+k after k refinement steps, negative = no convergence
 };
 
 template <typename R>
@@ -68,6 +93,11 @@ template <typename R>
 struct PruneArgs {
     const int32_t *sched;
     int n_sched;
+    const int32_t *osched;
+    int n_osched;
+    int tip_rows;  // taxa + 1 (all-ones row)
+    int S;
+    R *Uspill;     // [n_int][P_pad][64] U messages that did not fit in LDS
     const R *PA, *PTA, *Prow, *Rsum;
     const uint16_t *tips;  // [rows][P_pad]
     const uint64_t *amb;

@@ -1,8 +1,9 @@
 // HIP kernels of the MI355X MG94xREV codon likelihood engine (gfx950).
 //
 // Pipeline per evaluation (one stream):
-//   k_model    one workgroup per rate matrix: MG94 Q, mu, symmetric spectrum by
-//              parallel cyclic Jacobi in LDS (FP64)
+//   k_model    one workgroup per rate matrix: MG94 Q, mu, symmetric spectrum
+//              (warm-started Ogita-Aishima refinement on FP64 MFMA, or
+//              parallel cyclic Jacobi in LDS when cold)
 //   k_pmat     one workgroup per branch: P_b = sum_j w_j exp(Q_j tau_b) in FP64,
 //              written as MFMA A-operand tiles of P and P^T in the working type
 //   k_inside   one workgroup per pattern tile: post-order pruning, 64x64 by
@@ -68,17 +69,78 @@ __device__ __forceinline__ double nuc_rate(const double *th, int a, int b) {
 }
 
 // ---------------------------------------------------------------------------
-// k_model: Q, mu and the symmetric spectrum of D^1/2 Q D^-1/2.
+// k_model: Q, mu and the symmetric spectrum of D^1/2 Q D^-1/2 (FP64).
+//
+// Warm start (a.warm): the previous call's eigenvectors X are refined by the
+// Ogita-Aishima iteration (Ogita & Aishima, "Iterative refinement for symmetric
+// eigenvalue decomposition", JJIAM 35, 2018): R = I - X^T X, S = X^T A X,
+// l_i = s_ii / (1 - r_ii), E_ij = (s_ij + l_j r_ij) / (l_j - l_i) for separated
+// pairs and r_ij / 2 inside clusters, X <- X + X E; its four 64x64x64 products
+// run on FP64 MFMA.  Accepted once X^T A X is diagonal to 1e-9 ||A|| and
+// the update is below 1e-9 (the remaining error is then second order); after
+// eight steps, or when the iteration stops contracting, the matrix is solved
+// cold.
+// Cold: parallel cyclic Jacobi (round-robin pairing, 32 disjoint rotations per
+// round), rotations skipped below 1e-17 ||A||_F, converged after a sweep with
+// no rotation.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_model(ModelArgs a) {
-    const int m = blockIdx.x, tid = threadIdx.x;
-    __shared__ double A[64 * 64];
-    __shared__ double Z[64 * 64];
-    __shared__ double sq[64], isq[64];
+constexpr int LDA = 65;  // padded LDS row stride (doubles) for 64x64 FP64 tiles
+
+// C = op(X) * op(Y) for 64x64 FP64 tiles in LDS (row stride LDA) on
+// v_mfma_f64_16x16x4; NW waves share the 16 output blocks (wave w takes row
+// block w % 4 and every (NW/4)-th column block).
+template <bool TX, bool TY, int NW>
+__device__ __forceinline__ void gemm64(const double *X, const double *Y, double *C, int wave, int lr, int lg) {
+    constexpr int CS = NW / 4;
+    constexpr int NCB = 4 / CS;
+    const int rb = wave & 3, cb0 = wave >> 2;
+    f64x4 acc[NCB];
+#pragma unroll
+    for (int q = 0; q < NCB; ++q) acc[q] = splat<f64x4>(0.0);
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+        const int i = 16 * rb + lr, k = 4 * kk + lg;
+        const double av = TX ? X[k * LDA + i] : X[i * LDA + k];
+#pragma unroll
+        for (int q = 0; q < NCB; ++q) {
+            const int j = 16 * (cb0 + q * CS) + lr;
+            const double bv = TY ? Y[j * LDA + k] : Y[k * LDA + j];
+            acc[q] = MF<double>::mma(av, bv, acc[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NCB; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) C[(16 * rb + MF<double>::crow(lg, r)) * LDA + 16 * (cb0 + q * CS) + lr] = acc[q][r];
+}
+
+// Deterministic block sum of one double per thread (NT threads); red has NT
+// entries; every thread gets the result.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *red, int tid) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < NT / 64; ++k) s += red[k];
+    __syncthreads();
+    return s;
+}
+
+constexpr int MODEL_NT = 1024;
+__global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs a) {
+    constexpr int NT = MODEL_NT;
+    const int m = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    __shared__ double A[64 * LDA];
+    __shared__ double Z[64 * LDA];
+    __shared__ double T1[64 * LDA];
+    __shared__ double T2[64 * LDA];
+    __shared__ double sq[64], isq[64], lam[64];
     __shared__ double rc[32], rs[32], rt[32];
     __shared__ int rp[32], rq[32];
-    __shared__ double red[256];
-    __shared__ int nrot_sweep;
+    __shared__ double red[NT / 64];
+    __shared__ int any_rot;
 
     const double *prm = a.params;
     const double alpha = prm[a.pl.off_alpha], beta = prm[a.pl.off_omega + m];
@@ -86,7 +148,7 @@ __global__ __launch_bounds__(256) void k_model(ModelArgs a) {
     const double delta = prm[a.pl.off_delta], psi = prm[a.pl.off_psi];
     const int S = a.S;
 
-    for (int idx = tid; idx < 4096; idx += 256) {
+    for (int idx = tid; idx < 4096; idx += NT) {
         const int i = idx >> 6, j = idx & 63;
         double q = 0.0;
         if (i < S && j < S && i != j) {
@@ -104,14 +166,14 @@ __global__ __launch_bounds__(256) void k_model(ModelArgs a) {
                     (a.pos[dp[0] * 4 + dt[0]] * a.pos[dp[1] * 4 + dt[1]] * a.pos[dp[2] * 4 + dt[2]]);
             }
         }
-        A[idx] = q;
+        A[i * LDA + j] = q;
     }
     __syncthreads();
     if (tid < 64) {
         double s = 0.0;
         for (int j = 0; j < 64; ++j)
-            if (j != tid) s += A[tid * 64 + j];
-        A[tid * 64 + tid] = -s;
+            if (j != tid) s += A[tid * LDA + j];
+        A[tid * LDA + tid] = -s;
         const double p = a.pi[tid];
         if (tid < S) {
             const double v = sqrt(p > 1e-12 ? p : 1e-12);
@@ -125,119 +187,222 @@ __global__ __launch_bounds__(256) void k_model(ModelArgs a) {
     __syncthreads();
     if (tid == 0) {
         double mu = 0.0;
-        for (int i = 0; i < S; ++i) mu -= a.pi[i] * A[i * 64 + i];
+        for (int i = 0; i < S; ++i) mu -= a.pi[i] * A[i * LDA + i];
         a.mu[m] = mu;
     }
-    for (int idx = tid; idx < 4096; idx += 256) {
+    for (int idx = tid; idx < 4096; idx += NT) {
         const int i = idx >> 6, j = idx & 63;
-        Z[idx] = sq[i] * A[idx] * isq[j];
+        Z[i * LDA + j] = sq[i] * A[i * LDA + j] * isq[j];
     }
     __syncthreads();
     double fro = 0.0;
-    for (int idx = tid; idx < 4096; idx += 256) {
+    for (int idx = tid; idx < 4096; idx += NT) {
         const int i = idx >> 6, j = idx & 63;
-        const double v = 0.5 * (Z[i * 64 + j] + Z[j * 64 + i]);
-        A[idx] = v;
+        const double v = 0.5 * (Z[i * LDA + j] + Z[j * LDA + i]);
+        T1[i * LDA + j] = v;
         fro += v * v;
     }
-    red[tid] = fro;
-    __syncthreads();
-    for (int idx = tid; idx < 4096; idx += 256) Z[idx] = ((idx >> 6) == (idx & 63)) ? 1.0 : 0.0;
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
+    const double anorm = sqrt(block_sum<NT>(fro, red, tid));
+    for (int idx = tid; idx < 4096; idx += NT) {
+        const int i = idx >> 6, j = idx & 63;
+        A[i * LDA + j] = T1[i * LDA + j];
     }
-    const double thresh = 1e-17 * sqrt(red[0]);
+    __syncthreads();
 
-    int sweep = 0, converged = 0;
-    for (; sweep < 40 && !converged; ++sweep) {
-        if (tid == 0) nrot_sweep = 0;
+    int status = 0;
+    bool done = false;
+    double *Zg = a.Z + (size_t)m * 4096;
+    if (a.warm) {
+        // ---- Ogita-Aishima refinement of the previous eigenvectors ----
+        for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = Zg[idx];
         __syncthreads();
-        for (int round = 0; round < 63; ++round) {
-            int active = 0;
-            if (tid < 32) {
-                int p, q;
-                if (tid == 0) {
-                    p = 63;
-                    q = round;
-                } else {
-                    p = (round + tid) % 63;
-                    q = (round - tid + 63) % 63;
-                }
-                const double apq = A[p * 64 + q];
-                double c = 1.0, s = 0.0, t = 0.0;
-                if (fabs(apq) > thresh) {
-                    const double app = A[p * 64 + p], aqq = A[q * 64 + q];
-                    const double theta = (aqq - app) / (2.0 * apq);
-                    if (fabs(theta) > 1e150) {
-                        t = 0.5 / theta;
-                    } else {
-                        t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                    }
-                    c = 1.0 / sqrt(t * t + 1.0);
-                    s = t * c;
-                    active = 1;
-                }
-                rp[tid] = p;
-                rq[tid] = q;
-                rc[tid] = c;
-                rs[tid] = s;
-                rt[tid] = t;
-                if (active) atomicAdd(&nrot_sweep, 1);
+        double prev_en = 1e300;
+        for (int it = 0; it < 8 && !done; ++it) {
+            gemm64<true, false, NT / 64>(Z, Z, T1, w, lr, lg);  // X^T X
+            gemm64<false, false, NT / 64>(A, Z, T2, w, lr, lg);  // A X
+            __syncthreads();
+            // R = I - X^T X (into T1)
+            for (int idx = tid; idx < 4096; idx += NT) {
+                const int i = idx >> 6, j = idx & 63;
+                T1[i * LDA + j] = (i == j ? 1.0 : 0.0) - T1[i * LDA + j];
             }
-            if (!__syncthreads_or(active)) continue;
-            for (int b = tid; b < 1024; b += 256) {
-                const int I = b >> 5, J = b & 31;
-                if (I > J) continue;
-                const double sI = rs[I], sJ = rs[J];
-                if (sI == 0.0 && sJ == 0.0) continue;
-                const double cI = rc[I], cJ = rc[J];
-                const int p = rp[I], q = rq[I], r = rp[J], s = rq[J];
-                if (I == J) {
-                    const double apq = A[p * 64 + q], t = rt[I];
-                    A[p * 64 + p] -= t * apq;
-                    A[q * 64 + q] += t * apq;
-                    A[p * 64 + q] = 0.0;
-                    A[q * 64 + p] = 0.0;
-                } else {
-                    const double xpr = A[p * 64 + r], xps = A[p * 64 + s], xqr = A[q * 64 + r], xqs = A[q * 64 + s];
-                    const double ypr = cI * xpr - sI * xqr, yps = cI * xps - sI * xqs;
-                    const double yqr = sI * xpr + cI * xqr, yqs = sI * xps + cI * xqs;
-                    const double zpr = cJ * ypr - sJ * yps, zps = sJ * ypr + cJ * yps;
-                    const double zqr = cJ * yqr - sJ * yqs, zqs = sJ * yqr + cJ * yqs;
-                    A[p * 64 + r] = zpr; A[r * 64 + p] = zpr;
-                    A[p * 64 + s] = zps; A[s * 64 + p] = zps;
-                    A[q * 64 + r] = zqr; A[r * 64 + q] = zqr;
-                    A[q * 64 + s] = zqs; A[s * 64 + q] = zqs;
+            __syncthreads();
+            // S = X^T (A X): computed into Z-sized registers, stored after a barrier
+            {
+                constexpr int CS = (NT / 64) / 4, NCB = 4 / CS;
+                const int rb = w & 3, cb0 = w >> 2;
+                f64x4 acc[NCB];
+#pragma unroll
+                for (int q = 0; q < NCB; ++q) acc[q] = splat<f64x4>(0.0);
+                for (int kk = 0; kk < 16; ++kk) {
+                    const int i = 16 * rb + lr, k = 4 * kk + lg;
+                    const double av = Z[k * LDA + i];
+#pragma unroll
+                    for (int q = 0; q < NCB; ++q) acc[q] = MF<double>::mma(av, T2[k * LDA + 16 * (cb0 + q * CS) + lr], acc[q]);
                 }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < NCB; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) T2[(16 * rb + MF<double>::crow(lg, r)) * LDA + 16 * (cb0 + q * CS) + lr] = acc[q][r];
             }
-            for (int idx = tid; idx < 2048; idx += 256) {
-                const int k = idx & 31, i = idx >> 5;
-                const double s = rs[k];
-                if (s == 0.0) continue;
-                const double c = rc[k];
-                const int p = rp[k], q = rq[k];
-                const double zp = Z[i * 64 + p], zq = Z[i * 64 + q];
-                Z[i * 64 + p] = c * zp - s * zq;
-                Z[i * 64 + q] = s * zp + c * zq;
+            __syncthreads();
+            if (tid < 64) lam[tid] = T2[tid * LDA + tid] / (1.0 - T1[tid * LDA + tid]);
+            // cluster threshold delta = 2 (||S - D||_F + ||A||_F ||R||_F)
+            double so = 0.0, rr = 0.0;
+            for (int idx = tid; idx < 4096; idx += NT) {
+                const int i = idx >> 6, j = idx & 63;
+                const double r = T1[i * LDA + j];
+                rr += r * r;
+                if (i != j) so += T2[i * LDA + j] * T2[i * LDA + j];
+            }
+            const double soff = sqrt(block_sum<NT>(so, red, tid));
+            const double rnorm = sqrt(block_sum<NT>(rr, red, tid));
+            const double dlt = 2.0 * (soff + anorm * rnorm);
+            __syncthreads();
+            // E into T1 (overwrites R after reading it)
+            double en = 0.0;
+            for (int idx = tid; idx < 4096; idx += NT) {
+                const int i = idx >> 6, j = idx & 63;
+                const double r = T1[i * LDA + j];
+                double e;
+                if (i == j) {
+                    e = 0.5 * r;
+                } else {
+                    const double d = lam[j] - lam[i];
+                    e = (fabs(d) > dlt) ? (T2[i * LDA + j] + lam[j] * r) / d : 0.5 * r;
+                }
+                T2[i * LDA + j] = e;  // T2 (S) is no longer needed after this element
+                en += e * e;
+            }
+            const double enorm = sqrt(block_sum<NT>(en, red, tid));
+            // X <- X + X E
+            gemm64<false, false, NT / 64>(Z, T2, T1, w, lr, lg);
+            __syncthreads();
+            for (int idx = tid; idx < 4096; idx += NT) {
+                const int i = idx >> 6, j = idx & 63;
+                Z[i * LDA + j] += T1[i * LDA + j];
+            }
+            __syncthreads();
+            if (!(enorm < 0.5) || (it >= 3 && enorm > 0.5 * prev_en)) break;  // not converging: solve cold
+            prev_en = enorm;
+            // Accept only when X actually diagonalises A (off-diagonal of X^T A X
+            // below 1e-9 ||A||) and the correction is second-order small: with a
+            // large cluster threshold every pair is frozen and E alone says nothing.
+            if (soff <= 1e-9 * anorm && enorm < 1e-9) {
+                done = true;
+                status = 100 + it + 1;
+            }
+        }
+        if (done) {
+            if (tid < 64) {
+                const double l = lam[tid];
+                lam[tid] = l > 0.0 ? 0.0 : l;
             }
             __syncthreads();
         }
+    }
+    if (!done) {
+        // ---- cold: parallel cyclic Jacobi ----
+        for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = ((idx >> 6) == (idx & 63)) ? 1.0 : 0.0;
+        const double thresh = 1e-17 * anorm;
+        int sweep = 0;
+        bool conv = false;
         __syncthreads();
-        converged = (nrot_sweep == 0);
+        for (; sweep < 40 && !conv; ++sweep) {
+            if (tid == 0) any_rot = 0;
+            __syncthreads();
+            for (int round = 0; round < 63; ++round) {
+                int active = 0;
+                if (tid < 32) {
+                    int p, q;
+                    if (tid == 0) {
+                        p = 63;
+                        q = round;
+                    } else {
+                        p = (round + tid) % 63;
+                        q = (round - tid + 63) % 63;
+                    }
+                    const double apq = A[p * LDA + q];
+                    double c = 1.0, s = 0.0, t = 0.0;
+                    if (fabs(apq) > thresh) {
+                        const double app = A[p * LDA + p], aqq = A[q * LDA + q];
+                        const double theta = (aqq - app) / (2.0 * apq);
+                        if (fabs(theta) > 1e150)
+                            t = 0.5 / theta;
+                        else
+                            t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        s = t * c;
+                        active = 1;
+                        any_rot = 1;
+                    }
+                    rp[tid] = p;
+                    rq[tid] = q;
+                    rc[tid] = c;
+                    rs[tid] = s;
+                    rt[tid] = t;
+                }
+                if (!__syncthreads_or(active)) continue;
+                // 2x2 blocks (I, J), I <= J, diagonal-major thread mapping.
+                for (int b = tid; b < 1024; b += NT) {
+                    const int I = b & 31, J = ((b >> 5) + I) & 31;
+                    if (I > J) continue;
+                    const double sI = rs[I], sJ = rs[J];
+                    if (sI == 0.0 && sJ == 0.0) continue;
+                    const double cI = rc[I], cJ = rc[J];
+                    const int p = rp[I], q = rq[I], r = rp[J], s = rq[J];
+                    if (I == J) {
+                        const double apq = A[p * LDA + q], t = rt[I];
+                        A[p * LDA + p] -= t * apq;
+                        A[q * LDA + q] += t * apq;
+                        A[p * LDA + q] = 0.0;
+                        A[q * LDA + p] = 0.0;
+                    } else {
+                        const double xpr = A[p * LDA + r], xps = A[p * LDA + s], xqr = A[q * LDA + r], xqs = A[q * LDA + s];
+                        const double ypr = cI * xpr - sI * xqr, yps = cI * xps - sI * xqs;
+                        const double yqr = sI * xpr + cI * xqr, yqs = sI * xps + cI * xqs;
+                        const double zpr = cJ * ypr - sJ * yps, zps = sJ * ypr + cJ * yps;
+                        const double zqr = cJ * yqr - sJ * yqs, zqs = sJ * yqr + cJ * yqs;
+                        A[p * LDA + r] = zpr; A[r * LDA + p] = zpr;
+                        A[p * LDA + s] = zps; A[s * LDA + p] = zps;
+                        A[q * LDA + r] = zqr; A[r * LDA + q] = zqr;
+                        A[q * LDA + s] = zqs; A[s * LDA + q] = zqs;
+                    }
+                }
+                for (int idx = tid; idx < 2048; idx += NT) {
+                    const int k = idx & 31, i = idx >> 5;
+                    const double s = rs[k];
+                    if (s == 0.0) continue;
+                    const double c = rc[k];
+                    const int p = rp[k], q = rq[k];
+                    const double zp = Z[i * LDA + p], zq = Z[i * LDA + q];
+                    Z[i * LDA + p] = c * zp - s * zq;
+                    Z[i * LDA + q] = s * zp + c * zq;
+                }
+                __syncthreads();
+            }
+            __syncthreads();
+            conv = (any_rot == 0);
+            __syncthreads();
+        }
+        status = conv ? sweep : -1;
+        if (tid < 64) {
+            const double l = A[tid * LDA + tid];
+            lam[tid] = l > 0.0 ? 0.0 : l;
+        }
         __syncthreads();
     }
-    double *lam = a.lam + m * 64, *U = a.U + m * 4096, *V = a.V + m * 4096;
-    if (tid < 64) {
-        const double l = A[tid * 64 + tid];
-        lam[tid] = l > 0.0 ? 0.0 : l;
-    }
-    for (int idx = tid; idx < 4096; idx += 256) {
+    double *lo = a.lam + m * 64, *U = a.U + m * 4096, *V = a.V + m * 4096;
+    if (tid < 64) lo[tid] = lam[tid];
+    for (int idx = tid; idx < 4096; idx += NT) {
         const int i = idx >> 6, k = idx & 63;
-        U[idx] = isq[i] * Z[idx];               // U[i][k]
-        V[k * 64 + i] = Z[idx] * sq[i];         // V[k][i]
+        const double z = Z[i * LDA + k];
+        Zg[idx] = z;
+        U[idx] = isq[i] * z;        // U[i][k]
+        V[k * 64 + i] = z * sq[i];  // V[k][i]
     }
-    if (tid == 0) a.status[m] = converged ? sweep : -1;
+    if (tid == 0) a.status[m] = status;
 }
 
 // Write one branch's P (FP64 in LDS) as row-major, row sums and the MFMA
@@ -248,7 +413,7 @@ __device__ __forceinline__ void emit_p(const double *P, int node, int S, R *Prow
     R *Prow = Prow_all + (size_t)node * 4096, *PA = PA_all + (size_t)node * 4096, *PTA = PTA_all + (size_t)node * 4096;
     for (int idx = tid; idx < 4096; idx += 256) {
         Prow[idx] = (R)P[idx];
-        const int w = idx >> 10, kk = (idx >> 6) & 15, l = idx & 63;
+        const int w = idx >> 10, l = (idx >> 4) & 63, kk = idx & 15;  // [w][l][kk]
         const int row = 16 * w + (l & 15), col = 4 * kk + (l >> 4);
         PA[idx] = (R)P[row * 64 + col];
         PTA[idx] = (R)P[col * 64 + row];
@@ -348,23 +513,43 @@ __device__ __forceinline__ void load_tile(R *dst, const R *__restrict__ src, int
     }
 }
 
-template <typename R>
-__device__ __forceinline__ R tip_val(const R *Pn, const R *Rn, const uint64_t *amb, unsigned code, int s) {
-    if (code < 64u) return Pn[s * 64 + code];
-    if (code == TIP_ONES) return Rn[s];
-    uint64_t msk = amb[code & 0x7FFFu];
-    R acc = 0;
-    while (msk) {
-        const int j = __builtin_ctzll(msk);
-        acc += Pn[s * 64 + j];
-        msk &= msk - 1;
+// Schedule entries are read through the scalar cache (constant address
+// space: the schedule is immutable while a kernel runs).
+#define HY_CONST __attribute__((address_space(4)))
+template <int N> struct Entry { int v[N]; };
+template <int N> __device__ __forceinline__ Entry<N> load_entry(const int32_t *sched, int k) {
+    const int32_t HY_CONST *p = (const int32_t HY_CONST *)(sched + (size_t)k * N);
+    Entry<N> e;
+#pragma unroll
+    for (int i = 0; i < N; ++i) e.v[i] = p[i];
+    return e;
+}
+// Vector-memory variant for prefetching: its completion is tracked apart
+// from LDS traffic; uniform() moves an arrived entry to scalar registers.
+template <int N> __device__ __forceinline__ Entry<N> load_entry_v(const int32_t *sched, int k) {
+    const int4 *p = reinterpret_cast<const int4 *>(sched + (size_t)k * N);
+    Entry<N> e;
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) {
+        const int4 q = p[i];
+        e.v[4 * i] = q.x;
+        e.v[4 * i + 1] = q.y;
+        e.v[4 * i + 2] = q.z;
+        e.v[4 * i + 3] = q.w;
     }
-    return acc;
+    return e;
+}
+template <int N> __device__ __forceinline__ Entry<N> uniform(const Entry<N> &x) {
+    Entry<N> e;
+#pragma unroll
+    for (int i = 0; i < N; ++i) e.v[i] = __builtin_amdgcn_readfirstlane(x.v[i]);
+    return e;
 }
 
 // Column-wise power-of-two normalisation of a 64xT tile held in C layout by
 // 4 waves (wave w holds rows 16w..16w+15).  ex[nb] receives the exponent e
 // with max in [0.5, 1) after scaling by 2^-e (0 for an all-zero column).
+// One barrier: consecutive calls must alternate between two cmax buffers.
 template <typename R, int T>
 __device__ __forceinline__ void normalize_cols(typename MF<R>::acc *x, int *ex, R *cmax, int w, int lr, int lg) {
     constexpr int NB = T / 16;
@@ -391,106 +576,272 @@ __device__ __forceinline__ void normalize_cols(typename MF<R>::acc *x, int *ex, 
         int e = 0;
         if (m > (R)0) e = frexp_e(m);
         ex[nb] = e;
-        x[nb][0] = ldexp_r(x[nb][0], -e);
-        x[nb][1] = ldexp_r(x[nb][1], -e);
-        x[nb][2] = ldexp_r(x[nb][2], -e);
-        x[nb][3] = ldexp_r(x[nb][3], -e);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[nb][r] = ldexp_r(x[nb][r], -e);
     }
-    __syncthreads();
 }
 
-// Child message X = P_child * D_child (or the tip gather) in C layout.
-template <typename R, int T>
-__device__ __forceinline__ void child_message(const PruneArgs<R> &a, int pn, int ref, int tile, int p0, R *Xs,
-                                              typename MF<R>::acc *X, int *Ex, int tid, int w, int l, int lr, int lg) {
-    constexpr int NB = T / 16;
-    typedef typename MF<R>::acc acc_t;
-    const int kind = ref >> 28, idx = ref & REF_MASK;
-    if (kind == KIND_TIP) {
-        const R *Pn = a.Prow + (size_t)pn * 4096;
-        const R *Rn = a.Rsum + (size_t)pn * 64;
+// Edge tile of node pn in MFMA A-operand order: lane l of wave w reads its 16
+// values as one contiguous run (PA/PTA layout [node][w][l][kk]).
+template <typename R>
+__device__ __forceinline__ void fetch_edge(R *dst, const R *__restrict__ tab, int pn, int w, int l) {
+    const R *p = tab + (((size_t)pn * 4 + w) * 64 + l) * 16;
+    if constexpr (sizeof(R) == 4) {
+        const float4 *q = reinterpret_cast<const float4 *>(p);
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const unsigned code = a.tips[(size_t)idx * a.P_pad + p0 + 16 * nb + lr];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) X[nb][r] = tip_val(Pn, Rn, a.amb, code, 16 * w + MF<R>::crow(lg, r));
-            Ex[nb] = 0;
+        for (int i = 0; i < 4; ++i) {
+            const float4 v = q[i];
+            dst[4 * i] = v.x;
+            dst[4 * i + 1] = v.y;
+            dst[4 * i + 2] = v.z;
+            dst[4 * i + 3] = v.w;
         }
-        return;
-    }
-    const R *src = a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T);
-    load_tile<R, T>(Xs, src, tid);
-    __syncthreads();
-    if (kind == KIND_IDENT) {
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) X[nb][r] = Xs[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr];
     } else {
-        const R *pa = a.PA + ((size_t)pn * 4 + w) * 1024;
+        const double2 *q = reinterpret_cast<const double2 *>(p);
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) X[nb] = splat<acc_t>(0.0);
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            const R av = pa[kk * 64 + l];
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) X[nb] = MF<R>::mma(av, Xs[(4 * kk + lg) * T + 16 * nb + lr], X[nb]);
+        for (int i = 0; i < 8; ++i) {
+            const double2 v = q[i];
+            dst[2 * i] = v.x;
+            dst[2 * i + 1] = v.y;
         }
     }
+}
+
+// X = P * B for the wave's 16 rows: A operand in registers, B a 64xT tile
+// (row stride T) in LDS or global memory.
+template <typename R, int T>
+__device__ __forceinline__ void mfma_tile(const R *av, const R *Bs, typename MF<R>::acc *X, int lr, int lg) {
+    constexpr int NB = T / 16;
+    typedef typename MF<R>::acc acc_t;
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) Ex[nb] = a.E[(size_t)idx * a.P_pad + p0 + 16 * nb + lr];
-    __syncthreads();
+    for (int nb = 0; nb < NB; ++nb) X[nb] = splat<acc_t>(0.0);
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) X[nb] = MF<R>::mma(av[kk], Bs[(4 * kk + lg) * T + 16 * nb + lr], X[nb]);
+}
+
+// X = P * B with the lane's B operands already in registers (bv[kk * NB + nb]).
+template <typename R, int T>
+__device__ __forceinline__ void mfma_regs(const R *av, const R *bv, typename MF<R>::acc *X) {
+    constexpr int NB = T / 16;
+    typedef typename MF<R>::acc acc_t;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) X[nb] = splat<acc_t>(0.0);
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) X[nb] = MF<R>::mma(av[kk], bv[kk * NB + nb], X[nb]);
+}
+
+// Tip child: X = P_pn * indicator(code) in C layout, gathered from the
+// row-major P (one column per definite codon) or its row sums (all-ones
+// tips).  Only issues loads: lanes whose code is partially ambiguous get bit
+// (slot * NB + nb) set in *amb_bits and are completed by fix_tip() when the
+// operand is consumed, so no wait is placed on the gathers here.
+template <typename R, int T, bool TL>
+__device__ __forceinline__ unsigned tip_code(const PruneArgs<R> &a, const uint16_t *tcodes, int row, int p0, int t) {
+    return TL ? tcodes[row * T + t] : a.tips[(size_t)row * a.P_pad + p0 + t];
+}
+
+template <typename R, int T, bool TL>
+__device__ __forceinline__ void gather_tip(typename MF<R>::acc *x, unsigned &amb_bits, int slot, bool is_tip,
+                                           const PruneArgs<R> &a, const uint16_t *tcodes, int pn, int row, int p0,
+                                           int w, int lr, int lg) {
+    // Branch-free in the loaded values (every slot issues its loads, non-tip
+    // slots from a harmless address) so that no register merge forces a wait
+    // on the gathers before they are consumed.
+    const bool gather = is_tip && pn >= 0;
+    const R *P = a.Prow + (size_t)(gather ? pn : 0) * 4096;
+    const R *Rs = a.Rsum + (size_t)(gather ? pn : 0) * 64;
+#pragma unroll
+    for (int nb = 0; nb < T / 16; ++nb) {
+        const unsigned code = is_tip ? tip_code<R, T, TL>(a, tcodes, row, p0, 16 * nb + lr) : 0u;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int s = 16 * w + MF<R>::crow(lg, r);
+            x[nb][r] = (gather && code < 64u) ? P[s * 64 + code] : Rs[s];
+        }
+        if (is_tip && (pn < 0 || (code >= 64u && code != TIP_ONES))) amb_bits |= 1u << (slot * (T / 16) + nb);
+    }
+}
+
+// Completes the partially ambiguous lanes of a gathered tip product:
+// X[s][t] = sum of P[s][j] over the compatible codons j.
+// pn < 0 (single-node tree): X is the tip's indicator itself.
+template <typename R, int T, bool TL>
+__device__ __forceinline__ void fix_tip(typename MF<R>::acc *x, unsigned amb_bits, int slot, const PruneArgs<R> &a,
+                                        const uint16_t *tcodes, int pn, int row, int p0, int w, int lr, int lg,
+                                        uint64_t all) {
+#pragma unroll
+    for (int nb = 0; nb < T / 16; ++nb) {
+        if (amb_bits & (1u << (slot * (T / 16) + nb))) {
+            const unsigned code = tip_code<R, T, TL>(a, tcodes, row, p0, 16 * nb + lr);
+            if (pn < 0) {
+                const uint64_t m = code < 64u ? (1ull << code) : (code == TIP_ONES ? all : a.amb[code & 0x7FFFu]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) x[nb][r] = (R)((m >> (16 * w + MF<R>::crow(lg, r))) & 1ull);
+                continue;
+            }
+            const uint64_t m0 = a.amb[code & 0x7FFFu] & all;
+            const R *P = a.Prow + (size_t)pn * 4096;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int s = 16 * w + MF<R>::crow(lg, r);
+                R acc = 0;
+                for (uint64_t m = m0; m; m &= m - 1) acc += P[s * 64 + __builtin_ctzll(m)];
+                x[nb][r] = acc;
+            }
+        }
+    }
+}
+
+template <typename R, int T>
+__device__ __forceinline__ void store_c(R *dst, const typename MF<R>::acc *x, int w, int lr, int lg) {
+#pragma unroll
+    for (int nb = 0; nb < T / 16; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr] = x[nb][r];
+}
+
+template <typename R, int T>
+__device__ __forceinline__ void load_c(typename MF<R>::acc *x, const R *src, int w, int lr, int lg) {
+#pragma unroll
+    for (int nb = 0; nb < T / 16; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[nb][r] = src[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr];
+}
+
+template <int T, bool TL>
+__device__ __forceinline__ void stage_tip_codes(uint16_t *tcodes, const uint16_t *tips, int rows, int P_pad, int p0,
+                                                int tid) {
+    if constexpr (TL)
+        for (int idx = tid; idx < rows * T; idx += 256) tcodes[idx] = tips[(size_t)(idx / T) * P_pad + p0 + idx % T];
 }
 
 // ---------------------------------------------------------------------------
-// k_inside: post-order pruning of one pattern tile.
+// k_inside: post-order pruning of one pattern tile.  Child partials stay in
+// LDS slots assigned by the host schedule (read straight from HBM for nodes
+// that did not get a slot); every node's partials are also written to HBM
+// for the outside pass and the adjoint.  The next node's operands (edge
+// tiles of internal children, gathered tip products) are fetched while the
+// current node is computed; schedule entries arrive two nodes ahead.
+// TL: the tile's tip codes are staged in LDS.
 // ---------------------------------------------------------------------------
-template <typename R, int T>
-__global__ __launch_bounds__(256) void k_inside(PruneArgs<R> a) {
-    constexpr int NB = T / 16;
+template <typename R, int T, bool TL>
+__global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_inside(PruneArgs<R> a) {
+    constexpr int NB = T / 16, NS = inside_slots(T, sizeof(R)), TIPC = TL ? tip_lds_codes(T, sizeof(R)) : 1;
+    constexpr bool PIPE = !(sizeof(R) == 8 && T == 64);  // register budget of double-buffered operands
+    typedef Entry<SCHED_STRIDE> Ent;
     typedef typename MF<R>::acc acc_t;
-    __shared__ R Xs[64 * T];
-    __shared__ R cmax[4 * T];
+    __shared__ R Ds[NS * 64 * T];
+    __shared__ int Es[NS * T];
+    __shared__ R cmax[2][4 * T];
     __shared__ double red[4 * T];
+    __shared__ uint16_t tcodes[TIPC];
     const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
-    const int p0 = tile * T;
+    const int p0 = tile * T, n = a.n_sched;
+    stage_tip_codes<T, TL>(tcodes, a.tips, a.tip_rows, a.P_pad, p0, tid);
+    const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
+    __syncthreads();
+    auto operands = [&](const Ent &e, R (*pa)[16], acc_t (*xt)[NB], unsigned &ab) {
+        const int nc = e.v[2];
+        ab = 0;
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            const int pn = e.v[6 + 3 * c], ref = e.v[7 + 3 * c], kind = ref >> 28;
+            if (c < nc && kind == KIND_INT) fetch_edge<R>(pa[c], a.PA, pn, w, l);
+            gather_tip<R, T, TL>(xt[c], ab, c, c < nc && kind == KIND_TIP, a, tcodes, pn, ref & REF_MASK, p0, w, lr,
+                                 lg);
+        }
+    };
+    Ent ec = load_entry<SCHED_STRIDE>(a.sched, 0);
+    Ent en = load_entry<SCHED_STRIDE>(a.sched, n > 1 ? 1 : 0);
+    R pa[MAXC][16], pa2[MAXC][16];
+    acc_t xt[MAXC][NB], xt2[MAXC][NB];
+    unsigned ab = 0, ab2 = 0;  // partially ambiguous tip lanes
+    operands(ec, pa, xt, ab);
+    int cb = 0;
     acc_t Dc[NB];
     int Ec[NB];
-    for (int k = 0; k < a.n_sched; ++k) {
-        const int32_t *e = a.sched + k * SCHED_STRIDE;
-        const int slot = e[1], nc = e[2];
+    for (int k = 0; k < n; ++k) {
+        const Ent enn = load_entry_v<SCHED_STRIDE>(a.sched, k + 2 < n ? k + 2 : n - 1);
+        if constexpr (PIPE) {
+            if (k + 1 < n) operands(en, pa2, xt2, ab2);
+        }
+        const int gidx = ec.v[1], nc = ec.v[2], out = ec.v[3];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             Dc[nb] = splat<acc_t>(1.0);
             Ec[nb] = 0;
         }
-        for (int c = 0; c < nc; ++c) {
-            acc_t X[NB];
-            int Ex[NB];
-            child_message<R, T>(a, e[6 + 3 * c], e[7 + 3 * c], tile, p0, Xs, X, Ex, tid, w, l, lr, lg);
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                Dc[nb] *= X[nb];
-                Ec[nb] += Ex[nb];
-            }
-            if (c >= 1 && c + 1 < nc) {
-                int ex[NB];
-                normalize_cols<R, T>(Dc, ex, cmax, w, lr, lg);
+        for (int c = 0; c < MAXC; ++c) {
+            if (c < nc) {
+                const int ref = ec.v[7 + 3 * c], src = ec.v[8 + 3 * c];
+                const int kind = ref >> 28, idx = ref & REF_MASK;
+                acc_t X[NB];
+                if (kind == KIND_TIP) {
+                    if (ab) fix_tip<R, T, TL>(xt[c], ab, c, a, tcodes, ec.v[6 + 3 * c], idx, p0, w, lr, lg, all);
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) Ec[nb] += ex[nb];
+                    for (int nb = 0; nb < NB; ++nb) X[nb] = xt[c][nb];
+                } else if (src >= 0) {  // partials in an LDS slot
+                    const R *Bs = Ds + src * 64 * T;
+                    if (kind == KIND_IDENT)
+                        load_c<R, T>(X, Bs, w, lr, lg);
+                    else
+                        mfma_tile<R, T>(pa[c], Bs, X, lr, lg);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) Ec[nb] += Es[src * T + 16 * nb + lr];
+                } else {  // partials only in HBM
+                    const R *Bg = a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T);
+                    if (kind == KIND_IDENT)
+                        load_c<R, T>(X, Bg, w, lr, lg);
+                    else
+                        mfma_tile<R, T>(pa[c], Bg, X, lr, lg);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) Ec[nb] += a.E[(size_t)idx * a.P_pad + p0 + 16 * nb + lr];
+                }
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) Dc[nb] *= X[nb];
+                if (c >= 1 && c + 1 < nc) {
+                    int ex[NB];
+                    normalize_cols<R, T>(Dc, ex, cmax[cb], w, lr, lg);
+                    cb ^= 1;
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) Ec[nb] += ex[nb];
+                }
             }
         }
         int ex[NB];
-        normalize_cols<R, T>(Dc, ex, cmax, w, lr, lg);
-        R *dst = a.D + ((size_t)slot * a.n_tiles + tile) * (64 * T);
+        normalize_cols<R, T>(Dc, ex, cmax[cb], w, lr, lg);
+        cb ^= 1;
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            Ec[nb] += ex[nb];
+        for (int nb = 0; nb < NB; ++nb) Ec[nb] += ex[nb];
+        store_c<R, T>(a.D + ((size_t)gidx * a.n_tiles + tile) * (64 * T), Dc, w, lr, lg);
+        if (out >= 0) store_c<R, T>(Ds + out * 64 * T, Dc, w, lr, lg);
+        if (w == 0 && lg == 0) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dst[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr] = Dc[nb][r];
-            if (w == 0 && lg == 0) a.E[(size_t)slot * a.P_pad + p0 + 16 * nb + lr] = Ec[nb];
+            for (int nb = 0; nb < NB; ++nb) {
+                a.E[(size_t)gidx * a.P_pad + p0 + 16 * nb + lr] = Ec[nb];
+                if (out >= 0) Es[out * T + 16 * nb + lr] = Ec[nb];
+            }
         }
         __syncthreads();
+        ec = en;
+        en = uniform(enn);
+        if constexpr (PIPE) {
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) {
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) pa[c][kk] = pa2[c][kk];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) xt[c][nb] = xt2[c][nb];
+            }
+            ab = ab2;
+        } else {
+            if (k + 1 < n) operands(ec, pa, xt, ab);
+        }
     }
     // Root likelihood: L_t = sum_s pi_s D_root[s][t] (FP64 accumulation).
 #pragma unroll
@@ -517,88 +868,265 @@ __global__ __launch_bounds__(256) void k_inside(PruneArgs<R> a) {
 }
 
 // ---------------------------------------------------------------------------
-// k_outside: pre-order outside messages V_c = U_u (.) prod_{siblings} X_s.
+// k_outside: pre-order outside messages V_c = U_u (.) prod_{siblings} X_s with
+// X_s = P_s D_s, U_c = P_c^T V_c.  The message of the node processed next
+// stays in registers, other pending messages in LDS slots (HBM when none is
+// free).  The next node's operands are fetched once the current node's
+// products are formed; with DREG the children's partials come along as MFMA
+// B operands, otherwise they are read from HBM when used.
 // ---------------------------------------------------------------------------
-template <typename R, int T>
+template <typename R, int T, bool TL>
 __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
-    constexpr int NB = T / 16;
+    constexpr int NB = T / 16, NS = outside_slots(T, sizeof(R)), TIPC = TL ? tip_lds_codes(T, sizeof(R)) : 1;
+    constexpr bool DREG = NB * sizeof(R) <= 8;  // f32 T <= 32, f64 T == 16
+    constexpr int DV = DREG ? 16 * NB : 1;
+    typedef Entry<OSCHED_STRIDE> Ent;
     typedef typename MF<R>::acc acc_t;
-    __shared__ R Xs[64 * T];
-    __shared__ R cmax[4 * T];
+    __shared__ R Us[NS * 64 * T];
+    __shared__ int Hs[NS * T];
+    __shared__ R Xc[MAXC * 64 * T];
+    __shared__ R cmax[2][MAXC * 4 * T];
+    __shared__ uint16_t tcodes[TIPC];
     const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
-    const int p0 = tile * T;
-    for (int k = a.n_sched - 1; k >= 0; --k) {
-        const int32_t *e = a.sched + k * SCHED_STRIDE;
-        const int node = e[0], nc = e[2], vs = e[3], ident = e[4];
-        acc_t U[NB];
-        int H[NB];
-        if (vs < 0) {
+    const int p0 = tile * T, n = a.n_osched;
+    stage_tip_codes<T, TL>(tcodes, a.tips, a.tip_rows, a.P_pad, p0, tid);
+    const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
+    __syncthreads();
+    R pa[MAXC][16], pt[MAXC][16], dv[MAXC][DV];
+    int de[MAXC][NB];
+    acc_t xt[MAXC][NB];
+    unsigned ab = 0;  // partially ambiguous tip lanes of the prefetched operands
+    acc_t upi[NB], us[NB];  // root message; prefetched message of a node whose U was spilled
+    int hs[NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
+    for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) U[nb][r] = (R)a.pi[16 * w + MF<R>::crow(lg, r)];
-                H[nb] = 0;
-            }
-        } else {
-            load_tile<R, T>(Xs, a.V + ((size_t)vs * a.n_tiles + tile) * (64 * T), tid);
-            __syncthreads();
-            if (ident) {
+        for (int r = 0; r < 4; ++r) upi[nb][r] = (R)a.pi[16 * w + MF<R>::crow(lg, r)];
+    auto operands = [&](const Ent &e) {
+        const int nc = e.v[2];
+        ab = 0;
+        {
+            // spilled U message (branch-free: other entries read a harmless tile)
+            const bool sp = e.v[5] == U_SPILL;
+            const R *ub = sp ? a.Uspill + ((size_t)e.v[1] * a.n_tiles + tile) * (64 * T) : a.D + (size_t)tile * (64 * T);
+            const int32_t *hb = sp ? a.G + (size_t)e.v[3] * a.P_pad : a.E;
+            load_c<R, T>(us, ub, w, lr, lg);
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
+            for (int nb = 0; nb < NB; ++nb) hs[nb] = hb[p0 + 16 * nb + lr];
+        }
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) U[nb][r] = Xs[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr];
-            } else {
-                const R *pa = a.PTA + ((size_t)node * 4 + w) * 1024;
+        for (int c = 0; c < MAXC; ++c) {
+            const int pn = e.v[6 + 4 * c], ref = e.v[7 + 4 * c], kind = ref >> 28, idx = ref & REF_MASK;
+            gather_tip<R, T, TL>(xt[c], ab, c, c < nc && kind == KIND_TIP, a, tcodes, pn, idx, p0, w, lr, lg);
+            if (c < nc) {
+                if (kind != KIND_TIP) {
+                    if (kind == KIND_INT) fetch_edge<R>(pa[c], a.PA, pn, w, l);
+                    const R *src = a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T);
+                    if constexpr (DREG) {
+                        if (kind == KIND_IDENT) {
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) U[nb] = splat<acc_t>(0.0);
+                            for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-                for (int kk = 0; kk < 16; ++kk) {
-                    const R av = pa[kk * 64 + l];
+                                for (int r = 0; r < 4; ++r)
+                                    dv[c][nb * 4 + r] = src[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr];
+                        } else {
 #pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) U[nb] = MF<R>::mma(av, Xs[(4 * kk + lg) * T + 16 * nb + lr], U[nb]);
+                            for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+                                for (int nb = 0; nb < NB; ++nb)
+                                    dv[c][kk * NB + nb] = src[(4 * kk + lg) * T + 16 * nb + lr];
+                        }
+                    }
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) de[c][nb] = a.E[(size_t)idx * a.P_pad + p0 + 16 * nb + lr];
                 }
             }
+        }
+    };
+    Ent ec = load_entry<OSCHED_STRIDE>(a.osched, 0);
+    Ent en = load_entry<OSCHED_STRIDE>(a.osched, n > 1 ? 1 : 0);
+    operands(ec);
+    acc_t Un[NB];
+    int Hn[NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) H[nb] = a.G[(size_t)vs * a.P_pad + p0 + 16 * nb + lr];
-            __syncthreads();
+    for (int nb = 0; nb < NB; ++nb) {
+        Un[nb] = splat<acc_t>(0.0);
+        Hn[nb] = 0;
+    }
+    int cb = 0;
+    for (int k = 0; k < n; ++k) {
+        const Ent enn = load_entry_v<OSCHED_STRIDE>(a.osched, k + 2 < n ? k + 2 : n - 1);
+        const int nc = ec.v[2], usrc = ec.v[5];
+        bool any_int = false;
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            if (c < nc) {
+                const int pn = ec.v[6 + 4 * c], kind = ec.v[7 + 4 * c] >> 28;
+                if (kind == KIND_INT) {
+                    any_int = true;
+                    fetch_edge<R>(pt[c], a.PTA, pn, w, l);
+                }
+            }
+        }
+        acc_t U[NB];
+        int H[NB];
+        if (usrc == U_ROOT) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                U[nb] = upi[nb];
+                H[nb] = 0;
+            }
+        } else if (usrc == U_NEXT) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                U[nb] = Un[nb];
+                H[nb] = Hn[nb];
+            }
+        } else if (usrc >= 0) {
+            load_c<R, T>(U, Us + usrc * 64 * T, w, lr, lg);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) H[nb] = Hs[usrc * T + 16 * nb + lr];
+        } else {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                U[nb] = us[nb];
+                H[nb] = hs[nb];
+            }
         }
         acc_t X[MAXC][NB];
         int Ex[MAXC][NB];
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c)
-            if (c < nc) child_message<R, T>(a, e[6 + 3 * c], e[7 + 3 * c], tile, p0, Xs, X[c], Ex[c], tid, w, l, lr, lg);
-#pragma unroll
         for (int c = 0; c < MAXC; ++c) {
-            if (c >= nc) continue;
-            acc_t Vv[NB];
-            int Gv[NB];
+            if (c < nc) {
+                const int ref = ec.v[7 + 4 * c];
+                const int kind = ref >> 28, idx = ref & REF_MASK;
+                if (kind == KIND_TIP) {
+                    if (ab) fix_tip<R, T, TL>(xt[c], ab, c, a, tcodes, ec.v[6 + 4 * c], idx, p0, w, lr, lg, all);
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                Vv[nb] = U[nb];
-                Gv[nb] = H[nb];
-            }
+                    for (int nb = 0; nb < NB; ++nb) {
+                        X[c][nb] = xt[c][nb];
+                        Ex[c][nb] = 0;
+                    }
+                } else {
+                    if constexpr (DREG) {
+                        if (kind == KIND_IDENT) {
 #pragma unroll
-            for (int c2 = 0; c2 < MAXC; ++c2) {
-                if (c2 == c || c2 >= nc) continue;
+                            for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    Vv[nb] *= X[c2][nb];
-                    Gv[nb] += Ex[c2][nb];
+                                for (int r = 0; r < 4; ++r) X[c][nb][r] = dv[c][nb * 4 + r];
+                        } else {
+                            mfma_regs<R, T>(pa[c], dv[c], X[c]);
+                        }
+                    } else {
+                        const R *src = a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T);
+                        if (kind == KIND_IDENT)
+                            load_c<R, T>(X[c], src, w, lr, lg);
+                        else
+                            mfma_tile<R, T>(pa[c], src, X[c], lr, lg);
+                    }
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) Ex[c][nb] = de[c][nb];
                 }
             }
-            int ex[NB];
-            normalize_cols<R, T>(Vv, ex, cmax, w, lr, lg);
-            const int vsc = e[8 + 3 * c];
-            R *dst = a.V + ((size_t)vsc * a.n_tiles + tile) * (64 * T);
+        }
+        // operands of the next node (the registers above are free now)
+        if (k + 1 < n) operands(en);
+        acc_t V[MAXC][NB];
+        int G[MAXC][NB];
+        R *cm = cmax[cb];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                Gv[nb] += ex[nb];
+        for (int c = 0; c < MAXC; ++c) {
+            if (c < nc) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) dst[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr] = Vv[nb][r];
-                if (w == 0 && lg == 0) a.G[(size_t)vsc * a.P_pad + p0 + 16 * nb + lr] = Gv[nb];
+                for (int nb = 0; nb < NB; ++nb) {
+                    V[c][nb] = U[nb];
+                    G[c][nb] = H[nb];
+                }
+#pragma unroll
+                for (int c2 = 0; c2 < MAXC; ++c2) {
+                    if (c2 != c && c2 < nc) {
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb) {
+                            V[c][nb] *= X[c2][nb];
+                            G[c][nb] += Ex[c2][nb];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    R mx = V[c][nb][0];
+                    mx = mx > V[c][nb][1] ? mx : V[c][nb][1];
+                    mx = mx > V[c][nb][2] ? mx : V[c][nb][2];
+                    mx = mx > V[c][nb][3] ? mx : V[c][nb][3];
+                    R o = __shfl_xor(mx, 16);
+                    mx = mx > o ? mx : o;
+                    o = __shfl_xor(mx, 32);
+                    mx = mx > o ? mx : o;
+                    if (lg == 0) cm[(c * 4 + w) * T + 16 * nb + lr] = mx;
+                }
             }
         }
         __syncthreads();
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            if (c < nc) {
+                const int ref = ec.v[7 + 4 * c], vsc = ec.v[8 + 4 * c];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    const int t = 16 * nb + lr;
+                    const R *cc = cm + c * 4 * T;
+                    R mx = cc[t];
+                    mx = mx > cc[T + t] ? mx : cc[T + t];
+                    mx = mx > cc[2 * T + t] ? mx : cc[2 * T + t];
+                    mx = mx > cc[3 * T + t] ? mx : cc[3 * T + t];
+                    const int ex = (mx > (R)0) ? frexp_e(mx) : 0;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) V[c][nb][r] = ldexp_r(V[c][nb][r], -ex);
+                    G[c][nb] += ex;
+                }
+                store_c<R, T>(a.V + ((size_t)vsc * a.n_tiles + tile) * (64 * T), V[c], w, lr, lg);
+                if ((ref >> 28) == KIND_INT) store_c<R, T>(Xc + c * 64 * T, V[c], w, lr, lg);
+                if (w == 0 && lg == 0) {
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) a.G[(size_t)vsc * a.P_pad + p0 + 16 * nb + lr] = G[c][nb];
+                }
+            }
+        }
+        cb ^= 1;
+        if (any_int) __syncthreads();
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            if (c < nc) {
+                const int ref = ec.v[7 + 4 * c], dst = ec.v[9 + 4 * c];
+                const int kind = ref >> 28, idx = ref & REF_MASK;
+                if (kind != KIND_TIP) {
+                    acc_t Uc[NB];
+                    if (kind == KIND_IDENT) {
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb) Uc[nb] = V[c][nb];
+                    } else {
+                        mfma_tile<R, T>(pt[c], Xc + c * 64 * T, Uc, lr, lg);
+                    }
+                    if (dst == U_NEXT) {
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb) {
+                            Un[nb] = Uc[nb];
+                            Hn[nb] = G[c][nb];
+                        }
+                    } else if (dst >= 0) {
+                        store_c<R, T>(Us + dst * 64 * T, Uc, w, lr, lg);
+                        if (w == 0 && lg == 0) {
+#pragma unroll
+                            for (int nb = 0; nb < NB; ++nb) Hs[dst * T + 16 * nb + lr] = G[c][nb];
+                        }
+                    } else {
+                        store_c<R, T>(a.Uspill + ((size_t)idx * a.n_tiles + tile) * (64 * T), Uc, w, lr, lg);
+                    }
+                }
+            }
+        }
+        ec = en;
+        en = uniform(enn);
     }
 }
 
@@ -754,64 +1282,49 @@ __global__ __launch_bounds__(256) void k_proj(ProjArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// k_frechet: per (branch, component) spectral projections (FP64).
-// out[0] = <A_b, P_m(tau)>, out[1] = w <A_b, Q_m P_m(tau)>,
-// out[2+phi] = w <M_phi (.) E_b, U^T A_b V^T>.
+// k_frechet: one workgroup per branch; for each mixture component j of the
+// branch's class (matrix m, weight w_j) the spectral projections in FP64:
+// out[0] = <A_b, P_m(tau)>, out[1] = w_j <A_b, Q_m P_m(tau)>,
+// out[2+phi] = w_j <M_phi (.) E_b, U^T A_b V^T>  (likelihood.hpp:1440-1520).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_frechet(FrechetArgs a) {
-    const int bi = blockIdx.x, jj = blockIdx.y, tid = threadIdx.x;
+    const int bi = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
-    const int J = a.comp_ptr[cls] + jj;
-    if (J >= a.comp_ptr[cls + 1]) return;
-    const int m = a.comp_mat[J];
-    const double wj = a.params[a.pl.off_w + J];
-    const double tau = a.tau[node];
-    __shared__ double As[64 * 64];
-    __shared__ double Bs[64 * 64];
+    __shared__ double As[64 * LDA];
+    __shared__ double Bs[64 * LDA];
+    __shared__ double Ts[64 * LDA];
+    __shared__ double At[64 * LDA];
     __shared__ double lam[64], ex[64];
-    __shared__ double red[8][4];
+    __shared__ double red[4];
+    const double tau = a.tau[node];
     for (int idx = tid; idx < 4096; idx += 256) {
         double s = 0.0;
         for (int c = 0; c < a.n_chunks; ++c) s += a.Apart[((size_t)bi * a.n_chunks + c) * 4096 + idx];
-        As[idx] = s;
-        Bs[idx] = a.V[m * 4096 + idx];
+        As[(idx >> 6) * LDA + (idx & 63)] = s;
     }
-    if (tid < 64) {
-        lam[tid] = a.lam[m * 64 + tid];
-        ex[tid] = exp(lam[tid] * tau);
-    }
-    __syncthreads();
-    const int i0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
-    double t[4][4];
-    for (int ii = 0; ii < 4; ++ii)
-        for (int q = 0; q < 4; ++q) t[ii][q] = 0.0;
-    // T1 = A V^T : T1[i][j] = sum_k A[i][k] V[j][k]
-    for (int k = 0; k < 64; ++k)
-        for (int ii = 0; ii < 4; ++ii) {
-            const double av = As[(i0 + ii) * 64 + k];
-            for (int q = 0; q < 4; ++q) t[ii][q] = fma(av, Bs[(c0 + q) * 64 + k], t[ii][q]);
+    const int j0 = a.comp_ptr[cls], j1 = a.comp_ptr[cls + 1];
+    for (int J = j0; J < j1; ++J) {
+        const int m = a.comp_mat[J];
+        const double wj = a.params[a.pl.off_w + J];
+        __syncthreads();
+        for (int idx = tid; idx < 4096; idx += 256) Bs[(idx >> 6) * LDA + (idx & 63)] = a.V[m * 4096 + idx];
+        if (tid < 64) {
+            lam[tid] = a.lam[m * 64 + tid];
+            ex[tid] = exp(lam[tid] * tau);
         }
-    __syncthreads();
-    for (int ii = 0; ii < 4; ++ii)
-        for (int q = 0; q < 4; ++q) As[(i0 + ii) * 64 + c0 + q] = t[ii][q];
-    for (int idx = tid; idx < 4096; idx += 256) Bs[idx] = a.U[m * 4096 + idx];
-    __syncthreads();
-    // At = U^T T1 : At[i][j] = sum_k U[k][i] T1[k][j]
-    for (int ii = 0; ii < 4; ++ii)
-        for (int q = 0; q < 4; ++q) t[ii][q] = 0.0;
-    for (int k = 0; k < 64; ++k)
-        for (int ii = 0; ii < 4; ++ii) {
-            const double u = Bs[k * 64 + i0 + ii];
-            for (int q = 0; q < 4; ++q) t[ii][q] = fma(u, As[k * 64 + c0 + q], t[ii][q]);
-        }
-    double acc[8];
-    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-    const double *M = a.M + (size_t)m * 6 * 4096;
-    for (int ii = 0; ii < 4; ++ii) {
-        const int i = i0 + ii;
-        for (int q = 0; q < 4; ++q) {
-            const int j = c0 + q;
-            const double at = t[ii][q];
+        __syncthreads();
+        gemm64<false, true, 4>(As, Bs, Ts, w, lr, lg);  // T1 = A V^T
+        __syncthreads();
+        for (int idx = tid; idx < 4096; idx += 256) Bs[(idx >> 6) * LDA + (idx & 63)] = a.U[m * 4096 + idx];
+        __syncthreads();
+        gemm64<true, false, 4>(Bs, Ts, At, w, lr, lg);  // A~ = U^T T1
+        __syncthreads();
+        double acc[8];
+        for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+        const double *M = a.M + (size_t)m * 6 * 4096;
+        for (int idx = tid; idx < 4096; idx += 256) {
+            const int i = idx >> 6, j = idx & 63;
+            const double at = At[i * LDA + j];
             if (i == j) {
                 acc[0] += ex[i] * at;
                 acc[1] += lam[i] * ex[i] * at;
@@ -819,26 +1332,24 @@ __global__ __launch_bounds__(256) void k_frechet(FrechetArgs a) {
             const double d = lam[i] - lam[j];
             const double E = (fabs(d) < 1e-9) ? tau * ex[i] : (ex[i] - ex[j]) / d;
             const double ea = E * at;
-            for (int phi = 0; phi < 6; ++phi) acc[2 + phi] += M[(size_t)phi * 4096 + i * 64 + j] * ea;
+#pragma unroll
+            for (int phi = 0; phi < 6; ++phi) acc[2 + phi] += M[(size_t)phi * 4096 + idx] * ea;
         }
-    }
-    // Deterministic block reduction.
-    const int lane = tid & 63, wv = tid >> 6;
-    for (int q = 0; q < 8; ++q) {
-        double v = acc[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-        if (lane == 0) red[q][wv] = v;
-    }
-    __syncthreads();
-    if (tid < 8) {
-        double v = ((red[tid][0] + red[tid][1]) + red[tid][2]) + red[tid][3];
-        if (tid >= 1) v *= wj;
-        a.out[((size_t)bi * a.max_comp + jj) * 8 + tid] = v;
+        double res[8];
+        for (int q = 0; q < 8; ++q) res[q] = block_sum<256>(acc[q], red, tid);
+        if (tid == 0) {
+            double *o = a.out + ((size_t)bi * a.max_comp + (J - j0)) * 8;
+            o[0] = res[0];
+            for (int q = 1; q < 8; ++q) o[q] = wj * res[q];
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_final: deterministic reductions into the result vector.
+// k_final: deterministic reductions into the result vector.  lnL: per-thread
+// compensated sums over contiguous pattern ranges, combined in thread order.
+// Gradient sums over branches: one wave per output, lanes striding branches,
+// fixed shuffle-tree combination.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void neum(double &s, double &c, double x) {
     const double t = s + x;
@@ -849,18 +1360,21 @@ __device__ __forceinline__ void neum(double &s, double &c, double x) {
     s = t;
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return __shfl(v, 0);
+}
+
 __global__ __launch_bounds__(256) void k_final(FinalArgs a) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     __shared__ double ps[256], pc[256];
-    __shared__ double sum_tau[64];
+    __shared__ double sum_tau[64], mubar[64];
     const int P = a.n_patterns;
     const int per = (P + 255) / 256;
     double s = 0.0, c = 0.0;
     for (int p = tid * per; p < (tid + 1) * per && p < P; ++p) {
-        const double w = a.weights[p];
-        const double ll = a.site_ll[p];
         if (a.want_grad && !(a.Lroot[p] > 0.0)) continue;
-        neum(s, c, w * ll);
+        neum(s, c, a.weights[p] * a.site_ll[p]);
     }
     ps[tid] = s;
     pc[tid] = c;
@@ -875,84 +1389,77 @@ __global__ __launch_bounds__(256) void k_final(FinalArgs a) {
     }
     if (!a.want_grad) return;
     const double *prm = a.params;
-    const int NR = a.n_real_nodes, NM = a.pl.n_mat, NCP = a.pl.n_comp;
+    const int NR = a.n_real_nodes, NM = a.pl.n_mat, NCP = a.pl.n_comp, MC = a.max_comp;
     double *gbl = a.res + 1, *gom = gbl + NR, *gw = gom + NM, *gth = gw + NCP;
-    // per-class conversion and sum_b tau_b dlnL/dtau_b
     if (tid < a.n_class) {
-        double st = 0.0;
-        for (int bi = 0; bi < a.n_branch; ++bi) {
-            if (a.blist[bi * BR_STRIDE + 3] != tid) continue;
-            const int node = a.blist[bi * BR_STRIDE];
-            double dt = 0.0;
-            for (int j = 0; j < a.max_comp; ++j) dt += a.fr[((size_t)bi * a.max_comp + j) * 8 + 1];
-            st += a.tau[node] * dt;
-        }
-        sum_tau[tid] = st;
+        double mb = 0.0;
+        for (int j = a.comp_ptr[tid]; j < a.comp_ptr[tid + 1]; ++j) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
+        mubar[tid] = mb < 1e-12 ? 1e-12 : mb;
     }
-    __syncthreads();
     for (int n = tid; n < NR; n += 256) gbl[n] = 0.0;
     __syncthreads();
+    for (int cl = wave; cl < a.n_class; cl += 4) {
+        double v = 0.0;
+        for (int bi = lane; bi < a.n_branch; bi += 64) {
+            if (a.blist[bi * BR_STRIDE + 3] != cl) continue;
+            double dt = 0.0;
+            for (int j = 0; j < MC; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+            v += a.tau[a.blist[bi * BR_STRIDE]] * dt;
+        }
+        v = wave_sum(v);
+        if (lane == 0) sum_tau[cl] = v;
+    }
     for (int bi = tid; bi < a.n_branch; bi += 256) {
         const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
-        double conv = 1.0;
-        if (a.convention == 1) {
-            double mb = 0.0;
-            for (int j = a.comp_ptr[cls]; j < a.comp_ptr[cls + 1]; ++j) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
-            if (mb < 1e-12) mb = 1e-12;
-            conv = 3.0 / mb;
-        }
+        const double conv = (a.convention == 1) ? 3.0 / mubar[cls] : 1.0;
         double dt = 0.0;
-        for (int j = 0; j < a.max_comp; ++j) dt += a.fr[((size_t)bi * a.max_comp + j) * 8 + 1];
+        for (int j = 0; j < MC; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
         if (node < NR) gbl[node] = conv * dt;
     }
-    // omega (phi 0) and theta (phi 1..5): one output per thread, branch order.
-    if (tid < NM + 5) {
-        const bool is_om = tid < NM;
-        const int phi = is_om ? 0 : (tid - NM + 1);
-        double g = 0.0;
-        for (int bi = 0; bi < a.n_branch; ++bi) {
-            const int cls = a.blist[bi * BR_STRIDE + 3];
-            for (int j = a.comp_ptr[cls]; j < a.comp_ptr[cls + 1]; ++j) {
-                if (is_om && a.comp_mat[j] != tid) continue;
-                g += a.fr[((size_t)bi * a.max_comp + (j - a.comp_ptr[cls])) * 8 + 2 + phi];
-            }
-        }
-        if (a.convention == 1) {
-            for (int cl = 0; cl < a.n_class; ++cl) {
-                double mb = 0.0;
-                for (int j = a.comp_ptr[cl]; j < a.comp_ptr[cl + 1]; ++j) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
-                if (mb < 1e-12) mb = 1e-12;
-                const double f = sum_tau[cl] / mb;
-                for (int j = a.comp_ptr[cl]; j < a.comp_ptr[cl + 1]; ++j) {
-                    const int mm = a.comp_mat[j];
-                    if (is_om && mm != tid) continue;
-                    g -= f * prm[a.pl.off_w + j] * a.dmu[mm * 6 + phi];
+    __syncthreads();
+    const int ntask = NM + 5 + NCP;
+    for (int t = wave; t < ntask; t += 4) {
+        double v = 0.0;
+        if (t < NM + 5) {
+            const bool is_om = t < NM;
+            const int phi = is_om ? 0 : (t - NM + 1);
+            for (int bi = lane; bi < a.n_branch; bi += 64) {
+                const int cls = a.blist[bi * BR_STRIDE + 3];
+                for (int j = a.comp_ptr[cls]; j < a.comp_ptr[cls + 1]; ++j) {
+                    if (is_om && a.comp_mat[j] != t) continue;
+                    v += a.fr[((size_t)bi * MC + (j - a.comp_ptr[cls])) * 8 + 2 + phi];
                 }
             }
+            v = wave_sum(v);
+            if (lane == 0) {
+                if (a.convention == 1) {
+                    for (int cl = 0; cl < a.n_class; ++cl) {
+                        const double f = sum_tau[cl] / mubar[cl];
+                        for (int j = a.comp_ptr[cl]; j < a.comp_ptr[cl + 1]; ++j) {
+                            const int mm = a.comp_mat[j];
+                            if (is_om && mm != t) continue;
+                            v -= f * prm[a.pl.off_w + j] * a.dmu[mm * 6 + phi];
+                        }
+                    }
+                }
+                if (is_om)
+                    gom[t] = v;
+                else
+                    gth[phi - 1] = v;
+            }
+        } else {
+            const int J = t - NM - 5;
+            int cls = 0;
+            while (cls + 1 < a.n_class && a.comp_ptr[cls + 1] <= J) ++cls;
+            const int jj = J - a.comp_ptr[cls];
+            for (int bi = lane; bi < a.n_branch; bi += 64)
+                if (a.blist[bi * BR_STRIDE + 3] == cls) v += a.fr[((size_t)bi * MC + jj) * 8 + 0];
+            v = wave_sum(v);
+            if (lane == 0) {
+                if (a.convention == 1) v -= sum_tau[cls] / mubar[cls] * a.mu[a.comp_mat[J]];
+                gw[J] = v;
+            }
         }
-        if (is_om)
-            gom[tid] = g;
-        else
-            gth[phi - 1] = g;
-    }
-    // mixture weights
-    if (tid >= 64 && tid - 64 < NCP) {
-        const int J = tid - 64;
-        int cls = 0;
-        while (cls + 1 <= a.n_class && a.comp_ptr[cls + 1] <= J) ++cls;
-        const int jj = J - a.comp_ptr[cls];
-        double g = 0.0;
-        for (int bi = 0; bi < a.n_branch; ++bi) {
-            if (a.blist[bi * BR_STRIDE + 3] != cls) continue;
-            g += a.fr[((size_t)bi * a.max_comp + jj) * 8 + 0];
-        }
-        if (a.convention == 1) {
-            double mb = 0.0;
-            for (int j = a.comp_ptr[cls]; j < a.comp_ptr[cls + 1]; ++j) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
-            if (mb < 1e-12) mb = 1e-12;
-            g -= sum_tau[cls] / mb * a.mu[a.comp_mat[J]];
-        }
-        gw[J] = g;
     }
 }
 
@@ -960,26 +1467,40 @@ __global__ __launch_bounds__(256) void k_final(FinalArgs a) {
 // Launchers
 // ---------------------------------------------------------------------------
 void launch_model(const ModelArgs &a, int n_mat, hipStream_t s) {
-    hipLaunchKernelGGL(k_model, dim3(n_mat), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_model, dim3(n_mat), dim3(MODEL_NT), 0, s, a);
 }
 
 template <typename R> void launch_pmat(const PmatArgs<R> &a, hipStream_t s) {
     hipLaunchKernelGGL(k_pmat<R>, dim3(a.n_branch), dim3(256), 0, s, a);
 }
 
+template <typename R, int T> void launch_inside_t(const PruneArgs<R> &a, hipStream_t s) {
+    if (a.tip_rows * T <= tip_lds_codes(T, sizeof(R)))
+        hipLaunchKernelGGL((k_inside<R, T, true>), dim3(a.n_tiles), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_inside<R, T, false>), dim3(a.n_tiles), dim3(256), 0, s, a);
+}
+
 template <typename R> void launch_inside(const PruneArgs<R> &a, int T, hipStream_t s) {
     switch (T) {
-        case 16: hipLaunchKernelGGL((k_inside<R, 16>), dim3(a.n_tiles), dim3(256), 0, s, a); break;
-        case 32: hipLaunchKernelGGL((k_inside<R, 32>), dim3(a.n_tiles), dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_inside<R, 64>), dim3(a.n_tiles), dim3(256), 0, s, a); break;
+        case 16: launch_inside_t<R, 16>(a, s); break;
+        case 32: launch_inside_t<R, 32>(a, s); break;
+        default: launch_inside_t<R, 64>(a, s); break;
     }
+}
+
+template <typename R, int T> void launch_outside_t(const PruneArgs<R> &a, hipStream_t s) {
+    if (a.tip_rows * T <= tip_lds_codes(T, sizeof(R)))
+        hipLaunchKernelGGL((k_outside<R, T, true>), dim3(a.n_tiles), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_outside<R, T, false>), dim3(a.n_tiles), dim3(256), 0, s, a);
 }
 
 template <typename R> void launch_outside(const PruneArgs<R> &a, int T, hipStream_t s) {
     switch (T) {
-        case 16: hipLaunchKernelGGL((k_outside<R, 16>), dim3(a.n_tiles), dim3(256), 0, s, a); break;
-        case 32: hipLaunchKernelGGL((k_outside<R, 32>), dim3(a.n_tiles), dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_outside<R, 64>), dim3(a.n_tiles), dim3(256), 0, s, a); break;
+        case 16: launch_outside_t<R, 16>(a, s); break;
+        case 32: launch_outside_t<R, 32>(a, s); break;
+        default: launch_outside_t<R, 64>(a, s); break;
     }
 }
 
@@ -1001,7 +1522,7 @@ void launch_proj(const ProjArgs &a, int n_mat, hipStream_t s) {
 }
 
 void launch_frechet(const FrechetArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_frechet, dim3(a.n_branch, a.max_comp), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_frechet, dim3(a.n_branch), dim3(256), 0, s, a);
 }
 
 void launch_final(const FinalArgs &a, hipStream_t s) { hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, a); }

@@ -25,7 +25,8 @@ HY_WANT_SITE_LL = 2
 HY_WANT_GRAD = 4
 
 EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
-           "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes")
+           "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
+           "hy_set_warm_start", "hy_last_eigen_status")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -64,6 +65,8 @@ def load_library(path: str = LIB_PATH):
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.hy_last_timings.argtypes = [C.c_void_p, _f64p]
     L.hy_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+    L.hy_set_warm_start.argtypes = [C.c_void_p, C.c_int]
+    L.hy_last_eigen_status.argtypes = [C.c_void_p, _i32p, C.c_int]
     _lib = L
     return L
 
@@ -180,6 +183,14 @@ class Engine:
         self._check(self.L.hy_loglik_from_P(self.h, precision, _p(P, _f64p), _p(pi, _f64p), C.byref(ll),
                                             _p(s, _f64p)))
         return (ll.value, s[:self.num_patterns]) if site else ll.value
+
+    def set_warm_start(self, enable):
+        self._check(self.L.hy_set_warm_start(self.h, 1 if enable else 0))
+
+    def eigen_status(self):
+        st = np.zeros(self.n_mat, dtype=np.int32)
+        self._check(self.L.hy_last_eigen_status(self.h, _p(st, _i32p), self.n_mat))
+        return st
 
     def last_timings(self):
         t = np.zeros(8)

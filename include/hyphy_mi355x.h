@@ -125,6 +125,18 @@ int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *
  * workgroup; 0 = automatic).  Applies from the next hy_eval. */
 int hy_set_tile(hy_engine h, int patterns_per_tile);
 
+/* Warm starts (default on): each hy_eval refines the previous call's
+ * eigenvectors of every rate matrix (Ogita-Aishima iteration, FP64) and only
+ * falls back to a cold Jacobi solve when that does not converge; results agree
+ * with cold solves to rounding.  Invalidated by hy_set_problem, hy_set_model
+ * and hy_set_mixture.  No counterpart in the reference, which re-solves every
+ * MG94Matrix::update (rate_matrix.hpp:236-248). */
+int hy_set_warm_start(hy_engine h, int enable);
+
+/* Eigensolver status of the last hy_eval per rate matrix: Jacobi sweeps for a
+ * cold solve, 100 + k after k refinement steps of a warm start. */
+int hy_last_eigen_status(hy_engine h, int *status, int n);
+
 /* Timing of the last hy_eval's device work, in milliseconds, per stage:
  * [0] spectra, [1] P(t), [2] inside, [3] outside, [4] adjoint accumulation,
  * [5] Frechet projection, [6] reductions, [7] total.  Only filled when the

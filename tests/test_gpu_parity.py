@@ -80,3 +80,33 @@ def test_tile_widths_agree(T):
     r = eng.eval(bl, [0.5], [1.0], sim["theta"], precision=FP64)
     assert rel(r["log_likelihood"], ll) < 1e-10
     assert grad_err(r["grad_bl"], g) < 1e-8
+
+
+def test_warm_start_matches_cold():
+    """Warm-started spectra (refined from the previous call) agree with cold
+    Jacobi solves; a large parameter jump falls back to the cold solve."""
+    omegas, weights = [0.2, 1.5], [0.7, 0.3]
+    tree, aln, sim = sim_problem(16, 400, 11, omegas=omegas, weights=weights)
+    N = len(tree.nodes)
+    engines = []
+    for warm in (True, False):
+        e = make_engine(tree, aln)
+        e.set_mixture(2, np.zeros(N, dtype=np.int32), [[0, 1]], convention=1)
+        e.set_warm_start(warm)
+        engines.append(e)
+    bl = branch_lengths(tree)
+    for step in range(5):
+        f = 1.0 + 1e-3 * step
+        om = [o * f for o in omegas] if step < 4 else [o * 3.0 for o in omegas]
+        th = np.asarray(sim["theta"]) * (2.0 - f)
+        rw = engines[0].eval(bl, om, weights, th, precision=FP64)
+        rc = engines[1].eval(bl, om, weights, th, precision=FP64)
+        st = engines[0].eigen_status()
+        print("step", step, "status", st, rw["log_likelihood"], rc["log_likelihood"])
+        if 0 < step < 4:
+            assert (st >= 100).all()
+        if step == 4:
+            assert (st < 100).all()
+        assert rel(rw["log_likelihood"], rc["log_likelihood"]) < 1e-12
+        for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
+            assert grad_err(rw[k], rc[k]) < 1e-9, k
