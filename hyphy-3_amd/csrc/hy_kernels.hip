@@ -1403,7 +1403,8 @@ __global__ __launch_bounds__(256) void k_final(FinalArgs a) {
         for (int bi = lane; bi < a.n_branch; bi += 64) {
             if (a.blist[bi * BR_STRIDE + 3] != cl) continue;
             double dt = 0.0;
-            for (int j = 0; j < MC; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+            const int nj = a.comp_ptr[cl + 1] - a.comp_ptr[cl];  // components of this branch's class
+            for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
             v += a.tau[a.blist[bi * BR_STRIDE]] * dt;
         }
         v = wave_sum(v);
@@ -1413,7 +1414,8 @@ __global__ __launch_bounds__(256) void k_final(FinalArgs a) {
         const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
         const double conv = (a.convention == 1) ? 3.0 / mubar[cls] : 1.0;
         double dt = 0.0;
-        for (int j = 0; j < MC; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+        const int nj = a.comp_ptr[cls + 1] - a.comp_ptr[cls];
+        for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
         if (node < NR) gbl[node] = conv * dt;
     }
     __syncthreads();
