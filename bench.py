@@ -54,7 +54,9 @@ def build_problem(cfg):
 
 
 def stage_work(arrays, K, b):
-    """Algorithmic flops and HBM bytes per launch of each pruning kernel."""
+    """Algorithmic flops and HBM bytes per launch of each pruning kernel
+    (b = bytes per partial entry; 64 padded states per pattern; tip edges are
+    gathers, 0 flop; flops count the 61 real states)."""
     parent = arrays["parent"]
     cp = arrays["child_ptr"]
     n_nodes = len(parent)
@@ -65,12 +67,21 @@ def stage_work(arrays, K, b):
     e_int = n_int - 1  # internal nodes other than the root
     B = n_nodes - 1
     P = int(len(arrays["weights"]))
+    S = S_ALG
     blk = b * 64 * P  # one node's partials (64 padded states x P patterns)
+    tips = 2 * (n_tax + 1) * P  # uint16 tip codes
     return {
-        "inside": dict(flops=2.0 * S_ALG ** 2 * P * e_int, bytes=blk * n_int + blk * e_int + 2 * n_tax * P + 8 * n_int * P),
-        "outside": dict(flops=2.0 * S_ALG ** 2 * P * (e_int + e_int), bytes=blk * e_int + blk * e_int + blk * B + 8 * B * P),
-        "adjoint": dict(flops=2.0 * S_ALG ** 2 * P * e_int + 2.0 * S_ALG * P * n_tax, bytes=blk * B + blk * e_int + 8 * B * P),
-        "pmat": dict(flops=2.0 * S_ALG ** 3 * B * K, bytes=3 * b * 4096 * B),
+        # writes D and E of every internal node, reads the tip codes
+        "inside": dict(flops=2.0 * S * S * P * e_int, bytes=blk * n_int + 4 * n_int * P + tips),
+        # reads D and E of internal children, writes V and G of every branch;
+        # X_c = P_c D_c is recomputed (2 S^2 P per internal edge, counted) instead
+        # of being stored by the inside pass
+        "outside": dict(flops=4.0 * S * S * P * e_int, bytes=blk * e_int + blk * B + 4 * (e_int + B) * P + tips),
+        # A_b = sum_p c_p V_b D_b^T: reads V of every branch and D of internal
+        # children, exponents and root likelihoods; writes 64x64 FP64 per branch
+        "adjoint": dict(flops=2.0 * S * S * P * e_int + 2.0 * S * P * n_tax,
+                        bytes=blk * B + blk * e_int + 4 * (B + e_int + 1) * P + 8 * P + tips + 8 * 4096 * B),
+        "pmat": dict(flops=2.0 * S ** 3 * B * K, bytes=3 * b * 4096 * B),
         "_meta": dict(P=P, n_int=n_int, e_int=e_int, B=B, n_tax=n_tax, root=root),
     }
 
@@ -135,7 +146,9 @@ def cpu_baseline(cfg, sim, seconds):
     N = len(tree.nodes)
     mix = ho.Mixture(cfg["omegas"], np.zeros(N, dtype=np.int32), [[(k, w) for k, w in enumerate(cfg["weights"])]], 0)
     bl = np.array([0.0 if n.id == tree.root else n.bl * sim["conv"] for n in tree.nodes])
-    threads = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     # BUSTED tile rule (busted.hpp:385-401): clamp(ceil(P/threads), 16, 64)
     P = prob.num_patterns
     tile = 64 if threads <= 1 else int(min(64, max(16, -(-P // threads))))
@@ -164,28 +177,31 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of known size (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build measures one MI355X; multi-GPU runs are not part of it "
                                    "(the path would shard patterns across GPUs, see DESIGN.md)"}))
         sys.exit(2)
-    os.environ["HY_PROFILE"] = "1"
     cfg = CONFIGS[args.config]
     K = len(cfg["omegas"])
+    if args.copy_calibration:
+        copy_calibration()
+        return
     sim, aln, tree, arrays, tau = build_problem(cfg)
     print(f"[bench] {args.config}: taxa={cfg['taxa']} codons={cfg['codons']} patterns={len(arrays['weights'])} "
           f"nodes={len(arrays['parent'])} K={K}", file=sys.stderr, flush=True)
-    eng = make_engine(arrays, K, args.tile)
     eval_args = (tau, cfg["omegas"], cfg["weights"], sim["theta"])
-    eng.set_warm_start(False)
-    ms32c, _, st32c = run_steps(eng, eval_args, 1, max(3, args.steps // 4), 1, True)
-    eng.set_warm_start(True)
-    ms32, out32, st32 = run_steps(eng, eval_args, 1, args.steps, args.warmup, True)
+    # timed engine: no per-stage events
+    os.environ.pop("HY_PROFILE", None)
+    eng = make_engine(arrays, K, args.tile)
+    ms32, out32, _ = run_steps(eng, eval_args, 1, args.steps, args.warmup, False)
     print(f"[bench] fp32 {ms32 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
     res = {}
     if not args.no_fp64:
-        ms64, out64, st64 = run_steps(eng, eval_args, 0, args.steps, args.warmup, True)
+        ms64, out64, _ = run_steps(eng, eval_args, 0, args.steps, args.warmup, False)
         print(f"[bench] fp64 {ms64 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
         g32 = np.concatenate([out32["grad_bl"], out32["grad_omega"], out32["grad_w"], out32["grad_theta"]])
         g64 = np.concatenate([out64["grad_bl"], out64["grad_omega"], out64["grad_w"], out64["grad_theta"]])
@@ -196,25 +212,33 @@ def main():
             "abs_err_logl_vs_fp64": abs(out32["log_likelihood"] - out64["log_likelihood"]),
             "rel_err_grad_vs_fp64_max_component": float(np.max(np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-2 * scale))),
             "rel_err_grad_vs_fp64_norm": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
-            "stage_ms_fp64": {s: float(st64[i]) for i, s in enumerate(STAGES)},
             "logL": out64["log_likelihood"],
         })
-    work = stage_work(arrays, K, 4)
+    eng.close()
+    # profiled engine: HIP events around every stage on the engine's stream
+    os.environ["HY_PROFILE"] = "1"
+    peng = make_engine(arrays, K, args.tile)
+    _, _, st32 = run_steps(peng, eval_args, 1, args.steps, args.warmup, True)
     stage_ms = {s: float(st32[i]) for i, s in enumerate(STAGES)}
+    if not args.no_fp64:
+        _, _, st64 = run_steps(peng, eval_args, 0, max(3, args.steps // 2), 1, True)
+        res["stage_ms_fp64"] = {s: float(st64[i]) for i, s in enumerate(STAGES)}
+    peng.set_warm_start(False)
+    ms32c, _, st32c = run_steps(peng, eval_args, 1, max(3, args.steps // 4), 1, True)
+    peng.close()
+    work = stage_work(arrays, K, 4)
+    tile = eng_tile(args.tile, len(arrays["weights"]))
     cand = {s: stage_ms[s] for s in ("inside", "outside", "adjoint")}
     dom = max(cand, key=cand.get)
-    t_dom = stage_ms[dom] * 1e-3
-    fl, by = work[dom]["flops"], work[dom]["bytes"]
-    if fl / by > PEAK_FP32_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9):
-        roof = {"bound": "mfma", "achieved": fl / t_dom / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
-    else:
-        roof = {"bound": "hbm", "achieved": by / t_dom / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["kernel"] = {"inside": "k_inside<float>", "outside": "k_outside<float>", "adjoint": "k_adjoint<float>"}[dom]
-    roof["algorithmic_flops"] = fl
-    roof["algorithmic_bytes"] = by
-    roof["kernel_ms"] = stage_ms[dom]
+    roof = roofline(dom, work[dom], stage_ms[dom] * 1e-3)
+    roof["traffic"] = committed_traffic(KERNEL_NAMES[dom], args.config, tile)
+    meta = work["_meta"]
+    # whole-evaluation bound (algorithmic work of one logL + full gradient, FP32)
+    S = S_ALG
+    f_alg = 6.0 * S * S * meta["P"] * meta["e_int"] + 8.0 * S ** 3 * meta["B"] * K
+    b_alg = 4 * 64 * meta["P"] * 4 * meta["n_int"] + meta["n_tax"] * meta["P"] + 8 * meta["P"] * meta["n_int"] + \
+        2 * 4 * 4096 * meta["B"] * K
+    t_bound = max(f_alg / (PEAK_FP32_TFLOPS * 1e12), b_alg / (PEAK_HBM_GBS * 1e9))
     line = {
         "metric": METRIC, "value": 1.0 / ms32, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms32 * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -222,11 +246,16 @@ def main():
         "config": {"workload": f"{args.config}: {cfg['taxa']} taxa x {cfg['codons']} codons x {K} omega categories, "
                                "logL + full gradient (branches, omegas, weights, GTR)",
                    "taxa": cfg["taxa"], "codons": cfg["codons"], "patterns": int(len(arrays["weights"])),
-                   "categories": K, "seed": cfg["seed"], "tile": eng_tile(args.tile, len(arrays["weights"]))},
-        "roofline": roof, "stage_ms_fp32": stage_ms,
-        "ms_per_step_cold_spectra": ms32c * 1e3, "stage_ms_fp32_cold_spectra": {s: float(st32c[i]) for i, s in enumerate(STAGES)},
+                   "categories": K, "seed": cfg["seed"], "tile": tile},
+        "roofline": roof,
+        "eval_roofline": {"flops": f_alg, "bytes": b_alg, "t_bound_ms": t_bound * 1e3,
+                          "frac": t_bound / ms32, "note": "max(F/157.3 TFLOP/s, B/8 TB/s) over the measured eval"},
+        "stage_ms_fp32": stage_ms,
+        "ms_per_step_cold_spectra": ms32c * 1e3,
+        "stage_ms_fp32_cold_spectra": {s: float(st32c[i]) for i, s in enumerate(STAGES)},
         "steps_note": "consecutive steps evaluate parameters perturbed by 1e-3 relative (optimizer iterates); spectra are "
-                      "warm-started from the previous step (cold-start figures reported beside)",
+                      "warm-started from the previous step (cold-start figures reported beside); stage times come from "
+                      "a second engine with HIP events around each stage",
     }
     line.update(res)
     if not args.no_cpu:
@@ -236,6 +265,50 @@ def main():
         except Exception as ex:  # the oracle library may not be built on a bare box
             line["cpu_baseline"] = {"value": None, "error": str(ex)}
     print(json.dumps(line), flush=True)
+
+
+KERNEL_NAMES = {"inside": "k_inside", "outside": "k_outside", "adjoint": "k_adjoint"}
+
+
+def roofline(stage, w, t):
+    """Roofline entry of one kernel from its algorithmic flops and HBM bytes per
+    launch and its measured average launch time t (s); the binding resource is
+    the one with the larger ideal time."""
+    fl, by = w["flops"], w["bytes"]
+    if fl / (PEAK_FP32_TFLOPS * 1e12) >= by / (PEAK_HBM_GBS * 1e9):
+        r = {"bound": "mfma", "achieved": fl / t / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        r = {"bound": "hbm", "achieved": by / t / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    r["frac"] = r["achieved"] / r["peak"]
+    r.update({"kernel": KERNEL_NAMES[stage] + "<float>", "algorithmic_flops": fl, "algorithmic_bytes": by,
+              "kernel_ms": t * 1e3})
+    return r
+
+
+def committed_traffic(kernel, config, tile):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), read from
+    the summary committed under profiles/; None when no measurement matches."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    e = data.get(f"{kernel}/{config}/T{tile}")
+    return None if e is None else e.get("bytes_per_launch")
+
+
+def copy_calibration():
+    """A plain device copy of known size (256 MiB read + 256 MiB written), run
+    under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale the counters."""
+    import torch
+    n = 64 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
 
 
 def eng_tile(req, P):
