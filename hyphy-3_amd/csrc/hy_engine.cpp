@@ -52,6 +52,7 @@ struct hy_engine_s {
     std::vector<std::vector<int32_t>> ext_children;
     std::vector<uint8_t> ext_ident;
     int ext_root = -1, sched_key = 0;
+    const void *ident_tiles[2] = {nullptr, nullptr};  // PA buffer whose identity tiles are current
     DevBuf d_osched, d_Uspill[2];
     int n_uspill = 0;
     std::vector<int32_t> real_parent;
@@ -170,6 +171,33 @@ void build_layout(hy_engine h) {
 
 void build_schedules(hy_engine h, int T, int esz);
 
+// Identity P of the artificial nodes in the Prow, PA/PTA ([w][l][kk]) and
+// Rsum layouts of the working type (esz = 4 or 8 bytes).
+int upload_identity_tiles(hy_engine h, int ri, int esz) {
+    std::vector<double> prow(4096, 0.0), pa(4096, 0.0), rs(64, 0.0);
+    for (int i = 0; i < 64; ++i) prow[i * 64 + i] = 1.0;
+    for (int idx = 0; idx < 4096; ++idx) {
+        const int w = idx >> 10, l = (idx >> 4) & 63, kk = idx & 15;
+        pa[idx] = (16 * w + (l & 15)) == (4 * kk + (l >> 4)) ? 1.0 : 0.0;
+    }
+    for (int i = 0; i < h->S; ++i) rs[i] = 1.0;
+    std::vector<float> prow32(prow.begin(), prow.end()), pa32(pa.begin(), pa.end()), rs32(rs.begin(), rs.end());
+    const void *srcs[4] = {esz == 8 ? (const void *)prow.data() : prow32.data(),
+                           esz == 8 ? (const void *)pa.data() : pa32.data(),
+                           esz == 8 ? (const void *)pa.data() : pa32.data(),
+                           esz == 8 ? (const void *)rs.data() : rs32.data()};
+    DevBuf *dst[4] = {&h->d_Prow[ri], &h->d_PA[ri], &h->d_PTA[ri], &h->d_Rsum[ri]};
+    const size_t per[4] = {4096, 4096, 4096, 64};
+    for (int n = 0; n < h->n_ext; ++n) {
+        if (!h->ext_ident[n]) continue;
+        for (int k = 0; k < 4; ++k)
+            HIP_TRY(h, hipMemcpyAsync((char *)dst[k]->p + (size_t)n * per[k] * esz, srcs[k], per[k] * esz,
+                                      hipMemcpyHostToDevice, h->stream));
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return HY_OK;
+}
+
 // (Re)allocate everything that depends on the tile width and precision.
 int prepare_buffers(hy_engine h, int prec) {
     const int T = choose_tile(h->P, h->T_req);
@@ -212,6 +240,12 @@ int prepare_buffers(hy_engine h, int prec) {
     TRY(ensure(h, h->d_PA[ri], ne * 4096 * rs));
     TRY(ensure(h, h->d_PTA[ri], ne * 4096 * rs));
     TRY(ensure(h, h->d_Rsum[ri], ne * 64 * rs));
+    if (h->ident_tiles[ri] != h->d_PA[ri].p) {
+        // Edges added to resolve polytomies (identity transition): exact
+        // identity tiles in every P layout; k_pmat never overwrites them.
+        TRY(upload_identity_tiles(h, ri, (int)rs));
+        h->ident_tiles[ri] = h->d_PA[ri].p;
+    }
     // adjoint chunking: aim for >= 1024 workgroups (4 per CU), each covering a
     // contiguous run of tiles
     int nb = std::max(1, h->n_branch);
@@ -260,7 +294,7 @@ void build_schedules(hy_engine h, int T, int esz) {
             const int row = (h->leaf_taxon_ext[c] >= 0) ? h->leaf_taxon_ext[c] : h->num_taxa;
             return (KIND_TIP << 28) | row;
         }
-        return ((h->ext_ident[c] ? KIND_IDENT : KIND_INT) << 28) | h->int_slot[c];
+        return (KIND_INT << 28) | h->int_slot[c];  // identity edges carry an exact identity P
     };
     // ---- inside: free-list LDS slots, spill to HBM when none is free ----
     std::vector<int32_t> where(n_ext, -1);  // LDS slot of a computed internal node (-1: HBM)
@@ -730,7 +764,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
             const int row = (c < num_nodes && leaf_taxon[c] >= 0) ? leaf_taxon[c] : num_taxa;
             return (KIND_TIP << 28) | row;
         }
-        return ((ident[c] ? KIND_IDENT : KIND_INT) << 28) | int_slot[c];
+        return (KIND_INT << 28) | int_slot[c];
     };
     // internal-node counts of subtrees (visit larger subtrees first)
     std::vector<int> sub(n_ext, 0);
@@ -824,6 +858,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     for (int n = 0; n < num_nodes; ++n) h->leaf_taxon_ext[n] = leaf_taxon[n];
     h->sched.clear();
     h->sched_key = 0;
+    h->ident_tiles[0] = h->ident_tiles[1] = nullptr;
     h->blist = std::move(blist);
     h->real_parent.assign(parent, parent + num_nodes);
     h->codes = std::move(codes);

@@ -7,9 +7,10 @@
 namespace hy {
 
 // Child / branch reference encoding: (kind << 28) | index.
+// Artificial nodes that resolve polytomies are internal nodes whose edge
+// matrix is an exact identity (uploaded once per problem).
 constexpr int KIND_INT = 0;    // internal node, index = D slot, edge matrix P(node)
 constexpr int KIND_TIP = 1;    // leaf, index = tip row, edge matrix P(node)
-constexpr int KIND_IDENT = 2;  // internal node joined by an identity edge (resolved polytomy)
 constexpr int REF_MASK = 0x0FFFFFFF;
 constexpr uint16_t TIP_ONES = 0xFFFFu;  // all-ones partial (gap, unmatched leaf, padding)
 constexpr uint16_t TIP_AMB = 0x8000u;   // | index into the ambiguity-mask table

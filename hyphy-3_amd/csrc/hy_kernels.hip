@@ -621,6 +621,16 @@ __device__ __forceinline__ void mfma_tile(const R *av, const R *Bs, typename MF<
         for (int nb = 0; nb < NB; ++nb) X[nb] = MF<R>::mma(av[kk], Bs[(4 * kk + lg) * T + 16 * nb + lr], X[nb]);
 }
 
+// Makes the MFMA results in x available before the next instruction, so
+// that following loads may reuse the registers those MFMAs read.
+template <typename R, int T>
+__device__ __forceinline__ void consume(const typename MF<R>::acc *x) {
+#pragma unroll
+    for (int nb = 0; nb < T / 16; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) asm volatile("" ::"v"(x[nb][r]));
+}
+
 // X = P * B with the lane's B operands already in registers (bv[kk * NB + nb]).
 template <typename R, int T>
 __device__ __forceinline__ void mfma_regs(const R *av, const R *bv, typename MF<R>::acc *X) {
@@ -712,6 +722,51 @@ __device__ __forceinline__ void load_c(typename MF<R>::acc *x, const R *src, int
         for (int r = 0; r < 4; ++r) x[nb][r] = src[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr];
 }
 
+// Inside partials in HBM use a "B-operand-major" tile layout: element
+// (state s, pattern t) of a 64 x T tile sits at
+//   ((t / 16 * 4 + s % 4) * 16 + t % 16) * 16 + s / 4,
+// so the 16 MFMA B operands of lane (lg, lr) for pattern block nb
+// (states 4kk + lg, kk = 0..15) are one contiguous run.
+__device__ __forceinline__ int bmaj(int s, int t) { return (((t >> 4) * 4 + (s & 3)) * 16 + (t & 15)) * 16 + (s >> 2); }
+
+// B operands of lane (lg, lr) from a B-major tile: bv[kk * NB + nb].
+template <typename R, int T>
+__device__ __forceinline__ void fetch_b(R *bv, const R *__restrict__ tile, int lr, int lg) {
+    constexpr int NB = T / 16;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        const R *p = tile + ((nb * 4 + lg) * 16 + lr) * 16;
+        if constexpr (sizeof(R) == 4) {
+            const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float4 v = q[i];
+                bv[(4 * i) * NB + nb] = v.x;
+                bv[(4 * i + 1) * NB + nb] = v.y;
+                bv[(4 * i + 2) * NB + nb] = v.z;
+                bv[(4 * i + 3) * NB + nb] = v.w;
+            }
+        } else {
+            const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const double2 v = q[i];
+                bv[(2 * i) * NB + nb] = v.x;
+                bv[(2 * i + 1) * NB + nb] = v.y;
+            }
+        }
+    }
+}
+
+// C-layout values (wave w's rows) into a B-major tile.
+template <typename R, int T>
+__device__ __forceinline__ void store_bmaj(R *tile, const typename MF<R>::acc *x, int w, int lr, int lg) {
+#pragma unroll
+    for (int nb = 0; nb < T / 16; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tile[bmaj(16 * w + MF<R>::crow(lg, r), 16 * nb + lr)] = x[nb][r];
+}
+
 template <int T, bool TL>
 __device__ __forceinline__ void stage_tip_codes(uint16_t *tcodes, const uint16_t *tips, int rows, int P_pad, int p0,
                                                 int tid) {
@@ -786,19 +841,13 @@ __global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_in
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) X[nb] = xt[c][nb];
                 } else if (src >= 0) {  // partials in an LDS slot
-                    const R *Bs = Ds + src * 64 * T;
-                    if (kind == KIND_IDENT)
-                        load_c<R, T>(X, Bs, w, lr, lg);
-                    else
-                        mfma_tile<R, T>(pa[c], Bs, X, lr, lg);
+                    mfma_tile<R, T>(pa[c], Ds + src * 64 * T, X, lr, lg);
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) Ec[nb] += Es[src * T + 16 * nb + lr];
                 } else {  // partials only in HBM
-                    const R *Bg = a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T);
-                    if (kind == KIND_IDENT)
-                        load_c<R, T>(X, Bg, w, lr, lg);
-                    else
-                        mfma_tile<R, T>(pa[c], Bg, X, lr, lg);
+                    R bv[16 * NB];
+                    fetch_b<R, T>(bv, a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
+                    mfma_regs<R, T>(pa[c], bv, X);
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) Ec[nb] += a.E[(size_t)idx * a.P_pad + p0 + 16 * nb + lr];
                 }
@@ -818,7 +867,7 @@ __global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_in
         cb ^= 1;
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) Ec[nb] += ex[nb];
-        store_c<R, T>(a.D + ((size_t)gidx * a.n_tiles + tile) * (64 * T), Dc, w, lr, lg);
+        store_bmaj<R, T>(a.D + ((size_t)gidx * a.n_tiles + tile) * (64 * T), Dc, w, lr, lg);
         if (out >= 0) store_c<R, T>(Ds + out * 64 * T, Dc, w, lr, lg);
         if (w == 0 && lg == 0) {
 #pragma unroll
@@ -920,23 +969,8 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
             gather_tip<R, T, TL>(xt[c], ab, c, c < nc && kind == KIND_TIP, a, tcodes, pn, idx, p0, w, lr, lg);
             if (c < nc) {
                 if (kind != KIND_TIP) {
-                    if (kind == KIND_INT) fetch_edge<R>(pa[c], a.PA, pn, w, l);
-                    const R *src = a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T);
-                    if constexpr (DREG) {
-                        if (kind == KIND_IDENT) {
-#pragma unroll
-                            for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-                                for (int r = 0; r < 4; ++r)
-                                    dv[c][nb * 4 + r] = src[(16 * w + MF<R>::crow(lg, r)) * T + 16 * nb + lr];
-                        } else {
-#pragma unroll
-                            for (int kk = 0; kk < 16; ++kk)
-#pragma unroll
-                                for (int nb = 0; nb < NB; ++nb)
-                                    dv[c][kk * NB + nb] = src[(4 * kk + lg) * T + 16 * nb + lr];
-                        }
-                    }
+                    fetch_edge<R>(pa[c], a.PA, pn, w, l);
+                    if constexpr (DREG) fetch_b<R, T>(dv[c], a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) de[c][nb] = a.E[(size_t)idx * a.P_pad + p0 + 16 * nb + lr];
                 }
@@ -959,15 +993,8 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
         const int nc = ec.v[2], usrc = ec.v[5];
         bool any_int = false;
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c) {
-            if (c < nc) {
-                const int pn = ec.v[6 + 4 * c], kind = ec.v[7 + 4 * c] >> 28;
-                if (kind == KIND_INT) {
-                    any_int = true;
-                    fetch_edge<R>(pt[c], a.PTA, pn, w, l);
-                }
-            }
-        }
+        for (int c = 0; c < MAXC; ++c)
+            if (c < nc && (ec.v[7 + 4 * c] >> 28) == KIND_INT) any_int = true;
         acc_t U[NB];
         int H[NB];
         if (usrc == U_ROOT) {
@@ -1009,28 +1036,18 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
                     }
                 } else {
                     if constexpr (DREG) {
-                        if (kind == KIND_IDENT) {
-#pragma unroll
-                            for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-                                for (int r = 0; r < 4; ++r) X[c][nb][r] = dv[c][nb * 4 + r];
-                        } else {
-                            mfma_regs<R, T>(pa[c], dv[c], X[c]);
-                        }
+                        mfma_regs<R, T>(pa[c], dv[c], X[c]);
                     } else {
-                        const R *src = a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T);
-                        if (kind == KIND_IDENT)
-                            load_c<R, T>(X[c], src, w, lr, lg);
-                        else
-                            mfma_tile<R, T>(pa[c], src, X[c], lr, lg);
+                        R bv[16 * NB];
+                        fetch_b<R, T>(bv, a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
+                        mfma_regs<R, T>(pa[c], bv, X[c]);
+                        consume<R, T>(X[c]);  // the next child's loads may reuse bv's registers
                     }
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) Ex[c][nb] = de[c][nb];
                 }
             }
         }
-        // operands of the next node (the registers above are free now)
-        if (k + 1 < n) operands(en);
         acc_t V[MAXC][NB];
         int G[MAXC][NB];
         R *cm = cmax[cb];
@@ -1066,6 +1083,14 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
                 }
             }
         }
+        // Loads into registers that earlier MFMAs read as operands are issued
+        // only after those MFMAs' results have been consumed (X by the
+        // products above, the previous U_c by U): transposed edge tiles of this
+        // node and all operands of the next one.
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+            if (c < nc && (ec.v[7 + 4 * c] >> 28) == KIND_INT) fetch_edge<R>(pt[c], a.PTA, ec.v[6 + 4 * c], w, l);
+        if (k + 1 < n) operands(en);
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) {
@@ -1101,12 +1126,7 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
                 const int kind = ref >> 28, idx = ref & REF_MASK;
                 if (kind != KIND_TIP) {
                     acc_t Uc[NB];
-                    if (kind == KIND_IDENT) {
-#pragma unroll
-                        for (int nb = 0; nb < NB; ++nb) Uc[nb] = V[c][nb];
-                    } else {
-                        mfma_tile<R, T>(pt[c], Xc + c * 64 * T, Uc, lr, lg);
-                    }
+                    mfma_tile<R, T>(pt[c], Xc + c * 64 * T, Uc, lr, lg);
                     if (dst == U_NEXT) {
 #pragma unroll
                         for (int nb = 0; nb < NB; ++nb) {
@@ -1194,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> a) {
                     loc[cb] = MF<R>::mma(av, tip_onehot<R>(code, 16 * cb + lr, a.amb, a.S), loc[cb]);
             } else {
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) loc[cb] = MF<R>::mma(av, Ds[(16 * cb + lr) * T + pl], loc[cb]);
+                for (int cb = 0; cb < 4; ++cb) loc[cb] = MF<R>::mma(av, Ds[bmaj(16 * cb + lr, pl)], loc[cb]);
             }
         }
 #pragma unroll
