@@ -78,6 +78,8 @@ struct hy_engine_s {
     bool warm_enable = true, warm_valid = false;
     DevBuf d_Prow[2], d_Rsum[2], d_PA[2], d_PTA[2];
     DevBuf d_D[2], d_Vp[2], d_E, d_G, d_Lroot, d_Eroot, d_site, d_Apart, d_fr, d_res;
+    DevBuf d_Asum, d_W, d_wl, d_proj;
+    std::vector<int32_t> wl;  // uploaded copy of the per-matrix W slot lists
     int n_chunks = 1, tiles_per_chunk = 1;
 
     size_t device_bytes = 0;
@@ -254,6 +256,26 @@ int prepare_buffers(hy_engine h, int prec) {
     h->n_chunks = (h->n_tiles + h->tiles_per_chunk - 1) / h->tiles_per_chunk;
     TRY(ensure(h, h->d_Apart, (size_t)nb * h->n_chunks * 4096 * 8));
     TRY(ensure(h, h->d_fr, (size_t)nb * h->max_comp * 8 * 8));
+    TRY(ensure(h, h->d_Asum, (size_t)nb * 4096 * 8));
+    TRY(ensure(h, h->d_W, (size_t)nb * h->max_comp * 4096 * 8));
+    TRY(ensure(h, h->d_proj, (size_t)h->n_mat * 6 * HPROJ_CHUNKS * 8));
+    {
+        // per matrix: the W slots (branch * max_comp + component index) whose
+        // component uses it, branch-major (fixed summation order)
+        std::vector<std::vector<int32_t>> per(h->n_mat);
+        for (int bi = 0; bi < h->n_branch; ++bi) {
+            const int cls = h->blist[bi * BR_STRIDE + 3];
+            for (int j = h->comp_ptr[cls]; j < h->comp_ptr[cls + 1]; ++j)
+                per[h->comp_mat[j]].push_back(bi * h->max_comp + (j - h->comp_ptr[cls]));
+        }
+        std::vector<int32_t> wl(h->n_mat + 1, 0);
+        for (int m = 0; m < h->n_mat; ++m) wl[m + 1] = wl[m] + (int32_t)per[m].size();
+        for (int m = 0; m < h->n_mat; ++m) wl.insert(wl.end(), per[m].begin(), per[m].end());
+        if (wl != h->wl || !h->d_wl.p) {
+            TRY(upload(h, h->d_wl, wl.data(), wl.size() * 4));
+            h->wl = wl;
+        }
+    }
     const int nm = h->n_mat;
     TRY(ensure(h, h->d_lam, (size_t)nm * 64 * 8));
     TRY(ensure(h, h->d_U, (size_t)nm * 4096 * 8));
@@ -528,6 +550,12 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         fa.M = (const double *)h->d_M.p;
         fa.tau = (const double *)h->d_tau.p;
         fa.out = (double *)h->d_fr.p;
+        fa.n_mat = h->n_mat;
+        fa.Asum = (double *)h->d_Asum.p;
+        fa.W = (double *)h->d_W.p;
+        fa.wl_ptr = (const int32_t *)h->d_wl.p;
+        fa.wl_idx = (const int32_t *)h->d_wl.p + (h->n_mat + 1);
+        fa.proj = (double *)h->d_proj.p;
         if (h->n_branch > 0) launch_frechet(fa, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[6], s));
     } else if (prof) {
@@ -553,6 +581,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     fn.tau = (const double *)h->d_tau.p;
     fn.mu = (const double *)h->d_mu.p;
     fn.dmu = (const double *)h->d_dmu.p;
+    fn.proj = (const double *)h->d_proj.p;
     fn.res = (double *)h->d_res.p;
     launch_final(fn, s);
     HIP_TRY(h, hipGetLastError());
@@ -602,7 +631,7 @@ void hy_destroy(hy_engine h) {
                       &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
                       &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
-                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_res};
+                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_res, &h->d_Asum, &h->d_W, &h->d_wl, &h->d_proj};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);

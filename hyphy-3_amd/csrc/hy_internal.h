@@ -138,14 +138,20 @@ struct ProjArgs {
     double *dmu;  // [n_mat][6]
 };
 
+constexpr int HPROJ_CHUNKS = 16;  // 4096 entries / 256 threads
+
 struct FrechetArgs {
     const int32_t *blist;
-    int n_branch, n_chunks, max_comp;
+    int n_branch, n_chunks, max_comp, n_mat;
     const int32_t *comp_ptr, *comp_mat;
     const double *params;
     ParamLayout pl;
     const double *Apart, *lam, *U, *V, *M, *tau;
-    double *out;  // [n_branch][max_comp][8]
+    double *Asum;                    // [n_branch][4096] adjoint summed over pattern chunks
+    double *W;                       // [n_branch][max_comp][4096] w_j E o A~
+    const int32_t *wl_ptr, *wl_idx;  // per matrix: W slots (branch * max_comp + component) using it
+    double *out;                     // [n_branch][max_comp][8]: [0] <A, P_m>, [1] w_j d lnL / d tau share
+    double *proj;                    // [n_mat][6][HPROJ_CHUNKS] partial <M_phi, H_m>
 };
 
 struct FinalArgs {
@@ -158,7 +164,7 @@ struct FinalArgs {
     int convention;
     const double *params;
     ParamLayout pl;
-    const double *fr, *tau, *mu, *dmu;
+    const double *fr, *tau, *mu, *dmu, *proj;
     double *res;  // [0] lnL, [1..n_real] grad_bl, then omega[n_mat], w[n_comp], theta[5]
 };
 
@@ -170,7 +176,7 @@ template <typename R> void launch_inside(const PruneArgs<R> &a, int T, hipStream
 template <typename R> void launch_outside(const PruneArgs<R> &a, int T, hipStream_t s);
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s);
 void launch_proj(const ProjArgs &a, int n_mat, hipStream_t s);
-void launch_frechet(const FrechetArgs &a, hipStream_t s);
+void launch_frechet(const FrechetArgs &a, hipStream_t s);  // chunk sums, spectral transforms, projections
 void launch_final(const FinalArgs &a, hipStream_t s);
 
 }  // namespace hy

@@ -114,6 +114,31 @@ __device__ __forceinline__ void gemm64(const double *X, const double *Y, double 
         for (int r = 0; r < 4; ++r) C[(16 * rb + MF<double>::crow(lg, r)) * LDA + 16 * (cb0 + q * CS) + lr] = acc[q][r];
 }
 
+// gemm64 in two halves: products into registers, then the store, so the
+// result may overwrite an operand once every wave has finished reading.
+template <bool TX, bool TY>
+__device__ __forceinline__ void gemm64_acc(const double *X, const double *Y, f64x4 *acc, int rb, int lr, int lg) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = splat<f64x4>(0.0);
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+        const int i = 16 * rb + lr, k = 4 * kk + lg;
+        const double av = TX ? X[k * LDA + i] : X[i * LDA + k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = 16 * q + lr;
+            acc[q] = MF<double>::mma(av, TY ? Y[j * LDA + k] : Y[k * LDA + j], acc[q]);
+        }
+    }
+}
+
+__device__ __forceinline__ void gemm64_store(const f64x4 *acc, double *C, int rb, int lr, int lg) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) C[(16 * rb + MF<double>::crow(lg, r)) * LDA + 16 * q + lr] = acc[q][r];
+}
+
 // Deterministic block sum of one double per thread (NT threads); red has NT
 // entries; every thread gets the result.
 template <int NT>
@@ -1302,66 +1327,112 @@ __global__ __launch_bounds__(256) void k_proj(ProjArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// k_frechet: one workgroup per branch; for each mixture component j of the
-// branch's class (matrix m, weight w_j) the spectral projections in FP64:
-// out[0] = <A_b, P_m(tau)>, out[1] = w_j <A_b, Q_m P_m(tau)>,
-// out[2+phi] = w_j <M_phi (.) E_b, U^T A_b V^T>  (likelihood.hpp:1440-1520).
+// Fréchet stage (likelihood.hpp:1435-1520), in three kernels:
+//   k_asum     A_b = sum over pattern chunks of the adjoint partials (fixed
+//              order, deterministic)
+//   k_frechet  one workgroup per (branch, mixture component j of its class):
+//              A~ = U_m^T A_b V_m^T (two FP64 MFMA GEMMs), the component's share
+//              of d lnL / d tau_b = w_j sum_i lambda_i e^{lambda_i tau} A~_ii,
+//              <A_b, P_m(tau_b)> = sum_i e^{lambda_i tau} A~_ii, and
+//              W = w_j E o A~ with E_ij = (e^{l_i t} - e^{l_j t}) / (l_i - l_j)
+//              (t e^{l_i t} when |l_i - l_j| < 1e-9)
+//   k_hproj    per (matrix m, 256-entry chunk): H_m = sum of the W of every
+//              (branch, component) using m, projected on M_phi = V dQ_phi U
+//              for phi = omega, theta_AC..theta_GT
 // ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_asum(FrechetArgs a) {
+    const int bi = blockIdx.x, idx = blockIdx.y * 256 + threadIdx.x;
+    const double *src = a.Apart + (size_t)bi * a.n_chunks * 4096 + idx;
+    double s = 0.0;
+    int c = 0;
+    for (; c + 4 <= a.n_chunks; c += 4) {
+        const double x0 = src[(size_t)c * 4096], x1 = src[(size_t)(c + 1) * 4096];
+        const double x2 = src[(size_t)(c + 2) * 4096], x3 = src[(size_t)(c + 3) * 4096];
+        s += x0;
+        s += x1;
+        s += x2;
+        s += x3;
+    }
+    for (; c < a.n_chunks; ++c) s += src[(size_t)c * 4096];
+    a.Asum[(size_t)bi * 4096 + idx] = s;
+}
+
 __global__ __launch_bounds__(256) void k_frechet(FrechetArgs a) {
-    const int bi = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    const int bi = blockIdx.x, jj = blockIdx.y, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
+    const int J = a.comp_ptr[cls] + jj;
+    if (J >= a.comp_ptr[cls + 1]) return;
     __shared__ double As[64 * LDA];
     __shared__ double Bs[64 * LDA];
-    __shared__ double Ts[64 * LDA];
-    __shared__ double At[64 * LDA];
     __shared__ double lam[64], ex[64];
     __shared__ double red[4];
+    const int m = a.comp_mat[J];
+    const double wj = a.params[a.pl.off_w + J];
     const double tau = a.tau[node];
     for (int idx = tid; idx < 4096; idx += 256) {
-        double s = 0.0;
-        for (int c = 0; c < a.n_chunks; ++c) s += a.Apart[((size_t)bi * a.n_chunks + c) * 4096 + idx];
-        As[(idx >> 6) * LDA + (idx & 63)] = s;
+        As[(idx >> 6) * LDA + (idx & 63)] = a.Asum[(size_t)bi * 4096 + idx];
+        Bs[(idx >> 6) * LDA + (idx & 63)] = a.V[(size_t)m * 4096 + idx];
     }
-    const int j0 = a.comp_ptr[cls], j1 = a.comp_ptr[cls + 1];
-    for (int J = j0; J < j1; ++J) {
-        const int m = a.comp_mat[J];
-        const double wj = a.params[a.pl.off_w + J];
-        __syncthreads();
-        for (int idx = tid; idx < 4096; idx += 256) Bs[(idx >> 6) * LDA + (idx & 63)] = a.V[m * 4096 + idx];
-        if (tid < 64) {
-            lam[tid] = a.lam[m * 64 + tid];
-            ex[tid] = exp(lam[tid] * tau);
+    if (tid < 64) {
+        lam[tid] = a.lam[m * 64 + tid];
+        ex[tid] = exp(lam[tid] * tau);
+    }
+    __syncthreads();
+    f64x4 acc[4];
+    gemm64_acc<false, true>(As, Bs, acc, w, lr, lg);  // A V^T
+    __syncthreads();
+    gemm64_store(acc, As, w, lr, lg);
+    for (int idx = tid; idx < 4096; idx += 256) Bs[(idx >> 6) * LDA + (idx & 63)] = a.U[(size_t)m * 4096 + idx];
+    __syncthreads();
+    gemm64_acc<true, false>(Bs, As, acc, w, lr, lg);  // A~ = U^T (A V^T)
+    __syncthreads();
+    gemm64_store(acc, As, w, lr, lg);
+    __syncthreads();
+    double d0 = 0.0, d1 = 0.0;
+    double *Wout = a.W + ((size_t)bi * a.max_comp + jj) * 4096;
+    for (int idx = tid; idx < 4096; idx += 256) {
+        const int i = idx >> 6, j = idx & 63;
+        const double at = As[i * LDA + j];
+        if (i == j) {
+            d0 += ex[i] * at;
+            d1 += lam[i] * ex[i] * at;
         }
-        __syncthreads();
-        gemm64<false, true, 4>(As, Bs, Ts, w, lr, lg);  // T1 = A V^T
-        __syncthreads();
-        for (int idx = tid; idx < 4096; idx += 256) Bs[(idx >> 6) * LDA + (idx & 63)] = a.U[m * 4096 + idx];
-        __syncthreads();
-        gemm64<true, false, 4>(Bs, Ts, At, w, lr, lg);  // A~ = U^T T1
-        __syncthreads();
-        double acc[8];
-        for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-        const double *M = a.M + (size_t)m * 6 * 4096;
-        for (int idx = tid; idx < 4096; idx += 256) {
-            const int i = idx >> 6, j = idx & 63;
-            const double at = At[i * LDA + j];
-            if (i == j) {
-                acc[0] += ex[i] * at;
-                acc[1] += lam[i] * ex[i] * at;
-            }
-            const double d = lam[i] - lam[j];
-            const double E = (fabs(d) < 1e-9) ? tau * ex[i] : (ex[i] - ex[j]) / d;
-            const double ea = E * at;
+        const double d = lam[i] - lam[j];
+        const double E = (fabs(d) < 1e-9) ? tau * ex[i] : (ex[i] - ex[j]) / d;
+        Wout[idx] = wj * E * at;
+    }
+    const double r0 = block_sum<256>(d0, red, tid);
+    const double r1 = block_sum<256>(d1, red, tid);
+    if (tid == 0) {
+        double *o = a.out + ((size_t)bi * a.max_comp + jj) * 8;
+        o[0] = r0;
+        o[1] = wj * r1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hproj(FrechetArgs a) {
+    const int m = blockIdx.x, ch = blockIdx.y, tid = threadIdx.x, idx = ch * 256 + tid;
+    __shared__ double red[4];
+    const int e0 = a.wl_ptr[m], e1 = a.wl_ptr[m + 1];
+    double h = 0.0;
+    int e = e0;
+    for (; e + 4 <= e1; e += 4) {
+        const double x0 = a.W[(size_t)a.wl_idx[e] * 4096 + idx], x1 = a.W[(size_t)a.wl_idx[e + 1] * 4096 + idx];
+        const double x2 = a.W[(size_t)a.wl_idx[e + 2] * 4096 + idx], x3 = a.W[(size_t)a.wl_idx[e + 3] * 4096 + idx];
+        h += x0;
+        h += x1;
+        h += x2;
+        h += x3;
+    }
+    for (; e < e1; ++e) h += a.W[(size_t)a.wl_idx[e] * 4096 + idx];
+    const double *M = a.M + (size_t)m * 6 * 4096 + idx;
+    double v[6];
 #pragma unroll
-            for (int phi = 0; phi < 6; ++phi) acc[2 + phi] += M[(size_t)phi * 4096 + idx] * ea;
-        }
-        double res[8];
-        for (int q = 0; q < 8; ++q) res[q] = block_sum<256>(acc[q], red, tid);
-        if (tid == 0) {
-            double *o = a.out + ((size_t)bi * a.max_comp + (J - j0)) * 8;
-            o[0] = res[0];
-            for (int q = 1; q < 8; ++q) o[q] = wj * res[q];
-        }
+    for (int phi = 0; phi < 6; ++phi) v[phi] = M[(size_t)phi * 4096] * h;
+#pragma unroll
+    for (int phi = 0; phi < 6; ++phi) {
+        const double r = block_sum<256>(v[phi], red, tid);
+        if (tid == 0) a.proj[((size_t)m * 6 + phi) * HPROJ_CHUNKS + ch] = r;
     }
 }
 
@@ -1445,13 +1516,10 @@ __global__ __launch_bounds__(256) void k_final(FinalArgs a) {
         if (t < NM + 5) {
             const bool is_om = t < NM;
             const int phi = is_om ? 0 : (t - NM + 1);
-            for (int bi = lane; bi < a.n_branch; bi += 64) {
-                const int cls = a.blist[bi * BR_STRIDE + 3];
-                for (int j = a.comp_ptr[cls]; j < a.comp_ptr[cls + 1]; ++j) {
-                    if (is_om && a.comp_mat[j] != t) continue;
-                    v += a.fr[((size_t)bi * MC + (j - a.comp_ptr[cls])) * 8 + 2 + phi];
-                }
-            }
+            // <M_{m,phi}, H_m> partials: one matrix for omega_t, all matrices for theta
+            const int m0 = is_om ? t : 0, m1 = is_om ? t + 1 : NM;
+            for (int q = lane; q < (m1 - m0) * HPROJ_CHUNKS; q += 64)
+                v += a.proj[((size_t)(m0 + q / HPROJ_CHUNKS) * 6 + phi) * HPROJ_CHUNKS + q % HPROJ_CHUNKS];
             v = wave_sum(v);
             if (lane == 0) {
                 if (a.convention == 1) {
@@ -1544,7 +1612,9 @@ void launch_proj(const ProjArgs &a, int n_mat, hipStream_t s) {
 }
 
 void launch_frechet(const FrechetArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_frechet, dim3(a.n_branch), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_asum, dim3(a.n_branch, 16), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_frechet, dim3(a.n_branch, a.max_comp), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_hproj, dim3(a.n_mat, HPROJ_CHUNKS), dim3(256), 0, s, a);
 }
 
 void launch_final(const FinalArgs &a, hipStream_t s) { hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, a); }
