@@ -198,10 +198,12 @@ def main():
     os.environ.pop("HY_PROFILE", None)
     eng = make_engine(arrays, K, args.tile)
     ms32, out32, _ = run_steps(eng, eval_args, 1, args.steps, args.warmup, False)
+    tile32 = eng.last_tile()
     print(f"[bench] fp32 {ms32 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
     res = {}
     if not args.no_fp64:
         ms64, out64, _ = run_steps(eng, eval_args, 0, args.steps, args.warmup, False)
+        res["fp64_tile"] = eng.last_tile()
         print(f"[bench] fp64 {ms64 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
         g32 = np.concatenate([out32["grad_bl"], out32["grad_omega"], out32["grad_w"], out32["grad_theta"]])
         g64 = np.concatenate([out64["grad_bl"], out64["grad_omega"], out64["grad_w"], out64["grad_theta"]])
@@ -227,7 +229,7 @@ def main():
     ms32c, _, st32c = run_steps(peng, eval_args, 1, max(3, args.steps // 4), 1, True)
     peng.close()
     work = stage_work(arrays, K, 4)
-    tile = eng_tile(args.tile, len(arrays["weights"]))
+    tile = tile32
     cand = {s: stage_ms[s] for s in ("inside", "outside", "adjoint")}
     dom = max(cand, key=cand.get)
     roof = roofline(dom, work[dom], stage_ms[dom] * 1e-3)
@@ -310,15 +312,6 @@ def copy_calibration():
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
 
-
-def eng_tile(req, P):
-    if req:
-        return req
-    if -(-P // 64) >= 512:
-        return 64
-    if -(-P // 32) >= 512:
-        return 32
-    return 16
 
 
 if __name__ == "__main__":

@@ -34,6 +34,19 @@ struct DevBuf {
 struct hy_engine_s {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // side stream: dQ projections overlap the pruning passes
+    hipEvent_t ev_model = nullptr, ev_proj = nullptr;
+    // Captured evaluation graphs (parameter upload, every kernel on both
+    // streams, result download), replayed while the configuration epoch is
+    // unchanged.
+    struct GraphEntry {
+        int prec, grad, warm;
+        uint64_t epoch;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t epoch = 1;
+    bool use_graphs = true;
     std::string err;
     bool profile = false;
     hipEvent_t ev[9] = {};
@@ -52,6 +65,7 @@ struct hy_engine_s {
     std::vector<std::vector<int32_t>> ext_children;
     std::vector<uint8_t> ext_ident;
     int ext_root = -1, sched_key = 0;
+    int n_cu = 256, last_tile = 0;
     const void *ident_tiles[2] = {nullptr, nullptr};  // PA buffer whose identity tiles are current
     DevBuf d_osched, d_Uspill[2];
     int n_uspill = 0;
@@ -126,6 +140,7 @@ int ensure(hy_engine h, DevBuf &b, size_t bytes) {
     }
     b.bytes = bytes;
     h->device_bytes += bytes;
+    ++h->epoch;  // device pointers changed: captured graphs are stale
     return HY_OK;
 }
 
@@ -143,12 +158,26 @@ int upload(hy_engine h, DevBuf &b, const void *src, size_t bytes) {
         if (_rc != HY_OK) return _rc; \
     } while (0)
 
-int choose_tile(int P, int req) {
+// Pattern-tile width: the pruning kernels are latency-bound per tree node, so
+// the fewest "rounds" of resident workgroups win; a wider tile costs more per
+// node (measured ~1.3x for 32, ~2x for 64).  FP32 tiles of 16 fit two
+// workgroups per CU, every other variant one.
+int choose_tile(int P, int req, int prec, int n_cu) {
     if (req == 16 || req == 32 || req == 64) return req;
-    const int target = 512;  // >= 2 workgroups per CU on 256 CUs
-    if ((P + 63) / 64 >= target) return 64;
-    if ((P + 31) / 32 >= target) return 32;
-    return 16;
+    const double node_cost[3] = {1.0, 1.3, 2.0};
+    int best = 16;
+    double best_cost = 1e300;
+    for (int k = 0; k < 3; ++k) {
+        const int T = 16 << k;
+        const int slots = n_cu * ((prec == HY_PREC_FP32 && T == 16) ? 2 : 1);
+        const int tiles = (P + T - 1) / T;
+        const double cost = ((tiles + slots - 1) / slots) * node_cost[k];
+        if (cost <= best_cost) {
+            best_cost = cost;
+            best = T;
+        }
+    }
+    return best;
 }
 
 void build_layout(hy_engine h) {
@@ -202,10 +231,11 @@ int upload_identity_tiles(hy_engine h, int ri, int esz) {
 
 // (Re)allocate everything that depends on the tile width and precision.
 int prepare_buffers(hy_engine h, int prec) {
-    const int T = choose_tile(h->P, h->T_req);
+    const int T = choose_tile(h->P, h->T_req, prec, h->n_cu);
     const int ri = prec == HY_PREC_FP64 ? 1 : 0;
     const size_t rs = prec == HY_PREC_FP64 ? 8 : 4;
     if (T != h->T) {
+        ++h->epoch;
         h->T = T;
         h->n_tiles = (h->P + T - 1) / T;
         h->P_pad = h->n_tiles * T;
@@ -224,6 +254,7 @@ int prepare_buffers(hy_engine h, int prec) {
         }
     }
     if (h->sched_key != T * 16 + (int)rs) {
+        ++h->epoch;
         build_schedules(h, T, (int)rs);
         TRY(upload(h, h->d_sched, h->sched.data(), h->sched.size() * 4));
         TRY(upload(h, h->d_osched, h->osched.data(), h->osched.size() * 4));
@@ -248,11 +279,13 @@ int prepare_buffers(hy_engine h, int prec) {
         TRY(upload_identity_tiles(h, ri, (int)rs));
         h->ident_tiles[ri] = h->d_PA[ri].p;
     }
-    // adjoint chunking: aim for >= 1024 workgroups (4 per CU), each covering a
-    // contiguous run of tiles
+    // adjoint chunking: workgroups take whole 64-pattern groups (64 / T tiles);
+    // aim for >= 1024 workgroups (4 per CU)
     int nb = std::max(1, h->n_branch);
-    int nch = std::min(h->n_tiles, std::max(1, (1024 + nb - 1) / nb));
-    h->tiles_per_chunk = (h->n_tiles + nch - 1) / nch;
+    const int G = 64 / T;
+    const int n_groups = (h->n_tiles + G - 1) / G;
+    const int nchg = std::min(n_groups, std::max(1, (1024 + nb - 1) / nb));
+    h->tiles_per_chunk = ((n_groups + nchg - 1) / nchg) * G;
     h->n_chunks = (h->n_tiles + h->tiles_per_chunk - 1) / h->tiles_per_chunk;
     TRY(ensure(h, h->d_Apart, (size_t)nb * h->n_chunks * 4096 * 8));
     TRY(ensure(h, h->d_fr, (size_t)nb * h->max_comp * 8 * 8));
@@ -372,8 +405,12 @@ void build_schedules(hy_engine h, int T, int esz) {
         e[4] = h->ext_ident[u];
         e[5] = usrc[u];
         if (usrc[u] >= 0) free_slots.push_back(usrc[u]);
-        for (size_t c = 0; c < ch[u].size(); ++c) {
-            const int cn = ch[u][c];
+        // internal children first, tips last (the kernel keeps register-prefetched
+        // operands for the first two children only)
+        std::vector<int32_t> och(ch[u]);
+        std::stable_partition(och.begin(), och.end(), [&](int x) { return !ch[x].empty(); });
+        for (size_t c = 0; c < och.size(); ++c) {
+            const int cn = och[c];
             e[6 + 4 * c] = cn;
             e[7 + 4 * c] = ref_of(cn);
             e[8 + 4 * c] = h->vslot[cn];
@@ -434,6 +471,28 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         ma.warm = (h->warm_enable && h->warm_valid) ? 1 : 0;
         ma.status = (int *)h->d_status.p;
         launch_model(ma, h->n_mat, s);
+        if (grad) {
+            // M_phi = V dQ_phi U and d mu / d phi depend on the spectra only
+            HIP_TRY(h, hipEventRecord(h->ev_model, s));
+            HIP_TRY(h, hipStreamWaitEvent(h->stream2, h->ev_model, 0));
+            ProjArgs pj{};
+            pj.nd = (const int8_t *)h->d_nd.p;
+            pj.dpos = (const int8_t *)h->d_dpos.p;
+            pj.dfrom = (const int8_t *)h->d_dfrom.p;
+            pj.dto = (const int8_t *)h->d_dto.p;
+            pj.syn = (const uint8_t *)h->d_syn.p;
+            pj.pos = (const double *)h->d_pos.p;
+            pj.pi = (const double *)h->d_pi.p;
+            pj.S = h->S;
+            pj.params = params;
+            pj.pl = h->pl;
+            pj.U = (const double *)h->d_U.p;
+            pj.V = (const double *)h->d_V.p;
+            pj.M = (double *)h->d_M.p;
+            pj.dmu = (double *)h->d_dmu.p;
+            launch_proj(pj, h->n_mat, h->stream2);
+            HIP_TRY(h, hipEventRecord(h->ev_proj, h->stream2));
+        }
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[1], s));
         PmatArgs<R> pa{};
         pa.blist = (const int32_t *)h->d_blist.p;
@@ -518,22 +577,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         aa.Apart = (double *)h->d_Apart.p;
         if (h->n_branch > 0) launch_adjoint<R>(aa, h->T, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[5], s));
-        ProjArgs pj{};
-        pj.nd = (const int8_t *)h->d_nd.p;
-        pj.dpos = (const int8_t *)h->d_dpos.p;
-        pj.dfrom = (const int8_t *)h->d_dfrom.p;
-        pj.dto = (const int8_t *)h->d_dto.p;
-        pj.syn = (const uint8_t *)h->d_syn.p;
-        pj.pos = (const double *)h->d_pos.p;
-        pj.pi = (const double *)h->d_pi.p;
-        pj.S = h->S;
-        pj.params = params;
-        pj.pl = h->pl;
-        pj.U = (const double *)h->d_U.p;
-        pj.V = (const double *)h->d_V.p;
-        pj.M = (double *)h->d_M.p;
-        pj.dmu = (double *)h->d_dmu.p;
-        launch_proj(pj, h->n_mat, s);
+        HIP_TRY(h, hipStreamWaitEvent(s, h->ev_proj, 0));
         FrechetArgs fa{};
         fa.blist = (const int32_t *)h->d_blist.p;
         fa.n_branch = h->n_branch;
@@ -609,14 +653,20 @@ int hy_create(int device, hy_engine *out) {
                     prop.gcnArchName);
     hy_engine h = new hy_engine_s();
     h->device = device;
+    h->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_model, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_proj, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete h;
         return fail(nullptr, HY_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
     }
     const char *pe = std::getenv("HY_PROFILE");
     h->profile = pe && pe[0] == '1';
+    const char *ng = std::getenv("HY_NO_GRAPH");
+    h->use_graphs = !(ng && ng[0] == '1');
     if (h->profile)
         for (auto &ev : h->ev) (void)hipEventCreate(&ev);
     *out = h;
@@ -647,6 +697,11 @@ void hy_destroy(hy_engine h) {
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->profile)
         for (auto &ev : h->ev) (void)hipEventDestroy(ev);
+    for (auto &g : h->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (h->ev_model) (void)hipEventDestroy(h->ev_model);
+    if (h->ev_proj) (void)hipEventDestroy(h->ev_proj);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -654,6 +709,7 @@ void hy_destroy(hy_engine h) {
 const char *hy_last_error(hy_engine h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
 int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double *pi) {
+    if (h) ++h->epoch;  // captured graphs are stale
     if (!h) return HY_ERR_ARG;
     if (!pos_freqs || !pi) return fail(h, HY_ERR_ARG, "hy_set_model: NULL frequencies");
     HIP_TRY(h, hipSetDevice(h->device));
@@ -706,6 +762,7 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
 int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int32_t *child_ptr,
                    const int32_t *child_idx, const int32_t *leaf_taxon, int num_taxa, int num_patterns,
                    const int8_t *states, const uint64_t *masks, const double *weights, int S) {
+    if (h) ++h->epoch;  // captured graphs are stale
     if (!h) return HY_ERR_ARG;
     HIP_TRY(h, hipSetDevice(h->device));
     if (num_nodes < 1 || !parent || !child_ptr || !leaf_taxon)
@@ -914,6 +971,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
 
 int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_class, const int32_t *comp_ptr,
                    const int32_t *comp_mat, int time_convention) {
+    if (h) ++h->epoch;  // captured graphs are stale
     if (!h) return HY_ERR_ARG;
     if (!h->have_problem) return fail(h, HY_ERR_STATE, "hy_set_mixture before hy_set_problem");
     if (n_mat < 1 || n_mat > 48 || n_class < 1 || n_class > 64 || !comp_ptr || !comp_mat)
@@ -960,6 +1018,7 @@ int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_cl
 }
 
 int hy_set_tile(hy_engine h, int patterns_per_tile) {
+    if (h) ++h->epoch;  // captured graphs are stale
     if (!h) return HY_ERR_ARG;
     if (patterns_per_tile != 0 && patterns_per_tile != 16 && patterns_per_tile != 32 && patterns_per_tile != 64)
         return fail(h, HY_ERR_ARG, "hy_set_tile: tile must be 0, 16, 32 or 64 (got %d)", patterns_per_tile);
@@ -967,15 +1026,73 @@ int hy_set_tile(hy_engine h, int patterns_per_tile) {
     return HY_OK;
 }
 
-static int finish_eval(hy_engine h, bool check_status, double *out_logL, double *out_site_ll, double *out_grad_bl,
-                       double *out_grad_omega, double *out_grad_w, double *out_grad_theta, bool grad) {
+static int finish_host(hy_engine h, bool check_status, double *out_logL, double *out_grad_bl, double *out_grad_omega,
+                       double *out_grad_w, double *out_grad_theta, bool grad);
+
+// Result download of an eager evaluation (the graph path captures it).
+static int download_results(hy_engine h, bool check_status) {
     const size_t nres = 1 + (size_t)h->num_nodes + h->n_mat + h->n_comp + 5;
     HIP_TRY(h, hipMemcpyAsync(h->h_res, h->d_res.p, nres * 8, hipMemcpyDeviceToHost, h->stream));
     if (check_status)
         HIP_TRY(h, hipMemcpyAsync(h->h_status, h->d_status.p, sizeof(int) * h->n_mat, hipMemcpyDeviceToHost, h->stream));
+    return HY_OK;
+}
+
+static int finish_eval(hy_engine h, bool check_status, double *out_logL, double *out_site_ll, double *out_grad_bl,
+                       double *out_grad_omega, double *out_grad_w, double *out_grad_theta, bool grad) {
+    TRY(download_results(h, check_status));
     if (out_site_ll && h->P > 0)
         HIP_TRY(h, hipMemcpyAsync(out_site_ll, h->d_site.p, sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return finish_host(h, check_status, out_logL, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta, grad);
+}
+
+// Evaluates through a captured graph: parameters (already packed in the pinned
+// h_params) up, all kernels, results down.  Captures on first use per
+// (precision, gradient, warm-start) and whenever the configuration epoch moved.
+static int launch_graph(hy_engine h, int prec, bool grad) {
+    const int warm = (h->warm_enable && h->warm_valid) ? 1 : 0;
+    hy_engine_s::GraphEntry *ge = nullptr;
+    for (auto &g : h->graphs)
+        if (g.prec == prec && g.grad == (int)grad && g.warm == warm) ge = &g;
+    if (ge && ge->epoch != h->epoch) {
+        (void)hipGraphExecDestroy(ge->exec);
+        ge->exec = nullptr;
+    }
+    if (!ge || !ge->exec) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        int rc = HY_OK;
+        const hipError_t e1 = hipMemcpyAsync(h->d_params.p, h->h_params, sizeof(double) * h->pl.total,
+                                             hipMemcpyHostToDevice, h->stream);
+        if (e1 != hipSuccess) rc = fail(h, HY_ERR_HIP, "graph capture: %s", hipGetErrorString(e1));
+        const unsigned flags = HY_WANT_LOGL | (grad ? HY_WANT_GRAD : 0u);
+        if (rc == HY_OK) rc = prec == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
+        if (rc == HY_OK) rc = download_results(h, true);
+        const hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+        if (rc != HY_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e2 != hipSuccess) return fail(h, HY_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e3 != hipSuccess) return fail(h, HY_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+        if (!ge) {
+            h->graphs.push_back({prec, (int)grad, warm, h->epoch, exec});
+            ge = &h->graphs.back();
+        } else {
+            ge->exec = exec;
+            ge->epoch = h->epoch;
+        }
+    }
+    HIP_TRY(h, hipGraphLaunch(ge->exec, h->stream));
+    return HY_OK;
+}
+
+static int finish_host(hy_engine h, bool check_status, double *out_logL, double *out_grad_bl, double *out_grad_omega,
+                       double *out_grad_w, double *out_grad_theta, bool grad) {
     if (h->profile) {
         float ms;
         for (int k = 0; k < 7; ++k) {
@@ -1035,11 +1152,18 @@ int hy_eval(hy_engine h, int precision, unsigned flags, const double *bl, double
     for (int k = 0; k < 5; ++k) hp[pl.off_theta + k] = theta[k];
     hp[pl.off_delta] = delta;
     hp[pl.off_psi] = psi;
+    const bool grad = (flags & HY_WANT_GRAD) != 0;
+    if (h->use_graphs && !h->profile) {
+        TRY(launch_graph(h, precision, grad));
+        if (out_site_ll)
+            HIP_TRY(h, hipMemcpyAsync(out_site_ll, h->d_site.p, sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        return finish_host(h, true, out_logL, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta, grad);
+    }
     HIP_TRY(h, hipMemcpyAsync(h->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, h->stream));
     const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
     if (rc) return rc;
-    return finish_eval(h, true, out_logL, out_site_ll, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta,
-                       (flags & HY_WANT_GRAD) != 0);
+    return finish_eval(h, true, out_logL, out_site_ll, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta, grad);
 }
 
 int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
@@ -1065,7 +1189,14 @@ int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *
     return finish_eval(h, false, out_logL, out_site_ll, nullptr, nullptr, nullptr, nullptr, false);
 }
 
+int hy_last_tile(hy_engine h, int *patterns_per_tile) {
+    if (!h || !patterns_per_tile) return fail(h, HY_ERR_ARG, "hy_last_tile: NULL argument");
+    *patterns_per_tile = h->T;
+    return HY_OK;
+}
+
 int hy_set_warm_start(hy_engine h, int enable) {
+    if (h) ++h->epoch;  // captured graphs are stale
     if (!h) return HY_ERR_ARG;
     h->warm_enable = enable != 0;
     if (!h->warm_enable) h->warm_valid = false;

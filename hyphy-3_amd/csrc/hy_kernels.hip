@@ -966,7 +966,11 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
     stage_tip_codes<T, TL>(tcodes, a.tips, a.tip_rows, a.P_pad, p0, tid);
     const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
     __syncthreads();
-    R pa[MAXC][16], pt[MAXC][16], dv[MAXC][DV];
+    // Register-prefetched operands cover the first RS children; the host
+    // orders internal children first and tips last, so a third internal child
+    // (only at nodes with three internal children) is loaded when used.
+    constexpr int RS = 2;
+    R pa[RS][16], pt[RS][16], dv[RS][DV];
     int de[MAXC][NB];
     acc_t xt[MAXC][NB];
     unsigned ab = 0;  // partially ambiguous tip lanes of the prefetched operands
@@ -994,8 +998,11 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
             gather_tip<R, T, TL>(xt[c], ab, c, c < nc && kind == KIND_TIP, a, tcodes, pn, idx, p0, w, lr, lg);
             if (c < nc) {
                 if (kind != KIND_TIP) {
-                    fetch_edge<R>(pa[c], a.PA, pn, w, l);
-                    if constexpr (DREG) fetch_b<R, T>(dv[c], a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
+                    if (c < RS) {
+                        fetch_edge<R>(pa[c], a.PA, pn, w, l);
+                        if constexpr (DREG)
+                            fetch_b<R, T>(dv[c], a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
+                    }
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) de[c][nb] = a.E[(size_t)idx * a.P_pad + p0 + 16 * nb + lr];
                 }
@@ -1060,13 +1067,20 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
                         Ex[c][nb] = 0;
                     }
                 } else {
-                    if constexpr (DREG) {
-                        mfma_regs<R, T>(pa[c], dv[c], X[c]);
+                    if (c < RS && DREG) {
+                        mfma_regs<R, T>(pa[c < RS ? c : 0], dv[c < RS ? c : 0], X[c]);
                     } else {
-                        R bv[16 * NB];
+                        // operands loaded at use (the next child's loads may
+                        // reuse these registers once X is consumed)
+                        R bv[16 * NB], av[16];
                         fetch_b<R, T>(bv, a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
-                        mfma_regs<R, T>(pa[c], bv, X[c]);
-                        consume<R, T>(X[c]);  // the next child's loads may reuse bv's registers
+                        if (c < RS)
+                            mfma_regs<R, T>(pa[c < RS ? c : 0], bv, X[c]);
+                        else {
+                            fetch_edge<R>(av, a.PA, ec.v[6 + 4 * c], w, l);
+                            mfma_regs<R, T>(av, bv, X[c]);
+                        }
+                        consume<R, T>(X[c]);
                     }
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) Ex[c][nb] = de[c][nb];
@@ -1113,7 +1127,7 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
         // products above, the previous U_c by U): transposed edge tiles of this
         // node and all operands of the next one.
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c)
+        for (int c = 0; c < RS; ++c)
             if (c < nc && (ec.v[7 + 4 * c] >> 28) == KIND_INT) fetch_edge<R>(pt[c], a.PTA, ec.v[6 + 4 * c], w, l);
         if (k + 1 < n) operands(en);
         __syncthreads();
@@ -1151,7 +1165,13 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
                 const int kind = ref >> 28, idx = ref & REF_MASK;
                 if (kind != KIND_TIP) {
                     acc_t Uc[NB];
-                    mfma_tile<R, T>(pt[c], Xc + c * 64 * T, Uc, lr, lg);
+                    if (c < RS) {
+                        mfma_tile<R, T>(pt[c < RS ? c : 0], Xc + c * 64 * T, Uc, lr, lg);
+                    } else {
+                        R av[16];
+                        fetch_edge<R>(av, a.PTA, ec.v[6 + 4 * c], w, l);
+                        mfma_tile<R, T>(av, Xc + c * 64 * T, Uc, lr, lg);
+                    }
                     if (dst == U_NEXT) {
 #pragma unroll
                         for (int nb = 0; nb < NB; ++nb) {
@@ -1189,63 +1209,99 @@ __device__ __forceinline__ R tip_onehot(unsigned code, int j, const uint64_t *am
 
 template <typename R, int T>
 __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> a) {
+    // Works on groups of 64 patterns (G pruning tiles, contiguous in HBM):
+    // the next group is fetched into registers while the current one is
+    // multiplied from LDS (double-buffered).
+    constexpr int G = 64 / T, PER = 4096 / 256;
     typedef typename MF<R>::acc acc_t;
-    __shared__ R Vs[64 * T];
-    __shared__ R Ds[64 * T];
-    __shared__ R cs[T];
-    __shared__ unsigned codes[T];
+    __shared__ R Vs[2][4096];  // G tiles of V, each [64 states][T] row-major
+    __shared__ R Ds[2][4096];  // G tiles of D, each B-operand-major
+    __shared__ R cs[2][64];
+    __shared__ unsigned codes[2][64];
     const int chunk = blockIdx.x, bi = blockIdx.y;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
     const int32_t *br = a.blist + bi * BR_STRIDE;
     const int vslot = br[1], ref = br[2];
     const int kind = ref >> 28, idx = ref & REF_MASK;
+    const bool tip = kind == KIND_TIP;
+    const int t0 = chunk * a.tiles_per_chunk;
+    const int t1 = min(t0 + a.tiles_per_chunk, a.n_tiles);
+    const int ng = (t1 - t0 + G - 1) / G;
+    R rv[PER], rd[PER];
+    R rc = 0;
+    unsigned rcode = 0;
+    auto fetch = [&](int g) {
+        const int tg0 = t0 + g * G;
+        const R *Vg = a.V + ((size_t)vslot * a.n_tiles + tg0) * (64 * T);
+        const R *Dg = a.D + ((size_t)idx * a.n_tiles + tg0) * (64 * T);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 256 + tid;
+            const bool valid = tg0 + e / (64 * T) < t1;
+            rv[q] = valid ? Vg[e] : (R)0;
+            rd[q] = (valid && !tip) ? Dg[e] : (R)0;
+        }
+        if (tid < 64) {
+            const int p = tg0 * T + tid;
+            double c = 0.0;
+            rcode = 0;
+            if (p < t1 * T) {
+                const double L = a.Lroot[p], wt = a.weights[p];
+                if (wt > 0.0 && L > 0.0) {
+                    int ex = a.G[(size_t)vslot * a.P_pad + p] - a.Eroot[p];
+                    if (!tip) ex += a.E[(size_t)idx * a.P_pad + p];
+                    c = ldexp(wt / L, ex);
+                }
+                if (tip) rcode = a.tips[(size_t)idx * a.P_pad + p];
+            }
+            rc = (R)c;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            Vs[buf][q * 256 + tid] = rv[q];
+            Ds[buf][q * 256 + tid] = rd[q];
+        }
+        if (tid < 64) {
+            cs[buf][tid] = rc;
+            codes[buf][tid] = rcode;
+        }
+    };
     f64x4 acc[4];
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[cb] = splat<f64x4>(0.0);
-    const int t0 = chunk * a.tiles_per_chunk;
-    int t1 = t0 + a.tiles_per_chunk;
-    if (t1 > a.n_tiles) t1 = a.n_tiles;
-    for (int tile = t0; tile < t1; ++tile) {
-        const int p0 = tile * T;
-        load_tile<R, T>(Vs, a.V + ((size_t)vslot * a.n_tiles + tile) * (64 * T), tid);
-        if (kind == KIND_TIP) {
-            if (tid < T) codes[tid] = a.tips[(size_t)idx * a.P_pad + p0 + tid];
-        } else {
-            load_tile<R, T>(Ds, a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), tid);
-        }
-        if (tid < T) {
-            const int p = p0 + tid;
-            const double L = a.Lroot[p], wt = a.weights[p];
-            double c = 0.0;
-            if (wt > 0.0 && L > 0.0) {
-                int ex = a.G[(size_t)vslot * a.P_pad + p] - a.Eroot[p];
-                if (kind != KIND_TIP) ex += a.E[(size_t)idx * a.P_pad + p];
-                c = ldexp(wt / L, ex);
-            }
-            cs[tid] = (R)c;
-        }
-        __syncthreads();
+    if (ng > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    for (int g = 0; g < ng; ++g) {
+        const int buf = g & 1;
+        if (g + 1 < ng) fetch(g + 1);
         acc_t loc[4];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) loc[cb] = splat<acc_t>(0.0);
 #pragma unroll
-        for (int kk = 0; kk < T / 4; ++kk) {
-            const int pl = 4 * kk + lg;
-            const R av = Vs[(16 * w + lr) * T + pl] * cs[pl];
-            if (kind == KIND_TIP) {
-                const unsigned code = codes[pl];
+        for (int kk = 0; kk < 16; ++kk) {
+            const int p = 4 * kk + lg, ti = p / T, pt = p % T;
+            const R av = Vs[buf][ti * (64 * T) + (16 * w + lr) * T + pt] * cs[buf][p];
+            if (tip) {
+                const unsigned code = codes[buf][p];
 #pragma unroll
                 for (int cb = 0; cb < 4; ++cb)
                     loc[cb] = MF<R>::mma(av, tip_onehot<R>(code, 16 * cb + lr, a.amb, a.S), loc[cb]);
             } else {
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) loc[cb] = MF<R>::mma(av, Ds[bmaj(16 * cb + lr, pl)], loc[cb]);
+                for (int cb = 0; cb < 4; ++cb)
+                    loc[cb] = MF<R>::mma(av, Ds[buf][ti * (64 * T) + bmaj(16 * cb + lr, pt)], loc[cb]);
             }
         }
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[cb][r] += (double)loc[cb][r];
+        if (g + 1 < ng) stash(buf ^ 1);
         __syncthreads();
     }
     double *out = a.Apart + ((size_t)bi * a.n_chunks + chunk) * 4096;
@@ -1463,9 +1519,19 @@ __global__ __launch_bounds__(256) void k_final(FinalArgs a) {
     const int P = a.n_patterns;
     const int per = (P + 255) / 256;
     double s = 0.0, c = 0.0;
-    for (int p = tid * per; p < (tid + 1) * per && p < P; ++p) {
-        if (a.want_grad && !(a.Lroot[p] > 0.0)) continue;
-        neum(s, c, a.weights[p] * a.site_ll[p]);
+    const int pe = min((tid + 1) * per, P);
+    for (int p0 = tid * per; p0 < pe; p0 += 8) {
+        // eight independent loads in flight, then the compensated sums in order
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int p = p0 + u;
+            x[u] = 0.0;
+            if (p < pe && !(a.want_grad && !(a.Lroot[p] > 0.0))) x[u] = a.weights[p] * a.site_ll[p];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (p0 + u < pe) neum(s, c, x[u]);
     }
     ps[tid] = s;
     pc[tid] = c;

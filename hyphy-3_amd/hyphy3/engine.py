@@ -26,7 +26,7 @@ HY_WANT_GRAD = 4
 
 EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
-           "hy_set_warm_start", "hy_last_eigen_status")
+           "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -67,6 +67,7 @@ def load_library(path: str = LIB_PATH):
     L.hy_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
     L.hy_set_warm_start.argtypes = [C.c_void_p, C.c_int]
     L.hy_last_eigen_status.argtypes = [C.c_void_p, _i32p, C.c_int]
+    L.hy_last_tile.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -196,6 +197,11 @@ class Engine:
         t = np.zeros(8)
         self._check(self.L.hy_last_timings(self.h, _p(t, _f64p)))
         return t
+
+    def last_tile(self):
+        t = C.c_int()
+        self._check(self.L.hy_last_tile(self.h, C.byref(t)))
+        return t.value
 
     def device_bytes(self):
         v = C.c_size_t()

@@ -122,8 +122,14 @@ int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *
                      double *out_site_ll);
 
 /* Pattern-tile width used by the pruning kernels (16, 32 or 64 patterns per
- * workgroup; 0 = automatic).  Applies from the next hy_eval. */
+ * workgroup; 0 = automatic, chosen per precision from the pattern count and
+ * the device's CU count).  Applies from the next hy_eval; a change of tile
+ * width (e.g. when alternating precisions) re-tiles the resident problem. */
 int hy_set_tile(hy_engine h, int patterns_per_tile);
+
+/* Tile width the last hy_eval / hy_loglik_from_P used.  No counterpart in the
+ * reference (tuning introspection). */
+int hy_last_tile(hy_engine h, int *patterns_per_tile);
 
 /* Warm starts (default on): each hy_eval refines the previous call's
  * eigenvectors of every rate matrix (Ogita-Aishima iteration, FP64) and only
