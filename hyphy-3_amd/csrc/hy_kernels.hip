@@ -831,8 +831,8 @@ __global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_in
         for (int c = 0; c < MAXC; ++c) {
             const int pn = e.v[6 + 3 * c], ref = e.v[7 + 3 * c], kind = ref >> 28;
             if (c < nc && kind == KIND_INT) fetch_edge<R>(pa[c], a.PA, pn, w, l);
-            gather_tip<R, T, TL>(xt[c], ab, c, c < nc && kind == KIND_TIP, a, tcodes, pn, ref & REF_MASK, p0, w, lr,
-                                 lg);
+            if (c < nc && kind == KIND_TIP)
+                gather_tip<R, T, TL>(xt[c], ab, c, true, a, tcodes, pn, ref & REF_MASK, p0, w, lr, lg);
         }
     };
     Ent ec = load_entry<SCHED_STRIDE>(a.sched, 0);
@@ -983,19 +983,15 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
     auto operands = [&](const Ent &e) {
         const int nc = e.v[2];
         ab = 0;
-        {
-            // spilled U message (branch-free: other entries read a harmless tile)
-            const bool sp = e.v[5] == U_SPILL;
-            const R *ub = sp ? a.Uspill + ((size_t)e.v[1] * a.n_tiles + tile) * (64 * T) : a.D + (size_t)tile * (64 * T);
-            const int32_t *hb = sp ? a.G + (size_t)e.v[3] * a.P_pad : a.E;
-            load_c<R, T>(us, ub, w, lr, lg);
+        if (e.v[5] == U_SPILL) {  // message spilled to HBM by the parent
+            load_c<R, T>(us, a.Uspill + ((size_t)e.v[1] * a.n_tiles + tile) * (64 * T), w, lr, lg);
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) hs[nb] = hb[p0 + 16 * nb + lr];
+            for (int nb = 0; nb < NB; ++nb) hs[nb] = a.G[(size_t)e.v[3] * a.P_pad + p0 + 16 * nb + lr];
         }
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) {
             const int pn = e.v[6 + 4 * c], ref = e.v[7 + 4 * c], kind = ref >> 28, idx = ref & REF_MASK;
-            gather_tip<R, T, TL>(xt[c], ab, c, c < nc && kind == KIND_TIP, a, tcodes, pn, idx, p0, w, lr, lg);
+            if (c < nc && kind == KIND_TIP) gather_tip<R, T, TL>(xt[c], ab, c, true, a, tcodes, pn, idx, p0, w, lr, lg);
             if (c < nc) {
                 if (kind != KIND_TIP) {
                     if (c < RS) {
