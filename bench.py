@@ -38,6 +38,8 @@ CONFIGS = {
                gaps=0.25),
     "c5": dict(taxa=256, codons=10000, omegas=[0.1, 0.8, 3.0], weights=[0.6, 0.35, 0.05], seed=505, gaps=0.0),
 }
+# Many-genes batch: sizes log-uniform, categories uniform on 1..4.
+C3 = dict(genes=1024, seed=303, taxa=(8, 128), codons=(100, 3000), omegas=[0.1, 0.8, 3.0, 0.4])
 STAGES = ["spectra", "pmat", "inside", "outside", "adjoint", "frechet", "final"]
 
 
@@ -86,9 +88,9 @@ def stage_work(arrays, K, b):
     }
 
 
-def make_engine(arrays, K, T=0):
+def make_engine(arrays, K, T=0, eng=None):
     from hyphy3.engine import Engine
-    eng = Engine(0)
+    eng = eng or Engine(0)
     children = [list(arrays["child_idx"][arrays["child_ptr"][i]:arrays["child_ptr"][i + 1]])
                 for i in range(len(arrays["parent"]))]
     eng.set_problem(arrays["parent"], children, arrays["leaf_taxon"], arrays["states"], arrays["masks"],
@@ -172,7 +174,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
+    ap.add_argument("--genes", type=int, default=C3["genes"], help="c3 only: number of genes in the batch")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -185,6 +188,9 @@ def main():
         print(json.dumps({"error": "this build measures one MI355X; multi-GPU runs are not part of it "
                                    "(the path would shard patterns across GPUs, see DESIGN.md)"}))
         sys.exit(2)
+    if args.config == "c3":
+        bench_genes(args)
+        return
     cfg = CONFIGS[args.config]
     K = len(cfg["omegas"])
     if args.copy_calibration:
@@ -267,6 +273,133 @@ def main():
         except Exception as ex:  # the oracle library may not be built on a bare box
             line["cpu_baseline"] = {"value": None, "error": str(ex)}
     print(json.dumps(line), flush=True)
+
+
+def gene_specs(n_genes, seed):
+    """(taxa, codons, categories, seed) of every gene of the batch."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for g in range(n_genes):
+        n = int(round(np.exp(rng.uniform(*np.log(C3["taxa"])))))
+        L = int(round(np.exp(rng.uniform(*np.log(C3["codons"])))))
+        K = int(rng.integers(1, 5))
+        out.append((n, L, K, seed * 100000 + g))
+    return out
+
+
+def build_gene(spec):
+    n, L, K, seed = spec
+    w = list(np.full(K, 1.0 / K))
+    cfg = dict(taxa=n, codons=L, omegas=C3["omegas"][:K], weights=w, seed=seed, gaps=0.0)
+    sim, aln, tree, arrays, tau = build_problem(cfg)
+    return cfg, {"newick": sim["newick"], "fasta": sim["fasta"], "theta": sim["theta"], "conv": sim["conv"]}, arrays, tau
+
+
+def bench_genes(args):
+    """Batched many-genes workload: one step = logL + full gradient of every
+    gene of the batch through one hy_eval_batch call."""
+    from multiprocessing import get_context
+    from hyphy3.engine import create_group, eval_batch
+    specs = gene_specs(args.genes, C3["seed"])
+    nproc = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    with get_context("fork").Pool(nproc) as pool:  # before any device work
+        genes = pool.map(build_gene, specs, chunksize=8)
+    print(f"[bench] c3: {len(genes)} genes built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    engines, bases = create_group(len(genes)), []
+    S = S_ALG
+    f_alg = 0.0
+    pats = 0
+    for (cfg, sim, arrays, tau), eng in zip(genes, engines):
+        K = len(cfg["omegas"])
+        make_engine(arrays, K, eng=eng)
+        bases.append((tau, list(cfg["omegas"]), list(cfg["weights"]), sim["theta"]))
+        meta = stage_work(arrays, K, 4)["_meta"]
+        f_alg += 6.0 * S * S * meta["P"] * meta["e_int"] + 8.0 * S ** 3 * meta["B"] * K
+        pats += meta["P"]
+    dev_bytes = sum(e.device_bytes() for e in engines)
+    print(f"[bench] c3: {len(engines)} engines set up", file=sys.stderr, flush=True)
+
+    def step(i, prec):
+        return eval_batch(engines, [step_params(b, i) for b in bases], precision=prec)
+
+    res = {}
+    for prec, name in ((1, "fp32"), (0, "fp64")):
+        if prec == 0 and args.no_fp64:
+            continue
+        for i in range(args.warmup):
+            step(i, prec)
+            print(f"[bench] c3 {name} warmup step {i} done", file=sys.stderr, flush=True)
+        sync()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            out = step(args.warmup + i, prec)
+        sync()
+        res[name] = ((time.perf_counter() - t0) / args.steps, out)
+        print(f"[bench] c3 {name} {res[name][0] * 1e3:.2f} ms/batch", file=sys.stderr, flush=True)
+    ms32, out32 = res["fp32"]
+    t_bound = f_alg / (PEAK_FP32_TFLOPS * 1e12)
+    line = {
+        "metric": "batched genes: LL+gradient evals/sec of the whole batch (FP32-MFMA)", "value": 1.0 / ms32,
+        "unit": "batch evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms32 * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"c3: {len(genes)} genes, taxa log-U{C3['taxa']}, codons log-U{C3['codons']}, "
+                               "K ~ U{1..4}; logL + full gradient per gene via hy_eval_batch",
+                   "genes": len(genes), "patterns_total": int(pats), "seed": C3["seed"]},
+        "gene_evals_per_s": len(genes) / ms32,
+        "eval_roofline": {"flops": f_alg, "t_bound_ms": t_bound * 1e3, "frac": t_bound / ms32,
+                          "note": "sum over genes of 6 S^2 P E_int + 8 S^3 B K at 157.3 TFLOP/s"},
+        "device_bytes": int(dev_bytes),
+    }
+    if "fp64" in res:
+        ms64, out64 = res["fp64"]
+        line["fp64_ms_per_step"] = ms64 * 1e3
+        line["fp32_over_fp64"] = ms64 / ms32
+        line["max_rel_err_logl_vs_fp64"] = max(abs(a["log_likelihood"] - b["log_likelihood"]) / abs(b["log_likelihood"])
+                                               for a, b in zip(out32, out64))
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_genes(genes, args.cpu_seconds, f_alg)
+        if line["cpu_baseline"].get("value"):
+            line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+    for e in engines:
+        e.close()
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_genes(genes, seconds, f_total):
+    """Oracle on the first genes of the batch until `seconds` pass, scaled to
+    the whole batch by algorithmic flops."""
+    from oracle import data_ref as dr
+    from oracle import hyoracle as ho
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    S = S_ALG
+    t0 = time.perf_counter()
+    f_done, n = 0.0, 0
+    for cfg, sim, arrays, tau in genes:
+        names, seqs = dr.parse_fasta(sim["fasta"])
+        aln = dr.AlignmentRef(names, seqs)
+        tree = dr.TreeRef.from_newick(sim["newick"], keep_rooted=False)
+        prob = ho.Problem.from_refs(tree, aln)
+        K = len(cfg["omegas"])
+        N = len(tree.nodes)
+        mix = ho.Mixture(cfg["omegas"], np.zeros(N, dtype=np.int32), [[(k, w) for k, w in enumerate(cfg["weights"])]], 0)
+        bl = np.array([0.0 if nd.id == tree.root else nd.bl * sim["conv"] for nd in tree.nodes])
+        P = prob.num_patterns
+        tile = 64 if threads <= 1 else int(min(64, max(16, -(-P // threads))))
+        ho.mixture_eval(prob, ho.Code(0), ho.params(1.0, 1.0, sim["theta"]), aln.pos_freqs, aln.cf3x4, mix, bl,
+                        want_grad=True, tile=tile, threads=threads)
+        meta = stage_work(arrays, K, 4)["_meta"]
+        f_done += 6.0 * S * S * meta["P"] * meta["e_int"] + 8.0 * S ** 3 * meta["B"] * K
+        n += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    el = time.perf_counter() - t0
+    frac = f_done / f_total
+    return {"value": frac / el, "unit": "batch evals/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of {len(genes)} genes ({100 * frac:.1f}% of the batch's flops) in {el:.1f} s, "
+                      "scaled to the batch by algorithmic flops (FP64 restatement of the reference algorithm, OpenMP)"}
 
 
 KERNEL_NAMES = {"inside": "k_inside", "outside": "k_outside", "adjoint": "k_adjoint"}

@@ -7,10 +7,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -31,8 +33,26 @@ struct DevBuf {
 
 }  // namespace
 
+// The streams and fork/join events an engine evaluates on.  An engine made by
+// hy_create owns one set; engines made together by hy_create_group share a
+// few sets round-robin, so that thousands of per-gene engines do not hold
+// thousands of streams and events.  Destroyed with its last engine.
+struct StreamSet {
+    int device = 0;
+    hipStream_t s1 = nullptr, s2 = nullptr;
+    hipEvent_t ev_model = nullptr, ev_proj = nullptr;
+    ~StreamSet() {
+        (void)hipSetDevice(device);
+        if (ev_model) (void)hipEventDestroy(ev_model);
+        if (ev_proj) (void)hipEventDestroy(ev_proj);
+        if (s2) (void)hipStreamDestroy(s2);
+        if (s1) (void)hipStreamDestroy(s1);
+    }
+};
+
 struct hy_engine_s {
     int device = 0;
+    std::shared_ptr<StreamSet> streams;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // side stream: dQ projections overlap the pruning passes
     hipEvent_t ev_model = nullptr, ev_proj = nullptr;
@@ -74,6 +94,14 @@ struct hy_engine_s {
     std::vector<uint64_t> amb;
     std::vector<double> weights;
     DevBuf d_sched, d_blist, d_tips, d_amb, d_weights;
+    // Per-evaluation pattern weights replacing d_weights in the adjoint and the
+    // final reductions (posterior-weighted passes of the synonymous-rate
+    // mixture); nullptr = the problem's weights.  Evaluations with it set run
+    // eagerly (graphs bake the weight pointer in).
+    DevBuf d_wover;
+    const double *wover = nullptr;
+    double *h_site = nullptr;  // pinned staging of per-pattern lnL
+    size_t h_site_cap = 0;
     int T_req = 0, T = 0, n_tiles = 0, P_pad = 0;
 
     // mixture
@@ -568,7 +596,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         aa.G = pr.G;
         aa.Eroot = pr.Eroot;
         aa.Lroot = pr.Lroot;
-        aa.weights = (const double *)h->d_weights.p;
+        aa.weights = h->wover ? h->wover : (const double *)h->d_weights.p;
         aa.tips = pr.tips;
         aa.amb = pr.amb;
         aa.P_pad = h->P_pad;
@@ -607,7 +635,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     }
     FinalArgs fn{};
     fn.site_ll = (const double *)h->d_site.p;
-    fn.weights = (const double *)h->d_weights.p;
+    fn.weights = h->wover ? h->wover : (const double *)h->d_weights.p;
     fn.Lroot = (const double *)h->d_Lroot.p;
     fn.n_patterns = h->P;
     fn.want_grad = grad ? 1 : 0;
@@ -635,6 +663,40 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
 
 }  // namespace
 
+namespace {
+
+hipError_t make_stream_set(int device, std::shared_ptr<StreamSet> &out) {
+    auto ss = std::make_shared<StreamSet>();
+    ss->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ss->s1, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ss->s2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ss->ev_model, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ss->ev_proj, hipEventDisableTiming);
+    if (e == hipSuccess) out = ss;
+    return e;
+}
+
+hy_engine new_engine(int device, int n_cu, const std::shared_ptr<StreamSet> &ss) {
+    hy_engine h = new hy_engine_s();
+    h->device = device;
+    h->n_cu = n_cu > 0 ? n_cu : 256;
+    h->streams = ss;
+    h->stream = ss->s1;
+    h->stream2 = ss->s2;
+    h->ev_model = ss->ev_model;
+    h->ev_proj = ss->ev_proj;
+    const char *pe = std::getenv("HY_PROFILE");
+    h->profile = pe && pe[0] == '1';
+    const char *ng = std::getenv("HY_NO_GRAPH");
+    h->use_graphs = !(ng && ng[0] == '1');
+    if (h->profile)
+        for (auto &ev : h->ev) (void)hipEventCreate(&ev);
+    return h;
+}
+
+}  // namespace
+
 extern "C" {
 
 int hy_create(int device, hy_engine *out) {
@@ -651,25 +713,39 @@ int hy_create(int device, hy_engine *out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(nullptr, HY_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950 (MI355X)", device,
                     prop.gcnArchName);
-    hy_engine h = new hy_engine_s();
-    h->device = device;
-    h->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_model, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_proj, hipEventDisableTiming);
-    if (e != hipSuccess) {
-        delete h;
-        return fail(nullptr, HY_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+    std::shared_ptr<StreamSet> ss;
+    e = make_stream_set(device, ss);
+    if (e != hipSuccess) return fail(nullptr, HY_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+    *out = new_engine(device, prop.multiProcessorCount, ss);
+    return HY_OK;
+}
+
+int hy_create_group(int device, int n, int n_stream_sets, hy_engine *out) {
+    if (!out || n < 1) return fail(nullptr, HY_ERR_ARG, "hy_create_group: need n >= 1 and an output array");
+    for (int i = 0; i < n; ++i) out[i] = nullptr;
+    int nd = 0;
+    hipError_t e = hipGetDeviceCount(&nd);
+    if (e != hipSuccess || nd == 0)
+        return fail(nullptr, HY_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= nd) return fail(nullptr, HY_ERR_ARG, "device %d out of range (%d devices)", device, nd);
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return fail(nullptr, HY_ERR_HIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, HY_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950 (MI355X)", device,
+                    prop.gcnArchName);
+    const int ns = std::max(1, std::min(n, n_stream_sets > 0 ? n_stream_sets : 8));
+    std::vector<std::shared_ptr<StreamSet>> sets(ns);
+    for (auto &ss : sets) {
+        e = make_stream_set(device, ss);
+        if (e != hipSuccess) return fail(nullptr, HY_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
     }
-    const char *pe = std::getenv("HY_PROFILE");
-    h->profile = pe && pe[0] == '1';
-    const char *ng = std::getenv("HY_NO_GRAPH");
-    h->use_graphs = !(ng && ng[0] == '1');
-    if (h->profile)
-        for (auto &ev : h->ev) (void)hipEventCreate(&ev);
-    *out = h;
+    for (int i = 0; i < n; ++i) {
+        out[i] = new_engine(device, prop.multiProcessorCount, sets[i % ns]);
+        // Grouped engines launch eagerly: thousands of instantiated graphs
+        // (each holding its own runtime resources) are not kept per gene.
+        out[i]->use_graphs = false;
+    }
     return HY_OK;
 }
 
@@ -681,7 +757,7 @@ void hy_destroy(hy_engine h) {
                       &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
                       &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
-                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_res, &h->d_Asum, &h->d_W, &h->d_wl, &h->d_proj};
+                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_res, &h->d_Asum, &h->d_W, &h->d_wl, &h->d_proj, &h->d_wover};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -695,14 +771,11 @@ void hy_destroy(hy_engine h) {
     if (h->h_params) (void)hipHostFree(h->h_params);
     if (h->h_res) (void)hipHostFree(h->h_res);
     if (h->h_status) (void)hipHostFree(h->h_status);
+    if (h->h_site) (void)hipHostFree(h->h_site);
     if (h->profile)
         for (auto &ev : h->ev) (void)hipEventDestroy(ev);
     for (auto &g : h->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (h->ev_model) (void)hipEventDestroy(h->ev_model);
-    if (h->ev_proj) (void)hipEventDestroy(h->ev_proj);
-    if (h->stream2) (void)hipStreamDestroy(h->stream2);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
 
@@ -1080,6 +1153,14 @@ static int launch_graph(hy_engine h, int prec, bool grad) {
         (void)hipGraphDestroy(graph);
         if (e3 != hipSuccess) return fail(h, HY_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
         if (!ge) {
+            // the cold-start graph of this kind is not used again once spectra
+            // are warm: release it rather than keep its runtime resources
+            for (auto it = h->graphs.begin(); it != h->graphs.end(); ++it)
+                if (warm && it->prec == prec && it->grad == (int)grad && it->warm == 0) {
+                    if (it->exec) (void)hipGraphExecDestroy(it->exec);
+                    h->graphs.erase(it);
+                    break;
+                }
             h->graphs.push_back({prec, (int)grad, warm, h->epoch, exec});
             ge = &h->graphs.back();
         } else {
@@ -1122,25 +1203,36 @@ static int finish_host(hy_engine h, bool check_status, double *out_logL, double 
     return HY_OK;
 }
 
-int hy_eval(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
-            const double *comp_w, const double *theta, double delta, double psi, double *out_logL,
-            double *out_site_ll, double *out_grad_bl, double *out_grad_omega, double *out_grad_w,
-            double *out_grad_theta) {
-    if (!h) return HY_ERR_ARG;
-    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "hy_eval before hy_set_problem/hy_set_model");
-    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
-        return fail(h, HY_ERR_ARG, "hy_eval: unknown precision %d", precision);
-    if (!bl || !omegas || !comp_w || !theta) return fail(h, HY_ERR_ARG, "hy_eval: NULL parameter array");
-    if (h->P == 0) {
-        if (out_logL) *out_logL = 0.0;
-        if (flags & HY_WANT_GRAD) {
-            if (out_grad_bl) std::fill(out_grad_bl, out_grad_bl + h->num_nodes, 0.0);
-            if (out_grad_omega) std::fill(out_grad_omega, out_grad_omega + h->n_mat, 0.0);
-            if (out_grad_w) std::fill(out_grad_w, out_grad_w + h->n_comp, 0.0);
-            if (out_grad_theta) std::fill(out_grad_theta, out_grad_theta + 5, 0.0);
-        }
-        return HY_OK;
+}  // extern "C"
+
+namespace {
+
+// Zero outputs of an evaluation over an empty alignment.
+void zero_outputs(hy_engine h, unsigned flags, double *out_logL, double *out_grad_bl, double *out_grad_omega,
+                  double *out_grad_w, double *out_grad_theta) {
+    if (out_logL) *out_logL = 0.0;
+    if (flags & HY_WANT_GRAD) {
+        if (out_grad_bl) std::fill(out_grad_bl, out_grad_bl + h->num_nodes, 0.0);
+        if (out_grad_omega) std::fill(out_grad_omega, out_grad_omega + h->n_mat, 0.0);
+        if (out_grad_w) std::fill(out_grad_w, out_grad_w + h->n_comp, 0.0);
+        if (out_grad_theta) std::fill(out_grad_theta, out_grad_theta + 5, 0.0);
     }
+}
+
+int check_eval_args(hy_engine h, int precision, const double *bl, const double *omegas, const double *comp_w,
+                    const double *theta, const char *who) {
+    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "%s before hy_set_problem/hy_set_model", who);
+    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
+        return fail(h, HY_ERR_ARG, "%s: unknown precision %d", who, precision);
+    if (!bl || !omegas || !comp_w || !theta) return fail(h, HY_ERR_ARG, "%s: NULL parameter array", who);
+    return HY_OK;
+}
+
+// Packs the parameters and queues one evaluation (captured graph, or eager
+// launches) and its result downloads on the handle's stream; the per-pattern
+// lnL goes to pinned staging when want_site.  eval_complete() waits for it.
+int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
+                const double *comp_w, const double *theta, double delta, double psi, bool want_site) {
     HIP_TRY(h, hipSetDevice(h->device));
     TRY(prepare_buffers(h, precision));
     double *hp = h->h_params;
@@ -1153,17 +1245,207 @@ int hy_eval(hy_engine h, int precision, unsigned flags, const double *bl, double
     hp[pl.off_delta] = delta;
     hp[pl.off_psi] = psi;
     const bool grad = (flags & HY_WANT_GRAD) != 0;
-    if (h->use_graphs && !h->profile) {
-        TRY(launch_graph(h, precision, grad));
-        if (out_site_ll)
-            HIP_TRY(h, hipMemcpyAsync(out_site_ll, h->d_site.p, sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(h, hipStreamSynchronize(h->stream));
-        return finish_host(h, true, out_logL, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta, grad);
+    if (want_site && h->h_site_cap < (size_t)h->P) {
+        if (h->h_site) (void)hipHostFree(h->h_site);
+        h->h_site = nullptr;
+        h->h_site_cap = 0;
+        HIP_TRY(h, hipHostMalloc((void **)&h->h_site, sizeof(double) * h->P, hipHostMallocDefault));
+        h->h_site_cap = h->P;
     }
-    HIP_TRY(h, hipMemcpyAsync(h->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, h->stream));
-    const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
-    if (rc) return rc;
-    return finish_eval(h, true, out_logL, out_site_ll, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta, grad);
+    if (h->use_graphs && !h->profile && !h->wover) {
+        TRY(launch_graph(h, precision, grad));
+    } else {
+        HIP_TRY(h, hipMemcpyAsync(h->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, h->stream));
+        const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
+        if (rc) return rc;
+        TRY(download_results(h, true));
+    }
+    if (want_site)
+        HIP_TRY(h, hipMemcpyAsync(h->h_site, h->d_site.p, sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
+    return HY_OK;
+}
+
+int eval_complete(hy_engine h, bool grad, double *out_logL, double *out_site_ll, double *out_grad_bl,
+                  double *out_grad_omega, double *out_grad_w, double *out_grad_theta) {
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (out_site_ll) std::memcpy(out_site_ll, h->h_site, sizeof(double) * h->P);
+    return finish_host(h, true, out_logL, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta, grad);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hy_eval(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
+            const double *comp_w, const double *theta, double delta, double psi, double *out_logL,
+            double *out_site_ll, double *out_grad_bl, double *out_grad_omega, double *out_grad_w,
+            double *out_grad_theta) {
+    if (!h) return HY_ERR_ARG;
+    TRY(check_eval_args(h, precision, bl, omegas, comp_w, theta, "hy_eval"));
+    if (h->P == 0) {
+        zero_outputs(h, flags, out_logL, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta);
+        return HY_OK;
+    }
+    TRY(eval_submit(h, precision, flags, bl, alpha, omegas, comp_w, theta, delta, psi, out_site_ll != nullptr));
+    return eval_complete(h, (flags & HY_WANT_GRAD) != 0, out_logL, out_site_ll, out_grad_bl, out_grad_omega,
+                         out_grad_w, out_grad_theta);
+}
+
+int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
+                const double *comp_w, const double *theta, double delta, double psi, int n_syn,
+                const double *syn_rates, const double *syn_weights, double *out_logL, double *out_site_ll,
+                double *out_grad_bl, double *out_grad_omega, double *out_grad_w, double *out_grad_theta,
+                double *out_grad_syn_rates, double *out_grad_syn_weights) {
+    // BUSTEDAnalyzer::evaluate_log_l / compute_busted_branch_gradients with M
+    // synonymous-rate classes (busted.hpp:554-581, 607-649, 1191-1216,
+    // 1335-1396): class m scales every branch time by syn_rates[m]; per
+    // pattern lnL_p = log sum_m v_m L_m(p) (log-sum-exp, classes with
+    // v_m <= 1e-12 skipped); gradients are posterior-weighted sums of the
+    // class gradients, computed here as one evaluation per class whose
+    // pattern weights are w_p * post_m(p).
+    if (!h) return HY_ERR_ARG;
+    TRY(check_eval_args(h, precision, bl, omegas, comp_w, theta, "hy_eval_srv"));
+    if (n_syn < 1 || !syn_rates || !syn_weights) return fail(h, HY_ERR_ARG, "hy_eval_srv: need n_syn >= 1 rates and weights");
+    for (int m = 0; m < n_syn; ++m)
+        if (!(syn_rates[m] >= 0.0) || !std::isfinite(syn_rates[m]) || !(syn_weights[m] >= 0.0) || !std::isfinite(syn_weights[m]))
+            return fail(h, HY_ERR_ARG, "hy_eval_srv: syn_rates[%d] / syn_weights[%d] must be finite and >= 0", m, m);
+    const bool grad = (flags & HY_WANT_GRAD) != 0;
+    if (grad) {
+        if (out_grad_syn_rates) std::fill(out_grad_syn_rates, out_grad_syn_rates + n_syn, 0.0);
+        if (out_grad_syn_weights) std::fill(out_grad_syn_weights, out_grad_syn_weights + n_syn, 0.0);
+    }
+    if (h->P == 0) {
+        zero_outputs(h, flags, out_logL, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta);
+        return HY_OK;
+    }
+    const int P = h->P, N = h->num_nodes, M = n_syn;
+    std::vector<double> blm(N), sll((size_t)M * P, -1e20);
+    std::vector<char> live(M);
+    for (int m = 0; m < M; ++m) {
+        live[m] = syn_weights[m] > 1e-12;
+        if (!live[m]) continue;
+        for (int n = 0; n < N; ++n) blm[n] = syn_rates[m] * bl[n];
+        TRY(eval_submit(h, precision, HY_WANT_LOGL, blm.data(), alpha, omegas, comp_w, theta, delta, psi, true));
+        TRY(eval_complete(h, false, nullptr, sll.data() + (size_t)m * P, nullptr, nullptr, nullptr, nullptr));
+    }
+    // Per-pattern log-sum-exp (busted.hpp:621-646) and class posteriors.
+    std::vector<double> pll(P, -1e20), post((size_t)M * P, 0.0);
+    for (int p = 0; p < P; ++p) {
+        double mx = -1e20;
+        for (int m = 0; m < M; ++m)
+            if (live[m] && sll[(size_t)m * P + p] > -1e19) mx = std::max(mx, std::log(syn_weights[m]) + sll[(size_t)m * P + p]);
+        if (mx <= -1e19) continue;
+        double se = 0.0;
+        for (int m = 0; m < M; ++m)
+            if (live[m] && sll[(size_t)m * P + p] > -1e19) {
+                const double e = std::exp(std::log(syn_weights[m]) + sll[(size_t)m * P + p] - mx);
+                post[(size_t)m * P + p] = e;
+                se += e;
+            }
+        if (se > 0.0) {
+            pll[p] = mx + std::log(se);
+            for (int m = 0; m < M; ++m) post[(size_t)m * P + p] /= se;
+        } else {
+            for (int m = 0; m < M; ++m) post[(size_t)m * P + p] = 0.0;
+        }
+    }
+    {
+        double s = 0.0, c = 0.0;  // Neumaier in pattern order (busted.hpp:652-656)
+        for (int p = 0; p < P; ++p) {
+            const double x = h->weights[p] * pll[p], t = s + x;
+            c += (std::fabs(s) >= std::fabs(x)) ? (s - t) + x : (x - t) + s;
+            s = t;
+        }
+        if (out_logL) *out_logL = s + c;
+    }
+    if (out_site_ll) std::memcpy(out_site_ll, pll.data(), sizeof(double) * P);
+    if (!grad) return HY_OK;
+    std::vector<double> gbl(N, 0.0), gom(h->n_mat, 0.0), gw(h->n_comp, 0.0), gth(5, 0.0);
+    std::vector<double> tb(N), to(h->n_mat), tw(h->n_comp), tt(5), wp((size_t)h->P_pad, 0.0);
+    int rc = HY_OK;
+    for (int m = 0; m < M && rc == HY_OK; ++m) {
+        if (!live[m]) continue;
+        // posterior-weighted pattern weights; terms with post <= 1e-12 are
+        // dropped as in busted.hpp:1383-1386
+        double sw = 0.0;
+        for (int p = 0; p < P; ++p) {
+            const double q = post[(size_t)m * P + p];
+            wp[p] = q > 1e-12 ? h->weights[p] * q : 0.0;
+            sw += h->weights[p] * q;
+        }
+        if (out_grad_syn_weights) out_grad_syn_weights[m] = sw / syn_weights[m];
+        rc = prepare_buffers(h, precision);
+        if (rc == HY_OK) rc = ensure(h, h->d_wover, (size_t)h->P_pad * 8);
+        if (rc == HY_OK) rc = upload(h, h->d_wover, wp.data(), (size_t)h->P_pad * 8);
+        if (rc != HY_OK) break;
+        for (int n = 0; n < N; ++n) blm[n] = syn_rates[m] * bl[n];
+        h->wover = (const double *)h->d_wover.p;
+        rc = eval_submit(h, precision, HY_WANT_LOGL | HY_WANT_GRAD, blm.data(), alpha, omegas, comp_w, theta, delta,
+                         psi, false);
+        if (rc == HY_OK) rc = eval_complete(h, true, nullptr, nullptr, tb.data(), to.data(), tw.data(), tt.data());
+        h->wover = nullptr;
+        if (rc != HY_OK) break;
+        double gr = 0.0;
+        for (int n = 0; n < N; ++n) {
+            gbl[n] += syn_rates[m] * tb[n];
+            gr += bl[n] * tb[n];
+        }
+        if (out_grad_syn_rates) out_grad_syn_rates[m] = gr;
+        for (int k = 0; k < h->n_mat; ++k) gom[k] += to[k];
+        for (int j = 0; j < h->n_comp; ++j) gw[j] += tw[j];
+        for (int k = 0; k < 5; ++k) gth[k] += tt[k];
+    }
+    if (rc != HY_OK) return rc;
+    if (out_grad_bl) std::memcpy(out_grad_bl, gbl.data(), sizeof(double) * N);
+    if (out_grad_omega) std::memcpy(out_grad_omega, gom.data(), sizeof(double) * h->n_mat);
+    if (out_grad_w) std::memcpy(out_grad_w, gw.data(), sizeof(double) * h->n_comp);
+    if (out_grad_theta) std::memcpy(out_grad_theta, gth.data(), sizeof(double) * 5);
+    return HY_OK;
+}
+
+int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned flags) {
+    // Every item's evaluation is queued on its own handle's stream before any
+    // is waited for, so independent genes overlap on the device.
+    if (n_items < 0 || (n_items > 0 && !items)) return HY_ERR_ARG;
+    {
+        std::vector<hy_engine> hs;
+        for (int i = 0; i < n_items; ++i)
+            if (items[i].h) hs.push_back(items[i].h);
+        std::sort(hs.begin(), hs.end());
+        if (std::adjacent_find(hs.begin(), hs.end()) != hs.end()) {
+            for (int i = 0; i < n_items; ++i) items[i].status = HY_ERR_ARG;
+            return HY_ERR_ARG;  // a handle holds one pending evaluation at a time
+        }
+    }
+    int first = HY_OK;
+    std::vector<char> queued(n_items, 0);
+    for (int i = 0; i < n_items; ++i) {
+        hy_batch_item &it = items[i];
+        hy_engine h = it.h;
+        it.status = HY_OK;
+        if (!h) {
+            it.status = HY_ERR_ARG;
+        } else if ((it.status = check_eval_args(h, precision, it.bl, it.omegas, it.comp_w, it.theta, "hy_eval_batch")) ==
+                   HY_OK) {
+            if (h->P == 0) {
+                zero_outputs(h, flags, it.out_logL, it.out_grad_bl, it.out_grad_omega, it.out_grad_w, it.out_grad_theta);
+            } else {
+                it.status = eval_submit(h, precision, flags, it.bl, it.alpha, it.omegas, it.comp_w, it.theta, it.delta,
+                                        it.psi, it.out_site_ll != nullptr);
+                queued[i] = it.status == HY_OK;
+            }
+        }
+        if (it.status != HY_OK && first == HY_OK) first = it.status;
+    }
+    for (int i = 0; i < n_items; ++i) {
+        if (!queued[i]) continue;
+        hy_batch_item &it = items[i];
+        it.status = eval_complete(it.h, (flags & HY_WANT_GRAD) != 0, it.out_logL, it.out_site_ll, it.out_grad_bl,
+                                  it.out_grad_omega, it.out_grad_w, it.out_grad_theta);
+        if (it.status != HY_OK && first == HY_OK) first = it.status;
+    }
+    return first;
 }
 
 int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,

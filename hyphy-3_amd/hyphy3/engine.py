@@ -26,7 +26,7 @@ HY_WANT_GRAD = 4
 
 EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
-           "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile")
+           "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_eval_srv", "hy_eval_batch", "hy_create_group")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -34,6 +34,14 @@ _u64p = C.POINTER(C.c_uint64)
 _f64p = C.POINTER(C.c_double)
 
 _lib = None
+
+
+class BatchItem(C.Structure):
+    """hy_batch_item of the C ABI."""
+    _fields_ = [("h", C.c_void_p), ("bl", _f64p), ("alpha", C.c_double), ("omegas", _f64p), ("comp_w", _f64p),
+                ("theta", _f64p), ("delta", C.c_double), ("psi", C.c_double), ("out_logL", _f64p),
+                ("out_site_ll", _f64p), ("out_grad_bl", _f64p), ("out_grad_omega", _f64p), ("out_grad_w", _f64p),
+                ("out_grad_theta", _f64p), ("status", C.c_int)]
 
 
 class HyError(RuntimeError):
@@ -68,6 +76,11 @@ def load_library(path: str = LIB_PATH):
     L.hy_set_warm_start.argtypes = [C.c_void_p, C.c_int]
     L.hy_last_eigen_status.argtypes = [C.c_void_p, _i32p, C.c_int]
     L.hy_last_tile.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.hy_eval_srv.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
+                              C.c_double, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                              _f64p]
+    L.hy_eval_batch.argtypes = [C.c_int, C.POINTER(BatchItem), C.c_int, C.c_uint]
+    L.hy_create_group.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     _lib = L
     return L
 
@@ -83,12 +96,15 @@ def _f64(a):
 class Engine:
     """One MI355X likelihood engine (one device, one stream)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, handle=None):
         self.L = load_library()
-        h = C.c_void_p()
-        rc = self.L.hy_create(device, C.byref(h))
-        if rc != HY_OK:
-            raise HyError(rc, self.L.hy_last_error(None).decode())
+        if handle is None:
+            h = C.c_void_p()
+            rc = self.L.hy_create(device, C.byref(h))
+            if rc != HY_OK:
+                raise HyError(rc, self.L.hy_last_error(None).decode())
+        else:
+            h = C.c_void_p(handle)
         self.h = h
         self.num_nodes = 0
         self.num_patterns = 0
@@ -176,6 +192,34 @@ class Engine:
             out.update(grad_bl=gbl, grad_omega=gom, grad_w=gw, grad_theta=gth)
         return out
 
+    def eval_srv(self, bl, omegas, comp_w, theta, syn_rates, syn_weights, alpha=1.0, delta=0.0, psi=0.0,
+                 precision=HY_PREC_FP64, grad=True, site=False):
+        """Synonymous-rate mixture (M = len(syn_rates) classes) over the
+        branch-site mixture; adds grad_syn_rates / grad_syn_weights."""
+        bl = _f64(bl)
+        om = _f64(np.atleast_1d(omegas))
+        cw = _f64(np.atleast_1d(comp_w))
+        th = _f64(theta)
+        sr = _f64(np.atleast_1d(syn_rates))
+        sw = _f64(np.atleast_1d(syn_weights))
+        M = len(sr)
+        if len(sw) != M:
+            raise ValueError("syn_rates and syn_weights differ in length")
+        ll = C.c_double()
+        s = np.zeros(max(self.num_patterns, 1)) if site else None
+        g = [np.zeros(k) if grad else None for k in (self.num_nodes, self.n_mat, self.n_comp, 5, M, M)]
+        flags = HY_WANT_LOGL | (HY_WANT_SITE_LL if site else 0) | (HY_WANT_GRAD if grad else 0)
+        self._check(self.L.hy_eval_srv(self.h, precision, flags, _p(bl, _f64p), alpha, _p(om, _f64p), _p(cw, _f64p),
+                                       _p(th, _f64p), delta, psi, M, _p(sr, _f64p), _p(sw, _f64p), C.byref(ll),
+                                       _p(s, _f64p), *[_p(x, _f64p) for x in g]))
+        out = {"log_likelihood": ll.value}
+        if site:
+            out["site_ll"] = s[:self.num_patterns]
+        if grad:
+            out.update(grad_bl=g[0], grad_omega=g[1], grad_w=g[2], grad_theta=g[3], grad_syn_rates=g[4],
+                       grad_syn_weights=g[5])
+        return out
+
     def loglik_from_P(self, P, pi, precision=HY_PREC_FP64, site=False):
         P = _f64(P)
         pi = _f64(pi)
@@ -207,3 +251,53 @@ class Engine:
         v = C.c_size_t()
         self._check(self.L.hy_device_bytes(self.h, C.byref(v)))
         return v.value
+
+
+def create_group(n, device=0, n_stream_sets=8):
+    """n engines sharing a few stream sets (hy_create_group), for batches of
+    many small problems evaluated with eval_batch."""
+    L = load_library()
+    hs = (C.c_void_p * n)()
+    rc = L.hy_create_group(device, n, n_stream_sets, hs)
+    if rc != HY_OK:
+        raise HyError(rc, L.hy_last_error(None).decode())
+    return [Engine(device, handle=hs[i]) for i in range(n)]
+
+
+def eval_batch(engines, params, precision=HY_PREC_FP64, grad=True, site=False):
+    """Evaluates many engines (one gene each) in one hy_eval_batch call.
+    params[i] = (bl, omegas, comp_w, theta[, alpha]); returns a list of result
+    dicts shaped like Engine.eval's."""
+    L = load_library()
+    n = len(engines)
+    items = (BatchItem * max(n, 1))()
+    keep, outs = [], []
+    flags = HY_WANT_LOGL | (HY_WANT_SITE_LL if site else 0) | (HY_WANT_GRAD if grad else 0)
+    for i, (e, prm) in enumerate(zip(engines, params)):
+        bl, om, cw, th = (_f64(np.atleast_1d(x)) for x in prm[:4])
+        alpha = float(prm[4]) if len(prm) > 4 else 1.0
+        ll = np.zeros(1)
+        s = np.zeros(max(e.num_patterns, 1)) if site else None
+        g = [np.zeros(k) if grad else None for k in (e.num_nodes, e.n_mat, e.n_comp, 5)]
+        keep.append((bl, om, cw, th))
+        outs.append((ll, s, g))
+        it = items[i]
+        it.h = e.h
+        it.bl, it.omegas, it.comp_w, it.theta = (_p(x, _f64p) for x in (bl, om, cw, th))
+        it.alpha, it.delta, it.psi = alpha, 0.0, 0.0
+        it.out_logL = _p(ll, _f64p)
+        it.out_site_ll = _p(s, _f64p)
+        it.out_grad_bl, it.out_grad_omega, it.out_grad_w, it.out_grad_theta = (_p(x, _f64p) for x in g)
+    rc = L.hy_eval_batch(n, items, precision, flags)
+    if rc != HY_OK:
+        bad = next(i for i in range(n) if items[i].status != HY_OK)
+        raise HyError(rc, f"item {bad}: " + L.hy_last_error(engines[bad].h).decode())
+    res = []
+    for e, (ll, s, g) in zip(engines, outs):
+        r = {"log_likelihood": float(ll[0])}
+        if site:
+            r["site_ll"] = s[:e.num_patterns]
+        if grad:
+            r.update(grad_bl=g[0], grad_omega=g[1], grad_w=g[2], grad_theta=g[3])
+        res.append(r)
+    return res

@@ -24,6 +24,9 @@
  *                    BUSTEDAnalyzer::evaluate_log_l (busted.hpp:509-659) and
  *                    compute_busted_branch_gradients (busted.hpp:1138-1429)
  *                    with M = 1 synonymous class
+ *   hy_eval_srv      the same two BUSTED entries with M > 1 synonymous-rate
+ *                    classes (busted.hpp:554-581, 607-649, 1335-1396)
+ *   hy_eval_batch    a batch of independent problems (genes), one per handle
  *   hy_loglik_from_P compute_tree_log_likelihood_from_P (likelihood.hpp:766-806)
  */
 #ifndef HYPHY_MI355X_H
@@ -62,6 +65,13 @@ typedef struct hy_engine_s *hy_engine;
 /* Create an engine on HIP device `device`. */
 int hy_create(int device, hy_engine *out);
 void hy_destroy(hy_engine h);
+/* n engines on one device that share min(n, n_stream_sets) stream sets
+ * (n_stream_sets <= 0: 8) round-robin, for batches of many small problems
+ * (one engine per gene, evaluated together by hy_eval_batch).  Engines of a
+ * group must be driven from one host thread at a time and launch their
+ * kernels directly (no captured graphs).  Each is released with hy_destroy.
+ * No counterpart in the reference. */
+int hy_create_group(int device, int n, int n_stream_sets, hy_engine *out);
 /* Message of the last failing call on this handle (or of a failed hy_create
  * when h is NULL).  Valid until the next call on the handle. */
 const char *hy_last_error(hy_engine h);
@@ -115,6 +125,45 @@ int hy_eval(hy_engine h, int precision, unsigned flags, const double *bl, double
             const double *comp_w, const double *theta, double delta, double psi, double *out_logL,
             double *out_site_ll, double *out_grad_bl, double *out_grad_omega, double *out_grad_w,
             double *out_grad_theta);
+
+/* Synonymous-rate (SRV) mixture on top of the branch-site mixture, as
+ * BUSTEDAnalyzer::evaluate_log_l (busted.hpp:509-659) and
+ * compute_busted_branch_gradients (busted.hpp:1138-1429) with M = n_syn:
+ * class m multiplies every branch time by syn_rates[m] and has weight
+ * syn_weights[m]; lnL = sum_p w_p log sum_m v_m L_m(p) (log-sum-exp; classes
+ * with v_m <= 1e-12 are skipped, a pattern with no positive term gets
+ * -1e20).  Gradients are posterior-weighted over the classes and extend the
+ * reference's branch gradients to omega, mixture weights, theta and
+ * out_grad_syn_rates[n_syn] / out_grad_syn_weights[n_syn] (d lnL / d v_m,
+ * 0 for skipped classes).  out_site_ll receives the combined per-pattern lnL.
+ * Costs about 2 * n_syn evaluations (a forward pass per class for the
+ * posteriors, then a gradient pass per class). */
+int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
+                const double *comp_w, const double *theta, double delta, double psi, int n_syn,
+                const double *syn_rates, const double *syn_weights, double *out_logL, double *out_site_ll,
+                double *out_grad_bl, double *out_grad_omega, double *out_grad_w, double *out_grad_theta,
+                double *out_grad_syn_rates, double *out_grad_syn_weights);
+
+/* Many independent problems (genes) in one call: item i evaluates handle
+ * items[i].h exactly as hy_eval would with the item's parameters and
+ * outputs.  All evaluations are queued (each on its handle's stream) before
+ * any is waited for, so the genes run concurrently on the device.  Handles
+ * must be distinct.  Returns HY_OK or the first failing item's code; every
+ * item's own code is left in items[i].status.  Replaces the per-gene loops
+ * of the reference's drivers over LikelihoodEngine calls (one problem per
+ * call, likelihood.hpp:257-292, 1228-1236). */
+typedef struct hy_batch_item {
+    hy_engine h;
+    const double *bl;
+    double alpha;
+    const double *omegas;
+    const double *comp_w;
+    const double *theta;
+    double delta, psi;
+    double *out_logL, *out_site_ll, *out_grad_bl, *out_grad_omega, *out_grad_w, *out_grad_theta;
+    int status;
+} hy_batch_item;
+int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned flags);
 
 /* lnL from caller-supplied transition matrices P[num_nodes*S*S] (row-major,
  * entry of the root ignored) and root frequencies pi[S]. */

@@ -1132,6 +1132,126 @@ done:
     return rc;
 }
 
+/* BUSTED synonymous-rate classes, BUSTEDAnalyzer::evaluate_log_l
+ * (busted.hpp:554-581 P_mix[m] at syn_rates[m] * tau, 607-649 per-pattern
+ * log-sum-exp with log syn_weights, classes with weight <= 1e-12 skipped,
+ * 652-656 Neumaier over w_p * pll_p) and compute_busted_branch_gradients
+ * (busted.hpp:1191-1216, 1335-1396: per-pattern class posteriors post_m(p),
+ * gradient terms w_p * post_m(p) * dL_m/L_m, post <= 1e-12 dropped).  The
+ * class-m terms form a branch-site mixture evaluation whose pattern weights are
+ * w_p * post_m(p), so each class is one or_mixture_eval call at branch lengths
+ * syn_rates[m] * bl.  Beyond the reference's branch gradients this also
+ * returns omega / weight / theta gradients (summed over classes) and
+ * d lnL / d syn_rates[m] = sum_b bl_b dlnL_m/d(bl_m,b),
+ * d lnL / d syn_weights[m] = sum_p w_p post_m(p) / v_m. */
+int or_srv_eval(const or_tree *tr, const or_code *code, const or_mg94_params *base, const double *pos,
+                const double *pi, const or_mixture *mix, const double *bl, int32_t n_syn, const double *syn_rates,
+                const double *syn_weights, int32_t want_grad, int32_t threads, or_mixture_out *out,
+                double *grad_syn_rates, double *grad_syn_weights) {
+    const int N = tr->num_nodes, np = tr->num_patterns, M = n_syn, NM = mix->n_mat;
+    const int ncomp = mix->comp_ptr[mix->n_class];
+    if (M < 1) return -1;
+    double *blm = (double *)malloc(sizeof(double) * N);
+    double *sll = (double *)malloc(sizeof(double) * (size_t)M * (np > 0 ? np : 1));
+    double *post = (double *)calloc((size_t)M * (np > 0 ? np : 1), sizeof(double));
+    double *pll = (double *)malloc(sizeof(double) * (np > 0 ? np : 1));
+    double *wp = (double *)malloc(sizeof(double) * (np > 0 ? np : 1));
+    int rc = 0;
+    for (size_t i = 0; i < (size_t)M * np; ++i) sll[i] = -1e20;
+    for (int m = 0; m < M && rc == 0; ++m) {
+        if (!(syn_weights[m] > 1e-12)) continue;
+        for (int n = 0; n < N; ++n) blm[n] = syn_rates[m] * bl[n];
+        or_mixture_out o = {0};
+        o.site_ll = sll + (size_t)m * np;
+        rc = or_mixture_eval(tr, code, base, pos, pi, mix, blm, 0, 0, threads, &o);
+    }
+    if (rc == 0) {
+        for (int p = 0; p < np; ++p) {
+            double mx = -1e20;
+            pll[p] = -1e20;
+            for (int m = 0; m < M; ++m) {
+                const double x = sll[(size_t)m * np + p];
+                if (syn_weights[m] > 1e-12 && x > -1e19 && log(syn_weights[m]) + x > mx) mx = log(syn_weights[m]) + x;
+            }
+            if (mx <= -1e19) continue;
+            double se = 0.0;
+            for (int m = 0; m < M; ++m) {
+                const double x = sll[(size_t)m * np + p];
+                if (syn_weights[m] > 1e-12 && x > -1e19) {
+                    post[(size_t)m * np + p] = exp(log(syn_weights[m]) + x - mx);
+                    se += post[(size_t)m * np + p];
+                }
+            }
+            if (se > 0.0) {
+                pll[p] = mx + log(se);
+                for (int m = 0; m < M; ++m) post[(size_t)m * np + p] /= se;
+            } else {
+                for (int m = 0; m < M; ++m) post[(size_t)m * np + p] = 0.0;
+            }
+        }
+        for (int p = 0; p < np; ++p) wp[p] = tr->weights[p] * pll[p];
+        out->log_likelihood = or_neumaier_sum(wp, np);
+        if (out->site_ll)
+            for (int p = 0; p < np; ++p) out->site_ll[p] = pll[p];
+    }
+    if (rc == 0 && want_grad) {
+        double *gbl = (double *)calloc((size_t)N, sizeof(double)), *tb = (double *)calloc((size_t)N, sizeof(double));
+        double *gom = (double *)calloc((size_t)NM, sizeof(double)), *to = (double *)calloc((size_t)NM, sizeof(double));
+        double *gw = (double *)calloc((size_t)(ncomp > 0 ? ncomp : 1), sizeof(double));
+        double *tw = (double *)calloc((size_t)(ncomp > 0 ? ncomp : 1), sizeof(double));
+        double gth[5] = {0, 0, 0, 0, 0}, tt[5];
+        for (int m = 0; m < M && rc == 0; ++m) {
+            if (grad_syn_rates) grad_syn_rates[m] = 0.0;
+            if (grad_syn_weights) grad_syn_weights[m] = 0.0;
+            if (!(syn_weights[m] > 1e-12)) continue;
+            double sw = 0.0;
+            for (int p = 0; p < np; ++p) {
+                const double q = post[(size_t)m * np + p];
+                wp[p] = q > 1e-12 ? tr->weights[p] * q : 0.0;
+                sw += tr->weights[p] * q;
+            }
+            if (grad_syn_weights) grad_syn_weights[m] = sw / syn_weights[m];
+            or_tree trm = *tr;
+            trm.weights = wp;
+            for (int n = 0; n < N; ++n) blm[n] = syn_rates[m] * bl[n];
+            or_mixture_out o = {0};
+            o.grad_bl = tb;
+            o.grad_omega = to;
+            o.grad_w = tw;
+            o.grad_theta = tt;
+            rc = or_mixture_eval(&trm, code, base, pos, pi, mix, blm, 1, 0, threads, &o);
+            if (rc != 0) break;
+            double gr = 0.0;
+            for (int n = 0; n < N; ++n) {
+                gbl[n] += syn_rates[m] * tb[n];
+                gr += bl[n] * tb[n];
+            }
+            if (grad_syn_rates) grad_syn_rates[m] = gr;
+            for (int k = 0; k < NM; ++k) gom[k] += to[k];
+            for (int j = 0; j < ncomp; ++j) gw[j] += tw[j];
+            for (int k = 0; k < 5; ++k) gth[k] += tt[k];
+        }
+        if (rc == 0) {
+            if (out->grad_bl) memcpy(out->grad_bl, gbl, sizeof(double) * N);
+            if (out->grad_omega) memcpy(out->grad_omega, gom, sizeof(double) * NM);
+            if (out->grad_w) memcpy(out->grad_w, gw, sizeof(double) * ncomp);
+            if (out->grad_theta) memcpy(out->grad_theta, gth, sizeof(gth));
+        }
+        free(gbl);
+        free(tb);
+        free(gom);
+        free(to);
+        free(gw);
+        free(tw);
+    }
+    free(blm);
+    free(sll);
+    free(post);
+    free(pll);
+    free(wp);
+    return rc;
+}
+
 /* compute_joint_frechet_gradients (likelihood.hpp:1228-1594): one component
  * per class, 3/mu time convention, optional test/background partition. */
 double or_joint_frechet(const or_tree *tr, const or_code *code, const or_mg94_params *p, const double *pos,

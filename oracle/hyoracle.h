@@ -131,6 +131,12 @@ int or_mixture_eval(const or_tree *tr, const or_code *code, const or_mg94_params
                     const double *bl, int32_t want_grad, int32_t tile, int32_t threads,
                     or_mixture_out *out);
 
+/* BUSTED synonymous-rate classes over the mixture (busted.hpp:509-659,
+ * 1138-1429); grad_syn_rates / grad_syn_weights [n_syn] may be NULL. */
+int or_srv_eval(const or_tree *tr, const or_code *code, const or_mg94_params *base,
+                const double *pos_freqs, const double *pi, const or_mixture *mix, const double *bl,
+                int32_t n_syn, const double *syn_rates, const double *syn_weights, int32_t want_grad,
+                int32_t threads, or_mixture_out *out, double *grad_syn_rates, double *grad_syn_weights);
 double or_neumaier_sum(const double *x, int64_t n);
 
 #ifdef __cplusplus

@@ -82,6 +82,10 @@ def lib():
                                       C.POINTER(OrMixture), _f64p, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(OrMixtureOut)]
         L.or_mixture_eval.restype = C.c_int32
+        L.or_srv_eval.argtypes = [C.POINTER(OrTree), C.POINTER(OrCode), C.POINTER(OrParams), _f64p, _f64p,
+                                  C.POINTER(OrMixture), _f64p, C.c_int32, _f64p, _f64p, C.c_int32, C.c_int32,
+                                  C.POINTER(OrMixtureOut), _f64p, _f64p]
+        L.or_srv_eval.restype = C.c_int32
         L.or_neumaier_sum.argtypes = [_f64p, C.c_int64]
         L.or_neumaier_sum.restype = C.c_double
         _lib = L
@@ -323,3 +327,32 @@ def mixture_eval(prob, code, base, pos, pi, mix, bl, want_grad=True, tile=0, thr
         raise RuntimeError(f"or_mixture_eval failed ({rc})")
     return {"log_likelihood": out.log_likelihood, "site_ll": site[:prob.num_patterns], "grad_bl": gbl,
             "grad_omega": gom, "grad_w": gw[:mix.n_comp], "grad_theta": gth, "dtau": dtau}
+
+
+def srv_eval(prob, code, base, pos, pi, mix, bl, syn_rates, syn_weights, want_grad=True, threads=1):
+    """BUSTED synonymous-rate classes over the branch-site mixture
+    (busted.hpp:509-659, 1138-1429); see or_srv_eval."""
+    pos = f64(pos).reshape(12)
+    pi = f64(pi)
+    blc = f64(bl)
+    sr = f64(np.atleast_1d(syn_rates))
+    sw = f64(np.atleast_1d(syn_weights))
+    M = len(sr)
+    N = prob.num_nodes
+    site = np.zeros(max(prob.num_patterns, 1))
+    gbl, gom, gw, gth = np.zeros(N), np.zeros(len(mix.omegas)), np.zeros(max(mix.n_comp, 1)), np.zeros(5)
+    gsr, gsw = np.zeros(M), np.zeros(M)
+    out = OrMixtureOut()
+    out.site_ll = ptr(site, _f64p)
+    out.grad_bl = ptr(gbl, _f64p)
+    out.grad_omega = ptr(gom, _f64p)
+    out.grad_w = ptr(gw, _f64p)
+    out.grad_theta = ptr(gth, _f64p)
+    rc = lib().or_srv_eval(C.byref(prob.t), C.byref(code.c), C.byref(base), ptr(pos, _f64p), ptr(pi, _f64p),
+                           C.byref(mix.m), ptr(blc, _f64p), M, ptr(sr, _f64p), ptr(sw, _f64p),
+                           1 if want_grad else 0, threads, C.byref(out), ptr(gsr, _f64p), ptr(gsw, _f64p))
+    if rc != 0:
+        raise RuntimeError(f"or_srv_eval failed ({rc})")
+    return {"log_likelihood": out.log_likelihood, "site_ll": site[:prob.num_patterns], "grad_bl": gbl,
+            "grad_omega": gom, "grad_w": gw[:mix.n_comp], "grad_theta": gth, "grad_syn_rates": gsr,
+            "grad_syn_weights": gsw}

@@ -34,9 +34,9 @@ def tree_arrays(tree, aln):
     return parent, children, l2t
 
 
-def make_engine(tree, aln, code_id=0):
+def make_engine(tree, aln, code_id=0, eng=None):
     from hyphy3.engine import Engine
-    eng = Engine(0)
+    eng = eng or Engine(0)
     parent, children, l2t = tree_arrays(tree, aln)
     eng.set_problem(parent, children, l2t, aln.states, aln.pmasks, aln.weights, S=aln.code.S)
     eng.set_model(code_id, aln.pos_freqs, aln.cf3x4)
