@@ -8,6 +8,8 @@
 //   compute_branch_length_gradients     likelihood.hpp:1102-1211
 //   compute_joint_frechet_gradients     likelihood.hpp:1213-1594
 //   build_leaf_to_taxon_map / find_taxon_index / neumaier_sum  :34-178
+//   compute_inside_batch_in_place / compute_inside_outside_batch_in_place
+//       with their workspaces                                   :295-762
 // plus compute_mixture_gradients for branch-site omega mixtures (BUSTED's
 // P_mix, include/hyphy/analyses/busted.hpp:509-659, 1138-1429) with full
 // omega / weight / GTR gradients.
@@ -18,7 +20,9 @@
 
 #include <array>
 #include <cmath>
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -29,6 +33,24 @@
 #include "hyphy_mi355x.h"
 
 namespace hyphy::core {
+
+// Column-major dense matrices with the storage order of the reference's
+// Eigen::MatrixXd / MatrixXi workspace fields (element (i, j) at i + j * rows).
+template <typename T>
+struct DenseMatrix {
+    std::vector<T> v;
+    long r = 0, c = 0;
+    DenseMatrix() = default;
+    DenseMatrix(long rows, long cols, T fill = T(0)) : v(size_t(rows) * cols, fill), r(rows), c(cols) {}
+    long rows() const { return r; }
+    long cols() const { return c; }
+    T &operator()(long i, long j) { return v[size_t(i) + size_t(j) * r]; }
+    T operator()(long i, long j) const { return v[size_t(i) + size_t(j) * r]; }
+    T *col(long j) { return v.data() + size_t(j) * r; }
+    const T *col(long j) const { return v.data() + size_t(j) * r; }
+};
+using Matrix = DenseMatrix<Scalar>;
+using IntMatrix = DenseMatrix<int>;
 
 struct MG94Parameters {
     Scalar alpha = 1.0;
@@ -182,6 +204,7 @@ public:
                                               const std::vector<size_t> *leaf_to_taxon = nullptr,
                                               Scalar branch_scale = 1.0) {
         Handle &h = bind(tree, aln, leaf_to_taxon, mg.code ? mg.code->id : code_id(aln));
+        ++h.epoch;  // this evaluation replaces the device partials
         set_frame(h, aln, mg.pos, mg.pi);
         std::vector<double> bl(tree.num_nodes(), 0.0);
         for (const auto &n : tree.nodes) bl[n.id] = n.branch_length * branch_scale;
@@ -199,6 +222,7 @@ public:
                                                      const std::vector<Scalar> &pi, int S,
                                                      const std::vector<size_t> *leaf_to_taxon = nullptr) {
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
+        ++h.epoch;  // this evaluation replaces the device partials
         std::vector<double> flat(tree.num_nodes() * size_t(S) * S, 0.0);
         for (size_t n = 0; n < tree.num_nodes() && n < P_branches.size(); ++n)
             if (P_branches[n].size() == size_t(S) * S) std::copy(P_branches[n].begin(), P_branches[n].end(), flat.begin() + n * S * S);
@@ -211,6 +235,7 @@ public:
         const Tree &tree, const Alignment &aln, const MG94Parameters &params,
         const std::vector<Scalar> &branch_lengths = {}, const std::vector<size_t> *leaf_to_taxon = nullptr) {
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
+        ++h.epoch;  // this evaluation replaces the device partials
         set_frame(h, aln, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4);
         single_mixture(h, HY_TIME_RAW);
         const auto bl = lengths(tree, branch_lengths);
@@ -230,6 +255,7 @@ public:
                                                               const std::vector<bool> *is_test_branch = nullptr,
                                                               Scalar omega_bg = -1.0) {
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
+        ++h.epoch;  // this evaluation replaces the device partials
         set_frame(h, aln, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4);
         const size_t N = tree.num_nodes();
         const bool part = is_test_branch && is_test_branch->size() == N && omega_bg >= 0.0;
@@ -270,6 +296,7 @@ public:
         if (omegas.empty() || omegas.size() != weights.size())
             throw std::invalid_argument("compute_mixture_gradients: omegas and weights must be non-empty and equal length");
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
+        ++h.epoch;  // this evaluation replaces the device partials
         set_frame(h, aln, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4);
         const int K = int(omegas.size());
         std::vector<int32_t> cm(K), bc(tree.num_nodes(), 0);
@@ -293,7 +320,191 @@ public:
         return r;
     }
 
+    // ---- workspace entries (read back from the device) ----
+    // Field names and meaning of the reference's workspaces
+    // (likelihood.hpp:295-340): D/V/P_D/U[node] are S x B column-major,
+    // scaled by whole powers of 1e150 recorded in node_log_scale
+    // (-345.38776394910684 per boost) and boost_counts (the node's own
+    // boosts), so that V_v . P_D[v] = likelihood(b) for every branch and
+    // log_likelihood(b) = log(likelihood(b)) + node_log_scale(root, b).
+    // Boosts are assigned by the reference's end-of-node rule; a node whose
+    // partial child product already fell below 1e-150 may carry its boosts
+    // one level differently (same true values).
+    struct BatchedInsideWorkspace {
+        std::vector<Matrix> D;
+        Matrix node_log_scale;  // num_nodes x B
+        std::vector<Scalar> root_L, root_log_L;
+    };
+    struct BatchedInsideOutsideWorkspace {
+        std::vector<Matrix> D, V, P_D, U;
+        IntMatrix boost_counts;  // num_nodes x B
+        Matrix node_log_scale;   // num_nodes x B
+        std::vector<Scalar> likelihood, log_likelihood;
+    };
+
+    // compute_inside_batch_in_place (likelihood.hpp:343-476) for patterns
+    // [p_start, p_end): P_branches[node] row-major S x S (root ignored).
+    static void compute_inside_batch_in_place(const Tree &tree, const Alignment &aln, size_t p_start, size_t p_end,
+                                              const std::vector<size_t> &leaf_to_taxon,
+                                              const std::vector<std::vector<Scalar>> &P_branches,
+                                              const std::vector<Scalar> &pi, int S, BatchedInsideWorkspace &ws) {
+        Readback rb;
+        read_back(tree, aln, p_start, p_end, leaf_to_taxon, P_branches, pi, S, false, rb);
+        ws.D = std::move(rb.D);
+        ws.node_log_scale = std::move(rb.log_scale);
+        ws.root_L = std::move(rb.L);
+        ws.root_log_L = std::move(rb.logL);
+    }
+
+    // compute_inside_outside_batch_in_place (likelihood.hpp:479-762).
+    static void compute_inside_outside_batch_in_place(const Tree &tree, const Alignment &aln, size_t p_start,
+                                                      size_t p_end, const std::vector<size_t> &leaf_to_taxon,
+                                                      const std::vector<std::vector<Scalar>> &P_branches,
+                                                      const std::vector<Scalar> &pi, int S,
+                                                      BatchedInsideOutsideWorkspace &ws) {
+        Readback rb;
+        read_back(tree, aln, p_start, p_end, leaf_to_taxon, P_branches, pi, S, true, rb);
+        ws.D = std::move(rb.D);
+        ws.V = std::move(rb.V);
+        ws.P_D = std::move(rb.PD);
+        ws.U = std::move(rb.U);
+        ws.boost_counts = std::move(rb.boost);
+        ws.node_log_scale = std::move(rb.log_scale);
+        ws.likelihood = std::move(rb.L);
+        ws.log_likelihood = std::move(rb.logL);
+    }
+
 private:
+    struct Readback {
+        std::vector<Matrix> D, V, PD, U;
+        IntMatrix boost;
+        Matrix log_scale;
+        std::vector<Scalar> L, logL;
+    };
+
+    // One device evaluation per distinct (problem, P, pi, pass) serves every
+    // pattern range the caller's tile loop asks for.
+    static void read_back(const Tree &tree, const Alignment &aln, size_t p_start, size_t p_end,
+                          const std::vector<size_t> &leaf_to_taxon, const std::vector<std::vector<Scalar>> &P_branches,
+                          const std::vector<Scalar> &pi, int S, bool outside, Readback &rb) {
+        constexpr double LOG_SCALER = 345.38776394910684, LOG2_1E150 = 498.28921423310435;
+        if (tree.has_wheatstone_bridges())
+            throw std::invalid_argument("Wheatstone-bridged trees are not supported by the MI355X engine");
+        Handle &h = bind(tree, aln, &leaf_to_taxon, code_id(aln));
+        const size_t N = tree.num_nodes();
+        if (p_end <= p_start || p_end > aln.patterns.size())
+            throw std::out_of_range("compute_inside_*_batch_in_place: pattern range outside the alignment");
+        if (size_t(S) != size_t(h.S)) throw std::invalid_argument("compute_inside_*_batch_in_place: S mismatch");
+        std::vector<double> flat(N * size_t(S) * S, 0.0);
+        for (size_t n = 0; n < N && n < P_branches.size(); ++n)
+            if (P_branches[n].size() == size_t(S) * S) std::copy(P_branches[n].begin(), P_branches[n].end(), flat.begin() + n * S * S);
+        uint64_t key = 1469598103934665603ull ^ (outside ? 0x9e37ull : 0x51ull) ^ h.key;
+        auto mixd = [&](double x) {
+            uint64_t u;
+            std::memcpy(&u, &x, 8);
+            key = (key ^ u) * 1099511628211ull;
+        };
+        for (double x : flat) mixd(x);
+        for (double x : pi) mixd(x);
+        mixd(double(get_precision()));
+        if (key != h.part_key || h.part_epoch != h.epoch) {
+            double ll = 0.0;
+            check(h, outside ? hy_inside_outside_from_P(h.e, get_precision(), flat.data(), pi.data(), &ll, nullptr)
+                             : hy_loglik_from_P(h.e, get_precision(), flat.data(), pi.data(), &ll, nullptr));
+            h.part_key = key;
+            h.part_epoch = h.epoch;
+        }
+        const long B = long(p_end - p_start);
+        std::vector<double> Dm(N * B * S, 0.0), Vm(outside ? N * B * S : 0, 0.0);
+        std::vector<int32_t> Em(N * B, 0), Gm(outside ? N * B : 0, 0);
+        check(h, hy_read_partials(h.e, int(p_start), int(p_end), Dm.data(), Em.data(), outside ? Vm.data() : nullptr,
+                                  outside ? Gm.data() : nullptr, nullptr, nullptr));
+        // true D of internal nodes = Dm * 2^Em; leaves from the alignment
+        // (init_leaf_likelihood, likelihood.hpp:53-87)
+        rb.D.assign(N, Matrix(S, B));
+        rb.log_scale = Matrix(long(N), B);
+        rb.boost = IntMatrix(long(N), B);
+        std::vector<long> cum(N * B, 0);  // total boosts of each subtree
+        const uint64_t all = (S >= 64) ? ~uint64_t(0) : ((uint64_t(1) << S) - 1);
+        auto scaled = [&](double m, double log2f) { return std::ldexp(m * std::exp2(log2f - std::floor(log2f)), int(std::floor(log2f))); };
+        for (int nid : tree.post_order) {
+            const auto &node = tree.nodes[nid];
+            Matrix &Dn = rb.D[nid];
+            for (long b = 0; b < B; ++b) {
+                const size_t p = p_start + b;
+                if (node.is_leaf) {
+                    const size_t t = leaf_to_taxon[nid];
+                    double *c = Dn.col(b);
+                    const auto &pat = aln.patterns[p];
+                    const int st = t == size_t(-1) ? -1 : pat.states[t];
+                    const uint64_t mk = t == size_t(-1) ? 0 : pat.ambiguous_masks[t];
+                    for (int s2 = 0; s2 < S; ++s2)
+                        c[s2] = (t == size_t(-1)) ? 1.0
+                                : (st >= 0 && st < S) ? (s2 == st ? 1.0 : 0.0)
+                                : (mk != 0 && (mk & all) != all) ? ((mk >> s2) & 1 ? 1.0 : 0.0)
+                                                                 : 1.0;
+                    continue;
+                }
+                long csum = 0;
+                for (auto c : node.children) csum += cum[size_t(c) * B + b];
+                const double *m = Dm.data() + (size_t(nid) * B + b) * S;
+                double mx = 0.0;
+                for (int s2 = 0; s2 < S; ++s2) mx = std::max(mx, m[s2]);
+                const int e = Em[size_t(nid) * B + b];
+                long boost = 0;
+                if (mx > 0.0) {
+                    // reference rule: multiply by 1e150 while max < 1e-150
+                    const double l2 = std::log2(mx) + e + double(csum) * LOG2_1E150;
+                    while (l2 + double(boost) * LOG2_1E150 < -LOG2_1E150) ++boost;
+                }
+                cum[size_t(nid) * B + b] = csum + boost;
+                rb.boost(nid, b) = int(boost);
+                rb.log_scale(nid, b) = -double(csum + boost) * LOG_SCALER;
+                const double f = double(e) + double(csum + boost) * LOG2_1E150;
+                double *c = Dn.col(b);
+                for (int s2 = 0; s2 < S; ++s2) c[s2] = scaled(m[s2], f);
+            }
+        }
+        const int root = tree.root_id;
+        rb.L.assign(B, 0.0);
+        rb.logL.assign(B, -1e20);
+        for (long b = 0; b < B; ++b) {
+            double L = 0.0;
+            for (int s2 = 0; s2 < S; ++s2) L += rb.D[root](s2, b) * pi[s2];
+            rb.L[b] = L;
+            if (L > 0.0) rb.logL[b] = std::log(L) + rb.log_scale(root, b);
+        }
+        if (!outside) return;
+        // P_D = P D, V from the outside messages (V_v . P_D[v] = L), U = P^T V
+        rb.PD.assign(N, Matrix(S, B));
+        rb.V.assign(N, Matrix(S, B));
+        rb.U.assign(N, Matrix(S, B));
+        for (size_t n = 0; n < N; ++n) {
+            if (int(n) == root) {
+                for (long b = 0; b < B; ++b)
+                    for (int s2 = 0; s2 < S; ++s2) rb.U[n](s2, b) = pi[s2];
+                continue;
+            }
+            const double *P = flat.data() + n * S * S;
+            for (long b = 0; b < B; ++b) {
+                const double *d = rb.D[n].col(b);
+                double *pd = rb.PD[n].col(b);
+                for (int i = 0; i < S; ++i) {
+                    double acc = 0.0;
+                    for (int k = 0; k < S; ++k) acc += P[i * S + k] * d[k];
+                    pd[i] = acc;
+                }
+                const double f = double(Gm[n * B + b]) + double(cum[size_t(root) * B + b] - cum[n * B + b]) * LOG2_1E150;
+                const double *vm = Vm.data() + (n * B + b) * S;
+                double *vv = rb.V[n].col(b), *uu = rb.U[n].col(b);
+                for (int s2 = 0; s2 < S; ++s2) vv[s2] = scaled(vm[s2], f);
+                for (int j = 0; j < S; ++j) uu[j] = 0.0;
+                for (int i = 0; i < S; ++i)
+                    for (int j = 0; j < S; ++j) uu[j] += P[i * S + j] * vv[i];
+            }
+        }
+    }
+
     struct Handle {
         hy_engine e = nullptr;
         const void *tree_ptr = nullptr;
@@ -301,7 +512,8 @@ private:
         uint64_t key = 0;
         long mixture_key = -1;
         std::vector<double> pos, pi;
-        int code = -1;
+        int code = -1, S = 61;
+        uint64_t epoch = 0, part_key = 0, part_epoch = ~uint64_t(0);  // read-back cache
         ~Handle() {
             if (e) hy_destroy(e);
         }
@@ -373,7 +585,9 @@ private:
                                     int(aln.patterns.size()), st.data(), mk.data(), w.data(), S));
             h.key = key;
             h.code = code;
+            h.S = S;
             h.mixture_key = -1;
+            ++h.epoch;
         }
         return h;
     }

@@ -102,6 +102,10 @@ struct hy_engine_s {
     const double *wover = nullptr;
     double *h_site = nullptr;  // pinned staging of per-pattern lnL
     size_t h_site_cap = 0;
+    // Partials left by the last evaluation (read back by hy_read_partials):
+    // precision index (-1 none) and whether the outside pass ran.
+    int part_ri = -1;
+    bool part_outside = false;
     int T_req = 0, T = 0, n_tiles = 0, P_pad = 0;
 
     // mixture
@@ -470,10 +474,16 @@ void refresh_blist_classes(hy_engine h) {
     }
 }
 
+// Internal flag of run_eval: the outside pass without the adjoint stages
+// (workspace read-back of inside / outside partials).
+constexpr unsigned RUN_OUTSIDE = 1u << 30;
+
 template <typename R>
 int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     const int ri = sizeof(R) == 8 ? 1 : 0;
     const bool grad = (flags & HY_WANT_GRAD) != 0;
+    h->part_ri = ri;
+    h->part_outside = grad || (flags & RUN_OUTSIDE) != 0;
     hipStream_t s = h->stream;
     const bool prof = h->profile;
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[0], s));
@@ -630,8 +640,10 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         fa.proj = (double *)h->d_proj.p;
         if (h->n_branch > 0) launch_frechet(fa, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[6], s));
-    } else if (prof) {
-        for (int k = 4; k <= 6; ++k) HIP_TRY(h, hipEventRecord(h->ev[k], s));
+    } else {
+        if (flags & RUN_OUTSIDE) launch_outside<R>(pr, h->T, s);
+        if (prof)
+            for (int k = 4; k <= 6; ++k) HIP_TRY(h, hipEventRecord(h->ev[k], s));
     }
     FinalArgs fn{};
     fn.site_ll = (const double *)h->d_site.p;
@@ -1448,15 +1460,20 @@ int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned fla
     return first;
 }
 
-int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
-                     double *out_site_ll) {
+}  // extern "C"
+
+namespace {
+
+int eval_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
+                double *out_site_ll, unsigned flags, const char *who) {
     if (!h) return HY_ERR_ARG;
-    if (!h->have_problem) return fail(h, HY_ERR_STATE, "hy_loglik_from_P before hy_set_problem");
+    if (!h->have_problem) return fail(h, HY_ERR_STATE, "%s before hy_set_problem", who);
     if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
-        return fail(h, HY_ERR_ARG, "hy_loglik_from_P: unknown precision %d", precision);
-    if (!P || !pi) return fail(h, HY_ERR_ARG, "hy_loglik_from_P: NULL P or pi");
+        return fail(h, HY_ERR_ARG, "%s: unknown precision %d", who, precision);
+    if (!P || !pi) return fail(h, HY_ERR_ARG, "%s: NULL P or pi", who);
     if (h->P == 0) {
         if (out_logL) *out_logL = 0.0;
+        h->part_ri = -1;
         return HY_OK;
     }
     HIP_TRY(h, hipSetDevice(h->device));
@@ -1465,10 +1482,94 @@ int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *
     for (int i = 0; i < h->S; ++i) pp[i] = pi[i];
     TRY(upload(h, h->d_pi_custom, pp.data(), 64 * sizeof(double)));
     TRY(upload(h, h->d_Phost, P, sizeof(double) * (size_t)h->num_nodes * h->S * h->S));
-    const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, HY_WANT_LOGL, pp.data())
-                                             : run_eval<float>(h, HY_WANT_LOGL, pp.data());
+    const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, flags, pp.data())
+                                             : run_eval<float>(h, flags, pp.data());
     if (rc) return rc;
     return finish_eval(h, false, out_logL, out_site_ll, nullptr, nullptr, nullptr, nullptr, false);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
+                     double *out_site_ll) {
+    return eval_from_P(h, precision, P, pi, out_logL, out_site_ll, HY_WANT_LOGL, "hy_loglik_from_P");
+}
+
+int hy_inside_outside_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
+                             double *out_site_ll) {
+    return eval_from_P(h, precision, P, pi, out_logL, out_site_ll, HY_WANT_LOGL | RUN_OUTSIDE,
+                       "hy_inside_outside_from_P");
+}
+
+int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E, double *V, int32_t *G, double *L,
+                     int32_t *LE) {
+    if (!h) return HY_ERR_ARG;
+    if (h->part_ri < 0) return fail(h, HY_ERR_STATE, "hy_read_partials: no evaluation has left partials");
+    if (p_start < 0 || p_end > h->P || p_start >= p_end)
+        return fail(h, HY_ERR_ARG, "hy_read_partials: pattern range [%d, %d) outside [0, %d)", p_start, p_end, h->P);
+    if ((V || G) && !h->part_outside)
+        return fail(h, HY_ERR_STATE, "hy_read_partials: the last evaluation ran no outside pass");
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const int ri = h->part_ri, T = h->T, S = h->S, B = p_end - p_start, N = h->num_nodes;
+    const size_t esz = ri ? 8 : 4, tile_elems = (size_t)64 * T;
+    const int t0 = p_start / T, nt = (p_end - 1) / T - t0 + 1;
+    auto value = [&](const std::vector<char> &buf, size_t idx) {
+        return ri ? reinterpret_cast<const double *>(buf.data())[idx] : (double)reinterpret_cast<const float *>(buf.data())[idx];
+    };
+    // rows x (tile range) of a [rows][n_tiles][64 T] buffer
+    auto fetch_tiles = [&](const void *base, int rows, std::vector<char> &out) -> int {
+        out.resize((size_t)rows * nt * tile_elems * esz);
+        HIP_TRY(h, hipMemcpy2DAsync(out.data(), nt * tile_elems * esz, (const char *)base + t0 * tile_elems * esz,
+                                    (size_t)h->n_tiles * tile_elems * esz, nt * tile_elems * esz, rows,
+                                    hipMemcpyDeviceToHost, h->stream));
+        return HY_OK;
+    };
+    // rows x [p_start, p_end) of a [rows][P_pad] int32 buffer
+    auto fetch_cols = [&](const void *base, int rows, std::vector<int32_t> &out) -> int {
+        out.resize((size_t)rows * B);
+        HIP_TRY(h, hipMemcpy2DAsync(out.data(), (size_t)B * 4, (const char *)base + (size_t)p_start * 4,
+                                    (size_t)h->P_pad * 4, (size_t)B * 4, rows, hipMemcpyDeviceToHost, h->stream));
+        return HY_OK;
+    };
+    std::vector<char> dbuf, vbuf;
+    std::vector<int32_t> ebuf, gbuf;
+    if (D || E) {
+        TRY(fetch_tiles(h->d_D[ri].p, h->n_int, dbuf));
+        TRY(fetch_cols(h->d_E.p, h->n_int, ebuf));
+    }
+    if (V || G) {
+        TRY(fetch_tiles(h->d_Vp[ri].p, h->n_vslot, vbuf));
+        TRY(fetch_cols(h->d_G.p, h->n_vslot, gbuf));
+    }
+    std::vector<double> lr(B);
+    std::vector<int32_t> ler(B);
+    if (L) HIP_TRY(h, hipMemcpyAsync(lr.data(), (const double *)h->d_Lroot.p + p_start, B * 8, hipMemcpyDeviceToHost, h->stream));
+    if (LE) HIP_TRY(h, hipMemcpyAsync(ler.data(), (const int32_t *)h->d_Eroot.p + p_start, B * 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int u = 0; u < N; ++u) {
+        const int gi = h->int_slot[u], vs = (u == h->ext_root) ? -1 : h->vslot[u];
+        for (int b = 0; b < B; ++b) {
+            const int p = p_start + b, t = p / T - t0, tl = p % T;
+            if (gi >= 0) {
+                const size_t base = ((size_t)gi * nt + t) * tile_elems;
+                if (D)
+                    for (int st = 0; st < S; ++st) D[((size_t)u * B + b) * S + st] = value(dbuf, base + bmaj_index(st, tl));
+                if (E) E[(size_t)u * B + b] = ebuf[(size_t)gi * B + b];
+            }
+            if (vs >= 0) {
+                const size_t base = ((size_t)vs * nt + t) * tile_elems;
+                if (V)
+                    for (int st = 0; st < S; ++st) V[((size_t)u * B + b) * S + st] = value(vbuf, base + (size_t)st * T + tl);
+                if (G) G[(size_t)u * B + b] = gbuf[(size_t)vs * B + b];
+            }
+        }
+    }
+    if (L) std::memcpy(L, lr.data(), B * 8);
+    if (LE) std::memcpy(LE, ler.data(), B * 4);
+    return HY_OK;
 }
 
 int hy_last_tile(hy_engine h, int *patterns_per_tile) {

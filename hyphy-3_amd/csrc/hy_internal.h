@@ -39,6 +39,11 @@ constexpr int U_NEXT = -4;  // message handed to the next entry in registers
 // LDS slot capacities of the pruning kernels for tile width T and element
 // size esz (bytes of partials per slot = 64 * T * esz); each kernel stays
 // within the 160 KiB of LDS a gfx950 workgroup may use.
+// Inside partials in HBM are stored "B-operand-major": element (state s,
+// pattern t) of a 64 x T tile sits at ((t/16*4 + s%4)*16 + t%16)*16 + s/4, so
+// that the 16 MFMA B operands of one lane are contiguous.
+constexpr int bmaj_index(int s, int t) { return (((t >> 4) * 4 + (s & 3)) * 16 + (t & 15)) * 16 + (s >> 2); }
+
 constexpr int inside_slots(int T, int esz) { return T == 16 ? 12 : (T == 32 ? 6 : 3); }
 constexpr int outside_slots(int T, int esz) { return T == 16 ? 10 : (T == 32 ? 4 : (esz == 8 ? 1 : 2)); }
 // Tip codes of a tile kept in LDS when (taxa + 1) * T fits.

@@ -752,7 +752,7 @@ __device__ __forceinline__ void load_c(typename MF<R>::acc *x, const R *src, int
 //   ((t / 16 * 4 + s % 4) * 16 + t % 16) * 16 + s / 4,
 // so the 16 MFMA B operands of lane (lg, lr) for pattern block nb
 // (states 4kk + lg, kk = 0..15) are one contiguous run.
-__device__ __forceinline__ int bmaj(int s, int t) { return (((t >> 4) * 4 + (s & 3)) * 16 + (t & 15)) * 16 + (s >> 2); }
+__device__ __forceinline__ int bmaj(int s, int t) { return bmaj_index(s, t); }
 
 // B operands of lane (lg, lr) from a B-major tile: bv[kk * NB + nb].
 template <typename R, int T>

@@ -14,6 +14,34 @@ using namespace hyphy::core;
 
 namespace {
 
+// [node][state][pattern] array of a workspace's per-node S x B matrices
+py::array_t<double> node_stack(const std::vector<Matrix> &m) {
+    const py::ssize_t N = py::ssize_t(m.size()), S = N ? m[0].rows() : 0, B = N ? m[0].cols() : 0;
+    py::array_t<double> a({N, S, B});
+    auto w = a.mutable_unchecked<3>();
+    for (py::ssize_t n = 0; n < N; ++n)
+        for (py::ssize_t i = 0; i < S; ++i)
+            for (py::ssize_t b = 0; b < B; ++b) w(n, i, b) = m[n](i, b);
+    return a;
+}
+
+template <typename T>
+py::array_t<T> dense(const DenseMatrix<T> &m) {
+    py::array_t<T> a({py::ssize_t(m.rows()), py::ssize_t(m.cols())});
+    auto w = a.template mutable_unchecked<2>();
+    for (long i = 0; i < m.rows(); ++i)
+        for (long j = 0; j < m.cols(); ++j) w(i, j) = m(i, j);
+    return a;
+}
+
+std::vector<std::vector<double>> p_list(const py::array_t<double, py::array::c_style | py::array::forcecast> &P) {
+    if (P.ndim() != 3 || P.shape(1) != P.shape(2)) throw std::invalid_argument("P_branches must be num_nodes x S x S");
+    std::vector<std::vector<double>> out(P.shape(0));
+    const size_t SS = size_t(P.shape(1)) * P.shape(2);
+    for (py::ssize_t n = 0; n < P.shape(0); ++n) out[n].assign(P.data() + n * SS, P.data() + (n + 1) * SS);
+    return out;
+}
+
 py::dict problem_arrays(const Tree &tree, const Alignment &aln) {
     const auto l2t = LikelihoodEngine::build_leaf_to_taxon_map(tree, aln);
     const size_t N = tree.num_nodes();
@@ -238,6 +266,51 @@ PYBIND11_MODULE(_hyphy3, m) {
           },
           py::arg("tree"), py::arg("aln"), py::arg("params"), py::arg("omegas"), py::arg("weights"),
           py::arg("branch_lengths") = std::vector<double>{}, py::arg("time_convention") = 0, py::arg("want_grad") = true);
+
+    m.def("compute_inside_batch_in_place",
+          [](const Tree &tree, const Alignment &aln, size_t p_start, size_t p_end,
+             const py::array_t<double, py::array::c_style | py::array::forcecast> &P, const std::vector<double> &pi) {
+              const auto l2t = LikelihoodEngine::build_leaf_to_taxon_map(tree, aln);
+              const auto Pl = p_list(P);
+              LikelihoodEngine::BatchedInsideWorkspace ws;
+              {
+                  py::gil_scoped_release nogil;
+                  LikelihoodEngine::compute_inside_batch_in_place(tree, aln, p_start, p_end, l2t, Pl, pi, int(pi.size()), ws);
+              }
+              py::dict d;
+              d["D"] = node_stack(ws.D);
+              d["node_log_scale"] = dense(ws.node_log_scale);
+              d["root_L"] = ws.root_L;
+              d["root_log_L"] = ws.root_log_L;
+              return d;
+          },
+          py::arg("tree"), py::arg("aln"), py::arg("p_start"), py::arg("p_end"), py::arg("P_branches"), py::arg("pi"),
+          "Workspace of compute_inside_batch_in_place (arrays [node][state][pattern])");
+
+    m.def("compute_inside_outside_batch_in_place",
+          [](const Tree &tree, const Alignment &aln, size_t p_start, size_t p_end,
+             const py::array_t<double, py::array::c_style | py::array::forcecast> &P, const std::vector<double> &pi) {
+              const auto l2t = LikelihoodEngine::build_leaf_to_taxon_map(tree, aln);
+              const auto Pl = p_list(P);
+              LikelihoodEngine::BatchedInsideOutsideWorkspace ws;
+              {
+                  py::gil_scoped_release nogil;
+                  LikelihoodEngine::compute_inside_outside_batch_in_place(tree, aln, p_start, p_end, l2t, Pl, pi,
+                                                                          int(pi.size()), ws);
+              }
+              py::dict d;
+              d["D"] = node_stack(ws.D);
+              d["V"] = node_stack(ws.V);
+              d["P_D"] = node_stack(ws.P_D);
+              d["U"] = node_stack(ws.U);
+              d["boost_counts"] = dense(ws.boost_counts);
+              d["node_log_scale"] = dense(ws.node_log_scale);
+              d["likelihood"] = ws.likelihood;
+              d["log_likelihood"] = ws.log_likelihood;
+              return d;
+          },
+          py::arg("tree"), py::arg("aln"), py::arg("p_start"), py::arg("p_end"), py::arg("P_branches"), py::arg("pi"),
+          "Workspace of compute_inside_outside_batch_in_place (arrays [node][state][pattern])");
 
     m.def("problem_arrays", &problem_arrays, py::arg("tree"), py::arg("aln"),
           "Flat tree / pattern arrays in the layout of the C ABI's hy_set_problem and hy_set_model");

@@ -26,7 +26,8 @@ HY_WANT_GRAD = 4
 
 EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
-           "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_eval_srv", "hy_eval_batch", "hy_create_group")
+           "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
+           "hy_inside_outside_from_P", "hy_read_partials")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)

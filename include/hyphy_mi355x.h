@@ -170,6 +170,28 @@ int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned fla
 int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
                      double *out_site_ll);
 
+/* As hy_loglik_from_P, followed by the outside (pre-order) pass, so that
+ * hy_read_partials can return outside messages as well:
+ * compute_inside_outside_batch_in_place (likelihood.hpp:479-762) over all
+ * patterns. */
+int hy_inside_outside_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
+                             double *out_site_ll);
+
+/* Partials of the last evaluation on h for patterns [p_start, p_end)
+ * (B = p_end - p_start), the read-back behind the reference's workspaces
+ * (BatchedInsideWorkspace / BatchedInsideOutsideWorkspace,
+ * likelihood.hpp:295-340).  Values are stored as mantissa x 2^exponent:
+ *   D[num_nodes][B][S], E[num_nodes][B]  conditional likelihood below each
+ *       internal node u, D_u = D[u] * 2^E[u] (leaf entries untouched);
+ *   V[num_nodes][B][S], G[num_nodes][B]  outside message of the branch above
+ *       each non-root node v, V_v = V[v] * 2^G[v], with
+ *       V_v . (P_v D_v) = L for every v (needs an outside pass: a gradient
+ *       hy_eval or hy_inside_outside_from_P);
+ *   L[B], LE[B]  root likelihood L * 2^LE.
+ * Any output may be NULL.  Values are in the evaluation's precision. */
+int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E, double *V, int32_t *G, double *L,
+                     int32_t *LE);
+
 /* Pattern-tile width used by the pruning kernels (16, 32 or 64 patterns per
  * workgroup; 0 = automatic, chosen per precision from the pattern count and
  * the device's CU count).  Applies from the next hy_eval; a change of tile

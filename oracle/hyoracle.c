@@ -644,6 +644,35 @@ static void inside_outside_tile(const or_tree *tr, int p0, int p1, const double 
     }
 }
 
+/* The workspace of compute_inside_batch_in_place (outside = 0) or
+ * compute_inside_outside_batch_in_place (outside = 1) for patterns [p0, p1)
+ * in one tile, copied out as [node][state][pattern] (D, V, PD, U; V/PD/U and
+ * boost only when outside) and [node][pattern] (boost, logsc). */
+int or_workspace(const or_tree *tr, const double *P, const double *pi, int32_t p0, int32_t p1, int32_t outside,
+                 double *D, double *V, double *PD, double *U, int32_t *boost, double *logsc, double *L, double *logL) {
+    const int B = p1 - p0, N = tr->num_nodes, S = tr->S;
+    if (B <= 0 || p0 < 0 || p1 > tr->num_patterns) return -1;
+    ws_t w;
+    ws_init(&w, tr, B, outside);
+    if (outside)
+        inside_outside_tile(tr, p0, p1, P, pi, &w);
+    else
+        inside_tile(tr, p0, p1, P, pi, &w);
+    const size_t nm = (size_t)N * S * B;
+    memcpy(D, w.D, nm * sizeof(double));
+    if (outside) {
+        memcpy(V, w.V, nm * sizeof(double));
+        memcpy(PD, w.PD, nm * sizeof(double));
+        memcpy(U, w.U, nm * sizeof(double));
+        memcpy(boost, w.boost, sizeof(int) * N * B);
+    }
+    memcpy(logsc, w.logsc, sizeof(double) * N * B);
+    memcpy(L, w.L, sizeof(double) * B);
+    memcpy(logL, w.logL, sizeof(double) * B);
+    ws_free(&w);
+    return 0;
+}
+
 static int clamp_threads(int threads, int work) {
     int t = threads > 0 ? threads : 1;
 #ifdef _OPENMP

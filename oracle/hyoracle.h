@@ -137,6 +137,10 @@ int or_srv_eval(const or_tree *tr, const or_code *code, const or_mg94_params *ba
                 const double *pos_freqs, const double *pi, const or_mixture *mix, const double *bl,
                 int32_t n_syn, const double *syn_rates, const double *syn_weights, int32_t want_grad,
                 int32_t threads, or_mixture_out *out, double *grad_syn_rates, double *grad_syn_weights);
+/* Reference workspaces (likelihood.hpp:295-340) of one pattern tile. */
+int or_workspace(const or_tree *tr, const double *P, const double *pi, int32_t p0, int32_t p1, int32_t outside,
+                 double *D, double *V, double *PD, double *U, int32_t *boost, double *logsc, double *L,
+                 double *logL);
 double or_neumaier_sum(const double *x, int64_t n);
 
 #ifdef __cplusplus

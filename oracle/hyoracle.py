@@ -86,6 +86,9 @@ def lib():
                                   C.POINTER(OrMixture), _f64p, C.c_int32, _f64p, _f64p, C.c_int32, C.c_int32,
                                   C.POINTER(OrMixtureOut), _f64p, _f64p]
         L.or_srv_eval.restype = C.c_int32
+        L.or_workspace.argtypes = [C.POINTER(OrTree), _f64p, _f64p, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p,
+                                   _f64p, _f64p, _i32p, _f64p, _f64p, _f64p]
+        L.or_workspace.restype = C.c_int32
         L.or_neumaier_sum.argtypes = [_f64p, C.c_int64]
         L.or_neumaier_sum.restype = C.c_double
         _lib = L
@@ -356,3 +359,24 @@ def srv_eval(prob, code, base, pos, pi, mix, bl, syn_rates, syn_weights, want_gr
     return {"log_likelihood": out.log_likelihood, "site_ll": site[:prob.num_patterns], "grad_bl": gbl,
             "grad_omega": gom, "grad_w": gw[:mix.n_comp], "grad_theta": gth, "grad_syn_rates": gsr,
             "grad_syn_weights": gsw}
+
+
+def workspace(prob, P, pi, p0, p1, outside=True):
+    """The reference's tile workspace (compute_inside[_outside]_batch_in_place)
+    for patterns [p0, p1): arrays [node][state][pattern] and [node][pattern]."""
+    N, S, B = prob.num_nodes, prob.t.S, p1 - p0
+    D = np.zeros((N, S, B))
+    V, PD, U = (np.zeros((N, S, B)) for _ in range(3))
+    boost = np.zeros((N, B), dtype=np.int32)
+    logsc = np.zeros((N, B))
+    L, logL = np.zeros(B), np.zeros(B)
+    Pc = f64(P)
+    rc = lib().or_workspace(C.byref(prob.t), ptr(Pc, _f64p), ptr(f64(pi), _f64p), p0, p1, 1 if outside else 0,
+                            ptr(D, _f64p), ptr(V, _f64p), ptr(PD, _f64p), ptr(U, _f64p), ptr(boost, _i32p),
+                            ptr(logsc, _f64p), ptr(L, _f64p), ptr(logL, _f64p))
+    if rc != 0:
+        raise RuntimeError("or_workspace failed")
+    out = {"D": D, "node_log_scale": logsc, "L": L, "logL": logL}
+    if outside:
+        out.update(V=V, P_D=PD, U=U, boost_counts=boost)
+    return out
