@@ -75,7 +75,7 @@ struct hy_engine_s {
     // model frame
     bool have_model = false;
     int code_id = 0, S = 61;
-    DevBuf d_nd, d_dpos, d_dfrom, d_dto, d_syn, d_pos, d_pi, d_pi_custom, d_Phost;
+    DevBuf d_nd, d_dpos, d_dfrom, d_dto, d_syn, d_qdesc, d_pos, d_pi, d_pi_custom, d_Phost;
 
     // problem
     bool have_problem = false;
@@ -491,6 +491,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     const double *params = (const double *)h->d_params.p;
     if (!custom_pi) {
         ModelArgs ma{};
+        ma.qdesc = (const int32_t *)h->d_qdesc.p;
         ma.nd = (const int8_t *)h->d_nd.p;
         ma.dpos = (const int8_t *)h->d_dpos.p;
         ma.dfrom = (const int8_t *)h->d_dfrom.p;
@@ -765,7 +766,7 @@ void hy_destroy(hy_engine h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    DevBuf *bufs[] = {&h->d_nd, &h->d_dpos, &h->d_dfrom, &h->d_dto, &h->d_syn, &h->d_pos, &h->d_pi,
+    DevBuf *bufs[] = {&h->d_qdesc, &h->d_nd, &h->d_dpos, &h->d_dfrom, &h->d_dto, &h->d_syn, &h->d_pos, &h->d_pi,
                       &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
                       &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
@@ -810,6 +811,7 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
                     h->S);
     std::vector<int8_t> nd(4096, 0), dpos(4096 * 3, -1), dfrom(4096 * 3, -1), dto(4096 * 3, -1);
     std::vector<uint8_t> syn(4096, 0);
+    std::vector<int32_t> qdesc(4096, 0);
     for (int i = 0; i < S; ++i)
         for (int j = 0; j < S; ++j) {
             if (i == j) continue;
@@ -817,11 +819,16 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
             const int k = i * 64 + j;
             nd[k] = d.num_diffs;
             syn[k] = d.synonymous ? 1 : 0;
+            int32_t qd = (d.num_diffs & 3) | (d.synonymous ? QDESC_SYN : 0);
             for (int q = 0; q < 3; ++q) {
                 dpos[3 * k + q] = d.diff_pos[q];
                 dfrom[3 * k + q] = d.diff_from[q];
                 dto[3 * k + q] = d.diff_to[q];
+                if (q < d.num_diffs)
+                    qd |= (qdesc_theta_index(d.diff_from[q], d.diff_to[q]) | ((d.diff_pos[q] * 4 + d.diff_to[q]) << 3))
+                          << (3 + 7 * q);
             }
+            qdesc[k] = qd;
         }
     std::vector<double> pp(64, 0.0);
     for (int i = 0; i < S; ++i) {
@@ -835,6 +842,7 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
     TRY(upload(h, h->d_dfrom, dfrom.data(), dfrom.size()));
     TRY(upload(h, h->d_dto, dto.data(), dto.size()));
     TRY(upload(h, h->d_syn, syn.data(), syn.size()));
+    TRY(upload(h, h->d_qdesc, qdesc.data(), qdesc.size() * 4));
     TRY(upload(h, h->d_pos, pos_freqs, 12 * sizeof(double)));
     TRY(upload(h, h->d_pi, pp.data(), 64 * sizeof(double)));
     h->code_id = code_id;

@@ -61,7 +61,20 @@ struct ParamLayout {
     int off_bl, off_alpha, off_omega, off_w, off_theta, off_delta, off_psi, total;
 };
 
+// Packed description of codon pair (i, j), one int32 per entry of the 64 x 64
+// generator: bits 0-1 number of differing positions (0 = none or not a
+// sense pair), bit 2 synonymous, then per difference k (k = 0..2) 7 bits at
+// 3 + 7k: exchangeability index (0..4 = AC, AT, CG, CT, GT; 5 = AG) and
+// 4 bits position-frequency index (position * 4 + target nucleotide).
+constexpr int QDESC_SYN = 4;
+constexpr int qdesc_theta_index(int a, int b) {
+    return (a > b) ? qdesc_theta_index(b, a)
+                   : (a == 0 && b == 1) ? 0 : (a == 0 && b == 3) ? 1 : (a == 1 && b == 2) ? 2
+                   : (a == 1 && b == 3) ? 3 : (a == 2 && b == 3) ? 4 : 5;
+}
+
 struct ModelArgs {
+    const int32_t *qdesc;                   // [64*64] packed pair descriptors
     const int8_t *nd, *dpos, *dfrom, *dto;  // [64*64], [64*64*3] x3
     const uint8_t *syn;                     // [64*64]
     const double *pos;                      // [12]

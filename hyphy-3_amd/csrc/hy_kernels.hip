@@ -13,7 +13,7 @@
 //   k_proj     (matrix, parameter) workgroups: M = V_pi dQ U_pi
 //   k_frechet  (branch, component) workgroups: U^T A V^T, spectral Frechet
 //              projections (FP64)
-//   k_final    one workgroup: deterministic compensated reductions
+//   k_final    one workgroup of 16 waves: deterministic compensated reductions
 //
 // MFMA lane layout (v_mfma_{f32,f64}_16x16x4, measured on gfx950): lane l
 // supplies A[l%16][l/16] and B[l/16][l%16]; accumulator element r of lane l
@@ -153,6 +153,29 @@ __device__ __forceinline__ double block_sum(double v, double *red, int tid) {
     return s;
 }
 
+// Two block sums at once (same contract as block_sum; red has 2 NT / 64 entries).
+template <int NT>
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *red, int tid) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off);
+        b += __shfl_down(b, off);
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) {
+        red[2 * (tid >> 6)] = a;
+        red[2 * (tid >> 6) + 1] = b;
+    }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0;
+    for (int k = 0; k < NT / 64; ++k) {
+        sa += red[2 * k];
+        sb += red[2 * k + 1];
+    }
+    __syncthreads();
+    a = sa;
+    b = sb;
+}
+
 constexpr int MODEL_NT = 1024;
 __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs a) {
     constexpr int NT = MODEL_NT;
@@ -164,31 +187,37 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs a) {
     __shared__ double sq[64], isq[64], lam[64];
     __shared__ double rc[32], rs[32], rt[32];
     __shared__ int rp[32], rq[32];
-    __shared__ double red[NT / 64];
+    __shared__ double red[2 * NT / 64];
     __shared__ int any_rot;
+    __shared__ double thl[8], posl[12];
 
     const double *prm = a.params;
     const double alpha = prm[a.pl.off_alpha], beta = prm[a.pl.off_omega + m];
-    const double *th = prm + a.pl.off_theta;
     const double delta = prm[a.pl.off_delta], psi = prm[a.pl.off_psi];
     const int S = a.S;
+    if (tid < 8) thl[tid] = tid < 5 ? prm[a.pl.off_theta + tid] : 1.0;  // index 5: A<->G
+    if (tid < 12) posl[tid] = a.pos[tid];
+    __syncthreads();
 
+    // Q_ij from the packed pair descriptor (see QDESC_*): one coalesced load
+    // per entry, exchangeabilities and position frequencies from LDS.
     for (int idx = tid; idx < 4096; idx += NT) {
         const int i = idx >> 6, j = idx & 63;
+        const int d = a.qdesc[idx], nd = d & 3;
         double q = 0.0;
-        if (i < S && j < S && i != j) {
-            const int k = i * 64 + j;
-            const double rm = a.syn[k] ? alpha : beta;
-            const int nd = a.nd[k];
-            const int8_t *dp = a.dpos + 3 * k, *df = a.dfrom + 3 * k, *dt = a.dto + 3 * k;
+        if (i < S && j < S && i != j && nd > 0) {
+            const double rm = (d & QDESC_SYN) ? alpha : beta;
+            const double r0 = thl[(d >> 3) & 7], p0 = posl[(d >> 6) & 15];
             if (nd == 1) {
-                q = rm * nuc_rate(th, df[0], dt[0]) * a.pos[dp[0] * 4 + dt[0]];
-            } else if (nd == 2 && delta > 0.0) {
-                q = rm * delta * (nuc_rate(th, df[0], dt[0]) * nuc_rate(th, df[1], dt[1])) *
-                    (a.pos[dp[0] * 4 + dt[0]] * a.pos[dp[1] * 4 + dt[1]]);
-            } else if (nd == 3 && psi > 0.0) {
-                q = rm * psi * (nuc_rate(th, df[0], dt[0]) * nuc_rate(th, df[1], dt[1]) * nuc_rate(th, df[2], dt[2])) *
-                    (a.pos[dp[0] * 4 + dt[0]] * a.pos[dp[1] * 4 + dt[1]] * a.pos[dp[2] * 4 + dt[2]]);
+                q = rm * r0 * p0;
+            } else {
+                const double r1 = thl[(d >> 10) & 7], p1 = posl[(d >> 13) & 15];
+                if (nd == 2) {
+                    if (delta > 0.0) q = rm * delta * (r0 * r1) * (p0 * p1);
+                } else if (psi > 0.0) {
+                    const double r2 = thl[(d >> 17) & 7], p2 = posl[(d >> 20) & 15];
+                    q = rm * psi * (r0 * r1 * r2) * (p0 * p1 * p2);
+                }
             }
         }
         A[i * LDA + j] = q;
@@ -210,10 +239,10 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs a) {
         }
     }
     __syncthreads();
-    if (tid == 0) {
-        double mu = 0.0;
-        for (int i = 0; i < S; ++i) mu -= a.pi[i] * A[i * LDA + i];
-        a.mu[m] = mu;
+    if (tid < 64) {
+        double t = tid < S ? -a.pi[tid] * A[tid * LDA + tid] : 0.0;
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
+        if (tid == 0) a.mu[m] = t;
     }
     for (int idx = tid; idx < 4096; idx += NT) {
         const int i = idx >> 6, j = idx & 63;
@@ -274,19 +303,21 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs a) {
             __syncthreads();
             if (tid < 64) lam[tid] = T2[tid * LDA + tid] / (1.0 - T1[tid * LDA + tid]);
             // cluster threshold delta = 2 (||S - D||_F + ||A||_F ||R||_F)
-            double so = 0.0, rr = 0.0;
+            double so = 0.0, unused = 0.0;
             for (int idx = tid; idx < 4096; idx += NT) {
                 const int i = idx >> 6, j = idx & 63;
-                const double r = T1[i * LDA + j];
-                rr += r * r;
                 if (i != j) so += T2[i * LDA + j] * T2[i * LDA + j];
             }
-            const double soff = sqrt(block_sum<NT>(so, red, tid));
-            const double rnorm = sqrt(block_sum<NT>(rr, red, tid));
-            const double dlt = 2.0 * (soff + anorm * rnorm);
-            __syncthreads();
-            // E into T1 (overwrites R after reading it)
-            double en = 0.0;
+            block_sum2<NT>(so, unused, red, tid);
+            const double soff = sqrt(so);
+            // Pairs closer than the off-diagonal mass are treated as a cluster
+            // (orthogonalised only).  The threshold leaves out Ogita-Aishima's
+            // ||A|| ||R|| term: after an update R is second order, and that term
+            // froze close pairs for an extra step; a step that does not
+            // contract still falls back to the cold solve.
+            const double dlt = 2.0 * soff;
+            // E into T2 (S is no longer needed after each element)
+            double en = 0.0, fz = 0.0;
             for (int idx = tid; idx < 4096; idx += NT) {
                 const int i = idx >> 6, j = idx & 63;
                 const double r = T1[i * LDA + j];
@@ -295,12 +326,19 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs a) {
                     e = 0.5 * r;
                 } else {
                     const double d = lam[j] - lam[i];
-                    e = (fabs(d) > dlt) ? (T2[i * LDA + j] + lam[j] * r) / d : 0.5 * r;
+                    const double sij = T2[i * LDA + j];
+                    if (fabs(d) > dlt) {
+                        e = (sij + lam[j] * r) / d;
+                    } else {
+                        e = 0.5 * r;
+                        fz += sij * sij;
+                    }
                 }
-                T2[i * LDA + j] = e;  // T2 (S) is no longer needed after this element
+                T2[i * LDA + j] = e;
                 en += e * e;
             }
-            const double enorm = sqrt(block_sum<NT>(en, red, tid));
+            block_sum2<NT>(en, fz, red, tid);
+            const double enorm = sqrt(en), frozen = sqrt(fz);
             // X <- X + X E
             gemm64<false, false, NT / 64>(Z, T2, T1, w, lr, lg);
             __syncthreads();
@@ -310,11 +348,15 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs a) {
             }
             __syncthreads();
             if (!(enorm < 0.5) || (it >= 3 && enorm > 0.5 * prev_en)) break;  // not converging: solve cold
+            // Accept X + X E when X already diagonalised A (off-diagonal of
+            // X^T A X below 1e-9 ||A||) and the correction is second-order
+            // small, or when the steps contract quadratically (the next
+            // correction, predicted as |E|^3 / |E_prev|^2, is below 1e-14)
+            // and no cluster hides an off-diagonal entry above 1e-14 ||A||.
+            const bool quad = it >= 1 && enorm < 1e-5 && enorm * enorm * enorm <= 1e-14 * prev_en * prev_en &&
+                              frozen <= 1e-14 * anorm;
             prev_en = enorm;
-            // Accept only when X actually diagonalises A (off-diagonal of X^T A X
-            // below 1e-9 ||A||) and the correction is second-order small: with a
-            // large cluster threshold every pair is frozen and E alone says nothing.
-            if (soff <= 1e-9 * anorm && enorm < 1e-9) {
+            if ((soff <= 1e-9 * anorm && enorm < 1e-9) || quad) {
                 done = true;
                 status = 100 + it + 1;
             }
@@ -1489,10 +1531,12 @@ __global__ __launch_bounds__(256) void k_hproj(FrechetArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// k_final: deterministic reductions into the result vector.  lnL: per-thread
-// compensated sums over contiguous pattern ranges, combined in thread order.
-// Gradient sums over branches: one wave per output, lanes striding branches,
-// fixed shuffle-tree combination.
+// k_final: deterministic reductions into the result vector, one workgroup of
+// 16 waves.  lnL: per-thread compensated sums over contiguous pattern ranges,
+// combined inside each wave by a butterfly of error-free additions and across
+// waves in wave order.  With gradients the upper eight waves assemble them
+// meanwhile: sums over branches with lanes striding branches and a fixed
+// shuffle tree.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void neum(double &s, double &c, double x) {
     const double t = s + x;
@@ -1508,72 +1552,88 @@ __device__ __forceinline__ double wave_sum(double v) {
     return __shfl(v, 0);
 }
 
-__global__ __launch_bounds__(256) void k_final(FinalArgs a) {
+// (s, c) += (s', c') with the rounding error of s + s' kept exactly (TwoSum);
+// symmetric in its operands, so every lane of a butterfly gets the same sum.
+__device__ __forceinline__ void comp_add(double &s, double &c, double s2, double c2) {
+    const double t = s + s2, bp = t - s, ap = t - bp;
+    c += c2 + ((s - ap) + (s2 - bp));
+    s = t;
+}
+
+constexpr int FINAL_NT = 1024;
+__global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    __shared__ double ps[256], pc[256];
+    __shared__ double wsum[16], wcmp[16];
     __shared__ double sum_tau[64], mubar[64];
+    const bool grad = a.want_grad != 0;
+    const int NL = grad ? FINAL_NT / 2 : FINAL_NT;  // threads on the lnL sum
     const int P = a.n_patterns;
-    const int per = (P + 255) / 256;
-    double s = 0.0, c = 0.0;
-    const int pe = min((tid + 1) * per, P);
-    for (int p0 = tid * per; p0 < pe; p0 += 8) {
-        // eight independent loads in flight, then the compensated sums in order
-        double x[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int p = p0 + u;
-            x[u] = 0.0;
-            if (p < pe && !(a.want_grad && !(a.Lroot[p] > 0.0))) x[u] = a.weights[p] * a.site_ll[p];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (p0 + u < pe) neum(s, c, x[u]);
-    }
-    ps[tid] = s;
-    pc[tid] = c;
-    __syncthreads();
-    if (tid == 0) {
-        double S0 = 0.0, C0 = 0.0;
-        for (int i = 0; i < 256; ++i) {
-            neum(S0, C0, ps[i]);
-            C0 += pc[i];
-        }
-        a.res[0] = S0 + C0;
-    }
-    if (!a.want_grad) return;
     const double *prm = a.params;
     const int NR = a.n_real_nodes, NM = a.pl.n_mat, NCP = a.pl.n_comp, MC = a.max_comp;
     double *gbl = a.res + 1, *gom = gbl + NR, *gw = gom + NM, *gth = gw + NCP;
-    if (tid < a.n_class) {
-        double mb = 0.0;
-        for (int j = a.comp_ptr[tid]; j < a.comp_ptr[tid + 1]; ++j) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
-        mubar[tid] = mb < 1e-12 ? 1e-12 : mb;
-    }
-    for (int n = tid; n < NR; n += 256) gbl[n] = 0.0;
-    __syncthreads();
-    for (int cl = wave; cl < a.n_class; cl += 4) {
-        double v = 0.0;
-        for (int bi = lane; bi < a.n_branch; bi += 64) {
-            if (a.blist[bi * BR_STRIDE + 3] != cl) continue;
-            double dt = 0.0;
-            const int nj = a.comp_ptr[cl + 1] - a.comp_ptr[cl];  // components of this branch's class
-            for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
-            v += a.tau[a.blist[bi * BR_STRIDE]] * dt;
+    const int gt = tid - NL, gwave = wave - NL / 64, NGW = (FINAL_NT - NL) / 64;  // gradient threads / waves
+    if (tid < NL) {
+        const int per = (P + NL - 1) / NL;
+        double s = 0.0, c = 0.0;
+        const int pe = min((tid + 1) * per, P);
+        for (int p0 = tid * per; p0 < pe; p0 += 8) {
+            // eight independent loads in flight, then the compensated sums in order
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int p = p0 + u;
+                x[u] = 0.0;
+                if (p < pe && !(grad && !(a.Lroot[p] > 0.0))) x[u] = a.weights[p] * a.site_ll[p];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (p0 + u < pe) neum(s, c, x[u]);
         }
-        v = wave_sum(v);
-        if (lane == 0) sum_tau[cl] = v;
-    }
-    for (int bi = tid; bi < a.n_branch; bi += 256) {
-        const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
-        const double conv = (a.convention == 1) ? 3.0 / mubar[cls] : 1.0;
-        double dt = 0.0;
-        const int nj = a.comp_ptr[cls + 1] - a.comp_ptr[cls];
-        for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
-        if (node < NR) gbl[node] = conv * dt;
+        for (int off = 1; off < 64; off <<= 1) comp_add(s, c, __shfl_xor(s, off), __shfl_xor(c, off));
+        if (lane == 0) {
+            wsum[wave] = s;
+            wcmp[wave] = c;
+        }
+    } else {
+        if (gt < a.n_class) {
+            double mb = 0.0;
+            for (int j = a.comp_ptr[gt]; j < a.comp_ptr[gt + 1]; ++j) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
+            mubar[gt] = mb < 1e-12 ? 1e-12 : mb;
+        }
+        for (int n = gt; n < NR; n += FINAL_NT - NL) gbl[n] = 0.0;
     }
     __syncthreads();
+    if (grad && tid >= NL) {
+        for (int cl = gwave; cl < a.n_class; cl += NGW) {
+            double v = 0.0;
+            for (int bi = lane; bi < a.n_branch; bi += 64) {
+                if (a.blist[bi * BR_STRIDE + 3] != cl) continue;
+                double dt = 0.0;
+                const int nj = a.comp_ptr[cl + 1] - a.comp_ptr[cl];  // components of this branch's class
+                for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+                v += a.tau[a.blist[bi * BR_STRIDE]] * dt;
+            }
+            v = wave_sum(v);
+            if (lane == 0) sum_tau[cl] = v;
+        }
+        for (int bi = gt; bi < a.n_branch; bi += FINAL_NT - NL) {
+            const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
+            const double conv = (a.convention == 1) ? 3.0 / mubar[cls] : 1.0;
+            double dt = 0.0;
+            const int nj = a.comp_ptr[cls + 1] - a.comp_ptr[cls];
+            for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+            if (node < NR) gbl[node] = conv * dt;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double S0 = 0.0, C0 = 0.0;
+        for (int w = 0; w < NL / 64; ++w) comp_add(S0, C0, wsum[w], wcmp[w]);
+        a.res[0] = S0 + C0;
+    }
+    if (!grad || tid < NL) return;
     const int ntask = NM + 5 + NCP;
-    for (int t = wave; t < ntask; t += 4) {
+    for (int t = gwave; t < ntask; t += NGW) {
         double v = 0.0;
         if (t < NM + 5) {
             const bool is_om = t < NM;
@@ -1679,7 +1739,7 @@ void launch_frechet(const FrechetArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_hproj, dim3(a.n_mat, HPROJ_CHUNKS), dim3(256), 0, s, a);
 }
 
-void launch_final(const FinalArgs &a, hipStream_t s) { hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, a); }
+void launch_final(const FinalArgs &a, hipStream_t s) { hipLaunchKernelGGL(k_final, dim3(1), dim3(FINAL_NT), 0, s, a); }
 
 template void launch_pmat<float>(const PmatArgs<float> &, hipStream_t);
 template void launch_pmat<double>(const PmatArgs<double> &, hipStream_t);

@@ -110,3 +110,20 @@ def test_warm_start_matches_cold():
         assert rel(rw["log_likelihood"], rc["log_likelihood"]) < 1e-12
         for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
             assert grad_err(rw[k], rc[k]) < 1e-9, k
+
+
+def test_multihit_rates_forward():
+    """Double and triple nucleotide changes (delta, psi; rate_matrix.hpp:205-219)
+    in the forward pass, FP64 and FP32, against the oracle."""
+    tree, aln, sim = sim_problem(10, 180, 12)
+    N = len(tree.nodes)
+    prob = ho.Problem.from_refs(tree, aln)
+    bl = branch_lengths(tree) * sim["conv"]
+    eng = make_engine(tree, aln)
+    eng.set_mixture(1, np.zeros(N, dtype=np.int32), [[0]], convention=0)
+    for delta, psi in ((0.3, 0.0), (0.0, 0.15), (0.25, 0.08)):
+        p = ho.params(1.0, 0.6, sim["theta"], delta, psi)
+        ref = ho.mg94_loglik(prob, ho.Code(0), p, aln.pos_freqs, aln.cf3x4, bl)
+        for prec, tl in ((FP64, 1e-10), (FP32, 1e-5)):
+            r = eng.eval(bl, [0.6], [1.0], sim["theta"], delta=delta, psi=psi, precision=prec, grad=False)
+            assert rel(r["log_likelihood"], ref) < tl, (delta, psi, prec)
