@@ -299,7 +299,7 @@ def bench_genes(args):
     """Batched many-genes workload: one step = logL + full gradient of every
     gene of the batch through one hy_eval_batch call."""
     from multiprocessing import get_context
-    from hyphy3.engine import create_group, eval_batch
+    from hyphy3.engine import Batch, create_group
     specs = gene_specs(args.genes, C3["seed"])
     nproc = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
     t0 = time.perf_counter()
@@ -320,8 +320,10 @@ def bench_genes(args):
     dev_bytes = sum(e.device_bytes() for e in engines)
     print(f"[bench] c3: {len(engines)} engines set up", file=sys.stderr, flush=True)
 
+    batch = Batch(engines)
+
     def step(i, prec):
-        return eval_batch(engines, [step_params(b, i) for b in bases], precision=prec)
+        return batch.eval([step_params(b, i) for b in bases], precision=prec)
 
     res = {}
     for prec, name in ((1, "fp32"), (0, "fp64")):
@@ -335,7 +337,7 @@ def bench_genes(args):
         for i in range(args.steps):
             out = step(args.warmup + i, prec)
         sync()
-        res[name] = ((time.perf_counter() - t0) / args.steps, out)
+        res[name] = ((time.perf_counter() - t0) / args.steps, [dict(r) for r in out])
         print(f"[bench] c3 {name} {res[name][0] * 1e3:.2f} ms/batch", file=sys.stderr, flush=True)
     ms32, out32 = res["fp32"]
     t_bound = f_alg / (PEAK_FP32_TFLOPS * 1e12)

@@ -50,9 +50,17 @@ struct StreamSet {
     }
 };
 
+struct GroupState;
+
 struct hy_engine_s {
     int device = 0;
     std::shared_ptr<StreamSet> streams;
+    std::shared_ptr<GroupState> group;  // engines made by hy_create_group
+    // Fused batch context (an internal engine holding the merged problem of a
+    // group): gene table, tile -> gene and matrix -> gene maps.
+    int n_genes = 1;
+    DevBuf d_genes, d_tile_gene, d_mat_gene;
+    size_t n_res = 0;  // doubles in the result vector
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // side stream: dQ projections overlap the pruning passes
     hipEvent_t ev_model = nullptr, ev_proj = nullptr;
@@ -76,6 +84,7 @@ struct hy_engine_s {
     bool have_model = false;
     int code_id = 0, S = 61;
     DevBuf d_nd, d_dpos, d_dfrom, d_dto, d_syn, d_qdesc, d_pos, d_pi, d_pi_custom, d_Phost;
+    std::vector<double> pos_host, pi_host;  // [12], [64] as uploaded
 
     // problem
     bool have_problem = false;
@@ -129,6 +138,22 @@ struct hy_engine_s {
     int n_chunks = 1, tiles_per_chunk = 1;
 
     size_t device_bytes = 0;
+};
+
+struct GroupState {
+    hy_engine ctx = nullptr;
+    std::vector<hy_engine> members;  // genes of the merged problem, in order
+    std::vector<uint64_t> epochs;
+    int prec = -1;
+    bool no_fuse = false;  // HY_NO_FUSE=1: evaluate group batches engine by engine
+    struct Slice {
+        int node0, n_ext, n_real, mat0, n_mat, comp0, n_comp, pbase, P;
+        size_t res_off;
+    };
+    std::vector<Slice> slices;
+    ~GroupState() {
+        if (ctx) hy_destroy(ctx);
+    }
 };
 
 namespace {
@@ -355,6 +380,7 @@ int prepare_buffers(hy_engine h, int prec) {
     build_layout(h);
     TRY(ensure(h, h->d_params, (size_t)h->pl.total * 8));
     const size_t nres = 1 + (size_t)h->num_nodes + h->n_mat + h->n_comp + 5;
+    h->n_res = nres;
     TRY(ensure(h, h->d_res, nres * 8));
     if (h->h_params_cap < (size_t)h->pl.total) {
         if (h->h_params) (void)hipHostFree(h->h_params);
@@ -509,6 +535,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         ma.Z = (double *)h->d_Z.p;
         ma.warm = (h->warm_enable && h->warm_valid) ? 1 : 0;
         ma.status = (int *)h->d_status.p;
+        ma.mat_gene = (const int32_t *)h->d_mat_gene.p;
         launch_model(ma, h->n_mat, s);
         if (grad) {
             // M_phi = V dQ_phi U and d mu / d phi depend on the spectra only
@@ -529,6 +556,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
             pj.V = (const double *)h->d_V.p;
             pj.M = (double *)h->d_M.p;
             pj.dmu = (double *)h->d_dmu.p;
+            pj.mat_gene = (const int32_t *)h->d_mat_gene.p;
             launch_proj(pj, h->n_mat, h->stream2);
             HIP_TRY(h, hipEventRecord(h->ev_proj, h->stream2));
         }
@@ -566,6 +594,8 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     }
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[2], s));
 
+    if (!h->d_sched.p || !h->d_osched.p || h->sched.empty() || h->osched.empty())
+        return fail(h, HY_ERR_STATE, "internal: pruning schedules not on the device");
     PruneArgs<R> pr{};
     pr.sched = (const int32_t *)h->d_sched.p;
     pr.n_sched = (int)(h->sched.size() / SCHED_STRIDE);
@@ -591,6 +621,8 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     pr.Lroot = (double *)h->d_Lroot.p;
     pr.Eroot = (int32_t *)h->d_Eroot.p;
     pr.site_ll = (double *)h->d_site.p;
+    pr.genes = (const GeneInfo *)h->d_genes.p;
+    pr.tile_gene = (const int32_t *)h->d_tile_gene.p;
     launch_inside<R>(pr, h->T, s);
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[3], s));
     if (grad) {
@@ -614,6 +646,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         aa.n_tiles = h->n_tiles;
         aa.S = h->S;
         aa.Apart = (double *)h->d_Apart.p;
+        aa.genes = (const GeneInfo *)h->d_genes.p;
         if (h->n_branch > 0) launch_adjoint<R>(aa, h->T, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[5], s));
         HIP_TRY(h, hipStreamWaitEvent(s, h->ev_proj, 0));
@@ -668,7 +701,8 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     fn.dmu = (const double *)h->d_dmu.p;
     fn.proj = (const double *)h->d_proj.p;
     fn.res = (double *)h->d_res.p;
-    launch_final(fn, s);
+    fn.genes = (const GeneInfo *)h->d_genes.p;
+    launch_final(fn, h->n_genes, s);
     HIP_TRY(h, hipGetLastError());
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[7], s));
     return HY_OK;
@@ -753,20 +787,31 @@ int hy_create_group(int device, int n, int n_stream_sets, hy_engine *out) {
         e = make_stream_set(device, ss);
         if (e != hipSuccess) return fail(nullptr, HY_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
     }
+    // the context engine that evaluates the group's merged problem
+    std::shared_ptr<StreamSet> cs;
+    e = make_stream_set(device, cs);
+    if (e != hipSuccess) return fail(nullptr, HY_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+    auto gs = std::make_shared<GroupState>();
+    gs->ctx = new_engine(device, prop.multiProcessorCount, cs);
+    const char *nf = std::getenv("HY_NO_FUSE");
+    gs->no_fuse = nf && nf[0] == '1';
     for (int i = 0; i < n; ++i) {
         out[i] = new_engine(device, prop.multiProcessorCount, sets[i % ns]);
-        // Grouped engines launch eagerly: thousands of instantiated graphs
-        // (each holding its own runtime resources) are not kept per gene.
+        // Grouped engines evaluated one by one launch eagerly: thousands of
+        // instantiated graphs (each holding runtime resources) are not kept
+        // per gene.  Batches of a group run fused (one graph).
         out[i]->use_graphs = false;
+        out[i]->group = gs;
     }
     return HY_OK;
 }
 
 void hy_destroy(hy_engine h) {
     if (!h) return;
+    if (h->group) h->group->members.clear();  // the merged problem no longer matches
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    DevBuf *bufs[] = {&h->d_qdesc, &h->d_nd, &h->d_dpos, &h->d_dfrom, &h->d_dto, &h->d_syn, &h->d_pos, &h->d_pi,
+    DevBuf *bufs[] = {&h->d_genes, &h->d_tile_gene, &h->d_mat_gene, &h->d_qdesc, &h->d_nd, &h->d_dpos, &h->d_dfrom, &h->d_dto, &h->d_syn, &h->d_pos, &h->d_pi,
                       &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
                       &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
@@ -847,6 +892,8 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
     TRY(upload(h, h->d_pi, pp.data(), 64 * sizeof(double)));
     h->code_id = code_id;
     h->S = S;
+    h->pos_host.assign(pos_freqs, pos_freqs + 12);
+    h->pi_host = pp;
     h->have_model = true;
     h->warm_valid = false;
     return HY_OK;
@@ -980,10 +1027,8 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     std::vector<int32_t> blist;
     for (int b = 0; b < num_nodes; ++b) {
         if (b == root) continue;
-        blist.push_back(b);
-        blist.push_back(vslot[b]);
-        blist.push_back(ref_of(b));
-        blist.push_back(0);
+        const int32_t e[BR_STRIDE] = {b, vslot[b], ref_of(b), 0, 0, 0, 0, 0};
+        blist.insert(blist.end(), e, e + BR_STRIDE);
     }
     // --- tip codes ---
     std::vector<uint16_t> codes((size_t)num_taxa * std::max(num_patterns, 0));
@@ -1054,7 +1099,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     h->comp_mat = {0};
     refresh_blist_classes(h);
     h->warm_valid = false;
-    TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), 4) * 4));
+    TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), BR_STRIDE) * 4));
     TRY(upload(h, h->d_amb, h->amb.data(), h->amb.size() * 8));
     TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
     TRY(upload(h, h->d_comp_mat, h->comp_mat.data(), h->comp_mat.size() * 4));
@@ -1104,7 +1149,7 @@ int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_cl
     h->branch_class = bc;
     refresh_blist_classes(h);
     h->warm_valid = false;
-    TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), 4) * 4));
+    TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), BR_STRIDE) * 4));
     TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
     TRY(upload(h, h->d_comp_mat, h->comp_mat.data(), h->comp_mat.size() * 4));
     return HY_OK;
@@ -1124,7 +1169,7 @@ static int finish_host(hy_engine h, bool check_status, double *out_logL, double 
 
 // Result download of an eager evaluation (the graph path captures it).
 static int download_results(hy_engine h, bool check_status) {
-    const size_t nres = 1 + (size_t)h->num_nodes + h->n_mat + h->n_comp + 5;
+    const size_t nres = h->n_res;
     HIP_TRY(h, hipMemcpyAsync(h->h_res, h->d_res.p, nres * 8, hipMemcpyDeviceToHost, h->stream));
     if (check_status)
         HIP_TRY(h, hipMemcpyAsync(h->h_status, h->d_status.p, sizeof(int) * h->n_mat, hipMemcpyDeviceToHost, h->stream));
@@ -1424,6 +1469,421 @@ int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, do
     return HY_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Fused batches: the problems (genes) of a group's engines merged into one
+// problem of an internal engine, so that every stage is one launch over all
+// genes.  Kernels find each gene's slice through the gene table (tiles and
+// branches), matrix -> gene and tile -> gene maps; node, matrix, component,
+// class and branch indices are batch-global, partial and pattern indices
+// gene-local.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+bool fusable(const std::vector<hy_engine> &hs) {
+    if (hs.size() < 2) return false;
+    const hy_engine h0 = hs[0];
+    if (!h0->group || !h0->group->ctx) return false;
+    for (hy_engine h : hs)
+        if (!h || h->group != h0->group || !h->have_problem || !h->have_model || h->P == 0 || h->S != h0->S ||
+            h->code_id != h0->code_id || h->convention != h0->convention)
+            return false;
+    return true;
+}
+
+// Builds the merged problem of the engines hs in the group's context engine.
+int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
+    hy_engine c = gs.ctx;
+    const int esz = prec == HY_PREC_FP64 ? 8 : 4, ri = prec == HY_PREC_FP64 ? 1 : 0;
+    const int G = (int)hs.size();
+    // tile width: fewest rounds of resident workgroups x per-node cost
+    int T = 16;
+    {
+        const double node_cost[3] = {1.0, 1.3, 2.0};
+        double best = 1e300;
+        for (int k = 0; k < 3; ++k) {
+            const int Tk = 16 << k;
+            long tiles = 0;
+            for (hy_engine h : hs) tiles += (h->P + Tk - 1) / Tk;
+            const long slots = (long)c->n_cu * ((prec == HY_PREC_FP32 && Tk == 16) ? 2 : 1);
+            const double cost = (double)((tiles + slots - 1) / slots) * node_cost[k];
+            if (cost < best) {
+                best = cost;
+                T = Tk;
+            }
+        }
+    }
+    std::vector<GeneInfo> genes(G);
+    std::vector<int32_t> sched, osched, blist, comp_ptr{0}, comp_mat, tile_gene, mat_gene, wl;
+    std::vector<uint16_t> tips;
+    std::vector<uint64_t> amb;
+    std::vector<double> weights, pos, pi;
+    std::vector<uint8_t> ident;
+    gs.slices.assign(G, {});
+    long long tip_base = 0, d_base = 0, e_base = 0, v_base = 0, g_base = 0, u_base = 0;
+    int tile0 = 0, pbase = 0, node0 = 0, mat0 = 0, comp0 = 0, cls0 = 0, br0 = 0, max_taxa = 0, max_comp = 1;
+    size_t res_off = 0;
+    for (int g = 0; g < G; ++g) max_comp = std::max(max_comp, hs[g]->max_comp);
+    for (int g = 0; g < G; ++g) {
+        hy_engine h = hs[g];
+        // the member's own schedule state (host copy and the key telling its
+        // prepare_buffers that the device copy is current) is left untouched
+        std::vector<int32_t> own_sched, own_osched;
+        own_sched.swap(h->sched);
+        own_osched.swap(h->osched);
+        const int own_uspill = h->n_uspill, own_key = h->sched_key;
+        build_schedules(h, T, esz);
+        std::vector<int32_t> g_sched, g_osched;
+        g_sched.swap(h->sched);
+        g_osched.swap(h->osched);
+        const int g_uspill = h->n_uspill;
+        h->sched.swap(own_sched);
+        h->osched.swap(own_osched);
+        h->n_uspill = own_uspill;
+        h->sched_key = own_key;
+        const int ntiles = (h->P + T - 1) / T, P_pad = ntiles * T;
+        GeneInfo &gi = genes[g];
+        gi = GeneInfo{};
+        gi.tile0 = tile0;
+        gi.ntiles = ntiles;
+        gi.pbase = pbase;
+        gi.P_pad = P_pad;
+        gi.P = h->P;
+        gi.sched_off = (int)(sched.size() / SCHED_STRIDE);
+        gi.n_sched = (int)(g_sched.size() / SCHED_STRIDE);
+        gi.osched_off = (int)(osched.size() / OSCHED_STRIDE);
+        gi.n_osched = (int)(g_osched.size() / OSCHED_STRIDE);
+        gi.tip_rows = h->num_taxa + 1;
+        gi.tiles_per_chunk = ntiles;
+        gi.br0 = br0;
+        gi.n_br = h->n_branch;
+        gi.node0 = node0;
+        gi.n_real_nodes = h->num_nodes;
+        gi.mat0 = mat0;
+        gi.n_mat = h->n_mat;
+        gi.comp0 = comp0;
+        gi.n_comp = h->n_comp;
+        gi.cls0 = cls0;
+        gi.n_cls = h->n_class;
+        gi.res_off = (int)res_off;
+        gi.tip_base = tip_base;
+        gi.d_base = d_base;
+        gi.e_base = e_base;
+        gi.v_base = v_base;
+        gi.g_base = g_base;
+        gi.u_base = u_base;
+        gs.slices[g] = {node0, h->n_ext, h->num_nodes, mat0, h->n_mat, comp0, h->n_comp, pbase, h->P, res_off};
+        // schedules: edge (node) ids become batch-global
+        for (size_t i = 0; i < g_sched.size(); i += SCHED_STRIDE) {
+            int32_t e[SCHED_STRIDE];
+            std::copy(g_sched.begin() + i, g_sched.begin() + i + SCHED_STRIDE, e);
+            e[0] += node0;
+            for (int cc = 0; cc < e[2] && cc < MAXC; ++cc)
+                if (e[6 + 3 * cc] >= 0) e[6 + 3 * cc] += node0;
+            sched.insert(sched.end(), e, e + SCHED_STRIDE);
+        }
+        for (size_t i = 0; i < g_osched.size(); i += OSCHED_STRIDE) {
+            int32_t e[OSCHED_STRIDE];
+            std::copy(g_osched.begin() + i, g_osched.begin() + i + OSCHED_STRIDE, e);
+            e[0] += node0;
+            for (int cc = 0; cc < e[2] && cc < MAXC; ++cc) e[6 + 4 * cc] += node0;
+            osched.insert(osched.end(), e, e + OSCHED_STRIDE);
+        }
+        for (int b = 0; b < h->n_branch; ++b) {
+            int32_t e[BR_STRIDE];
+            std::copy(h->blist.begin() + (size_t)b * BR_STRIDE, h->blist.begin() + (size_t)(b + 1) * BR_STRIDE, e);
+            e[0] += node0;
+            e[3] += cls0;
+            e[4] = g;
+            blist.insert(blist.end(), e, e + BR_STRIDE);
+        }
+        for (int cc = 0; cc < h->n_class; ++cc) comp_ptr.push_back(comp0 + h->comp_ptr[cc + 1]);
+        for (int j = 0; j < h->n_comp; ++j) comp_mat.push_back(mat0 + h->comp_mat[j]);
+        // tip codes [taxa + 1][P_pad], ambiguity indices batch-global
+        const int amb0 = (int)amb.size();
+        if (amb0 + (int)h->amb.size() >= 0x7FFF) return fail(c, HY_ERR_UNSUPPORTED, "batch: more than 32766 ambiguity masks");
+        amb.insert(amb.end(), h->amb.begin(), h->amb.end());
+        const size_t t0 = tips.size();
+        tips.resize(t0 + (size_t)(h->num_taxa + 1) * P_pad, TIP_ONES);
+        for (int t = 0; t < h->num_taxa; ++t)
+            for (int p = 0; p < h->P; ++p) {
+                uint16_t code = h->codes[(size_t)t * h->P + p];
+                if (code != TIP_ONES && (code & TIP_AMB)) code = (uint16_t)(TIP_AMB | ((code & 0x7FFF) + amb0));
+                tips[t0 + (size_t)t * P_pad + p] = code;
+            }
+        weights.resize(pbase + P_pad, 0.0);
+        std::copy(h->weights.begin(), h->weights.end(), weights.begin() + pbase);
+        for (int m = 0; m < h->n_mat; ++m) mat_gene.push_back(g);
+        for (int t = 0; t < ntiles; ++t) tile_gene.push_back(g);
+        pos.insert(pos.end(), h->pos_host.begin(), h->pos_host.end());
+        pi.insert(pi.end(), h->pi_host.begin(), h->pi_host.end());
+        ident.insert(ident.end(), h->ext_ident.begin(), h->ext_ident.end());
+        max_taxa = std::max(max_taxa, h->num_taxa);
+        tile0 += ntiles;
+        pbase += P_pad;
+        node0 += h->n_ext;
+        mat0 += h->n_mat;
+        comp0 += h->n_comp;
+        cls0 += h->n_class;
+        br0 += h->n_branch;
+        tip_base += (long long)(h->num_taxa + 1) * P_pad;
+        d_base += (long long)h->n_int * ntiles * 64 * T;
+        e_base += (long long)h->n_int * P_pad;
+        v_base += (long long)h->n_vslot * ntiles * 64 * T;
+        g_base += (long long)h->n_vslot * P_pad;
+        if (g_uspill) u_base += (long long)h->n_int * ntiles * 64 * T;
+        res_off += 1 + (size_t)h->num_nodes + h->n_mat + h->n_comp + 5;
+    }
+    // Host-side bounds check of every index the kernels will follow through
+    // the merged tables (gene-local partial / tip / slot indices, batch-global
+    // nodes, classes, components and matrices); nothing is launched otherwise.
+    {
+        const int NSI = inside_slots(T, esz), NSO = outside_slots(T, esz);
+        auto bad = [&](int g, const char *what) { return fail(c, HY_ERR_STATE, "batch tables: gene %d: %s", g, what); };
+        for (int g = 0; g < G; ++g) {
+            const hy_engine h = hs[g];
+            const GeneInfo &gi = genes[g];
+            auto ref_ok = [&](int32_t ref) {
+                const int kind = ref >> 28, idx = ref & REF_MASK;
+                return kind == KIND_TIP ? idx < gi.tip_rows : (kind == KIND_INT && idx < h->n_int);
+            };
+            for (int k = 0; k < gi.n_sched; ++k) {
+                const int32_t *e = &sched[(size_t)(gi.sched_off + k) * SCHED_STRIDE];
+                if (e[1] < 0 || e[1] >= h->n_int || e[2] < 1 || e[2] > MAXC || e[3] >= NSI) return bad(g, "inside entry");
+                for (int cc = 0; cc < e[2]; ++cc) {
+                    const int pn = e[6 + 3 * cc];
+                    if (pn != -1 && (pn < gi.node0 || pn >= gi.node0 + h->n_ext)) return bad(g, "inside edge");
+                    if (!ref_ok(e[7 + 3 * cc]) || e[8 + 3 * cc] >= NSI) return bad(g, "inside child");
+                }
+            }
+            for (int k = 0; k < gi.n_osched; ++k) {
+                const int32_t *e = &osched[(size_t)(gi.osched_off + k) * OSCHED_STRIDE];
+                if (e[1] < 0 || e[1] >= h->n_int || e[2] < 1 || e[2] > MAXC || e[3] >= h->n_vslot || e[5] >= NSO)
+                    return bad(g, "outside entry");
+                for (int cc = 0; cc < e[2]; ++cc) {
+                    const int pn = e[6 + 4 * cc];
+                    if (pn < gi.node0 || pn >= gi.node0 + h->n_ext) return bad(g, "outside edge");
+                    if (!ref_ok(e[7 + 4 * cc]) || e[8 + 4 * cc] < 0 || e[8 + 4 * cc] >= h->n_vslot || e[9 + 4 * cc] >= NSO)
+                        return bad(g, "outside child");
+                }
+            }
+            for (int b = 0; b < gi.n_br; ++b) {
+                const int32_t *e = &blist[(size_t)(gi.br0 + b) * BR_STRIDE];
+                if (e[0] < gi.node0 || e[0] >= gi.node0 + h->n_ext || e[1] < 0 || e[1] >= h->n_vslot || !ref_ok(e[2]) ||
+                    e[3] < gi.cls0 || e[3] >= gi.cls0 + gi.n_cls || e[4] != g)
+                    return bad(g, "branch entry");
+            }
+        }
+        for (size_t k = 0; k + 1 < comp_ptr.size(); ++k)
+            if (comp_ptr[k] < 0 || comp_ptr[k + 1] <= comp_ptr[k] || comp_ptr[k + 1] > comp0) return bad(-1, "class components");
+        for (int32_t m : comp_mat)
+            if (m < 0 || m >= mat0) return bad(-1, "component matrix");
+        for (uint16_t code : tips)
+            if (code != TIP_ONES && (code & TIP_AMB) && (size_t)(code & 0x7FFF) >= amb.size()) return bad(-1, "tip code");
+        if ((int)tile_gene.size() != tile0 || (int)mat_gene.size() != mat0) return bad(-1, "maps");
+    }
+    // per matrix: the W slots (branch * max_comp + component index) using it
+    {
+        std::vector<std::vector<int32_t>> per(mat0);
+        for (int bi = 0; bi < br0; ++bi) {
+            const int cls = blist[(size_t)bi * BR_STRIDE + 3];
+            for (int j = comp_ptr[cls]; j < comp_ptr[cls + 1]; ++j) per[comp_mat[j]].push_back(bi * max_comp + (j - comp_ptr[cls]));
+        }
+        wl.assign(mat0 + 1, 0);
+        for (int m = 0; m < mat0; ++m) wl[m + 1] = wl[m] + (int32_t)per[m].size();
+        for (int m = 0; m < mat0; ++m) wl.insert(wl.end(), per[m].begin(), per[m].end());
+    }
+    // context engine state
+    hy_engine h0 = hs[0];
+    c->S = h0->S;
+    c->code_id = h0->code_id;
+    c->convention = h0->convention;
+    c->n_genes = G;
+    c->T = T;
+    c->T_req = T;
+    c->n_tiles = tile0;
+    c->P_pad = pbase;
+    c->P = pbase;
+    c->num_taxa = max_taxa;
+    c->n_ext = node0;
+    c->num_nodes = node0;
+    c->n_mat = mat0;
+    c->n_comp = comp0;
+    c->n_class = cls0;
+    c->max_comp = max_comp;
+    c->n_branch = br0;
+    c->n_chunks = 1;
+    c->tiles_per_chunk = 1;
+    c->sched = sched;
+    c->osched = osched;
+    c->blist = blist;
+    c->ext_ident = ident;
+    c->n_res = res_off;
+    c->warm_valid = false;
+    c->have_problem = c->have_model = true;
+    {
+        ParamLayout &pl = c->pl;
+        pl.n_nodes = node0;
+        pl.n_mat = mat0;
+        pl.n_comp = comp0;
+        int o = 0;
+        pl.off_bl = o;
+        o += node0;
+        pl.off_alpha = o;
+        o += G;
+        pl.off_omega = o;
+        o += mat0;
+        pl.off_w = o;
+        o += comp0;
+        pl.off_theta = o;
+        o += 5 * G;
+        pl.off_delta = o;
+        o += G;
+        pl.off_psi = o;
+        o += G;
+        pl.total = o;
+    }
+    const size_t tile_elems = (size_t)64 * T;
+    TRY(upload(c, c->d_genes, genes.data(), genes.size() * sizeof(GeneInfo)));
+    TRY(upload(c, c->d_tile_gene, tile_gene.data(), std::max<size_t>(tile_gene.size(), 1) * 4));
+    TRY(upload(c, c->d_mat_gene, mat_gene.data(), mat_gene.size() * 4));
+    TRY(upload(c, c->d_sched, sched.data(), sched.size() * 4));
+    TRY(upload(c, c->d_osched, osched.data(), osched.size() * 4));
+    TRY(upload(c, c->d_blist, blist.data(), std::max<size_t>(blist.size(), BR_STRIDE) * 4));
+    TRY(upload(c, c->d_comp_ptr, comp_ptr.data(), comp_ptr.size() * 4));
+    TRY(upload(c, c->d_comp_mat, comp_mat.data(), comp_mat.size() * 4));
+    TRY(upload(c, c->d_tips, tips.data(), std::max<size_t>(tips.size(), 1) * 2));
+    TRY(upload(c, c->d_amb, amb.data(), std::max<size_t>(amb.size(), 1) * 8));
+    TRY(upload(c, c->d_weights, weights.data(), std::max<size_t>(weights.size(), 1) * 8));
+    TRY(upload(c, c->d_pos, pos.data(), pos.size() * 8));
+    TRY(upload(c, c->d_pi, pi.data(), pi.size() * 8));
+    TRY(upload(c, c->d_wl, wl.data(), wl.size() * 4));
+    c->wl = wl;
+    // genetic-code tables of the first gene (all genes share the code)
+    DevBuf *src[6] = {&h0->d_qdesc, &h0->d_nd, &h0->d_dpos, &h0->d_dfrom, &h0->d_dto, &h0->d_syn};
+    DevBuf *dst[6] = {&c->d_qdesc, &c->d_nd, &c->d_dpos, &c->d_dfrom, &c->d_dto, &c->d_syn};
+    for (int k = 0; k < 6; ++k) {
+        TRY(ensure(c, *dst[k], src[k]->bytes));
+        HIP_TRY(c, hipMemcpy(dst[k]->p, src[k]->p, src[k]->bytes, hipMemcpyDeviceToDevice));
+    }
+    TRY(ensure(c, c->d_D[ri], (size_t)std::max<long long>(d_base, 1) * esz));
+    TRY(ensure(c, c->d_Vp[ri], (size_t)std::max<long long>(v_base, 1) * esz));
+    TRY(ensure(c, c->d_Uspill[ri], (size_t)std::max<long long>(u_base, 1) * esz));
+    TRY(ensure(c, c->d_E, (size_t)std::max<long long>(e_base, 1) * 4));
+    TRY(ensure(c, c->d_G, (size_t)std::max<long long>(g_base, 1) * 4));
+    TRY(ensure(c, c->d_Lroot, (size_t)pbase * 8));
+    TRY(ensure(c, c->d_Eroot, (size_t)pbase * 4));
+    TRY(ensure(c, c->d_site, (size_t)pbase * 8));
+    TRY(ensure(c, c->d_Prow[ri], (size_t)node0 * 4096 * esz));
+    TRY(ensure(c, c->d_PA[ri], (size_t)node0 * 4096 * esz));
+    TRY(ensure(c, c->d_PTA[ri], (size_t)node0 * 4096 * esz));
+    TRY(ensure(c, c->d_Rsum[ri], (size_t)node0 * 64 * esz));
+    TRY(upload_identity_tiles(c, ri, esz));
+    c->ident_tiles[ri] = c->d_PA[ri].p;
+    TRY(ensure(c, c->d_tau, (size_t)node0 * 8));
+    TRY(ensure(c, c->d_lam, (size_t)mat0 * 64 * 8));
+    TRY(ensure(c, c->d_U, (size_t)mat0 * 4096 * 8));
+    TRY(ensure(c, c->d_V, (size_t)mat0 * 4096 * 8));
+    TRY(ensure(c, c->d_Z, (size_t)mat0 * 4096 * 8));
+    TRY(ensure(c, c->d_mu, (size_t)mat0 * 8));
+    TRY(ensure(c, c->d_status, (size_t)mat0 * 4));
+    TRY(ensure(c, c->d_M, (size_t)mat0 * 6 * 4096 * 8));
+    TRY(ensure(c, c->d_dmu, (size_t)mat0 * 6 * 8));
+    TRY(ensure(c, c->d_Apart, (size_t)br0 * 4096 * 8));
+    TRY(ensure(c, c->d_fr, (size_t)br0 * max_comp * 8 * 8));
+    TRY(ensure(c, c->d_Asum, (size_t)br0 * 4096 * 8));
+    TRY(ensure(c, c->d_W, (size_t)br0 * max_comp * 4096 * 8));
+    TRY(ensure(c, c->d_proj, (size_t)mat0 * 6 * HPROJ_CHUNKS * 8));
+    TRY(ensure(c, c->d_params, (size_t)c->pl.total * 8));
+    TRY(ensure(c, c->d_res, res_off * 8));
+    (void)tile_elems;
+    if (c->h_params_cap < (size_t)c->pl.total) {
+        if (c->h_params) (void)hipHostFree(c->h_params);
+        c->h_params = nullptr;
+        c->h_params_cap = 0;
+        HIP_TRY(c, hipHostMalloc((void **)&c->h_params, sizeof(double) * c->pl.total, hipHostMallocDefault));
+        c->h_params_cap = c->pl.total;
+    }
+    if (c->h_res_cap < res_off) {
+        if (c->h_res) (void)hipHostFree(c->h_res);
+        c->h_res = nullptr;
+        c->h_res_cap = 0;
+        HIP_TRY(c, hipHostMalloc((void **)&c->h_res, sizeof(double) * res_off, hipHostMallocDefault));
+        c->h_res_cap = res_off;
+    }
+    if (c->h_status) (void)hipHostFree(c->h_status);
+    c->h_status = nullptr;
+    HIP_TRY(c, hipHostMalloc((void **)&c->h_status, sizeof(int) * std::max(mat0, 64), hipHostMallocDefault));
+    if (c->h_site_cap < (size_t)pbase) {
+        if (c->h_site) (void)hipHostFree(c->h_site);
+        c->h_site = nullptr;
+        c->h_site_cap = 0;
+        HIP_TRY(c, hipHostMalloc((void **)&c->h_site, sizeof(double) * pbase, hipHostMallocDefault));
+        c->h_site_cap = pbase;
+    }
+    ++c->epoch;
+    gs.members = hs;
+    gs.epochs.clear();
+    for (hy_engine h : hs) gs.epochs.push_back(h->epoch);
+    gs.prec = prec;
+    return HY_OK;
+}
+
+int eval_fused(GroupState &gs, hy_batch_item *items, int n, int prec, unsigned flags) {
+    hy_engine c = gs.ctx;
+    const bool grad = (flags & HY_WANT_GRAD) != 0;
+    bool want_site = false;
+    for (int i = 0; i < n; ++i) want_site |= items[i].out_site_ll != nullptr;
+    HIP_TRY(c, hipSetDevice(c->device));
+    double *hp = c->h_params;
+    const ParamLayout &pl = c->pl;
+    for (int g = 0; g < n; ++g) {
+        const hy_batch_item &it = items[g];
+        const GroupState::Slice &sl = gs.slices[g];
+        for (int k = 0; k < sl.n_ext; ++k) hp[pl.off_bl + sl.node0 + k] = k < sl.n_real ? it.bl[k] : 0.0;
+        hp[pl.off_alpha + g] = it.alpha;
+        for (int m = 0; m < sl.n_mat; ++m) hp[pl.off_omega + sl.mat0 + m] = it.omegas[m];
+        for (int j = 0; j < sl.n_comp; ++j) hp[pl.off_w + sl.comp0 + j] = it.comp_w[j];
+        for (int k = 0; k < 5; ++k) hp[pl.off_theta + 5 * g + k] = it.theta[k];
+        hp[pl.off_delta + g] = it.delta;
+        hp[pl.off_psi + g] = it.psi;
+    }
+    if (c->use_graphs && !c->profile) {
+        TRY(launch_graph(c, prec, grad));
+    } else {
+        HIP_TRY(c, hipMemcpyAsync(c->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, c->stream));
+        const int rc = prec == HY_PREC_FP64 ? run_eval<double>(c, flags, nullptr) : run_eval<float>(c, flags, nullptr);
+        if (rc) return rc;
+        TRY(download_results(c, true));
+    }
+    if (want_site)
+        HIP_TRY(c, hipMemcpyAsync(c->h_site, c->d_site.p, sizeof(double) * c->P_pad, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    TRY(finish_host(c, true, nullptr, nullptr, nullptr, nullptr, nullptr, false));
+    for (int g = 0; g < n; ++g) {
+        hy_batch_item &it = items[g];
+        const GroupState::Slice &sl = gs.slices[g];
+        const double *r = c->h_res + sl.res_off;
+        if (it.out_logL) *it.out_logL = r[0];
+        if (it.out_site_ll) std::memcpy(it.out_site_ll, c->h_site + sl.pbase, sizeof(double) * sl.P);
+        if (grad) {
+            const double *gbl = r + 1, *gom = gbl + sl.n_real, *gw = gom + sl.n_mat, *gth = gw + sl.n_comp;
+            if (it.out_grad_bl) std::memcpy(it.out_grad_bl, gbl, sizeof(double) * sl.n_real);
+            if (it.out_grad_omega) std::memcpy(it.out_grad_omega, gom, sizeof(double) * sl.n_mat);
+            if (it.out_grad_w) std::memcpy(it.out_grad_w, gw, sizeof(double) * sl.n_comp);
+            if (it.out_grad_theta) std::memcpy(it.out_grad_theta, gth, sizeof(double) * 5);
+        }
+        it.status = HY_OK;
+    }
+    return HY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned flags) {
     // Every item's evaluation is queued on its own handle's stream before any
     // is waited for, so independent genes overlap on the device.
@@ -1436,6 +1896,29 @@ int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned fla
         if (std::adjacent_find(hs.begin(), hs.end()) != hs.end()) {
             for (int i = 0; i < n_items; ++i) items[i].status = HY_ERR_ARG;
             return HY_ERR_ARG;  // a handle holds one pending evaluation at a time
+        }
+    }
+    {
+        // one fused evaluation when every item is a distinct engine of one group
+        std::vector<hy_engine> hs;
+        bool ok = true;
+        for (int i = 0; i < n_items && ok; ++i) {
+            hy_batch_item &it = items[i];
+            ok = it.h && it.bl && it.omegas && it.comp_w && it.theta;
+            if (ok) hs.push_back(it.h);
+        }
+        if (ok && (precision == HY_PREC_FP64 || precision == HY_PREC_FP32) && fusable(hs) && !hs[0]->group->no_fuse) {
+            GroupState &gs = *hs[0]->group;
+            bool same = gs.members == hs && gs.prec == precision;
+            for (size_t i = 0; same && i < hs.size(); ++i) same = gs.epochs[i] == hs[i]->epoch;
+            int rc = same ? HY_OK : build_batch(gs, hs, precision);
+            if (rc == HY_OK) rc = eval_fused(gs, items, n_items, precision, flags);
+            if (rc != HY_OK) {
+                gs.members.clear();  // rebuild on the next call
+                for (int i = 0; i < n_items; ++i) items[i].status = rc;
+                hs[0]->err = gs.ctx->err;
+            }
+            return rc;
         }
     }
     int first = HY_OK;

@@ -51,8 +51,24 @@ constexpr int tip_lds_codes(int T, int esz) {
     return T == 16 ? 12288 : (esz == 4 ? 24576 : (T == 32 ? 16384 : 8192));
 }
 
-// Branch list entry (int32 x 4): node, V slot, child reference, class.
-constexpr int BR_STRIDE = 4;
+// Branch list entry: [0] node, [1] outside slot, [2] child reference (kind | index),
+// [3] mixture class, [4] gene (batched evaluations; 0 otherwise), [5..7] unused.
+constexpr int BR_STRIDE = 8;
+
+// One problem (gene) of a fused batched evaluation: its slice of every
+// batch-wide array.  Single-problem evaluations pass no table.
+struct GeneInfo {
+    int tile0, ntiles;            // pruning tiles [tile0, tile0 + ntiles)
+    int pbase, P_pad, P;          // pattern offset, padded and real pattern count
+    int sched_off, n_sched;       // inside schedule entries
+    int osched_off, n_osched;     // outside schedule entries
+    int tip_rows, tiles_per_chunk;
+    int br0, n_br;                // branches [br0, br0 + n_br) of the branch list
+    int node0, n_real_nodes;      // first extended node; nodes of the caller's tree
+    int mat0, n_mat, comp0, n_comp, cls0, n_cls;
+    int res_off, pad0;
+    long long tip_base, d_base, e_base, v_base, g_base, u_base;  // element offsets
+};
 
 // Per-evaluation parameter block (device, FP64), offsets in doubles.
 struct ParamLayout {
@@ -89,6 +105,7 @@ struct ModelArgs {
     double *Z;    // [n_mat][64][64] eigenvectors of the symmetrised generator (warm-start state)
     int warm;     // 1: refine Z from the previous call
     int *status;  // [n_mat] Jacobi sweeps, 100+k after k refinement steps, negative = no convergence
+    const int32_t *mat_gene;  // batched evaluation: gene of each matrix (pi, pos, alpha, theta, delta, psi)
 };
 
 template <typename R>
@@ -129,6 +146,8 @@ struct PruneArgs {
     double *Lroot; // [P_pad]
     int32_t *Eroot;
     double *site_ll;
+    const GeneInfo *genes;  // batched evaluation: gene table and tile -> gene map
+    const int32_t *tile_gene;
 };
 
 template <typename R>
@@ -142,6 +161,7 @@ struct AdjArgs {
     const uint64_t *amb;
     int P_pad, n_tiles, S;
     double *Apart;  // [n_branch][n_chunks][64][64]
+    const GeneInfo *genes;  // batched evaluation (branch gene in the branch list)
 };
 
 struct ProjArgs {
@@ -154,6 +174,7 @@ struct ProjArgs {
     const double *U, *V;
     double *M;    // [n_mat][6][64][64]
     double *dmu;  // [n_mat][6]
+    const int32_t *mat_gene;  // batched evaluation: gene of each matrix
 };
 
 constexpr int HPROJ_CHUNKS = 16;  // 4096 entries / 256 threads
@@ -184,6 +205,7 @@ struct FinalArgs {
     ParamLayout pl;
     const double *fr, *tau, *mu, *dmu, *proj;
     double *res;  // [0] lnL, [1..n_real] grad_bl, then omega[n_mat], w[n_comp], theta[5]
+    const GeneInfo *genes;  // batched evaluation: one workgroup per gene, results at res_off
 };
 
 // Launchers (hy_kernels.hip).
@@ -195,6 +217,6 @@ template <typename R> void launch_outside(const PruneArgs<R> &a, int T, hipStrea
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s);
 void launch_proj(const ProjArgs &a, int n_mat, hipStream_t s);
 void launch_frechet(const FrechetArgs &a, hipStream_t s);  // chunk sums, spectral transforms, projections
-void launch_final(const FinalArgs &a, hipStream_t s);
+void launch_final(const FinalArgs &a, int n_genes, hipStream_t s);  // one workgroup per gene
 
 }  // namespace hy

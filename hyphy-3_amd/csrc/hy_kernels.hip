@@ -177,7 +177,23 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red, in
 }
 
 constexpr int MODEL_NT = 1024;
-__global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs a) {
+// Batched evaluations: matrix m belongs to gene mat_gene[m], whose
+// frequencies and scalar parameters (alpha, theta, delta, psi) it uses.
+template <typename Args>
+__device__ __forceinline__ void matrix_gene_view(Args &a, int m) {
+    if (!a.mat_gene) return;
+    const int g = a.mat_gene[m];
+    a.pi += 64 * g;
+    a.pos += 12 * g;
+    a.pl.off_alpha += g;
+    a.pl.off_theta += 5 * g;
+    a.pl.off_delta += g;
+    a.pl.off_psi += g;
+}
+
+__global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
+    ModelArgs a = args;
+    matrix_gene_view(a, blockIdx.x);
     constexpr int NT = MODEL_NT;
     const int m = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     __shared__ double A[64 * LDA];
@@ -496,12 +512,11 @@ __device__ __forceinline__ void emit_p(const double *P, int node, int S, R *Prow
 // k_pmat: mixed transition matrices per branch.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_pmat(PmatArgs<R> a) {
-    const int bi = blockIdx.x, tid = threadIdx.x;
+__global__ __launch_bounds__(256, 2) void k_pmat(PmatArgs<R> a) {
+    const int bi = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
-    __shared__ double P[64 * 64];
-    __shared__ double Us[64 * 64];
-    __shared__ double Vs[64 * 64];
+    __shared__ double Us[64 * LDA];  // U diag(e^{lambda tau}); the mixed P at the end
+    __shared__ double Vs[64 * LDA];
     __shared__ double ex[64];
     const double *prm = a.params;
     const int j0 = a.comp_ptr[cls], j1 = a.comp_ptr[cls + 1];
@@ -514,36 +529,35 @@ __global__ __launch_bounds__(256) void k_pmat(PmatArgs<R> a) {
     }
     const double tau = prm[a.pl.off_bl + node] * conv;
     if (tid == 0) a.tau[node] = tau;
-    const int i0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
-    double acc[4][4];
-    for (int ii = 0; ii < 4; ++ii)
-        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = 0.0;
+    // P = sum_j w_j max(U diag(e^{lambda tau}) V, 0): one 64^3 FP64 MFMA product
+    // per component, wave w owning rows 16w..16w+15
+    f64x4 Pacc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Pacc[q] = splat<f64x4>(0.0);
     for (int j = j0; j < j1; ++j) {
         const int m = a.comp_mat[j];
-        const double w = prm[a.pl.off_w + j];
-        __syncthreads();
+        const double wj = prm[a.pl.off_w + j];
+        __syncthreads();  // the previous component's operand reads are done
         if (tid < 64) ex[tid] = exp(a.lam[m * 64 + tid] * tau);
         __syncthreads();
         for (int idx = tid; idx < 4096; idx += 256) {
-            Us[idx] = a.U[m * 4096 + idx] * ex[idx & 63];
-            Vs[idx] = a.V[m * 4096 + idx];
+            Us[(idx >> 6) * LDA + (idx & 63)] = a.U[m * 4096 + idx] * ex[idx & 63];
+            Vs[(idx >> 6) * LDA + (idx & 63)] = a.V[m * 4096 + idx];
         }
         __syncthreads();
-        double t[4][4];
-        for (int ii = 0; ii < 4; ++ii)
-            for (int jj = 0; jj < 4; ++jj) t[ii][jj] = 0.0;
-        for (int k = 0; k < 64; ++k) {
-            double u[4], v[4];
-            for (int ii = 0; ii < 4; ++ii) u[ii] = Us[(i0 + ii) * 64 + k];
-            for (int jj = 0; jj < 4; ++jj) v[jj] = Vs[k * 64 + c0 + jj];
-            for (int ii = 0; ii < 4; ++ii)
-                for (int jj = 0; jj < 4; ++jj) t[ii][jj] = fma(u[ii], v[jj], t[ii][jj]);
-        }
-        for (int ii = 0; ii < 4; ++ii)
-            for (int jj = 0; jj < 4; ++jj) acc[ii][jj] += w * (t[ii][jj] > 0.0 ? t[ii][jj] : 0.0);
+        f64x4 acc[4];
+        gemm64_acc<false, false>(Us, Vs, acc, w, lr, lg);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Pacc[q][r] += wj * (acc[q][r] > 0.0 ? acc[q][r] : 0.0);
     }
-    for (int ii = 0; ii < 4; ++ii)
-        for (int jj = 0; jj < 4; ++jj) P[(i0 + ii) * 64 + c0 + jj] = acc[ii][jj];
+    __syncthreads();
+    double *P = Us;  // row-major 64 x 64 (stride 64) in the operand buffer
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) P[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] = Pacc[q][r];
     __syncthreads();
     emit_p<R>(P, node, a.S, a.Prow, a.Rsum, a.PA, a.PTA, tid);
 }
@@ -841,6 +855,35 @@ __device__ __forceinline__ void stage_tip_codes(uint16_t *tcodes, const uint16_t
         for (int idx = tid; idx < rows * T; idx += 256) tcodes[idx] = tips[(size_t)(idx / T) * P_pad + p0 + idx % T];
 }
 
+// Batched evaluations: the workgroup's tile belongs to one gene; shift every
+// per-problem pointer and size of the arguments to that gene's slice and
+// return the gene-local tile index.  Single problems pass no table.
+template <typename R>
+__device__ __forceinline__ int gene_view(PruneArgs<R> &a, int tile) {
+    if (!a.genes) return tile;
+    const int gi = a.tile_gene[tile];
+    const GeneInfo g = a.genes[gi];
+    a.sched += (size_t)g.sched_off * SCHED_STRIDE;
+    a.n_sched = g.n_sched;
+    a.osched += (size_t)g.osched_off * OSCHED_STRIDE;
+    a.n_osched = g.n_osched;
+    a.tip_rows = g.tip_rows;
+    a.tips += g.tip_base;
+    a.D += g.d_base;
+    a.E += g.e_base;
+    a.V += g.v_base;
+    a.G += g.g_base;
+    a.Uspill += g.u_base;
+    a.Lroot += g.pbase;
+    a.Eroot += g.pbase;
+    a.site_ll += g.pbase;
+    a.pi += 64 * gi;
+    a.P_pad = g.P_pad;
+    a.n_tiles = g.ntiles;
+    a.n_patterns = g.P;
+    return tile - g.tile0;
+}
+
 // ---------------------------------------------------------------------------
 // k_inside: post-order pruning of one pattern tile.  Child partials stay in
 // LDS slots assigned by the host schedule (read straight from HBM for nodes
@@ -851,7 +894,9 @@ __device__ __forceinline__ void stage_tip_codes(uint16_t *tcodes, const uint16_t
 // TL: the tile's tip codes are staged in LDS.
 // ---------------------------------------------------------------------------
 template <typename R, int T, bool TL>
-__global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_inside(PruneArgs<R> a) {
+__global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_inside(PruneArgs<R> args) {
+    PruneArgs<R> a = args;
+    const int tile = gene_view(a, blockIdx.x);
     constexpr int NB = T / 16, NS = inside_slots(T, sizeof(R)), TIPC = TL ? tip_lds_codes(T, sizeof(R)) : 1;
     constexpr bool PIPE = !(sizeof(R) == 8 && T == 64);  // register budget of double-buffered operands
     typedef Entry<SCHED_STRIDE> Ent;
@@ -861,7 +906,7 @@ __global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_in
     __shared__ R cmax[2][4 * T];
     __shared__ double red[4 * T];
     __shared__ uint16_t tcodes[TIPC];
-    const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
     const int p0 = tile * T, n = a.n_sched;
     stage_tip_codes<T, TL>(tcodes, a.tips, a.tip_rows, a.P_pad, p0, tid);
     const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
@@ -992,7 +1037,9 @@ __global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_in
 // B operands, otherwise they are read from HBM when used.
 // ---------------------------------------------------------------------------
 template <typename R, int T, bool TL>
-__global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
+__global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
+    PruneArgs<R> a = args;
+    const int tile = gene_view(a, blockIdx.x);
     constexpr int NB = T / 16, NS = outside_slots(T, sizeof(R)), TIPC = TL ? tip_lds_codes(T, sizeof(R)) : 1;
     constexpr bool DREG = NB * sizeof(R) <= 8;  // f32 T <= 32, f64 T == 16
     constexpr int DV = DREG ? 16 * NB : 1;
@@ -1003,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> a) {
     __shared__ R Xc[MAXC * 64 * T];
     __shared__ R cmax[2][MAXC * 4 * T];
     __shared__ uint16_t tcodes[TIPC];
-    const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
     const int p0 = tile * T, n = a.n_osched;
     stage_tip_codes<T, TL>(tcodes, a.tips, a.tip_rows, a.P_pad, p0, tid);
     const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
@@ -1246,7 +1293,8 @@ __device__ __forceinline__ R tip_onehot(unsigned code, int j, const uint64_t *am
 }
 
 template <typename R, int T>
-__global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> a) {
+__global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
+    AdjArgs<R> a = args;
     // Works on groups of 64 patterns (G pruning tiles, contiguous in HBM):
     // the next group is fetched into registers while the current one is
     // multiplied from LDS (double-buffered).
@@ -1259,6 +1307,20 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> a) {
     const int chunk = blockIdx.x, bi = blockIdx.y;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
     const int32_t *br = a.blist + bi * BR_STRIDE;
+    if (a.genes) {  // batched evaluation: this branch's gene slice
+        const GeneInfo g = a.genes[br[4]];
+        a.D += g.d_base;
+        a.V += g.v_base;
+        a.E += g.e_base;
+        a.G += g.g_base;
+        a.Eroot += g.pbase;
+        a.Lroot += g.pbase;
+        a.weights += g.pbase;
+        a.tips += g.tip_base;
+        a.P_pad = g.P_pad;
+        a.n_tiles = g.ntiles;
+        a.tiles_per_chunk = g.tiles_per_chunk;
+    }
     const int vslot = br[1], ref = br[2];
     const int kind = ref >> 28, idx = ref & REF_MASK;
     const bool tip = kind == KIND_TIP;
@@ -1353,7 +1415,9 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> a) {
 // k_proj: M_phi = V_pi dQ_phi U_pi and dmu_phi = -sum_i pi_i dQ_phi[i][i]
 // (likelihood.hpp:1497-1562).  phi 0 = omega, 1..5 = theta AC, AT, CG, CT, GT.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_proj(ProjArgs a) {
+__global__ __launch_bounds__(256) void k_proj(ProjArgs args) {
+    ProjArgs a = args;
+    matrix_gene_view(a, blockIdx.x);
     const int m = blockIdx.x, phi = blockIdx.y, tid = threadIdx.x;
     __shared__ double dQ[64 * 64];
     __shared__ double T1[64 * 64];
@@ -1561,8 +1625,34 @@ __device__ __forceinline__ void comp_add(double &s, double &c, double s2, double
 }
 
 constexpr int FINAL_NT = 1024;
-__global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs a) {
+__global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
+    FinalArgs a = args;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // batched evaluation: workgroup g reduces gene g (its patterns, branches,
+    // classes, components and matrices); global indices in the branch list and
+    // component tables are offset by node0 / cls0 / comp0 / mat0
+    int node0 = 0, cls0 = 0, comp0 = 0, mat0 = 0;
+    if (a.genes) {
+        const GeneInfo g = a.genes[blockIdx.x];
+        a.site_ll += g.pbase;
+        a.weights += g.pbase;
+        a.Lroot += g.pbase;
+        a.n_patterns = g.P;
+        a.res += g.res_off;
+        a.blist += (size_t)g.br0 * BR_STRIDE;
+        a.fr += (size_t)g.br0 * a.max_comp * 8;
+        a.n_branch = g.n_br;
+        a.n_real_nodes = g.n_real_nodes;
+        a.comp_ptr += g.cls0;
+        a.n_class = g.n_cls;
+        a.proj += (size_t)g.mat0 * 6 * HPROJ_CHUNKS;
+        a.pl.n_mat = g.n_mat;
+        a.pl.n_comp = g.n_comp;
+        node0 = g.node0;
+        cls0 = g.cls0;
+        comp0 = g.comp0;
+        mat0 = g.mat0;
+    }
     __shared__ double wsum[16], wcmp[16];
     __shared__ double sum_tau[64], mubar[64];
     const bool grad = a.want_grad != 0;
@@ -1607,7 +1697,7 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs a) {
         for (int cl = gwave; cl < a.n_class; cl += NGW) {
             double v = 0.0;
             for (int bi = lane; bi < a.n_branch; bi += 64) {
-                if (a.blist[bi * BR_STRIDE + 3] != cl) continue;
+                if (a.blist[bi * BR_STRIDE + 3] != cls0 + cl) continue;
                 double dt = 0.0;
                 const int nj = a.comp_ptr[cl + 1] - a.comp_ptr[cl];  // components of this branch's class
                 for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
@@ -1617,12 +1707,12 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs a) {
             if (lane == 0) sum_tau[cl] = v;
         }
         for (int bi = gt; bi < a.n_branch; bi += FINAL_NT - NL) {
-            const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
+            const int node = a.blist[bi * BR_STRIDE] - node0, cls = a.blist[bi * BR_STRIDE + 3] - cls0;
             const double conv = (a.convention == 1) ? 3.0 / mubar[cls] : 1.0;
             double dt = 0.0;
             const int nj = a.comp_ptr[cls + 1] - a.comp_ptr[cls];
             for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
-            if (node < NR) gbl[node] = conv * dt;
+            if (node >= 0 && node < NR) gbl[node] = conv * dt;
         }
     }
     __syncthreads();
@@ -1649,7 +1739,7 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs a) {
                         const double f = sum_tau[cl] / mubar[cl];
                         for (int j = a.comp_ptr[cl]; j < a.comp_ptr[cl + 1]; ++j) {
                             const int mm = a.comp_mat[j];
-                            if (is_om && mm != t) continue;
+                            if (is_om && mm != mat0 + t) continue;
                             v -= f * prm[a.pl.off_w + j] * a.dmu[mm * 6 + phi];
                         }
                     }
@@ -1660,15 +1750,15 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs a) {
                     gth[phi - 1] = v;
             }
         } else {
-            const int J = t - NM - 5;
+            const int J = t - NM - 5, Jg = comp0 + J;
             int cls = 0;
-            while (cls + 1 < a.n_class && a.comp_ptr[cls + 1] <= J) ++cls;
-            const int jj = J - a.comp_ptr[cls];
+            while (cls + 1 < a.n_class && a.comp_ptr[cls + 1] <= Jg) ++cls;
+            const int jj = Jg - a.comp_ptr[cls];
             for (int bi = lane; bi < a.n_branch; bi += 64)
-                if (a.blist[bi * BR_STRIDE + 3] == cls) v += a.fr[((size_t)bi * MC + jj) * 8 + 0];
+                if (a.blist[bi * BR_STRIDE + 3] == cls0 + cls) v += a.fr[((size_t)bi * MC + jj) * 8 + 0];
             v = wave_sum(v);
             if (lane == 0) {
-                if (a.convention == 1) v -= sum_tau[cls] / mubar[cls] * a.mu[a.comp_mat[J]];
+                if (a.convention == 1) v -= sum_tau[cls] / mubar[cls] * a.mu[a.comp_mat[Jg]];
                 gw[J] = v;
             }
         }
@@ -1739,7 +1829,9 @@ void launch_frechet(const FrechetArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_hproj, dim3(a.n_mat, HPROJ_CHUNKS), dim3(256), 0, s, a);
 }
 
-void launch_final(const FinalArgs &a, hipStream_t s) { hipLaunchKernelGGL(k_final, dim3(1), dim3(FINAL_NT), 0, s, a); }
+void launch_final(const FinalArgs &a, int n_genes, hipStream_t s) {
+    hipLaunchKernelGGL(k_final, dim3(n_genes), dim3(FINAL_NT), 0, s, a);
+}
 
 template void launch_pmat<float>(const PmatArgs<float> &, hipStream_t);
 template void launch_pmat<double>(const PmatArgs<double> &, hipStream_t);

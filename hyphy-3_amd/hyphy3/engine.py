@@ -265,6 +265,61 @@ def create_group(n, device=0, n_stream_sets=8):
     return [Engine(device, handle=hs[i]) for i in range(n)]
 
 
+class Batch:
+    """Reusable hy_eval_batch call over a fixed list of engines: the item
+    table, parameter and output arrays are allocated once; eval() copies the
+    new parameters in and returns per-gene result dicts (views into the
+    output arrays, overwritten by the next call)."""
+
+    def __init__(self, engines):
+        self.L = load_library()
+        self.engines = list(engines)
+        n = len(self.engines)
+        self.items = (BatchItem * max(n, 1))()
+        self.arrays = []
+        for i, e in enumerate(self.engines):
+            a = {"bl": np.zeros(e.num_nodes), "om": np.zeros(e.n_mat), "cw": np.zeros(e.n_comp), "th": np.zeros(5),
+                 "ll": np.zeros(1), "site": np.zeros(max(e.num_patterns, 1)), "gbl": np.zeros(e.num_nodes),
+                 "gom": np.zeros(e.n_mat), "gw": np.zeros(e.n_comp), "gth": np.zeros(5)}
+            self.arrays.append(a)
+            it = self.items[i]
+            it.h = e.h
+            it.bl, it.omegas, it.comp_w, it.theta = (_p(a[k], _f64p) for k in ("bl", "om", "cw", "th"))
+            it.alpha, it.delta, it.psi = 1.0, 0.0, 0.0
+            it.out_logL = _p(a["ll"], _f64p)
+
+    def eval(self, params, precision=HY_PREC_FP64, grad=True, site=False):
+        """params[i] = (bl, omegas, comp_w, theta[, alpha])."""
+        n = len(self.engines)
+        for i, prm in enumerate(params):
+            a, it = self.arrays[i], self.items[i]
+            a["bl"][:] = prm[0]
+            a["om"][:] = prm[1]
+            a["cw"][:] = prm[2]
+            a["th"][:] = prm[3]
+            it.alpha = float(prm[4]) if len(prm) > 4 else 1.0
+            it.out_site_ll = _p(a["site"], _f64p) if site else None
+            if grad:
+                it.out_grad_bl, it.out_grad_omega, it.out_grad_w, it.out_grad_theta = (
+                    _p(a[k], _f64p) for k in ("gbl", "gom", "gw", "gth"))
+            else:
+                it.out_grad_bl = it.out_grad_omega = it.out_grad_w = it.out_grad_theta = None
+        flags = HY_WANT_LOGL | (HY_WANT_SITE_LL if site else 0) | (HY_WANT_GRAD if grad else 0)
+        rc = self.L.hy_eval_batch(n, self.items, precision, flags)
+        if rc != HY_OK:
+            bad = next((i for i in range(n) if self.items[i].status != HY_OK), 0)
+            raise HyError(rc, f"item {bad}: " + self.L.hy_last_error(self.engines[bad].h).decode())
+        out = []
+        for e, a in zip(self.engines, self.arrays):
+            r = {"log_likelihood": float(a["ll"][0])}
+            if site:
+                r["site_ll"] = a["site"][:e.num_patterns]
+            if grad:
+                r.update(grad_bl=a["gbl"], grad_omega=a["gom"], grad_w=a["gw"], grad_theta=a["gth"])
+            out.append(r)
+        return out
+
+
 def eval_batch(engines, params, precision=HY_PREC_FP64, grad=True, site=False):
     """Evaluates many engines (one gene each) in one hy_eval_batch call.
     params[i] = (bl, omegas, comp_w, theta[, alpha]); returns a list of result
