@@ -321,9 +321,16 @@ def bench_genes(args):
     print(f"[bench] c3: {len(engines)} engines set up", file=sys.stderr, flush=True)
 
     batch = Batch(engines)
+    # flat base parameters; step i perturbs them as step_params does, vectorised
+    bl0 = np.concatenate([b[0] for b in bases])
+    om0 = np.concatenate([b[1] for b in bases])
+    w0 = np.concatenate([b[2] for b in bases])
+    th0 = np.stack([np.asarray(b[3], dtype=float) for b in bases])
 
     def step(i, prec):
-        return batch.eval([step_params(b, i) for b in bases], precision=prec)
+        f = 1.0 + 1e-3 * (1.0 + np.sin(0.7 * i))
+        g = 1.0 + 1e-3 * (1.0 + np.cos(0.9 * i))
+        return batch.eval_flat(bl0 * g, om0 * f, w0, th0 * (2.0 - f), precision=prec)
 
     res = {}
     for prec, name in ((1, "fp32"), (0, "fp64")):

@@ -542,6 +542,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
             HIP_TRY(h, hipEventRecord(h->ev_model, s));
             HIP_TRY(h, hipStreamWaitEvent(h->stream2, h->ev_model, 0));
             ProjArgs pj{};
+            pj.qdesc = (const int32_t *)h->d_qdesc.p;
             pj.nd = (const int8_t *)h->d_nd.p;
             pj.dpos = (const int8_t *)h->d_dpos.p;
             pj.dfrom = (const int8_t *)h->d_dfrom.p;

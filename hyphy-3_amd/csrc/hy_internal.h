@@ -165,6 +165,7 @@ struct AdjArgs {
 };
 
 struct ProjArgs {
+    const int32_t *qdesc;  // [64*64] packed pair descriptors
     const int8_t *nd, *dpos, *dfrom, *dto;
     const uint8_t *syn;
     const double *pos, *pi;

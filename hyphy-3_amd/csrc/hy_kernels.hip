@@ -1415,73 +1415,58 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
 // k_proj: M_phi = V_pi dQ_phi U_pi and dmu_phi = -sum_i pi_i dQ_phi[i][i]
 // (likelihood.hpp:1497-1562).  phi 0 = omega, 1..5 = theta AC, AT, CG, CT, GT.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_proj(ProjArgs args) {
+__global__ __launch_bounds__(256, 2) void k_proj(ProjArgs args) {
     ProjArgs a = args;
     matrix_gene_view(a, blockIdx.x);
-    const int m = blockIdx.x, phi = blockIdx.y, tid = threadIdx.x;
-    __shared__ double dQ[64 * 64];
-    __shared__ double T1[64 * 64];
-    __shared__ double Ms[64 * 64];
+    const int m = blockIdx.x, phi = blockIdx.y, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    __shared__ double X[64 * LDA];  // dQ_phi, then dQ_phi U
+    __shared__ double Y[64 * LDA];  // U, then V
+    __shared__ double thl[8], posl[12];
     const double *prm = a.params;
     const double alpha = prm[a.pl.off_alpha], beta = prm[a.pl.off_omega + m];
-    const double *th = prm + a.pl.off_theta;
     const int S = a.S;
-    const int pairs[5][2] = {{0, 1}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    if (tid < 8) thl[tid] = tid < 5 ? prm[a.pl.off_theta + tid] : 1.0;
+    if (tid < 12) posl[tid] = a.pos[tid];
+    __syncthreads();
+    // dQ_phi: single-nucleotide changes only (likelihood.hpp:1529-1562);
+    // phi = 0: omega (nonsynonymous pairs), phi = 1..5: theta_AC..theta_GT
     for (int idx = tid; idx < 4096; idx += 256) {
-        const int i = idx >> 6, j = idx & 63;
+        const int i = idx >> 6, j = idx & 63, d = a.qdesc[idx];
         double q = 0.0;
-        if (i < S && j < S && i != j) {
-            const int k = i * 64 + j;
-            if (a.nd[k] == 1) {
-                const int from = a.dfrom[3 * k], to = a.dto[3 * k], ps = a.dpos[3 * k];
-                if (phi == 0) {
-                    if (!a.syn[k]) q = nuc_rate(th, from, to) * a.pos[ps * 4 + to];
-                } else {
-                    const int u = pairs[phi - 1][0], v = pairs[phi - 1][1];
-                    if ((from == u && to == v) || (from == v && to == u)) q = (a.syn[k] ? alpha : beta) * a.pos[ps * 4 + to];
-                }
-            }
+        if (i < S && j < S && i != j && (d & 3) == 1) {
+            const int ti = (d >> 3) & 7;
+            const double pf = posl[(d >> 6) & 15];
+            if (phi == 0)
+                q = (d & QDESC_SYN) ? 0.0 : thl[ti] * pf;
+            else if (ti == phi - 1)
+                q = ((d & QDESC_SYN) ? alpha : beta) * pf;
         }
-        dQ[idx] = q;
-        Ms[idx] = a.U[m * 4096 + idx];
+        X[i * LDA + j] = q;
+        Y[i * LDA + j] = a.U[(size_t)m * 4096 + idx];
     }
     __syncthreads();
     if (tid < 64) {
         double s = 0.0;
         for (int j = 0; j < 64; ++j)
-            if (j != tid) s += dQ[tid * 64 + j];
-        dQ[tid * 64 + tid] = -s;
+            if (j != tid) s += X[tid * LDA + j];
+        X[tid * LDA + tid] = -s;
+        double t = tid < S ? a.pi[tid] * s : 0.0;  // d mu / d phi = -sum_i pi_i dQ_ii
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
+        if (tid == 0) a.dmu[m * 6 + phi] = t;
     }
     __syncthreads();
-    if (tid == 0) {
-        double d = 0.0;
-        for (int i = 0; i < S; ++i) d -= a.pi[i] * dQ[i * 64 + i];
-        a.dmu[m * 6 + phi] = d;
-    }
-    const int i0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
-    double t[4][4];
-    for (int ii = 0; ii < 4; ++ii)
-        for (int jj = 0; jj < 4; ++jj) t[ii][jj] = 0.0;
-    for (int k = 0; k < 64; ++k)
-        for (int ii = 0; ii < 4; ++ii) {
-            const double d = dQ[(i0 + ii) * 64 + k];
-            for (int jj = 0; jj < 4; ++jj) t[ii][jj] = fma(d, Ms[k * 64 + c0 + jj], t[ii][jj]);
-        }
-    for (int ii = 0; ii < 4; ++ii)
-        for (int jj = 0; jj < 4; ++jj) T1[(i0 + ii) * 64 + c0 + jj] = t[ii][jj];
+    f64x4 acc[4];
+    gemm64_acc<false, false>(X, Y, acc, w, lr, lg);  // dQ U
     __syncthreads();
-    for (int idx = tid; idx < 4096; idx += 256) Ms[idx] = a.V[m * 4096 + idx];
+    gemm64_store(acc, X, w, lr, lg);
+    for (int idx = tid; idx < 4096; idx += 256) Y[(idx >> 6) * LDA + (idx & 63)] = a.V[(size_t)m * 4096 + idx];
     __syncthreads();
-    for (int ii = 0; ii < 4; ++ii)
-        for (int jj = 0; jj < 4; ++jj) t[ii][jj] = 0.0;
-    for (int k = 0; k < 64; ++k)
-        for (int ii = 0; ii < 4; ++ii) {
-            const double v = Ms[(i0 + ii) * 64 + k];
-            for (int jj = 0; jj < 4; ++jj) t[ii][jj] = fma(v, T1[k * 64 + c0 + jj], t[ii][jj]);
-        }
+    gemm64_acc<false, false>(Y, X, acc, w, lr, lg);  // V (dQ U)
     double *M = a.M + ((size_t)m * 6 + phi) * 4096;
-    for (int ii = 0; ii < 4; ++ii)
-        for (int jj = 0; jj < 4; ++jj) M[(i0 + ii) * 64 + c0 + jj] = t[ii][jj];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) M[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] = acc[q][r];
 }
 
 // ---------------------------------------------------------------------------

@@ -266,42 +266,58 @@ def create_group(n, device=0, n_stream_sets=8):
 
 
 class Batch:
-    """Reusable hy_eval_batch call over a fixed list of engines: the item
-    table, parameter and output arrays are allocated once; eval() copies the
-    new parameters in and returns per-gene result dicts (views into the
-    output arrays, overwritten by the next call)."""
+    """Reusable hy_eval_batch call over a fixed list of engines.  Parameters
+    and outputs live in flat arrays (gene g's slice at the offsets in
+    ``off_bl``, ``off_om``, ``off_cw``; theta rows ``th[g]``) allocated once;
+    ``eval`` takes per-gene tuples, ``eval_flat`` whole flat arrays.  Result
+    dicts are views into the output arrays, overwritten by the next call."""
 
     def __init__(self, engines):
         self.L = load_library()
         self.engines = list(engines)
         n = len(self.engines)
+        nn = [e.num_nodes for e in self.engines]
+        nm = [e.n_mat for e in self.engines]
+        nc = [e.n_comp for e in self.engines]
+        self.off_bl = np.concatenate([[0], np.cumsum(nn)]).astype(np.int64)
+        self.off_om = np.concatenate([[0], np.cumsum(nm)]).astype(np.int64)
+        self.off_cw = np.concatenate([[0], np.cumsum(nc)]).astype(np.int64)
+        self.bl = np.zeros(max(int(self.off_bl[-1]), 1))
+        self.om = np.zeros(max(int(self.off_om[-1]), 1))
+        self.cw = np.zeros(max(int(self.off_cw[-1]), 1))
+        self.th = np.zeros((max(n, 1), 5))
+        self.alpha = np.ones(max(n, 1))
+        self.ll = np.zeros(max(n, 1))
+        self.gbl = np.zeros_like(self.bl)
+        self.gom = np.zeros_like(self.om)
+        self.gw = np.zeros_like(self.cw)
+        self.gth = np.zeros_like(self.th)
+        self.site = [np.zeros(max(e.num_patterns, 1)) for e in self.engines]
         self.items = (BatchItem * max(n, 1))()
-        self.arrays = []
-        for i, e in enumerate(self.engines):
-            a = {"bl": np.zeros(e.num_nodes), "om": np.zeros(e.n_mat), "cw": np.zeros(e.n_comp), "th": np.zeros(5),
-                 "ll": np.zeros(1), "site": np.zeros(max(e.num_patterns, 1)), "gbl": np.zeros(e.num_nodes),
-                 "gom": np.zeros(e.n_mat), "gw": np.zeros(e.n_comp), "gth": np.zeros(5)}
-            self.arrays.append(a)
-            it = self.items[i]
+        d = C.sizeof(C.c_double)
+        base = lambda arr: arr.ctypes.data  # noqa: E731
+        self._ptr = lambda arr, off: C.cast(base(arr) + int(off) * d, _f64p)  # noqa: E731
+        for g, e in enumerate(self.engines):
+            it = self.items[g]
             it.h = e.h
-            it.bl, it.omegas, it.comp_w, it.theta = (_p(a[k], _f64p) for k in ("bl", "om", "cw", "th"))
-            it.alpha, it.delta, it.psi = 1.0, 0.0, 0.0
-            it.out_logL = _p(a["ll"], _f64p)
+            it.bl = self._ptr(self.bl, self.off_bl[g])
+            it.omegas = self._ptr(self.om, self.off_om[g])
+            it.comp_w = self._ptr(self.cw, self.off_cw[g])
+            it.theta = self._ptr(self.th, 5 * g)
+            it.delta, it.psi = 0.0, 0.0
+            it.out_logL = self._ptr(self.ll, g)
 
-    def eval(self, params, precision=HY_PREC_FP64, grad=True, site=False):
-        """params[i] = (bl, omegas, comp_w, theta[, alpha])."""
+    def _run(self, precision, grad, site):
         n = len(self.engines)
-        for i, prm in enumerate(params):
-            a, it = self.arrays[i], self.items[i]
-            a["bl"][:] = prm[0]
-            a["om"][:] = prm[1]
-            a["cw"][:] = prm[2]
-            a["th"][:] = prm[3]
-            it.alpha = float(prm[4]) if len(prm) > 4 else 1.0
-            it.out_site_ll = _p(a["site"], _f64p) if site else None
+        for g in range(n):
+            it = self.items[g]
+            it.alpha = float(self.alpha[g])
+            it.out_site_ll = _p(self.site[g], _f64p) if site else None
             if grad:
-                it.out_grad_bl, it.out_grad_omega, it.out_grad_w, it.out_grad_theta = (
-                    _p(a[k], _f64p) for k in ("gbl", "gom", "gw", "gth"))
+                it.out_grad_bl = self._ptr(self.gbl, self.off_bl[g])
+                it.out_grad_omega = self._ptr(self.gom, self.off_om[g])
+                it.out_grad_w = self._ptr(self.gw, self.off_cw[g])
+                it.out_grad_theta = self._ptr(self.gth, 5 * g)
             else:
                 it.out_grad_bl = it.out_grad_omega = it.out_grad_w = it.out_grad_theta = None
         flags = HY_WANT_LOGL | (HY_WANT_SITE_LL if site else 0) | (HY_WANT_GRAD if grad else 0)
@@ -309,15 +325,40 @@ class Batch:
         if rc != HY_OK:
             bad = next((i for i in range(n) if self.items[i].status != HY_OK), 0)
             raise HyError(rc, f"item {bad}: " + self.L.hy_last_error(self.engines[bad].h).decode())
+
+    def results(self, grad=True, site=False):
         out = []
-        for e, a in zip(self.engines, self.arrays):
-            r = {"log_likelihood": float(a["ll"][0])}
+        for g, e in enumerate(self.engines):
+            r = {"log_likelihood": float(self.ll[g])}
             if site:
-                r["site_ll"] = a["site"][:e.num_patterns]
+                r["site_ll"] = self.site[g][:e.num_patterns]
             if grad:
-                r.update(grad_bl=a["gbl"], grad_omega=a["gom"], grad_w=a["gw"], grad_theta=a["gth"])
+                r.update(grad_bl=self.gbl[self.off_bl[g]:self.off_bl[g + 1]],
+                         grad_omega=self.gom[self.off_om[g]:self.off_om[g + 1]],
+                         grad_w=self.gw[self.off_cw[g]:self.off_cw[g + 1]], grad_theta=self.gth[g])
             out.append(r)
         return out
+
+    def eval(self, params, precision=HY_PREC_FP64, grad=True, site=False):
+        """params[g] = (bl, omegas, comp_w, theta[, alpha])."""
+        for g, prm in enumerate(params):
+            self.bl[self.off_bl[g]:self.off_bl[g + 1]] = prm[0]
+            self.om[self.off_om[g]:self.off_om[g + 1]] = prm[1]
+            self.cw[self.off_cw[g]:self.off_cw[g + 1]] = prm[2]
+            self.th[g] = prm[3]
+            self.alpha[g] = float(prm[4]) if len(prm) > 4 else 1.0
+        self._run(precision, grad, site)
+        return self.results(grad, site)
+
+    def eval_flat(self, bl, om, cw, th, precision=HY_PREC_FP64, grad=True, site=False):
+        """Flat parameter arrays in this object's layout (alpha from
+        ``self.alpha``)."""
+        np.copyto(self.bl, bl)
+        np.copyto(self.om, om)
+        np.copyto(self.cw, cw)
+        np.copyto(self.th, th)
+        self._run(precision, grad, site)
+        return self.results(grad, site)
 
 
 def eval_batch(engines, params, precision=HY_PREC_FP64, grad=True, site=False):

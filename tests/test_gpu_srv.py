@@ -90,6 +90,15 @@ def test_batch_fused_matches_oracle_and_individual():
             assert rel(r["log_likelihood"], one["log_likelihood"]) < tl1
             for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
                 assert grad_err(r[k], one[k]) < tg1, (prec, k)
+    # the reusable flat-array evaluator runs the same fused path
+    from hyphy3.engine import Batch
+    b = Batch(engines)
+    rb = b.eval([g[1] for g in genes], precision=FP64, site=True)
+    rf = b.eval_flat(b.bl.copy(), b.om.copy(), b.cw.copy(), b.th.copy(), precision=FP64)
+    for (eng, prm, ref), r1, r2 in zip(genes, rb, rf):
+        assert rel(r1["log_likelihood"], ref["log_likelihood"]) < 1e-9
+        assert rel(r2["log_likelihood"], r1["log_likelihood"]) < 1e-12
+        assert grad_err(r1["grad_bl"], ref["grad_bl"]) < 1e-7
 
 
 def test_batch_engine_by_engine_is_bit_identical():
