@@ -1285,6 +1285,37 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
 // c_p = w_p / L_p * 2^(G_b + E_b - E_root).  MFMA in the working type within
 // a tile, FP64 accumulation across tiles.
 // ---------------------------------------------------------------------------
+// Adjoint operand tiles in LDS: (state s, pattern p of a 64-pattern group) at
+// s * ADJ_LD + (p % 4) * 16 + p / 4, so the 16 patterns 4 kk + lg a lane feeds
+// to the MFMA are contiguous; the 68-element row stride staggers rows over
+// the LDS banks.
+constexpr int ADJ_LD = 68;
+__device__ __forceinline__ int adj_idx(int s, int p) { return s * ADJ_LD + (p & 3) * 16 + (p >> 2); }
+
+// 16 contiguous, 16-byte aligned LDS elements into registers (128-bit loads).
+template <typename R>
+__device__ __forceinline__ void lds_run16(R *dst, const R *src) {
+    if constexpr (sizeof(R) == 4) {
+        const float4 *q = reinterpret_cast<const float4 *>(src);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float4 v = q[i];
+            dst[4 * i] = v.x;
+            dst[4 * i + 1] = v.y;
+            dst[4 * i + 2] = v.z;
+            dst[4 * i + 3] = v.w;
+        }
+    } else {
+        const double2 *q = reinterpret_cast<const double2 *>(src);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double2 v = q[i];
+            dst[2 * i] = v.x;
+            dst[2 * i + 1] = v.y;
+        }
+    }
+}
+
 template <typename R>
 __device__ __forceinline__ R tip_onehot(unsigned code, int j, const uint64_t *amb, int S) {
     if (code < 64u) return (R)(code == (unsigned)j ? 1 : 0);
@@ -1297,13 +1328,16 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
     AdjArgs<R> a = args;
     // Works on groups of 64 patterns (G pruning tiles, contiguous in HBM):
     // the next group is fetched into registers while the current one is
-    // multiplied from LDS (double-buffered).
+    // multiplied from LDS (double-buffered).  In LDS both operands are stored
+    // transposed to lane-contiguous runs (adj_idx): each lane reads its 16
+    // MFMA operands per block with 128-bit loads, bank-conflict free.
     constexpr int G = 64 / T, PER = 4096 / 256;
+    constexpr int CBI = sizeof(R) == 4 ? 4 : 2;  // column blocks whose B operands are held at once
     typedef typename MF<R>::acc acc_t;
-    __shared__ R Vs[2][4096];  // G tiles of V, each [64 states][T] row-major
-    __shared__ R Ds[2][4096];  // G tiles of D, each B-operand-major
-    __shared__ R cs[2][64];
-    __shared__ unsigned codes[2][64];
+    __shared__ __align__(16) R Vs[2][64 * ADJ_LD];
+    __shared__ __align__(16) R Ds[2][64 * ADJ_LD];
+    __shared__ __align__(16) R cs[2][64];
+    __shared__ __align__(16) unsigned codes[2][64];
     const int chunk = blockIdx.x, bi = blockIdx.y;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
     const int32_t *br = a.blist + bi * BR_STRIDE;
@@ -1360,12 +1394,15 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
     auto stash = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            Vs[buf][q * 256 + tid] = rv[q];
-            Ds[buf][q * 256 + tid] = rd[q];
+            const int e = q * 256 + tid, ti = e / (64 * T), el = e % (64 * T);
+            // V tiles are [state][pattern]; D tiles B-operand-major (bmaj_index)
+            Vs[buf][adj_idx(el / T, ti * T + el % T)] = rv[q];
+            const int sd = ((el & 15) << 2) | ((el >> 8) & 3), td = ((el >> 10) << 4) | ((el >> 4) & 15);
+            Ds[buf][adj_idx(sd, ti * T + td)] = rd[q];
         }
         if (tid < 64) {
-            cs[buf][tid] = rc;
-            codes[buf][tid] = rcode;
+            cs[buf][adj_idx(0, tid)] = rc;
+            codes[buf][adj_idx(0, tid)] = rcode;
         }
     };
     f64x4 acc[4];
@@ -1382,19 +1419,38 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
         acc_t loc[4];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) loc[cb] = splat<acc_t>(0.0);
+        // lane (lr, lg) of wave w: patterns 4 kk + lg, kk = 0..15
+        R av[16], cv[16];
+        lds_run16<R>(av, &Vs[buf][adj_idx(16 * w + lr, lg)]);
+        lds_run16<R>(cv, &cs[buf][adj_idx(0, lg)]);
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            const int p = 4 * kk + lg, ti = p / T, pt = p % T;
-            const R av = Vs[buf][ti * (64 * T) + (16 * w + lr) * T + pt] * cs[buf][p];
-            if (tip) {
-                const unsigned code = codes[buf][p];
+        for (int kk = 0; kk < 16; ++kk) av[kk] *= cv[kk];
+        if (tip) {
+            unsigned cd[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) cd[kk] = codes[buf][adj_idx(0, lg) + kk];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
 #pragma unroll
                 for (int cb = 0; cb < 4; ++cb)
-                    loc[cb] = MF<R>::mma(av, tip_onehot<R>(code, 16 * cb + lr, a.amb, a.S), loc[cb]);
-            } else {
+                    loc[cb] = MF<R>::mma(av[kk], tip_onehot<R>(cd[kk], 16 * cb + lr, a.amb, a.S), loc[cb]);
+        } else {
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb)
-                    loc[cb] = MF<R>::mma(av, Ds[buf][ti * (64 * T) + bmaj(16 * cb + lr, pt)], loc[cb]);
+            for (int c0 = 0; c0 < 4; c0 += CBI) {
+                R bv[CBI][16];
+#pragma unroll
+                for (int c = 0; c < CBI; ++c) lds_run16<R>(bv[c], &Ds[buf][adj_idx(16 * (c0 + c) + lr, lg)]);
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+                    for (int c = 0; c < CBI; ++c) loc[c0 + c] = MF<R>::mma(av[kk], bv[c][kk], loc[c0 + c]);
+                // the next block's operands may reuse bv's registers: let these
+                // MFMAs complete first
+                if (c0 + CBI < 4)
+#pragma unroll
+                    for (int c = 0; c < CBI; ++c)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) asm volatile("" ::"v"(loc[c0 + c][r]));
             }
         }
 #pragma unroll
