@@ -1292,7 +1292,8 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
 constexpr int ADJ_LD = 68;
 __device__ __forceinline__ int adj_idx(int s, int p) { return s * ADJ_LD + (p & 3) * 16 + (p >> 2); }
 
-// 16 contiguous, 16-byte aligned LDS elements into registers (128-bit loads).
+// 16 contiguous, 16-byte aligned elements (LDS or global) into registers
+// (128-bit loads).
 template <typename R>
 __device__ __forceinline__ void lds_run16(R *dst, const R *src) {
     if constexpr (sizeof(R) == 4) {
@@ -1368,12 +1369,22 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
         const int tg0 = t0 + g * G;
         const R *Vg = a.V + ((size_t)vslot * a.n_tiles + tg0) * (64 * T);
         const R *Dg = a.D + ((size_t)idx * a.n_tiles + tg0) * (64 * T);
+        // each thread loads one contiguous run of PER elements (128-bit loads;
+        // a run never crosses a tile)
+        {
+            const int e0 = tid * PER;
+            const bool valid = tg0 + e0 / (64 * T) < t1;
+            if (valid) {
+                lds_run16<R>(rv, Vg + e0);
+                if (!tip)
+                    lds_run16<R>(rd, Dg + e0);
+                else
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = q * 256 + tid;
-            const bool valid = tg0 + e / (64 * T) < t1;
-            rv[q] = valid ? Vg[e] : (R)0;
-            rd[q] = (valid && !tip) ? Dg[e] : (R)0;
+                    for (int q = 0; q < PER; ++q) rd[q] = (R)0;
+            } else {
+#pragma unroll
+                for (int q = 0; q < PER; ++q) rv[q] = rd[q] = (R)0;
+            }
         }
         if (tid < 64) {
             const int p = tg0 * T + tid;
@@ -1394,7 +1405,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
     auto stash = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const int e = q * 256 + tid, ti = e / (64 * T), el = e % (64 * T);
+            const int e = tid * PER + q, ti = e / (64 * T), el = e % (64 * T);
             // V tiles are [state][pattern]; D tiles B-operand-major (bmaj_index)
             Vs[buf][adj_idx(el / T, ti * T + el % T)] = rv[q];
             const int sd = ((el & 15) << 2) | ((el >> 8) & 3), td = ((el >> 10) << 4) | ((el >> 4) & 15);
