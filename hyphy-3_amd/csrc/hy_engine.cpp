@@ -134,6 +134,7 @@ struct hy_engine_s {
     DevBuf d_Prow[2], d_Rsum[2], d_PA[2], d_PTA[2];
     DevBuf d_D[2], d_Vp[2], d_E, d_G, d_Lroot, d_Eroot, d_site, d_Apart, d_fr, d_res;
     DevBuf d_Asum, d_W, d_wl, d_proj;
+    int grp_size = 1, n_grp = 1;  // Fréchet stage grouping of (branch, component) pairs per matrix
     std::vector<int32_t> wl;  // uploaded copy of the per-matrix W slot lists
     int n_chunks = 1, tiles_per_chunk = 1;
 
@@ -287,6 +288,20 @@ int upload_identity_tiles(hy_engine h, int ri, int esz) {
 }
 
 // (Re)allocate everything that depends on the tile width and precision.
+// Fréchet stage grouping: pairs (branch, component) per workgroup so that
+// about two workgroups per CU run, each matrix's U and V loaded once per group.
+template <typename Lists>
+void set_frechet_groups(hy_engine h, const Lists &per) {
+    size_t total = 0, longest = 1;
+    for (const auto &l : per) {
+        total += l.size();
+        longest = std::max(longest, l.size());
+    }
+    const size_t target = (size_t)2 * h->n_cu;
+    h->grp_size = (int)std::max<size_t>(1, (total + target - 1) / target);
+    h->n_grp = (int)((longest + h->grp_size - 1) / h->grp_size);
+}
+
 int prepare_buffers(hy_engine h, int prec) {
     const int T = choose_tile(h->P, h->T_req, prec, h->n_cu);
     const int ri = prec == HY_PREC_FP64 ? 1 : 0;
@@ -346,8 +361,7 @@ int prepare_buffers(hy_engine h, int prec) {
     h->n_chunks = (h->n_tiles + h->tiles_per_chunk - 1) / h->tiles_per_chunk;
     TRY(ensure(h, h->d_Apart, (size_t)nb * h->n_chunks * 4096 * 8));
     TRY(ensure(h, h->d_fr, (size_t)nb * h->max_comp * 8 * 8));
-    TRY(ensure(h, h->d_Asum, (size_t)nb * 4096 * 8));
-    TRY(ensure(h, h->d_W, (size_t)nb * h->max_comp * 4096 * 8));
+
     TRY(ensure(h, h->d_proj, (size_t)h->n_mat * 6 * HPROJ_CHUNKS * 8));
     {
         // per matrix: the W slots (branch * max_comp + component index) whose
@@ -365,7 +379,10 @@ int prepare_buffers(hy_engine h, int prec) {
             TRY(upload(h, h->d_wl, wl.data(), wl.size() * 4));
             h->wl = wl;
         }
+        set_frechet_groups(h, per);
     }
+    TRY(ensure(h, h->d_W, (size_t)h->n_mat * h->n_grp * 4096 * 8));
+    TRY(ensure(h, h->d_Asum, (size_t)nb * 4096 * 8));
     const int nm = h->n_mat;
     TRY(ensure(h, h->d_lam, (size_t)nm * 64 * 8));
     TRY(ensure(h, h->d_U, (size_t)nm * 4096 * 8));
@@ -669,6 +686,8 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         fa.out = (double *)h->d_fr.p;
         fa.n_mat = h->n_mat;
         fa.Asum = (double *)h->d_Asum.p;
+        fa.grp_size = h->grp_size;
+        fa.n_grp = h->n_grp;
         fa.W = (double *)h->d_W.p;
         fa.wl_ptr = (const int32_t *)h->d_wl.p;
         fa.wl_idx = (const int32_t *)h->d_wl.p + (h->n_mat + 1);
@@ -1695,6 +1714,7 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
         wl.assign(mat0 + 1, 0);
         for (int m = 0; m < mat0; ++m) wl[m + 1] = wl[m] + (int32_t)per[m].size();
         for (int m = 0; m < mat0; ++m) wl.insert(wl.end(), per[m].begin(), per[m].end());
+        set_frechet_groups(c, per);
     }
     // context engine state
     hy_engine h0 = hs[0];
@@ -1794,8 +1814,8 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
     TRY(ensure(c, c->d_dmu, (size_t)mat0 * 6 * 8));
     TRY(ensure(c, c->d_Apart, (size_t)br0 * 4096 * 8));
     TRY(ensure(c, c->d_fr, (size_t)br0 * max_comp * 8 * 8));
+    TRY(ensure(c, c->d_W, (size_t)mat0 * c->n_grp * 4096 * 8));
     TRY(ensure(c, c->d_Asum, (size_t)br0 * 4096 * 8));
-    TRY(ensure(c, c->d_W, (size_t)br0 * max_comp * 4096 * 8));
     TRY(ensure(c, c->d_proj, (size_t)mat0 * 6 * HPROJ_CHUNKS * 8));
     TRY(ensure(c, c->d_params, (size_t)c->pl.total * 8));
     TRY(ensure(c, c->d_res, res_off * 8));

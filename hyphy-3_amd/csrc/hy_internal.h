@@ -188,8 +188,9 @@ struct FrechetArgs {
     ParamLayout pl;
     const double *Apart, *lam, *U, *V, *M, *tau;
     double *Asum;                    // [n_branch][4096] adjoint summed over pattern chunks
-    double *W;                       // [n_branch][max_comp][4096] w_j E o A~
-    const int32_t *wl_ptr, *wl_idx;  // per matrix: W slots (branch * max_comp + component) using it
+    int grp_size, n_grp;             // (branch, component) pairs per workgroup; groups per matrix
+    double *W;                       // [n_mat][n_grp][4096] partial H_m of each group
+    const int32_t *wl_ptr, *wl_idx;  // per matrix: pair slots (branch * max_comp + component) using it
     double *out;                     // [n_branch][max_comp][8]: [0] <A, P_m>, [1] w_j d lnL / d tau share
     double *proj;                    // [n_mat][6][HPROJ_CHUNKS] partial <M_phi, H_m>
 };

@@ -1540,15 +1540,18 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs args) {
 // Fréchet stage (likelihood.hpp:1435-1520), in three kernels:
 //   k_asum     A_b = sum over pattern chunks of the adjoint partials (fixed
 //              order, deterministic)
-//   k_frechet  one workgroup per (branch, mixture component j of its class):
-//              A~ = U_m^T A_b V_m^T (two FP64 MFMA GEMMs), the component's share
-//              of d lnL / d tau_b = w_j sum_i lambda_i e^{lambda_i tau} A~_ii,
-//              <A_b, P_m(tau_b)> = sum_i e^{lambda_i tau} A~_ii, and
-//              W = w_j E o A~ with E_ij = (e^{l_i t} - e^{l_j t}) / (l_i - l_j)
-//              (t e^{l_i t} when |l_i - l_j| < 1e-9)
-//   k_hproj    per (matrix m, 256-entry chunk): H_m = sum of the W of every
-//              (branch, component) using m, projected on M_phi = V dQ_phi U
-//              for phi = omega, theta_AC..theta_GT
+//   k_frechet  per (matrix m, group of the (branch, component) pairs using m,
+//              in fixed branch order): U_m, V_m stay in LDS while the
+//              workgroup walks its pairs (the next A_b prefetched into
+//              registers); for each, A~ = U_m^T A_b V_m^T
+//              (two FP64 MFMA GEMMs), the component's share of
+//              d lnL / d tau_b = w_j sum_i lambda_i e^{lambda_i tau} A~_ii and
+//              <A_b, P_m(tau_b)> = sum_i e^{lambda_i tau} A~_ii, and the group's
+//              partial H_m += w_j E o A~ with E_ij = (e^{l_i t} - e^{l_j t}) /
+//              (l_i - l_j) (t e^{l_i t} when |l_i - l_j| < 1e-9), in registers
+//   k_hproj    per (matrix m, 256-entry chunk): H_m = sum of the group
+//              partials (fixed order), projected on M_phi = V dQ_phi U for
+//              phi = omega, theta_AC..theta_GT
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_asum(FrechetArgs a) {
     const int bi = blockIdx.x, idx = blockIdx.y * 256 + threadIdx.x;
@@ -1567,74 +1570,102 @@ __global__ __launch_bounds__(256) void k_asum(FrechetArgs a) {
     a.Asum[(size_t)bi * 4096 + idx] = s;
 }
 
-__global__ __launch_bounds__(256) void k_frechet(FrechetArgs a) {
-    const int bi = blockIdx.x, jj = blockIdx.y, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
-    const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
-    const int J = a.comp_ptr[cls] + jj;
-    if (J >= a.comp_ptr[cls + 1]) return;
+// GROUPED: the workgroup walks several pairs of one matrix with U and V
+// resident in LDS (three 64 x 64 buffers, one workgroup per CU); otherwise it
+// transforms one pair with two buffers (V, then U, reloaded), two per CU.
+template <bool GROUPED>
+__global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a) {
+    const int m = blockIdx.x, grp = blockIdx.y, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    __shared__ double Bs[64 * LDA];                  // V (and U unless GROUPED)
+    __shared__ double Us[GROUPED ? 64 * LDA : 1];    // U (GROUPED)
     __shared__ double As[64 * LDA];
-    __shared__ double Bs[64 * LDA];
     __shared__ double lam[64], ex[64];
-    __shared__ double red[4];
-    const int m = a.comp_mat[J];
-    const double wj = a.params[a.pl.off_w + J];
-    const double tau = a.tau[node];
-    for (int idx = tid; idx < 4096; idx += 256) {
-        As[(idx >> 6) * LDA + (idx & 63)] = a.Asum[(size_t)bi * 4096 + idx];
-        Bs[(idx >> 6) * LDA + (idx & 63)] = a.V[(size_t)m * 4096 + idx];
-    }
-    if (tid < 64) {
-        lam[tid] = a.lam[m * 64 + tid];
-        ex[tid] = exp(lam[tid] * tau);
-    }
-    __syncthreads();
-    f64x4 acc[4];
-    gemm64_acc<false, true>(As, Bs, acc, w, lr, lg);  // A V^T
-    __syncthreads();
-    gemm64_store(acc, As, w, lr, lg);
-    for (int idx = tid; idx < 4096; idx += 256) Bs[(idx >> 6) * LDA + (idx & 63)] = a.U[(size_t)m * 4096 + idx];
-    __syncthreads();
-    gemm64_acc<true, false>(Bs, As, acc, w, lr, lg);  // A~ = U^T (A V^T)
-    __syncthreads();
-    gemm64_store(acc, As, w, lr, lg);
-    __syncthreads();
-    double d0 = 0.0, d1 = 0.0;
-    double *Wout = a.W + ((size_t)bi * a.max_comp + jj) * 4096;
-    for (int idx = tid; idx < 4096; idx += 256) {
-        const int i = idx >> 6, j = idx & 63;
-        const double at = As[i * LDA + j];
-        if (i == j) {
-            d0 += ex[i] * at;
-            d1 += lam[i] * ex[i] * at;
+    __shared__ double red[8];
+    const int e0 = a.wl_ptr[m] + grp * a.grp_size, e1 = min(a.wl_ptr[m + 1], e0 + a.grp_size);
+    f64x4 H[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) H[q] = splat<f64x4>(0.0);
+    // A_b of the next pair is prefetched into registers while the current
+    // pair is transformed
+    double ra[16];
+    auto fetch_a = [&](int e) {
+        const int bi = a.wl_idx[e] / a.max_comp;
+        const double *src = a.Asum + (size_t)bi * 4096 + tid;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ra[k] = src[256 * k];
+    };
+    if (e0 < e1) {
+        fetch_a(e0);
+        for (int idx = tid; idx < 4096; idx += 256) {
+            if constexpr (GROUPED) Us[(idx >> 6) * LDA + (idx & 63)] = a.U[(size_t)m * 4096 + idx];
+            Bs[(idx >> 6) * LDA + (idx & 63)] = a.V[(size_t)m * 4096 + idx];
         }
-        const double d = lam[i] - lam[j];
-        const double E = (fabs(d) < 1e-9) ? tau * ex[i] : (ex[i] - ex[j]) / d;
-        Wout[idx] = wj * E * at;
+        if (tid < 64) lam[tid] = a.lam[m * 64 + tid];
     }
-    const double r0 = block_sum<256>(d0, red, tid);
-    const double r1 = block_sum<256>(d1, red, tid);
-    if (tid == 0) {
-        double *o = a.out + ((size_t)bi * a.max_comp + jj) * 8;
-        o[0] = r0;
-        o[1] = wj * r1;
+    for (int e = e0; e < e1; ++e) {
+        const int slot = a.wl_idx[e], bi = slot / a.max_comp, jj = slot - bi * a.max_comp;
+        const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
+        const int J = a.comp_ptr[cls] + jj;
+        const double wj = a.params[a.pl.off_w + J], tau = a.tau[node];
+        __syncthreads();  // the previous pair is done with As and ex
+        if (tid < 64) ex[tid] = exp(lam[tid] * tau);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int idx = tid + 256 * k;
+            As[(idx >> 6) * LDA + (idx & 63)] = ra[k];
+        }
+        if (e + 1 < e1) fetch_a(e + 1);
+        __syncthreads();
+        f64x4 acc[4];
+        gemm64_acc<false, true>(As, Bs, acc, w, lr, lg);  // A V^T
+        __syncthreads();
+        gemm64_store(acc, As, w, lr, lg);
+        if constexpr (!GROUPED)
+            for (int idx = tid; idx < 4096; idx += 256) Bs[(idx >> 6) * LDA + (idx & 63)] = a.U[(size_t)m * 4096 + idx];
+        __syncthreads();
+        gemm64_acc<true, false>(GROUPED ? Us : Bs, As, acc, w, lr, lg);  // A~ = U^T (A V^T), rows of this wave
+        double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * w + MF<double>::crow(lg, r), j = 16 * q + lr;
+                const double at = acc[q][r], ei = ex[i], d = lam[i] - lam[j];
+                if (i == j) {
+                    d0 += ei * at;
+                    d1 += lam[i] * ei * at;
+                }
+                const double E = (fabs(d) < 1e-9) ? tau * ei : (ei - ex[j]) / d;
+                H[q][r] += wj * E * at;
+            }
+        block_sum2<256>(d0, d1, red, tid);
+        if (tid == 0) {
+            double *o = a.out + ((size_t)bi * a.max_comp + jj) * 8;
+            o[0] = d0;
+            o[1] = wj * d1;
+        }
     }
+    double *Hp = a.W + ((size_t)m * a.n_grp + grp) * 4096;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Hp[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] = H[q][r];
 }
 
 __global__ __launch_bounds__(256) void k_hproj(FrechetArgs a) {
     const int m = blockIdx.x, ch = blockIdx.y, tid = threadIdx.x, idx = ch * 256 + tid;
     __shared__ double red[4];
-    const int e0 = a.wl_ptr[m], e1 = a.wl_ptr[m + 1];
+    const double *Hp = a.W + (size_t)m * a.n_grp * 4096 + idx;
     double h = 0.0;
-    int e = e0;
-    for (; e + 4 <= e1; e += 4) {
-        const double x0 = a.W[(size_t)a.wl_idx[e] * 4096 + idx], x1 = a.W[(size_t)a.wl_idx[e + 1] * 4096 + idx];
-        const double x2 = a.W[(size_t)a.wl_idx[e + 2] * 4096 + idx], x3 = a.W[(size_t)a.wl_idx[e + 3] * 4096 + idx];
-        h += x0;
-        h += x1;
-        h += x2;
-        h += x3;
+    int g = 0;
+    for (; g + 8 <= a.n_grp; g += 8) {  // eight loads in flight, then the sums in order
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = Hp[(size_t)(g + u) * 4096];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) h += x[u];
     }
-    for (; e < e1; ++e) h += a.W[(size_t)a.wl_idx[e] * 4096 + idx];
+    for (; g < a.n_grp; ++g) h += Hp[(size_t)g * 4096];
     const double *M = a.M + (size_t)m * 6 * 4096 + idx;
     double v[6];
 #pragma unroll
@@ -1877,7 +1908,10 @@ void launch_proj(const ProjArgs &a, int n_mat, hipStream_t s) {
 
 void launch_frechet(const FrechetArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_asum, dim3(a.n_branch, 16), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_frechet, dim3(a.n_branch, a.max_comp), dim3(256), 0, s, a);
+    if (a.grp_size > 1)
+        hipLaunchKernelGGL(k_frechet<true>, dim3(a.n_mat, a.n_grp), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_frechet<false>, dim3(a.n_mat, a.n_grp), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_hproj, dim3(a.n_mat, HPROJ_CHUNKS), dim3(256), 0, s, a);
 }
 
