@@ -230,7 +230,7 @@ int choose_tile(int P, int req, int prec, int n_cu) {
         const int slots = n_cu * ((prec == HY_PREC_FP32 && T == 16) ? 2 : 1);
         const int tiles = (P + T - 1) / T;
         const double cost = ((tiles + slots - 1) / slots) * node_cost[k];
-        if (cost <= best_cost) {
+        if (cost < best_cost) {  // ties go to the narrower tile (measured faster)
             best_cost = cost;
             best = T;
         }
