@@ -59,6 +59,7 @@ struct hy_engine_s {
     // Fused batch context (an internal engine holding the merged problem of a
     // group): gene table, tile -> gene and matrix -> gene maps.
     int n_genes = 1;
+    bool lean = false;  // schedules and inside launch of the lean variant
     DevBuf d_genes, d_tile_gene, d_mat_gene;
     size_t n_res = 0;  // doubles in the result vector
     hipStream_t stream = nullptr;
@@ -325,7 +326,8 @@ int prepare_buffers(hy_engine h, int prec) {
             release(h, h->d_Uspill[k]);
         }
     }
-    if (h->sched_key != T * 16 + (int)rs) {
+    h->lean = lean_inside(prec == HY_PREC_FP32, T, h->n_tiles, h->n_cu);
+    if (h->sched_key != T * 16 + (int)rs + (h->lean ? 4096 : 0)) {
         ++h->epoch;
         build_schedules(h, T, (int)rs);
         TRY(upload(h, h->d_sched, h->sched.data(), h->sched.size() * 4));
@@ -416,7 +418,7 @@ int prepare_buffers(hy_engine h, int prec) {
 // Inside / outside schedules with LDS slot assignment for tile width T and
 // element size esz.
 void build_schedules(hy_engine h, int T, int esz) {
-    const int NSI = inside_slots(T, esz), NSO = outside_slots(T, esz);
+    const int NSI = inside_slots(T, esz, h->lean), NSO = outside_slots(T, esz);
     const auto &ch = h->ext_children;
     const int n_ext = (int)ch.size();
     auto ref_of = [&](int c) -> int32_t {
@@ -507,7 +509,7 @@ void build_schedules(hy_engine h, int T, int esz) {
         h->osched.insert(h->osched.end(), e, e + OSCHED_STRIDE);
     }
     h->n_uspill = spills;
-    h->sched_key = T * 16 + esz;
+    h->sched_key = T * 16 + esz + (h->lean ? 4096 : 0);
 }
 
 void refresh_blist_classes(hy_engine h) {
@@ -641,7 +643,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     pr.site_ll = (double *)h->d_site.p;
     pr.genes = (const GeneInfo *)h->d_genes.p;
     pr.tile_gene = (const int32_t *)h->d_tile_gene.p;
-    launch_inside<R>(pr, h->T, s);
+    launch_inside<R>(pr, h->T, h->lean, s);
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[3], s));
     if (grad) {
         launch_outside<R>(pr, h->T, s);
@@ -1535,6 +1537,9 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
             }
         }
     }
+    long total_tiles = 0;
+    for (hy_engine h : hs) total_tiles += (h->P + T - 1) / T;
+    const bool lean = lean_inside(prec == HY_PREC_FP32, T, total_tiles, c->n_cu);
     std::vector<GeneInfo> genes(G);
     std::vector<int32_t> sched, osched, blist, comp_ptr{0}, comp_mat, tile_gene, mat_gene, wl;
     std::vector<uint16_t> tips;
@@ -1554,7 +1559,10 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
         own_sched.swap(h->sched);
         own_osched.swap(h->osched);
         const int own_uspill = h->n_uspill, own_key = h->sched_key;
+        const bool own_lean = h->lean;
+        h->lean = lean;
         build_schedules(h, T, esz);
+        h->lean = own_lean;
         std::vector<int32_t> g_sched, g_osched;
         g_sched.swap(h->sched);
         g_osched.swap(h->osched);
@@ -1660,7 +1668,7 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
     // the merged tables (gene-local partial / tip / slot indices, batch-global
     // nodes, classes, components and matrices); nothing is launched otherwise.
     {
-        const int NSI = inside_slots(T, esz), NSO = outside_slots(T, esz);
+        const int NSI = inside_slots(T, esz, lean), NSO = outside_slots(T, esz);
         auto bad = [&](int g, const char *what) { return fail(c, HY_ERR_STATE, "batch tables: gene %d: %s", g, what); };
         for (int g = 0; g < G; ++g) {
             const hy_engine h = hs[g];
@@ -1722,6 +1730,7 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
     c->code_id = h0->code_id;
     c->convention = h0->convention;
     c->n_genes = G;
+    c->lean = lean;
     c->T = T;
     c->T_req = T;
     c->n_tiles = tile0;

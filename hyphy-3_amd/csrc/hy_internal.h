@@ -44,7 +44,13 @@ constexpr int U_NEXT = -4;  // message handed to the next entry in registers
 // that the 16 MFMA B operands of one lane are contiguous.
 constexpr int bmaj_index(int s, int t) { return (((t >> 4) * 4 + (s & 3)) * 16 + (t & 15)) * 16 + (s >> 2); }
 
-constexpr int inside_slots(int T, int esz) { return T == 16 ? 12 : (T == 32 ? 6 : 3); }
+constexpr int inside_slots(int T, int esz, bool lean = false) { return lean ? 6 : (T == 16 ? 12 : (T == 32 ? 6 : 3)); }
+// "Lean" inside pass (FP32, T = 16, problems with many more tiles than CUs):
+// six LDS slots, tip codes read from HBM and no register double buffering,
+// so that four workgroups share a CU; throughput instead of latency.
+constexpr bool lean_inside(int prec_is_fp32, int T, long n_tiles, int n_cu) {
+    return prec_is_fp32 && T == 16 && n_tiles > 2L * n_cu;
+}
 constexpr int outside_slots(int T, int esz) { return T == 16 ? 10 : (T == 32 ? 4 : (esz == 8 ? 1 : 2)); }
 // Tip codes of a tile kept in LDS when (taxa + 1) * T fits.
 constexpr int tip_lds_codes(int T, int esz) {
@@ -214,7 +220,7 @@ struct FinalArgs {
 void launch_model(const ModelArgs &a, int n_mat, hipStream_t s);
 template <typename R> void launch_pmat(const PmatArgs<R> &a, hipStream_t s);
 template <typename R> void launch_pmat_host(const PmatArgs<R> &a, const double *Phost, hipStream_t s);
-template <typename R> void launch_inside(const PruneArgs<R> &a, int T, hipStream_t s);
+template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, hipStream_t s);
 template <typename R> void launch_outside(const PruneArgs<R> &a, int T, hipStream_t s);
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s);
 void launch_proj(const ProjArgs &a, int n_mat, hipStream_t s);

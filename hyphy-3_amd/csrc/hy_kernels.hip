@@ -893,12 +893,12 @@ __device__ __forceinline__ int gene_view(PruneArgs<R> &a, int tile) {
 // current node is computed; schedule entries arrive two nodes ahead.
 // TL: the tile's tip codes are staged in LDS.
 // ---------------------------------------------------------------------------
-template <typename R, int T, bool TL>
-__global__ __launch_bounds__(256, (sizeof(R) == 4 && T == 16) ? 2 : 1) void k_inside(PruneArgs<R> args) {
+template <typename R, int T, bool TL, bool LEAN = false>
+__global__ __launch_bounds__(256, LEAN ? 4 : ((sizeof(R) == 4 && T == 16) ? 2 : 1)) void k_inside(PruneArgs<R> args) {
     PruneArgs<R> a = args;
     const int tile = gene_view(a, blockIdx.x);
-    constexpr int NB = T / 16, NS = inside_slots(T, sizeof(R)), TIPC = TL ? tip_lds_codes(T, sizeof(R)) : 1;
-    constexpr bool PIPE = !(sizeof(R) == 8 && T == 64);  // register budget of double-buffered operands
+    constexpr int NB = T / 16, NS = inside_slots(T, sizeof(R), LEAN), TIPC = TL ? tip_lds_codes(T, sizeof(R)) : 1;
+    constexpr bool PIPE = !LEAN && !(sizeof(R) == 8 && T == 64);  // register budget of double-buffered operands
     typedef Entry<SCHED_STRIDE> Ent;
     typedef typename MF<R>::acc acc_t;
     __shared__ R Ds[NS * 64 * T];
@@ -1866,7 +1866,13 @@ template <typename R, int T> void launch_inside_t(const PruneArgs<R> &a, hipStre
         hipLaunchKernelGGL((k_inside<R, T, false>), dim3(a.n_tiles), dim3(256), 0, s, a);
 }
 
-template <typename R> void launch_inside(const PruneArgs<R> &a, int T, hipStream_t s) {
+template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, hipStream_t s) {
+    if constexpr (sizeof(R) == 4) {
+        if (lean && T == 16) {
+            hipLaunchKernelGGL((k_inside<R, 16, false, true>), dim3(a.n_tiles), dim3(256), 0, s, a);
+            return;
+        }
+    }
     switch (T) {
         case 16: launch_inside_t<R, 16>(a, s); break;
         case 32: launch_inside_t<R, 32>(a, s); break;
@@ -1923,8 +1929,8 @@ template void launch_pmat<float>(const PmatArgs<float> &, hipStream_t);
 template void launch_pmat<double>(const PmatArgs<double> &, hipStream_t);
 template void launch_pmat_host<float>(const PmatArgs<float> &, const double *, hipStream_t);
 template void launch_pmat_host<double>(const PmatArgs<double> &, const double *, hipStream_t);
-template void launch_inside<float>(const PruneArgs<float> &, int, hipStream_t);
-template void launch_inside<double>(const PruneArgs<double> &, int, hipStream_t);
+template void launch_inside<float>(const PruneArgs<float> &, int, bool, hipStream_t);
+template void launch_inside<double>(const PruneArgs<double> &, int, bool, hipStream_t);
 template void launch_outside<float>(const PruneArgs<float> &, int, hipStream_t);
 template void launch_outside<double>(const PruneArgs<double> &, int, hipStream_t);
 template void launch_adjoint<float>(const AdjArgs<float> &, int, hipStream_t);

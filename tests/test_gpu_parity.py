@@ -127,3 +127,28 @@ def test_multihit_rates_forward():
         for prec, tl in ((FP64, 1e-10), (FP32, 1e-5)):
             r = eng.eval(bl, [0.6], [1.0], sim["theta"], delta=delta, psi=psi, precision=prec, grad=False)
             assert rel(r["log_likelihood"], ref) < tl, (delta, psi, prec)
+
+
+def test_lean_inside_on_many_tiles():
+    """More FP32 tiles than two per CU switch the inside pass to its lean
+    variant (six LDS slots, tips from HBM, four workgroups per CU); results
+    match the oracle and the wide-tile path."""
+    omegas, weights = [0.2, 1.5], [0.7, 0.3]
+    tree, aln, sim = sim_problem(10, 9000, 77, omegas=omegas, weights=weights, gaps=0.05)
+    N = len(tree.nodes)
+    P = len(aln.weights)
+    assert P > 32 * 256  # > 2 tiles of 16 per CU on 256 CUs
+    bl = branch_lengths(tree)
+    mix = ho.Mixture(omegas, np.zeros(N, dtype=np.int32), [[(k, w) for k, w in enumerate(weights)]], 1)
+    ref = ho.mixture_eval(ho.Problem.from_refs(tree, aln), ho.Code(0), ho.params(1.0, 1.0, sim["theta"]),
+                          aln.pos_freqs, aln.cf3x4, mix, bl, threads=8)
+    eng = make_engine(tree, aln)
+    eng.set_mixture(2, np.zeros(N, dtype=np.int32), [[0, 1]], convention=1)
+    r = eng.eval(bl, omegas, weights, sim["theta"], precision=FP32)
+    assert eng.last_tile() == 16
+    assert rel(r["log_likelihood"], ref["log_likelihood"]) < 1e-5
+    for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
+        assert grad_err(r[k], ref[k]) < 1e-4, k
+    eng.set_tile(32)
+    r32 = eng.eval(bl, omegas, weights, sim["theta"], precision=FP32)
+    assert rel(r["log_likelihood"], r32["log_likelihood"]) < 1e-6
