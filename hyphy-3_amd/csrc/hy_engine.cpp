@@ -33,19 +33,15 @@ struct DevBuf {
 
 }  // namespace
 
-// The streams and fork/join events an engine evaluates on.  An engine made by
-// hy_create owns one set; engines made together by hy_create_group share a
-// few sets round-robin, so that thousands of per-gene engines do not hold
-// thousands of streams and events.  Destroyed with its last engine.
+// The stream an engine evaluates on.  An engine made by hy_create owns one;
+// engines made together by hy_create_group share a few round-robin, so that
+// thousands of per-gene engines do not hold thousands of streams.  Destroyed
+// with its last engine.
 struct StreamSet {
     int device = 0;
-    hipStream_t s1 = nullptr, s2 = nullptr;
-    hipEvent_t ev_model = nullptr, ev_proj = nullptr;
+    hipStream_t s1 = nullptr;
     ~StreamSet() {
         (void)hipSetDevice(device);
-        if (ev_model) (void)hipEventDestroy(ev_model);
-        if (ev_proj) (void)hipEventDestroy(ev_proj);
-        if (s2) (void)hipStreamDestroy(s2);
         if (s1) (void)hipStreamDestroy(s1);
     }
 };
@@ -63,11 +59,8 @@ struct hy_engine_s {
     DevBuf d_genes, d_tile_gene, d_mat_gene;
     size_t n_res = 0;  // doubles in the result vector
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;  // side stream: dQ projections overlap the pruning passes
-    hipEvent_t ev_model = nullptr, ev_proj = nullptr;
-    // Captured evaluation graphs (parameter upload, every kernel on both
-    // streams, result download), replayed while the configuration epoch is
-    // unchanged.
+    // Captured evaluation graphs (parameter upload, every kernel, result
+    // download), replayed while the configuration epoch is unchanged.
     struct GraphEntry {
         int prec, grad, warm;
         uint64_t epoch;
@@ -556,11 +549,10 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         ma.status = (int *)h->d_status.p;
         ma.mat_gene = (const int32_t *)h->d_mat_gene.p;
         launch_model(ma, h->n_mat, s);
+        // M_phi = V dQ_phi U and d mu / d phi depend on the spectra only: they
+        // are built by the workgroups of the P-build launch beyond the branches
+        ProjArgs pj{};
         if (grad) {
-            // M_phi = V dQ_phi U and d mu / d phi depend on the spectra only
-            HIP_TRY(h, hipEventRecord(h->ev_model, s));
-            HIP_TRY(h, hipStreamWaitEvent(h->stream2, h->ev_model, 0));
-            ProjArgs pj{};
             pj.qdesc = (const int32_t *)h->d_qdesc.p;
             pj.nd = (const int8_t *)h->d_nd.p;
             pj.dpos = (const int8_t *)h->d_dpos.p;
@@ -577,8 +569,6 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
             pj.M = (double *)h->d_M.p;
             pj.dmu = (double *)h->d_dmu.p;
             pj.mat_gene = (const int32_t *)h->d_mat_gene.p;
-            launch_proj(pj, h->n_mat, h->stream2);
-            HIP_TRY(h, hipEventRecord(h->ev_proj, h->stream2));
         }
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[1], s));
         PmatArgs<R> pa{};
@@ -599,7 +589,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         pa.Rsum = (R *)h->d_Rsum[ri].p;
         pa.PA = (R *)h->d_PA[ri].p;
         pa.PTA = (R *)h->d_PTA[ri].p;
-        if (h->n_branch > 0) launch_pmat<R>(pa, s);
+        launch_pmat<R>(pa, grad ? &pj : nullptr, h->n_mat, s);
     } else {
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[1], s));
         PmatArgs<R> pa{};
@@ -669,7 +659,6 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         aa.genes = (const GeneInfo *)h->d_genes.p;
         if (h->n_branch > 0) launch_adjoint<R>(aa, h->T, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[5], s));
-        HIP_TRY(h, hipStreamWaitEvent(s, h->ev_proj, 0));
         FrechetArgs fa{};
         fa.blist = (const int32_t *)h->d_blist.p;
         fa.n_branch = h->n_branch;
@@ -739,9 +728,6 @@ hipError_t make_stream_set(int device, std::shared_ptr<StreamSet> &out) {
     ss->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ss->s1, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ss->s2, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ss->ev_model, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ss->ev_proj, hipEventDisableTiming);
     if (e == hipSuccess) out = ss;
     return e;
 }
@@ -752,9 +738,6 @@ hy_engine new_engine(int device, int n_cu, const std::shared_ptr<StreamSet> &ss)
     h->n_cu = n_cu > 0 ? n_cu : 256;
     h->streams = ss;
     h->stream = ss->s1;
-    h->stream2 = ss->s2;
-    h->ev_model = ss->ev_model;
-    h->ev_proj = ss->ev_proj;
     const char *pe = std::getenv("HY_PROFILE");
     h->profile = pe && pe[0] == '1';
     const char *ng = std::getenv("HY_NO_GRAPH");

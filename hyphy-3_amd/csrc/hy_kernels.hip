@@ -5,12 +5,13 @@
 //              (warm-started Ogita-Aishima refinement on FP64 MFMA, or
 //              parallel cyclic Jacobi in LDS when cold)
 //   k_pmat     one workgroup per branch: P_b = sum_j w_j exp(Q_j tau_b) in FP64,
-//              written as MFMA A-operand tiles of P and P^T in the working type
+//              written as MFMA A-operand tiles of P and P^T in the working type;
+//              with gradients, further (matrix, parameter) workgroups build
+//              M = V_pi dQ U_pi (the projections below) in the same launch
 //   k_inside   one workgroup per pattern tile: post-order pruning, 64x64 by
 //              64xT child products on MFMA, per-pattern power-of-two rescale
 //   k_outside  one workgroup per pattern tile: pre-order outside messages
 //   k_adjoint  (branch, pattern chunk) workgroups: A_b = sum_p c_p V_b D_b^T
-//   k_proj     (matrix, parameter) workgroups: M = V_pi dQ U_pi
 //   k_frechet  (branch, component) workgroups: U^T A V^T, spectral Frechet
 //              projections (FP64)
 //   k_final    one workgroup of 16 waves: deterministic compensated reductions
@@ -512,11 +513,10 @@ __device__ __forceinline__ void emit_p(const double *P, int node, int S, R *Prow
 // k_pmat: mixed transition matrices per branch.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256, 2) void k_pmat(PmatArgs<R> a) {
-    const int bi = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+__device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, double *Us, double *Vs) {
+    // Us: U diag(e^{lambda tau}), the mixed P at the end; Vs: V (64 x LDA each)
+    const int tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
-    __shared__ double Us[64 * LDA];  // U diag(e^{lambda tau}); the mixed P at the end
-    __shared__ double Vs[64 * LDA];
     __shared__ double ex[64];
     const double *prm = a.params;
     const int j0 = a.comp_ptr[cls], j1 = a.comp_ptr[cls + 1];
@@ -1479,15 +1479,14 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
 }
 
 // ---------------------------------------------------------------------------
-// k_proj: M_phi = V_pi dQ_phi U_pi and dmu_phi = -sum_i pi_i dQ_phi[i][i]
+// Projections: M_phi = V_pi dQ_phi U_pi and dmu_phi = -sum_i pi_i dQ_phi[i][i]
 // (likelihood.hpp:1497-1562).  phi 0 = omega, 1..5 = theta AC, AT, CG, CT, GT.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256, 2) void k_proj(ProjArgs args) {
+__device__ __forceinline__ void proj_body(const ProjArgs &args, int m, int phi, double *X, double *Y) {
+    // X: dQ_phi, then dQ_phi U; Y: U, then V (64 x LDA each)
     ProjArgs a = args;
-    matrix_gene_view(a, blockIdx.x);
-    const int m = blockIdx.x, phi = blockIdx.y, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
-    __shared__ double X[64 * LDA];  // dQ_phi, then dQ_phi U
-    __shared__ double Y[64 * LDA];  // U, then V
+    matrix_gene_view(a, m);
+    const int tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     __shared__ double thl[8], posl[12];
     const double *prm = a.params;
     const double alpha = prm[a.pl.off_alpha], beta = prm[a.pl.off_omega + m];
@@ -1534,6 +1533,18 @@ __global__ __launch_bounds__(256, 2) void k_proj(ProjArgs args) {
     for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int r = 0; r < 4; ++r) M[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] = acc[q][r];
+}
+
+// P builds and (with gradients) the dQ projections depend only on the
+// spectra: one launch, workgroups [0, n_branch) build P, the rest project.
+template <typename R>
+__global__ __launch_bounds__(256, 2) void k_pmat(PmatArgs<R> a, ProjArgs pj, int n_proj) {
+    __shared__ double L0[64 * LDA], L1[64 * LDA];  // one pair of tiles, shared by both roles
+    const int b = blockIdx.x;
+    if (b < a.n_branch)
+        pmat_body<R>(a, b, L0, L1);
+    else if (b - a.n_branch < n_proj)
+        proj_body(pj, (b - a.n_branch) / 6, (b - a.n_branch) % 6, L0, L1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1855,8 +1866,10 @@ void launch_model(const ModelArgs &a, int n_mat, hipStream_t s) {
     hipLaunchKernelGGL(k_model, dim3(n_mat), dim3(MODEL_NT), 0, s, a);
 }
 
-template <typename R> void launch_pmat(const PmatArgs<R> &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_pmat<R>, dim3(a.n_branch), dim3(256), 0, s, a);
+template <typename R> void launch_pmat(const PmatArgs<R> &a, const ProjArgs *pj, int n_mat, hipStream_t s) {
+    const int n_proj = pj ? 6 * n_mat : 0;
+    if (a.n_branch + n_proj > 0)
+        hipLaunchKernelGGL(k_pmat<R>, dim3(a.n_branch + n_proj), dim3(256), 0, s, a, pj ? *pj : ProjArgs{}, n_proj);
 }
 
 template <typename R, int T> void launch_inside_t(const PruneArgs<R> &a, hipStream_t s) {
@@ -1908,9 +1921,6 @@ template <typename R> void launch_pmat_host(const PmatArgs<R> &a, const double *
     hipLaunchKernelGGL(k_pmat_host<R>, dim3(a.n_branch), dim3(256), 0, s, a, Phost);
 }
 
-void launch_proj(const ProjArgs &a, int n_mat, hipStream_t s) {
-    hipLaunchKernelGGL(k_proj, dim3(n_mat, 6), dim3(256), 0, s, a);
-}
 
 void launch_frechet(const FrechetArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_asum, dim3(a.n_branch, 16), dim3(256), 0, s, a);
@@ -1925,8 +1935,8 @@ void launch_final(const FinalArgs &a, int n_genes, hipStream_t s) {
     hipLaunchKernelGGL(k_final, dim3(n_genes), dim3(FINAL_NT), 0, s, a);
 }
 
-template void launch_pmat<float>(const PmatArgs<float> &, hipStream_t);
-template void launch_pmat<double>(const PmatArgs<double> &, hipStream_t);
+template void launch_pmat<float>(const PmatArgs<float> &, const ProjArgs *, int, hipStream_t);
+template void launch_pmat<double>(const PmatArgs<double> &, const ProjArgs *, int, hipStream_t);
 template void launch_pmat_host<float>(const PmatArgs<float> &, const double *, hipStream_t);
 template void launch_pmat_host<double>(const PmatArgs<double> &, const double *, hipStream_t);
 template void launch_inside<float>(const PruneArgs<float> &, int, bool, hipStream_t);
