@@ -1362,71 +1362,63 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
     const int t0 = chunk * a.tiles_per_chunk;
     const int t1 = min(t0 + a.tiles_per_chunk, a.n_tiles);
     const int ng = (t1 - t0 + G - 1) / G;
-    R rv[PER], rd[PER];
-    R rc = 0;
-    unsigned rcode = 0;
-    auto fetch = [&](int g) {
+    // Two register sets of fetched groups: group g + 2 is requested while
+    // group g is multiplied, so two groups' loads are in flight.  A fetch only
+    // issues loads, all from valid addresses (runs past the chunk re-read its
+    // first tile and get c_p = 0), so that no use or register merge of a
+    // loaded value makes the wave wait for them before the group is stashed.
+    struct Regs {
+        R v[PER], d[PER];
+        double L, wt;
+        int gv, er, ev, pv;
+        unsigned code;
+    };
+    const R *Vbr = a.V + (size_t)vslot * a.n_tiles * (64 * T);
+    const R *Dbr = tip ? Vbr : a.D + (size_t)idx * a.n_tiles * (64 * T);  // tips: D unused
+    const int32_t *Gr = a.G + (size_t)vslot * a.P_pad;
+    const int32_t *Er = a.E + (size_t)(tip ? 0 : idx) * a.P_pad;
+    const uint16_t *Cr = a.tips + (size_t)(tip ? idx : 0) * a.P_pad;
+    auto fetch = [&](Regs &x, int g) {
         const int tg0 = t0 + g * G;
-        const R *Vg = a.V + ((size_t)vslot * a.n_tiles + tg0) * (64 * T);
-        const R *Dg = a.D + ((size_t)idx * a.n_tiles + tg0) * (64 * T);
+        {
+            // every wave issues the same loads (waves 1..3 duplicate wave 0's)
+            const int p = tg0 * T + l;
+            x.pv = p < t1 * T;
+            const int pc = x.pv ? p : tg0 * T;
+            x.L = a.Lroot[pc];
+            x.wt = a.weights[pc];
+            x.gv = Gr[pc];
+            x.er = a.Eroot[pc];
+            x.ev = Er[pc];
+            x.code = Cr[pc];
+        }
         // each thread loads one contiguous run of PER elements (128-bit loads;
         // a run never crosses a tile)
-        {
-            const int e0 = tid * PER;
-            const bool valid = tg0 + e0 / (64 * T) < t1;
-            if (valid) {
-                lds_run16<R>(rv, Vg + e0);
-                if (!tip)
-                    lds_run16<R>(rd, Dg + e0);
-                else
-#pragma unroll
-                    for (int q = 0; q < PER; ++q) rd[q] = (R)0;
-            } else {
-#pragma unroll
-                for (int q = 0; q < PER; ++q) rv[q] = rd[q] = (R)0;
-            }
-        }
-        if (tid < 64) {
-            const int p = tg0 * T + tid;
-            double c = 0.0;
-            rcode = 0;
-            if (p < t1 * T) {
-                const double L = a.Lroot[p], wt = a.weights[p];
-                if (wt > 0.0 && L > 0.0) {
-                    int ex = a.G[(size_t)vslot * a.P_pad + p] - a.Eroot[p];
-                    if (!tip) ex += a.E[(size_t)idx * a.P_pad + p];
-                    c = ldexp(wt / L, ex);
-                }
-                if (tip) rcode = a.tips[(size_t)idx * a.P_pad + p];
-            }
-            rc = (R)c;
-        }
+        const int e0 = tid * PER;
+        const int ec = (tg0 + e0 / (64 * T) < t1) ? e0 : e0 % (64 * T);
+        lds_run16<R>(x.v, Vbr + (size_t)tg0 * (64 * T) + ec);
+        lds_run16<R>(x.d, Dbr + (size_t)tg0 * (64 * T) + ec);
     };
-    auto stash = [&](int buf) {
+    auto stash = [&](const Regs &x, int buf) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = tid * PER + q, ti = e / (64 * T), el = e % (64 * T);
             // V tiles are [state][pattern]; D tiles B-operand-major (bmaj_index)
-            Vs[buf][adj_idx(el / T, ti * T + el % T)] = rv[q];
+            Vs[buf][adj_idx(el / T, ti * T + el % T)] = x.v[q];
             const int sd = ((el & 15) << 2) | ((el >> 8) & 3), td = ((el >> 10) << 4) | ((el >> 4) & 15);
-            Ds[buf][adj_idx(sd, ti * T + td)] = rd[q];
+            Ds[buf][adj_idx(sd, ti * T + td)] = x.d[q];
         }
         if (tid < 64) {
-            cs[buf][adj_idx(0, tid)] = rc;
-            codes[buf][adj_idx(0, tid)] = rcode;
+            double c = 0.0;
+            if (x.pv && x.wt > 0.0 && x.L > 0.0) c = ldexp(x.wt / x.L, x.gv - x.er + (tip ? 0 : x.ev));
+            cs[buf][adj_idx(0, tid)] = (R)c;
+            codes[buf][adj_idx(0, tid)] = tip ? x.code : 0u;
         }
     };
     f64x4 acc[4];
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[cb] = splat<f64x4>(0.0);
-    if (ng > 0) {
-        fetch(0);
-        stash(0);
-    }
-    __syncthreads();
-    for (int g = 0; g < ng; ++g) {
-        const int buf = g & 1;
-        if (g + 1 < ng) fetch(g + 1);
+    auto multiply = [&](int buf) {
         acc_t loc[4];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) loc[cb] = splat<acc_t>(0.0);
@@ -1468,8 +1460,28 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[cb][r] += (double)loc[cb][r];
-        if (g + 1 < ng) stash(buf ^ 1);
+    };
+    // Group k travels in register set k % 2 and LDS buffer k % 2.  Fetches
+    // past the last group re-read it (from L2) so that every iteration issues
+    // the same loads: the compiler then waits for exactly the group a stash
+    // needs and leaves the newer group's loads in flight.
+    Regs x0, x1;
+    if (ng > 0) {
+        fetch(x0, 0);
+        stash(x0, 0);
+        fetch(x1, min(1, ng - 1));
         __syncthreads();
+        for (int g = 0; g < ng; g += 2) {
+            fetch(x0, min(g + 2, ng - 1));
+            multiply(0);
+            stash(x1, 1);
+            __syncthreads();
+            if (g + 1 >= ng) break;
+            fetch(x1, min(g + 3, ng - 1));
+            multiply(1);
+            stash(x0, 0);
+            __syncthreads();
+        }
     }
     double *out = a.Apart + ((size_t)bi * a.n_chunks + chunk) * 4096;
 #pragma unroll
