@@ -636,7 +636,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     launch_inside<R>(pr, h->T, h->lean, s);
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[3], s));
     if (grad) {
-        launch_outside<R>(pr, h->T, s);
+        launch_outside<R>(pr, h->T, custom_pi == nullptr, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[4], s));
         AdjArgs<R> aa{};
         aa.blist = (const int32_t *)h->d_blist.p;
@@ -686,7 +686,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         if (h->n_branch > 0) launch_frechet(fa, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[6], s));
     } else {
-        if (flags & RUN_OUTSIDE) launch_outside<R>(pr, h->T, s);
+        if (flags & RUN_OUTSIDE) launch_outside<R>(pr, h->T, custom_pi == nullptr, s);
         if (prof)
             for (int k = 4; k <= 6; ++k) HIP_TRY(h, hipEventRecord(h->ev[k], s));
     }

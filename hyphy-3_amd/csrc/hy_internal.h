@@ -222,7 +222,8 @@ void launch_model(const ModelArgs &a, int n_mat, hipStream_t s);
 template <typename R> void launch_pmat(const PmatArgs<R> &a, const ProjArgs *pj, int n_mat, hipStream_t s);
 template <typename R> void launch_pmat_host(const PmatArgs<R> &a, const double *Phost, hipStream_t s);
 template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, hipStream_t s);
-template <typename R> void launch_outside(const PruneArgs<R> &a, int T, hipStream_t s);
+// rev: every edge matrix is reversible with respect to a.pi (built from the model)
+template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, hipStream_t s);
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s);
 void launch_frechet(const FrechetArgs &a, hipStream_t s);  // chunk sums, spectral transforms, projections
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s);  // one workgroup per gene

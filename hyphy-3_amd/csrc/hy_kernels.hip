@@ -1035,8 +1035,14 @@ __global__ __launch_bounds__(256, LEAN ? 4 : ((sizeof(R) == 4 && T == 16) ? 2 : 
 // free).  The next node's operands are fetched once the current node's
 // products are formed; with DREG the children's partials come along as MFMA
 // B operands, otherwise they are read from HBM when used.
+// REV: the edge matrices are reversible with respect to the root
+// frequencies (every matrix built from the model), P^T = diag(pi) P
+// diag(pi)^-1, so U_c = pi (.) (P_c (V_c / pi)) reuses the P_c operands of the
+// X product; no transposed tiles are loaded, and the next node's P tiles go
+// to a second register set.  Otherwise (caller-supplied matrices) U_c is
+// formed from the transposed tiles.
 // ---------------------------------------------------------------------------
-template <typename R, int T, bool TL>
+template <typename R, int T, bool TL, bool REV>
 __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
     PruneArgs<R> a = args;
     const int tile = gene_view(a, blockIdx.x);
@@ -1059,6 +1065,8 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
     // orders internal children first and tips last, so a third internal child
     // (only at nodes with three internal children) is loaded when used.
     constexpr int RS = 2;
+    // pa: P tiles of the current node's children; pt: their transposes
+    // (!REV) or the next node's P tiles (REV)
     R pa[RS][16], pt[RS][16], dv[RS][DV];
     int de[MAXC][NB];
     acc_t xt[MAXC][NB];
@@ -1069,6 +1077,12 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) upi[nb][r] = (R)a.pi[16 * w + MF<R>::crow(lg, r)];
+    R ipi[4];  // 1 / pi of the lane's rows (0 where pi = 0)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double pv = a.pi[16 * w + MF<R>::crow(lg, r)];
+        ipi[r] = pv > 0.0 ? (R)(1.0 / pv) : (R)0;
+    }
     auto operands = [&](const Ent &e) {
         const int nc = e.v[2];
         ab = 0;
@@ -1084,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
             if (c < nc) {
                 if (kind != KIND_TIP) {
                     if (c < RS) {
-                        fetch_edge<R>(pa[c], a.PA, pn, w, l);
+                        fetch_edge<R>(REV ? pt[c] : pa[c], a.PA, pn, w, l);
                         if constexpr (DREG)
                             fetch_b<R, T>(dv[c], a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
                     }
@@ -1106,6 +1120,11 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
     }
     int cb = 0;
     for (int k = 0; k < n; ++k) {
+        if constexpr (REV)
+#pragma unroll
+            for (int c = 0; c < RS; ++c)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) pa[c][i] = pt[c][i];
         const Ent enn = load_entry_v<OSCHED_STRIDE>(a.osched, k + 2 < n ? k + 2 : n - 1);
         const int nc = ec.v[2], usrc = ec.v[5];
         bool any_int = false;
@@ -1211,9 +1230,10 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
         // only after those MFMAs' results have been consumed (X by the
         // products above, the previous U_c by U): transposed edge tiles of this
         // node and all operands of the next one.
+        if constexpr (!REV)
 #pragma unroll
-        for (int c = 0; c < RS; ++c)
-            if (c < nc && (ec.v[7 + 4 * c] >> 28) == KIND_INT) fetch_edge<R>(pt[c], a.PTA, ec.v[6 + 4 * c], w, l);
+            for (int c = 0; c < RS; ++c)
+                if (c < nc && (ec.v[7 + 4 * c] >> 28) == KIND_INT) fetch_edge<R>(pt[c], a.PTA, ec.v[6 + 4 * c], w, l);
         if (k + 1 < n) operands(en);
         __syncthreads();
 #pragma unroll
@@ -1234,7 +1254,18 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
                     G[c][nb] += ex;
                 }
                 store_c<R, T>(a.V + ((size_t)vsc * a.n_tiles + tile) * (64 * T), V[c], w, lr, lg);
-                if ((ref >> 28) == KIND_INT) store_c<R, T>(Xc + c * 64 * T, V[c], w, lr, lg);
+                if ((ref >> 28) == KIND_INT) {
+                    if constexpr (REV) {
+                        acc_t Vs_[NB];
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) Vs_[nb][r] = V[c][nb][r] * ipi[r];
+                        store_c<R, T>(Xc + c * 64 * T, Vs_, w, lr, lg);
+                    } else {
+                        store_c<R, T>(Xc + c * 64 * T, V[c], w, lr, lg);
+                    }
+                }
                 if (w == 0 && lg == 0) {
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) a.G[(size_t)vsc * a.P_pad + p0 + 16 * nb + lr] = G[c][nb];
@@ -1251,12 +1282,17 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
                 if (kind != KIND_TIP) {
                     acc_t Uc[NB];
                     if (c < RS) {
-                        mfma_tile<R, T>(pt[c < RS ? c : 0], Xc + c * 64 * T, Uc, lr, lg);
+                        mfma_tile<R, T>(REV ? pa[c < RS ? c : 0] : pt[c < RS ? c : 0], Xc + c * 64 * T, Uc, lr, lg);
                     } else {
                         R av[16];
-                        fetch_edge<R>(av, a.PTA, ec.v[6 + 4 * c], w, l);
+                        fetch_edge<R>(av, REV ? a.PA : a.PTA, ec.v[6 + 4 * c], w, l);
                         mfma_tile<R, T>(av, Xc + c * 64 * T, Uc, lr, lg);
                     }
+                    if constexpr (REV)
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) Uc[nb][r] *= upi[nb][r];
                     if (dst == U_NEXT) {
 #pragma unroll
                         for (int nb = 0; nb < NB; ++nb) {
@@ -1905,18 +1941,18 @@ template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean
     }
 }
 
-template <typename R, int T> void launch_outside_t(const PruneArgs<R> &a, hipStream_t s) {
+template <typename R, int T, bool REV> void launch_outside_t(const PruneArgs<R> &a, hipStream_t s) {
     if (a.tip_rows * T <= tip_lds_codes(T, sizeof(R)))
-        hipLaunchKernelGGL((k_outside<R, T, true>), dim3(a.n_tiles), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_outside<R, T, true, REV>), dim3(a.n_tiles), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((k_outside<R, T, false>), dim3(a.n_tiles), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_outside<R, T, false, REV>), dim3(a.n_tiles), dim3(256), 0, s, a);
 }
 
-template <typename R> void launch_outside(const PruneArgs<R> &a, int T, hipStream_t s) {
+template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, hipStream_t s) {
     switch (T) {
-        case 16: launch_outside_t<R, 16>(a, s); break;
-        case 32: launch_outside_t<R, 32>(a, s); break;
-        default: launch_outside_t<R, 64>(a, s); break;
+        case 16: rev ? launch_outside_t<R, 16, true>(a, s) : launch_outside_t<R, 16, false>(a, s); break;
+        case 32: rev ? launch_outside_t<R, 32, true>(a, s) : launch_outside_t<R, 32, false>(a, s); break;
+        default: rev ? launch_outside_t<R, 64, true>(a, s) : launch_outside_t<R, 64, false>(a, s); break;
     }
 }
 
@@ -1953,8 +1989,8 @@ template void launch_pmat_host<float>(const PmatArgs<float> &, const double *, h
 template void launch_pmat_host<double>(const PmatArgs<double> &, const double *, hipStream_t);
 template void launch_inside<float>(const PruneArgs<float> &, int, bool, hipStream_t);
 template void launch_inside<double>(const PruneArgs<double> &, int, bool, hipStream_t);
-template void launch_outside<float>(const PruneArgs<float> &, int, hipStream_t);
-template void launch_outside<double>(const PruneArgs<double> &, int, hipStream_t);
+template void launch_outside<float>(const PruneArgs<float> &, int, bool, hipStream_t);
+template void launch_outside<double>(const PruneArgs<double> &, int, bool, hipStream_t);
 template void launch_adjoint<float>(const AdjArgs<float> &, int, hipStream_t);
 template void launch_adjoint<double>(const AdjArgs<double> &, int, hipStream_t);
 
