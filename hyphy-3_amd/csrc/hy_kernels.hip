@@ -626,6 +626,13 @@ template <int N> __device__ __forceinline__ Entry<N> uniform(const Entry<N> &x) 
     for (int i = 0; i < N; ++i) e.v[i] = __builtin_amdgcn_readfirstlane(x.v[i]);
     return e;
 }
+// Materialises a uniform entry in scalar registers at this point of the
+// program: the wait for its (vector) load is placed here, before loads issued
+// later, instead of where the entry is first used.
+template <int N> __device__ __forceinline__ void pin(const Entry<N> &e) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) asm volatile("" ::"s"(e.v[i]));
+}
 
 // Column-wise power-of-two normalisation of a 64xT tile held in C layout by
 // 4 waves (wave w holds rows 16w..16w+15).  ex[nb] receives the exponent e
@@ -1230,6 +1237,10 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
         // only after those MFMAs' results have been consumed (X by the
         // products above, the previous U_c by U): transposed edge tiles of this
         // node and all operands of the next one.
+        // entry k + 2 to scalar registers before the next operands are
+        // requested, so that this wait does not cover them
+        const Ent enu = uniform(enn);
+        pin(enu);
         if constexpr (!REV)
 #pragma unroll
             for (int c = 0; c < RS; ++c)
@@ -1312,7 +1323,7 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
             }
         }
         ec = en;
-        en = uniform(enn);
+        en = enu;
     }
 }
 
