@@ -347,11 +347,13 @@ int prepare_buffers(hy_engine h, int prec) {
         h->ident_tiles[ri] = h->d_PA[ri].p;
     }
     // adjoint chunking: workgroups take whole 64-pattern groups (64 / T tiles);
-    // aim for >= 1024 workgroups (4 per CU)
+    // as many chunks per branch as fill the resident workgroup slots (two per
+    // CU in FP32, one in FP64) in one round, so no partial last round
     int nb = std::max(1, h->n_branch);
     const int G = 64 / T;
     const int n_groups = (h->n_tiles + G - 1) / G;
-    const int nchg = std::min(n_groups, std::max(1, (1024 + nb - 1) / nb));
+    const int slots = h->n_cu * (rs == 4 ? 2 : 1);
+    const int nchg = std::min(n_groups, std::max(1, slots / nb));
     h->tiles_per_chunk = ((n_groups + nchg - 1) / nchg) * G;
     h->n_chunks = (h->n_tiles + h->tiles_per_chunk - 1) / h->tiles_per_chunk;
     TRY(ensure(h, h->d_Apart, (size_t)nb * h->n_chunks * 4096 * 8));
