@@ -97,6 +97,7 @@ struct hy_engine_s {
     std::vector<uint64_t> amb;
     std::vector<double> weights;
     DevBuf d_sched, d_blist, d_tips, d_amb, d_weights;
+    int n_amb = 0;  // ambiguity masks in d_amb
     // Per-evaluation pattern weights replacing d_weights in the adjoint and the
     // final reductions (posterior-weighted passes of the synonymous-rate
     // mixture); nullptr = the problem's weights.  Evaluations with it set run
@@ -654,6 +655,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         aa.weights = h->wover ? h->wover : (const double *)h->d_weights.p;
         aa.tips = pr.tips;
         aa.amb = pr.amb;
+        aa.n_amb = h->n_amb;
         aa.P_pad = h->P_pad;
         aa.n_tiles = h->n_tiles;
         aa.S = h->S;
@@ -1108,6 +1110,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     h->warm_valid = false;
     TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), BR_STRIDE) * 4));
     TRY(upload(h, h->d_amb, h->amb.data(), h->amb.size() * 8));
+    h->n_amb = (int)h->amb.size();
     TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
     TRY(upload(h, h->d_comp_mat, h->comp_mat.data(), h->comp_mat.size() * 4));
     h->have_problem = true;
@@ -1771,6 +1774,7 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
     TRY(upload(c, c->d_comp_mat, comp_mat.data(), comp_mat.size() * 4));
     TRY(upload(c, c->d_tips, tips.data(), std::max<size_t>(tips.size(), 1) * 2));
     TRY(upload(c, c->d_amb, amb.data(), std::max<size_t>(amb.size(), 1) * 8));
+    c->n_amb = (int)amb.size();
     TRY(upload(c, c->d_weights, weights.data(), std::max<size_t>(weights.size(), 1) * 8));
     TRY(upload(c, c->d_pos, pos.data(), pos.size() * 8));
     TRY(upload(c, c->d_pi, pi.data(), pi.size() * 8));

@@ -168,6 +168,7 @@ struct AdjArgs {
     int P_pad, n_tiles, S;
     double *Apart;  // [n_branch][n_chunks][64][64]
     const GeneInfo *genes;  // batched evaluation (branch gene in the branch list)
+    int n_amb;              // entries of the ambiguity-mask table
 };
 
 struct ProjArgs {

@@ -1371,7 +1371,11 @@ __device__ __forceinline__ R tip_onehot(unsigned code, int j, const uint64_t *am
     return (R)((amb[code & 0x7FFFu] >> j) & 1ull);
 }
 
-template <typename R, int T>
+// Ambiguity masks staged in LDS by the adjoint (larger tables: AMBG variant,
+// read from HBM).
+constexpr int AMB_LDS = 512;
+
+template <typename R, int T, bool AMBG>
 __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
     AdjArgs<R> a = args;
     // Works on groups of 64 patterns (G pruning tiles, contiguous in HBM):
@@ -1385,7 +1389,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
     __shared__ __align__(16) R Vs[2][64 * ADJ_LD];
     __shared__ __align__(16) R Ds[2][64 * ADJ_LD];
     __shared__ __align__(16) R cs[2][64];
-    __shared__ __align__(16) unsigned codes[2][64];
+    __shared__ uint64_t ambl[AMBG ? 1 : AMB_LDS];
     const int chunk = blockIdx.x, bi = blockIdx.y;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
     const int32_t *br = a.blist + bi * BR_STRIDE;
@@ -1406,6 +1410,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
     const int vslot = br[1], ref = br[2];
     const int kind = ref >> 28, idx = ref & REF_MASK;
     const bool tip = kind == KIND_TIP;
+    const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
     const int t0 = chunk * a.tiles_per_chunk;
     const int t1 = min(t0 + a.tiles_per_chunk, a.n_tiles);
     const int ng = (t1 - t0 + G - 1) / G;
@@ -1418,7 +1423,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
         R v[PER], d[PER];
         double L, wt;
         int gv, er, ev, pv;
-        unsigned code;
+        unsigned tcode;  // tips: code of the pattern of this thread's D elements
     };
     const R *Vbr = a.V + (size_t)vslot * a.n_tiles * (64 * T);
     const R *Dbr = tip ? Vbr : a.D + (size_t)idx * a.n_tiles * (64 * T);  // tips: D unused
@@ -1437,7 +1442,6 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
             x.gv = Gr[pc];
             x.er = a.Eroot[pc];
             x.ev = Er[pc];
-            x.code = Cr[pc];
         }
         // each thread loads one contiguous run of PER elements (128-bit loads;
         // a run never crosses a tile)
@@ -1445,21 +1449,31 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
         const int ec = (tg0 + e0 / (64 * T) < t1) ? e0 : e0 % (64 * T);
         lds_run16<R>(x.v, Vbr + (size_t)tg0 * (64 * T) + ec);
         lds_run16<R>(x.d, Dbr + (size_t)tg0 * (64 * T) + ec);
+        // the 16 D elements of a thread share one pattern (B-operand-major tile)
+        const int el = ec % (64 * T);
+        x.tcode = Cr[(tg0 + ec / (64 * T)) * T + ((el >> 10) << 4) + ((el >> 4) & 15)];
     };
     auto stash = [&](const Regs &x, int buf) {
+        // tips: D is the indicator of the pattern's code (all sense codons for
+        // gaps, the ambiguity mask for partially ambiguous codes)
+        uint64_t tmask = 0;
+        if (tip) {
+            const unsigned cd = x.tcode;
+            tmask = cd < 64u ? (1ull << cd) : (cd == TIP_ONES ? all : 0ull);
+            if (cd >= 64u && cd != TIP_ONES) tmask = AMBG ? a.amb[cd & 0x7FFFu] : ambl[cd & 0x7FFFu];
+        }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = tid * PER + q, ti = e / (64 * T), el = e % (64 * T);
             // V tiles are [state][pattern]; D tiles B-operand-major (bmaj_index)
             Vs[buf][adj_idx(el / T, ti * T + el % T)] = x.v[q];
             const int sd = ((el & 15) << 2) | ((el >> 8) & 3), td = ((el >> 10) << 4) | ((el >> 4) & 15);
-            Ds[buf][adj_idx(sd, ti * T + td)] = x.d[q];
+            Ds[buf][adj_idx(sd, ti * T + td)] = tip ? (R)((tmask >> sd) & 1ull) : x.d[q];
         }
         if (tid < 64) {
             double c = 0.0;
             if (x.pv && x.wt > 0.0 && x.L > 0.0) c = ldexp(x.wt / x.L, x.gv - x.er + (tip ? 0 : x.ev));
             cs[buf][adj_idx(0, tid)] = (R)c;
-            codes[buf][adj_idx(0, tid)] = tip ? x.code : 0u;
         }
     };
     f64x4 acc[4];
@@ -1475,39 +1489,32 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
         lds_run16<R>(cv, &cs[buf][adj_idx(0, lg)]);
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) av[kk] *= cv[kk];
-        if (tip) {
-            unsigned cd[16];
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk) cd[kk] = codes[buf][adj_idx(0, lg) + kk];
+        for (int c0 = 0; c0 < 4; c0 += CBI) {
+            R bv[CBI][16];
+#pragma unroll
+            for (int c = 0; c < CBI; ++c) lds_run16<R>(bv[c], &Ds[buf][adj_idx(16 * (c0 + c) + lr, lg)]);
 #pragma unroll
             for (int kk = 0; kk < 16; ++kk)
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb)
-                    loc[cb] = MF<R>::mma(av[kk], tip_onehot<R>(cd[kk], 16 * cb + lr, a.amb, a.S), loc[cb]);
-        } else {
+                for (int c = 0; c < CBI; ++c) loc[c0 + c] = MF<R>::mma(av[kk], bv[c][kk], loc[c0 + c]);
+            // the next block's operands may reuse bv's registers: let these
+            // MFMAs complete first
+            if (c0 + CBI < 4)
 #pragma unroll
-            for (int c0 = 0; c0 < 4; c0 += CBI) {
-                R bv[CBI][16];
+                for (int c = 0; c < CBI; ++c)
 #pragma unroll
-                for (int c = 0; c < CBI; ++c) lds_run16<R>(bv[c], &Ds[buf][adj_idx(16 * (c0 + c) + lr, lg)]);
-#pragma unroll
-                for (int kk = 0; kk < 16; ++kk)
-#pragma unroll
-                    for (int c = 0; c < CBI; ++c) loc[c0 + c] = MF<R>::mma(av[kk], bv[c][kk], loc[c0 + c]);
-                // the next block's operands may reuse bv's registers: let these
-                // MFMAs complete first
-                if (c0 + CBI < 4)
-#pragma unroll
-                    for (int c = 0; c < CBI; ++c)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) asm volatile("" ::"v"(loc[c0 + c][r]));
-            }
+                    for (int r = 0; r < 4; ++r) asm volatile("" ::"v"(loc[c0 + c][r]));
         }
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[cb][r] += (double)loc[cb][r];
     };
+    if (!AMBG && tip) {
+        for (int i = tid; i < a.n_amb; i += 256) ambl[i] = a.amb[i];
+        __syncthreads();
+    }
     // Group k travels in register set k % 2 and LDS buffer k % 2.  Fetches
     // past the last group re-read it (from L2) so that every iteration issues
     // the same loads: the compiler then waits for exactly the group a stash
@@ -1969,10 +1976,15 @@ template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev
 
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s) {
     const dim3 grid(a.n_chunks, a.n_branch);
-    switch (T) {
-        case 16: hipLaunchKernelGGL((k_adjoint<R, 16>), grid, dim3(256), 0, s, a); break;
-        case 32: hipLaunchKernelGGL((k_adjoint<R, 32>), grid, dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_adjoint<R, 64>), grid, dim3(256), 0, s, a); break;
+    if (a.n_amb <= AMB_LDS) switch (T) {
+        case 16: hipLaunchKernelGGL((k_adjoint<R, 16, false>), grid, dim3(256), 0, s, a); break;
+        case 32: hipLaunchKernelGGL((k_adjoint<R, 32, false>), grid, dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_adjoint<R, 64, false>), grid, dim3(256), 0, s, a); break;
+    }
+    else switch (T) {
+        case 16: hipLaunchKernelGGL((k_adjoint<R, 16, true>), grid, dim3(256), 0, s, a); break;
+        case 32: hipLaunchKernelGGL((k_adjoint<R, 32, true>), grid, dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_adjoint<R, 64, true>), grid, dim3(256), 0, s, a); break;
     }
 }
 

@@ -61,6 +61,35 @@ def test_ambiguity_gaps_stops_and_unmatched_leaf():
     check_plain(tree, aln, 0.4, [1, 1, 1, 2, 1], branch_lengths(tree))
 
 
+def test_many_distinct_ambiguity_masks():
+    """More distinct partially ambiguous codons than the adjoint stages in LDS
+    (512): its ambiguity-mask table is then read from HBM."""
+    rng = np.random.default_rng(11)
+    iupac = list("ACGTRYKMSWBDHVN")
+    names = ["T%d" % i for i in range(10)]
+    seqs = {}
+    for n in names:
+        cods = []
+        for _ in range(300):
+            c = [str(rng.choice(list("ACGT"))) for _ in range(3)]
+            if rng.uniform() < 0.7:
+                for k in rng.choice(3, size=int(rng.integers(1, 3)), replace=False):
+                    c[k] = str(rng.choice(iupac[4:]))
+            cods.append("".join(c))
+        seqs[n] = "".join(cods)
+    aln = alignment(seqs)
+    m = aln.pmasks
+    full = int(np.max(m))
+    single = (m & (m - np.uint64(1))) == 0
+    distinct = {int(x) for x in m[~single].ravel() if int(x) != full}
+    assert len(distinct) > 512
+    rng2 = np.random.default_rng(12)
+    nwk = "(" + ",".join("%s:%.3f" % (n, rng2.uniform(0.05, 0.4)) for n in names[:4]) + ",(" + \
+          ",".join("%s:%.3f" % (n, rng2.uniform(0.05, 0.4)) for n in names[4:]) + "):0.1);"
+    tree = dr.TreeRef.from_newick(nwk)
+    check_plain(tree, aln, 0.7, [1.1, 0.9, 1.3, 2.2, 0.8], branch_lengths(tree))
+
+
 @pytest.mark.parametrize("n_codons", [1, 15, 17, 33, 250])
 def test_ragged_pattern_counts(n_codons):
     tree, aln, sim = sim_problem(9, n_codons, 40 + n_codons)
