@@ -126,7 +126,7 @@ struct hy_engine_s {
     int *h_status = nullptr;
     DevBuf d_params, d_lam, d_U, d_V, d_mu, d_status, d_tau, d_M, d_dmu, d_Z;
     bool warm_enable = true, warm_valid = false;
-    DevBuf d_Prow[2], d_Rsum[2], d_PA[2], d_PTA[2];
+    DevBuf d_Prow[2], d_PA[2], d_PTA[2];  // Prow: [n_ext][PRS] (P row-major, then its row sums)
     DevBuf d_D[2], d_Vp[2], d_E, d_G, d_Lroot, d_Eroot, d_site, d_Apart, d_fr, d_res;
     DevBuf d_Asum, d_W, d_wl, d_proj;
     int grp_size = 1, n_grp = 1;  // Fréchet stage grouping of (branch, component) pairs per matrix
@@ -255,26 +255,25 @@ void build_layout(hy_engine h) {
 
 void build_schedules(hy_engine h, int T, int esz);
 
-// Identity P of the artificial nodes in the Prow, PA/PTA ([w][l][kk]) and
-// Rsum layouts of the working type (esz = 4 or 8 bytes).
+// Identity P of the artificial nodes in the Prow (with row sums), PA/PTA
+// ([w][l][kk]) layouts of the working type (esz = 4 or 8 bytes).
 int upload_identity_tiles(hy_engine h, int ri, int esz) {
-    std::vector<double> prow(4096, 0.0), pa(4096, 0.0), rs(64, 0.0);
+    std::vector<double> prow(PRS, 0.0), pa(4096, 0.0);
     for (int i = 0; i < 64; ++i) prow[i * 64 + i] = 1.0;
+    for (int i = 0; i < h->S; ++i) prow[4096 + i] = 1.0;  // row sums over the sense codons
     for (int idx = 0; idx < 4096; ++idx) {
         const int w = idx >> 10, l = (idx >> 4) & 63, kk = idx & 15;
         pa[idx] = (16 * w + (l & 15)) == (4 * kk + (l >> 4)) ? 1.0 : 0.0;
     }
-    for (int i = 0; i < h->S; ++i) rs[i] = 1.0;
-    std::vector<float> prow32(prow.begin(), prow.end()), pa32(pa.begin(), pa.end()), rs32(rs.begin(), rs.end());
-    const void *srcs[4] = {esz == 8 ? (const void *)prow.data() : prow32.data(),
+    std::vector<float> prow32(prow.begin(), prow.end()), pa32(pa.begin(), pa.end());
+    const void *srcs[3] = {esz == 8 ? (const void *)prow.data() : prow32.data(),
                            esz == 8 ? (const void *)pa.data() : pa32.data(),
-                           esz == 8 ? (const void *)pa.data() : pa32.data(),
-                           esz == 8 ? (const void *)rs.data() : rs32.data()};
-    DevBuf *dst[4] = {&h->d_Prow[ri], &h->d_PA[ri], &h->d_PTA[ri], &h->d_Rsum[ri]};
-    const size_t per[4] = {4096, 4096, 4096, 64};
+                           esz == 8 ? (const void *)pa.data() : pa32.data()};
+    DevBuf *dst[3] = {&h->d_Prow[ri], &h->d_PA[ri], &h->d_PTA[ri]};
+    const size_t per[3] = {PRS, 4096, 4096};
     for (int n = 0; n < h->n_ext; ++n) {
         if (!h->ext_ident[n]) continue;
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 3; ++k)
             HIP_TRY(h, hipMemcpyAsync((char *)dst[k]->p + (size_t)n * per[k] * esz, srcs[k], per[k] * esz,
                                       hipMemcpyHostToDevice, h->stream));
     }
@@ -337,10 +336,9 @@ int prepare_buffers(hy_engine h, int prec) {
     TRY(ensure(h, h->d_Eroot, Pp * 4));
     TRY(ensure(h, h->d_site, Pp * 8));
     const size_t ne = (size_t)h->n_ext;
-    TRY(ensure(h, h->d_Prow[ri], ne * 4096 * rs));
+    TRY(ensure(h, h->d_Prow[ri], ne * PRS * rs));
     TRY(ensure(h, h->d_PA[ri], ne * 4096 * rs));
     TRY(ensure(h, h->d_PTA[ri], ne * 4096 * rs));
-    TRY(ensure(h, h->d_Rsum[ri], ne * 64 * rs));
     if (h->ident_tiles[ri] != h->d_PA[ri].p) {
         // Edges added to resolve polytomies (identity transition): exact
         // identity tiles in every P layout; k_pmat never overwrites them.
@@ -589,7 +587,6 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         pa.S = h->S;
         pa.tau = (double *)h->d_tau.p;
         pa.Prow = (R *)h->d_Prow[ri].p;
-        pa.Rsum = (R *)h->d_Rsum[ri].p;
         pa.PA = (R *)h->d_PA[ri].p;
         pa.PTA = (R *)h->d_PTA[ri].p;
         launch_pmat<R>(pa, grad ? &pj : nullptr, h->n_mat, s);
@@ -600,7 +597,6 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         pa.n_branch = h->n_branch;
         pa.S = h->S;
         pa.Prow = (R *)h->d_Prow[ri].p;
-        pa.Rsum = (R *)h->d_Rsum[ri].p;
         pa.PA = (R *)h->d_PA[ri].p;
         pa.PTA = (R *)h->d_PTA[ri].p;
         if (h->n_branch > 0) launch_pmat_host<R>(pa, (const double *)h->d_Phost.p, s);
@@ -620,7 +616,6 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     pr.PA = (const R *)h->d_PA[ri].p;
     pr.PTA = (const R *)h->d_PTA[ri].p;
     pr.Prow = (const R *)h->d_Prow[ri].p;
-    pr.Rsum = (const R *)h->d_Rsum[ri].p;
     pr.tips = (const uint16_t *)h->d_tips.p;
     pr.amb = (const uint64_t *)h->d_amb.p;
     pr.P_pad = h->P_pad;
@@ -828,7 +823,6 @@ void hy_destroy(hy_engine h) {
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
-        release(h, h->d_Rsum[k]);
         release(h, h->d_PA[k]);
         release(h, h->d_PTA[k]);
         release(h, h->d_D[k]);
@@ -1795,10 +1789,9 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
     TRY(ensure(c, c->d_Lroot, (size_t)pbase * 8));
     TRY(ensure(c, c->d_Eroot, (size_t)pbase * 4));
     TRY(ensure(c, c->d_site, (size_t)pbase * 8));
-    TRY(ensure(c, c->d_Prow[ri], (size_t)node0 * 4096 * esz));
+    TRY(ensure(c, c->d_Prow[ri], (size_t)node0 * PRS * esz));
     TRY(ensure(c, c->d_PA[ri], (size_t)node0 * 4096 * esz));
     TRY(ensure(c, c->d_PTA[ri], (size_t)node0 * 4096 * esz));
-    TRY(ensure(c, c->d_Rsum[ri], (size_t)node0 * 64 * esz));
     TRY(upload_identity_tiles(c, ri, esz));
     c->ident_tiles[ri] = c->d_PA[ri].p;
     TRY(ensure(c, c->d_tau, (size_t)node0 * 8));

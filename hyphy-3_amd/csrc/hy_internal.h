@@ -57,6 +57,11 @@ constexpr int tip_lds_codes(int T, int esz) {
     return T == 16 ? 12288 : (esz == 4 ? 24576 : (T == 32 ? 16384 : 8192));
 }
 
+// Row-major transition matrices: per node a block of PRS elements, the 64x64
+// matrix followed by its 64 row sums (tip gathers then address both from one
+// base).
+constexpr int PRS = 4096 + 64;
+
 // Branch list entry: [0] node, [1] outside slot, [2] child reference (kind | index),
 // [3] mixture class, [4] gene (batched evaluations; 0 otherwise), [5..7] unused.
 constexpr int BR_STRIDE = 8;
@@ -125,8 +130,7 @@ struct PmatArgs {
     const double *lam, *U, *V, *mu;
     int S;
     double *tau;  // [n_nodes]
-    R *Prow;      // [n_nodes][64][64]
-    R *Rsum;      // [n_nodes][64]
+    R *Prow;      // [n_nodes][PRS]   row-major P, then its row sums
     R *PA;        // [n_nodes][4][16][64]   A-operand swizzle of P
     R *PTA;       // [n_nodes][4][16][64]   A-operand swizzle of P^T
 };
@@ -140,7 +144,7 @@ struct PruneArgs {
     int tip_rows;  // taxa + 1 (all-ones row)
     int S;
     R *Uspill;     // [n_int][P_pad][64] U messages that did not fit in LDS
-    const R *PA, *PTA, *Prow, *Rsum;
+    const R *PA, *PTA, *Prow;  // Prow: [n_nodes][PRS]
     const uint16_t *tips;  // [rows][P_pad]
     const uint64_t *amb;
     int P_pad, n_tiles, n_patterns;

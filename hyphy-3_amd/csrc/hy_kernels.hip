@@ -489,12 +489,12 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
     if (tid == 0) a.status[m] = status;
 }
 
-// Write one branch's P (FP64 in LDS) as row-major, row sums and the MFMA
+// Write one branch's P (FP64 in LDS) as row-major with row sums and the MFMA
 // A-operand swizzles of P and P^T in the working type.
 template <typename R>
-__device__ __forceinline__ void emit_p(const double *P, int node, int S, R *Prow_all, R *Rsum_all, R *PA_all,
-                                       R *PTA_all, int tid) {
-    R *Prow = Prow_all + (size_t)node * 4096, *PA = PA_all + (size_t)node * 4096, *PTA = PTA_all + (size_t)node * 4096;
+__device__ __forceinline__ void emit_p(const double *P, int node, int S, R *Prow_all, R *PA_all, R *PTA_all,
+                                       int tid) {
+    R *Prow = Prow_all + (size_t)node * PRS, *PA = PA_all + (size_t)node * 4096, *PTA = PTA_all + (size_t)node * 4096;
     for (int idx = tid; idx < 4096; idx += 256) {
         Prow[idx] = (R)P[idx];
         const int w = idx >> 10, l = (idx >> 4) & 63, kk = idx & 15;  // [w][l][kk]
@@ -505,7 +505,7 @@ __device__ __forceinline__ void emit_p(const double *P, int node, int S, R *Prow
     if (tid < 64) {
         double s = 0.0;
         for (int j = 0; j < S; ++j) s += P[tid * 64 + j];
-        Rsum_all[(size_t)node * 64 + tid] = (R)s;
+        Prow[4096 + tid] = (R)s;
     }
 }
 
@@ -559,7 +559,7 @@ __device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, double *
 #pragma unroll
         for (int r = 0; r < 4; ++r) P[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] = Pacc[q][r];
     __syncthreads();
-    emit_p<R>(P, node, a.S, a.Prow, a.Rsum, a.PA, a.PTA, tid);
+    emit_p<R>(P, node, a.S, a.Prow, a.PA, a.PTA, tid);
 }
 
 // Caller-supplied P matrices (row-major S x S per node, FP64).
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_pmat_host(PmatArgs<R> a, const double *
         P[idx] = (i < S && j < S) ? Phost[((size_t)node * S + i) * S + j] : 0.0;
     }
     __syncthreads();
-    emit_p<R>(P, node, S, a.Prow, a.Rsum, a.PA, a.PTA, tid);
+    emit_p<R>(P, node, S, a.Prow, a.PA, a.PTA, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -742,23 +742,39 @@ __device__ __forceinline__ unsigned tip_code(const PruneArgs<R> &a, const uint16
     return TL ? tcodes[row * T + t] : a.tips[(size_t)row * a.P_pad + p0 + t];
 }
 
+// Tip codes of the lane's patterns for every child slot of a schedule entry
+// (TL: from LDS, all slots read before the first use so that one wait covers
+// them; non-tip slots read row 0).  gather_tip reads the code itself when
+// given no precomputed codes.
+template <typename R, int T, int NSLOT, int STRIDE>
+__device__ __forceinline__ void slot_codes(unsigned (*cds)[T / 16], const int *ev, int nc, const uint16_t *tcodes,
+                                           int lr) {
+#pragma unroll
+    for (int c = 0; c < NSLOT; ++c) {
+        const int ref = ev[7 + STRIDE * c];
+        const int row = (c < nc && (ref >> 28) == KIND_TIP) ? (ref & REF_MASK) : 0;
+#pragma unroll
+        for (int nb = 0; nb < T / 16; ++nb) cds[c][nb] = tcodes[row * T + 16 * nb + lr];
+    }
+}
+
 template <typename R, int T, bool TL>
 __device__ __forceinline__ void gather_tip(typename MF<R>::acc *x, unsigned &amb_bits, int slot, bool is_tip,
-                                           const PruneArgs<R> &a, const uint16_t *tcodes, int pn, int row, int p0,
-                                           int w, int lr, int lg) {
+                                           const PruneArgs<R> &a, const uint16_t *tcodes, const unsigned *cds, int pn,
+                                           int row, int p0, int w, int lr, int lg) {
     // Branch-free in the loaded values (every slot issues its loads, non-tip
     // slots from a harmless address) so that no register merge forces a wait
-    // on the gathers before they are consumed.
+    // on the gathers before they are consumed.  One base per node (row-major
+    // P followed by its row sums), 32-bit offsets.
     const bool gather = is_tip && pn >= 0;
-    const R *P = a.Prow + (size_t)(gather ? pn : 0) * 4096;
-    const R *Rs = a.Rsum + (size_t)(gather ? pn : 0) * 64;
+    const R *P = a.Prow + (size_t)(gather ? pn : 0) * PRS;
 #pragma unroll
     for (int nb = 0; nb < T / 16; ++nb) {
-        const unsigned code = is_tip ? tip_code<R, T, TL>(a, tcodes, row, p0, 16 * nb + lr) : 0u;
+        const unsigned code = is_tip ? ((TL && cds) ? cds[nb] : tip_code<R, T, TL>(a, tcodes, row, p0, 16 * nb + lr)) : 0u;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int s = 16 * w + MF<R>::crow(lg, r);
-            x[nb][r] = (gather && code < 64u) ? P[s * 64 + code] : Rs[s];
+            x[nb][r] = P[(gather && code < 64u) ? s * 64 + (int)code : 4096 + s];
         }
         if (is_tip && (pn < 0 || (code >= 64u && code != TIP_ONES))) amb_bits |= 1u << (slot * (T / 16) + nb);
     }
@@ -782,7 +798,7 @@ __device__ __forceinline__ void fix_tip(typename MF<R>::acc *x, unsigned amb_bit
                 continue;
             }
             const uint64_t m0 = a.amb[code & 0x7FFFu] & all;
-            const R *P = a.Prow + (size_t)pn * 4096;
+            const R *P = a.Prow + (size_t)pn * PRS;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int s = 16 * w + MF<R>::crow(lg, r);
@@ -921,12 +937,14 @@ __global__ __launch_bounds__(256, LEAN ? 4 : ((sizeof(R) == 4 && T == 16) ? 2 : 
     auto operands = [&](const Ent &e, R (*pa)[16], acc_t (*xt)[NB], unsigned &ab) {
         const int nc = e.v[2];
         ab = 0;
+        unsigned cds[MAXC][NB];
+        if constexpr (TL) slot_codes<R, T, MAXC, 3>(cds, e.v, nc, tcodes, lr);
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) {
             const int pn = e.v[6 + 3 * c], ref = e.v[7 + 3 * c], kind = ref >> 28;
             if (c < nc && kind == KIND_INT) fetch_edge<R>(pa[c], a.PA, pn, w, l);
             if (c < nc && kind == KIND_TIP)
-                gather_tip<R, T, TL>(xt[c], ab, c, true, a, tcodes, pn, ref & REF_MASK, p0, w, lr, lg);
+                gather_tip<R, T, TL>(xt[c], ab, c, true, a, tcodes, cds[c], pn, ref & REF_MASK, p0, w, lr, lg);
         }
     };
     Ent ec = load_entry<SCHED_STRIDE>(a.sched, 0);
@@ -1101,7 +1119,8 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) {
             const int pn = e.v[6 + 4 * c], ref = e.v[7 + 4 * c], kind = ref >> 28, idx = ref & REF_MASK;
-            if (c < nc && kind == KIND_TIP) gather_tip<R, T, TL>(xt[c], ab, c, true, a, tcodes, pn, idx, p0, w, lr, lg);
+            if (c < nc && kind == KIND_TIP)
+                gather_tip<R, T, TL>(xt[c], ab, c, true, a, tcodes, nullptr, pn, idx, p0, w, lr, lg);
             if (c < nc) {
                 if (kind != KIND_TIP) {
                     if (c < RS) {
