@@ -117,6 +117,7 @@ struct ModelArgs {
     int warm;     // 1: refine Z from the previous call
     int *status;  // [n_mat] Jacobi sweeps, 100+k after k refinement steps, negative = no convergence
     const int32_t *mat_gene;  // batched evaluation: gene of each matrix (pi, pos, alpha, theta, delta, psi)
+    double tol;               // warm refinement: accepted predicted correction (relative)
 };
 
 template <typename R>

@@ -368,10 +368,12 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
             // Accept X + X E when X already diagonalised A (off-diagonal of
             // X^T A X below 1e-9 ||A||) and the correction is second-order
             // small, or when the steps contract quadratically (the next
-            // correction, predicted as |E|^3 / |E_prev|^2, is below 1e-14)
-            // and no cluster hides an off-diagonal entry above 1e-14 ||A||.
-            const bool quad = it >= 1 && enorm < 1e-5 && enorm * enorm * enorm <= 1e-14 * prev_en * prev_en &&
-                              frozen <= 1e-14 * anorm;
+            // correction, predicted as |E|^3 / |E_prev|^2, is below tol)
+            // and no cluster hides an off-diagonal entry above tol ||A||.
+            // tol = 1e-14 for FP64 evaluations, 1e-10 for FP32 ones (whose
+            // transition matrices are rounded to FP32).
+            const bool quad = it >= 1 && enorm < 1e-5 && enorm * enorm * enorm <= a.tol * prev_en * prev_en &&
+                              frozen <= a.tol * anorm;
             prev_en = enorm;
             if ((soff <= 1e-9 * anorm && enorm < 1e-9) || quad) {
                 done = true;

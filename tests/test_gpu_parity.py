@@ -112,6 +112,38 @@ def test_warm_start_matches_cold():
             assert grad_err(rw[k], rc[k]) < 1e-9, k
 
 
+def test_warm_start_fp32_tolerance():
+    """FP32 evaluations accept warm-started spectra at a looser predicted
+    correction (1e-10 instead of 1e-14); their results stay within the FP32
+    tolerances of cold solves, and FP64 evaluations that follow on the same
+    engine are refined to the FP64 tolerance again."""
+    omegas, weights = [0.3, 2.0], [0.6, 0.4]
+    tree, aln, sim = sim_problem(16, 400, 21, omegas=omegas, weights=weights)
+    N = len(tree.nodes)
+    engines = []
+    for warm in (True, False):
+        e = make_engine(tree, aln)
+        e.set_mixture(2, np.zeros(N, dtype=np.int32), [[0, 1]], convention=1)
+        e.set_warm_start(warm)
+        engines.append(e)
+    bl = branch_lengths(tree)
+    for step in range(6):
+        f = 1.0 + 1e-3 * step
+        om = [o * f for o in omegas]
+        th = np.asarray(sim["theta"]) * (2.0 - f)
+        prec = FP32 if step < 4 else FP64
+        rw = engines[0].eval(bl, om, weights, th, precision=prec)
+        rc = engines[1].eval(bl, om, weights, th, precision=prec)
+        st = engines[0].eigen_status()
+        print("step", step, "prec", prec, "status", st, rw["log_likelihood"], rc["log_likelihood"])
+        if step > 0:
+            assert (st >= 100).all()
+        tl, tg = (1e-6, 1e-4) if prec == FP32 else (1e-12, 1e-9)
+        assert rel(rw["log_likelihood"], rc["log_likelihood"]) < tl
+        for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
+            assert grad_err(rw[k], rc[k]) < tg, k
+
+
 def test_multihit_rates_forward():
     """Double and triple nucleotide changes (delta, psi; rate_matrix.hpp:205-219)
     in the forward pass, FP64 and FP32, against the oracle."""
