@@ -92,6 +92,14 @@ struct hy_engine_s {
     const void *ident_tiles[2] = {nullptr, nullptr};  // PA buffer whose identity tiles are current
     DevBuf d_osched, d_Uspill[2];
     int n_uspill = 0;
+    // Subtree parts of the pruning passes (single problems with few tiles):
+    // per part {inside offset, count, outside offset, count} in schedule
+    // entries; the last row is the top part, which holds the root.  n_parts
+    // counts the subtree parts (0: one schedule for the whole tree).
+    std::vector<int32_t> parts;
+    DevBuf d_parts;
+    int n_parts = 0;
+    bool no_parts = false;  // HY_NO_PARTS=1: one schedule per pass
     std::vector<int32_t> real_parent;
     std::vector<uint16_t> codes;  // [num_taxa][P]
     std::vector<uint64_t> amb;
@@ -253,7 +261,7 @@ void build_layout(hy_engine h) {
     pl.total = o;
 }
 
-void build_schedules(hy_engine h, int T, int esz);
+void build_schedules(hy_engine h, int T, int esz, bool split);
 
 // Identity P of the artificial nodes in the Prow (with row sums), PA/PTA
 // ([w][l][kk]) layouts of the working type (esz = 4 or 8 bytes).
@@ -319,12 +327,18 @@ int prepare_buffers(hy_engine h, int prec) {
             release(h, h->d_Uspill[k]);
         }
     }
-    h->lean = lean_inside(prec == HY_PREC_FP32, T, h->n_tiles, h->n_cu);
-    if (h->sched_key != T * 16 + (int)rs + (h->lean ? 4096 : 0)) {
+    // Few tiles (fewer than two per CU): the pruning passes split the tree
+    // into subtree parts walked by separate workgroups (a shorter dependent
+    // chain per workgroup, more workgroups in flight); FP32 T = 16 then runs
+    // the lean inside variant (four workgroups per CU).
+    const bool split = !h->no_parts && h->n_tiles < 2 * h->n_cu;
+    h->lean = lean_inside(prec == HY_PREC_FP32, T, h->n_tiles, h->n_cu) || (split && prec == HY_PREC_FP32 && T == 16);
+    if (h->sched_key != T * 16 + (int)rs + (h->lean ? 4096 : 0) + (split ? 8192 : 0)) {
         ++h->epoch;
-        build_schedules(h, T, (int)rs);
+        build_schedules(h, T, (int)rs, split);
         TRY(upload(h, h->d_sched, h->sched.data(), h->sched.size() * 4));
         TRY(upload(h, h->d_osched, h->osched.data(), h->osched.size() * 4));
+        TRY(upload(h, h->d_parts, h->parts.data(), h->parts.size() * 4));
     }
     const size_t Pp = (size_t)h->P_pad;
     TRY(ensure(h, h->d_D[ri], (size_t)h->n_int * Pp * 64 * rs));
@@ -410,8 +424,12 @@ int prepare_buffers(hy_engine h, int prec) {
 }
 
 // Inside / outside schedules with LDS slot assignment for tile width T and
-// element size esz.
-void build_schedules(hy_engine h, int T, int esz) {
+// element size esz.  With split, the tree is cut into a top part holding the
+// root and subtree parts hanging off it: every part gets its own schedule
+// (and its own LDS slots), subtree parts are walked by separate workgroups
+// and exchange their results with the top part through HBM (inside: the
+// part root's partials; outside: the message into the part root, spilled).
+void build_schedules(hy_engine h, int T, int esz, bool split) {
     const int NSI = inside_slots(T, esz, h->lean), NSO = outside_slots(T, esz);
     const auto &ch = h->ext_children;
     const int n_ext = (int)ch.size();
@@ -422,88 +440,147 @@ void build_schedules(hy_engine h, int T, int esz) {
         }
         return (KIND_INT << 28) | h->int_slot[c];  // identity edges carry an exact identity P
     };
+    // ---- parts: grow the top from the root by cutting the largest subtree
+    // while it holds more than a quarter of the internal nodes (at least 8)
+    std::vector<int> part_of(n_ext, -1);  // -1: top part
+    std::vector<int> proots;
+    if (split && (int)h->order.size() >= 16) {
+        std::vector<int> nint(n_ext, 0);
+        for (int u : h->order) {
+            nint[u] = 1;
+            for (int c : ch[u]) nint[u] += nint[c];
+        }
+        const int target = std::max(8, ((int)h->order.size() + 3) / 4);
+        std::vector<int> cand;
+        for (int c : ch[h->ext_root])
+            if (!ch[c].empty()) cand.push_back(c);
+        for (;;) {
+            auto big = std::max_element(cand.begin(), cand.end(), [&](int x, int y) { return nint[x] < nint[y]; });
+            if (big == cand.end() || nint[*big] <= target) break;
+            const int u = *big;
+            cand.erase(big);
+            for (int c : ch[u])
+                if (!ch[c].empty()) cand.push_back(c);
+        }
+        // largest parts first (their workgroups start first)
+        std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return nint[x] > nint[y]; });
+        if (cand.size() >= 2) {
+            proots = cand;
+            for (size_t p = 0; p < proots.size(); ++p) {
+                std::vector<int> st{proots[p]};
+                while (!st.empty()) {
+                    const int u = st.back();
+                    st.pop_back();
+                    part_of[u] = (int)p;
+                    for (int c : ch[u]) st.push_back(c);
+                }
+            }
+        }
+    }
+    const int np = (int)proots.size();
+    std::vector<uint8_t> is_proot(n_ext, 0);
+    for (int r : proots) is_proot[r] = 1;
+    // post-order of every part (top last), filtered from the whole-tree order
+    std::vector<std::vector<int>> porder(np + 1);
+    for (int u : h->order) porder[part_of[u] < 0 ? np : part_of[u]].push_back(u);
+    h->parts.assign((size_t)(np + 1) * 4, 0);
     // ---- inside: free-list LDS slots, spill to HBM when none is free ----
     std::vector<int32_t> where(n_ext, -1);  // LDS slot of a computed internal node (-1: HBM)
-    std::vector<int> free_slots;
-    for (int s = NSI - 1; s >= 0; --s) free_slots.push_back(s);
     h->sched.clear();
-    for (int u : h->order) {
-        int32_t e[SCHED_STRIDE] = {0};
-        e[0] = u;
-        e[1] = h->int_slot[u];
-        e[2] = (int32_t)ch[u].size();
-        for (size_t c = 0; c < ch[u].size(); ++c) {
-            const int cn = ch[u][c];
-            e[6 + 3 * c] = cn;
-            e[7 + 3 * c] = ref_of(cn);
-            e[8 + 3 * c] = ch[cn].empty() ? -1 : where[cn];
+    for (int p = 0; p <= np; ++p) {
+        std::vector<int> free_slots;
+        for (int sl = NSI - 1; sl >= 0; --sl) free_slots.push_back(sl);
+        h->parts[p * 4 + 0] = (int32_t)(h->sched.size() / SCHED_STRIDE);
+        h->parts[p * 4 + 1] = (int32_t)porder[p].size();
+        for (int u : porder[p]) {
+            int32_t e[SCHED_STRIDE] = {0};
+            e[0] = u;
+            e[1] = h->int_slot[u];
+            e[2] = (int32_t)ch[u].size();
+            for (size_t c = 0; c < ch[u].size(); ++c) {
+                const int cn = ch[u][c];
+                e[6 + 3 * c] = cn;
+                e[7 + 3 * c] = ref_of(cn);
+                e[8 + 3 * c] = ch[cn].empty() ? -1 : where[cn];
+            }
+            if (ch[u].empty()) {
+                // single-node tree: the root is a leaf whose partials are its tip
+                // indicator (one tip child on an identity edge, node id -1)
+                e[2] = 1;
+                e[6] = -1;
+                e[7] = ref_of(u);
+                e[8] = -1;
+            }
+            for (int cn : ch[u])
+                if (!ch[cn].empty() && where[cn] >= 0) free_slots.push_back(where[cn]);
+            int out = -1;
+            if (u != h->ext_root && !is_proot[u] && !free_slots.empty()) {
+                out = free_slots.back();
+                free_slots.pop_back();
+            }
+            where[u] = out;
+            e[3] = out;
+            h->sched.insert(h->sched.end(), e, e + SCHED_STRIDE);
         }
-        if (ch[u].empty()) {
-            // single-node tree: the root is a leaf whose partials are its tip
-            // indicator (one tip child on an identity edge, node id -1)
-            e[2] = 1;
-            e[6] = -1;
-            e[7] = ref_of(u);
-            e[8] = -1;
-        }
-        for (int cn : ch[u])
-            if (!ch[cn].empty() && where[cn] >= 0) free_slots.push_back(where[cn]);
-        int out = -1;
-        if (u != h->ext_root && !free_slots.empty()) {
-            out = free_slots.back();
-            free_slots.pop_back();
-        }
-        where[u] = out;
-        e[3] = out;
-        h->sched.insert(h->sched.end(), e, e + SCHED_STRIDE);
     }
-    // ---- outside: reverse order; pending U messages in LDS slots ----
+    // ---- outside: top part first, then the subtree parts, each in reverse
+    // order; pending U messages in LDS slots, into a part root through HBM
     std::vector<int32_t> usrc(n_ext, U_SPILL);
     usrc[h->ext_root] = U_ROOT;
-    free_slots.clear();
-    for (int s = NSO - 1; s >= 0; --s) free_slots.push_back(s);
     h->osched.clear();
     int spills = 0;
-    const int n_order = (int)h->order.size();
-    for (int i = n_order - 1; i >= 0; --i) {
-        const int u = h->order[i];
-        const int next = i > 0 ? h->order[i - 1] : -1;  // entry processed after u
-        int32_t e[OSCHED_STRIDE] = {0};
-        e[0] = u;
-        e[1] = h->int_slot[u];
-        e[2] = (int32_t)ch[u].size();
-        e[3] = (u == h->ext_root) ? -1 : h->vslot[u];
-        e[4] = h->ext_ident[u];
-        e[5] = usrc[u];
-        if (usrc[u] >= 0) free_slots.push_back(usrc[u]);
-        // internal children first, tips last (the kernel keeps register-prefetched
-        // operands for the first two children only)
-        std::vector<int32_t> och(ch[u]);
-        std::stable_partition(och.begin(), och.end(), [&](int x) { return !ch[x].empty(); });
-        for (size_t c = 0; c < och.size(); ++c) {
-            const int cn = och[c];
-            e[6 + 4 * c] = cn;
-            e[7 + 4 * c] = ref_of(cn);
-            e[8 + 4 * c] = h->vslot[cn];
-            int dst = -1;
-            if (!ch[cn].empty()) {
-                if (cn == next) {
-                    dst = U_NEXT;
-                } else if (!free_slots.empty()) {
-                    dst = free_slots.back();
-                    free_slots.pop_back();
-                } else {
-                    dst = U_SPILL;
-                    ++spills;
+    for (int q = 0; q <= np; ++q) {
+        const int p = (q == 0) ? np : q - 1;
+        std::vector<int> free_slots;
+        for (int sl = NSO - 1; sl >= 0; --sl) free_slots.push_back(sl);
+        h->parts[p * 4 + 2] = (int32_t)(h->osched.size() / OSCHED_STRIDE);
+        h->parts[p * 4 + 3] = (int32_t)porder[p].size();
+        const std::vector<int> &ord = porder[p];
+        const int n_order = (int)ord.size();
+        for (int i = n_order - 1; i >= 0; --i) {
+            const int u = ord[i];
+            const int next = i > 0 ? ord[i - 1] : -1;  // entry processed after u
+            int32_t e[OSCHED_STRIDE] = {0};
+            e[0] = u;
+            e[1] = h->int_slot[u];
+            e[2] = (int32_t)ch[u].size();
+            e[3] = (u == h->ext_root) ? -1 : h->vslot[u];
+            e[4] = h->ext_ident[u];
+            e[5] = usrc[u];
+            if (usrc[u] >= 0) free_slots.push_back(usrc[u]);
+            // internal children first, tips last (the kernel keeps register-prefetched
+            // operands for the first two children only)
+            std::vector<int32_t> och(ch[u]);
+            std::stable_partition(och.begin(), och.end(), [&](int x) { return !ch[x].empty(); });
+            for (size_t c = 0; c < och.size(); ++c) {
+                const int cn = och[c];
+                e[6 + 4 * c] = cn;
+                e[7 + 4 * c] = ref_of(cn);
+                e[8 + 4 * c] = h->vslot[cn];
+                int dst = -1;
+                if (!ch[cn].empty()) {
+                    if (is_proot[cn]) {
+                        dst = U_SPILL;
+                        ++spills;
+                    } else if (cn == next) {
+                        dst = U_NEXT;
+                    } else if (!free_slots.empty()) {
+                        dst = free_slots.back();
+                        free_slots.pop_back();
+                    } else {
+                        dst = U_SPILL;
+                        ++spills;
+                    }
+                    usrc[cn] = dst;
                 }
-                usrc[cn] = dst;
+                e[9 + 4 * c] = dst;
             }
-            e[9 + 4 * c] = dst;
+            h->osched.insert(h->osched.end(), e, e + OSCHED_STRIDE);
         }
-        h->osched.insert(h->osched.end(), e, e + OSCHED_STRIDE);
     }
+    h->n_parts = np;
     h->n_uspill = spills;
-    h->sched_key = T * 16 + esz + (h->lean ? 4096 : 0);
+    h->sched_key = T * 16 + esz + (h->lean ? 4096 : 0) + (split ? 8192 : 0);
 }
 
 void refresh_blist_classes(hy_engine h) {
@@ -516,6 +593,22 @@ void refresh_blist_classes(hy_engine h) {
 // Internal flag of run_eval: the outside pass without the adjoint stages
 // (workspace read-back of inside / outside partials).
 constexpr unsigned RUN_OUTSIDE = 1u << 30;
+
+// Outside pass: the top part first (it sends the messages into the subtree
+// parts through HBM), then the subtree parts.
+template <typename R>
+void launch_outside_parts(hy_engine h, PruneArgs<R> pr, bool rev, hipStream_t s) {
+    if (h->n_parts > 0) {
+        pr.parts = (const int4 *)h->d_parts.p;
+        pr.part0 = h->n_parts;
+        launch_outside<R>(pr, h->T, rev, 1, s);
+        pr.part0 = 0;
+        launch_outside<R>(pr, h->T, rev, h->n_parts, s);
+    } else {
+        pr.parts = nullptr;
+        launch_outside<R>(pr, h->T, rev, 1, s);
+    }
+}
 
 template <typename R>
 int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
@@ -632,10 +725,20 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     pr.site_ll = (double *)h->d_site.p;
     pr.genes = (const GeneInfo *)h->d_genes.p;
     pr.tile_gene = (const int32_t *)h->d_tile_gene.p;
-    launch_inside<R>(pr, h->T, h->lean, s);
+    pr.root_out = 1;
+    if (h->n_parts > 0) {
+        // subtree parts, then the top part (root likelihood)
+        pr.parts = (const int4 *)h->d_parts.p;
+        pr.part0 = 0;
+        pr.root_out = 0;
+        launch_inside<R>(pr, h->T, h->lean, h->n_parts, s);
+        pr.part0 = h->n_parts;
+        pr.root_out = 1;
+    }
+    launch_inside<R>(pr, h->T, h->lean, 1, s);
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[3], s));
     if (grad) {
-        launch_outside<R>(pr, h->T, custom_pi == nullptr, s);
+        launch_outside_parts<R>(h, pr, custom_pi == nullptr, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[4], s));
         AdjArgs<R> aa{};
         aa.blist = (const int32_t *)h->d_blist.p;
@@ -686,7 +789,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         if (h->n_branch > 0) launch_frechet(fa, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[6], s));
     } else {
-        if (flags & RUN_OUTSIDE) launch_outside<R>(pr, h->T, custom_pi == nullptr, s);
+        if (flags & RUN_OUTSIDE) launch_outside_parts<R>(h, pr, custom_pi == nullptr, s);
         if (prof)
             for (int k = 4; k <= 6; ++k) HIP_TRY(h, hipEventRecord(h->ev[k], s));
     }
@@ -742,6 +845,8 @@ hy_engine new_engine(int device, int n_cu, const std::shared_ptr<StreamSet> &ss)
     h->profile = pe && pe[0] == '1';
     const char *ng = std::getenv("HY_NO_GRAPH");
     h->use_graphs = !(ng && ng[0] == '1');
+    const char *np = std::getenv("HY_NO_PARTS");
+    h->no_parts = np && np[0] == '1';
     if (h->profile)
         for (auto &ev : h->ev) (void)hipEventCreate(&ev);
     return h;
@@ -817,7 +922,7 @@ void hy_destroy(hy_engine h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     DevBuf *bufs[] = {&h->d_genes, &h->d_tile_gene, &h->d_mat_gene, &h->d_qdesc, &h->d_nd, &h->d_dpos, &h->d_dfrom, &h->d_dto, &h->d_syn, &h->d_pos, &h->d_pi,
-                      &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
+                      &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_parts, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
                       &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
                       &h->d_site, &h->d_Apart, &h->d_fr, &h->d_res, &h->d_Asum, &h->d_W, &h->d_wl, &h->d_proj, &h->d_wover};
@@ -1541,10 +1646,11 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
         std::vector<int32_t> own_sched, own_osched;
         own_sched.swap(h->sched);
         own_osched.swap(h->osched);
-        const int own_uspill = h->n_uspill, own_key = h->sched_key;
+        const int own_uspill = h->n_uspill, own_key = h->sched_key, own_nparts = h->n_parts;
+        std::vector<int32_t> own_parts(h->parts);
         const bool own_lean = h->lean;
         h->lean = lean;
-        build_schedules(h, T, esz);
+        build_schedules(h, T, esz, false);
         h->lean = own_lean;
         std::vector<int32_t> g_sched, g_osched;
         g_sched.swap(h->sched);
@@ -1554,6 +1660,8 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
         h->osched.swap(own_osched);
         h->n_uspill = own_uspill;
         h->sched_key = own_key;
+        h->n_parts = own_nparts;
+        h->parts.swap(own_parts);
         const int ntiles = (h->P + T - 1) / T, P_pad = ntiles * T;
         GeneInfo &gi = genes[g];
         gi = GeneInfo{};

@@ -159,6 +159,12 @@ struct PruneArgs {
     double *site_ll;
     const GeneInfo *genes;  // batched evaluation: gene table and tile -> gene map
     const int32_t *tile_gene;
+    // Subtree parts: row part0 + blockIdx.y of the table {inside offset,
+    // count, outside offset, count} selects the schedules (null: the whole
+    // schedules).  root_out: the inside schedule ends at the root (site
+    // log-likelihoods are written).
+    const int4 *parts;
+    int part0, root_out;
 };
 
 template <typename R>
@@ -227,9 +233,10 @@ void launch_model(const ModelArgs &a, int n_mat, hipStream_t s);
 // P builds (and, when pj is given, the dQ projections of n_mat matrices) in one launch
 template <typename R> void launch_pmat(const PmatArgs<R> &a, const ProjArgs *pj, int n_mat, hipStream_t s);
 template <typename R> void launch_pmat_host(const PmatArgs<R> &a, const double *Phost, hipStream_t s);
-template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, hipStream_t s);
+// ny: workgroups along y (subtree parts); the grid is n_tiles x ny
+template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, int ny, hipStream_t s);
 // rev: every edge matrix is reversible with respect to a.pi (built from the model)
-template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, hipStream_t s);
+template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, int ny, hipStream_t s);
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s);
 void launch_frechet(const FrechetArgs &a, hipStream_t s);  // chunk sums, spectral transforms, projections
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s);  // one workgroup per gene

@@ -885,6 +885,13 @@ __device__ __forceinline__ void stage_tip_codes(uint16_t *tcodes, const uint16_t
 // return the gene-local tile index.  Single problems pass no table.
 template <typename R>
 __device__ __forceinline__ int gene_view(PruneArgs<R> &a, int tile) {
+    if (a.parts) {  // subtree part blockIdx.y of a single problem
+        const int4 pp = a.parts[a.part0 + blockIdx.y];
+        a.sched += (size_t)pp.x * SCHED_STRIDE;
+        a.n_sched = pp.y;
+        a.osched += (size_t)pp.z * OSCHED_STRIDE;
+        a.n_osched = pp.w;
+    }
     if (!a.genes) return tile;
     const int gi = a.tile_gene[tile];
     const GeneInfo g = a.genes[gi];
@@ -1031,6 +1038,7 @@ __global__ __launch_bounds__(256, LEAN ? 4 : ((sizeof(R) == 4 && T == 16) ? 2 : 
             if (k + 1 < n) operands(ec, pa, xt, ab);
         }
     }
+    if (!a.root_out) return;  // a subtree part: its root's partials are in HBM
     // Root likelihood: L_t = sum_s pi_s D_root[s][t] (FP64 accumulation).
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
@@ -1959,39 +1967,39 @@ template <typename R> void launch_pmat(const PmatArgs<R> &a, const ProjArgs *pj,
         hipLaunchKernelGGL(k_pmat<R>, dim3(a.n_branch + n_proj), dim3(256), 0, s, a, pj ? *pj : ProjArgs{}, n_proj);
 }
 
-template <typename R, int T> void launch_inside_t(const PruneArgs<R> &a, hipStream_t s) {
+template <typename R, int T> void launch_inside_t(const PruneArgs<R> &a, int ny, hipStream_t s) {
     if (a.tip_rows * T <= tip_lds_codes(T, sizeof(R)))
-        hipLaunchKernelGGL((k_inside<R, T, true>), dim3(a.n_tiles), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_inside<R, T, true>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((k_inside<R, T, false>), dim3(a.n_tiles), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_inside<R, T, false>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
 }
 
-template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, hipStream_t s) {
+template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, int ny, hipStream_t s) {
     if constexpr (sizeof(R) == 4) {
         if (lean && T == 16) {
-            hipLaunchKernelGGL((k_inside<R, 16, false, true>), dim3(a.n_tiles), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_inside<R, 16, false, true>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
             return;
         }
     }
     switch (T) {
-        case 16: launch_inside_t<R, 16>(a, s); break;
-        case 32: launch_inside_t<R, 32>(a, s); break;
-        default: launch_inside_t<R, 64>(a, s); break;
+        case 16: launch_inside_t<R, 16>(a, ny, s); break;
+        case 32: launch_inside_t<R, 32>(a, ny, s); break;
+        default: launch_inside_t<R, 64>(a, ny, s); break;
     }
 }
 
-template <typename R, int T, bool REV> void launch_outside_t(const PruneArgs<R> &a, hipStream_t s) {
+template <typename R, int T, bool REV> void launch_outside_t(const PruneArgs<R> &a, int ny, hipStream_t s) {
     if (a.tip_rows * T <= tip_lds_codes(T, sizeof(R)))
-        hipLaunchKernelGGL((k_outside<R, T, true, REV>), dim3(a.n_tiles), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_outside<R, T, true, REV>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((k_outside<R, T, false, REV>), dim3(a.n_tiles), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_outside<R, T, false, REV>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
 }
 
-template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, hipStream_t s) {
+template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, int ny, hipStream_t s) {
     switch (T) {
-        case 16: rev ? launch_outside_t<R, 16, true>(a, s) : launch_outside_t<R, 16, false>(a, s); break;
-        case 32: rev ? launch_outside_t<R, 32, true>(a, s) : launch_outside_t<R, 32, false>(a, s); break;
-        default: rev ? launch_outside_t<R, 64, true>(a, s) : launch_outside_t<R, 64, false>(a, s); break;
+        case 16: rev ? launch_outside_t<R, 16, true>(a, ny, s) : launch_outside_t<R, 16, false>(a, ny, s); break;
+        case 32: rev ? launch_outside_t<R, 32, true>(a, ny, s) : launch_outside_t<R, 32, false>(a, ny, s); break;
+        default: rev ? launch_outside_t<R, 64, true>(a, ny, s) : launch_outside_t<R, 64, false>(a, ny, s); break;
     }
 }
 
@@ -2031,10 +2039,10 @@ template void launch_pmat<float>(const PmatArgs<float> &, const ProjArgs *, int,
 template void launch_pmat<double>(const PmatArgs<double> &, const ProjArgs *, int, hipStream_t);
 template void launch_pmat_host<float>(const PmatArgs<float> &, const double *, hipStream_t);
 template void launch_pmat_host<double>(const PmatArgs<double> &, const double *, hipStream_t);
-template void launch_inside<float>(const PruneArgs<float> &, int, bool, hipStream_t);
-template void launch_inside<double>(const PruneArgs<double> &, int, bool, hipStream_t);
-template void launch_outside<float>(const PruneArgs<float> &, int, bool, hipStream_t);
-template void launch_outside<double>(const PruneArgs<double> &, int, bool, hipStream_t);
+template void launch_inside<float>(const PruneArgs<float> &, int, bool, int, hipStream_t);
+template void launch_inside<double>(const PruneArgs<double> &, int, bool, int, hipStream_t);
+template void launch_outside<float>(const PruneArgs<float> &, int, bool, int, hipStream_t);
+template void launch_outside<double>(const PruneArgs<double> &, int, bool, int, hipStream_t);
 template void launch_adjoint<float>(const AdjArgs<float> &, int, hipStream_t);
 template void launch_adjoint<double>(const AdjArgs<double> &, int, hipStream_t);
 
