@@ -205,6 +205,7 @@ def main():
     eng = make_engine(arrays, K, args.tile)
     ms32, out32, _ = run_steps(eng, eval_args, 1, args.steps, args.warmup, False)
     tile32 = eng.last_tile()
+    parts32 = eng.last_parts()
     print(f"[bench] fp32 {ms32 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
     res = {}
     if not args.no_fp64:
@@ -239,6 +240,10 @@ def main():
     cand = {s: stage_ms[s] for s in ("inside", "outside", "adjoint")}
     dom = max(cand, key=cand.get)
     roof = roofline(dom, work[dom], stage_ms[dom] * 1e-3)
+    # with subtree parts the pruning passes are two launches per evaluation
+    # (subtree parts, top part): the kernel time, bytes and flops are per
+    # evaluation, summed over both
+    roof["launches_per_eval"] = 2 if (parts32 > 0 and dom in ("inside", "outside")) else 1
     roof["traffic"] = committed_traffic(KERNEL_NAMES[dom], args.config, tile)
     meta = work["_meta"]
     # whole-evaluation bound (algorithmic work of one logL + full gradient, FP32)
@@ -254,7 +259,7 @@ def main():
         "config": {"workload": f"{args.config}: {cfg['taxa']} taxa x {cfg['codons']} codons x {K} omega categories, "
                                "logL + full gradient (branches, omegas, weights, GTR)",
                    "taxa": cfg["taxa"], "codons": cfg["codons"], "patterns": int(len(arrays["weights"])),
-                   "categories": K, "seed": cfg["seed"], "tile": tile},
+                   "categories": K, "seed": cfg["seed"], "tile": tile, "subtree_parts": parts32},
         "roofline": roof,
         "eval_roofline": {"flops": f_alg, "bytes": b_alg, "t_bound_ms": t_bound * 1e3,
                           "frac": t_bound / ms32, "note": "max(F/157.3 TFLOP/s, B/8 TB/s) over the measured eval"},
@@ -430,16 +435,17 @@ def roofline(stage, w, t):
 
 
 def committed_traffic(kernel, config, tile):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), read from
-    the summary committed under profiles/; None when no measurement matches."""
+    """HBM bytes of `kernel` per evaluation (all its launches in one
+    evaluation) measured with rocprofv3 PMC counters (FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy kernel of known size), read from the summary committed
+    under profiles/; None when no measurement matches."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         data = json.load(open(path))
     except (OSError, ValueError):
         return None
     e = data.get(f"{kernel}/{config}/T{tile}")
-    return None if e is None else e.get("bytes_per_launch")
+    return None if e is None else e.get("bytes_per_eval", e.get("bytes_per_launch"))
 
 
 def copy_calibration():

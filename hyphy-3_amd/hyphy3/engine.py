@@ -26,7 +26,7 @@ HY_WANT_GRAD = 4
 
 EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
-           "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
+           "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
            "hy_inside_outside_from_P", "hy_read_partials")
 
 _i32p = C.POINTER(C.c_int32)
@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH):
     L.hy_set_warm_start.argtypes = [C.c_void_p, C.c_int]
     L.hy_last_eigen_status.argtypes = [C.c_void_p, _i32p, C.c_int]
     L.hy_last_tile.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.hy_last_parts.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.hy_eval_srv.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
                               C.c_double, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                               _f64p]
@@ -290,6 +291,12 @@ class Engine:
     def last_tile(self):
         t = C.c_int()
         self._check(self.L.hy_last_tile(self.h, C.byref(t)))
+        return t.value
+
+    def last_parts(self):
+        """Subtree parts of the last evaluation's pruning passes (0: none)."""
+        t = C.c_int()
+        self._check(self.L.hy_last_parts(self.h, C.byref(t)))
         return t.value
 
     def device_bytes(self):

@@ -202,6 +202,11 @@ int hy_set_tile(hy_engine h, int patterns_per_tile);
  * reference (tuning introspection). */
 int hy_last_tile(hy_engine h, int *patterns_per_tile);
 
+/* Subtree parts the last evaluation split each pruning pass into (0: one
+ * schedule for the whole tree; n > 0: n subtree parts plus the top part, two
+ * launches per pass).  No counterpart in the reference (tuning introspection). */
+int hy_last_parts(hy_engine h, int *n_parts);
+
 /* Warm starts (default on): each hy_eval refines the previous call's
  * eigenvectors of every rate matrix (Ogita-Aishima iteration, FP64) and only
  * falls back to a cold Jacobi solve when that does not converge; results agree
