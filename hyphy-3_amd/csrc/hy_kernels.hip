@@ -711,6 +711,14 @@ __device__ __forceinline__ void mfma_tile(const R *av, const R *Bs, typename MF<
         for (int nb = 0; nb < NB; ++nb) X[nb] = MF<R>::mma(av[kk], Bs[(4 * kk + lg) * T + 16 * nb + lr], X[nb]);
 }
 
+// Waits here for n loaded values (one wait for all of them), so that loads
+// issued together are not split into a load-and-wait per register group.
+template <typename R, int N>
+__device__ __forceinline__ void arrive(const R *v) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) asm volatile("" ::"v"(v[i]));
+}
+
 // Makes the MFMA results in x available before the next instruction, so
 // that following loads may reuse the registers those MFMAs read.
 template <typename R, int T>
@@ -993,6 +1001,7 @@ __global__ __launch_bounds__(256, LEAN ? 4 : ((sizeof(R) == 4 && T == 16) ? 2 : 
                 } else {  // partials only in HBM
                     R bv[16 * NB];
                     fetch_b<R, T>(bv, a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
+                    if constexpr (sizeof(R) == 4) arrive<R, 16 * NB>(bv);  // (FP64: interleaved is faster)
                     mfma_regs<R, T>(pa[c], bv, X);
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) Ec[nb] += a.E[(size_t)idx * a.P_pad + p0 + 16 * nb + lr];
@@ -1214,10 +1223,14 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
                         // reuse these registers once X is consumed)
                         R bv[16 * NB], av[16];
                         fetch_b<R, T>(bv, a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
-                        if (c < RS)
+                        if (c < RS) {
                             mfma_regs<R, T>(pa[c < RS ? c : 0], bv, X[c]);
-                        else {
+                        } else {
                             fetch_edge<R>(av, a.PA, ec.v[6 + 4 * c], w, l);
+                            if constexpr (DREG) {  // rare path here; FP64 wide tiles: interleaved is faster
+                                arrive<R, 16 * NB>(bv);
+                                arrive<R, 16>(av);
+                            }
                             mfma_regs<R, T>(av, bv, X[c]);
                         }
                         consume<R, T>(X[c]);
