@@ -57,7 +57,6 @@ struct hy_engine_s {
     int n_genes = 1;
     bool lean = false;  // schedules and inside launch of the lean variant
     DevBuf d_genes, d_tile_gene, d_mat_gene;
-    size_t n_res = 0;  // doubles in the result vector
     hipStream_t stream = nullptr;
     // Captured evaluation graphs (parameter upload, every kernel, result
     // download), replayed while the configuration epoch is unchanged.
@@ -132,11 +131,18 @@ struct hy_engine_s {
     double *h_res = nullptr;
     size_t h_res_cap = 0;
     int *h_status = nullptr;
-    DevBuf d_params, d_lam, d_U, d_V, d_mu, d_status, d_tau, d_M, d_dmu, d_Z;
+    // device addresses of the pinned h_params / h_res / h_status: k_model reads
+    // the parameters from h_params under graph capture (zc_params) and writes
+    // the eigensolver status, k_final writes the results, with no copies
+    const double *h_params_dev = nullptr;
+    double *h_res_dev = nullptr;
+    int *h_status_dev = nullptr;
+    bool zc_params = false;
+    DevBuf d_params, d_lam, d_U, d_V, d_mu, d_tau, d_M, d_dmu, d_Z;
     bool warm_enable = true, warm_valid = false;
     DevBuf d_Prow[2], d_PA[2], d_PTA[2];  // Prow: [n_ext][PRS] (P row-major, then its row sums)
-    DevBuf d_D[2], d_Vp[2], d_E, d_G, d_Lroot, d_Eroot, d_site, d_Apart, d_fr, d_res;
-    DevBuf d_Asum, d_W, d_wl, d_proj;
+    DevBuf d_D[2], d_Vp[2], d_E, d_G, d_Lroot, d_Eroot, d_site, d_Apart, d_fr;
+    DevBuf d_wl, d_proj;
     int grp_size = 1, n_grp = 1;  // Fréchet stage grouping of (branch, component) pairs per matrix
     std::vector<int32_t> wl;  // uploaded copy of the per-matrix W slot lists
     int n_chunks = 1, tiles_per_chunk = 1;
@@ -161,6 +167,11 @@ struct GroupState {
 };
 
 namespace {
+
+// Pinned buffers the kernels read or write directly (parameters, results,
+// eigensolver status): coherent, so device reads see the host's latest
+// writes and host reads see the kernels' writes without cache maintenance.
+constexpr unsigned kZeroCopyFlags = hipHostMallocMapped | hipHostMallocCoherent;
 
 int fail(hy_engine h, int code, const char *fmt, ...) {
     char buf[1024];
@@ -371,8 +382,6 @@ int prepare_buffers(hy_engine h, int prec) {
     h->n_chunks = (h->n_tiles + h->tiles_per_chunk - 1) / h->tiles_per_chunk;
     TRY(ensure(h, h->d_Apart, (size_t)nb * h->n_chunks * 4096 * 8));
     TRY(ensure(h, h->d_fr, (size_t)nb * h->max_comp * 8 * 8));
-
-    TRY(ensure(h, h->d_proj, (size_t)h->n_mat * 6 * HPROJ_CHUNKS * 8));
     {
         // per matrix: the W slots (branch * max_comp + component index) whose
         // component uses it, branch-major (fixed summation order)
@@ -391,8 +400,8 @@ int prepare_buffers(hy_engine h, int prec) {
         }
         set_frechet_groups(h, per);
     }
-    TRY(ensure(h, h->d_W, (size_t)h->n_mat * h->n_grp * 4096 * 8));
-    TRY(ensure(h, h->d_Asum, (size_t)nb * 4096 * 8));
+    // per (matrix, parameter, group) projections: sized after the grouping
+    TRY(ensure(h, h->d_proj, (size_t)h->n_mat * 6 * h->n_grp * 8));
     const int nm = h->n_mat;
     TRY(ensure(h, h->d_lam, (size_t)nm * 64 * 8));
     TRY(ensure(h, h->d_U, (size_t)nm * 4096 * 8));
@@ -400,26 +409,28 @@ int prepare_buffers(hy_engine h, int prec) {
     TRY(ensure(h, h->d_Z, (size_t)nm * 4096 * 8));
     TRY(ensure(h, h->d_V, (size_t)nm * 4096 * 8));
     TRY(ensure(h, h->d_mu, (size_t)nm * 8));
-    TRY(ensure(h, h->d_status, (size_t)nm * 4));
     TRY(ensure(h, h->d_M, (size_t)nm * 6 * 4096 * 8));
     TRY(ensure(h, h->d_dmu, (size_t)nm * 6 * 8));
     TRY(ensure(h, h->d_tau, ne * 8));
     build_layout(h);
     TRY(ensure(h, h->d_params, (size_t)h->pl.total * 8));
     const size_t nres = 1 + (size_t)h->num_nodes + h->n_mat + h->n_comp + 5;
-    h->n_res = nres;
-    TRY(ensure(h, h->d_res, nres * 8));
     if (h->h_params_cap < (size_t)h->pl.total) {
         if (h->h_params) (void)hipHostFree(h->h_params);
-        HIP_TRY(h, hipHostMalloc((void **)&h->h_params, sizeof(double) * h->pl.total, hipHostMallocDefault));
+        HIP_TRY(h, hipHostMalloc((void **)&h->h_params, sizeof(double) * h->pl.total, kZeroCopyFlags));
         h->h_params_cap = h->pl.total;
+        HIP_TRY(h, hipHostGetDevicePointer((void **)&h->h_params_dev, h->h_params, 0));
     }
     if (h->h_res_cap < nres) {
         if (h->h_res) (void)hipHostFree(h->h_res);
-        HIP_TRY(h, hipHostMalloc((void **)&h->h_res, sizeof(double) * nres, hipHostMallocDefault));
+        HIP_TRY(h, hipHostMalloc((void **)&h->h_res, sizeof(double) * nres, kZeroCopyFlags));
         h->h_res_cap = nres;
+        HIP_TRY(h, hipHostGetDevicePointer((void **)&h->h_res_dev, h->h_res, 0));
     }
-    if (!h->h_status) HIP_TRY(h, hipHostMalloc((void **)&h->h_status, sizeof(int) * 64, hipHostMallocDefault));
+    if (!h->h_status) {
+        HIP_TRY(h, hipHostMalloc((void **)&h->h_status, sizeof(int) * 64, kZeroCopyFlags));
+        HIP_TRY(h, hipHostGetDevicePointer((void **)&h->h_status_dev, h->h_status, 0));
+    }
     return HY_OK;
 }
 
@@ -642,9 +653,14 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         ma.mu = (double *)h->d_mu.p;
         ma.Z = (double *)h->d_Z.p;
         ma.warm = (h->warm_enable && h->warm_valid) ? 1 : 0;
-        ma.status = (int *)h->d_status.p;
+        ma.status = h->h_status_dev;
+        if (h->zc_params) {
+            ma.params_host = h->h_params_dev;
+            ma.params_dev = (double *)h->d_params.p;
+            ma.n_params = h->pl.total;
+        }
         ma.mat_gene = (const int32_t *)h->d_mat_gene.p;
-        ma.tol = sizeof(R) == 4 ? 1e-10 : 1e-14;
+        ma.tol = sizeof(R) == 4 ? 1e-8 : 1e-14;
         launch_model(ma, h->n_mat, s);
         // M_phi = V dQ_phi U and d mu / d phi depend on the spectra only: they
         // are built by the workgroups of the P-build launch beyond the branches
@@ -781,13 +797,14 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         fa.tau = (const double *)h->d_tau.p;
         fa.out = (double *)h->d_fr.p;
         fa.n_mat = h->n_mat;
-        fa.Asum = (double *)h->d_Asum.p;
         fa.grp_size = h->grp_size;
         fa.n_grp = h->n_grp;
-        fa.W = (double *)h->d_W.p;
         fa.wl_ptr = (const int32_t *)h->d_wl.p;
         fa.wl_idx = (const int32_t *)h->d_wl.p + (h->n_mat + 1);
         fa.proj = (double *)h->d_proj.p;
+        if (h->d_proj.bytes < (size_t)h->n_mat * 6 * h->n_grp * 8 ||
+            h->d_Apart.bytes < (size_t)std::max(1, h->n_branch) * h->n_chunks * 4096 * 8)
+            return fail(h, HY_ERR_STATE, "internal: Frechet stage buffers smaller than the grouping needs");
         if (h->n_branch > 0) launch_frechet(fa, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[6], s));
     } else {
@@ -816,7 +833,8 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     fn.mu = (const double *)h->d_mu.p;
     fn.dmu = (const double *)h->d_dmu.p;
     fn.proj = (const double *)h->d_proj.p;
-    fn.res = (double *)h->d_res.p;
+    fn.proj_stride = h->n_grp;
+    fn.res = h->h_res_dev;
     fn.genes = (const GeneInfo *)h->d_genes.p;
     launch_final(fn, h->n_genes, s);
     HIP_TRY(h, hipGetLastError());
@@ -926,8 +944,8 @@ void hy_destroy(hy_engine h) {
     DevBuf *bufs[] = {&h->d_genes, &h->d_tile_gene, &h->d_mat_gene, &h->d_qdesc, &h->d_nd, &h->d_dpos, &h->d_dfrom, &h->d_dto, &h->d_syn, &h->d_pos, &h->d_pi,
                       &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_parts, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
-                      &h->d_status, &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
-                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_res, &h->d_Asum, &h->d_W, &h->d_wl, &h->d_proj, &h->d_wover};
+                      &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
+                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_wl, &h->d_proj, &h->d_wover};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -1279,26 +1297,17 @@ int hy_set_tile(hy_engine h, int patterns_per_tile) {
 static int finish_host(hy_engine h, bool check_status, double *out_logL, double *out_grad_bl, double *out_grad_omega,
                        double *out_grad_w, double *out_grad_theta, bool grad);
 
-// Result download of an eager evaluation (the graph path captures it).
-static int download_results(hy_engine h, bool check_status) {
-    const size_t nres = h->n_res;
-    HIP_TRY(h, hipMemcpyAsync(h->h_res, h->d_res.p, nres * 8, hipMemcpyDeviceToHost, h->stream));
-    if (check_status)
-        HIP_TRY(h, hipMemcpyAsync(h->h_status, h->d_status.p, sizeof(int) * h->n_mat, hipMemcpyDeviceToHost, h->stream));
-    return HY_OK;
-}
-
 static int finish_eval(hy_engine h, bool check_status, double *out_logL, double *out_site_ll, double *out_grad_bl,
                        double *out_grad_omega, double *out_grad_w, double *out_grad_theta, bool grad) {
-    TRY(download_results(h, check_status));
     if (out_site_ll && h->P > 0)
         HIP_TRY(h, hipMemcpyAsync(out_site_ll, h->d_site.p, sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     return finish_host(h, check_status, out_logL, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta, grad);
 }
 
-// Evaluates through a captured graph: parameters (already packed in the pinned
-// h_params) up, all kernels, results down.  Captures on first use per
+// Evaluates through a captured graph: all kernels, with k_model reading the
+// parameters from the pinned h_params (packed before the launch) and the
+// results and eigensolver status written straight to pinned host memory.  Captures on first use per
 // (precision, gradient, warm-start) and whenever the configuration epoch moved.
 static int launch_graph(hy_engine h, int prec, bool grad) {
     const int warm = (h->warm_enable && h->warm_valid) ? 1 : 0;
@@ -1312,13 +1321,10 @@ static int launch_graph(hy_engine h, int prec, bool grad) {
     if (!ge || !ge->exec) {
         hipGraph_t graph = nullptr;
         HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
-        int rc = HY_OK;
-        const hipError_t e1 = hipMemcpyAsync(h->d_params.p, h->h_params, sizeof(double) * h->pl.total,
-                                             hipMemcpyHostToDevice, h->stream);
-        if (e1 != hipSuccess) rc = fail(h, HY_ERR_HIP, "graph capture: %s", hipGetErrorString(e1));
         const unsigned flags = HY_WANT_LOGL | (grad ? HY_WANT_GRAD : 0u);
-        if (rc == HY_OK) rc = prec == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
-        if (rc == HY_OK) rc = download_results(h, true);
+        h->zc_params = true;
+        const int rc = prec == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
+        h->zc_params = false;
         const hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
         if (rc != HY_OK) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -1435,7 +1441,6 @@ int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, do
         HIP_TRY(h, hipMemcpyAsync(h->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, h->stream));
         const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
         if (rc) return rc;
-        TRY(download_results(h, true));
     }
     if (want_site)
         HIP_TRY(h, hipMemcpyAsync(h->h_site, h->d_site.p, sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
@@ -1843,7 +1848,6 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
     c->osched = osched;
     c->blist = blist;
     c->ext_ident = ident;
-    c->n_res = res_off;
     c->warm_valid = false;
     c->have_problem = c->have_model = true;
     {
@@ -1911,34 +1915,34 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
     TRY(ensure(c, c->d_V, (size_t)mat0 * 4096 * 8));
     TRY(ensure(c, c->d_Z, (size_t)mat0 * 4096 * 8));
     TRY(ensure(c, c->d_mu, (size_t)mat0 * 8));
-    TRY(ensure(c, c->d_status, (size_t)mat0 * 4));
     TRY(ensure(c, c->d_M, (size_t)mat0 * 6 * 4096 * 8));
     TRY(ensure(c, c->d_dmu, (size_t)mat0 * 6 * 8));
     TRY(ensure(c, c->d_Apart, (size_t)br0 * 4096 * 8));
     TRY(ensure(c, c->d_fr, (size_t)br0 * max_comp * 8 * 8));
-    TRY(ensure(c, c->d_W, (size_t)mat0 * c->n_grp * 4096 * 8));
-    TRY(ensure(c, c->d_Asum, (size_t)br0 * 4096 * 8));
-    TRY(ensure(c, c->d_proj, (size_t)mat0 * 6 * HPROJ_CHUNKS * 8));
+    TRY(ensure(c, c->d_proj, (size_t)mat0 * 6 * c->n_grp * 8));
     TRY(ensure(c, c->d_params, (size_t)c->pl.total * 8));
-    TRY(ensure(c, c->d_res, res_off * 8));
     (void)tile_elems;
     if (c->h_params_cap < (size_t)c->pl.total) {
         if (c->h_params) (void)hipHostFree(c->h_params);
         c->h_params = nullptr;
         c->h_params_cap = 0;
-        HIP_TRY(c, hipHostMalloc((void **)&c->h_params, sizeof(double) * c->pl.total, hipHostMallocDefault));
+        HIP_TRY(c, hipHostMalloc((void **)&c->h_params, sizeof(double) * c->pl.total, kZeroCopyFlags));
         c->h_params_cap = c->pl.total;
+        HIP_TRY(c, hipHostGetDevicePointer((void **)&c->h_params_dev, c->h_params, 0));
     }
     if (c->h_res_cap < res_off) {
         if (c->h_res) (void)hipHostFree(c->h_res);
         c->h_res = nullptr;
         c->h_res_cap = 0;
-        HIP_TRY(c, hipHostMalloc((void **)&c->h_res, sizeof(double) * res_off, hipHostMallocDefault));
+        HIP_TRY(c, hipHostMalloc((void **)&c->h_res, sizeof(double) * res_off, kZeroCopyFlags));
         c->h_res_cap = res_off;
+        HIP_TRY(c, hipHostGetDevicePointer((void **)&c->h_res_dev, c->h_res, 0));
     }
     if (c->h_status) (void)hipHostFree(c->h_status);
     c->h_status = nullptr;
-    HIP_TRY(c, hipHostMalloc((void **)&c->h_status, sizeof(int) * std::max(mat0, 64), hipHostMallocDefault));
+    c->h_status_dev = nullptr;
+    HIP_TRY(c, hipHostMalloc((void **)&c->h_status, sizeof(int) * std::max(mat0, 64), kZeroCopyFlags));
+    HIP_TRY(c, hipHostGetDevicePointer((void **)&c->h_status_dev, c->h_status, 0));
     if (c->h_site_cap < (size_t)pbase) {
         if (c->h_site) (void)hipHostFree(c->h_site);
         c->h_site = nullptr;
@@ -1979,7 +1983,6 @@ int eval_fused(GroupState &gs, hy_batch_item *items, int n, int prec, unsigned f
         HIP_TRY(c, hipMemcpyAsync(c->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, c->stream));
         const int rc = prec == HY_PREC_FP64 ? run_eval<double>(c, flags, nullptr) : run_eval<float>(c, flags, nullptr);
         if (rc) return rc;
-        TRY(download_results(c, true));
     }
     if (want_site)
         HIP_TRY(c, hipMemcpyAsync(c->h_site, c->d_site.p, sizeof(double) * c->P_pad, hipMemcpyDeviceToHost, c->stream));

@@ -115,9 +115,15 @@ struct ModelArgs {
     double *mu;   // [n_mat]
     double *Z;    // [n_mat][64][64] eigenvectors of the symmetrised generator (warm-start state)
     int warm;     // 1: refine Z from the previous call
-    int *status;  // [n_mat] Jacobi sweeps, 100+k after k refinement steps, negative = no convergence
+    int *status;  // [n_mat] Jacobi sweeps, 100+k after k refinement steps (100: kept), negative = no convergence
     const int32_t *mat_gene;  // batched evaluation: gene of each matrix (pi, pos, alpha, theta, delta, psi)
     double tol;               // warm refinement: accepted predicted correction (relative)
+    // When set: the parameter block is read from pinned host memory and
+    // copied to params_dev (grid-strided over the workgroups) for the
+    // kernels that follow, replacing a host-to-device copy.
+    const double *params_host;
+    double *params_dev;
+    int n_params;
 };
 
 template <typename R>
@@ -196,21 +202,17 @@ struct ProjArgs {
     const int32_t *mat_gene;  // batched evaluation: gene of each matrix
 };
 
-constexpr int HPROJ_CHUNKS = 16;  // 4096 entries / 256 threads
-
 struct FrechetArgs {
     const int32_t *blist;
     int n_branch, n_chunks, max_comp, n_mat;
     const int32_t *comp_ptr, *comp_mat;
     const double *params;
     ParamLayout pl;
-    const double *Apart, *lam, *U, *V, *M, *tau;
-    double *Asum;                    // [n_branch][4096] adjoint summed over pattern chunks
+    const double *Apart, *lam, *U, *V, *M, *tau;  // Apart: [n_branch][n_chunks][4096]
     int grp_size, n_grp;             // (branch, component) pairs per workgroup; groups per matrix
-    double *W;                       // [n_mat][n_grp][4096] partial H_m of each group
     const int32_t *wl_ptr, *wl_idx;  // per matrix: pair slots (branch * max_comp + component) using it
     double *out;                     // [n_branch][max_comp][8]: [0] <A, P_m>, [1] w_j d lnL / d tau share
-    double *proj;                    // [n_mat][6][HPROJ_CHUNKS] partial <M_phi, H_m>
+    double *proj;                    // [n_mat][6][n_grp] <M_phi, H_m> of each group's partial H_m
 };
 
 struct FinalArgs {
@@ -224,6 +226,7 @@ struct FinalArgs {
     const double *params;
     ParamLayout pl;
     const double *fr, *tau, *mu, *dmu, *proj;
+    int proj_stride;  // groups per matrix in proj ([n_mat][6][proj_stride])
     double *res;  // [0] lnL, [1..n_real] grad_bl, then omega[n_mat], w[n_comp], theta[5]
     const GeneInfo *genes;  // batched evaluation: one workgroup per gene, results at res_off
 };

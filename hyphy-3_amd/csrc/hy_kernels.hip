@@ -88,35 +88,9 @@ __device__ __forceinline__ double nuc_rate(const double *th, int a, int b) {
 constexpr int LDA = 65;  // padded LDS row stride (doubles) for 64x64 FP64 tiles
 
 // C = op(X) * op(Y) for 64x64 FP64 tiles in LDS (row stride LDA) on
-// v_mfma_f64_16x16x4; NW waves share the 16 output blocks (wave w takes row
-// block w % 4 and every (NW/4)-th column block).
-template <bool TX, bool TY, int NW>
-__device__ __forceinline__ void gemm64(const double *X, const double *Y, double *C, int wave, int lr, int lg) {
-    constexpr int CS = NW / 4;
-    constexpr int NCB = 4 / CS;
-    const int rb = wave & 3, cb0 = wave >> 2;
-    f64x4 acc[NCB];
-#pragma unroll
-    for (int q = 0; q < NCB; ++q) acc[q] = splat<f64x4>(0.0);
-#pragma unroll 4
-    for (int kk = 0; kk < 16; ++kk) {
-        const int i = 16 * rb + lr, k = 4 * kk + lg;
-        const double av = TX ? X[k * LDA + i] : X[i * LDA + k];
-#pragma unroll
-        for (int q = 0; q < NCB; ++q) {
-            const int j = 16 * (cb0 + q * CS) + lr;
-            const double bv = TY ? Y[j * LDA + k] : Y[k * LDA + j];
-            acc[q] = MF<double>::mma(av, bv, acc[q]);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < NCB; ++q)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) C[(16 * rb + MF<double>::crow(lg, r)) * LDA + 16 * (cb0 + q * CS) + lr] = acc[q][r];
-}
-
-// gemm64 in two halves: products into registers, then the store, so the
-// result may overwrite an operand once every wave has finished reading.
+// v_mfma_f64_16x16x4, in two halves: products into registers (wave w owns row
+// block w of four waves), then the store, so the result may overwrite an
+// operand once every wave has finished reading.
 template <bool TX, bool TY>
 __device__ __forceinline__ void gemm64_acc(const double *X, const double *Y, f64x4 *acc, int rb, int lr, int lg) {
 #pragma unroll
@@ -208,7 +182,9 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
     __shared__ int any_rot;
     __shared__ double thl[8], posl[12];
 
-    const double *prm = a.params;
+    if (a.params_host)
+        for (int i = m * NT + tid; i < a.n_params; i += gridDim.x * NT) a.params_dev[i] = a.params_host[i];
+    const double *prm = a.params_host ? a.params_host : a.params;
     const double alpha = prm[a.pl.off_alpha], beta = prm[a.pl.off_omega + m];
     const double delta = prm[a.pl.off_delta], psi = prm[a.pl.off_psi];
     const int S = a.S;
@@ -238,13 +214,13 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
             }
         }
         A[i * LDA + j] = q;
+        // a wave holds one whole row (NT is a multiple of 64): Q_ii = -sum_j Q_ij
+        double rsum = q;
+        for (int off = 32; off > 0; off >>= 1) rsum += __shfl_xor(rsum, off);
+        if (j == i) A[i * LDA + i] = -rsum;
     }
     __syncthreads();
     if (tid < 64) {
-        double s = 0.0;
-        for (int j = 0; j < 64; ++j)
-            if (j != tid) s += A[tid * LDA + j];
-        A[tid * LDA + tid] = -s;
         const double p = a.pi[tid];
         if (tid < S) {
             const double v = sqrt(p > 1e-12 ? p : 1e-12);
@@ -288,67 +264,79 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
         for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = Zg[idx];
         __syncthreads();
         double prev_en = 1e300;
+        // Wave w owns the 16x16 block (rows 16 rb.., columns 16 cb..) of
+        // every product; S = X^T A X and X E stay in registers.
+        const int rb = w & 3, cb = w >> 2;
         for (int it = 0; it < 8 && !done; ++it) {
-            gemm64<true, false, NT / 64>(Z, Z, T1, w, lr, lg);  // X^T X
-            gemm64<false, false, NT / 64>(A, Z, T2, w, lr, lg);  // A X
-            __syncthreads();
-            // R = I - X^T X (into T1)
-            for (int idx = tid; idx < 4096; idx += NT) {
-                const int i = idx >> 6, j = idx & 63;
-                T1[i * LDA + j] = (i == j ? 1.0 : 0.0) - T1[i * LDA + j];
+            // X^T X and A X in one pass (shared X operand loads)
+            f64x4 aR = splat<f64x4>(0.0), aAX = splat<f64x4>(0.0);
+#pragma unroll 4
+            for (int kk = 0; kk < 16; ++kk) {
+                const int i = 16 * rb + lr, k = 4 * kk + lg, j = 16 * cb + lr;
+                const double xb = Z[k * LDA + j];
+                aR = MF<double>::mma(Z[k * LDA + i], xb, aR);
+                aAX = MF<double>::mma(A[i * LDA + k], xb, aAX);
+            }
+            // R = I - X^T X into T1, A X into T2
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * rb + MF<double>::crow(lg, r), j = 16 * cb + lr;
+                T1[i * LDA + j] = (i == j ? 1.0 : 0.0) - aR[r];
+                T2[i * LDA + j] = aAX[r];
             }
             __syncthreads();
-            // S = X^T (A X): computed into Z-sized registers, stored after a barrier
-            {
-                constexpr int CS = (NT / 64) / 4, NCB = 4 / CS;
-                const int rb = w & 3, cb0 = w >> 2;
-                f64x4 acc[NCB];
-#pragma unroll
-                for (int q = 0; q < NCB; ++q) acc[q] = splat<f64x4>(0.0);
-                for (int kk = 0; kk < 16; ++kk) {
-                    const int i = 16 * rb + lr, k = 4 * kk + lg;
-                    const double av = Z[k * LDA + i];
-#pragma unroll
-                    for (int q = 0; q < NCB; ++q) acc[q] = MF<double>::mma(av, T2[k * LDA + 16 * (cb0 + q * CS) + lr], acc[q]);
-                }
-                __syncthreads();
-#pragma unroll
-                for (int q = 0; q < NCB; ++q)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) T2[(16 * rb + MF<double>::crow(lg, r)) * LDA + 16 * (cb0 + q * CS) + lr] = acc[q][r];
+            // S = X^T (A X), kept in registers; l_i = s_ii / (1 - r_ii)
+            f64x4 aS = splat<f64x4>(0.0);
+#pragma unroll 4
+            for (int kk = 0; kk < 16; ++kk) {
+                const int i = 16 * rb + lr, k = 4 * kk + lg, j = 16 * cb + lr;
+                aS = MF<double>::mma(Z[k * LDA + i], T2[k * LDA + j], aS);
             }
-            __syncthreads();
-            if (tid < 64) lam[tid] = T2[tid * LDA + tid] / (1.0 - T1[tid * LDA + tid]);
-            // cluster threshold delta = 2 (||S - D||_F + ||A||_F ||R||_F)
-            double so = 0.0, unused = 0.0;
-            for (int idx = tid; idx < 4096; idx += NT) {
-                const int i = idx >> 6, j = idx & 63;
-                if (i != j) so += T2[i * LDA + j] * T2[i * LDA + j];
+            double so = 0.0, rq = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * rb + MF<double>::crow(lg, r), j = 16 * cb + lr;
+                const double rr = T1[i * LDA + j];
+                rq += rr * rr;
+                if (i != j)
+                    so += aS[r] * aS[r];
+                else
+                    lam[i] = aS[r] / (1.0 - rr);
             }
-            block_sum2<NT>(so, unused, red, tid);
+            // (the reduction's barriers also publish lam and end every read of T2)
+            block_sum2<NT>(so, rq, red, tid);
             const double soff = sqrt(so);
+            // Eigenvectors that already diagonalise A to 1e-12 (and are
+            // orthonormal to 1e-12) are kept as they are: a step from there
+            // only adds rounding noise to close pairs, which repeated
+            // evaluations at the same parameters would accumulate.
+            if (it == 0 && soff <= 1e-12 * anorm && sqrt(rq) <= 1e-12) {
+                done = true;
+                status = 100;
+                break;
+            }
             // Pairs closer than the off-diagonal mass are treated as a cluster
             // (orthogonalised only).  The threshold leaves out Ogita-Aishima's
             // ||A|| ||R|| term: after an update R is second order, and that term
             // froze close pairs for an extra step; a step that does not
             // contract still falls back to the cold solve.
             const double dlt = 2.0 * soff;
-            // E into T2 (S is no longer needed after each element)
+            // E into T2, from S in registers
             double en = 0.0, fz = 0.0;
-            for (int idx = tid; idx < 4096; idx += NT) {
-                const int i = idx >> 6, j = idx & 63;
-                const double r = T1[i * LDA + j];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * rb + MF<double>::crow(lg, r), j = 16 * cb + lr;
+                const double rr = T1[i * LDA + j];
                 double e;
                 if (i == j) {
-                    e = 0.5 * r;
+                    e = 0.5 * rr;
                 } else {
                     const double d = lam[j] - lam[i];
-                    const double sij = T2[i * LDA + j];
                     if (fabs(d) > dlt) {
-                        e = (sij + lam[j] * r) / d;
+                        e = (aS[r] + lam[j] * rr) / d;
                     } else {
-                        e = 0.5 * r;
-                        fz += sij * sij;
+                        e = 0.5 * rr;
+                        fz += aS[r] * aS[r];
                     }
                 }
                 T2[i * LDA + j] = e;
@@ -357,12 +345,15 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
             block_sum2<NT>(en, fz, red, tid);
             const double enorm = sqrt(en), frozen = sqrt(fz);
             // X <- X + X E
-            gemm64<false, false, NT / 64>(Z, T2, T1, w, lr, lg);
-            __syncthreads();
-            for (int idx = tid; idx < 4096; idx += NT) {
-                const int i = idx >> 6, j = idx & 63;
-                Z[i * LDA + j] += T1[i * LDA + j];
+            f64x4 aX = splat<f64x4>(0.0);
+#pragma unroll 4
+            for (int kk = 0; kk < 16; ++kk) {
+                const int i = 16 * rb + lr, k = 4 * kk + lg, j = 16 * cb + lr;
+                aX = MF<double>::mma(Z[i * LDA + k], T2[k * LDA + j], aX);
             }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Z[(16 * rb + MF<double>::crow(lg, r)) * LDA + 16 * cb + lr] += aX[r];
             __syncthreads();
             if (!(enorm < 0.5) || (it >= 3 && enorm > 0.5 * prev_en)) break;  // not converging: solve cold
             // Accept X + X E when X already diagonalised A (off-diagonal of
@@ -485,8 +476,8 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
         const int i = idx >> 6, k = idx & 63;
         const double z = Z[i * LDA + k];
         Zg[idx] = z;
-        U[idx] = isq[i] * z;        // U[i][k]
-        V[k * 64 + i] = z * sq[i];  // V[k][i]
+        U[idx] = isq[i] * z;                       // U[i][k]
+        V[idx] = Z[k * LDA + i] * sq[k];           // V[i][k] = Z[k][i] sqrt(pi_k)
     }
     if (tid == 0) a.status[m] = status;
 }
@@ -1672,23 +1663,6 @@ __global__ __launch_bounds__(256, 2) void k_pmat(PmatArgs<R> a, ProjArgs pj, int
 //              partials (fixed order), projected on M_phi = V dQ_phi U for
 //              phi = omega, theta_AC..theta_GT
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_asum(FrechetArgs a) {
-    const int bi = blockIdx.x, idx = blockIdx.y * 256 + threadIdx.x;
-    const double *src = a.Apart + (size_t)bi * a.n_chunks * 4096 + idx;
-    double s = 0.0;
-    int c = 0;
-    for (; c + 4 <= a.n_chunks; c += 4) {
-        const double x0 = src[(size_t)c * 4096], x1 = src[(size_t)(c + 1) * 4096];
-        const double x2 = src[(size_t)(c + 2) * 4096], x3 = src[(size_t)(c + 3) * 4096];
-        s += x0;
-        s += x1;
-        s += x2;
-        s += x3;
-    }
-    for (; c < a.n_chunks; ++c) s += src[(size_t)c * 4096];
-    a.Asum[(size_t)bi * 4096 + idx] = s;
-}
-
 // GROUPED: the workgroup walks several pairs of one matrix with U and V
 // resident in LDS (three 64 x 64 buffers, one workgroup per CU); otherwise it
 // transforms one pair with two buffers (V, then U, reloaded), two per CU.
@@ -1699,19 +1673,23 @@ __global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a)
     __shared__ double Us[GROUPED ? 64 * LDA : 1];    // U (GROUPED)
     __shared__ double As[64 * LDA];
     __shared__ double lam[64], ex[64];
-    __shared__ double red[8];
+    __shared__ double red[24];
     const int e0 = a.wl_ptr[m] + grp * a.grp_size, e1 = min(a.wl_ptr[m + 1], e0 + a.grp_size);
     f64x4 H[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) H[q] = splat<f64x4>(0.0);
-    // A_b of the next pair is prefetched into registers while the current
-    // pair is transformed
+    // A_b of the next pair (summed over its pattern chunks in chunk order)
+    // is fetched into registers while the current pair is transformed; with
+    // one chunk per branch the fetch only issues loads
     double ra[16];
     auto fetch_a = [&](int e) {
         const int bi = a.wl_idx[e] / a.max_comp;
-        const double *src = a.Asum + (size_t)bi * 4096 + tid;
+        const double *src = a.Apart + (size_t)bi * a.n_chunks * 4096 + tid;
 #pragma unroll
         for (int k = 0; k < 16; ++k) ra[k] = src[256 * k];
+        for (int c = 1; c < a.n_chunks; ++c)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) ra[k] += src[(size_t)c * 4096 + 256 * k];
     };
     if (e0 < e1) {
         fetch_a(e0);
@@ -1764,36 +1742,28 @@ __global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a)
             o[1] = wj * d1;
         }
     }
-    double *Hp = a.W + ((size_t)m * a.n_grp + grp) * 4096;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Hp[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] = H[q][r];
-}
-
-__global__ __launch_bounds__(256) void k_hproj(FrechetArgs a) {
-    const int m = blockIdx.x, ch = blockIdx.y, tid = threadIdx.x, idx = ch * 256 + tid;
-    __shared__ double red[4];
-    const double *Hp = a.W + (size_t)m * a.n_grp * 4096 + idx;
-    double h = 0.0;
-    int g = 0;
-    for (; g + 8 <= a.n_grp; g += 8) {  // eight loads in flight, then the sums in order
-        double x[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = Hp[(size_t)(g + u) * 4096];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) h += x[u];
-    }
-    for (; g < a.n_grp; ++g) h += Hp[(size_t)g * 4096];
-    const double *M = a.M + (size_t)m * 6 * 4096 + idx;
+    // <M_phi, H> of this group's partial H (the projection is linear, so
+    // k_final sums the groups' projections in group order)
+    const double *Mm = a.M + (size_t)m * 6 * 4096;
     double v[6];
 #pragma unroll
-    for (int phi = 0; phi < 6; ++phi) v[phi] = M[(size_t)phi * 4096] * h;
-#pragma unroll
     for (int phi = 0; phi < 6; ++phi) {
-        const double r = block_sum<256>(v[phi], red, tid);
-        if (tid == 0) a.proj[((size_t)m * 6 + phi) * HPROJ_CHUNKS + ch] = r;
+        v[phi] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                v[phi] += Mm[(size_t)phi * 4096 + (16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] * H[q][r];
     }
+#pragma unroll
+    for (int phi = 0; phi < 6; ++phi)
+        for (int off = 32; off > 0; off >>= 1) v[phi] += __shfl_down(v[phi], off);
+    __syncthreads();
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int phi = 0; phi < 6; ++phi) red[6 * w + phi] = v[phi];
+    __syncthreads();
+    if (tid < 6) a.proj[((size_t)m * 6 + tid) * a.n_grp + grp] = ((red[tid] + red[6 + tid]) + red[12 + tid]) + red[18 + tid];
 }
 
 // ---------------------------------------------------------------------------
@@ -1847,7 +1817,7 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
         a.n_real_nodes = g.n_real_nodes;
         a.comp_ptr += g.cls0;
         a.n_class = g.n_cls;
-        a.proj += (size_t)g.mat0 * 6 * HPROJ_CHUNKS;
+        a.proj += (size_t)g.mat0 * 6 * a.proj_stride;
         a.pl.n_mat = g.n_mat;
         a.pl.n_comp = g.n_comp;
         node0 = g.node0;
@@ -1932,8 +1902,8 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
             const int phi = is_om ? 0 : (t - NM + 1);
             // <M_{m,phi}, H_m> partials: one matrix for omega_t, all matrices for theta
             const int m0 = is_om ? t : 0, m1 = is_om ? t + 1 : NM;
-            for (int q = lane; q < (m1 - m0) * HPROJ_CHUNKS; q += 64)
-                v += a.proj[((size_t)(m0 + q / HPROJ_CHUNKS) * 6 + phi) * HPROJ_CHUNKS + q % HPROJ_CHUNKS];
+            const int PS = a.proj_stride;
+            for (int q = lane; q < (m1 - m0) * PS; q += 64) v += a.proj[((size_t)(m0 + q / PS) * 6 + phi) * PS + q % PS];
             v = wave_sum(v);
             if (lane == 0) {
                 if (a.convention == 1) {
@@ -2036,12 +2006,10 @@ template <typename R> void launch_pmat_host(const PmatArgs<R> &a, const double *
 
 
 void launch_frechet(const FrechetArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_asum, dim3(a.n_branch, 16), dim3(256), 0, s, a);
     if (a.grp_size > 1)
         hipLaunchKernelGGL(k_frechet<true>, dim3(a.n_mat, a.n_grp), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(k_frechet<false>, dim3(a.n_mat, a.n_grp), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_hproj, dim3(a.n_mat, HPROJ_CHUNKS), dim3(256), 0, s, a);
 }
 
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s) {

@@ -67,3 +67,30 @@ if TORCH_AVAILABLE:
 
     def mixture_log_likelihood(branch_lengths, omegas, weights, theta, tree, aln, alpha=1.0, time_convention=0):
         return MixtureLogLikelihood.apply(branch_lengths, omegas, weights, theta, tree, aln, alpha, time_convention)
+
+    class EngineLogLikelihood(torch.autograd.Function):
+        """ln L(bl, omegas, weights, theta) of a branch-site omega mixture on a
+        resident ``hyphy3.engine.Engine`` (problem, model and mixture already
+        set).  Forward runs one MI355X evaluation (inside pass, outside pass,
+        adjoint and spectral Frechet projections); backward scales the
+        gradients it returned.  ``branch_lengths`` holds one entry per node
+        (the root's is ignored)."""
+
+        @staticmethod
+        def forward(ctx, branch_lengths, omegas, weights, theta, engine, precision):
+            def host(t):
+                return t.detach().to("cpu", torch.float64).numpy()
+
+            r = engine.eval(host(branch_lengths), host(omegas), host(weights), host(theta), precision=precision)
+            dev, dt = branch_lengths.device, branch_lengths.dtype
+            ctx.save_for_backward(*(torch.as_tensor(r[k], dtype=dt, device=dev)
+                                    for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta")))
+            return torch.tensor(r["log_likelihood"], dtype=dt, device=dev)
+
+        @staticmethod
+        def backward(ctx, g):
+            gb, go, gw, gt = ctx.saved_tensors
+            return g * gb, g * go, g * gw, g * gt, None, None
+
+    def engine_log_likelihood(branch_lengths, omegas, weights, theta, engine, precision):
+        return EngineLogLikelihood.apply(branch_lengths, omegas, weights, theta, engine, precision)

@@ -216,7 +216,9 @@ int hy_last_parts(hy_engine h, int *n_parts);
 int hy_set_warm_start(hy_engine h, int enable);
 
 /* Eigensolver status of the last hy_eval per rate matrix: Jacobi sweeps for a
- * cold solve, 100 + k after k refinement steps of a warm start. */
+ * cold solve, 100 + k after k refinement steps of a warm start (100: the
+ * previous eigenvectors already diagonalised the matrix to 1e-12 and were
+ * kept unchanged). */
 int hy_last_eigen_status(hy_engine h, int *status, int n);
 
 /* Timing of the last hy_eval's device work, in milliseconds, per stage:
