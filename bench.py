@@ -180,6 +180,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the cost-model tile choice instead of timing the candidates first")
+    ap.add_argument("--fit", action="store_true",
+                    help="instead of the throughput run: torch L-BFGS maximum-likelihood fit of branch lengths, "
+                         "omegas and GTR rates from a perturbed start (FP32 phase, then FP64)")
+    ap.add_argument("--cpu-fit", action="store_true", help="with --fit: also the reference fit on the host (slow)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -196,6 +202,9 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.fit:
+        bench_fit(args, cfg)
+        return
     sim, aln, tree, arrays, tau = build_problem(cfg)
     print(f"[bench] {args.config}: taxa={cfg['taxa']} codons={cfg['codons']} patterns={len(arrays['weights'])} "
           f"nodes={len(arrays['parent'])} K={K}", file=sys.stderr, flush=True)
@@ -203,6 +212,17 @@ def main():
     # timed engine: no per-stage events
     os.environ.pop("HY_PROFILE", None)
     eng = make_engine(arrays, K, args.tile)
+    tuning = {}
+    if not args.no_autotune and not args.tile:
+        # measured tile/parts choice per precision, before (and outside) the timed steps
+        t0 = time.perf_counter()
+        for prec, name in ((1, "fp32"), (0, "fp64")):
+            if prec == 0 and args.no_fp64:
+                continue
+            tile, parts, ms = eng.autotune(*eval_args, precision=prec, reps=5)
+            tuning[name] = {"tile": tile, "parts": parts, "ms": ms}
+        tuning["seconds"] = time.perf_counter() - t0
+        print(f"[bench] autotune {tuning}", file=sys.stderr, flush=True)
     ms32, out32, _ = run_steps(eng, eval_args, 1, args.steps, args.warmup, False)
     tile32 = eng.last_tile()
     parts32 = eng.last_parts()
@@ -227,6 +247,9 @@ def main():
     # profiled engine: HIP events around every stage on the engine's stream
     os.environ["HY_PROFILE"] = "1"
     peng = make_engine(arrays, K, args.tile)
+    for prec, name in ((1, "fp32"), (0, "fp64")):
+        if name in tuning:
+            peng.set_tuning(prec, tuning[name]["tile"], tuning[name]["parts"])
     _, _, st32 = run_steps(peng, eval_args, 1, args.steps, args.warmup, True)
     stage_ms = {s: float(st32[i]) for i, s in enumerate(STAGES)}
     if not args.no_fp64:
@@ -271,12 +294,58 @@ def main():
                       "a second engine with HIP events around each stage",
     }
     line.update(res)
+    line["autotune"] = tuning or None
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(cfg, sim, args.cpu_seconds)
             line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
         except Exception as ex:  # the oracle library may not be built on a bare box
             line["cpu_baseline"] = {"value": None, "error": str(ex)}
+    print(json.dumps(line), flush=True)
+
+
+def bench_fit(args, cfg):
+    """Maximum-likelihood fit through hyphy3.fit (torch L-BFGS over the
+    engine-backed autograd function): start from the generating values
+    perturbed (branch lengths x1.3, omegas x1.5, theta 1), report the fitted
+    lnL, iterations and time; with --cpu-fit, the reference fit (scipy
+    L-BFGS-B over the FP64 oracle on the host cores) from the same start."""
+    import hyphy3
+    from hyphy3 import simulate
+    from hyphy3.fit import MixtureFit, make_engine as fit_engine
+    sim = simulate.simulate(cfg["taxa"], cfg["codons"], cfg["seed"], omegas=cfg["omegas"], weights=cfg["weights"],
+                            gap_fraction=cfg["gaps"])
+    aln = hyphy3.Alignment.from_text(sim["fasta"])
+    tree = hyphy3.Tree.from_newick(sim["newick"], keep_rooted=False)
+    K = len(cfg["omegas"])
+    eng, arrays = fit_engine(tree, aln, K)
+    bl0 = np.array([0.0 if n.id == tree.root_id else n.branch_length * 1.3 for n in tree.nodes])
+    om0 = [o * 1.5 for o in cfg["omegas"]]
+    th0 = np.ones(5)
+    fit = MixtureFit(eng, arrays["parent"], bl0, om0, cfg["weights"], th0)
+    r = fit.run(max_iter=1000, tol_grad=1e-6)
+    line = {"workload": f"{args.config} fit: {cfg['taxa']} taxa x {cfg['codons']} codons x {K} omega categories, "
+                        "branch lengths + omegas + GTR by torch L-BFGS (FP32 phase, then FP64)",
+            "log_likelihood": r["log_likelihood"], "iterations_fp32": r["iterations_fp32"],
+            "iterations_fp64": r["iterations_fp64"], "evaluations": r["evaluations"], "seconds": r["seconds"],
+            "max_abs_log_gradient": r["max_abs_log_gradient"], "omegas": list(r["omegas"]), "theta": list(r["theta"])}
+    eng.close()
+    if args.cpu_fit:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle import data_ref as dr
+        from helpers import oracle_fit
+        names, seqs = dr.parse_fasta(sim["fasta"])
+        raln = dr.AlignmentRef(names, seqs)
+        rtree = dr.TreeRef.from_newick(sim["newick"], keep_rooted=False)
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+        t0 = time.perf_counter()
+        ll_ref, _, om_ref, th_ref = oracle_fit(rtree, raln, om0, cfg["weights"], th0, bl0, threads=threads)
+        line["cpu_reference_fit"] = {"log_likelihood": ll_ref, "seconds": time.perf_counter() - t0, "cores": threads,
+                                     "omegas": list(om_ref), "theta": list(th_ref),
+                                     "kind": "scipy L-BFGS-B over the FP64 restatement of the reference algorithm"}
+        line["abs_diff_log_likelihood"] = abs(r["log_likelihood"] - ll_ref)
     print(json.dumps(line), flush=True)
 
 

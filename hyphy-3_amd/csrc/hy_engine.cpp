@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -118,6 +119,9 @@ struct hy_engine_s {
     int part_ri = -1;
     bool part_outside = false;
     int T_req = 0, T = 0, n_tiles = 0, P_pad = 0;
+    // hy_autotune's measured choice per precision ([0] FP32, [1] FP64): tile
+    // width (0: cost model) and subtree parts (-1: rule, 0 off, 1 on)
+    int tuned_T[2] = {0, 0}, tuned_parts[2] = {-1, -1};
 
     // mixture
     int n_mat = 1, n_class = 1, n_comp = 1, convention = HY_TIME_RAW, max_comp = 1;
@@ -316,8 +320,8 @@ void set_frechet_groups(hy_engine h, const Lists &per) {
 }
 
 int prepare_buffers(hy_engine h, int prec) {
-    const int T = choose_tile(h->P, h->T_req, prec, h->n_cu);
     const int ri = prec == HY_PREC_FP64 ? 1 : 0;
+    const int T = choose_tile(h->P, h->T_req ? h->T_req : h->tuned_T[ri], prec, h->n_cu);
     const size_t rs = prec == HY_PREC_FP64 ? 8 : 4;
     if (T != h->T) {
         ++h->epoch;
@@ -342,7 +346,8 @@ int prepare_buffers(hy_engine h, int prec) {
     // into subtree parts walked by separate workgroups (a shorter dependent
     // chain per workgroup, more workgroups in flight); FP32 T = 16 then runs
     // the lean inside variant (four workgroups per CU).
-    const bool split = !h->no_parts && h->n_tiles < 2 * h->n_cu;
+    bool split = !h->no_parts && h->n_tiles < 2 * h->n_cu;
+    if (!h->T_req && h->tuned_parts[ri] >= 0) split = !h->no_parts && h->tuned_parts[ri] == 1;
     h->lean = lean_inside(prec == HY_PREC_FP32, T, h->n_tiles, h->n_cu) || (split && prec == HY_PREC_FP32 && T == 16);
     if (h->sched_key != T * 16 + (int)rs + (h->lean ? 4096 : 0) + (split ? 8192 : 0)) {
         ++h->epoch;
@@ -1472,6 +1477,75 @@ int hy_eval(hy_engine h, int precision, unsigned flags, const double *bl, double
     TRY(eval_submit(h, precision, flags, bl, alpha, omegas, comp_w, theta, delta, psi, out_site_ll != nullptr));
     return eval_complete(h, (flags & HY_WANT_GRAD) != 0, out_logL, out_site_ll, out_grad_bl, out_grad_omega,
                          out_grad_w, out_grad_theta);
+}
+
+int hy_autotune(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
+                const double *comp_w, const double *theta, double delta, double psi, int reps, int *out_tile,
+                int *out_parts, double *out_ms) {
+    if (!h) return HY_ERR_ARG;
+    TRY(check_eval_args(h, precision, bl, omegas, comp_w, theta, "hy_autotune"));
+    if (reps < 1 || reps > 1000) return fail(h, HY_ERR_ARG, "hy_autotune: reps must be in 1..1000 (got %d)", reps);
+    const int ri = precision == HY_PREC_FP64 ? 1 : 0;
+    const unsigned fl = (flags & HY_WANT_GRAD) | HY_WANT_LOGL;
+    const int saved_req = h->T_req;
+    h->T_req = 0;
+    int best_T = 0, best_parts = -1;
+    double best = 1e300;
+    if (h->P > 0) {
+        for (int T = 16; T <= 64; T *= 2)
+            for (int parts = 0; parts <= 1; ++parts) {
+                if (parts && h->no_parts) continue;
+                h->tuned_T[ri] = T;
+                h->tuned_parts[ri] = parts;
+                ++h->epoch;
+                // one untimed evaluation (re-tiling, schedules, graph capture)
+                int rc = eval_submit(h, precision, fl, bl, alpha, omegas, comp_w, theta, delta, psi, false);
+                if (rc == HY_OK) rc = eval_complete(h, false, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                std::vector<double> t;
+                for (int r = 0; rc == HY_OK && r < reps; ++r) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    rc = eval_submit(h, precision, fl, bl, alpha, omegas, comp_w, theta, delta, psi, false);
+                    if (rc == HY_OK) rc = eval_complete(h, false, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                    t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+                }
+                if (rc != HY_OK) {
+                    h->T_req = saved_req;
+                    h->tuned_T[ri] = 0;
+                    h->tuned_parts[ri] = -1;
+                    ++h->epoch;
+                    return rc;
+                }
+                std::sort(t.begin(), t.end());
+                const double med = t[t.size() / 2];
+                if (med < best) {
+                    best = med;
+                    best_T = T;
+                    best_parts = h->n_parts > 0 ? 1 : 0;
+                }
+            }
+    }
+    h->tuned_T[ri] = best_T;
+    h->tuned_parts[ri] = best_parts;
+    h->T_req = saved_req;
+    ++h->epoch;
+    if (out_tile) *out_tile = best_T;
+    if (out_parts) *out_parts = best_parts;
+    if (out_ms) *out_ms = h->P > 0 ? best : 0.0;
+    return HY_OK;
+}
+
+int hy_set_tuning(hy_engine h, int precision, int patterns_per_tile, int parts) {
+    if (!h) return HY_ERR_ARG;
+    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
+        return fail(h, HY_ERR_ARG, "hy_set_tuning: unknown precision %d", precision);
+    if (patterns_per_tile != 0 && patterns_per_tile != 16 && patterns_per_tile != 32 && patterns_per_tile != 64)
+        return fail(h, HY_ERR_ARG, "hy_set_tuning: tile must be 0, 16, 32 or 64 (got %d)", patterns_per_tile);
+    if (parts < -1 || parts > 1) return fail(h, HY_ERR_ARG, "hy_set_tuning: parts must be -1, 0 or 1 (got %d)", parts);
+    const int ri = precision == HY_PREC_FP64 ? 1 : 0;
+    h->tuned_T[ri] = patterns_per_tile;
+    h->tuned_parts[ri] = parts;
+    ++h->epoch;
+    return HY_OK;
 }
 
 int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,

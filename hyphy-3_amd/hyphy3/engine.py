@@ -27,7 +27,7 @@ HY_WANT_GRAD = 4
 EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
-           "hy_inside_outside_from_P", "hy_read_partials")
+           "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -83,6 +83,9 @@ def load_library(path: str = LIB_PATH):
                               _f64p]
     L.hy_eval_batch.argtypes = [C.c_int, C.POINTER(BatchItem), C.c_int, C.c_uint]
     L.hy_create_group.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.hy_autotune.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
+                              C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _f64p]
+    L.hy_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     _lib = L
     return L
 
@@ -274,6 +277,26 @@ class Engine:
         self._check(self.L.hy_loglik_from_P(self.h, precision, _p(P, _f64p), _p(pi, _f64p), C.byref(ll),
                                             _p(s, _f64p)))
         return (ll.value, s[:self.num_patterns]) if site else ll.value
+
+    def autotune(self, bl, omegas, comp_w, theta, alpha=1.0, delta=0.0, psi=0.0, precision=HY_PREC_FP64, grad=True,
+                 reps=5):
+        """Times every tile width with and without subtree parts at these
+        parameters and keeps the fastest for this precision.  Returns
+        (tile, parts, ms per evaluation)."""
+        bl = _f64(bl)
+        om = _f64(np.atleast_1d(omegas))
+        cw = _f64(np.atleast_1d(comp_w))
+        th = _f64(theta)
+        tile, parts, ms = C.c_int(), C.c_int(), C.c_double()
+        flags = HY_WANT_LOGL | (HY_WANT_GRAD if grad else 0)
+        self._check(self.L.hy_autotune(self.h, precision, flags, _p(bl, _f64p), alpha, _p(om, _f64p), _p(cw, _f64p),
+                                       _p(th, _f64p), delta, psi, int(reps), C.byref(tile), C.byref(parts),
+                                       C.byref(ms)))
+        return tile.value, parts.value, ms.value
+
+    def set_tuning(self, precision, tile, parts):
+        """Applies an autotune result (tile 0/16/32/64, parts -1/0/1)."""
+        self._check(self.L.hy_set_tuning(self.h, precision, int(tile), int(parts)))
 
     def set_warm_start(self, enable):
         self._check(self.L.hy_set_warm_start(self.h, 1 if enable else 0))

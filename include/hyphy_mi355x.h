@@ -202,6 +202,25 @@ int hy_set_tile(hy_engine h, int patterns_per_tile);
  * reference (tuning introspection). */
 int hy_last_tile(hy_engine h, int *patterns_per_tile);
 
+/* Autotuner: times `reps` evaluations (flags as hy_eval, at the given
+ * parameters, after one untimed evaluation) for every pattern-tile width
+ * (16, 32, 64) with and without subtree parts, and keeps the fastest
+ * configuration for this precision: later evaluations use it unless
+ * hy_set_tile requested a tile.  Outputs (each may be NULL): the chosen tile,
+ * whether subtree parts are used (0/1) and its median time per evaluation in
+ * milliseconds (host clock around each synchronous evaluation).  No
+ * counterpart in the reference, which has a fixed tile rule
+ * (busted.hpp:385-401). */
+int hy_autotune(hy_engine h, int precision, unsigned flags, const double *bl, double alpha, const double *omegas,
+                const double *comp_w, const double *theta, double delta, double psi, int reps, int *out_tile,
+                int *out_parts, double *out_ms);
+
+/* Applies a tuning choice (as hy_autotune returns it) for one precision: tile
+ * 16, 32 or 64 (0: the cost model) and subtree parts 1/0 (-1: the rule), e.g.
+ * one measured in an earlier run.  hy_set_tile still takes precedence.  No
+ * counterpart in the reference. */
+int hy_set_tuning(hy_engine h, int precision, int patterns_per_tile, int parts);
+
 /* Subtree parts the last evaluation split each pruning pass into (0: one
  * schedule for the whole tree; n > 0: n subtree parts plus the top part, two
  * launches per pass).  No counterpart in the reference (tuning introspection). */

@@ -1,0 +1,43 @@
+"""Measured tile/parts autotuning (hy_autotune, hy_set_tuning): the chosen
+configuration is the one later evaluations use, results stay within the
+oracle tolerances under it, and hy_set_tile still takes precedence."""
+import numpy as np
+import pytest
+
+from helpers import branch_lengths, grad_err, make_engine, rel, sim_problem
+from oracle import hyoracle as ho
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_choice_is_used_and_exact():
+    from hyphy3.engine import HY_PREC_FP32, HY_PREC_FP64
+    omegas, weights = [0.2, 1.4], [0.75, 0.25]
+    tree, aln, sim = sim_problem(24, 900, 61, omegas=omegas, weights=weights)
+    N = len(tree.nodes)
+    bl = branch_lengths(tree)
+    mix = ho.Mixture(omegas, np.zeros(N, dtype=np.int32), [[(0, weights[0]), (1, weights[1])]], 1)
+    ref = ho.mixture_eval(ho.Problem.from_refs(tree, aln), ho.Code(0), ho.params(1.0, 1.0, sim["theta"]),
+                          aln.pos_freqs, aln.cf3x4, mix, bl)
+    eng = make_engine(tree, aln)
+    eng.set_mixture(2, np.zeros(N, dtype=np.int32), [[0, 1]], convention=1)
+    for prec, tl, tg in ((HY_PREC_FP32, 1e-5, 1e-4), (HY_PREC_FP64, 1e-9, 1e-7)):
+        tile, parts, ms = eng.autotune(bl, omegas, weights, sim["theta"], precision=prec, reps=2)
+        print("precision", prec, "tile", tile, "parts", parts, "ms %.3f" % ms)
+        assert tile in (16, 32, 64) and parts in (0, 1) and ms > 0.0
+        r = eng.eval(bl, omegas, weights, sim["theta"], precision=prec)
+        assert eng.last_tile() == tile
+        assert (eng.last_parts() > 0) == (parts == 1)
+        assert rel(r["log_likelihood"], ref["log_likelihood"]) < tl
+        for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
+            assert grad_err(r[k], ref[k]) < tg, (prec, k)
+    # an explicit tile overrides the tuning; 0 returns to it
+    eng.set_tile(64)
+    eng.eval(bl, omegas, weights, sim["theta"], precision=HY_PREC_FP32)
+    assert eng.last_tile() == 64
+    eng.set_tile(0)
+    eng.set_tuning(HY_PREC_FP32, 32, 0)
+    r = eng.eval(bl, omegas, weights, sim["theta"], precision=HY_PREC_FP32)
+    assert eng.last_tile() == 32 and eng.last_parts() == 0
+    assert rel(r["log_likelihood"], ref["log_likelihood"]) < 1e-5
+    eng.close()
