@@ -665,7 +665,8 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
             ma.n_params = h->pl.total;
         }
         ma.mat_gene = (const int32_t *)h->d_mat_gene.p;
-        ma.tol = sizeof(R) == 4 ? 1e-8 : 1e-14;
+        // accepted predicted eigenvector correction per precision
+        ma.tol = sizeof(R) == 4 ? 1e-10 : 1e-14;
         launch_model(ma, h->n_mat, s);
         // M_phi = V dQ_phi U and d mu / d phi depend on the spectra only: they
         // are built by the workgroups of the P-build launch beyond the branches

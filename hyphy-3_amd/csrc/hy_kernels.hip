@@ -362,7 +362,9 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
             // correction, predicted as |E|^3 / |E_prev|^2, is below tol)
             // and no cluster hides an off-diagonal entry above tol ||A||.
             // tol = 1e-14 for FP64 evaluations, 1e-10 for FP32 ones (whose
-            // transition matrices are rounded to FP32).
+            // transition matrices are rounded to FP32); the prediction is
+            // trusted for corrections below 1e-5 (from a correction of 1e-4
+            // FP32 gradients came out 6e-4 off their FP64 values at 64 taxa).
             const bool quad = it >= 1 && enorm < 1e-5 && enorm * enorm * enorm <= a.tol * prev_en * prev_en &&
                               frozen <= a.tol * anorm;
             prev_en = enorm;
@@ -527,16 +529,31 @@ __device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, double *
     f64x4 Pacc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) Pacc[q] = splat<f64x4>(0.0);
-    for (int j = j0; j < j1; ++j) {
+    // U, V and lambda of the next component are requested before the
+    // current component's product, so only the first component waits on them
+    double ru[16], rv[16], rl = 0.0;
+    auto fetch = [&](int j) {
         const int m = a.comp_mat[j];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            ru[k] = a.U[(size_t)m * 4096 + tid + 256 * k];
+            rv[k] = a.V[(size_t)m * 4096 + tid + 256 * k];
+        }
+        if (tid < 64) rl = a.lam[m * 64 + tid];
+    };
+    if (j0 < j1) fetch(j0);
+    for (int j = j0; j < j1; ++j) {
         const double wj = prm[a.pl.off_w + j];
         __syncthreads();  // the previous component's operand reads are done
-        if (tid < 64) ex[tid] = exp(a.lam[m * 64 + tid] * tau);
+        if (tid < 64) ex[tid] = exp(rl * tau);
         __syncthreads();
-        for (int idx = tid; idx < 4096; idx += 256) {
-            Us[(idx >> 6) * LDA + (idx & 63)] = a.U[m * 4096 + idx] * ex[idx & 63];
-            Vs[(idx >> 6) * LDA + (idx & 63)] = a.V[m * 4096 + idx];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int idx = tid + 256 * k;
+            Us[(idx >> 6) * LDA + (idx & 63)] = ru[k] * ex[idx & 63];
+            Vs[(idx >> 6) * LDA + (idx & 63)] = rv[k];
         }
+        if (j + 1 < j1) fetch(j + 1);
         __syncthreads();
         f64x4 acc[4];
         gemm64_acc<false, false>(Us, Vs, acc, w, lr, lg);
