@@ -92,6 +92,7 @@ struct hy_engine_s {
     const void *ident_tiles[2] = {nullptr, nullptr};  // PA buffer whose identity tiles are current
     DevBuf d_osched, d_Uspill[2];
     int n_uspill = 0;
+    bool lean_out = false;  // lean outside variant (FP32, T = 16, subtree parts)
     // Subtree parts of the pruning passes (single problems with few tiles):
     // per part {inside offset, count, outside offset, count} in schedule
     // entries; the last row is the top part, which holds the root.  n_parts
@@ -349,7 +350,8 @@ int prepare_buffers(hy_engine h, int prec) {
     bool split = !h->no_parts && h->n_tiles < 2 * h->n_cu;
     if (!h->T_req && h->tuned_parts[ri] >= 0) split = !h->no_parts && h->tuned_parts[ri] == 1;
     h->lean = lean_inside(prec == HY_PREC_FP32, T, h->n_tiles, h->n_cu) || (split && prec == HY_PREC_FP32 && T == 16);
-    if (h->sched_key != T * 16 + (int)rs + (h->lean ? 4096 : 0) + (split ? 8192 : 0)) {
+    h->lean_out = split && prec == HY_PREC_FP32 && T == 16;
+    if (h->sched_key != T * 16 + (int)rs + (h->lean ? 4096 : 0) + (split ? 8192 : 0) + (h->lean_out ? 16384 : 0)) {
         ++h->epoch;
         build_schedules(h, T, (int)rs, split);
         TRY(upload(h, h->d_sched, h->sched.data(), h->sched.size() * 4));
@@ -446,7 +448,7 @@ int prepare_buffers(hy_engine h, int prec) {
 // and exchange their results with the top part through HBM (inside: the
 // part root's partials; outside: the message into the part root, spilled).
 void build_schedules(hy_engine h, int T, int esz, bool split) {
-    const int NSI = inside_slots(T, esz, h->lean), NSO = outside_slots(T, esz);
+    const int NSI = inside_slots(T, esz, h->lean), NSO = outside_slots(T, esz, h->lean_out);
     const auto &ch = h->ext_children;
     const int n_ext = (int)ch.size();
     auto ref_of = [&](int c) -> int32_t {
@@ -598,7 +600,7 @@ void build_schedules(hy_engine h, int T, int esz, bool split) {
     }
     h->n_parts = np;
     h->n_uspill = spills;
-    h->sched_key = T * 16 + esz + (h->lean ? 4096 : 0) + (split ? 8192 : 0);
+    h->sched_key = T * 16 + esz + (h->lean ? 4096 : 0) + (split ? 8192 : 0) + (h->lean_out ? 16384 : 0);
 }
 
 void refresh_blist_classes(hy_engine h) {
@@ -619,12 +621,12 @@ void launch_outside_parts(hy_engine h, PruneArgs<R> pr, bool rev, hipStream_t s)
     if (h->n_parts > 0) {
         pr.parts = (const int4 *)h->d_parts.p;
         pr.part0 = h->n_parts;
-        launch_outside<R>(pr, h->T, rev, 1, s);
+        launch_outside<R>(pr, h->T, rev, h->lean_out, 1, s);
         pr.part0 = 0;
-        launch_outside<R>(pr, h->T, rev, h->n_parts, s);
+        launch_outside<R>(pr, h->T, rev, h->lean_out, h->n_parts, s);
     } else {
         pr.parts = nullptr;
-        launch_outside<R>(pr, h->T, rev, 1, s);
+        launch_outside<R>(pr, h->T, rev, h->lean_out, 1, s);
     }
 }
 
@@ -1730,10 +1732,12 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
         own_osched.swap(h->osched);
         const int own_uspill = h->n_uspill, own_key = h->sched_key, own_nparts = h->n_parts;
         std::vector<int32_t> own_parts(h->parts);
-        const bool own_lean = h->lean;
+        const bool own_lean = h->lean, own_lean_out = h->lean_out;
         h->lean = lean;
+        h->lean_out = false;  // batch schedules: whole trees, the default outside variant
         build_schedules(h, T, esz, false);
         h->lean = own_lean;
+        h->lean_out = own_lean_out;
         std::vector<int32_t> g_sched, g_osched;
         g_sched.swap(h->sched);
         g_osched.swap(h->osched);

@@ -51,7 +51,13 @@ constexpr int inside_slots(int T, int esz, bool lean = false) { return lean ? 6 
 constexpr bool lean_inside(int prec_is_fp32, int T, long n_tiles, int n_cu) {
     return prec_is_fp32 && T == 16 && n_tiles > 2L * n_cu;
 }
-constexpr int outside_slots(int T, int esz) { return T == 16 ? 10 : (T == 32 ? 4 : (esz == 8 ? 1 : 2)); }
+// "Lean" outside pass (FP32, T = 16, subtree parts): four LDS slots for
+// pending messages, tip codes read from HBM and the next node's operands
+// requested after the current node's products (no register double
+// buffering), so that four workgroups share a CU.
+constexpr int outside_slots(int T, int esz, bool lean = false) {
+    return lean ? 4 : (T == 16 ? 10 : (T == 32 ? 4 : (esz == 8 ? 1 : 2)));
+}
 // Tip codes of a tile kept in LDS when (taxa + 1) * T fits.
 constexpr int tip_lds_codes(int T, int esz) {
     return T == 16 ? 12288 : (esz == 4 ? 24576 : (T == 32 ? 16384 : 8192));
@@ -239,7 +245,7 @@ template <typename R> void launch_pmat_host(const PmatArgs<R> &a, const double *
 // ny: workgroups along y (subtree parts); the grid is n_tiles x ny
 template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, int ny, hipStream_t s);
 // rev: every edge matrix is reversible with respect to a.pi (built from the model)
-template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, int ny, hipStream_t s);
+template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, bool lean, int ny, hipStream_t s);
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s);
 void launch_frechet(const FrechetArgs &a, hipStream_t s);  // chunk sums, spectral transforms, projections
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s);  // one workgroup per gene

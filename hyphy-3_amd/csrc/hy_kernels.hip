@@ -1094,12 +1094,12 @@ __global__ __launch_bounds__(256, LEAN ? 4 : ((sizeof(R) == 4 && T == 16) ? 2 : 
 // to a second register set.  Otherwise (caller-supplied matrices) U_c is
 // formed from the transposed tiles.
 // ---------------------------------------------------------------------------
-template <typename R, int T, bool TL, bool REV>
-__global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
+template <typename R, int T, bool TL, bool REV, bool LEAN = false>
+__global__ __launch_bounds__(256, LEAN ? 3 : 1) void k_outside(PruneArgs<R> args) {
     PruneArgs<R> a = args;
     const int tile = gene_view(a, blockIdx.x);
-    constexpr int NB = T / 16, NS = outside_slots(T, sizeof(R)), TIPC = TL ? tip_lds_codes(T, sizeof(R)) : 1;
-    constexpr bool DREG = NB * sizeof(R) <= 8;  // f32 T <= 32, f64 T == 16
+    constexpr int NB = T / 16, NS = outside_slots(T, sizeof(R), LEAN), TIPC = TL ? tip_lds_codes(T, sizeof(R)) : 1;
+    constexpr bool DREG = !LEAN && NB * sizeof(R) <= 8;  // f32 T <= 32, f64 T == 16
     constexpr int DV = DREG ? 16 * NB : 1;
     typedef Entry<OSCHED_STRIDE> Ent;
     typedef typename MF<R>::acc acc_t;
@@ -1113,13 +1113,15 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
     stage_tip_codes<T, TL>(tcodes, a.tips, a.tip_rows, a.P_pad, p0, tid);
     const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
     __syncthreads();
-    // Register-prefetched operands cover the first RS children; the host
-    // orders internal children first and tips last, so a third internal child
-    // (only at nodes with three internal children) is loaded when used.
-    constexpr int RS = 2;
+    // Register-prefetched operands cover the first RS children (one in the
+    // lean variant); the host orders internal children first and tips last,
+    // so further internal children are loaded when used.
+    constexpr int RS = LEAN ? 1 : 2;
     // pa: P tiles of the current node's children; pt: their transposes
-    // (!REV) or the next node's P tiles (REV)
-    R pa[RS][16], pt[RS][16], dv[RS][DV];
+    // (!REV) or the next node's P tiles (REV; not in the lean variant, which
+    // requests the next node's tiles into pa once the current ones are used)
+    constexpr int PT = (REV && LEAN) ? 1 : RS;
+    R pa[RS][16], pt[PT][16], dv[RS][DV];
     int de[MAXC][NB];
     acc_t xt[MAXC][NB];
     unsigned ab = 0;  // partially ambiguous tip lanes of the prefetched operands
@@ -1151,7 +1153,7 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
             if (c < nc) {
                 if (kind != KIND_TIP) {
                     if (c < RS) {
-                        fetch_edge<R>(REV ? pt[c] : pa[c], a.PA, pn, w, l);
+                        fetch_edge<R>((REV && !LEAN) ? pt[c < PT ? c : 0] : pa[c], a.PA, pn, w, l);
                         if constexpr (DREG)
                             fetch_b<R, T>(dv[c], a.D + ((size_t)idx * a.n_tiles + tile) * (64 * T), lr, lg);
                     }
@@ -1173,11 +1175,11 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
     }
     int cb = 0;
     for (int k = 0; k < n; ++k) {
-        if constexpr (REV)
+        if constexpr (REV && !LEAN)
 #pragma unroll
             for (int c = 0; c < RS; ++c)
 #pragma unroll
-                for (int i = 0; i < 16; ++i) pa[c][i] = pt[c][i];
+                for (int i = 0; i < 16; ++i) pa[c][i] = pt[c < PT ? c : 0][i];
         const Ent enn = load_entry_v<OSCHED_STRIDE>(a.osched, k + 2 < n ? k + 2 : n - 1);
         const int nc = ec.v[2], usrc = ec.v[5];
         bool any_int = false;
@@ -1294,8 +1296,9 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
         if constexpr (!REV)
 #pragma unroll
             for (int c = 0; c < RS; ++c)
-                if (c < nc && (ec.v[7 + 4 * c] >> 28) == KIND_INT) fetch_edge<R>(pt[c], a.PTA, ec.v[6 + 4 * c], w, l);
-        if (k + 1 < n) operands(en);
+                if (c < nc && (ec.v[7 + 4 * c] >> 28) == KIND_INT) fetch_edge<R>(pt[c < PT ? c : 0], a.PTA, ec.v[6 + 4 * c], w, l);
+        if constexpr (!LEAN)
+            if (k + 1 < n) operands(en);
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) {
@@ -1343,7 +1346,7 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
                 if (kind != KIND_TIP) {
                     acc_t Uc[NB];
                     if (c < RS) {
-                        mfma_tile<R, T>(REV ? pa[c < RS ? c : 0] : pt[c < RS ? c : 0], Xc + c * 64 * T, Uc, lr, lg);
+                        mfma_tile<R, T>(REV ? pa[c < RS ? c : 0] : pt[c < PT ? c : 0], Xc + c * 64 * T, Uc, lr, lg);
                     } else {
                         R av[16];
                         fetch_edge<R>(av, REV ? a.PA : a.PTA, ec.v[6 + 4 * c], w, l);
@@ -1372,6 +1375,10 @@ __global__ __launch_bounds__(256) void k_outside(PruneArgs<R> args) {
                 }
             }
         }
+        // lean: the next node's operands once this node's U products have
+        // consumed the P tiles in pa
+        if constexpr (LEAN)
+            if (k + 1 < n) operands(en);
         ec = en;
         en = enu;
     }
@@ -1995,7 +2002,16 @@ template <typename R, int T, bool REV> void launch_outside_t(const PruneArgs<R> 
         hipLaunchKernelGGL((k_outside<R, T, false, REV>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
 }
 
-template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, int ny, hipStream_t s) {
+template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, bool lean, int ny, hipStream_t s) {
+    if constexpr (sizeof(R) == 4) {
+        if (lean && T == 16) {
+            if (rev)
+                hipLaunchKernelGGL((k_outside<R, 16, false, true, true>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_outside<R, 16, false, false, true>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
+            return;
+        }
+    }
     switch (T) {
         case 16: rev ? launch_outside_t<R, 16, true>(a, ny, s) : launch_outside_t<R, 16, false>(a, ny, s); break;
         case 32: rev ? launch_outside_t<R, 32, true>(a, ny, s) : launch_outside_t<R, 32, false>(a, ny, s); break;
@@ -2039,8 +2055,8 @@ template void launch_pmat_host<float>(const PmatArgs<float> &, const double *, h
 template void launch_pmat_host<double>(const PmatArgs<double> &, const double *, hipStream_t);
 template void launch_inside<float>(const PruneArgs<float> &, int, bool, int, hipStream_t);
 template void launch_inside<double>(const PruneArgs<double> &, int, bool, int, hipStream_t);
-template void launch_outside<float>(const PruneArgs<float> &, int, bool, int, hipStream_t);
-template void launch_outside<double>(const PruneArgs<double> &, int, bool, int, hipStream_t);
+template void launch_outside<float>(const PruneArgs<float> &, int, bool, bool, int, hipStream_t);
+template void launch_outside<double>(const PruneArgs<double> &, int, bool, bool, int, hipStream_t);
 template void launch_adjoint<float>(const AdjArgs<float> &, int, hipStream_t);
 template void launch_adjoint<double>(const AdjArgs<double> &, int, hipStream_t);
 

@@ -13,10 +13,13 @@ search; the bounds are enforced by clamping the log parameters (zero gradient
 beyond a bound).  Branch lengths are in expected substitutions per codon
 (tau = bl * 3 / mu_bar, likelihood.hpp:1246-1256).
 
-A fit runs in two phases: FP32-MFMA evaluations until L-BFGS converges, then
-FP64 evaluations from that point until it converges again, so the returned
-maximum is an FP64 one.  Mixture weights stay fixed unless ``fit_weights``
-(then they are a softmax of free logits).
+The objective is the mean log-likelihood per site (lnL / n_sites), so the
+gradient tolerance means the same at every alignment size.  A fit runs in two
+phases: FP32-MFMA evaluations until L-BFGS stalls at the FP32 noise level of
+the objective (changes below 1e-8 of its start value), then FP64 evaluations
+from that point until the largest gradient component falls below the
+tolerance, so the returned maximum is an FP64 one.  Mixture weights stay fixed
+unless ``fit_weights`` (then they are a softmax of free logits).
 """
 import time
 
@@ -56,8 +59,9 @@ class MixtureFit:
     of the per-node branch-length vector are free.
     """
 
-    def __init__(self, engine, parent, bl, omegas, weights, theta, fit_weights=False, history_size=8):
+    def __init__(self, engine, parent, bl, omegas, weights, theta, n_sites=1.0, fit_weights=False, history_size=8):
         self.engine = engine
+        self.scale = 1.0 / float(n_sites)
         parent = np.asarray(parent)
         self.n_nodes = len(parent)
         self.branches = torch.as_tensor(np.nonzero(parent >= 0)[0], dtype=torch.long)
@@ -94,10 +98,10 @@ class MixtureFit:
         w = torch.softmax(self.logits, 0) if self.fit_weights else self.w_fixed
         return bl, om, w, th
 
-    def _phase(self, precision, max_iter, tol_grad):
+    def _phase(self, precision, max_iter, tol_grad, tol_change):
         params = [self.x] + ([self.logits] if self.fit_weights else [])
         opt = torch.optim.LBFGS(params, lr=1.0, max_iter=max_iter, max_eval=int(max_iter * 1.5) + 10,
-                                history_size=self.history_size, tolerance_grad=tol_grad, tolerance_change=1e-14,
+                                history_size=self.history_size, tolerance_grad=tol_grad, tolerance_change=tol_change,
                                 line_search_fn="strong_wolfe")
         state = {"ll": None}
 
@@ -105,37 +109,47 @@ class MixtureFit:
             opt.zero_grad()
             bl, om, w, th = self._unpack()
             ll = engine_log_likelihood(bl, om, w, th, self.engine, precision)
-            loss = -ll
+            loss = -ll * self.scale
             loss.backward()
             self.evaluations += 1
-            state["ll"] = float(ll)
-            self.trace.append((precision, float(ll)))
+            state["ll"] = float(ll.detach())
+            self.trace.append((precision, state["ll"]))
             return loss
 
         opt.step(closure)
         st = opt.state[opt._params[0]]
         return int(st.get("n_iter", 0)), state["ll"]
 
-    def run(self, max_iter=500, tol_grad=1e-5, fp32_first=True):
+    def run(self, max_iter=500, tol_grad=1e-7, fp32_first=True):
         """Fits and returns a dict: log_likelihood (FP64, at the returned
         parameters), bl, omegas, weights, theta, iterations per phase,
-        evaluations, seconds."""
+        evaluations, seconds.  ``tol_grad`` bounds the largest gradient
+        component of lnL / n_sites with respect to the log parameters."""
         t0 = time.perf_counter()
+        with torch.no_grad():
+            bl, om, w, th = self._unpack()
+        f0 = abs(self.engine.eval(bl.numpy(), om.numpy(), w.numpy(), th.numpy(), precision=HY_PREC_FP64,
+                                  grad=False)["log_likelihood"]) * self.scale
         it32 = 0
         if fp32_first:
-            it32, _ = self._phase(HY_PREC_FP32, max_iter, tol_grad)
-        it64, _ = self._phase(HY_PREC_FP64, max_iter, tol_grad)
+            it32, _ = self._phase(HY_PREC_FP32, max_iter, tol_grad, 1e-8 * f0)
+        it64, _ = self._phase(HY_PREC_FP64, max_iter, tol_grad, 1e-14 * f0)
         bl, om, w, th = self.parameters()
         final = self.engine.eval(bl, om, w, th, precision=HY_PREC_FP64)
         g = np.concatenate([final["grad_bl"][self.branches.numpy()] * bl[self.branches.numpy()],
                             final["grad_omega"] * om, final["grad_theta"] * th])
+        # projected on the box: components pushing a parameter beyond the bound it sits at count as zero
+        x = self.x.detach().numpy()
+        lb, ub = self.lb.numpy(), self.ub.numpy()
+        g = np.where(((x <= lb) & (g < 0)) | ((x >= ub) & (g > 0)), 0.0, g)
         return {"log_likelihood": final["log_likelihood"], "bl": bl, "omegas": om, "weights": w, "theta": th,
                 "iterations_fp32": it32, "iterations_fp64": it64, "evaluations": self.evaluations,
                 "max_abs_log_gradient": float(np.max(np.abs(g))) if len(g) else 0.0,
+                "max_abs_log_gradient_per_site": float(np.max(np.abs(g))) * self.scale if len(g) else 0.0,
                 "seconds": time.perf_counter() - t0}
 
 
-def fit_mixture(tree, aln, omegas, weights, theta, bl=None, fit_weights=False, max_iter=500, tol_grad=1e-5,
+def fit_mixture(tree, aln, omegas, weights, theta, bl=None, fit_weights=False, max_iter=500, tol_grad=1e-7,
                 device=0):
     """Fits the branch-site omega mixture to ``aln`` on ``tree`` (branch
     lengths from the tree unless ``bl`` is given, one per node)."""
@@ -143,7 +157,8 @@ def fit_mixture(tree, aln, omegas, weights, theta, bl=None, fit_weights=False, m
     try:
         if bl is None:
             bl = np.array([n.branch_length for n in tree.nodes], dtype=np.float64)
-        fit = MixtureFit(eng, arrays["parent"], bl, omegas, weights, theta, fit_weights=fit_weights)
+        fit = MixtureFit(eng, arrays["parent"], bl, omegas, weights, theta, n_sites=float(np.sum(arrays["weights"])),
+                         fit_weights=fit_weights)
         return fit.run(max_iter=max_iter, tol_grad=tol_grad)
     finally:
         eng.close()

@@ -83,10 +83,11 @@ class OracleMixtureEngine:
                                mix, np.asarray(bl, dtype=np.float64), want_grad=grad, threads=self.threads)
 
 
-def oracle_fit(tree, aln, omegas, weights, theta, bl, threads=8):
+def oracle_fit(tree, aln, omegas, weights, theta, bl, threads=8, max_evals=15000):
     """Reference fit on the CPU: scipy's L-BFGS-B (8 correction pairs) over the
     FP64 oracle with the reference fitter's log parameterisation and bounds
-    (optimizer.hpp:964-989).  Returns (lnL, bl, omegas, theta)."""
+    (optimizer.hpp:964-989), at most max_evals evaluations.  Returns
+    (lnL, bl, omegas, theta)."""
     from scipy.optimize import minimize
     eng = OracleMixtureEngine(tree, aln, threads)
     parent = np.array([n.parent for n in tree.nodes])
@@ -107,6 +108,6 @@ def oracle_fit(tree, aln, omegas, weights, theta, bl, threads=8):
         return -r["log_likelihood"], -g
 
     res = minimize(f, x0, jac=True, method="L-BFGS-B", bounds=bounds,
-                   options={"maxcor": 8, "maxiter": 2000, "ftol": 1e-15, "gtol": 1e-8})
+                   options={"maxcor": 8, "maxiter": 2000, "maxfun": max_evals, "ftol": 1e-15, "gtol": 1e-8})
     b, om, th = unpack(res.x)
     return -res.fun, b, om, th

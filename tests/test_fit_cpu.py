@@ -16,8 +16,8 @@ def test_fit_logic_matches_reference_fit():
     om0, th0 = [0.5, 2.0], np.ones(5)
     ll_ref, b_ref, om_ref, th_ref = oracle_fit(tree, aln, om0, weights, th0, bl0)
     eng = OracleMixtureEngine(tree, aln)
-    fit = MixtureFit(eng, [n.parent for n in tree.nodes], bl0, om0, weights, th0)
-    r = fit.run(max_iter=400, tol_grad=1e-7)
+    fit = MixtureFit(eng, [n.parent for n in tree.nodes], bl0, om0, weights, th0, n_sites=float(np.sum(aln.weights)))
+    r = fit.run(max_iter=400, tol_grad=1e-9)
     print("fit", r["log_likelihood"], "reference fit", ll_ref, "evals", r["evaluations"])
     assert abs(r["log_likelihood"] - ll_ref) < 1e-4
     assert r["log_likelihood"] > eng.eval(bl0, om0, weights, th0)["log_likelihood"] + 1.0
@@ -37,6 +37,6 @@ def test_fit_respects_bounds():
     fit = MixtureFit(eng, [n.parent for n in tree.nodes], branch_lengths(tree), [1e-6], [1.0], np.ones(5))
     bl, om, w, th = fit.parameters()
     assert om[0] >= np.exp(LOG_RATE_BOUNDS[0]) * (1 - 1e-12)
-    r = fit.run(max_iter=100, tol_grad=1e-6)
+    r = fit.run(max_iter=100, tol_grad=1e-8)
     assert np.all(np.log(r["omegas"]) >= LOG_RATE_BOUNDS[0] - 1e-9)
     assert np.all(np.log(r["theta"]) <= LOG_RATE_BOUNDS[1] + 1e-9)

@@ -65,8 +65,8 @@ def test_lbfgs_fit_matches_reference_fit():
     ll_ref, b_ref, om_ref, th_ref = oracle_fit(tree, aln, om0, weights, th0, bl0, threads=16)
     eng, arrays = _engine_problem(sim, 2)
     assert list(arrays["parent"]) == [n.parent for n in tree.nodes]
-    fit = MixtureFit(eng, arrays["parent"], bl0, om0, weights, th0)
-    r = fit.run(max_iter=500, tol_grad=1e-7)
+    fit = MixtureFit(eng, arrays["parent"], bl0, om0, weights, th0, n_sites=float(np.sum(arrays["weights"])))
+    r = fit.run(max_iter=500, tol_grad=1e-9)
     print("GPU fit %.10f (FP32 its %d, FP64 its %d, %d evals, %.2f s); reference fit %.10f"
           % (r["log_likelihood"], r["iterations_fp32"], r["iterations_fp64"], r["evaluations"], r["seconds"], ll_ref))
     assert abs(r["log_likelihood"] - ll_ref) < 1e-4
