@@ -621,7 +621,9 @@ void launch_outside_parts(hy_engine h, PruneArgs<R> pr, bool rev, hipStream_t s)
     if (h->n_parts > 0) {
         pr.parts = (const int4 *)h->d_parts.p;
         pr.part0 = h->n_parts;
-        launch_outside<R>(pr, h->T, rev, h->lean_out, 1, s);
+        // the top part is one round of workgroups: the default variant (its
+        // LDS slots cover the lean schedule's), the parts the lean one
+        launch_outside<R>(pr, h->T, rev, false, 1, s);
         pr.part0 = 0;
         launch_outside<R>(pr, h->T, rev, h->lean_out, h->n_parts, s);
     } else {
@@ -761,7 +763,9 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         pr.part0 = h->n_parts;
         pr.root_out = 1;
     }
-    launch_inside<R>(pr, h->T, h->lean, 1, s);
+    // with parts, the top part (one round of workgroups) runs the default
+    // variant, whose LDS slots cover the lean schedule's
+    launch_inside<R>(pr, h->T, h->lean && h->n_parts == 0, 1, s);
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[3], s));
     if (grad) {
         launch_outside_parts<R>(h, pr, custom_pi == nullptr, s);

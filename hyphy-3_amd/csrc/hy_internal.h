@@ -58,6 +58,10 @@ constexpr bool lean_inside(int prec_is_fp32, int T, long n_tiles, int n_cu) {
 constexpr int outside_slots(int T, int esz, bool lean = false) {
     return lean ? 4 : (T == 16 ? 10 : (T == 32 ? 4 : (esz == 8 ? 1 : 2)));
 }
+// The top part of a split tree runs the default variants against schedules
+// built for the lean ones: their LDS slot counts must cover the lean ones.
+static_assert(inside_slots(16, 4, true) <= inside_slots(16, 4, false), "lean inside slots");
+static_assert(outside_slots(16, 4, true) <= outside_slots(16, 4, false), "lean outside slots");
 // Tip codes of a tile kept in LDS when (taxa + 1) * T fits.
 constexpr int tip_lds_codes(int T, int esz) {
     return T == 16 ? 12288 : (esz == 4 ? 24576 : (T == 32 ? 16384 : 8192));
