@@ -1667,6 +1667,69 @@ int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, do
     return HY_OK;
 }
 
+int hy_busted_expected_weights(hy_engine h, int precision, const double *bl, double alpha, const double *omegas,
+                               const double *comp_w, const double *theta, double delta, double psi, int n_syn,
+                               const double *syn_rates, const double *syn_weights, int test_class,
+                               double *out_weights, double *out_syn_weights) {
+    // BUSTEDAnalyzer::compute_expected_weights (busted.hpp:662-976) and
+    // compute_expected_syn_weights (busted.hpp:978-1135) without the error
+    // sink and bridges.  The reference accumulates, per test branch b,
+    // component k and class m, w_p w_k post_m(p) V_b . (P_k D_b) / L_m(p); under
+    // the raw time convention that sum over p and b is w_k dlnL/dw_k (the
+    // posterior-weighted mixture-weight gradient), and sum_p w_p post_m(p) is
+    // v_m dlnL/dv_m, so one gradient evaluation gives both updates.
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem || !h->have_model)
+        return fail(h, HY_ERR_STATE, "hy_busted_expected_weights before hy_set_problem/hy_set_model");
+    if (h->convention != HY_TIME_RAW)
+        return fail(h, HY_ERR_STATE, "hy_busted_expected_weights: the mixture must use HY_TIME_RAW branch times");
+    if (test_class < 0 || test_class >= h->n_class)
+        return fail(h, HY_ERR_ARG, "hy_busted_expected_weights: test class %d out of range", test_class);
+    if (n_syn < 1 || !syn_rates || !syn_weights || !out_weights || !out_syn_weights)
+        return fail(h, HY_ERR_ARG, "hy_busted_expected_weights: bad synonymous classes or outputs");
+    const int j0 = h->comp_ptr[test_class], j1 = h->comp_ptr[test_class + 1], K = j1 - j0;
+    std::vector<double> site(std::max(h->P, 1)), gbl(h->num_nodes), gom(h->n_mat), gw(h->n_comp), gth(5),
+        gsr(n_syn), gsw(n_syn);
+    double ll = 0.0;
+    TRY(hy_eval_srv(h, precision, HY_WANT_LOGL | HY_WANT_SITE_LL | HY_WANT_GRAD, bl, alpha, omegas, comp_w, theta,
+                    delta, psi, n_syn, syn_rates, syn_weights, &ll, site.data(), gbl.data(), gom.data(), gw.data(),
+                    gth.data(), gsr.data(), gsw.data()));
+    // pattern weight of the patterns with a positive likelihood in some class
+    double total = 0.0;
+    for (int p = 0; p < h->P; ++p)
+        if (site[p] > -1e19) total += h->weights[p];
+    int n_test = 0;
+    for (int n = 0; n < h->num_nodes; ++n)
+        if (h->real_parent[n] >= 0 && (h->branch_class.empty() ? 0 : h->branch_class[n]) == test_class) ++n_test;
+    auto normalise = [](double *x, int n, const double *num, double den, const double *prev) {
+        if (den > 0.0) {
+            double sum = 0.0;
+            for (int i = 0; i < n; ++i) {
+                x[i] = std::clamp(num[i] / den, 1e-6, 1.0 - 1e-6);
+                sum += x[i];
+            }
+            for (int i = 0; i < n; ++i) x[i] /= sum;
+        } else {
+            for (int i = 0; i < n; ++i) x[i] = prev[i];
+        }
+    };
+    if (K <= 1) {
+        for (int k = 0; k < K; ++k) out_weights[k] = 1.0;
+    } else {
+        std::vector<double> num(K);
+        for (int k = 0; k < K; ++k) num[k] = comp_w[j0 + k] * gw[j0 + k];
+        normalise(out_weights, K, num.data(), (double)n_test * total, comp_w + j0);
+    }
+    if (n_syn <= 1) {
+        out_syn_weights[0] = 1.0;
+    } else {
+        std::vector<double> num(n_syn);
+        for (int m = 0; m < n_syn; ++m) num[m] = syn_weights[m] * gsw[m];
+        normalise(out_syn_weights, n_syn, num.data(), total, syn_weights);
+    }
+    return HY_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -27,7 +27,7 @@ HY_WANT_GRAD = 4
 EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
-           "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning")
+           "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -86,6 +86,8 @@ def load_library(path: str = LIB_PATH):
     L.hy_autotune.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
                               C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _f64p]
     L.hy_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.hy_busted_expected_weights.argtypes = [C.c_void_p, C.c_int, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
+                                             C.c_double, C.c_int, _f64p, _f64p, C.c_int, _f64p, _f64p]
     _lib = L
     return L
 
@@ -169,6 +171,7 @@ class Engine:
         self._check(self.L.hy_set_mixture(self.h, n_mat, len(classes), _p(bc, _i32p), _p(cp, _i32p),
                                           _p(cm, _i32p), convention))
         self.n_mat, self.n_comp = n_mat, len(cm)
+        self._class_sizes = [len(c) for c in classes]
 
     def set_tile(self, T):
         self._check(self.L.hy_set_tile(self.h, int(T)))
@@ -268,6 +271,28 @@ class Engine:
             out.update(grad_bl=g[0], grad_omega=g[1], grad_w=g[2], grad_theta=g[3], grad_syn_rates=g[4],
                        grad_syn_weights=g[5])
         return out
+
+    def busted_expected_weights(self, bl, omegas, comp_w, theta, syn_rates=(1.0,), syn_weights=(1.0,), test_class=0,
+                                alpha=1.0, delta=0.0, psi=0.0, precision=HY_PREC_FP64):
+        """BUSTED's expectation step (busted.hpp:662-1135, no error sink):
+        updated omega-component weights of ``test_class`` and synonymous-rate
+        weights.  Needs raw branch times (HY_TIME_RAW mixture)."""
+        bl = _f64(bl)
+        om = _f64(np.atleast_1d(omegas))
+        cw = _f64(np.atleast_1d(comp_w))
+        th = _f64(theta)
+        sr = _f64(np.atleast_1d(syn_rates))
+        sw = _f64(np.atleast_1d(syn_weights))
+        if len(sr) != len(sw):
+            raise ValueError("syn_rates and syn_weights differ in length")
+        K = int(self._class_sizes[test_class]) if hasattr(self, "_class_sizes") else len(cw)
+        nw = np.zeros(max(K, 1))
+        nv = np.zeros(len(sr))
+        self._check(self.L.hy_busted_expected_weights(self.h, precision, _p(bl, _f64p), alpha, _p(om, _f64p),
+                                                      _p(cw, _f64p), _p(th, _f64p), delta, psi, len(sr),
+                                                      _p(sr, _f64p), _p(sw, _f64p), int(test_class), _p(nw, _f64p),
+                                                      _p(nv, _f64p)))
+        return nw[:K], nv
 
     def loglik_from_P(self, P, pi, precision=HY_PREC_FP64, site=False):
         P = _f64(P)

@@ -26,6 +26,8 @@
  *                    with M = 1 synonymous class
  *   hy_eval_srv      the same two BUSTED entries with M > 1 synonymous-rate
  *                    classes (busted.hpp:554-581, 607-649, 1335-1396)
+ *   hy_busted_expected_weights  BUSTED's expectation step for the omega and
+ *                    synonymous-rate weights (busted.hpp:662-1135)
  *   hy_eval_batch    a batch of independent problems (genes), one per handle
  *   hy_loglik_from_P compute_tree_log_likelihood_from_P (likelihood.hpp:766-806)
  */
@@ -143,6 +145,22 @@ int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, do
                 const double *syn_rates, const double *syn_weights, double *out_logL, double *out_site_ll,
                 double *out_grad_bl, double *out_grad_omega, double *out_grad_w, double *out_grad_theta,
                 double *out_grad_syn_rates, double *out_grad_syn_weights);
+
+/* BUSTED expectation step: BUSTEDAnalyzer::compute_expected_weights
+ * (busted.hpp:662-976) and compute_expected_syn_weights (busted.hpp:978-1135)
+ * without the error sink.  Needs a mixture with HY_TIME_RAW branch times (bl =
+ * the reference's s * branch_tau).  out_weights[k] (k over the components of
+ * test_class) = clamp(sum over test branches and patterns of w_p w_k post_m(p)
+ * V_b . (P_k D_b) / L_m(p), divided by (test branches x the weight of the
+ * patterns with a positive likelihood), 1e-6, 1 - 1e-6), renormalised ({1}
+ * when the class has one component); out_syn_weights[m] = clamp(sum_p w_p
+ * post_m(p) / that pattern weight, 1e-6, 1 - 1e-6), renormalised ({1} when
+ * n_syn = 1).  Computed from one hy_eval_srv gradient evaluation (the
+ * accumulated sums are w_k dlnL/dw_k and v_m dlnL/dv_m). */
+int hy_busted_expected_weights(hy_engine h, int precision, const double *bl, double alpha, const double *omegas,
+                               const double *comp_w, const double *theta, double delta, double psi, int n_syn,
+                               const double *syn_rates, const double *syn_weights, int test_class,
+                               double *out_weights, double *out_syn_weights);
 
 /* Many independent problems (genes) in one call: item i evaluates handle
  * items[i].h exactly as hy_eval would with the item's parameters and

@@ -380,3 +380,77 @@ def workspace(prob, P, pi, p0, p1, outside=True):
     if outside:
         out.update(V=V, P_D=PD, U=U, boost_counts=boost)
     return out
+
+
+def busted_expected_weights(prob, code, base, pos, pi, weights_p, omegas, weights, omega_bg, is_test, tau,
+                            syn_rates=(1.0,), syn_weights=(1.0,)):
+    """BUSTEDAnalyzer::compute_expected_weights (busted.hpp:662-976) and
+    compute_expected_syn_weights (busted.hpp:978-1135) without the error sink
+    and bridges, restated literally over the reference's inside-outside
+    workspaces: per synonymous class m the mixed P(r_m tau) (test branches
+    sum_k w_k P_k, background P_bg, :715-733), the workspace of the whole
+    alignment (:791-799), log-sum-exp posteriors over classes (:802-823), and per
+    test branch, component and class the share w_k post_m V_b . (P_k D_b) / L
+    (:826-845); totals over patterns with a positive likelihood (:847-866,
+    :942-975).  tau: raw branch times per node; weights_p: pattern weights.
+    Returns (new omega weights, new synonymous weights)."""
+    N, S = prob.num_nodes, code.S
+    K, M = len(omegas), len(syn_rates)
+    P = prob.num_patterns
+    dec = []
+    for om in list(omegas) + [omega_bg]:
+        b = params(1.0, om, list(base.theta), base.delta, base.psi)
+        Q, _ = mg94_Q(code, b, pos, pi)
+        dec.append(decompose(Q, pi))
+    wsm, Pc = [], []
+    for m in range(M):
+        comp = np.zeros((K, N, S, S))
+        mix = np.zeros((N, S, S))
+        for n in range(N):
+            for k in range(K):
+                comp[k, n] = transition(*dec[k], syn_rates[m] * tau[n])
+            mix[n] = sum(weights[k] * comp[k, n] for k in range(K)) if is_test[n] else \
+                transition(*dec[K], syn_rates[m] * tau[n])
+        Pc.append(comp)
+        wsm.append(workspace(prob, mix, pi, 0, P, outside=True) if syn_weights[m] > 1e-12 else None)
+    # posteriors over synonymous classes
+    terms = np.full((M, P), -np.inf)
+    for m in range(M):
+        if wsm[m] is None:
+            continue
+        ok = wsm[m]["L"] > 0.0
+        terms[m, ok] = np.log(syn_weights[m]) + wsm[m]["logL"][ok]
+    mx = terms.max(axis=0)
+    valid = mx > -1e19
+    post = np.zeros((M, P))
+    e = np.where(np.isfinite(terms), np.exp(terms - np.where(valid, mx, 0.0)), 0.0)
+    post[:, valid] = e[:, valid] / e[:, valid].sum(axis=0)
+    acc = np.zeros(K)
+    tot = 0.0
+    for n in range(N):
+        if not is_test[n]:
+            continue
+        for k in range(K):
+            for m in range(M):
+                if wsm[m] is None:
+                    continue
+                ws = wsm[m]
+                PD = Pc[m][k, n] @ ws["D"][n]
+                ratio = np.einsum("sb,sb->b", ws["V"][n], PD)
+                use = (post[m] > 1e-12) & (ws["L"] > 0.0)
+                acc[k] += np.sum(weights_p[use] * weights[k] * post[m, use] * ratio[use] / ws["L"][use])
+        tot += np.sum(weights_p[post.sum(axis=0) > 0.0])
+
+    def norm(num, den, prev):
+        if den <= 0.0:
+            return np.array(prev, dtype=float)
+        x = np.clip(num / den, 1e-6, 1.0 - 1e-6)
+        return x / x.sum()
+
+    new_w = np.ones(1) if K <= 1 else norm(acc, tot, weights)
+    if M <= 1:
+        new_v = np.ones(1)
+    else:
+        syn_acc = (weights_p[None, :] * post).sum(axis=1)
+        new_v = norm(syn_acc, np.sum(weights_p[valid]), syn_weights)
+    return new_w, new_v
