@@ -186,6 +186,8 @@ def main():
                     help="instead of the throughput run: torch L-BFGS maximum-likelihood fit of branch lengths, "
                          "omegas and GTR rates from a perturbed start (FP32 phase, then FP64)")
     ap.add_argument("--cpu-fit", action="store_true", help="with --fit: also the reference fit on the host (slow)")
+    ap.add_argument("--estep", action="store_true",
+                    help="instead of the throughput run: BUSTED expectation steps (omega and synonymous-rate weights)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -204,6 +206,9 @@ def main():
         return
     if args.fit:
         bench_fit(args, cfg)
+        return
+    if args.estep:
+        bench_estep(args, cfg)
         return
     sim, aln, tree, arrays, tau = build_problem(cfg)
     print(f"[bench] {args.config}: taxa={cfg['taxa']} codons={cfg['codons']} patterns={len(arrays['weights'])} "
@@ -360,6 +365,46 @@ def bench_fit(args, cfg):
             "kind": "scipy L-BFGS-B over the FP64 restatement of the reference algorithm, started at the GPU optimum, "
                     "at most 40 evaluations"}
         print(json.dumps(line), flush=True)
+
+
+def bench_estep(args, cfg):
+    """BUSTED expectation step (hy_busted_expected_weights) on the workload:
+    every branch a test branch with the K omega components, raw branch times,
+    M = 1 and M = 3 synonymous-rate classes; per-call time and the updated
+    weights.  The CPU column is the oracle's literal restatement over the
+    reference workspaces for M = 1 (one call)."""
+    sim, aln, tree, arrays, tau = build_problem(cfg)
+    K = len(cfg["omegas"])
+    eng = make_engine(arrays, K)
+    res = {"workload": f"{args.config} E-step: {cfg['taxa']} taxa x {cfg['codons']} codons x {K} omega components "
+                       "on every branch", "calls": args.steps}
+    for M, sr, sv in ((1, [1.0], [1.0]), (3, [0.4, 1.0, 2.2], [0.3, 0.5, 0.2])):
+        for prec, name in ((1, "fp32"), (0, "fp64")):
+            for _ in range(args.warmup):
+                eng.busted_expected_weights(tau, cfg["omegas"], cfg["weights"], sim["theta"], sr, sv, precision=prec)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                nw, nv = eng.busted_expected_weights(tau, cfg["omegas"], cfg["weights"], sim["theta"], sr, sv,
+                                                     precision=prec)
+            res[f"ms_per_call_M{M}_{name}"] = (time.perf_counter() - t0) / args.steps * 1e3
+            res[f"weights_M{M}_{name}"] = list(nw)
+            res[f"syn_weights_M{M}_{name}"] = list(nv)
+    eng.close()
+    if not args.no_cpu:
+        from oracle import data_ref as dr
+        from oracle import hyoracle as ho
+        names, seqs = dr.parse_fasta(sim["fasta"])
+        raln = dr.AlignmentRef(names, seqs)
+        rtree = dr.TreeRef.from_newick(sim["newick"], keep_rooted=False)
+        is_test = np.array([n.id != rtree.root for n in rtree.nodes])
+        t0 = time.perf_counter()
+        rw, _ = ho.busted_expected_weights(ho.Problem.from_refs(rtree, raln), ho.Code(0),
+                                           ho.params(1.0, 1.0, sim["theta"]), raln.pos_freqs, raln.cf3x4,
+                                           raln.weights, cfg["omegas"], cfg["weights"], 1.0, is_test, tau)
+        res["cpu_literal_M1"] = {"seconds": time.perf_counter() - t0, "weights": list(rw), "cores": 1,
+                                 "kind": "oracle: literal restatement over the reference workspaces (numpy)"}
+        res["max_rel_diff_fp64_vs_literal"] = float(np.max(np.abs(np.array(res["weights_M1_fp64"]) - rw) / rw))
+    print(json.dumps(res), flush=True)
 
 
 def gene_specs(n_genes, seed):
