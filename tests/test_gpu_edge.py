@@ -320,3 +320,27 @@ def test_graph_replays_match_eager():
             np.testing.assert_array_equal(re[k], rg[k])
         if step == 2:
             np.testing.assert_array_equal(re["site_ll"], rg["site_ll"])
+
+
+def test_balanced_tree_lean_outside_spills():
+    """A perfectly balanced 256-taxon tree: its subtree parts are six levels
+    deep, so the FP32 lean outside pass (four LDS slots for pending messages)
+    spills messages to HBM; FP64 and FP32 against the oracle."""
+    rng = np.random.default_rng(17)
+
+    def clade(lo, hi):
+        if hi - lo == 1:
+            return "T%d:0.08" % lo
+        mid = (lo + hi) // 2
+        return "(%s,%s):0.05" % (clade(lo, mid), clade(mid, hi))
+
+    n, L = 256, 200
+    nwk = "(%s,%s);" % (clade(0, n // 2), clade(n // 2, n))
+    codons = dr.Code().sense_codons
+    anc = rng.choice(codons, size=L)
+    seqs = {}
+    for i in range(n):
+        s = [c if rng.uniform() > 0.15 else rng.choice(codons) for c in anc]
+        seqs["T%d" % i] = "".join(s)
+    tree = dr.TreeRef.from_newick(nwk)
+    check_plain(tree, alignment(seqs), 0.4, [1.2, 0.8, 1.0, 2.0, 0.9], branch_lengths(tree))
