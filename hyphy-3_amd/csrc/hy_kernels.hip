@@ -151,6 +151,29 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red, in
     b = sb;
 }
 
+// Two block sums with one barrier: red2 (2 NT / 64 entries) must not be the
+// buffer of the previous call (callers alternate two buffers; a thread that
+// reaches this call's barrier has finished reading the other one).
+template <int NT>
+__device__ __forceinline__ void block_sum2_once(double &a, double &b, double *red2, int tid) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off);
+        b += __shfl_down(b, off);
+    }
+    if ((tid & 63) == 0) {
+        red2[2 * (tid >> 6)] = a;
+        red2[2 * (tid >> 6) + 1] = b;
+    }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0;
+    for (int k = 0; k < NT / 64; ++k) {
+        sa += red2[2 * k];
+        sb += red2[2 * k + 1];
+    }
+    a = sa;
+    b = sb;
+}
+
 constexpr int MODEL_NT = 1024;
 // Batched evaluations: matrix m belongs to gene mat_gene[m], whose
 // frequencies and scalar parameters (alpha, theta, delta, psi) it uses.
@@ -178,7 +201,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
     __shared__ double sq[64], isq[64], lam[64];
     __shared__ double rc[32], rs[32], rt[32];
     __shared__ int rp[32], rq[32];
-    __shared__ double red[2 * NT / 64];
+    __shared__ double red[4 * NT / 64];  // block_sum; two alternating buffers of block_sum2_once
     __shared__ int any_rot;
     __shared__ double thl[8], posl[12];
 
@@ -303,8 +326,8 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
                 else
                     lam[i] = aS[r] / (1.0 - rr);
             }
-            // (the reduction's barriers also publish lam and end every read of T2)
-            block_sum2<NT>(so, rq, red, tid);
+            // (the reduction's barrier also publishes lam and ends every read of T2)
+            block_sum2_once<NT>(so, rq, red, tid);
             const double soff = sqrt(so);
             // Eigenvectors that already diagonalise A to 1e-12 (and are
             // orthonormal to 1e-12) are kept as they are: a step from there
@@ -342,7 +365,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
                 T2[i * LDA + j] = e;
                 en += e * e;
             }
-            block_sum2<NT>(en, fz, red, tid);
+            block_sum2_once<NT>(en, fz, red + 2 * NT / 64, tid);
             const double enorm = sqrt(en), frozen = sqrt(fz);
             // X <- X + X E
             f64x4 aX = splat<f64x4>(0.0);
