@@ -172,8 +172,8 @@ def cpu_baseline(cfg, sim, seconds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
     ap.add_argument("--genes", type=int, default=C3["genes"], help="c3 only: number of genes in the batch")
     ap.add_argument("--tile", type=int, default=0)
