@@ -1844,6 +1844,7 @@ __device__ __forceinline__ void comp_add(double &s, double &c, double s2, double
 }
 
 constexpr int FINAL_NT = 1024;
+constexpr int FINAL_LDS_BR = 2048;  // branches whose per-branch terms k_final stages in LDS
 __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
     FinalArgs a = args;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1874,7 +1875,11 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
     }
     __shared__ double wsum[16], wcmp[16];
     __shared__ double sum_tau[64], mubar[64];
+    // per branch (up to FINAL_LDS_BR): class, node, sum_j dlnL/dtau share and tau times it
+    __shared__ int s_cls[FINAL_LDS_BR], s_node[FINAL_LDS_BR];
+    __shared__ double s_dt[FINAL_LDS_BR], s_tdt[FINAL_LDS_BR];
     const bool grad = a.want_grad != 0;
+    const bool staged = a.n_branch <= FINAL_LDS_BR;
     const int NL = grad ? FINAL_NT / 2 : FINAL_NT;  // threads on the lnL sum
     const int P = a.n_patterns;
     const double *prm = a.params;
@@ -1910,12 +1915,27 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
             mubar[gt] = mb < 1e-12 ? 1e-12 : mb;
         }
         for (int n = gt; n < NR; n += FINAL_NT - NL) gbl[n] = 0.0;
+        if (grad && staged)
+            for (int bi = gt; bi < a.n_branch; bi += FINAL_NT - NL) {
+                const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3] - cls0;
+                double dt = 0.0;
+                const int nj = a.comp_ptr[cls + 1] - a.comp_ptr[cls];  // components of this branch's class
+                for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+                s_cls[bi] = cls;
+                s_node[bi] = node - node0;
+                s_dt[bi] = dt;
+                s_tdt[bi] = a.tau[node] * dt;
+            }
     }
     __syncthreads();
     if (grad && tid >= NL) {
         for (int cl = gwave; cl < a.n_class; cl += NGW) {
             double v = 0.0;
             for (int bi = lane; bi < a.n_branch; bi += 64) {
+                if (staged) {
+                    if (s_cls[bi] == cl) v += s_tdt[bi];
+                    continue;
+                }
                 if (a.blist[bi * BR_STRIDE + 3] != cls0 + cl) continue;
                 double dt = 0.0;
                 const int nj = a.comp_ptr[cl + 1] - a.comp_ptr[cl];  // components of this branch's class
@@ -1926,11 +1946,19 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
             if (lane == 0) sum_tau[cl] = v;
         }
         for (int bi = gt; bi < a.n_branch; bi += FINAL_NT - NL) {
-            const int node = a.blist[bi * BR_STRIDE] - node0, cls = a.blist[bi * BR_STRIDE + 3] - cls0;
-            const double conv = (a.convention == 1) ? 3.0 / mubar[cls] : 1.0;
+            int node, cls;
             double dt = 0.0;
-            const int nj = a.comp_ptr[cls + 1] - a.comp_ptr[cls];
-            for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+            if (staged) {
+                node = s_node[bi];
+                cls = s_cls[bi];
+                dt = s_dt[bi];
+            } else {
+                node = a.blist[bi * BR_STRIDE] - node0;
+                cls = a.blist[bi * BR_STRIDE + 3] - cls0;
+                const int nj = a.comp_ptr[cls + 1] - a.comp_ptr[cls];
+                for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+            }
+            const double conv = (a.convention == 1) ? 3.0 / mubar[cls] : 1.0;
             if (node >= 0 && node < NR) gbl[node] = conv * dt;
         }
     }
