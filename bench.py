@@ -194,7 +194,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build measures one MI355X; multi-GPU runs are not part of it "
-                                   "(the path would shard patterns across GPUs, see DESIGN.md)"}))
+                                   "(the path would shard patterns across GPUs)"}))
         sys.exit(2)
     if args.config == "c3":
         bench_genes(args)

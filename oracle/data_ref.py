@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY: the checker for the product's C++ data layer and
 the input builder for the FP64 CPU oracle.  Only tests/, ``smoke()`` and
-bench.py's ``cpu_baseline`` leg import it.
+bench.py's CPU legs (``cpu_baseline``, the reference-fit and E-step checks)
+import it.
 
 Restated reference semantics (paths relative to the reference root):
   * FASTA parsing               include/hyphy/core/alignment.hpp:118-190

@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY: the parity checker and CPU baseline for the MI355X
  * library.  Loaded only by tests/, the smoke() entry point and bench.py's
- * cpu_baseline leg.  Nothing in the product links, loads or calls this code.
+ * CPU legs (cpu_baseline, the reference-fit and E-step checks).  Nothing in
+ * the product links, loads or calls this code.
  *
  * Reference files restated here (paths relative to the reference root):
  *   include/hyphy/core/genetic_code.hpp   (sense-codon numbering, diff table)

@@ -1,8 +1,9 @@
 """ctypes front end for the FP64 CPU oracle (libhyoracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()``
-and bench.py's ``cpu_baseline`` leg, as the checker or the CPU baseline.
-The product never imports this module.
+and bench.py's CPU legs (``cpu_baseline``, the reference-fit check of
+``--fit --cpu-fit`` and the CPU column of ``--estep``), as the checker or the
+CPU baseline.  The product never imports this module.
 """
 from __future__ import annotations
 
