@@ -69,3 +69,46 @@ def test_random_problem(seed, n, L):
             assert rel(r["log_likelihood"], ref["log_likelihood"]) < tl, msg
             for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
                 assert grad_err(r[k], ref[k]) < tg, msg + (k,)
+
+
+LARGE = [(201, 48, 900, 3, 0, 0.0), (202, 80, 1500, 2, 1, 0.1), (203, 128, 600, 1, 0, 0.05),
+         (204, 180, 1200, 3, 1, 0.0), (205, 100, 2400, 2, 0, 0.1), (206, 150, 300, 3, 1, 0.2)]
+
+
+@pytest.mark.parametrize("seed,n,L,K,conv,gaps", LARGE)
+def test_larger_problems_split_trees(seed, n, L, K, conv, gaps):
+    """Simulated alignments large enough for subtree parts, the lean FP32
+    pruning variants (with pending messages spilled to HBM in deep parts) and
+    the default variants on the top part; the engine's own tile choice, then
+    the autotuner's, against the oracle; some with test/background classes."""
+    from helpers import sim_problem
+    rng = np.random.default_rng(seed)
+    omegas = list(rng.uniform(0.05, 3.0, size=K))
+    w = list(rng.dirichlet(np.ones(K)))
+    tree, aln, sim = sim_problem(n, L, seed, omegas=omegas, weights=w, gaps=gaps)
+    N = len(tree.nodes)
+    two = seed % 2 == 0
+    bc = np.array([0 if (not two or i % 3) else 1 for i in range(N)], dtype=np.int32)
+    om_all = omegas + ([0.6] if two else [])
+    classes_o = [[(k, w[k]) for k in range(K)]] + ([[(K, 1.0)]] if two else [])
+    classes_e = [list(range(K))] + ([[K]] if two else [])
+    cw = w + ([1.0] if two else [])
+    bl = branch_lengths(tree)
+    mix = ho.Mixture(om_all, bc, classes_o, conv)
+    ref = ho.mixture_eval(ho.Problem.from_refs(tree, aln), ho.Code(0), ho.params(1.0, 1.0, sim["theta"]),
+                          aln.pos_freqs, aln.cf3x4, mix, bl, threads=16)
+    eng = make_engine(tree, aln)
+    eng.set_mixture(len(om_all), bc, classes_e, convention=conv)
+    for tuned in (False, True):
+        for prec, tl, tg in ((0, 1e-9, 1e-7), (1, 1e-5, 1e-4)):
+            if tuned:
+                eng.autotune(bl, om_all, cw, sim["theta"], precision=prec, reps=1)
+            r = eng.eval(bl, om_all, cw, sim["theta"], precision=prec)
+            msg = (seed, n, L, K, conv, gaps, tuned, prec, eng.last_tile(), eng.last_parts())
+            print(msg, r["log_likelihood"], ref["log_likelihood"])
+            if not tuned:
+                assert eng.last_parts() > 0, msg  # few tiles: split trees
+            assert rel(r["log_likelihood"], ref["log_likelihood"]) < tl, msg
+            for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
+                assert grad_err(r[k], ref[k]) < tg, msg + (k,)
+    eng.close()
