@@ -465,9 +465,17 @@ def bench_genes(args):
         return batch.eval_flat(bl0 * g, om0 * f, w0, th0 * (2.0 - f), precision=prec)
 
     res = {}
+    tuning = {}
     for prec, name in ((1, "fp32"), (0, "fp64")):
         if prec == 0 and args.no_fp64:
             continue
+        if not args.no_autotune:
+            # measured tile of the fused batch, before (and outside) the timed steps
+            step(0, prec)
+            t0 = time.perf_counter()
+            tile, ms = batch.autotune(precision=prec, reps=2)
+            tuning[name] = {"tile": tile, "ms": ms, "seconds": time.perf_counter() - t0}
+            print(f"[bench] c3 {name} autotune {tuning[name]}", file=sys.stderr, flush=True)
         for i in range(args.warmup):
             step(i, prec)
             print(f"[bench] c3 {name} warmup step {i} done", file=sys.stderr, flush=True)
@@ -491,6 +499,7 @@ def bench_genes(args):
         "eval_roofline": {"flops": f_alg, "t_bound_ms": t_bound * 1e3, "frac": t_bound / ms32,
                           "note": "sum over genes of 6 S^2 P E_int + 8 S^3 B K at 157.3 TFLOP/s"},
         "device_bytes": int(dev_bytes),
+        "autotune": tuning or None,
     }
     if "fp64" in res:
         ms64, out64 = res["fp64"]

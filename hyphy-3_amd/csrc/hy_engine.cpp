@@ -160,6 +160,7 @@ struct GroupState {
     std::vector<hy_engine> members;  // genes of the merged problem, in order
     std::vector<uint64_t> epochs;
     int prec = -1;
+    int tuned_T[2] = {0, 0};  // hy_autotune_batch's tile per precision ([0] FP32, [1] FP64; 0: cost model)
     bool no_fuse = false;  // HY_NO_FUSE=1: evaluate group batches engine by engine
     struct Slice {
         int node0, n_ext, n_real, mat0, n_mat, comp0, n_comp, pbase, P;
@@ -1759,9 +1760,11 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
     hy_engine c = gs.ctx;
     const int esz = prec == HY_PREC_FP64 ? 8 : 4, ri = prec == HY_PREC_FP64 ? 1 : 0;
     const int G = (int)hs.size();
-    // tile width: fewest rounds of resident workgroups x per-node cost
-    int T = 16;
-    {
+    // tile width: the batch autotuner's, else fewest rounds of resident
+    // workgroups x per-node cost
+    int T = gs.tuned_T[ri];
+    if (T != 16 && T != 32 && T != 64) {
+        T = 16;
         const double node_cost[3] = {1.0, 1.3, 2.0};
         double best = 1e300;
         for (int k = 0; k < 3; ++k) {
@@ -2221,6 +2224,51 @@ int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned fla
         if (it.status != HY_OK && first == HY_OK) first = it.status;
     }
     return first;
+}
+
+int hy_autotune_batch(int n_items, hy_batch_item *items, int precision, unsigned flags, int reps, int *out_tile,
+                      double *out_ms) {
+    // Times the fused batch evaluation at every pattern-tile width and keeps
+    // the fastest for this precision in the group (the batch is rebuilt).
+    if (n_items < 2 || !items || reps < 1 || reps > 1000) return HY_ERR_ARG;
+    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32) return HY_ERR_ARG;
+    std::vector<hy_engine> hs;
+    for (int i = 0; i < n_items; ++i) {
+        if (!items[i].h) return HY_ERR_ARG;
+        hs.push_back(items[i].h);
+    }
+    if (!fusable(hs) || hs[0]->group->no_fuse)
+        return fail(hs[0], HY_ERR_STATE, "hy_autotune_batch: the items are not one fusable group batch");
+    GroupState &gs = *hs[0]->group;
+    const int ri = precision == HY_PREC_FP64 ? 1 : 0;
+    int best_T = 0;
+    double best = 1e300;
+    for (int T = 16; T <= 64; T *= 2) {
+        gs.tuned_T[ri] = T;
+        gs.members.clear();  // rebuild the merged problem at this tile width
+        int rc = hy_eval_batch(n_items, items, precision, flags);
+        std::vector<double> t;
+        for (int r = 0; rc == HY_OK && r < reps; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = hy_eval_batch(n_items, items, precision, flags);
+            t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        if (rc != HY_OK) {
+            gs.tuned_T[ri] = 0;
+            gs.members.clear();
+            return rc;
+        }
+        std::sort(t.begin(), t.end());
+        if (t[t.size() / 2] < best) {
+            best = t[t.size() / 2];
+            best_T = T;
+        }
+    }
+    gs.tuned_T[ri] = best_T;
+    gs.members.clear();
+    if (out_tile) *out_tile = best_T;
+    if (out_ms) *out_ms = best;
+    return HY_OK;
 }
 
 }  // extern "C"

@@ -27,7 +27,8 @@ HY_WANT_GRAD = 4
 EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set_model", "hy_set_mixture",
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
-           "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights")
+           "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
+           "hy_autotune_batch")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -88,6 +89,8 @@ def load_library(path: str = LIB_PATH):
     L.hy_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     L.hy_busted_expected_weights.argtypes = [C.c_void_p, C.c_int, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
                                              C.c_double, C.c_int, _f64p, _f64p, C.c_int, _f64p, _f64p]
+    L.hy_autotune_batch.argtypes = [C.c_int, C.POINTER(BatchItem), C.c_int, C.c_uint, C.c_int, C.POINTER(C.c_int),
+                                    _f64p]
     _lib = L
     return L
 
@@ -424,6 +427,19 @@ class Batch:
         if rc != HY_OK:
             bad = next((i for i in range(n) if self.items[i].status != HY_OK), 0)
             raise HyError(rc, f"item {bad}: " + self.L.hy_last_error(self.engines[bad].h).decode())
+
+    def autotune(self, precision=HY_PREC_FP64, grad=True, reps=3):
+        """Times the fused batch at every tile width with the current flat
+        parameters (set by ``eval``/``eval_flat`` or directly) and keeps the
+        fastest for this precision.  Returns (tile, ms per batch)."""
+        self._run(precision, grad, False)  # output pointers for the tuning calls
+        n = len(self.engines)
+        flags = HY_WANT_LOGL | (HY_WANT_GRAD if grad else 0)
+        tile, ms = C.c_int(), C.c_double()
+        rc = self.L.hy_autotune_batch(n, self.items, precision, flags, int(reps), C.byref(tile), C.byref(ms))
+        if rc != HY_OK:
+            raise HyError(rc, self.L.hy_last_error(self.engines[0].h).decode())
+        return tile.value, ms.value
 
     def results(self, grad=True, site=False):
         out = []

@@ -183,6 +183,16 @@ typedef struct hy_batch_item {
 } hy_batch_item;
 int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned flags);
 
+/* Autotuner of fused gene batches: times `reps` hy_eval_batch calls (after
+ * an untimed one) of a batch that fuses (every item a distinct engine of one
+ * group) at every pattern-tile width (16, 32, 64) and keeps the fastest for
+ * this precision in the group; later fused batches of the group use it.
+ * Outputs (each may be NULL): the tile and its median time per batch in
+ * milliseconds.  The items' outputs receive the last evaluation's results.
+ * No counterpart in the reference. */
+int hy_autotune_batch(int n_items, hy_batch_item *items, int precision, unsigned flags, int reps, int *out_tile,
+                      double *out_ms);
+
 /* lnL from caller-supplied transition matrices P[num_nodes*S*S] (row-major,
  * entry of the root ignored) and root frequencies pi[S]. */
 int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,

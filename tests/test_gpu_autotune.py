@@ -41,3 +41,32 @@ def test_autotune_choice_is_used_and_exact():
     assert eng.last_tile() == 32 and eng.last_parts() == 0
     assert rel(r["log_likelihood"], ref["log_likelihood"]) < 1e-5
     eng.close()
+
+
+def test_batch_autotune_choice_is_used():
+    """hy_autotune_batch: a fused batch of genes keeps its measured tile and
+    its results stay equal to per-gene evaluations."""
+    from hyphy3.engine import HY_PREC_FP32, HY_PREC_FP64, Batch, create_group
+    probs = [sim_problem(n, L, 90 + i) for i, (n, L) in enumerate(((8, 120), (14, 300), (11, 90), (20, 200)))]
+    engines = create_group(len(probs))
+    params = []
+    for (tree, aln, sim), eng in zip(probs, engines):
+        make_engine(tree, aln, eng=eng)
+        N = len(tree.nodes)
+        eng.set_mixture(1, np.zeros(N, dtype=np.int32), [[0]], convention=1)
+        params.append((branch_lengths(tree), [0.4], [1.0], sim["theta"]))
+    batch = Batch(engines)
+    for prec, tol in ((HY_PREC_FP32, 1e-5), (HY_PREC_FP64, 1e-12)):
+        batch.eval(params, precision=prec)
+        tile, ms = batch.autotune(precision=prec, reps=1)
+        assert tile in (16, 32, 64) and ms > 0.0
+        out = batch.eval(params, precision=prec)
+        for (tree, aln, sim), prm, r in zip(probs, params, out):
+            ref = make_engine(tree, aln)
+            ref.set_mixture(1, np.zeros(len(tree.nodes), dtype=np.int32), [[0]], convention=1)
+            rr = ref.eval(*prm, precision=HY_PREC_FP64)
+            assert rel(r["log_likelihood"], rr["log_likelihood"]) < max(tol, 1e-9)
+            assert grad_err(r["grad_bl"], rr["grad_bl"]) < (1e-4 if prec == HY_PREC_FP32 else 1e-7)
+            ref.close()
+    for e in engines:
+        e.close()
