@@ -248,6 +248,8 @@ def main():
             "rel_err_grad_vs_fp64_norm": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
             "logL": out64["log_likelihood"],
         })
+    # device memory the engine holds (both precisions' buffers once both have run)
+    res["device_bytes"] = int(eng.device_bytes())
     eng.close()
     # profiled engine: HIP events around every stage on the engine's stream
     os.environ["HY_PROFILE"] = "1"
