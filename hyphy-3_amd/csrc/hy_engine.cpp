@@ -317,7 +317,11 @@ void set_frechet_groups(hy_engine h, const Lists &per) {
         longest = std::max(longest, l.size());
     }
     const size_t target = (size_t)2 * h->n_cu;
-    h->grp_size = (int)std::max<size_t>(1, (total + target - 1) / target);
+    // about two groups per CU, and no group longer than the mean list (many
+    // short lists of uneven length, e.g. gene batches, would otherwise leave
+    // the longest lists to a few workgroups at the end)
+    const size_t mean = per.empty() ? 1 : std::max<size_t>(1, (total + per.size() - 1) / per.size());
+    h->grp_size = (int)std::max<size_t>(1, std::min((total + target - 1) / target, mean));
     h->n_grp = (int)((longest + h->grp_size - 1) / h->grp_size);
 }
 

@@ -1722,6 +1722,10 @@ __global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a)
     __shared__ double lam[64], ex[64];
     __shared__ double red[24];
     const int e0 = a.wl_ptr[m] + grp * a.grp_size, e1 = min(a.wl_ptr[m + 1], e0 + a.grp_size);
+    if (e0 >= e1) {  // a matrix with fewer pairs than groups: zero projections
+        if (tid < 6) a.proj[((size_t)m * 6 + tid) * a.n_grp + grp] = 0.0;
+        return;
+    }
     f64x4 H[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) H[q] = splat<f64x4>(0.0);
