@@ -148,6 +148,11 @@ struct hy_engine_s {
     DevBuf d_Prow[2], d_PA[2], d_PTA[2];  // Prow: [n_ext][PRS] (P row-major, then its row sums)
     DevBuf d_D[2], d_Vp[2], d_E, d_G, d_Lroot, d_Eroot, d_site, d_Apart, d_fr;
     DevBuf d_wl, d_proj;
+    // BUSTED's error sink: components with comp_mat == HY_SINK, the 64 x 64
+    // sink matrix (hy_set_sink) and the (branch, component) slots using it
+    bool has_sink = false, sink_set = false;
+    DevBuf d_sink, d_sinkwl;
+    std::vector<int32_t> sink_wl;
     int grp_size = 1, n_grp = 1;  // Fréchet stage grouping of (branch, component) pairs per matrix
     std::vector<int32_t> wl;  // uploaded copy of the per-matrix W slot lists
     int n_chunks = 1, tiles_per_chunk = 1;
@@ -398,10 +403,21 @@ int prepare_buffers(hy_engine h, int prec) {
         // per matrix: the W slots (branch * max_comp + component index) whose
         // component uses it, branch-major (fixed summation order)
         std::vector<std::vector<int32_t>> per(h->n_mat);
+        std::vector<int32_t> swl;
         for (int bi = 0; bi < h->n_branch; ++bi) {
             const int cls = h->blist[bi * BR_STRIDE + 3];
-            for (int j = h->comp_ptr[cls]; j < h->comp_ptr[cls + 1]; ++j)
-                per[h->comp_mat[j]].push_back(bi * h->max_comp + (j - h->comp_ptr[cls]));
+            for (int j = h->comp_ptr[cls]; j < h->comp_ptr[cls + 1]; ++j) {
+                const int32_t slot = bi * h->max_comp + (j - h->comp_ptr[cls]);
+                if (h->comp_mat[j] < 0)
+                    swl.push_back(slot);
+                else
+                    per[h->comp_mat[j]].push_back(slot);
+            }
+        }
+        if (swl != h->sink_wl || (!swl.empty() && !h->d_sinkwl.p)) {
+            if (!swl.empty()) TRY(upload(h, h->d_sinkwl, swl.data(), swl.size() * 4));
+            h->sink_wl = swl;
+            ++h->epoch;  // the sink launch's grid changed
         }
         std::vector<int32_t> wl(h->n_mat + 1, 0);
         for (int m = 0; m < h->n_mat; ++m) wl[m + 1] = wl[m] + (int32_t)per[m].size();
@@ -716,6 +732,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         pa.Prow = (R *)h->d_Prow[ri].p;
         pa.PA = (R *)h->d_PA[ri].p;
         pa.PTA = (R *)h->d_PTA[ri].p;
+        pa.sink = (const double *)h->d_sink.p;
         launch_pmat<R>(pa, grad ? &pj : nullptr, h->n_mat, s);
     } else {
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[1], s));
@@ -773,7 +790,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     launch_inside<R>(pr, h->T, h->lean && h->n_parts == 0, 1, s);
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[3], s));
     if (grad) {
-        launch_outside_parts<R>(h, pr, custom_pi == nullptr, s);
+        launch_outside_parts<R>(h, pr, custom_pi == nullptr && !h->has_sink, s);
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[4], s));
         AdjArgs<R> aa{};
         aa.blist = (const int32_t *)h->d_blist.p;
@@ -823,9 +840,20 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
             h->d_Apart.bytes < (size_t)std::max(1, h->n_branch) * h->n_chunks * 4096 * 8)
             return fail(h, HY_ERR_STATE, "internal: Frechet stage buffers smaller than the grouping needs");
         if (h->n_branch > 0) launch_frechet(fa, s);
+        if (!h->sink_wl.empty()) {
+            if (!h->d_sink.p) return fail(h, HY_ERR_STATE, "internal: error-sink matrix not on the device");
+            SinkArgs sa{};
+            sa.Apart = fa.Apart;
+            sa.n_chunks = h->n_chunks;
+            sa.max_comp = h->max_comp;
+            sa.slots = (const int32_t *)h->d_sinkwl.p;
+            sa.sink = (const double *)h->d_sink.p;
+            sa.out = fa.out;
+            launch_sink(sa, (int)h->sink_wl.size(), s);
+        }
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[6], s));
     } else {
-        if (flags & RUN_OUTSIDE) launch_outside_parts<R>(h, pr, custom_pi == nullptr, s);
+        if (flags & RUN_OUTSIDE) launch_outside_parts<R>(h, pr, custom_pi == nullptr && !h->has_sink, s);
         if (prof)
             for (int k = 4; k <= 6; ++k) HIP_TRY(h, hipEventRecord(h->ev[k], s));
     }
@@ -962,7 +990,8 @@ void hy_destroy(hy_engine h) {
                       &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_parts, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
                       &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
-                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_wl, &h->d_proj, &h->d_wover};
+                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_wl, &h->d_proj, &h->d_wover,
+                      &h->d_sink, &h->d_sinkwl};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -1272,9 +1301,17 @@ int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_cl
     }
     const int n_comp = comp_ptr[n_class];
     if (n_comp > 192) return fail(h, HY_ERR_ARG, "hy_set_mixture: at most 192 components");
-    for (int j = 0; j < n_comp; ++j)
+    bool has_sink = false;
+    for (int j = 0; j < n_comp; ++j) {
+        if (comp_mat[j] == HY_SINK) {
+            has_sink = true;
+            continue;
+        }
         if (comp_mat[j] < 0 || comp_mat[j] >= n_mat)
             return fail(h, HY_ERR_ARG, "hy_set_mixture: comp_mat[%d]=%d out of range", j, comp_mat[j]);
+    }
+    if (has_sink && time_convention != HY_TIME_RAW)
+        return fail(h, HY_ERR_ARG, "hy_set_mixture: error-sink components need HY_TIME_RAW");
     std::vector<int32_t> bc(h->num_nodes, 0);
     if (branch_class) {
         for (int n = 0; n < h->num_nodes; ++n) {
@@ -1293,12 +1330,36 @@ int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_cl
     h->convention = time_convention;
     h->comp_ptr.assign(comp_ptr, comp_ptr + n_class + 1);
     h->comp_mat.assign(comp_mat, comp_mat + n_comp);
+    if (has_sink != h->has_sink) ++h->epoch;  // outside variant and sink launch change
+    h->has_sink = has_sink;
     h->branch_class = bc;
     refresh_blist_classes(h);
     h->warm_valid = false;
     TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), BR_STRIDE) * 4));
     TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
     TRY(upload(h, h->d_comp_mat, h->comp_mat.data(), h->comp_mat.size() * 4));
+    return HY_OK;
+}
+
+int hy_set_sink(hy_engine h, const double *freqs) {
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem) return fail(h, HY_ERR_STATE, "hy_set_sink before hy_set_problem");
+    if (!freqs) return fail(h, HY_ERR_ARG, "hy_set_sink: NULL frequencies");
+    const int S = h->S;
+    for (int i = 0; i < S; ++i)
+        if (!(freqs[i] >= 0.0 && freqs[i] <= 1.0))
+            return fail(h, HY_ERR_ARG, "hy_set_sink: frequency %d = %g outside [0, 1]", i, freqs[i]);
+    // P_sink(i, j) = f_j / max(1e-12, 1 - f_i), i != j (busted.hpp:193-206)
+    std::vector<double> P(4096, 0.0);
+    for (int i = 0; i < S; ++i) {
+        const double inv = 1.0 / std::max(1e-12, 1.0 - freqs[i]);
+        for (int j = 0; j < S; ++j)
+            if (j != i) P[(size_t)i * 64 + j] = freqs[j] * inv;
+    }
+    const void *before = h->d_sink.p;
+    TRY(upload(h, h->d_sink, P.data(), P.size() * 8));
+    if (h->d_sink.p != before) ++h->epoch;
+    h->sink_set = true;
     return HY_OK;
 }
 
@@ -1425,6 +1486,7 @@ int check_eval_args(hy_engine h, int precision, const double *bl, const double *
     if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
         return fail(h, HY_ERR_ARG, "%s: unknown precision %d", who, precision);
     if (!bl || !omegas || !comp_w || !theta) return fail(h, HY_ERR_ARG, "%s: NULL parameter array", who);
+    if (h->has_sink && !h->sink_set) return fail(h, HY_ERR_STATE, "%s: error-sink component without hy_set_sink", who);
     return HY_OK;
 }
 
@@ -1688,6 +1750,8 @@ int hy_busted_expected_weights(hy_engine h, int precision, const double *bl, dou
         return fail(h, HY_ERR_STATE, "hy_busted_expected_weights before hy_set_problem/hy_set_model");
     if (h->convention != HY_TIME_RAW)
         return fail(h, HY_ERR_STATE, "hy_busted_expected_weights: the mixture must use HY_TIME_RAW branch times");
+    if (h->has_sink)
+        return fail(h, HY_ERR_UNSUPPORTED, "hy_busted_expected_weights: mixtures with an error sink are not supported");
     if (test_class < 0 || test_class >= h->n_class)
         return fail(h, HY_ERR_ARG, "hy_busted_expected_weights: test class %d out of range", test_class);
     if (n_syn < 1 || !syn_rates || !syn_weights || !out_weights || !out_syn_weights)
@@ -1879,6 +1943,7 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
             blist.insert(blist.end(), e, e + BR_STRIDE);
         }
         for (int cc = 0; cc < h->n_class; ++cc) comp_ptr.push_back(comp0 + h->comp_ptr[cc + 1]);
+        if (h->has_sink) return fail(c, HY_ERR_UNSUPPORTED, "batch: error-sink components are not batched");
         for (int j = 0; j < h->n_comp; ++j) comp_mat.push_back(mat0 + h->comp_mat[j]);
         // tip codes [taxa + 1][P_pad], ambiguity indices batch-global
         const int amb0 = (int)amb.size();

@@ -150,6 +150,7 @@ struct PmatArgs {
     R *Prow;      // [n_nodes][PRS]   row-major P, then its row sums
     R *PA;        // [n_nodes][4][16][64]   A-operand swizzle of P
     R *PTA;       // [n_nodes][4][16][64]   A-operand swizzle of P^T
+    const double *sink;  // [64][64] row-major error-sink matrix, read for components with comp_mat < 0
 };
 
 template <typename R>
@@ -251,6 +252,17 @@ template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean
 // rev: every edge matrix is reversible with respect to a.pi (built from the model)
 template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev, bool lean, int ny, hipStream_t s);
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s);
+// <A_b, P_sink> of the error-sink components: one workgroup per (branch,
+// component) slot, written to out[slot * 8] (and a zero time derivative at
+// out[slot * 8 + 1])
+struct SinkArgs {
+    const double *Apart;  // [n_branch][n_chunks][4096]
+    int n_chunks, max_comp;
+    const int32_t *slots;  // branch * max_comp + component index
+    const double *sink;    // [64][64] row-major
+    double *out;           // [n_branch][max_comp][8]
+};
+void launch_sink(const SinkArgs &a, int n_slots, hipStream_t s);
 void launch_frechet(const FrechetArgs &a, hipStream_t s);  // chunk sums, spectral transforms, projections
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s);  // one workgroup per gene
 

@@ -564,9 +564,17 @@ __device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, double *
         }
         if (tid < 64) rl = a.lam[m * 64 + tid];
     };
-    if (j0 < j1) fetch(j0);
-    for (int j = j0; j < j1; ++j) {
+    // error-sink components (comp_mat < 0) have no spectrum: skipped here,
+    // added as w_j P_sink after the products
+    auto next_mat = [&](int j) {
+        while (j < j1 && a.comp_mat[j] < 0) ++j;
+        return j;
+    };
+    int j = next_mat(j0);
+    if (j < j1) fetch(j);
+    while (j < j1) {
         const double wj = prm[a.pl.off_w + j];
+        const int jn = next_mat(j + 1);
         __syncthreads();  // the previous component's operand reads are done
         if (tid < 64) ex[tid] = exp(rl * tau);
         __syncthreads();
@@ -576,7 +584,7 @@ __device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, double *
             Us[(idx >> 6) * LDA + (idx & 63)] = ru[k] * ex[idx & 63];
             Vs[(idx >> 6) * LDA + (idx & 63)] = rv[k];
         }
-        if (j + 1 < j1) fetch(j + 1);
+        if (jn < j1) fetch(jn);
         __syncthreads();
         f64x4 acc[4];
         gemm64_acc<false, false>(Us, Vs, acc, w, lr, lg);
@@ -584,7 +592,17 @@ __device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, double *
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int r = 0; r < 4; ++r) Pacc[q][r] += wj * (acc[q][r] > 0.0 ? acc[q][r] : 0.0);
+        j = jn;
     }
+    for (int js = j0; js < j1; ++js)
+        if (a.comp_mat[js] < 0) {
+            const double ws = prm[a.pl.off_w + js];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    Pacc[q][r] += ws * a.sink[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr];
+        }
     __syncthreads();
     double *P = Us;  // row-major 64 x 64 (stride 64) in the operand buffer
 #pragma unroll
@@ -2092,6 +2110,31 @@ template <typename R> void launch_pmat_host(const PmatArgs<R> &a, const double *
     hipLaunchKernelGGL(k_pmat_host<R>, dim3(a.n_branch), dim3(256), 0, s, a, Phost);
 }
 
+
+// BUSTED's error sink (busted.hpp:208-216 evaluates V^T P_sink D per
+// pattern): the weight gradient of a sink component is <A_b, P_sink> with the
+// branch's adjoint A_b summed over its pattern chunks in chunk order.
+__global__ __launch_bounds__(256) void k_sink(SinkArgs a) {
+    const int tid = threadIdx.x, slot = a.slots[blockIdx.x], bi = slot / a.max_comp;
+    __shared__ double red[24];
+    const double *src = a.Apart + (size_t)bi * a.n_chunks * 4096 + tid;
+    double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        double v = src[256 * k];
+        for (int c = 1; c < a.n_chunks; ++c) v += src[(size_t)c * 4096 + 256 * k];
+        d0 += v * a.sink[tid + 256 * k];
+    }
+    block_sum2<256>(d0, d1, red, tid);
+    if (tid == 0) {
+        a.out[(size_t)slot * 8] = d0;
+        a.out[(size_t)slot * 8 + 1] = 0.0;
+    }
+}
+
+void launch_sink(const SinkArgs &a, int n_slots, hipStream_t s) {
+    hipLaunchKernelGGL(k_sink, dim3(n_slots), dim3(256), 0, s, a);
+}
 
 void launch_frechet(const FrechetArgs &a, hipStream_t s) {
     if (a.grp_size > 1)

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "libhyphy_mi355x.so")
 HY_OK = 0
 HY_PREC_FP64 = 0
 HY_PREC_FP32 = 1
+HY_SINK = -1  # matrix index naming BUSTED's error sink in set_mixture classes
 HY_TIME_RAW = 0
 HY_TIME_CONV = 1
 HY_WANT_LOGL = 1
@@ -28,7 +29,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
-           "hy_autotune_batch")
+           "hy_autotune_batch", "hy_set_sink")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -73,6 +74,7 @@ def load_library(path: str = LIB_PATH):
                           C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.hy_loglik_from_P.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
+    L.hy_set_sink.argtypes = [C.c_void_p, _f64p]
     L.hy_last_timings.argtypes = [C.c_void_p, _f64p]
     L.hy_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
     L.hy_set_warm_start.argtypes = [C.c_void_p, C.c_int]
@@ -163,7 +165,8 @@ class Engine:
         self._check(self.L.hy_set_model(self.h, code_id, _p(pos, _f64p), _p(pi, _f64p)))
 
     def set_mixture(self, n_mat, branch_class, classes, convention=HY_TIME_RAW):
-        """classes: list (per class) of lists of matrix indices."""
+        """classes: list (per class) of lists of matrix indices; HY_SINK names
+        the error sink (set_sink), whose weight multiplies P_sink."""
         cp, cm = [0], []
         for comps in classes:
             cm.extend(int(m) for m in comps)
@@ -175,6 +178,11 @@ class Engine:
                                           _p(cm, _i32p), convention))
         self.n_mat, self.n_comp = n_mat, len(cm)
         self._class_sizes = [len(c) for c in classes]
+
+    def set_sink(self, freqs):
+        """Error-sink matrix from the codon frequencies (busted.hpp:193-206)."""
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        self._check(self.L.hy_set_sink(self.h, _p(f, _f64p)))
 
     def set_tile(self, T):
         self._check(self.L.hy_set_tile(self.h, int(T)))

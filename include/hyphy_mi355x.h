@@ -54,6 +54,8 @@ extern "C" {
 #define HY_PREC_FP32 1 /* FP32 MFMA contractions, FP64 spectra and reductions  */
 
 /* Branch time convention (see hy_set_mixture). */
+/* comp_mat entry naming BUSTED's error sink (see hy_set_mixture, hy_set_sink). */
+#define HY_SINK (-1)
 #define HY_TIME_RAW 0  /* tau_b = bl_b                                          */
 #define HY_TIME_CONV 1 /* tau_b = bl_b * 3 / mubar_class                        */
 
@@ -109,9 +111,21 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
  *   matrix comp_mat[j] with weight comp_w[j] (weights given at eval time):
  *       P_b = sum_j comp_w[j] * exp(Q_{comp_mat[j]} tau_b)
  *   time_convention HY_TIME_RAW or HY_TIME_CONV (mubar_c = sum_j w_j mu_j).
+ *   comp_mat[j] == HY_SINK makes component j BUSTED's error sink
+ *   (busted.hpp:571-577): P_b gains comp_w[j] * P_sink, independent of tau_b
+ *   (matrix from hy_set_sink; HY_TIME_RAW only; not in gene batches).  The
+ *   reference's (1 - eps) sum_k w_k P_k + eps P_sink is comp_w = ((1 - eps)
+ *   w_k, eps); out_grad_w of the sink component is sum_b <A_b, P_sink>.
  * Default after hy_set_problem: one matrix, one class, one component, RAW. */
 int hy_set_mixture(hy_engine h, int n_mat, int n_class, const int32_t *branch_class, const int32_t *comp_ptr,
                    const int32_t *comp_mat, int time_convention);
+
+/* Error-sink matrix P_sink(i, j) = f_j / max(1e-12, 1 - f_i) for i != j, zero
+ * diagonal, from the codon frequencies f[S] (the reference builds it from the
+ * alignment's F3x4 codon frequencies: BUSTEDAnalyzer::build_sink_kernel,
+ * busted.hpp:193-206).  Needed before evaluating a mixture with HY_SINK
+ * components. */
+int hy_set_sink(hy_engine h, const double *freqs);
 
 /* Evaluate lnL (and gradients) at the given parameters.
  *   bl[num_nodes]        branch lengths (root entry ignored)

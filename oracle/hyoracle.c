@@ -925,12 +925,32 @@ int or_mixture_eval(const or_tree *tr, const or_code *code, const or_mg94_params
             return -2;
         }
     }
+    /* Error sink (busted.hpp:193-206): only with raw time, whose branch
+     * lengths do not depend on the components' rates. */
+    double *Psink = NULL;
+    for (int j = 0; j < ncomp; ++j)
+        if (mix->comp_mat[j] < 0) {
+            if (mix->comp_mat[j] != -1 || mix->convention != 0 || !mix->sink_freqs) {
+                free(Q); free(ev); free(mu); free(pm);
+                return -3;
+            }
+            if (!Psink) {
+                Psink = (double *)calloc(SS, sizeof(double));
+                for (int i = 0; i < S; ++i) {
+                    const double d = 1.0 - mix->sink_freqs[i];
+                    const double inv = 1.0 / (d > 1e-12 ? d : 1e-12);
+                    for (int k = 0; k < S; ++k)
+                        if (k != i) Psink[(size_t)i * S + k] = mix->sink_freqs[k] * inv;
+                }
+            }
+        }
     /* 2. Per-class time conversion. */
     double *mubar = (double *)malloc(sizeof(double) * NC * 2);
     double *conv = mubar + NC;
     for (int c = 0; c < NC; ++c) {
         double mb = 0.0;
-        for (int j = mix->comp_ptr[c]; j < mix->comp_ptr[c + 1]; ++j) mb += mix->comp_w[j] * mu[mix->comp_mat[j]];
+        for (int j = mix->comp_ptr[c]; j < mix->comp_ptr[c + 1]; ++j)
+            if (mix->comp_mat[j] >= 0) mb += mix->comp_w[j] * mu[mix->comp_mat[j]];
         mubar[c] = mb > 1e-12 ? mb : 1e-12;
         conv[c] = mix->convention == 1 ? 3.0 / mubar[c] : 1.0;
     }
@@ -950,6 +970,10 @@ int or_mixture_eval(const or_tree *tr, const or_code *code, const or_mg94_params
         for (int j = mix->comp_ptr[c]; j < mix->comp_ptr[c + 1]; ++j) {
             const int m = mix->comp_mat[j];
             const double w = mix->comp_w[j];
+            if (m < 0) { /* the sink: w * P_sink, no time derivative (busted.hpp:575-576) */
+                for (size_t i = 0; i < SS; ++i) Pn[i] += w * Psink[i];
+                continue;
+            }
             or_transition(S, ev + S * m, U + SS * m, Vm + SS * m, tau[n], Pk);
             for (size_t i = 0; i < SS; ++i) Pn[i] += w * Pk[i];
             const double *Qm = Q + SS * m;
@@ -1056,6 +1080,12 @@ int or_mixture_eval(const or_tree *tr, const or_code *code, const or_mg94_params
             for (int j = mix->comp_ptr[c]; j < mix->comp_ptr[c + 1]; ++j) {
                 const int m = mix->comp_mat[j];
                 const double wj = mix->comp_w[j];
+                if (m < 0) { /* d/dw_sink = <A_b, P_sink> */
+                    double s_w = 0.0;
+                    for (size_t i = 0; i < SS; ++i) s_w += An[i] * Psink[i];
+                    gw[j] += s_w;
+                    continue;
+                }
                 const double *Um = U + SS * m, *Vp = Vm + SS * m, *lam = ev + S * m;
                 /* d/dw_j = <A_b, P_m(tau_b)> */
                 or_transition(S, lam, Um, Vp, tb, Pm);
@@ -1150,6 +1180,7 @@ int or_mixture_eval(const or_tree *tr, const or_code *code, const or_mg94_params
         free(Athr);
     }
 done:
+    free(Psink);
     free(pll);
     free(tau);
     free(dP);

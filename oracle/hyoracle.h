@@ -81,6 +81,11 @@ typedef struct {
     const int32_t *comp_ptr;     /* [n_class+1]             */
     const int32_t *comp_mat;     /* [comp_ptr[n_class]]     */
     const double *comp_w;        /* [comp_ptr[n_class]]     */
+    /* BUSTED's error sink (busted.hpp:193-206, 571-576): a component with
+     * comp_mat[j] == -1 contributes comp_w[j] * P_sink, the time-independent
+     * P_sink(i, j) = f_j / max(1e-12, 1 - f_i) (i != j, zero diagonal) of the
+     * codon frequencies f = sink_freqs [S]; needs convention 0. */
+    const double *sink_freqs;
 } or_mixture;
 
 /* Outputs of or_mixture_eval. Arrays may be NULL when not wanted. */

@@ -45,7 +45,7 @@ class OrParams(C.Structure):
 class OrMixture(C.Structure):
     _fields_ = [("n_mat", C.c_int32), ("n_class", C.c_int32), ("convention", C.c_int32), ("pad0", C.c_int32),
                 ("omegas", _f64p), ("branch_class", _i32p), ("comp_ptr", _i32p), ("comp_mat", _i32p),
-                ("comp_w", _f64p)]
+                ("comp_w", _f64p), ("sink_freqs", _f64p)]
 
 
 class OrMixtureOut(C.Structure):
@@ -279,8 +279,9 @@ def joint_frechet(prob, code, p, pos, pi, bl, is_test=None, omega_bg=-1.0, threa
 class Mixture:
     """Branch-site mixture specification mirrored by the GPU library."""
 
-    def __init__(self, omegas, branch_class, classes, convention=0):
-        # classes: list of lists of (matrix index, weight)
+    def __init__(self, omegas, branch_class, classes, convention=0, sink_freqs=None):
+        # classes: list of lists of (matrix index, weight); matrix index -1 is
+        # BUSTED's error sink built from sink_freqs (busted.hpp:193-206)
         self.omegas = f64(omegas)
         self.branch_class = np.ascontiguousarray(branch_class, dtype=np.int32)
         cp = [0]
@@ -303,6 +304,8 @@ class Mixture:
         m.comp_ptr = ptr(self.comp_ptr, _i32p)
         m.comp_mat = ptr(self.comp_mat, _i32p)
         m.comp_w = ptr(self.comp_w, _f64p)
+        self.sink_freqs = None if sink_freqs is None else f64(sink_freqs)
+        m.sink_freqs = None if sink_freqs is None else ptr(self.sink_freqs, _f64p)
         self.m = m
         self.n_comp = len(cm)
 

@@ -1,0 +1,74 @@
+"""BUSTED's error sink on the MI355X engine (GPU) against the CPU oracle.
+
+Test branches mix omega components with the sink, (1 - eps) sum_k w_k P_k +
+eps P_sink (busted.hpp:571-577), background branches use one matrix; logL and
+the full gradient (including the sink weight's) in both precisions and every
+tile width, plus the argument checks of the sink's API."""
+import numpy as np
+import pytest
+
+from helpers import branch_lengths, grad_err, make_engine, rel
+from oracle import data_ref as dr
+from oracle import hyoracle as ho
+from test_gpu_sweep import random_alignment, random_newick
+from hyphy3.engine import HY_SINK, HY_TIME_CONV, HyError
+
+pytestmark = pytest.mark.gpu
+
+S = len(dr.Code().sense_codons)
+
+
+@pytest.mark.parametrize("seed,n,L,K", [(0, 12, 40, 2), (1, 31, 97, 3), (2, 48, 130, 1)])
+def test_sink_mixture_parity(seed, n, L, K):
+    rng = np.random.default_rng(500 + seed)
+    tree = dr.TreeRef.from_newick(random_newick(rng, n))
+    aln = dr.AlignmentRef(["T%d" % i for i in range(n)], random_alignment(rng, n, L, gaps=0.05 * seed))
+    N = len(tree.nodes)
+    bc = (rng.uniform(size=N) < 0.4).astype(np.int32)
+    omegas = [0.7] + list(rng.uniform(0.05, 4.0, size=K))
+    w = rng.dirichlet(np.ones(K))
+    eps = 0.03 + 0.05 * seed
+    f = rng.dirichlet(np.ones(S))
+    theta = rng.uniform(0.5, 2.0, size=5)
+    comp_w = np.concatenate([[1.0], (1.0 - eps) * w, [eps]])
+    classes = [[(0, 1.0)], [(1 + k, comp_w[1 + k]) for k in range(K)] + [(-1, eps)]]
+    mix = ho.Mixture(omegas, bc, classes, 0, sink_freqs=f)
+    bl = branch_lengths(tree)
+    ref = ho.mixture_eval(ho.Problem.from_refs(tree, aln), ho.Code(0), ho.params(1.0, 1.0, theta), aln.pos_freqs,
+                          aln.cf3x4, mix, bl)
+    eng = make_engine(tree, aln)
+    try:
+        eng.set_mixture(K + 1, bc, [[0], list(range(1, K + 1)) + [HY_SINK]])
+        eng.set_sink(f)
+        for T in (16, 32, 64):
+            eng.set_tile(T)
+            for prec, tl, tg in ((0, 1e-9, 1e-7), (1, 1e-5, 1e-4)):
+                r = eng.eval(bl, omegas, comp_w, theta, precision=prec)
+                msg = (seed, n, L, K, T, prec)
+                assert rel(r["log_likelihood"], ref["log_likelihood"]) < tl, msg
+                for k in ("grad_bl", "grad_omega", "grad_w", "grad_theta"):
+                    assert grad_err(r[k], ref[k]) < tg, msg + (k,)
+    finally:
+        eng.close()
+
+
+def test_sink_api_checks():
+    rng = np.random.default_rng(7)
+    n, L = 6, 10
+    tree = dr.TreeRef.from_newick(random_newick(rng, n))
+    aln = dr.AlignmentRef(["T%d" % i for i in range(n)], random_alignment(rng, n, L, gaps=0.0))
+    N = len(tree.nodes)
+    eng = make_engine(tree, aln)
+    try:
+        with pytest.raises(HyError):
+            eng.set_mixture(1, np.zeros(N, dtype=np.int32), [[0, HY_SINK]], convention=HY_TIME_CONV)
+        eng.set_mixture(1, np.zeros(N, dtype=np.int32), [[0, HY_SINK]])
+        with pytest.raises(HyError):  # no sink matrix yet
+            eng.eval(branch_lengths(tree), [0.5], [0.9, 0.1], np.ones(5))
+        with pytest.raises(HyError):
+            eng.set_sink(np.full(S, 2.0))
+        eng.set_sink(np.full(S, 1.0 / S))
+        r = eng.eval(branch_lengths(tree), [0.5], [0.9, 0.1], np.ones(5))
+        assert np.isfinite(r["log_likelihood"])
+    finally:
+        eng.close()
