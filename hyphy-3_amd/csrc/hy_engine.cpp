@@ -1065,6 +1065,8 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
     TRY(upload(h, h->d_qdesc, qdesc.data(), qdesc.size() * 4));
     TRY(upload(h, h->d_pos, pos_freqs, 12 * sizeof(double)));
     TRY(upload(h, h->d_pi, pp.data(), 64 * sizeof(double)));
+    // a sink matrix is indexed by the sense codons of the code it was built for
+    if (h->have_model && code_id != h->code_id) h->sink_set = false;
     h->code_id = code_id;
     h->S = S;
     h->pos_host.assign(pos_freqs, pos_freqs + 12);
@@ -1272,6 +1274,12 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     h->convention = HY_TIME_RAW;
     h->comp_ptr = {0, 1};
     h->comp_mat = {0};
+    // the default mixture has no error sink, and a sink matrix of the previous
+    // problem (other S or code) must not be reused: hy_set_sink is needed again
+    h->has_sink = false;
+    h->sink_set = false;
+    h->sink_wl.clear();
+    ++h->epoch;
     refresh_blist_classes(h);
     h->warm_valid = false;
     TRY(upload(h, h->d_blist, h->blist.data(), std::max<size_t>(h->blist.size(), BR_STRIDE) * 4));

@@ -541,7 +541,8 @@ __device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, double *
     double conv = 1.0;
     if (a.convention == 1) {
         double mb = 0.0;
-        for (int j = j0; j < j1; ++j) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
+        for (int j = j0; j < j1; ++j)
+            if (a.comp_mat[j] >= 0) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];  // sink: no mu
         if (mb < 1e-12) mb = 1e-12;
         conv = 3.0 / mb;
     }
@@ -1933,7 +1934,9 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
     } else {
         if (gt < a.n_class) {
             double mb = 0.0;
-            for (int j = a.comp_ptr[gt]; j < a.comp_ptr[gt + 1]; ++j) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
+            // error-sink components (comp_mat < 0) have no rate matrix and no mu
+            for (int j = a.comp_ptr[gt]; j < a.comp_ptr[gt + 1]; ++j)
+                if (a.comp_mat[j] >= 0) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];
             mubar[gt] = mb < 1e-12 ? 1e-12 : mb;
         }
         for (int n = gt; n < NR; n += FINAL_NT - NL) gbl[n] = 0.0;
@@ -2008,7 +2011,7 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
                         const double f = sum_tau[cl] / mubar[cl];
                         for (int j = a.comp_ptr[cl]; j < a.comp_ptr[cl + 1]; ++j) {
                             const int mm = a.comp_mat[j];
-                            if (is_om && mm != mat0 + t) continue;
+                            if (mm < 0 || (is_om && mm != mat0 + t)) continue;
                             v -= f * prm[a.pl.off_w + j] * a.dmu[mm * 6 + phi];
                         }
                     }
@@ -2027,7 +2030,7 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
                 if (a.blist[bi * BR_STRIDE + 3] == cls0 + cls) v += a.fr[((size_t)bi * MC + jj) * 8 + 0];
             v = wave_sum(v);
             if (lane == 0) {
-                if (a.convention == 1) v -= sum_tau[cls] / mubar[cls] * a.mu[a.comp_mat[Jg]];
+                if (a.convention == 1 && a.comp_mat[Jg] >= 0) v -= sum_tau[cls] / mubar[cls] * a.mu[a.comp_mat[Jg]];
                 gw[J] = v;
             }
         }

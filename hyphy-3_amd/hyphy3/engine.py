@@ -158,10 +158,23 @@ class Engine:
                                           _p(mk, _u64p), _p(w, _f64p), S))
         self.num_nodes, self.num_patterns = n, P
         self.n_mat, self.n_comp = 1, 1
+        self._class_sizes = [1]  # hy_set_problem restores the one-component mixture
+        self.S = int(S)
+
+    def _check_states(self, name, a):
+        # the C side reads exactly S values (one per sense codon)
+        S = getattr(self, "S", None)
+        if S is not None and a.shape != (S,):
+            raise ValueError("%s: expected %d codon frequencies (one per sense codon), got shape %s"
+                             % (name, S, a.shape))
 
     def set_model(self, code_id, pos_freqs, pi):
-        pos = _f64(pos_freqs).reshape(12)
+        pos = _f64(pos_freqs)
+        if pos.size != 12:
+            raise ValueError("set_model: pos_freqs must hold 3 x 4 position frequencies, got %d" % pos.size)
+        pos = pos.reshape(12)
         pi = _f64(pi)
+        self._check_states("set_model", pi)
         self._check(self.L.hy_set_model(self.h, code_id, _p(pos, _f64p), _p(pi, _f64p)))
 
     def set_mixture(self, n_mat, branch_class, classes, convention=HY_TIME_RAW):
@@ -182,6 +195,7 @@ class Engine:
     def set_sink(self, freqs):
         """Error-sink matrix from the codon frequencies (busted.hpp:193-206)."""
         f = np.ascontiguousarray(freqs, dtype=np.float64)
+        self._check_states("set_sink", f)
         self._check(self.L.hy_set_sink(self.h, _p(f, _f64p)))
 
     def set_tile(self, T):
