@@ -102,7 +102,21 @@ PYBIND11_MODULE(_hyphy3, m) {
         .def_readonly("num_sense_codons", &GeneticCode::num_sense_codons)
         .def_readonly("sense_codons", &GeneticCode::sense_codons)
         .def_readonly("stop_codons", &GeneticCode::stop_codons)
-        .def("is_synonymous", &GeneticCode::is_synonymous);
+        .def("is_synonymous", &GeneticCode::is_synonymous)
+        .def_static("from_id", [](int id) { return std::const_pointer_cast<GeneticCode>(GeneticCode::from_id(id)); },
+                    py::arg("id"))
+        // amino-acid letter of every codon in ACGT order ('*' = stop)
+        .def_property_readonly("codon_aa64", [](const GeneticCode &g) { return std::string(g.codon_aa64, 64); })
+        // diff_matrix entry of sense pair (u, v): (num_diffs, synonymous,
+        // diff_pos[3], diff_from[3], diff_to[3])
+        .def("diff", [](const GeneticCode &g, int u, int v) {
+            if (u < 0 || v < 0 || u >= g.num_sense_codons || v >= g.num_sense_codons)
+                throw py::index_error("sense codon index out of range");
+            const auto &d = g.diff_matrix[u][v];
+            auto tri = [](const int8_t *x) { return py::make_tuple(int(x[0]), int(x[1]), int(x[2])); };
+            return py::make_tuple(int(d.num_diffs), bool(d.synonymous), tri(d.diff_pos), tri(d.diff_from),
+                                  tri(d.diff_to));
+        }, py::arg("u"), py::arg("v"));
 
     py::class_<TreeNode>(m, "TreeNode")
         .def_readonly("id", &TreeNode::id)

@@ -155,6 +155,88 @@ def test_q_invariants(k):
                 assert abs(Q[i, j] - expect) < 1e-14 * max(1.0, expect)
 
 
+def q_from_strings(alpha, beta, th, delta, psi, pos):
+    """MG94 generator built from the codon strings alone (rate_matrix.hpp:
+    190-223): an entry for every pair differing at n positions is
+    alpha|beta (synonymous or not, from the standard amino-acid table) times
+    1, delta or psi for n = 1, 2, 3, times the product over the differing
+    positions q of theta(from, to) * pi_q(to); theta_AG = 1; diagonal minus
+    the row sum."""
+    cod = dr.Code().sense_codons
+    aa = dr.Code().aa64
+    idx = lambda c: 16 * "ACGT".index(c[0]) + 4 * "ACGT".index(c[1]) + "ACGT".index(c[2])  # noqa: E731
+    pair = {(0, 1): th[0], (0, 3): th[1], (1, 2): th[2], (1, 3): th[3], (2, 3): th[4], (0, 2): 1.0}
+    hit = {1: 1.0, 2: delta, 3: psi}
+    S = len(cod)
+    Q = np.zeros((S, S))
+    for i in range(S):
+        for j in range(S):
+            diff = [q for q in range(3) if cod[i][q] != cod[j][q]]
+            if not diff:
+                continue
+            v = (alpha if aa[idx(cod[i])] == aa[idx(cod[j])] else beta) * hit[len(diff)]
+            for q in diff:
+                a, b = "ACGT".index(cod[i][q]), "ACGT".index(cod[j][q])
+                v *= pair[(min(a, b), max(a, b))] * pos[q, b]
+            Q[i, j] = v
+        Q[i, i] = -Q[i].sum()
+    return Q
+
+
+@pytest.mark.parametrize("delta,psi", [(0.05, 0.0), (0.0, 0.02), (0.3, 0.07)])
+def test_q_multihit_entries(delta, psi):
+    """Two- and three-nucleotide entries (rate_matrix.hpp:205-219), the
+    diagonal and mu against the string-built generator."""
+    rng = np.random.default_rng(17)
+    pos = rng.dirichlet([5, 5, 5, 5], size=3)
+    nucs = [((c >> 4) & 3, (c >> 2) & 3, c & 3) for c in dr.Code().sense]
+    pi = np.array([pos[0, a] * pos[1, b] * pos[2, c] for a, b, c in nucs])
+    pi /= pi.sum()
+    alpha, beta, th = 1.3, 0.4, (1.5, 0.5, 2.0, 0.7, 1.1)
+    Q, mu = ho.mg94_Q(CODE, ho.params(alpha, beta, th, delta=delta, psi=psi), pos, pi)
+    Qs = q_from_strings(alpha, beta, th, delta, psi, pos)
+    np.testing.assert_allclose(Q, Qs, rtol=1e-14, atol=1e-17)
+    assert abs(mu + (pi * np.diag(Qs)).sum()) < 1e-14
+    cod = dr.Code().sense_codons
+    n2 = sum(1 for i in range(61) for j in range(61) if sum(a != b for a, b in zip(cod[i], cod[j])) == 2)
+    n3 = sum(1 for i in range(61) for j in range(61) if sum(a != b for a, b in zip(cod[i], cod[j])) == 3)
+    assert (np.count_nonzero(Q) - 61 == 61 * 60 - (0 if delta else n2) - (0 if psi else n3))
+
+
+def test_multihit_loglik_brute_force():
+    """lnL with delta, psi > 0 on a four-taxon tree against the sum over all
+    ancestral states with P = exp(Q t) of the string-built generator
+    (independent Taylor exponential)."""
+    nwk = "((A:0.21,B:0.13):0.17,C:0.31,D:0.45);"
+    seqs = {"A": "ATGAAACCCGGGTTTCAT", "B": "CTGAAGCCAGGACTTGAT", "C": "ATAAGACGCGGTTTCGCT",
+            "D": "TTGCAACCTGCATTTGCA"}
+    aln = dr.AlignmentRef(list(seqs), list(seqs.values()))
+    tree = dr.TreeRef.from_newick(nwk)
+    prob = ho.Problem.from_refs(tree, aln)
+    alpha, beta, th, delta, psi = 1.1, 0.6, (1.2, 0.8, 0.9, 2.5, 1.3), 0.25, 0.08
+    p = ho.params(alpha, beta, th, delta=delta, psi=psi)
+    bl = branch_lengths(tree)
+    ll, site = ho.mg94_loglik(prob, CODE, p, aln.pos_freqs, aln.cf3x4, bl, site=True)
+    Qs = q_from_strings(alpha, beta, th, delta, psi, aln.pos_freqs)
+    P = {n.id: ho.expm_taylor(Qs, n.bl) for n in tree.nodes if n.id != tree.root}
+    l2t = tree.leaf_to_taxon(aln.names)
+    root = tree.nodes[tree.root]
+    u = [c for c in root.children if not tree.nodes[c].is_leaf][0]
+    lu, lr = tree.nodes[u].children, [c for c in root.children if c != u]
+    tot = 0.0
+    for k in range(aln.num_patterns):
+        x = {n: tip_vec(aln.states[k, l2t[n]], aln.pmasks[k, l2t[n]]) for n in lu + lr}
+        a_u = (P[lu[0]] @ x[lu[0]]) * (P[lu[1]] @ x[lu[1]])
+        c_r = (P[lr[0]] @ x[lr[0]]) * (P[lr[1]] @ x[lr[1]])
+        L = math.fsum((aln.cf3x4[:, None] * c_r[:, None] * P[u] * a_u[None, :]).ravel())
+        assert abs(math.log(L) - site[k]) < 1e-11
+        tot += aln.weights[k] * math.log(L)
+    assert abs(tot - ll) < 1e-10
+    # the multi-hit terms matter here: without them lnL differs
+    ll0 = ho.mg94_loglik(prob, CODE, ho.params(alpha, beta, th), aln.pos_freqs, aln.cf3x4, bl)
+    assert abs(ll0 - ll) > 1e-3
+
+
 @pytest.mark.parametrize("k", range(len(PARAM_SETS)))
 def test_transition_invariants(k):
     Q, mu, pi, pos, p = model(PARAM_SETS[k], k)
