@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -113,6 +114,18 @@ struct hy_engine_s {
     // eagerly (graphs bake the weight pointer in).
     DevBuf d_wover;
     const double *wover = nullptr;
+    // hy_loglik_from_P_noalloc: device address of the fixed scratch on this
+    // handle's device (looked up at creation; nullptr with the HIP error in
+    // scratch_err when that failed) and the host layouts of its fixed
+    // configuration (16-pattern tiles, one whole-tree inside schedule), built
+    // by hy_set_problem: tip codes [taxa + 1][na_P_pad], padded pattern
+    // weights, the inside schedule and the identity edges' nodes
+    void *scratch = nullptr;
+    hipError_t scratch_err = hipSuccess;
+    std::vector<uint16_t> na_tips;
+    std::vector<double> na_weights;
+    std::vector<int32_t> na_sched, na_ident;
+    int na_P_pad = 0;
     double *h_site = nullptr;  // pinned staging of per-pattern lnL
     size_t h_site_cap = 0;
     // Partials left by the last evaluation (read back by hy_read_partials):
@@ -283,6 +296,12 @@ void build_layout(hy_engine h) {
     pl.total = o;
 }
 
+// Inside / outside schedules and subtree parts of one tiling (see make_schedules).
+struct Schedules {
+    std::vector<int32_t> sched, osched, parts;
+    int n_parts = 0, n_uspill = 0;
+};
+void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean, bool lean_out, Schedules &o);
 void build_schedules(hy_engine h, int T, int esz, bool split);
 
 // Identity P of the artificial nodes in the Prow (with row sums), PA/PTA
@@ -468,8 +487,8 @@ int prepare_buffers(hy_engine h, int prec) {
 // (and its own LDS slots), subtree parts are walked by separate workgroups
 // and exchange their results with the top part through HBM (inside: the
 // part root's partials; outside: the message into the part root, spilled).
-void build_schedules(hy_engine h, int T, int esz, bool split) {
-    const int NSI = inside_slots(T, esz, h->lean), NSO = outside_slots(T, esz, h->lean_out);
+void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean, bool lean_out, Schedules &o) {
+    const int NSI = inside_slots(T, esz, lean), NSO = outside_slots(T, esz, lean_out);
     const auto &ch = h->ext_children;
     const int n_ext = (int)ch.size();
     auto ref_of = [&](int c) -> int32_t {
@@ -524,15 +543,15 @@ void build_schedules(hy_engine h, int T, int esz, bool split) {
     // post-order of every part (top last), filtered from the whole-tree order
     std::vector<std::vector<int>> porder(np + 1);
     for (int u : h->order) porder[part_of[u] < 0 ? np : part_of[u]].push_back(u);
-    h->parts.assign((size_t)(np + 1) * 4, 0);
+    o.parts.assign((size_t)(np + 1) * 4, 0);
     // ---- inside: free-list LDS slots, spill to HBM when none is free ----
     std::vector<int32_t> where(n_ext, -1);  // LDS slot of a computed internal node (-1: HBM)
-    h->sched.clear();
+    o.sched.clear();
     for (int p = 0; p <= np; ++p) {
         std::vector<int> free_slots;
         for (int sl = NSI - 1; sl >= 0; --sl) free_slots.push_back(sl);
-        h->parts[p * 4 + 0] = (int32_t)(h->sched.size() / SCHED_STRIDE);
-        h->parts[p * 4 + 1] = (int32_t)porder[p].size();
+        o.parts[p * 4 + 0] = (int32_t)(o.sched.size() / SCHED_STRIDE);
+        o.parts[p * 4 + 1] = (int32_t)porder[p].size();
         for (int u : porder[p]) {
             int32_t e[SCHED_STRIDE] = {0};
             e[0] = u;
@@ -561,21 +580,21 @@ void build_schedules(hy_engine h, int T, int esz, bool split) {
             }
             where[u] = out;
             e[3] = out;
-            h->sched.insert(h->sched.end(), e, e + SCHED_STRIDE);
+            o.sched.insert(o.sched.end(), e, e + SCHED_STRIDE);
         }
     }
     // ---- outside: top part first, then the subtree parts, each in reverse
     // order; pending U messages in LDS slots, into a part root through HBM
     std::vector<int32_t> usrc(n_ext, U_SPILL);
     usrc[h->ext_root] = U_ROOT;
-    h->osched.clear();
+    o.osched.clear();
     int spills = 0;
     for (int q = 0; q <= np; ++q) {
         const int p = (q == 0) ? np : q - 1;
         std::vector<int> free_slots;
         for (int sl = NSO - 1; sl >= 0; --sl) free_slots.push_back(sl);
-        h->parts[p * 4 + 2] = (int32_t)(h->osched.size() / OSCHED_STRIDE);
-        h->parts[p * 4 + 3] = (int32_t)porder[p].size();
+        o.parts[p * 4 + 2] = (int32_t)(o.osched.size() / OSCHED_STRIDE);
+        o.parts[p * 4 + 3] = (int32_t)porder[p].size();
         const std::vector<int> &ord = porder[p];
         const int n_order = (int)ord.size();
         for (int i = n_order - 1; i >= 0; --i) {
@@ -616,13 +635,24 @@ void build_schedules(hy_engine h, int T, int esz, bool split) {
                 }
                 e[9 + 4 * c] = dst;
             }
-            h->osched.insert(h->osched.end(), e, e + OSCHED_STRIDE);
+            o.osched.insert(o.osched.end(), e, e + OSCHED_STRIDE);
         }
     }
-    h->n_parts = np;
-    h->n_uspill = spills;
+    o.n_parts = np;
+    o.n_uspill = spills;
+}
+
+void build_schedules(hy_engine h, int T, int esz, bool split) {
+    Schedules o;
+    make_schedules(h, T, esz, split, h->lean, h->lean_out, o);
+    h->sched = std::move(o.sched);
+    h->osched = std::move(o.osched);
+    h->parts = std::move(o.parts);
+    h->n_parts = o.n_parts;
+    h->n_uspill = o.n_uspill;
     h->sched_key = T * 16 + esz + (h->lean ? 4096 : 0) + (split ? 8192 : 0) + (h->lean_out ? 16384 : 0);
 }
+
 
 void refresh_blist_classes(hy_engine h) {
     for (int bi = 0; bi < h->n_branch; ++bi) {
@@ -914,6 +944,9 @@ hy_engine new_engine(int device, int n_cu, const std::shared_ptr<StreamSet> &ss)
     h->no_parts = np && np[0] == '1';
     if (h->profile)
         for (auto &ev : h->ev) (void)hipEventCreate(&ev);
+    // fixed scratch of the allocation-free entry point on this device
+    h->scratch_err = noalloc_scratch(device, &h->scratch);
+    if (h->scratch_err != hipSuccess) h->scratch = nullptr;
     return h;
 }
 
@@ -1287,6 +1320,24 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     h->n_amb = (int)h->amb.size();
     TRY(upload(h, h->d_comp_ptr, h->comp_ptr.data(), h->comp_ptr.size() * 4));
     TRY(upload(h, h->d_comp_mat, h->comp_mat.data(), h->comp_mat.size() * 4));
+    // host layouts of the allocation-free path: 16-pattern tiles, one
+    // whole-tree inside schedule (its LDS slot count is the same in both
+    // precisions)
+    {
+        constexpr int T = 16;
+        h->na_P_pad = (h->P + T - 1) / T * T;
+        h->na_tips.assign((size_t)(h->num_taxa + 1) * h->na_P_pad, TIP_ONES);
+        for (int t = 0; t < h->num_taxa; ++t)
+            std::memcpy(&h->na_tips[(size_t)t * h->na_P_pad], &h->codes[(size_t)t * h->P], sizeof(uint16_t) * h->P);
+        h->na_weights.assign(h->na_P_pad, 0.0);
+        std::copy(h->weights.begin(), h->weights.end(), h->na_weights.begin());
+        Schedules o;
+        make_schedules(h, T, 4, false, false, false, o);
+        h->na_sched = std::move(o.sched);
+        h->na_ident.clear();
+        for (int n = 0; n < h->n_ext; ++n)
+            if (h->ext_ident[n]) h->na_ident.push_back(n);
+    }
     h->have_problem = true;
     return HY_OK;
 }
@@ -2389,6 +2440,169 @@ int hy_inside_outside_from_P(hy_engine h, int precision, const double *P, const 
                              double *out_site_ll) {
     return eval_from_P(h, precision, P, pi, out_logL, out_site_ll, HY_WANT_LOGL | RUN_OUTSIDE,
                        "hy_inside_outside_from_P");
+}
+
+}  // extern "C"
+
+namespace {
+
+// Byte offsets of the allocation-free path's buffers in the fixed scratch.
+struct NoallocLayout {
+    size_t P, pi, tips, w, sched, Prow, PA, PTA, D, E, L, LE, site, res, total;
+};
+
+NoallocLayout noalloc_layout(const hy_engine_s *h, size_t rs) {
+    NoallocLayout o{};
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t off = at;
+        at += (bytes + 255) / 256 * 256;
+        return off;
+    };
+    const size_t Pp = (size_t)h->na_P_pad, ne = (size_t)h->n_ext;
+    o.P = take((size_t)h->num_nodes * h->S * h->S * 8);
+    o.pi = take(64 * 8);
+    o.tips = take(h->na_tips.size() * sizeof(uint16_t));
+    o.w = take(Pp * 8);
+    o.sched = take(h->na_sched.size() * 4);
+    o.Prow = take(ne * PRS * rs);
+    o.PA = take(ne * 4096 * rs);
+    o.PTA = take(ne * 4096 * rs);
+    o.D = take((size_t)h->n_int * Pp * 64 * rs);
+    o.E = take((size_t)h->n_int * Pp * 4);
+    o.L = take(Pp * 8);
+    o.LE = take(Pp * 4);
+    o.site = take(Pp * 8);
+    o.res = take(8);
+    o.total = at;
+    return o;
+}
+
+// Identity edge tiles (polytomy nodes) of one element type in the Prow and
+// PA / PTA layouts, as upload_identity_tiles builds them.
+template <typename R>
+void identity_tiles(int S, std::array<R, PRS> &prow, std::array<R, 4096> &pa) {
+    prow.fill(R(0));
+    pa.fill(R(0));
+    for (int i = 0; i < 64; ++i) prow[i * 64 + i] = R(1);
+    for (int i = 0; i < S; ++i) prow[4096 + i] = R(1);
+    for (int idx = 0; idx < 4096; ++idx) {
+        const int w = idx >> 10, l = (idx >> 4) & 63, kk = idx & 15;
+        if ((16 * w + (l & 15)) == (4 * kk + (l >> 4))) pa[idx] = R(1);
+    }
+}
+
+template <typename R>
+int loglik_noalloc(hy_engine h, const double *P, const double *pi, double *out_logL, double *out_site_ll,
+                   const NoallocLayout &lay) {
+    hipStream_t s = h->stream;
+    char *base = (char *)h->scratch;  // device address from hipGetSymbolAddress
+    auto at = [&](size_t off) { return (void *)(base + off); };
+    // inputs: caller's P and pi, the problem's host layouts
+    HIP_TRY(h, hipMemcpyAsync(at(lay.P), P, (size_t)h->num_nodes * h->S * h->S * 8, hipMemcpyHostToDevice, s));
+    std::array<double, 64> pp{};
+    for (int i = 0; i < h->S; ++i) pp[i] = pi[i];
+    HIP_TRY(h, hipMemcpyAsync(at(lay.pi), pp.data(), sizeof(pp), hipMemcpyHostToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(at(lay.tips), h->na_tips.data(), h->na_tips.size() * sizeof(uint16_t),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(at(lay.w), h->na_weights.data(), h->na_weights.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(at(lay.sched), h->na_sched.data(), h->na_sched.size() * 4, hipMemcpyHostToDevice, s));
+    if (!h->na_ident.empty()) {
+        // the scratch keeps whatever the last call on this device left:
+        // identity tiles are written on every call
+        std::array<R, PRS> prow;
+        std::array<R, 4096> pa;
+        identity_tiles<R>(h->S, prow, pa);
+        for (int n : h->na_ident) {
+            HIP_TRY(h, hipMemcpyAsync((char *)at(lay.Prow) + (size_t)n * PRS * sizeof(R), prow.data(), sizeof(prow),
+                                      hipMemcpyHostToDevice, s));
+            HIP_TRY(h, hipMemcpyAsync((char *)at(lay.PA) + (size_t)n * 4096 * sizeof(R), pa.data(), sizeof(pa),
+                                      hipMemcpyHostToDevice, s));
+            HIP_TRY(h, hipMemcpyAsync((char *)at(lay.PTA) + (size_t)n * 4096 * sizeof(R), pa.data(), sizeof(pa),
+                                      hipMemcpyHostToDevice, s));
+        }
+        // (host arrays are read by the copies before this function returns)
+        HIP_TRY(h, hipStreamSynchronize(s));
+    }
+    // P tiles of every branch
+    PmatArgs<R> pa{};
+    pa.blist = (const int32_t *)h->d_blist.p;
+    pa.n_branch = h->n_branch;
+    pa.S = h->S;
+    pa.Prow = (R *)at(lay.Prow);
+    pa.PA = (R *)at(lay.PA);
+    pa.PTA = (R *)at(lay.PTA);
+    if (h->n_branch > 0) launch_pmat_host<R>(pa, (const double *)at(lay.P), s);
+    // inside pass over the whole tree, one launch of 16-pattern tiles; every
+    // D, E, L, LE and site entry read later is written here first
+    PruneArgs<R> pr{};
+    pr.sched = (const int32_t *)at(lay.sched);
+    pr.n_sched = (int)(h->na_sched.size() / SCHED_STRIDE);
+    pr.tip_rows = h->num_taxa + 1;
+    pr.S = h->S;
+    pr.PA = (const R *)at(lay.PA);
+    pr.PTA = (const R *)at(lay.PTA);
+    pr.Prow = (const R *)at(lay.Prow);
+    pr.tips = (const uint16_t *)at(lay.tips);
+    pr.amb = (const uint64_t *)h->d_amb.p;
+    pr.P_pad = h->na_P_pad;
+    pr.n_tiles = h->na_P_pad / 16;
+    pr.n_patterns = h->P;
+    pr.pi = (const double *)at(lay.pi);
+    pr.D = (R *)at(lay.D);
+    pr.E = (int32_t *)at(lay.E);
+    pr.Lroot = (double *)at(lay.L);
+    pr.Eroot = (int32_t *)at(lay.LE);
+    pr.site_ll = (double *)at(lay.site);
+    pr.root_out = 1;
+    launch_inside<R>(pr, 16, false, 1, s);
+    // deterministic compensated lnL (no gradient)
+    FinalArgs fn{};
+    fn.site_ll = pr.site_ll;
+    fn.weights = (const double *)at(lay.w);
+    fn.Lroot = pr.Lroot;
+    fn.n_patterns = h->P;
+    fn.want_grad = 0;
+    fn.n_real_nodes = h->num_nodes;
+    fn.res = (double *)at(lay.res);
+    launch_final(fn, 1, s);
+    HIP_TRY(h, hipGetLastError());
+    if (out_logL) HIP_TRY(h, hipMemcpyAsync(out_logL, at(lay.res), 8, hipMemcpyDeviceToHost, s));
+    if (out_site_ll)
+        HIP_TRY(h, hipMemcpyAsync(out_site_ll, at(lay.site), (size_t)h->P * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipStreamSynchronize(s));
+    return HY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hy_loglik_from_P_noalloc(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
+                             double *out_site_ll) {
+    const char *who = "hy_loglik_from_P_noalloc";
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem) return fail(h, HY_ERR_STATE, "%s before hy_set_problem", who);
+    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
+        return fail(h, HY_ERR_ARG, "%s: unknown precision %d", who, precision);
+    if (!P || !pi) return fail(h, HY_ERR_ARG, "%s: NULL P or pi", who);
+    if (!h->scratch)
+        return fail(h, HY_ERR_HIP, "%s: the fixed scratch of device %d is not addressable: %s", who, h->device,
+                    hipGetErrorString(h->scratch_err));
+    if (h->P == 0) {
+        if (out_logL) *out_logL = 0.0;
+        return HY_OK;
+    }
+    const size_t rs = precision == HY_PREC_FP64 ? 8 : 4;
+    const NoallocLayout lay = noalloc_layout(h, rs);
+    if (lay.total > NOALLOC_SCRATCH_BYTES)
+        return fail(h, HY_ERR_ARG,
+                    "%s: this problem (%d internal nodes x %d patterns, %s) needs %zu bytes of scratch, more than "
+                    "the fixed %zu; use hy_loglik_from_P",
+                    who, h->n_int, h->P, rs == 8 ? "FP64" : "FP32", lay.total, (size_t)NOALLOC_SCRATCH_BYTES);
+    HIP_TRY(h, hipSetDevice(h->device));
+    return rs == 8 ? loglik_noalloc<double>(h, P, pi, out_logL, out_site_ll, lay)
+                   : loglik_noalloc<float>(h, P, pi, out_logL, out_site_ll, lay);
 }
 
 int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E, double *V, int32_t *G, double *L,

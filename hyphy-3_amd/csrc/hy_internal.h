@@ -266,4 +266,13 @@ void launch_sink(const SinkArgs &a, int n_slots, hipStream_t s);
 void launch_frechet(const FrechetArgs &a, hipStream_t s);  // chunk sums, spectral transforms, projections
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s);  // one workgroup per gene
 
+// Fixed device scratch of the allocation-free entry point
+// (hy_loglik_from_P_noalloc): a file-scope __device__ array in
+// hy_kernels.hip, NOALLOC_SCRATCH_BYTES per device.  noalloc_scratch returns
+// its device address on `device` (looked up once per device with
+// hipGetSymbolAddress, then cached).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(128) << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+hipError_t noalloc_scratch(int device, void **out);
+
 }  // namespace hy

@@ -24,6 +24,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 
 namespace hy {
 
@@ -2148,6 +2149,40 @@ void launch_frechet(const FrechetArgs &a, hipStream_t s) {
 
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s) {
     hipLaunchKernelGGL(k_final, dim3(n_genes), dim3(FINAL_NT), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed scratch of the allocation-free entry point (hy_loglik_from_P_noalloc):
+// one array of NOALLOC_SCRATCH_BYTES per device that loads this library, held
+// until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the device symbol): the
+// device address comes from hipGetSymbolAddress, once per device, with that
+// device current.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(int device, void **out) {
+    static std::mutex mu;
+    static void *addr[NOALLOC_MAX_DEVICES] = {};
+    if (!out || device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) {
+            const hipError_t e2 = hipSetDevice(cur);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *out = addr[device];
+    return hipSuccess;
 }
 
 template void launch_pmat<float>(const PmatArgs<float> &, const ProjArgs *, int, hipStream_t);

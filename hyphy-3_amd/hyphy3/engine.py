@@ -16,6 +16,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "libhyphy_mi355x.so")
 
 HY_OK = 0
+HY_ERR_ARG = -1
+HY_ERR_STATE = -3
+HY_ERR_NOMEM = -4
+HY_ERR_UNSUPPORTED = -5
 HY_PREC_FP64 = 0
 HY_PREC_FP32 = 1
 HY_SINK = -1  # matrix index naming BUSTED's error sink in set_mixture classes
@@ -29,7 +33,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
-           "hy_autotune_batch", "hy_set_sink")
+           "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -73,6 +77,7 @@ def load_library(path: str = LIB_PATH):
     L.hy_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
                           C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.hy_loglik_from_P.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
+    L.hy_loglik_from_P_noalloc.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.hy_set_sink.argtypes = [C.c_void_p, _f64p]
     L.hy_last_timings.argtypes = [C.c_void_p, _f64p]
@@ -326,6 +331,20 @@ class Engine:
         s = np.zeros(max(self.num_patterns, 1)) if site else None
         self._check(self.L.hy_loglik_from_P(self.h, precision, _p(P, _f64p), _p(pi, _f64p), C.byref(ll),
                                             _p(s, _f64p)))
+        return (ll.value, s[:self.num_patterns]) if site else ll.value
+
+    def loglik_from_P_noalloc(self, P, pi, precision=HY_PREC_FP64, site=False):
+        """loglik_from_P through hy_loglik_from_P_noalloc: no allocation on
+        the call (fixed per-device scratch; calls on one device must not
+        overlap)."""
+        P = _f64(P)
+        pi = _f64(pi)
+        if P.size != self.num_nodes * self.S * self.S or pi.size < self.S:
+            raise ValueError("loglik_from_P_noalloc: P must hold num_nodes x S x S values and pi S values")
+        ll = C.c_double()
+        s = np.zeros(max(self.num_patterns, 1)) if site else None
+        self._check(self.L.hy_loglik_from_P_noalloc(self.h, precision, _p(P, _f64p), _p(pi, _f64p), C.byref(ll),
+                                                    _p(s, _f64p)))
         return (ll.value, s[:self.num_patterns]) if site else ll.value
 
     def autotune(self, bl, omegas, comp_w, theta, alpha=1.0, delta=0.0, psi=0.0, precision=HY_PREC_FP64, grad=True,

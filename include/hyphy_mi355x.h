@@ -219,6 +219,26 @@ int hy_loglik_from_P(hy_engine h, int precision, const double *P, const double *
 int hy_inside_outside_from_P(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
                              double *out_site_ll);
 
+/* As hy_loglik_from_P (same inputs, outputs and result to rounding), but
+ * allocating nothing: no device, pinned or library allocation on any call.
+ * No counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * The problem must be set (hy_set_problem, which allocates, is set-up).
+ * Every per-call device buffer (the caller's P and pi, tip codes, transition
+ * tiles, inside partials and exponents, root and site likelihoods, the
+ * result) lives in one fixed scratch array of 128 MiB per device, part of the
+ * library's code object (held until the process ends), in a fixed
+ * configuration: 16-pattern tiles, one whole-tree inside schedule.  A problem
+ * needs about num_nodes * S^2 * 8 + (internal nodes) * patterns * (256 FP32 |
+ * 512 FP64) bytes; larger ones are refused with HY_ERR_ARG (use
+ * hy_loglik_from_P).  All calls on one device share the array: calls that
+ * overlap on one device (from several threads or handles) race and must be
+ * serialised by the caller.  Every value read is written by the same call,
+ * so nothing carries over from the previous one.  Blocks until the results
+ * are in out_logL / out_site_ll[num_patterns]. */
+int hy_loglik_from_P_noalloc(hy_engine h, int precision, const double *P, const double *pi, double *out_logL,
+                             double *out_site_ll);
+
 /* Partials of the last evaluation on h for patterns [p_start, p_end)
  * (B = p_end - p_start), the read-back behind the reference's workspaces
  * (BatchedInsideWorkspace / BatchedInsideOutsideWorkspace,
