@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -254,16 +255,50 @@ int upload(hy_engine h, DevBuf &b, const void *src, size_t bytes) {
         if (_rc != HY_OK) return _rc; \
     } while (0)
 
+// Instantiated evaluation graphs are bounded process-wide: each holds runtime
+// resources, and a run that instantiated about 3,000 of them (one per gene
+// engine) ended in a host segmentation fault inside the runtime.  At the cap
+// (HY_MAX_GRAPHS, default 512) evaluations launch their kernels directly, the
+// same kernels in the same order.
+std::atomic<int> g_live_graphs{0};
+
+int graph_cap() {
+    const char *e = std::getenv("HY_MAX_GRAPHS");
+    if (!e || !*e) return 512;
+    const int v = std::atoi(e);
+    return v < 0 ? 0 : v;
+}
+
+bool acquire_graph_slot() {
+    const int cap = graph_cap();
+    int cur = g_live_graphs.load();
+    while (cur < cap)
+        if (g_live_graphs.compare_exchange_weak(cur, cur + 1)) return true;
+    return false;
+}
+
+void destroy_graph_exec(hipGraphExec_t &exec) {
+    if (!exec) return;
+    (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    g_live_graphs.fetch_sub(1);
+}
+
 // Pattern-tile width: the pruning kernels are latency-bound per tree node, so
 // the fewest "rounds" of resident workgroups win; a wider tile costs more per
 // node (measured ~1.3x for 32, ~2x for 64).  FP32 tiles of 16 fit two
 // workgroups per CU, every other variant one.
+// FP64 runs tiles of at most 32 patterns: its 64-pattern outside pass needs
+// more registers than a lane has (the compiler spilled 469-627 VGPRs to
+// scratch), so those kernels are not built and a request for 64 runs at 32.
+constexpr int max_tile(int prec) { return prec == HY_PREC_FP64 ? 32 : 64; }
+
 int choose_tile(int P, int req, int prec, int n_cu) {
-    if (req == 16 || req == 32 || req == 64) return req;
+    if (req == 16 || req == 32 || req == 64) return std::min(req, max_tile(prec));
     const double node_cost[3] = {1.0, 1.3, 2.0};
     int best = 16;
     double best_cost = 1e300;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; (16 << k) <= max_tile(prec); ++k) {
         const int T = 16 << k;
         const int slots = n_cu * ((prec == HY_PREC_FP32 && T == 16) ? 2 : 1);
         const int tiles = (P + T - 1) / T;
@@ -686,6 +721,8 @@ void launch_outside_parts(hy_engine h, PruneArgs<R> pr, bool rev, hipStream_t s)
 template <typename R>
 int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     const int ri = sizeof(R) == 8 ? 1 : 0;
+    if (h->T > max_tile(ri ? HY_PREC_FP64 : HY_PREC_FP32))
+        return fail(h, HY_ERR_STATE, "internal: %d-pattern tiles are not built for this precision", h->T);
     const bool grad = (flags & HY_WANT_GRAD) != 0;
     h->part_ri = ri;
     h->part_outside = grad || (flags & RUN_OUTSIDE) != 0;
@@ -1040,8 +1077,7 @@ void hy_destroy(hy_engine h) {
     if (h->h_site) (void)hipHostFree(h->h_site);
     if (h->profile)
         for (auto &ev : h->ev) (void)hipEventDestroy(ev);
-    for (auto &g : h->graphs)
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (auto &g : h->graphs) destroy_graph_exec(g.exec);
     delete h;
 }
 
@@ -1446,16 +1482,19 @@ static int finish_eval(hy_engine h, bool check_status, double *out_logL, double 
 // parameters from the pinned h_params (packed before the launch) and the
 // results and eigensolver status written straight to pinned host memory.  Captures on first use per
 // (precision, gradient, warm-start) and whenever the configuration epoch moved.
+// Replays (capturing first if needed) the evaluation graph of this kind.
+// Returns HY_OK when launched, an error code, or GRAPH_NO_SLOT when the
+// process-wide graph cap is reached: the caller then launches eagerly.
+constexpr int GRAPH_NO_SLOT = 1;
+
 static int launch_graph(hy_engine h, int prec, bool grad) {
     const int warm = (h->warm_enable && h->warm_valid) ? 1 : 0;
     hy_engine_s::GraphEntry *ge = nullptr;
     for (auto &g : h->graphs)
         if (g.prec == prec && g.grad == (int)grad && g.warm == warm) ge = &g;
-    if (ge && ge->epoch != h->epoch) {
-        (void)hipGraphExecDestroy(ge->exec);
-        ge->exec = nullptr;
-    }
+    if (ge && ge->epoch != h->epoch) destroy_graph_exec(ge->exec);
     if (!ge || !ge->exec) {
+        if (!acquire_graph_slot()) return GRAPH_NO_SLOT;
         hipGraph_t graph = nullptr;
         HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
         const unsigned flags = HY_WANT_LOGL | (grad ? HY_WANT_GRAD : 0u);
@@ -1463,21 +1502,24 @@ static int launch_graph(hy_engine h, int prec, bool grad) {
         const int rc = prec == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
         h->zc_params = false;
         const hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
-        if (rc != HY_OK) {
+        if (rc != HY_OK || e2 != hipSuccess) {
+            g_live_graphs.fetch_sub(1);  // the slot taken above
             if (graph) (void)hipGraphDestroy(graph);
-            return rc;
+            return rc != HY_OK ? rc : fail(h, HY_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
         }
-        if (e2 != hipSuccess) return fail(h, HY_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
         hipGraphExec_t exec = nullptr;
         const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
-        if (e3 != hipSuccess) return fail(h, HY_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+        if (e3 != hipSuccess) {
+            g_live_graphs.fetch_sub(1);
+            return fail(h, HY_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+        }
         if (!ge) {
             // the cold-start graph of this kind is not used again once spectra
             // are warm: release it rather than keep its runtime resources
             for (auto it = h->graphs.begin(); it != h->graphs.end(); ++it)
                 if (warm && it->prec == prec && it->grad == (int)grad && it->warm == 0) {
-                    if (it->exec) (void)hipGraphExecDestroy(it->exec);
+                    destroy_graph_exec(it->exec);
                     h->graphs.erase(it);
                     break;
                 }
@@ -1573,9 +1615,10 @@ int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, do
         HIP_TRY(h, hipHostMalloc((void **)&h->h_site, sizeof(double) * h->P, hipHostMallocDefault));
         h->h_site_cap = h->P;
     }
-    if (h->use_graphs && !h->profile && !h->wover) {
-        TRY(launch_graph(h, precision, grad));
-    } else {
+    int grc = GRAPH_NO_SLOT;
+    if (h->use_graphs && !h->profile && !h->wover) grc = launch_graph(h, precision, grad);
+    if (grc < 0) return grc;
+    if (grc == GRAPH_NO_SLOT) {
         HIP_TRY(h, hipMemcpyAsync(h->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, h->stream));
         const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
         if (rc) return rc;
@@ -1625,7 +1668,7 @@ int hy_autotune(hy_engine h, int precision, unsigned flags, const double *bl, do
     int best_T = 0, best_parts = -1;
     double best = 1e300;
     if (h->P > 0) {
-        for (int T = 16; T <= 64; T *= 2)
+        for (int T = 16; T <= max_tile(precision); T *= 2)
             for (int parts = 0; parts <= 1; ++parts) {
                 if (parts && h->no_parts) continue;
                 h->tuned_T[ri] = T;
@@ -1796,26 +1839,42 @@ int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, do
 int hy_busted_expected_weights(hy_engine h, int precision, const double *bl, double alpha, const double *omegas,
                                const double *comp_w, const double *theta, double delta, double psi, int n_syn,
                                const double *syn_rates, const double *syn_weights, int test_class,
-                               double *out_weights, double *out_syn_weights) {
+                               double max_eps_sink, double *out_weights, double *out_syn_weights) {
     // BUSTEDAnalyzer::compute_expected_weights (busted.hpp:662-976) and
-    // compute_expected_syn_weights (busted.hpp:978-1135) without the error
-    // sink and bridges.  The reference accumulates, per test branch b,
-    // component k and class m, w_p w_k post_m(p) V_b . (P_k D_b) / L_m(p); under
-    // the raw time convention that sum over p and b is w_k dlnL/dw_k (the
+    // compute_expected_syn_weights (busted.hpp:978-1135) without bridges.
+    // The reference accumulates, per test branch b, component k and class m,
+    // w_p w_k post_m(p) V_b . (P_k D_b) / L_m(p); under the raw time
+    // convention that sum over p and b is w_k dlnL/dw_k (the
     // posterior-weighted mixture-weight gradient), and sum_p w_p post_m(p) is
-    // v_m dlnL/dv_m, so one gradient evaluation gives both updates.
+    // v_m dlnL/dv_m, so one gradient evaluation gives both updates.  The
+    // error sink's share eps sum post_m V_b . P_sink D_b / L_m (:846-860) is
+    // likewise eps dlnL/d eps, the sink component's weight gradient.
     if (!h) return HY_ERR_ARG;
     if (!h->have_problem || !h->have_model)
         return fail(h, HY_ERR_STATE, "hy_busted_expected_weights before hy_set_problem/hy_set_model");
     if (h->convention != HY_TIME_RAW)
         return fail(h, HY_ERR_STATE, "hy_busted_expected_weights: the mixture must use HY_TIME_RAW branch times");
-    if (h->has_sink)
-        return fail(h, HY_ERR_UNSUPPORTED, "hy_busted_expected_weights: mixtures with an error sink are not supported");
     if (test_class < 0 || test_class >= h->n_class)
         return fail(h, HY_ERR_ARG, "hy_busted_expected_weights: test class %d out of range", test_class);
     if (n_syn < 1 || !syn_rates || !syn_weights || !out_weights || !out_syn_weights)
         return fail(h, HY_ERR_ARG, "hy_busted_expected_weights: bad synonymous classes or outputs");
-    const int j0 = h->comp_ptr[test_class], j1 = h->comp_ptr[test_class + 1], K = j1 - j0;
+    const int j0 = h->comp_ptr[test_class], j1 = h->comp_ptr[test_class + 1];
+    // the test class's error-sink component (at most one), and its omega components
+    int j_sink = -1;
+    std::vector<int> jw;
+    for (int j = j0; j < j1; ++j) {
+        if (h->comp_mat[j] >= 0) {
+            jw.push_back(j);
+        } else if (j_sink >= 0) {
+            return fail(h, HY_ERR_ARG, "hy_busted_expected_weights: test class %d has more than one error sink",
+                        test_class);
+        } else {
+            j_sink = j;
+        }
+    }
+    if (j_sink >= 0 && !(max_eps_sink >= 1e-7 && max_eps_sink <= 1.0))
+        return fail(h, HY_ERR_ARG, "hy_busted_expected_weights: max_eps_sink %g outside [1e-7, 1]", max_eps_sink);
+    const int K = (int)jw.size();
     std::vector<double> site(std::max(h->P, 1)), gbl(h->num_nodes), gom(h->n_mat), gw(h->n_comp), gth(5),
         gsr(n_syn), gsw(n_syn);
     double ll = 0.0;
@@ -1841,12 +1900,23 @@ int hy_busted_expected_weights(hy_engine h, int precision, const double *bl, dou
             for (int i = 0; i < n; ++i) x[i] = prev[i];
         }
     };
-    if (K <= 1) {
-        for (int k = 0; k < K; ++k) out_weights[k] = 1.0;
-    } else {
-        std::vector<double> num(K);
-        for (int k = 0; k < K; ++k) num[k] = comp_w[j0 + k] * gw[j0 + k];
-        normalise(out_weights, K, num.data(), (double)n_test * total, comp_w + j0);
+    // out_weights follows the class's component order: omega components get
+    // their normalised shares, the sink component its new eps
+    const double den = (double)n_test * total;
+    std::vector<double> num(std::max(K, 1)), prev(std::max(K, 1)), neww(std::max(K, 1), 1.0);
+    // (with a sink the class's weights are (1 - eps) w_k; the unchanged
+    // fallback is the reference's w_k)
+    const double bio = (j_sink >= 0 && comp_w[j_sink] < 1.0) ? 1.0 - comp_w[j_sink] : 1.0;
+    for (int k = 0; k < K; ++k) {
+        num[k] = comp_w[jw[k]] * gw[jw[k]];
+        prev[k] = comp_w[jw[k]] / bio;
+    }
+    if (K > 1) normalise(neww.data(), K, num.data(), den, prev.data());
+    for (int k = 0; k < K; ++k) out_weights[jw[k] - j0] = neww[k];
+    if (j_sink >= 0) {
+        const double eps = comp_w[j_sink];
+        out_weights[j_sink - j0] = den > 0.0 ? std::clamp(eps * gw[j_sink] / std::max(1.0, den), 1e-7, max_eps_sink)
+                                             : eps;
     }
     if (n_syn <= 1) {
         out_syn_weights[0] = 1.0;
@@ -1894,7 +1964,7 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
         T = 16;
         const double node_cost[3] = {1.0, 1.3, 2.0};
         double best = 1e300;
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; (16 << k) <= max_tile(prec); ++k) {
             const int Tk = 16 << k;
             long tiles = 0;
             for (hy_engine h : hs) tiles += (h->P + Tk - 1) / Tk;
@@ -1906,6 +1976,7 @@ int build_batch(GroupState &gs, const std::vector<hy_engine> &hs, int prec) {
             }
         }
     }
+    T = std::min(T, max_tile(prec));
     long total_tiles = 0;
     for (hy_engine h : hs) total_tiles += (h->P + T - 1) / T;
     const bool lean = lean_inside(prec == HY_PREC_FP32, T, total_tiles, c->n_cu);
@@ -2254,9 +2325,10 @@ int eval_fused(GroupState &gs, hy_batch_item *items, int n, int prec, unsigned f
         hp[pl.off_delta + g] = it.delta;
         hp[pl.off_psi + g] = it.psi;
     }
-    if (c->use_graphs && !c->profile) {
-        TRY(launch_graph(c, prec, grad));
-    } else {
+    int grc = GRAPH_NO_SLOT;
+    if (c->use_graphs && !c->profile) grc = launch_graph(c, prec, grad);
+    if (grc < 0) return grc;
+    if (grc == GRAPH_NO_SLOT) {
         HIP_TRY(c, hipMemcpyAsync(c->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, c->stream));
         const int rc = prec == HY_PREC_FP64 ? run_eval<double>(c, flags, nullptr) : run_eval<float>(c, flags, nullptr);
         if (rc) return rc;
@@ -2371,7 +2443,7 @@ int hy_autotune_batch(int n_items, hy_batch_item *items, int precision, unsigned
     const int ri = precision == HY_PREC_FP64 ? 1 : 0;
     int best_T = 0;
     double best = 1e300;
-    for (int T = 16; T <= 64; T *= 2) {
+    for (int T = 16; T <= max_tile(precision); T *= 2) {
         gs.tuned_T[ri] = T;
         gs.members.clear();  // rebuild the merged problem at this tile width
         int rc = hy_eval_batch(n_items, items, precision, flags);
@@ -2705,6 +2777,12 @@ int hy_last_timings(hy_engine h, double *ms8) {
     if (!h || !ms8) return HY_ERR_ARG;
     if (!h->profile) return fail(h, HY_ERR_STATE, "timings need HY_PROFILE=1 at hy_create");
     std::memcpy(ms8, h->timings, sizeof(h->timings));
+    return HY_OK;
+}
+
+int hy_graph_count(int *live, int *cap) {
+    if (live) *live = g_live_graphs.load();
+    if (cap) *cap = graph_cap();
     return HY_OK;
 }
 

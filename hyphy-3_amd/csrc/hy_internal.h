@@ -197,6 +197,9 @@ struct AdjArgs {
     double *Apart;  // [n_branch][n_chunks][64][64]
     const GeneInfo *genes;  // batched evaluation (branch gene in the branch list)
     int n_amb;              // entries of the ambiguity-mask table
+    // Branch selection (checkpointed evaluations): workgroup row y handles
+    // branch list entry bsel[y] (null: entry y)
+    const int32_t *bsel;
 };
 
 struct ProjArgs {

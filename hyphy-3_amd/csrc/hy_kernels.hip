@@ -1490,7 +1490,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
     __shared__ __align__(16) R Ds[2][64 * ADJ_LD];
     __shared__ __align__(16) R cs[2][64];
     __shared__ uint64_t ambl[AMBG ? 1 : AMB_LDS];
-    const int chunk = blockIdx.x, bi = blockIdx.y;
+    const int chunk = blockIdx.x, bi = a.bsel ? a.bsel[blockIdx.y] : (int)blockIdx.y;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
     const int32_t *br = a.blist + bi * BR_STRIDE;
     if (a.genes) {  // batched evaluation: this branch's gene slice
@@ -2058,6 +2058,8 @@ template <typename R, int T> void launch_inside_t(const PruneArgs<R> &a, int ny,
         hipLaunchKernelGGL((k_inside<R, T, false>), dim3(a.n_tiles, ny), dim3(256), 0, s, a);
 }
 
+// FP64 tiles stop at 32 patterns (the host never asks for more): the
+// 64-pattern FP64 variants are not instantiated.
 template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean, int ny, hipStream_t s) {
     if constexpr (sizeof(R) == 4) {
         if (lean && T == 16) {
@@ -2068,7 +2070,9 @@ template <typename R> void launch_inside(const PruneArgs<R> &a, int T, bool lean
     switch (T) {
         case 16: launch_inside_t<R, 16>(a, ny, s); break;
         case 32: launch_inside_t<R, 32>(a, ny, s); break;
-        default: launch_inside_t<R, 64>(a, ny, s); break;
+        default:
+            if constexpr (sizeof(R) == 4) launch_inside_t<R, 64>(a, ny, s);
+            break;
     }
 }
 
@@ -2092,21 +2096,29 @@ template <typename R> void launch_outside(const PruneArgs<R> &a, int T, bool rev
     switch (T) {
         case 16: rev ? launch_outside_t<R, 16, true>(a, ny, s) : launch_outside_t<R, 16, false>(a, ny, s); break;
         case 32: rev ? launch_outside_t<R, 32, true>(a, ny, s) : launch_outside_t<R, 32, false>(a, ny, s); break;
-        default: rev ? launch_outside_t<R, 64, true>(a, ny, s) : launch_outside_t<R, 64, false>(a, ny, s); break;
+        default:
+            if constexpr (sizeof(R) == 4)
+                rev ? launch_outside_t<R, 64, true>(a, ny, s) : launch_outside_t<R, 64, false>(a, ny, s);
+            break;
     }
 }
 
+// a.n_branch workgroup rows (branch list entries, or a.bsel's selection)
 template <typename R> void launch_adjoint(const AdjArgs<R> &a, int T, hipStream_t s) {
     const dim3 grid(a.n_chunks, a.n_branch);
     if (a.n_amb <= AMB_LDS) switch (T) {
         case 16: hipLaunchKernelGGL((k_adjoint<R, 16, false>), grid, dim3(256), 0, s, a); break;
         case 32: hipLaunchKernelGGL((k_adjoint<R, 32, false>), grid, dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_adjoint<R, 64, false>), grid, dim3(256), 0, s, a); break;
+        default:
+            if constexpr (sizeof(R) == 4) hipLaunchKernelGGL((k_adjoint<R, 64, false>), grid, dim3(256), 0, s, a);
+            break;
     }
     else switch (T) {
         case 16: hipLaunchKernelGGL((k_adjoint<R, 16, true>), grid, dim3(256), 0, s, a); break;
         case 32: hipLaunchKernelGGL((k_adjoint<R, 32, true>), grid, dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_adjoint<R, 64, true>), grid, dim3(256), 0, s, a); break;
+        default:
+            if constexpr (sizeof(R) == 4) hipLaunchKernelGGL((k_adjoint<R, 64, true>), grid, dim3(256), 0, s, a);
+            break;
     }
 }
 

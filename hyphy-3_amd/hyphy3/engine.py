@@ -33,7 +33,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
-           "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc")
+           "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -78,6 +78,7 @@ def load_library(path: str = LIB_PATH):
                           C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.hy_loglik_from_P.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
     L.hy_loglik_from_P_noalloc.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
+    L.hy_graph_count.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.hy_set_sink.argtypes = [C.c_void_p, _f64p]
     L.hy_last_timings.argtypes = [C.c_void_p, _f64p]
@@ -95,7 +96,7 @@ def load_library(path: str = LIB_PATH):
                               C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _f64p]
     L.hy_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     L.hy_busted_expected_weights.argtypes = [C.c_void_p, C.c_int, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
-                                             C.c_double, C.c_int, _f64p, _f64p, C.c_int, _f64p, _f64p]
+                                             C.c_double, C.c_int, _f64p, _f64p, C.c_int, C.c_double, _f64p, _f64p]
     L.hy_autotune_batch.argtypes = [C.c_int, C.POINTER(BatchItem), C.c_int, C.c_uint, C.c_int, C.POINTER(C.c_int),
                                     _f64p]
     _lib = L
@@ -303,10 +304,12 @@ class Engine:
         return out
 
     def busted_expected_weights(self, bl, omegas, comp_w, theta, syn_rates=(1.0,), syn_weights=(1.0,), test_class=0,
-                                alpha=1.0, delta=0.0, psi=0.0, precision=HY_PREC_FP64):
-        """BUSTED's expectation step (busted.hpp:662-1135, no error sink):
-        updated omega-component weights of ``test_class`` and synonymous-rate
-        weights.  Needs raw branch times (HY_TIME_RAW mixture)."""
+                                alpha=1.0, delta=0.0, psi=0.0, precision=HY_PREC_FP64, max_eps_sink=0.01):
+        """BUSTED's expectation step (busted.hpp:662-1135): updated weights of
+        the components of ``test_class`` (omega components renormalised; an
+        error-sink component gets the new eps, at most max_eps_sink) and
+        synonymous-rate weights.  Needs raw branch times (HY_TIME_RAW
+        mixture)."""
         bl = _f64(bl)
         om = _f64(np.atleast_1d(omegas))
         cw = _f64(np.atleast_1d(comp_w))
@@ -320,8 +323,8 @@ class Engine:
         nv = np.zeros(len(sr))
         self._check(self.L.hy_busted_expected_weights(self.h, precision, _p(bl, _f64p), alpha, _p(om, _f64p),
                                                       _p(cw, _f64p), _p(th, _f64p), delta, psi, len(sr),
-                                                      _p(sr, _f64p), _p(sw, _f64p), int(test_class), _p(nw, _f64p),
-                                                      _p(nv, _f64p)))
+                                                      _p(sr, _f64p), _p(sw, _f64p), int(test_class),
+                                                      float(max_eps_sink), _p(nw, _f64p), _p(nv, _f64p)))
         return nw[:K], nv
 
     def loglik_from_P(self, P, pi, precision=HY_PREC_FP64, site=False):
@@ -395,6 +398,14 @@ class Engine:
         v = C.c_size_t()
         self._check(self.L.hy_device_bytes(self.h, C.byref(v)))
         return v.value
+
+
+def graph_count():
+    """(instantiated evaluation graphs alive in the process, their cap)."""
+    L = load_library()
+    live, cap = C.c_int(), C.c_int()
+    L.hy_graph_count(C.byref(live), C.byref(cap))
+    return live.value, cap.value
 
 
 def create_group(n, device=0, n_stream_sets=8):

@@ -161,20 +161,26 @@ int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, do
                 double *out_grad_syn_rates, double *out_grad_syn_weights);
 
 /* BUSTED expectation step: BUSTEDAnalyzer::compute_expected_weights
- * (busted.hpp:662-976) and compute_expected_syn_weights (busted.hpp:978-1135)
- * without the error sink.  Needs a mixture with HY_TIME_RAW branch times (bl =
- * the reference's s * branch_tau).  out_weights[k] (k over the components of
- * test_class) = clamp(sum over test branches and patterns of w_p w_k post_m(p)
- * V_b . (P_k D_b) / L_m(p), divided by (test branches x the weight of the
- * patterns with a positive likelihood), 1e-6, 1 - 1e-6), renormalised ({1}
- * when the class has one component); out_syn_weights[m] = clamp(sum_p w_p
- * post_m(p) / that pattern weight, 1e-6, 1 - 1e-6), renormalised ({1} when
- * n_syn = 1).  Computed from one hy_eval_srv gradient evaluation (the
- * accumulated sums are w_k dlnL/dw_k and v_m dlnL/dv_m). */
+ * (busted.hpp:662-976, including the error-sink update :846-860, 964-966)
+ * and compute_expected_syn_weights (busted.hpp:978-1135).  Needs a mixture
+ * with HY_TIME_RAW branch times (bl = the reference's s * branch_tau).
+ * out_weights[k] (k over the components of test_class, in order): for an
+ * omega component, clamp(sum over test branches and patterns of w_p w_k
+ * post_m(p) V_b . (P_k D_b) / L_m(p), divided by (test branches x the weight
+ * of the patterns with a positive likelihood), 1e-6, 1 - 1e-6), renormalised
+ * over the omega components ({1} when the class has one); for the class's
+ * error-sink component (comp_mat == HY_SINK, weight eps, the omega components
+ * then weighted (1 - eps) w_k as in the reference), the new eps =
+ * clamp(eps sum ... V_b . P_sink D_b / L_m(p) / max(1, that divisor), 1e-7,
+ * max_eps_sink) (max_eps_sink is ignored without a sink).
+ * out_syn_weights[m] = clamp(sum_p w_p post_m(p) / that pattern weight, 1e-6,
+ * 1 - 1e-6), renormalised ({1} when n_syn = 1).  Computed from one
+ * hy_eval_srv gradient evaluation (the accumulated sums are w_k dlnL/dw_k,
+ * eps dlnL/d eps and v_m dlnL/dv_m). */
 int hy_busted_expected_weights(hy_engine h, int precision, const double *bl, double alpha, const double *omegas,
                                const double *comp_w, const double *theta, double delta, double psi, int n_syn,
                                const double *syn_rates, const double *syn_weights, int test_class,
-                               double *out_weights, double *out_syn_weights);
+                               double max_eps_sink, double *out_weights, double *out_syn_weights);
 
 /* Many independent problems (genes) in one call: item i evaluates handle
  * items[i].h exactly as hy_eval would with the item's parameters and
@@ -255,8 +261,8 @@ int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E,
                      int32_t *LE);
 
 /* Pattern-tile width used by the pruning kernels (16, 32 or 64 patterns per
- * workgroup; 0 = automatic, chosen per precision from the pattern count and
- * the device's CU count).  Applies from the next hy_eval; a change of tile
+ * workgroup; FP64 evaluations run 64 as 32; 0 = automatic, chosen per
+ * precision from the pattern count and the device's CU count).  Applies from the next hy_eval; a change of tile
  * width (e.g. when alternating precisions) re-tiles the resident problem. */
 int hy_set_tile(hy_engine h, int patterns_per_tile);
 
@@ -307,6 +313,14 @@ int hy_last_eigen_status(hy_engine h, int *status, int n);
  * [5] Frechet projection, [6] reductions, [7] total.  Only filled when the
  * engine was created with HY_PROFILE=1 in the environment. */
 int hy_last_timings(hy_engine h, double *ms8);
+
+/* Instantiated evaluation graphs alive in this process and their cap.
+ * Evaluations capture one HIP graph per (precision, gradient, warm start)
+ * and replay it; at most *cap graphs (HY_MAX_GRAPHS in the environment,
+ * default 512) exist at once across all handles, and evaluations that would
+ * need another launch their kernels directly (same kernels, same results).
+ * No counterpart in the reference. */
+int hy_graph_count(int *live, int *cap);
 
 /* Device memory currently held by the engine, in bytes. */
 int hy_device_bytes(hy_engine h, size_t *out);

@@ -387,18 +387,29 @@ def workspace(prob, P, pi, p0, p1, outside=True):
 
 
 def busted_expected_weights(prob, code, base, pos, pi, weights_p, omegas, weights, omega_bg, is_test, tau,
-                            syn_rates=(1.0,), syn_weights=(1.0,)):
+                            syn_rates=(1.0,), syn_weights=(1.0,), eps_sink=None, max_eps_sink=0.01):
     """BUSTEDAnalyzer::compute_expected_weights (busted.hpp:662-976) and
-    compute_expected_syn_weights (busted.hpp:978-1135) without the error sink
-    and bridges, restated literally over the reference's inside-outside
-    workspaces: per synonymous class m the mixed P(r_m tau) (test branches
-    sum_k w_k P_k, background P_bg, :715-733), the workspace of the whole
-    alignment (:791-799), log-sum-exp posteriors over classes (:802-823), and per
-    test branch, component and class the share w_k post_m V_b . (P_k D_b) / L
-    (:826-845); totals over patterns with a positive likelihood (:847-866,
-    :942-975).  tau: raw branch times per node; weights_p: pattern weights.
-    Returns (new omega weights, new synonymous weights)."""
+    compute_expected_syn_weights (busted.hpp:978-1135) without bridges,
+    restated literally over the reference's inside-outside workspaces: per
+    synonymous class m the mixed P(r_m tau) (test branches sum_k b w_k P_k
+    with b = 1 - eps_sink, plus eps_sink P_sink with the error sink;
+    background P_bg, :715-733), the workspace of the whole alignment
+    (:791-799), log-sum-exp posteriors over classes (:802-823), and per test
+    branch, component and class the share b w_k post_m V_b . (P_k D_b) / L
+    (:826-845), with the sink the share eps post_m V_b . P_sink D_b / L
+    (eval_sink_bilinear, :208-216, :846-860); totals over patterns with a
+    positive likelihood (:847-866, :942-975), the new eps clamped to
+    [1e-7, max_eps_sink] (:964-966).  tau: raw branch times per node;
+    weights_p: pattern weights; the sink matrix comes from pi
+    (build_sink_kernel, :193-206).  Returns (new omega weights, new
+    synonymous weights), and the new eps when eps_sink is given."""
     N, S = prob.num_nodes, code.S
+    sink = eps_sink is not None
+    bio = (1.0 - eps_sink) if sink else 1.0
+    if sink:
+        inv = 1.0 / np.maximum(1e-12, 1.0 - np.asarray(pi, dtype=float))
+        P_sink = np.asarray(pi, dtype=float)[None, :] * inv[:, None]
+        np.fill_diagonal(P_sink, 0.0)
     K, M = len(omegas), len(syn_rates)
     P = prob.num_patterns
     dec = []
@@ -413,8 +424,12 @@ def busted_expected_weights(prob, code, base, pos, pi, weights_p, omegas, weight
         for n in range(N):
             for k in range(K):
                 comp[k, n] = transition(*dec[k], syn_rates[m] * tau[n])
-            mix[n] = sum(weights[k] * comp[k, n] for k in range(K)) if is_test[n] else \
-                transition(*dec[K], syn_rates[m] * tau[n])
+            if is_test[n]:
+                mix[n] = sum(bio * weights[k] * comp[k, n] for k in range(K))
+                if sink:
+                    mix[n] = mix[n] + eps_sink * P_sink
+            else:
+                mix[n] = transition(*dec[K], syn_rates[m] * tau[n])
         Pc.append(comp)
         wsm.append(workspace(prob, mix, pi, 0, P, outside=True) if syn_weights[m] > 1e-12 else None)
     # posteriors over synonymous classes
@@ -430,7 +445,9 @@ def busted_expected_weights(prob, code, base, pos, pi, weights_p, omegas, weight
     e = np.where(np.isfinite(terms), np.exp(terms - np.where(valid, mx, 0.0)), 0.0)
     post[:, valid] = e[:, valid] / e[:, valid].sum(axis=0)
     acc = np.zeros(K)
+    sink_acc = 0.0
     tot = 0.0
+    counted = post.sum(axis=0) > 0.0
     for n in range(N):
         if not is_test[n]:
             continue
@@ -442,8 +459,19 @@ def busted_expected_weights(prob, code, base, pos, pi, weights_p, omegas, weight
                 PD = Pc[m][k, n] @ ws["D"][n]
                 ratio = np.einsum("sb,sb->b", ws["V"][n], PD)
                 use = (post[m] > 1e-12) & (ws["L"] > 0.0)
-                acc[k] += np.sum(weights_p[use] * weights[k] * post[m, use] * ratio[use] / ws["L"][use])
-        tot += np.sum(weights_p[post.sum(axis=0) > 0.0])
+                acc[k] += np.sum(weights_p[use] * bio * weights[k] * post[m, use] * ratio[use] / ws["L"][use])
+        if sink:
+            sink_ratio = np.zeros(P)
+            for m in range(M):
+                if wsm[m] is None:
+                    continue
+                ws = wsm[m]
+                # V^T max(0, P_sink D) per pattern
+                bil = np.einsum("sb,sb->b", ws["V"][n], np.maximum(0.0, P_sink @ ws["D"][n]))
+                use = counted & (post[m] > 1e-12) & (ws["L"] > 0.0)
+                sink_ratio[use] += post[m, use] * bil[use] / ws["L"][use]
+            sink_acc += np.sum(weights_p[counted] * eps_sink * sink_ratio[counted])
+        tot += np.sum(weights_p[counted])
 
     def norm(num, den, prev):
         if den <= 0.0:
@@ -457,4 +485,7 @@ def busted_expected_weights(prob, code, base, pos, pi, weights_p, omegas, weight
     else:
         syn_acc = (weights_p[None, :] * post).sum(axis=1)
         new_v = norm(syn_acc, np.sum(weights_p[valid]), syn_weights)
-    return new_w, new_v
+    if not sink:
+        return new_w, new_v
+    new_eps = float(np.clip(sink_acc / max(1.0, tot), 1e-7, max_eps_sink)) if tot > 0.0 else eps_sink
+    return new_w, new_v, new_eps

@@ -24,7 +24,7 @@ def test_autotune_choice_is_used_and_exact():
     for prec, tl, tg in ((HY_PREC_FP32, 1e-5, 1e-4), (HY_PREC_FP64, 1e-9, 1e-7)):
         tile, parts, ms = eng.autotune(bl, omegas, weights, sim["theta"], precision=prec, reps=2)
         print("precision", prec, "tile", tile, "parts", parts, "ms %.3f" % ms)
-        assert tile in (16, 32, 64) and parts in (0, 1) and ms > 0.0
+        assert tile in ((16, 32, 64) if prec == HY_PREC_FP32 else (16, 32)) and parts in (0, 1) and ms > 0.0
         r = eng.eval(bl, omegas, weights, sim["theta"], precision=prec)
         assert eng.last_tile() == tile
         assert (eng.last_parts() > 0) == (parts == 1)
@@ -35,7 +35,14 @@ def test_autotune_choice_is_used_and_exact():
     eng.set_tile(64)
     eng.eval(bl, omegas, weights, sim["theta"], precision=HY_PREC_FP32)
     assert eng.last_tile() == 64
+    # FP64 has no 64-pattern kernels (their outside pass would spill): 32
+    r = eng.eval(bl, omegas, weights, sim["theta"], precision=HY_PREC_FP64)
+    assert eng.last_tile() == 32
+    assert rel(r["log_likelihood"], ref["log_likelihood"]) < 1e-9
     eng.set_tile(0)
+    eng.set_tuning(HY_PREC_FP64, 64, 1)
+    eng.eval(bl, omegas, weights, sim["theta"], precision=HY_PREC_FP64)
+    assert eng.last_tile() == 32
     eng.set_tuning(HY_PREC_FP32, 32, 0)
     r = eng.eval(bl, omegas, weights, sim["theta"], precision=HY_PREC_FP32)
     assert eng.last_tile() == 32 and eng.last_parts() == 0
@@ -59,7 +66,7 @@ def test_batch_autotune_choice_is_used():
     for prec, tol in ((HY_PREC_FP32, 1e-5), (HY_PREC_FP64, 1e-12)):
         batch.eval(params, precision=prec)
         tile, ms = batch.autotune(precision=prec, reps=1)
-        assert tile in (16, 32, 64) and ms > 0.0
+        assert tile in ((16, 32, 64) if prec == HY_PREC_FP32 else (16, 32)) and ms > 0.0
         out = batch.eval(params, precision=prec)
         for (tree, aln, sim), prm, r in zip(probs, params, out):
             ref = make_engine(tree, aln)

@@ -184,3 +184,35 @@ def test_lean_inside_on_many_tiles():
     eng.set_tile(32)
     r32 = eng.eval(bl, omegas, weights, sim["theta"], precision=FP32)
     assert rel(r["log_likelihood"], r32["log_likelihood"]) < 1e-6
+
+
+def test_graph_cap_bounds_live_graphs():
+    """At most HY_MAX_GRAPHS instantiated evaluation graphs exist in the
+    process; engines past the cap launch eagerly with the same results, and
+    destroying engines returns their graphs' slots."""
+    import gc
+    import os
+    from hyphy3.engine import graph_count
+    gc.collect()
+    live0, _ = graph_count()
+    os.environ["HY_MAX_GRAPHS"] = str(live0 + 3)
+    engines = []
+    try:
+        tree, aln, sim = sim_problem(12, 150, 77)
+        bl = branch_lengths(tree)
+        ll_ref, g_ref = ho.branch_length_gradients(ho.Problem.from_refs(tree, aln), ho.Code(0),
+                                                   ho.params(1.0, 0.5, sim["theta"]), aln.pos_freqs, aln.cf3x4, bl)
+        for i in range(5):
+            eng = make_engine(tree, aln)
+            engines.append(eng)
+            for rep in range(3):
+                r = eng.eval(bl, [0.5], [1.0], sim["theta"], precision=FP64)
+                assert rel(r["log_likelihood"], ll_ref) < 1e-9 and grad_err(r["grad_bl"], g_ref) < 1e-7, (i, rep)
+                live, cap = graph_count()
+                assert cap == live0 + 3 and live <= cap, (i, rep, live, cap)
+        assert graph_count()[0] == live0 + 3  # the cap was reached: later engines ran eagerly
+    finally:
+        for e in engines:
+            e.close()
+        del os.environ["HY_MAX_GRAPHS"]
+    assert graph_count()[0] == live0

@@ -68,3 +68,33 @@ def test_estep_single_component():
     new_w, _ = ho.busted_expected_weights(prob, ho.Code(0), base, aln.pos_freqs, aln.cf3x4, aln.weights, [0.7], [1.0],
                                           OMEGA_BG, is_test, tau)
     assert np.array_equal(new_w, [1.0])
+
+
+def test_estep_error_sink_update():
+    """With the error sink (busted.hpp:846-860, 964-966): the literal sink
+    share eps * sum post V . P_sink D / L equals eps dlnL/d eps of the
+    oracle's mixture gradient (the sink component's weight gradient), the
+    omega shares (1 - eps) w_k dlnL/dw_k, all unclamped shares sum to one,
+    and the new eps is clamped to [1e-7, max_eps_sink]."""
+    tree, aln, prob, base, _, is_test, tau, N = _setup(74)
+    code = ho.Code(0)
+    eps = 0.04
+    bc = np.where(is_test, 0, 1).astype(np.int32)
+    mix = ho.Mixture(OMEGAS + [OMEGA_BG], bc, [[(k, (1 - eps) * W[k]) for k in range(3)] + [(-1, eps)], [(3, 1.0)]],
+                     0, sink_freqs=aln.cf3x4)
+    for sr, sv in (([1.0], [1.0]), ([0.5, 1.8], [0.6, 0.4])):
+        new_w, new_v, new_eps = ho.busted_expected_weights(prob, code, base, aln.pos_freqs, aln.cf3x4, aln.weights,
+                                                           OMEGAS, W, OMEGA_BG, is_test, tau, sr, sv, eps_sink=eps,
+                                                           max_eps_sink=0.5)
+        r = ho.srv_eval(prob, code, base, aln.pos_freqs, aln.cf3x4, mix, tau, sr, sv)
+        ok = r["site_ll"] > -1e19
+        tot = int(np.sum(is_test)) * np.sum(aln.weights[ok])
+        num = (1 - eps) * np.array(W) * r["grad_w"][:3]
+        sink_share = eps * r["grad_w"][3]
+        x = np.clip(num / tot, 1e-6, 1 - 1e-6)
+        np.testing.assert_allclose(new_w, x / x.sum(), rtol=1e-10)
+        assert abs(new_eps - sink_share / max(1.0, tot)) < 1e-10 * max(new_eps, 1e-12)
+        assert abs((num.sum() + sink_share) / tot - 1.0) < 1e-10  # EM: shares sum to one
+    _, _, capped = ho.busted_expected_weights(prob, code, base, aln.pos_freqs, aln.cf3x4, aln.weights, OMEGAS, W,
+                                              OMEGA_BG, is_test, tau, eps_sink=eps, max_eps_sink=1e-5)
+    assert capped == 1e-5
