@@ -190,6 +190,9 @@ def main():
                     help="instead of the throughput run: BUSTED expectation steps (omega and synonymous-rate weights)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--memory-budget", type=float, default=0.0,
+                    help="GB allowed for the partials; below what every node's partials need the engine "
+                         "checkpoints inside partials (about sqrt(internal nodes) parts recomputed)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -217,6 +220,9 @@ def main():
     # timed engine: no per-stage events
     os.environ.pop("HY_PROFILE", None)
     eng = make_engine(arrays, K, args.tile)
+    budget = int(args.memory_budget * 1e9)
+    if budget:
+        eng.set_memory_budget(budget)
     tuning = {}
     if not args.no_autotune and not args.tile:
         # measured tile/parts choice per precision, before (and outside) the timed steps
@@ -231,11 +237,13 @@ def main():
     ms32, out32, _ = run_steps(eng, eval_args, 1, args.steps, args.warmup, False)
     tile32 = eng.last_tile()
     parts32 = eng.last_parts()
+    ck32 = eng.last_checkpoint()
     print(f"[bench] fp32 {ms32 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
     res = {}
     if not args.no_fp64:
         ms64, out64, _ = run_steps(eng, eval_args, 0, args.steps, args.warmup, False)
         res["fp64_tile"] = eng.last_tile()
+        res["partial_bytes_fp64"] = eng.last_checkpoint()[1]
         print(f"[bench] fp64 {ms64 * 1e3:.3f} ms/eval", file=sys.stderr, flush=True)
         g32 = np.concatenate([out32["grad_bl"], out32["grad_omega"], out32["grad_w"], out32["grad_theta"]])
         g64 = np.concatenate([out64["grad_bl"], out64["grad_omega"], out64["grad_w"], out64["grad_theta"]])
@@ -254,6 +262,8 @@ def main():
     # profiled engine: HIP events around every stage on the engine's stream
     os.environ["HY_PROFILE"] = "1"
     peng = make_engine(arrays, K, args.tile)
+    if budget:
+        peng.set_memory_budget(budget)
     for prec, name in ((1, "fp32"), (0, "fp64")):
         if name in tuning:
             peng.set_tuning(prec, tuning[name]["tile"], tuning[name]["parts"])
@@ -301,6 +311,10 @@ def main():
                       "a second engine with HIP events around each stage",
     }
     line.update(res)
+    line["partial_bytes_fp32"] = ck32[1]
+    line["checkpoint"] = {"budget_bytes": budget or None, "parts_recomputed": ck32[0],
+                          "note": "parts 0: every node's partials resident; otherwise inside partials of the top "
+                                  "part and part roots resident, each part's interior recomputed"}
     line["autotune"] = tuning or None
     if not args.no_cpu:
         try:

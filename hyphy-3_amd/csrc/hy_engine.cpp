@@ -103,6 +103,18 @@ struct hy_engine_s {
     DevBuf d_parts;
     int n_parts = 0;
     bool no_parts = false;  // HY_NO_PARTS=1: one schedule per pass
+    // Checkpointed evaluations: when the partials of every node (D, V, their
+    // exponents, spilled messages) exceed the budget, only the top part's
+    // partials and the part roots' stay resident and each part's interior is
+    // recomputed into a shared segment before its outside and adjoint passes.
+    // mem_budget: bytes allowed for those buffers (0: the device's free
+    // memory); ck_groups: offsets of the adjoint branch groups in d_bsel.
+    bool ck = false;
+    size_t mem_budget = 0, partial_bytes = 0;
+    int ck_dslots = 0, ck_vslots = 0;
+    std::string ck_error;  // non-empty: the checkpointed layout failed its host-side check
+    std::vector<int32_t> ck_groups, ck_bsel, blist_ck;
+    DevBuf d_blist_ck, d_bsel;
     std::vector<int32_t> real_parent;
     std::vector<uint16_t> codes;  // [num_taxa][P]
     std::vector<uint64_t> amb;
@@ -335,8 +347,19 @@ void build_layout(hy_engine h) {
 struct Schedules {
     std::vector<int32_t> sched, osched, parts;
     int n_parts = 0, n_uspill = 0;
+    // Checkpointed layout (ck): D index and V slot of every extended node.
+    // Persistent slots first (D: the top part's nodes and the part roots;
+    // V: branches below top-part nodes), then one segment the parts share in
+    // turn (D: a part's other internal nodes; V: branches below them).
+    std::vector<int32_t> islot, vsl;
+    int n_dslot = 0, n_vslot = 0, n_dpersist = 0, n_vpersist = 0;
+    // branch list indices grouped by the part of the branch's parent: the
+    // top part first, then part 0, 1, ...; group g is bsel[bgroup[g] ..
+    // bgroup[g + 1])
+    std::vector<int32_t> bsel, bgroup;
 };
-void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean, bool lean_out, Schedules &o);
+void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean, bool lean_out, Schedules &o,
+                    bool ck = false);
 void build_schedules(hy_engine h, int T, int esz, bool split);
 
 // Identity P of the artificial nodes in the Prow (with row sums), PA/PTA
@@ -413,21 +436,72 @@ int prepare_buffers(hy_engine h, int prec) {
     // the lean inside variant (four workgroups per CU).
     bool split = !h->no_parts && h->n_tiles < 2 * h->n_cu;
     if (!h->T_req && h->tuned_parts[ri] >= 0) split = !h->no_parts && h->tuned_parts[ri] == 1;
-    h->lean = lean_inside(prec == HY_PREC_FP32, T, h->n_tiles, h->n_cu) || (split && prec == HY_PREC_FP32 && T == 16);
-    h->lean_out = split && prec == HY_PREC_FP32 && T == 16;
-    if (h->sched_key != T * 16 + (int)rs + (h->lean ? 4096 : 0) + (split ? 8192 : 0) + (h->lean_out ? 16384 : 0)) {
+    const size_t Pp = (size_t)h->P_pad;
+    // Checkpointing: the resident layout's partial buffers (D, V, U spills at
+    // worst, exponents) against the budget, by default the device's free
+    // memory plus what this handle's partial buffers already hold
+    {
+        const size_t full = (size_t)(2 * h->n_int + h->n_vslot) * Pp * 64 * rs + (size_t)(h->n_int + h->n_vslot) * Pp * 4;
+        size_t held = h->d_E.bytes + h->d_G.bytes;
+        for (int k = 0; k < 2; ++k) held += h->d_D[k].bytes + h->d_Vp[k].bytes + h->d_Uspill[k].bytes;
+        bool ck;
+        if (h->mem_budget) {
+            ck = full > h->mem_budget;
+        } else if (!h->ck && h->d_D[ri].bytes >= (size_t)h->n_int * Pp * 64 * rs &&
+                   h->d_Vp[ri].bytes >= (size_t)h->n_vslot * Pp * 64 * rs) {
+            ck = false;  // the resident layout is allocated already (no query per evaluation)
+        } else {
+            size_t fr = 0, tot = 0;
+            size_t budget = ~size_t(0);
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+                const size_t margin = size_t(1) << 30;
+                budget = fr + held > margin ? fr + held - margin : 1;
+            }
+            ck = full > budget;
+        }
+        if (ck != h->ck) {
+            // switching layouts: the partial buffers change size (release,
+            // so the checkpointed layout really holds less memory)
+            for (int k = 0; k < 2; ++k) {
+                release(h, h->d_D[k]);
+                release(h, h->d_Vp[k]);
+                release(h, h->d_Uspill[k]);
+            }
+            release(h, h->d_E);
+            release(h, h->d_G);
+            h->ck = ck;
+            ++h->epoch;
+        }
+    }
+    if (h->ck) split = true;
+    h->lean = !h->ck && (lean_inside(prec == HY_PREC_FP32, T, h->n_tiles, h->n_cu) ||
+                         (split && prec == HY_PREC_FP32 && T == 16));
+    h->lean_out = !h->ck && split && prec == HY_PREC_FP32 && T == 16;
+    if (h->sched_key != T * 16 + (int)rs + (h->lean ? 4096 : 0) + (split ? 8192 : 0) + (h->lean_out ? 16384 : 0) +
+                            (h->ck ? 32768 : 0)) {
         ++h->epoch;
         build_schedules(h, T, (int)rs, split);
+        if (h->ck && !h->ck_error.empty()) {
+            h->sched_key = 0;
+            return fail(h, HY_ERR_STATE, "internal: checkpointed layout check failed (%s)", h->ck_error.c_str());
+        }
         TRY(upload(h, h->d_sched, h->sched.data(), h->sched.size() * 4));
         TRY(upload(h, h->d_osched, h->osched.data(), h->osched.size() * 4));
         TRY(upload(h, h->d_parts, h->parts.data(), h->parts.size() * 4));
+        if (h->ck) {
+            // the adjoint's branch list with the checkpointed slots, and its groups
+            TRY(upload(h, h->d_blist_ck, h->blist_ck.data(), h->blist_ck.size() * 4));
+            TRY(upload(h, h->d_bsel, h->ck_bsel.data(), std::max<size_t>(h->ck_bsel.size(), 1) * 4));
+        }
     }
-    const size_t Pp = (size_t)h->P_pad;
-    TRY(ensure(h, h->d_D[ri], (size_t)h->n_int * Pp * 64 * rs));
-    TRY(ensure(h, h->d_Vp[ri], (size_t)h->n_vslot * Pp * 64 * rs));
-    TRY(ensure(h, h->d_Uspill[ri], h->n_uspill ? (size_t)h->n_int * Pp * 64 * rs : 256));
-    TRY(ensure(h, h->d_E, (size_t)h->n_int * Pp * 4));
-    TRY(ensure(h, h->d_G, (size_t)h->n_vslot * Pp * 4));
+    const size_t nd = h->ck ? (size_t)h->ck_dslots : (size_t)h->n_int;
+    const size_t nv = h->ck ? (size_t)h->ck_vslots : (size_t)h->n_vslot;
+    TRY(ensure(h, h->d_D[ri], nd * Pp * 64 * rs));
+    TRY(ensure(h, h->d_Vp[ri], nv * Pp * 64 * rs));
+    TRY(ensure(h, h->d_Uspill[ri], h->n_uspill ? nd * Pp * 64 * rs : 256));
+    TRY(ensure(h, h->d_E, nd * Pp * 4));
+    TRY(ensure(h, h->d_G, nv * Pp * 4));
+    h->partial_bytes = h->d_D[ri].bytes + h->d_Vp[ri].bytes + h->d_Uspill[ri].bytes + h->d_E.bytes + h->d_G.bytes;
     TRY(ensure(h, h->d_Lroot, Pp * 8));
     TRY(ensure(h, h->d_Eroot, Pp * 4));
     TRY(ensure(h, h->d_site, Pp * 8));
@@ -448,7 +522,13 @@ int prepare_buffers(hy_engine h, int prec) {
     const int G = 64 / T;
     const int n_groups = (h->n_tiles + G - 1) / G;
     const int slots = h->n_cu * (rs == 4 ? 2 : 1);
-    const int nchg = std::min(n_groups, std::max(1, slots / nb));
+    // checkpointed: one adjoint launch per branch group, sized for the largest
+    int nbl = nb;
+    if (h->ck) {
+        nbl = 1;
+        for (size_t g = 0; g + 1 < h->ck_groups.size(); ++g) nbl = std::max(nbl, h->ck_groups[g + 1] - h->ck_groups[g]);
+    }
+    const int nchg = std::min(n_groups, std::max(1, slots / nbl));
     h->tiles_per_chunk = ((n_groups + nchg - 1) / nchg) * G;
     h->n_chunks = (h->n_tiles + h->tiles_per_chunk - 1) / h->tiles_per_chunk;
     TRY(ensure(h, h->d_Apart, (size_t)nb * h->n_chunks * 4096 * 8));
@@ -522,22 +602,56 @@ int prepare_buffers(hy_engine h, int prec) {
 // (and its own LDS slots), subtree parts are walked by separate workgroups
 // and exchange their results with the top part through HBM (inside: the
 // part root's partials; outside: the message into the part root, spilled).
-void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean, bool lean_out, Schedules &o) {
+void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean, bool lean_out, Schedules &o,
+                    bool ck) {
     const int NSI = inside_slots(T, esz, lean), NSO = outside_slots(T, esz, lean_out);
     const auto &ch = h->ext_children;
     const int n_ext = (int)ch.size();
+    // D index / V slot maps: the problem's own, or the checkpointed layout
+    // (filled below once the parts are known)
+    const int32_t *isl = h->int_slot.data(), *vsl = h->vslot.data();
     auto ref_of = [&](int c) -> int32_t {
         if (ch[c].empty()) {
             const int row = (h->leaf_taxon_ext[c] >= 0) ? h->leaf_taxon_ext[c] : h->num_taxa;
             return (KIND_TIP << 28) | row;
         }
-        return (KIND_INT << 28) | h->int_slot[c];  // identity edges carry an exact identity P
+        return (KIND_INT << 28) | isl[c];  // identity edges carry an exact identity P
     };
     // ---- parts: grow the top from the root by cutting the largest subtree
-    // while it holds more than a quarter of the internal nodes (at least 8)
+    // while it holds more than a quarter of the internal nodes (at least 8);
+    // checkpointed: more than about sqrt(internal nodes) (at least 4)
     std::vector<int> part_of(n_ext, -1);  // -1: top part
     std::vector<int> proots;
-    if (split && (int)h->order.size() >= 16) {
+    if (ck && split && (int)h->order.size() >= 8) {
+        std::vector<int> nint(n_ext, 0);
+        for (int u : h->order) {
+            nint[u] = 1;
+            for (int c : ch[u]) nint[u] += nint[c];
+        }
+        const int target = std::max(4, (int)std::ceil(std::sqrt((double)h->order.size())));
+        std::vector<int> cand;
+        for (int c : ch[h->ext_root])
+            if (!ch[c].empty()) cand.push_back(c);
+        for (;;) {
+            auto big = std::max_element(cand.begin(), cand.end(), [&](int x, int y) { return nint[x] < nint[y]; });
+            if (big == cand.end() || nint[*big] <= target) break;
+            const int u = *big;
+            cand.erase(big);
+            for (int c : ch[u])
+                if (!ch[c].empty()) cand.push_back(c);
+        }
+        std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return nint[x] > nint[y]; });
+        proots = cand;
+        for (size_t p = 0; p < proots.size(); ++p) {
+            std::vector<int> st{proots[p]};
+            while (!st.empty()) {
+                const int u = st.back();
+                st.pop_back();
+                part_of[u] = (int)p;
+                for (int c : ch[u]) st.push_back(c);
+            }
+        }
+    } else if (split && (int)h->order.size() >= 16) {
         std::vector<int> nint(n_ext, 0);
         for (int u : h->order) {
             nint[u] = 1;
@@ -578,6 +692,65 @@ void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean,
     // post-order of every part (top last), filtered from the whole-tree order
     std::vector<std::vector<int>> porder(np + 1);
     for (int u : h->order) porder[part_of[u] < 0 ? np : part_of[u]].push_back(u);
+    o.islot.clear();
+    o.vsl.clear();
+    o.bsel.clear();
+    o.bgroup.clear();
+    o.n_dslot = o.n_vslot = 0;
+    if (ck) {
+        std::vector<int> par(n_ext, -1);
+        for (int u = 0; u < n_ext; ++u)
+            for (int c : ch[u]) par[c] = u;
+        // D: persistent top-part nodes and part roots, then the segment
+        o.islot.assign(n_ext, -1);
+        int pd = 0;
+        for (int u : h->order)
+            if (part_of[u] < 0 || is_proot[u]) o.islot[u] = pd++;
+        int seg = 0;
+        for (int p = 0; p < np; ++p) {
+            int k = 0;
+            for (int u : porder[p])
+                if (!is_proot[u]) o.islot[u] = pd + k++;
+            seg = std::max(seg, k);
+        }
+        o.n_dslot = std::max(1, pd + seg);
+        o.n_dpersist = pd;
+        // V: branches below top-part nodes persistent, the others in the
+        // segment of their parent's part (the problem's V slot order within each)
+        std::vector<int> byv(n_ext, -1);
+        for (int c = 0; c < n_ext; ++c)
+            if (c != h->ext_root) byv[h->vslot[c]] = c;
+        o.vsl.assign(n_ext, -1);
+        std::vector<int> vk(np, 0);
+        int pv = 0;
+        for (int v = 0; v < n_ext; ++v) {
+            const int c = byv[v];
+            if (c >= 0 && part_of[par[c]] < 0) o.vsl[c] = pv++;
+        }
+        int vseg = 0;
+        for (int v = 0; v < n_ext; ++v) {
+            const int c = byv[v];
+            if (c < 0 || part_of[par[c]] < 0) continue;
+            const int p = part_of[par[c]];
+            o.vsl[c] = pv + vk[p]++;
+            vseg = std::max(vseg, vk[p]);
+        }
+        o.n_vslot = std::max(1, pv + vseg);
+        o.n_vpersist = pv;
+        // adjoint groups of the branch list
+        std::vector<std::vector<int32_t>> grp(np + 1);
+        for (int bi = 0; bi < h->n_branch; ++bi) {
+            const int c = h->blist[(size_t)bi * BR_STRIDE];
+            grp[part_of[par[c]] < 0 ? 0 : part_of[par[c]] + 1].push_back(bi);
+        }
+        o.bgroup.push_back(0);
+        for (auto &g : grp) {
+            o.bsel.insert(o.bsel.end(), g.begin(), g.end());
+            o.bgroup.push_back((int32_t)o.bsel.size());
+        }
+        isl = o.islot.data();
+        vsl = o.vsl.data();
+    }
     o.parts.assign((size_t)(np + 1) * 4, 0);
     // ---- inside: free-list LDS slots, spill to HBM when none is free ----
     std::vector<int32_t> where(n_ext, -1);  // LDS slot of a computed internal node (-1: HBM)
@@ -590,7 +763,7 @@ void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean,
         for (int u : porder[p]) {
             int32_t e[SCHED_STRIDE] = {0};
             e[0] = u;
-            e[1] = h->int_slot[u];
+            e[1] = isl[u];
             e[2] = (int32_t)ch[u].size();
             for (size_t c = 0; c < ch[u].size(); ++c) {
                 const int cn = ch[u][c];
@@ -637,9 +810,9 @@ void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean,
             const int next = i > 0 ? ord[i - 1] : -1;  // entry processed after u
             int32_t e[OSCHED_STRIDE] = {0};
             e[0] = u;
-            e[1] = h->int_slot[u];
+            e[1] = isl[u];
             e[2] = (int32_t)ch[u].size();
-            e[3] = (u == h->ext_root) ? -1 : h->vslot[u];
+            e[3] = (u == h->ext_root) ? -1 : vsl[u];
             e[4] = h->ext_ident[u];
             e[5] = usrc[u];
             if (usrc[u] >= 0) free_slots.push_back(usrc[u]);
@@ -651,7 +824,7 @@ void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean,
                 const int cn = och[c];
                 e[6 + 4 * c] = cn;
                 e[7 + 4 * c] = ref_of(cn);
-                e[8 + 4 * c] = h->vslot[cn];
+                e[8 + 4 * c] = vsl[cn];
                 int dst = -1;
                 if (!ch[cn].empty()) {
                     if (is_proot[cn]) {
@@ -677,15 +850,99 @@ void make_schedules(const hy_engine_s *h, int T, int esz, bool split, bool lean,
     o.n_uspill = spills;
 }
 
+// Host-side check of a checkpointed layout before anything reaches the
+// device: every D index, V slot and tip row within its buffer; the top part's
+// entries use persistent slots only; each part's entries reference only the
+// part's own nodes (the segment is shared by the parts in turn); every branch
+// in exactly one adjoint group.  Returns an empty string when valid.
+std::string check_ck_layout(const hy_engine_s *h, const Schedules &o) {
+    char msg[256];
+    const int np = o.n_parts;
+    if ((int)o.parts.size() != 4 * (np + 1) || (int)o.bgroup.size() != np + 2) return "part tables";
+    const auto &ch = h->ext_children;
+    const int n_ext = (int)ch.size();
+    auto ref_ok = [&](int ref, bool top) {
+        const int kind = ref >> 28, idx = ref & REF_MASK;
+        if (kind == KIND_TIP) return idx >= 0 && idx <= h->num_taxa;
+        return idx >= 0 && idx < (top ? o.n_dpersist : o.n_dslot);
+    };
+    for (int p = 0; p <= np; ++p) {
+        const bool top = p == np;
+        std::vector<uint8_t> mine(n_ext, 0);
+        const int i0 = o.parts[4 * p], ni = o.parts[4 * p + 1], q0 = o.parts[4 * p + 2], nq = o.parts[4 * p + 3];
+        if (i0 < 0 || ni < 0 || (size_t)(i0 + ni) * SCHED_STRIDE > o.sched.size() || q0 < 0 || nq < 0 ||
+            (size_t)(q0 + nq) * OSCHED_STRIDE > o.osched.size())
+            return "part ranges";
+        for (int k = i0; k < i0 + ni; ++k) {
+            const int32_t *e = &o.sched[(size_t)k * SCHED_STRIDE];
+            if (e[0] < 0 || e[0] >= n_ext) return "inside node";
+            mine[e[0]] = 1;
+        }
+        for (int k = i0; k < i0 + ni; ++k) {
+            const int32_t *e = &o.sched[(size_t)k * SCHED_STRIDE];
+            if (e[1] < 0 || e[1] >= o.n_dslot || (top && e[1] >= o.n_dpersist) || e[2] < 1 || e[2] > MAXC) {
+                snprintf(msg, sizeof(msg), "inside entry %d of part %d", k, p);
+                return msg;
+            }
+            for (int c = 0; c < e[2]; ++c) {
+                const int cn = e[6 + 3 * c], ref = e[7 + 3 * c];
+                if (!ref_ok(ref, top)) return "inside child reference";
+                if ((ref >> 28) == KIND_INT && !top && !mine[cn]) return "inside child outside its part";
+            }
+        }
+        for (int k = q0; k < q0 + nq; ++k) {
+            const int32_t *e = &o.osched[(size_t)k * OSCHED_STRIDE];
+            if (e[0] < 0 || e[0] >= n_ext || !mine[e[0]]) return "outside node";
+            if (e[1] < 0 || e[1] >= o.n_dslot || (top && e[1] >= o.n_dpersist)) return "outside D index";
+            if (e[0] != h->ext_root && (e[3] < 0 || e[3] >= o.n_vslot)) return "outside V slot";
+            for (int c = 0; c < e[2]; ++c) {
+                const int cn = e[6 + 4 * c], ref = e[7 + 4 * c], vs = e[8 + 4 * c];
+                if (!ref_ok(ref, false) || vs < 0 || vs >= (top ? o.n_vpersist : o.n_vslot))
+                    return "outside child reference or V slot";
+                if ((ref >> 28) == KIND_INT && !top && !mine[cn]) return "outside child outside its part";
+            }
+        }
+    }
+    std::vector<int> seen(h->n_branch, 0);
+    for (int32_t bi : o.bsel) {
+        if (bi < 0 || bi >= h->n_branch) return "branch selection";
+        ++seen[bi];
+    }
+    for (int bi = 0; bi < h->n_branch; ++bi) {
+        if (seen[bi] != 1) return "branch not in exactly one group";
+        const int c = h->blist[(size_t)bi * BR_STRIDE];
+        if (o.vsl[c] < 0 || o.vsl[c] >= o.n_vslot || (!ch[c].empty() && (o.islot[c] < 0 || o.islot[c] >= o.n_dslot)))
+            return "branch slots";
+    }
+    return std::string();
+}
+
 void build_schedules(hy_engine h, int T, int esz, bool split) {
     Schedules o;
-    make_schedules(h, T, esz, split, h->lean, h->lean_out, o);
+    make_schedules(h, T, esz, split, h->lean, h->lean_out, o, h->ck);
+    if (h->ck) h->ck_error = check_ck_layout(h, o);
     h->sched = std::move(o.sched);
     h->osched = std::move(o.osched);
     h->parts = std::move(o.parts);
     h->n_parts = o.n_parts;
     h->n_uspill = o.n_uspill;
-    h->sched_key = T * 16 + esz + (h->lean ? 4096 : 0) + (split ? 8192 : 0) + (h->lean_out ? 16384 : 0);
+    if (h->ck) {
+        h->ck_dslots = o.n_dslot;
+        h->ck_vslots = o.n_vslot;
+        h->ck_groups = std::move(o.bgroup);
+        h->ck_bsel = std::move(o.bsel);
+        // branch list entries [1] (V slot) and [2] (child reference) in the
+        // checkpointed layout, same order as the branch list
+        h->blist_ck = h->blist;
+        for (int bi = 0; bi < h->n_branch; ++bi) {
+            int32_t *e = &h->blist_ck[(size_t)bi * BR_STRIDE];
+            const int c = e[0];
+            e[1] = o.vsl[c];
+            if ((e[2] >> 28) == KIND_INT) e[2] = (KIND_INT << 28) | o.islot[c];
+        }
+    }
+    h->sched_key = T * 16 + esz + (h->lean ? 4096 : 0) + (split ? 8192 : 0) + (h->lean_out ? 16384 : 0) +
+                   (h->ck ? 32768 : 0);
 }
 
 
@@ -843,7 +1100,18 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     pr.genes = (const GeneInfo *)h->d_genes.p;
     pr.tile_gene = (const int32_t *)h->d_tile_gene.p;
     pr.root_out = 1;
-    if (h->n_parts > 0) {
+    if (h->ck && h->n_parts > 0) {
+        // checkpointed: the parts one after another (they share the segment),
+        // each leaving its root's partials; then the top part
+        pr.parts = (const int4 *)h->d_parts.p;
+        pr.root_out = 0;
+        for (int p = 0; p < h->n_parts; ++p) {
+            pr.part0 = p;
+            launch_inside<R>(pr, h->T, false, 1, s);
+        }
+        pr.part0 = h->n_parts;
+        pr.root_out = 1;
+    } else if (h->n_parts > 0) {
         // subtree parts, then the top part (root likelihood)
         pr.parts = (const int4 *)h->d_parts.p;
         pr.part0 = 0;
@@ -857,10 +1125,9 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     launch_inside<R>(pr, h->T, h->lean && h->n_parts == 0, 1, s);
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[3], s));
     if (grad) {
-        launch_outside_parts<R>(h, pr, custom_pi == nullptr && !h->has_sink, s);
-        if (prof) HIP_TRY(h, hipEventRecord(h->ev[4], s));
+        const bool rev = custom_pi == nullptr && !h->has_sink;
         AdjArgs<R> aa{};
-        aa.blist = (const int32_t *)h->d_blist.p;
+        aa.blist = (const int32_t *)(h->ck ? h->d_blist_ck.p : h->d_blist.p);
         aa.n_branch = h->n_branch;
         aa.n_chunks = h->n_chunks;
         aa.tiles_per_chunk = h->tiles_per_chunk;
@@ -879,7 +1146,39 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         aa.S = h->S;
         aa.Apart = (double *)h->d_Apart.p;
         aa.genes = (const GeneInfo *)h->d_genes.p;
-        if (h->n_branch > 0) launch_adjoint<R>(aa, h->T, s);
+        if (h->ck && h->n_parts > 0) {
+            // checkpointed: outside pass of the top part and the adjoint of the
+            // branches below it; then per part: its inside partials again
+            // (into the segment), its outside pass, the adjoint of its branches
+            auto adjoint_group = [&](int g) {
+                const int b0 = h->ck_groups[g], nbg = h->ck_groups[g + 1] - b0;
+                if (nbg <= 0) return;
+                AdjArgs<R> ag = aa;
+                ag.bsel = (const int32_t *)h->d_bsel.p + b0;
+                ag.n_branch = nbg;
+                launch_adjoint<R>(ag, h->T, s);
+            };
+            PruneArgs<R> po = pr;
+            po.parts = (const int4 *)h->d_parts.p;
+            po.part0 = h->n_parts;
+            launch_outside<R>(po, h->T, rev, false, 1, s);
+            adjoint_group(0);
+            for (int p = 0; p < h->n_parts; ++p) {
+                PruneArgs<R> pi = pr;
+                pi.parts = (const int4 *)h->d_parts.p;
+                pi.part0 = p;
+                pi.root_out = 0;
+                launch_inside<R>(pi, h->T, false, 1, s);
+                po.part0 = p;
+                launch_outside<R>(po, h->T, rev, false, 1, s);
+                adjoint_group(p + 1);
+            }
+            if (prof) HIP_TRY(h, hipEventRecord(h->ev[4], s));
+        } else {
+            launch_outside_parts<R>(h, pr, rev, s);
+            if (prof) HIP_TRY(h, hipEventRecord(h->ev[4], s));
+            if (h->n_branch > 0) launch_adjoint<R>(aa, h->T, s);
+        }
         if (prof) HIP_TRY(h, hipEventRecord(h->ev[5], s));
         FrechetArgs fa{};
         fa.blist = (const int32_t *)h->d_blist.p;
@@ -2489,6 +2788,11 @@ int eval_from_P(hy_engine h, int precision, const double *P, const double *pi, d
     }
     HIP_TRY(h, hipSetDevice(h->device));
     TRY(prepare_buffers(h, precision));
+    if (h->ck && (flags & RUN_OUTSIDE))
+        return fail(h, HY_ERR_UNSUPPORTED,
+                    "%s: the partials of this problem exceed the memory budget (checkpointed layout); the "
+                    "workspace read-back needs every node's partials resident (raise hy_set_memory_budget)",
+                    who);
     std::vector<double> pp(64, 0.0);
     for (int i = 0; i < h->S; ++i) pp[i] = pi[i];
     TRY(upload(h, h->d_pi_custom, pp.data(), 64 * sizeof(double)));
@@ -2681,6 +2985,9 @@ int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E,
                      int32_t *LE) {
     if (!h) return HY_ERR_ARG;
     if (h->part_ri < 0) return fail(h, HY_ERR_STATE, "hy_read_partials: no evaluation has left partials");
+    if (h->ck)
+        return fail(h, HY_ERR_UNSUPPORTED,
+                    "hy_read_partials: the last evaluation kept only checkpointed partials (memory budget)");
     if (p_start < 0 || p_end > h->P || p_start >= p_end)
         return fail(h, HY_ERR_ARG, "hy_read_partials: pattern range [%d, %d) outside [0, %d)", p_start, p_end, h->P);
     if ((V || G) && !h->part_outside)
@@ -2777,6 +3084,21 @@ int hy_last_timings(hy_engine h, double *ms8) {
     if (!h || !ms8) return HY_ERR_ARG;
     if (!h->profile) return fail(h, HY_ERR_STATE, "timings need HY_PROFILE=1 at hy_create");
     std::memcpy(ms8, h->timings, sizeof(h->timings));
+    return HY_OK;
+}
+
+int hy_set_memory_budget(hy_engine h, size_t bytes) {
+    if (!h) return HY_ERR_ARG;
+    h->mem_budget = bytes;
+    h->sched_key = 0;  // the layout is decided again at the next evaluation
+    ++h->epoch;
+    return HY_OK;
+}
+
+int hy_last_checkpoint(hy_engine h, int *n_parts, size_t *partial_bytes) {
+    if (!h) return HY_ERR_ARG;
+    if (n_parts) *n_parts = h->ck ? std::max(h->n_parts, 0) : 0;
+    if (partial_bytes) *partial_bytes = h->partial_bytes;
     return HY_OK;
 }
 

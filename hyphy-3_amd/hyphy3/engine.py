@@ -33,7 +33,8 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_eval", "hy_loglik_from_P", "hy_set_tile", "hy_last_timings", "hy_device_bytes",
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
-           "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count")
+           "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count",
+           "hy_set_memory_budget", "hy_last_checkpoint")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -77,8 +78,11 @@ def load_library(path: str = LIB_PATH):
     L.hy_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
                           C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.hy_loglik_from_P.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
+    L.hy_inside_outside_from_P.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
     L.hy_loglik_from_P_noalloc.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
     L.hy_graph_count.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.hy_set_memory_budget.argtypes = [C.c_void_p, C.c_size_t]
+    L.hy_last_checkpoint.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.hy_set_sink.argtypes = [C.c_void_p, _f64p]
     L.hy_last_timings.argtypes = [C.c_void_p, _f64p]
@@ -336,6 +340,16 @@ class Engine:
                                             _p(s, _f64p)))
         return (ll.value, s[:self.num_patterns]) if site else ll.value
 
+    def inside_outside_from_P(self, P, pi, precision=HY_PREC_FP64):
+        """hy_inside_outside_from_P: lnL from caller-supplied matrices, with
+        the outside pass run so that the partials can be read back."""
+        P = _f64(P)
+        pi = _f64(pi)
+        ll = C.c_double()
+        self._check(self.L.hy_inside_outside_from_P(self.h, precision, _p(P, _f64p), _p(pi, _f64p), C.byref(ll),
+                                                    None))
+        return ll.value
+
     def loglik_from_P_noalloc(self, P, pi, precision=HY_PREC_FP64, site=False):
         """loglik_from_P through hy_loglik_from_P_noalloc: no allocation on
         the call (fixed per-device scratch; calls on one device must not
@@ -393,6 +407,18 @@ class Engine:
         t = C.c_int()
         self._check(self.L.hy_last_parts(self.h, C.byref(t)))
         return t.value
+
+    def set_memory_budget(self, nbytes):
+        """Bytes allowed for the partials (0: the device's free memory); above
+        it evaluations checkpoint and recompute inside partials."""
+        self._check(self.L.hy_set_memory_budget(self.h, int(nbytes)))
+
+    def last_checkpoint(self):
+        """(parts recomputed by the last evaluation, 0 = all partials
+        resident; bytes of its partial buffers)."""
+        n, b = C.c_int(), C.c_size_t()
+        self._check(self.L.hy_last_checkpoint(self.h, C.byref(n), C.byref(b)))
+        return n.value, b.value
 
     def device_bytes(self):
         v = C.c_size_t()

@@ -314,6 +314,27 @@ int hy_last_eigen_status(hy_engine h, int *status, int n);
  * engine was created with HY_PROFILE=1 in the environment. */
 int hy_last_timings(hy_engine h, double *ms8);
 
+/* Memory budget of the partials (inside partials D, outside messages V,
+ * spilled messages and the exponents) in bytes; 0 (default) = the device's
+ * free memory at the evaluation (plus what the engine's partial buffers hold
+ * already, less 1 GiB).  When every node's partials would exceed it, the
+ * engine checkpoints (inside partials kept at about sqrt(internal nodes)
+ * nodes and recomputed): the tree is cut into parts of at most about
+ * sqrt(internal nodes) internal nodes; only the partials of the top part
+ * and of the part roots stay resident, and each part's interior is
+ * recomputed into one segment shared by the parts right before that part's
+ * outside pass and adjoint.  Same results to rounding; costs about one more
+ * inside pass; the workspace read-back (hy_inside_outside_from_P,
+ * hy_read_partials) is refused then.  Applies from the next evaluation.  No
+ * counterpart in the reference, which materialises every node's partials
+ * (likelihood.hpp:1310-1315). */
+int hy_set_memory_budget(hy_engine h, size_t bytes);
+
+/* Checkpointing of the last evaluation: *n_parts parts recomputed (0: all
+ * partials resident) and *partial_bytes held by the partial buffers of its
+ * precision.  No counterpart in the reference. */
+int hy_last_checkpoint(hy_engine h, int *n_parts, size_t *partial_bytes);
+
 /* Instantiated evaluation graphs alive in this process and their cap.
  * Evaluations capture one HIP graph per (precision, gradient, warm start)
  * and replay it; at most *cap graphs (HY_MAX_GRAPHS in the environment,
