@@ -190,6 +190,9 @@ def main():
                     help="instead of the throughput run: BUSTED expectation steps (omega and synonymous-rate weights)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--gtr", action="store_true",
+                    help="instead of the codon run: the nucleotide GTR stage (lnL + branch and exchangeability "
+                         "gradients) on the workload's nucleotide view, against the oracle on the host")
     ap.add_argument("--memory-budget", type=float, default=0.0,
                     help="GB allowed for the partials; below what every node's partials need the engine "
                          "checkpoints inside partials (about sqrt(internal nodes) parts recomputed)")
@@ -212,6 +215,9 @@ def main():
         return
     if args.estep:
         bench_estep(args, cfg)
+        return
+    if args.gtr:
+        bench_gtr(args, cfg)
         return
     sim, aln, tree, arrays, tau = build_problem(cfg)
     print(f"[bench] {args.config}: taxa={cfg['taxa']} codons={cfg['codons']} patterns={len(arrays['weights'])} "
@@ -420,6 +426,75 @@ def bench_estep(args, cfg):
         res["cpu_literal_M1"] = {"seconds": time.perf_counter() - t0, "weights": list(rw), "cores": 1,
                                  "kind": "oracle: literal restatement over the reference workspaces (numpy)"}
         res["max_rel_diff_fp64_vs_literal"] = float(np.max(np.abs(np.array(res["weights_M1_fp64"]) - rw) / rw))
+    print(json.dumps(res), flush=True)
+
+
+def bench_gtr(args, cfg):
+    """Nucleotide GTR stage (hy_gtr_eval; compute_gtr_log_likelihood,
+    likelihood.hpp:181-254) on the workload's nucleotide view: lnL plus the
+    branch-length and exchangeability gradients per step, FP64 and FP32; the
+    oracle (FP64, host cores) evaluates the same lnL + gradient for the CPU
+    column."""
+    import hyphy3
+    from hyphy3 import simulate
+    from hyphy3.engine import Engine
+    sim = simulate.simulate(cfg["taxa"], cfg["codons"], cfg["seed"], omegas=cfg["omegas"], weights=cfg["weights"],
+                            gap_fraction=cfg["gaps"])
+    aln = hyphy3.Alignment.from_text(sim["fasta"])
+    nuc = aln.to_nucleotide_alignment()
+    tree = hyphy3.Tree.from_newick(sim["newick"], keep_rooted=False)
+    arrays = hyphy3.problem_arrays(tree, nuc)
+    children = [list(arrays["child_idx"][arrays["child_ptr"][i]:arrays["child_ptr"][i + 1]])
+                for i in range(len(arrays["parent"]))]
+    eng = Engine(0)
+    eng.set_problem(arrays["parent"], children, arrays["leaf_taxon"], arrays["states"], arrays["masks"],
+                    arrays["weights"], S=4)
+    bl = np.array([0.0 if n.id == tree.root_id else n.branch_length for n in tree.nodes])
+    theta, freqs = np.asarray(sim["theta"], dtype=float), np.asarray(arrays["pi"], dtype=float)
+    res = {"workload": f"{args.config} nucleotide view: {cfg['taxa']} taxa x {3 * cfg['codons']} sites "
+                       f"({len(arrays['weights'])} patterns), GTR lnL + gradients (branches, 5 exchangeabilities)",
+           "steps": args.steps, "warmup": args.warmup}
+    outs = {}
+    for prec, name in ((1, "fp32"), (0, "fp64")):
+        for i in range(args.warmup):
+            eng.gtr_eval(bl * (1.0 + 1e-3 * i), theta, freqs, precision=prec)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            out = eng.gtr_eval(bl * (1.0 + 1e-3 * (i % 7)), theta, freqs, precision=prec)
+        dt = (time.perf_counter() - t0) / args.steps
+        res[f"{name}_ms_per_eval"] = dt * 1e3
+        res[f"{name}_evals_per_s"] = 1.0 / dt
+        outs[name] = eng.gtr_eval(bl, theta, freqs, precision=prec)
+    g32 = np.concatenate([outs["fp32"]["grad_bl"], outs["fp32"]["grad_theta"]])
+    g64 = np.concatenate([outs["fp64"]["grad_bl"], outs["fp64"]["grad_theta"]])
+    res["rel_err_logl_fp32_vs_fp64"] = abs(outs["fp32"]["log_likelihood"] - outs["fp64"]["log_likelihood"]) / \
+        abs(outs["fp64"]["log_likelihood"])
+    res["rel_err_grad_fp32_vs_fp64_max_component"] = float(np.max(np.abs(g32 - g64) / np.maximum(
+        np.abs(g64), 1e-2 * np.abs(g64).max())))
+    res["logL"] = outs["fp64"]["log_likelihood"]
+    eng.close()
+    if not args.no_cpu:
+        from oracle import data_ref as dr
+        from oracle import hyoracle as ho
+        names, seqs = dr.parse_fasta(sim["fasta"])
+        naln = dr.NucleotideRef(dr.AlignmentRef(names, seqs))
+        rtree = dr.TreeRef.from_newick(sim["newick"], keep_rooted=False)
+        prob = ho.nuc_problem(rtree, naln)
+        rbl = np.array([0.0 if n.id == rtree.root else n.bl for n in rtree.nodes])
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+        n, t0 = 0, time.perf_counter()
+        while True:
+            ref = ho.gtr_eval(prob, theta, naln.nuc_freqs, rbl, threads=threads)
+            n += 1
+            if time.perf_counter() - t0 >= args.cpu_seconds or n >= 50:
+                break
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n / el, "unit": "evals/s", "cores": threads, "kind": "port",
+                               "sample": f"full problem, {n} lnL + gradient evaluations in {el:.1f} s (oracle: "
+                                         "OpenMP pruning and inside-outside workspace, numpy gradient assembly)"}
+        res["rel_err_logl_fp64_vs_oracle"] = abs(res["logL"] - ref["log_likelihood"]) / abs(ref["log_likelihood"])
     print(json.dumps(res), flush=True)
 
 

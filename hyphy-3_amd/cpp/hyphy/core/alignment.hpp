@@ -444,6 +444,61 @@ public:
     }
 
     // Flat pattern-major arrays for the GPU engine.
+    // Nucleotide view for the GTR stage (Alignment::to_nucleotide_alignment,
+    // alignment.hpp:832-890 of the reference): one site per nucleotide
+    // column, state = A/C/G/T index or -1, mask = the IUPAC 4-bit mask (any
+    // other character: all four), patterns keyed on the column of masks in
+    // first-occurrence order; no codons (num_codons = 0), the nucleotide
+    // frequencies of this alignment.  An alignment without codons is its own
+    // nucleotide view.
+    Alignment to_nucleotide_alignment() const {
+        if (num_codons == 0 && !patterns.empty()) return *this;
+        Alignment n;
+        n.taxon_names = taxon_names;
+        n.taxon_to_index = taxon_to_index;
+        n.lower_taxon_to_index = lower_taxon_to_index;
+        n.num_taxa = num_taxa;
+        n.num_nucleotides = num_nucleotides;
+        n.num_codons = 0;
+        n.code = code;
+        n.embedded_tree_newick = embedded_tree_newick;
+        n.nuc_frequencies = nuc_frequencies;
+        n.sequences.resize(num_taxa);
+        for (size_t t = 0; t < num_taxa; ++t) {
+            n.sequences[t].name = sequences[t].name;
+            n.sequences[t].raw_seq = sequences[t].raw_seq;
+        }
+        n.site_to_pattern.assign(num_nucleotides, 0);
+        // the column's masks identify the pattern (states follow from them)
+        std::unordered_map<std::string, size_t> seen;
+        std::string col(num_taxa, '\0');
+        for (size_t site = 0; site < num_nucleotides; ++site) {
+            for (size_t t = 0; t < num_taxa; ++t) col[t] = char('A' + char_to_iupac_mask(sequences[t].raw_seq[site]));
+            auto it = seen.find(col);
+            if (it != seen.end()) {
+                n.patterns[it->second].weight += 1;
+                n.patterns[it->second].sites.push_back(site);
+                n.site_to_pattern[site] = it->second;
+                continue;
+            }
+            const size_t idx = n.patterns.size();
+            seen.emplace(col, idx);
+            SitePattern sp;
+            sp.states.resize(num_taxa);
+            sp.ambiguous_masks.resize(num_taxa);
+            for (size_t t = 0; t < num_taxa; ++t) {
+                const char ch = sequences[t].raw_seq[site];
+                sp.states[t] = char_to_nuc(ch);
+                sp.ambiguous_masks[t] = char_to_iupac_mask(ch);
+            }
+            sp.weight = 1;
+            sp.sites.push_back(site);
+            n.patterns.push_back(std::move(sp));
+            n.site_to_pattern[site] = idx;
+        }
+        return n;
+    }
+
     void pattern_arrays(std::vector<int8_t> &states, std::vector<uint64_t> &masks, std::vector<double> &weights) const {
         const size_t P = patterns.size();
         states.resize(P * num_taxa);

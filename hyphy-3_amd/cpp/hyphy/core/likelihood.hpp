@@ -116,6 +116,38 @@ public:
     }
 };
 
+// Nucleotide GTR model (rate_matrix.hpp:14-104): exchangeabilities with
+// A<->G fixed to 1 and the frequencies; the engine builds the normalised
+// generator, its spectrum and the transition matrices.
+struct GTRParameters {
+    Scalar theta_AC = 1.0;
+    Scalar theta_AT = 1.0;
+    Scalar theta_CG = 1.0;
+    Scalar theta_CT = 1.0;
+    Scalar theta_GT = 1.0;
+    std::array<double, 5> theta() const { return {theta_AC, theta_AT, theta_CG, theta_CT, theta_GT}; }
+};
+
+class GTRMatrix {
+public:
+    GTRParameters params;
+    std::array<Scalar, 4> pi{0, 0, 0, 0};
+    Scalar scale_factor = 1.0;  // mean rate of the unnormalised generator
+
+    void update(const GTRParameters &p, const std::array<Scalar, 4> &frequencies) {
+        params = p;
+        pi = frequencies;
+        const auto th = p.theta();
+        const double r[4][4] = {{0, th[0], 1, th[1]}, {th[0], 0, th[2], th[3]}, {1, th[2], 0, th[4]},
+                                {th[1], th[3], th[4], 0}};
+        double mu = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                if (i != j) mu += pi[i] * r[i][j] * pi[j];
+        scale_factor = mu > 0.0 ? mu : 1.0;
+    }
+};
+
 struct JointMG94Gradients {
     Scalar log_likelihood = 0.0;
     std::vector<Scalar> grad_branches;
@@ -214,6 +246,23 @@ public:
         const auto th = mg.params.theta();
         check(h, hy_eval(h.e, get_precision(), HY_WANT_LOGL, bl.data(), mg.params.alpha, &om, &w, th.data(),
                          mg.params.delta, mg.params.psi, &ll, nullptr, nullptr, nullptr, nullptr, nullptr));
+        return ll;
+    }
+
+    // compute_gtr_log_likelihood (likelihood.hpp:181-254): aln is the
+    // nucleotide view (Alignment::to_nucleotide_alignment)
+    static Scalar compute_gtr_log_likelihood(const Tree &tree, const Alignment &aln, const GTRMatrix &gtr,
+                                             const std::vector<size_t> *leaf_to_taxon = nullptr) {
+        if (aln.num_codons != 0)
+            throw std::invalid_argument("compute_gtr_log_likelihood needs the nucleotide view of the alignment");
+        Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
+        ++h.epoch;
+        std::vector<double> bl(tree.num_nodes(), 0.0);
+        for (const auto &n : tree.nodes) bl[n.id] = n.branch_length;
+        const auto th = gtr.params.theta();
+        double ll = 0.0;
+        check(h, hy_gtr_eval(h.e, get_precision(), HY_WANT_LOGL, bl.data(), th.data(), gtr.pi.data(), &ll, nullptr,
+                             nullptr, nullptr));
         return ll;
     }
 
@@ -563,7 +612,9 @@ private:
         std::vector<size_t> local;
         const std::vector<size_t> &l2t = leaf_to_taxon ? *leaf_to_taxon : (local = build_leaf_to_taxon_map(tree, aln));
         const uint64_t key = fingerprint(tree, aln, l2t);
-        if (key != h.key || h.code != code) {
+        // a nucleotide view (no codons) is a 4-state problem
+        const int S = aln.num_codons == 0 ? 4 : GeneticCode::from_id(code)->num_sense_codons;
+        if (key != h.key || h.code != code || h.S != S) {
             const int N = int(tree.num_nodes());
             std::vector<int32_t> parent(N), cp(N + 1, 0), ci, lt(N, -1);
             for (const auto &n : tree.nodes) {
@@ -578,7 +629,6 @@ private:
             std::vector<uint64_t> mk;
             std::vector<double> w;
             aln.pattern_arrays(st, mk, w);
-            const int S = GeneticCode::from_id(code)->num_sense_codons;
             h.pos.clear();
             h.code = -1;
             check(h, hy_set_problem(h.e, N, parent.data(), cp.data(), ci.data(), lt.data(), int(aln.num_taxa),

@@ -135,6 +135,14 @@ struct hy_engine_s {
     // weights, the inside schedule and the identity edges' nodes
     void *scratch = nullptr;
     hipError_t scratch_err = hipSuccess;
+    // nucleotide GTR stage (problems set with S = 4): node tables, padded tips
+    // and weights (built at the first hy_gtr_eval after hy_set_problem) and
+    // per-evaluation buffers
+    bool gtr_ready = false;
+    int g_P_pad = 0, g_n_sched = 0;
+    DevBuf d_gsched, d_gnref, d_gnbr, d_gtips, d_gw, d_gP, d_gpi, d_gD, d_gE, d_gU, d_gH, d_gL, d_gLE, d_gsite,
+        d_gApart, d_gA, d_gres;
+    std::vector<uint8_t> g_ident;  // extended nodes whose edge is an identity
     std::vector<uint16_t> na_tips;
     std::vector<double> na_weights;
     std::vector<int32_t> na_sched, na_ident;
@@ -1360,7 +1368,9 @@ void hy_destroy(hy_engine h) {
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
                       &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
                       &h->d_site, &h->d_Apart, &h->d_fr, &h->d_wl, &h->d_proj, &h->d_wover,
-                      &h->d_sink, &h->d_sinkwl};
+                      &h->d_sink, &h->d_sinkwl, &h->d_blist_ck, &h->d_bsel, &h->d_gsched, &h->d_gnref,
+                      &h->d_gnbr, &h->d_gtips, &h->d_gw, &h->d_gP, &h->d_gpi, &h->d_gD, &h->d_gE, &h->d_gU,
+                      &h->d_gH, &h->d_gL, &h->d_gLE, &h->d_gsite, &h->d_gApart, &h->d_gA, &h->d_gres};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -1673,6 +1683,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
         for (int n = 0; n < h->n_ext; ++n)
             if (h->ext_ident[n]) h->na_ident.push_back(n);
     }
+    h->gtr_ready = false;
     h->have_problem = true;
     return HY_OK;
 }
@@ -2979,6 +2990,339 @@ int hy_loglik_from_P_noalloc(hy_engine h, int precision, const double *P, const 
     HIP_TRY(h, hipSetDevice(h->device));
     return rs == 8 ? loglik_noalloc<double>(h, P, pi, out_logL, out_site_ll, lay)
                    : loglik_noalloc<float>(h, P, pi, out_logL, out_site_ll, lay);
+}
+
+}  // extern "C"
+
+namespace {
+
+// ---- nucleotide GTR stage ----
+// GTR exchangeability pairs in theta order AC, AT, CG, CT, GT (AG == 1).
+constexpr int GTR_A[5] = {0, 0, 1, 1, 2}, GTR_B[5] = {1, 3, 2, 3, 3};
+
+// Cyclic Jacobi eigensolver of a symmetric 4 x 4 matrix (row-major S,
+// destroyed): eigenvalues lam, eigenvectors in the columns of Z.
+void sym_eigen4(double *S, double *lam, double *Z) {
+    for (int i = 0; i < 16; ++i) Z[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) (i == j ? tot : off) += S[4 * i + j] * S[4 * i + j];
+        if (off <= 1e-32 * (tot + off)) break;
+        for (int p = 0; p < 3; ++p)
+            for (int q = p + 1; q < 4; ++q) {
+                const double apq = S[4 * p + q];
+                if (apq == 0.0) continue;
+                const double th = (S[4 * q + q] - S[4 * p + p]) / (2.0 * apq);
+                const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < 4; ++k) {  // columns p, q
+                    const double a = S[4 * k + p], b = S[4 * k + q];
+                    S[4 * k + p] = c * a - sn * b;
+                    S[4 * k + q] = sn * a + c * b;
+                }
+                for (int k = 0; k < 4; ++k) {  // rows p, q
+                    const double a = S[4 * p + k], b = S[4 * q + k];
+                    S[4 * p + k] = c * a - sn * b;
+                    S[4 * q + k] = sn * a + c * b;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double a = Z[4 * k + p], b = Z[4 * k + q];
+                    Z[4 * k + p] = c * a - sn * b;
+                    Z[4 * k + q] = sn * a + c * b;
+                }
+            }
+    }
+    for (int i = 0; i < 4; ++i) lam[i] = S[5 * i];
+}
+
+// GTRMatrix::update (rate_matrix.hpp:29-90): normalised Q, mu, and the
+// spectrum of D^1/2 Q D^-1/2 (pi floored at 1e-12 in the square roots,
+// eigenvalues clamped <= 0): P(t) = U diag(e^{lam t}) V.
+struct Gtr4 {
+    double Q[16], mu, lam[4], U[16], V[16];
+};
+
+Gtr4 gtr_model(const double *theta, const double *pi) {
+    Gtr4 g{};
+    double r[16] = {0};
+    for (int k = 0; k < 5; ++k) r[4 * GTR_A[k] + GTR_B[k]] = r[4 * GTR_B[k] + GTR_A[k]] = theta[k];
+    r[4 * 0 + 2] = r[4 * 2 + 0] = 1.0;  // A <-> G
+    for (int i = 0; i < 4; ++i) {
+        double rs = 0.0;
+        for (int j = 0; j < 4; ++j)
+            if (j != i) rs += (g.Q[4 * i + j] = r[4 * i + j] * pi[j]);
+        g.Q[5 * i] = -rs;
+    }
+    g.mu = 0.0;
+    for (int i = 0; i < 4; ++i) g.mu -= pi[i] * g.Q[5 * i];
+    if (g.mu > 0.0)
+        for (double &q : g.Q) q /= g.mu;
+    double sq[4], isq[4], S[16], Z[16];
+    for (int i = 0; i < 4; ++i) {
+        sq[i] = std::sqrt(std::max(pi[i], 1e-12));
+        isq[i] = 1.0 / sq[i];
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) S[4 * i + j] = sq[i] * g.Q[4 * i + j] * isq[j];
+    for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j) S[4 * i + j] = S[4 * j + i] = 0.5 * (S[4 * i + j] + S[4 * j + i]);
+    sym_eigen4(S, g.lam, Z);
+    for (int i = 0; i < 4; ++i) g.lam[i] = std::min(g.lam[i], 0.0);
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k) {
+            g.U[4 * i + k] = isq[i] * Z[4 * i + k];
+            g.V[4 * k + i] = Z[4 * i + k] * sq[i];
+        }
+    return g;
+}
+
+// transition_matrix (rate_matrix.hpp:93-99): U diag(e^{lam t}) V, clamped >= 0
+void gtr_transition(const Gtr4 &g, double t, double *P) {
+    double e[4];
+    for (int k = 0; k < 4; ++k) e[k] = std::exp(g.lam[k] * t);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < 4; ++k) v += g.U[4 * i + k] * e[k] * g.V[4 * k + j];
+            P[4 * i + j] = std::max(v, 0.0);
+        }
+}
+
+// Node tables and padded patterns of the GTR stage (once per problem).
+int gtr_prepare(hy_engine h) {
+    if (h->gtr_ready) return HY_OK;
+    const auto &ch = h->ext_children;
+    const int n_ext = (int)ch.size();
+    std::vector<int32_t> nref(n_ext), nbr(n_ext, -1), sched;
+    for (int n = 0; n < n_ext; ++n) {
+        if (ch[n].empty()) {
+            const int row = h->leaf_taxon_ext[n] >= 0 ? h->leaf_taxon_ext[n] : h->num_taxa;
+            nref[n] = (KIND_TIP << 28) | row;
+        } else {
+            nref[n] = (KIND_INT << 28) | h->int_slot[n];
+        }
+    }
+    for (int bi = 0; bi < h->n_branch; ++bi) nbr[h->blist[(size_t)bi * BR_STRIDE]] = bi;
+    h->g_ident.assign(n_ext, 0);
+    for (int n = 0; n < n_ext; ++n) h->g_ident[n] = h->ext_ident[n];
+    for (int u : h->order) {
+        int32_t e[GTR_STRIDE] = {u, h->int_slot[u], (int32_t)ch[u].size(), 0, 0, 0, 0, 0};
+        for (size_t c = 0; c < ch[u].size(); ++c) e[3 + c] = ch[u][c];
+        if (ch[u].empty()) {  // a single-leaf tree: the root's own tip on an identity edge
+            e[2] = 1;
+            e[3] = u;
+            nref[u] = (KIND_TIP << 28) | (h->leaf_taxon_ext[u] >= 0 ? h->leaf_taxon_ext[u] : h->num_taxa);
+            h->g_ident[u] = 1;
+        }
+        sched.insert(sched.end(), e, e + GTR_STRIDE);
+    }
+    if (sched.empty() || sched[sched.size() - GTR_STRIDE] != h->ext_root)
+        return fail(h, HY_ERR_STATE, "internal: GTR schedule does not end at the root");
+    const int Pp = std::max(GTR_BLOCK, (h->P + GTR_BLOCK - 1) / GTR_BLOCK * GTR_BLOCK);
+    std::vector<uint16_t> tips((size_t)(h->num_taxa + 1) * Pp, TIP_ONES);
+    for (int t = 0; t < h->num_taxa; ++t)
+        std::memcpy(&tips[(size_t)t * Pp], &h->codes[(size_t)t * h->P], sizeof(uint16_t) * h->P);
+    std::vector<double> w(Pp, 0.0);
+    std::copy(h->weights.begin(), h->weights.end(), w.begin());
+    TRY(upload(h, h->d_gsched, sched.data(), sched.size() * 4));
+    TRY(upload(h, h->d_gnref, nref.data(), nref.size() * 4));
+    TRY(upload(h, h->d_gnbr, nbr.data(), nbr.size() * 4));
+    TRY(upload(h, h->d_gtips, tips.data(), tips.size() * 2));
+    TRY(upload(h, h->d_gw, w.data(), w.size() * 8));
+    h->g_P_pad = Pp;
+    h->g_n_sched = (int)(sched.size() / GTR_STRIDE);
+    h->gtr_ready = true;
+    return HY_OK;
+}
+
+template <typename R>
+int gtr_run(hy_engine h, bool grad, const Gtr4 &g, const double *bl, const double *pi, double *out_logL,
+            double *out_site_ll, std::vector<double> &A) {
+    const size_t rs = sizeof(R), Pp = (size_t)h->g_P_pad, ni = (size_t)std::max(h->n_int, 1);
+    const int n_ext = h->n_ext, n_waves = (int)(Pp / 64);
+    TRY(ensure(h, h->d_gP, (size_t)n_ext * 16 * rs));
+    TRY(ensure(h, h->d_gpi, 4 * 8));
+    TRY(ensure(h, h->d_gD, ni * 4 * Pp * rs));
+    TRY(ensure(h, h->d_gE, ni * Pp * 4));
+    TRY(ensure(h, h->d_gL, Pp * 8));
+    TRY(ensure(h, h->d_gLE, Pp * 4));
+    TRY(ensure(h, h->d_gsite, Pp * 8));
+    TRY(ensure(h, h->d_gres, 8));
+    if (grad) {
+        TRY(ensure(h, h->d_gU, ni * 4 * Pp * rs));
+        TRY(ensure(h, h->d_gH, ni * Pp * 4));
+        TRY(ensure(h, h->d_gApart, (size_t)std::max(h->n_branch, 1) * n_waves * 16 * 8));
+        TRY(ensure(h, h->d_gA, (size_t)std::max(h->n_branch, 1) * 16 * 8));
+    }
+    // transition matrices of every extended node (identity edges and the root: I)
+    std::vector<R> Pm((size_t)n_ext * 16, R(0));
+    for (int n = 0; n < n_ext; ++n) {
+        double P[16];
+        if (h->g_ident[n] || n == h->ext_root) {
+            for (int i = 0; i < 16; ++i) P[i] = (i % 5 == 0) ? 1.0 : 0.0;
+        } else {
+            gtr_transition(g, bl[n], P);
+        }
+        for (int i = 0; i < 16; ++i) Pm[(size_t)n * 16 + i] = (R)P[i];
+    }
+    hipStream_t s = h->stream;
+    HIP_TRY(h, hipMemcpyAsync(h->d_gP.p, Pm.data(), Pm.size() * rs, hipMemcpyHostToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(h->d_gpi.p, pi, 4 * 8, hipMemcpyHostToDevice, s));
+    GtrArgs<R> a{};
+    a.sched = (const int32_t *)h->d_gsched.p;
+    a.n_sched = h->g_n_sched;
+    a.nref = (const int32_t *)h->d_gnref.p;
+    a.nbr = (const int32_t *)h->d_gnbr.p;
+    a.Pm = (const R *)h->d_gP.p;
+    a.tips = (const uint16_t *)h->d_gtips.p;
+    a.amb = (const uint64_t *)h->d_amb.p;
+    a.pi = (const double *)h->d_gpi.p;
+    a.weights = (const double *)h->d_gw.p;
+    a.P_pad = (int)Pp;
+    a.P = h->P;
+    a.n_waves = n_waves;
+    a.D = (R *)h->d_gD.p;
+    a.E = (int32_t *)h->d_gE.p;
+    a.U = (R *)h->d_gU.p;
+    a.H = (int32_t *)h->d_gH.p;
+    a.Lroot = (double *)h->d_gL.p;
+    a.Eroot = (int32_t *)h->d_gLE.p;
+    a.site_ll = (double *)h->d_gsite.p;
+    a.Apart = (double *)h->d_gApart.p;
+    launch_gtr<R>(a, grad, s);
+    GtrReduceArgs r{};
+    r.Apart = a.Apart;
+    r.site_ll = a.site_ll;
+    r.weights = a.weights;
+    r.n_branch = h->n_branch;
+    r.n_waves = n_waves;
+    r.P = h->P;
+    r.A = (double *)h->d_gA.p;
+    r.res = (double *)h->d_gres.p;
+    launch_gtr_reduce(r, grad, s);
+    HIP_TRY(h, hipGetLastError());
+    double ll = 0.0;
+    HIP_TRY(h, hipMemcpyAsync(&ll, h->d_gres.p, 8, hipMemcpyDeviceToHost, s));
+    if (out_site_ll) HIP_TRY(h, hipMemcpyAsync(out_site_ll, h->d_gsite.p, (size_t)h->P * 8, hipMemcpyDeviceToHost, s));
+    if (grad) {
+        A.assign((size_t)h->n_branch * 16, 0.0);
+        if (h->n_branch > 0)
+            HIP_TRY(h, hipMemcpyAsync(A.data(), h->d_gA.p, A.size() * 8, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(h, hipStreamSynchronize(s));
+    if (out_logL) *out_logL = ll;
+    return HY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hy_gtr_eval(hy_engine h, int precision, unsigned flags, const double *bl, const double *theta, const double *freqs,
+                double *out_logL, double *out_site_ll, double *out_grad_bl, double *out_grad_theta) {
+    const char *who = "hy_gtr_eval";
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem) return fail(h, HY_ERR_STATE, "%s before hy_set_problem", who);
+    if (h->S != 4) return fail(h, HY_ERR_STATE, "%s: the problem has %d states; the GTR stage needs S = 4", who, h->S);
+    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
+        return fail(h, HY_ERR_ARG, "%s: unknown precision %d", who, precision);
+    if (!bl || !theta || !freqs) return fail(h, HY_ERR_ARG, "%s: NULL branch lengths, rates or frequencies", who);
+    for (int k = 0; k < 5; ++k)
+        if (!(theta[k] >= 0.0 && std::isfinite(theta[k])))
+            return fail(h, HY_ERR_ARG, "%s: theta[%d] = %g", who, k, theta[k]);
+    for (int k = 0; k < 4; ++k)
+        if (!(freqs[k] >= 0.0 && freqs[k] <= 1.0)) return fail(h, HY_ERR_ARG, "%s: freqs[%d] = %g", who, k, freqs[k]);
+    for (int n = 0; n < h->num_nodes; ++n)
+        if (n != h->root && !(bl[n] >= 0.0 && std::isfinite(bl[n])))
+            return fail(h, HY_ERR_ARG, "%s: branch length %d = %g", who, n, bl[n]);
+    const bool grad = (flags & HY_WANT_GRAD) != 0;
+    if (h->P == 0) {
+        if (out_logL) *out_logL = 0.0;
+        if (grad && out_grad_bl) std::fill(out_grad_bl, out_grad_bl + h->num_nodes, 0.0);
+        if (grad && out_grad_theta) std::fill(out_grad_theta, out_grad_theta + 5, 0.0);
+        return HY_OK;
+    }
+    HIP_TRY(h, hipSetDevice(h->device));
+    TRY(gtr_prepare(h));
+    const Gtr4 g = gtr_model(theta, freqs);
+    // branch lengths of the extended nodes (identity edges: unused)
+    std::vector<double> blx(h->n_ext, 0.0);
+    for (int n = 0; n < h->num_nodes; ++n) blx[n] = (n == h->root) ? 0.0 : bl[n];
+    std::vector<double> A;
+    TRY(precision == HY_PREC_FP64
+            ? gtr_run<double>(h, grad, g, blx.data(), freqs, out_logL, (flags & HY_WANT_SITE_LL) ? out_site_ll : nullptr, A)
+            : gtr_run<float>(h, grad, g, blx.data(), freqs, out_logL, (flags & HY_WANT_SITE_LL) ? out_site_ll : nullptr, A));
+    if (!grad) return HY_OK;
+    // d lnL / d t_b = <A_b, Q P_b>; d lnL / d theta_k = sum_b <U^T A_b V^T, (V dQ_k U) o E(t_b)>
+    // with dQ_k the derivative of the normalised generator
+    std::vector<double> gbl(h->num_nodes, 0.0);
+    double gth[5] = {0, 0, 0, 0, 0}, M[5][16];
+    for (int k = 0; k < 5; ++k) {
+        const int a = GTR_A[k], b = GTR_B[k];
+        double dQ[16] = {0};
+        dQ[4 * a + b] = freqs[b];
+        dQ[4 * b + a] = freqs[a];
+        dQ[5 * a] = -freqs[b];
+        dQ[5 * b] = -freqs[a];
+        const double dmu = 2.0 * freqs[a] * freqs[b];
+        double dn[16], T[16];
+        for (int i = 0; i < 16; ++i) dn[i] = g.mu > 0.0 ? (dQ[i] - g.Q[i] * dmu) / g.mu : dQ[i];
+        for (int i = 0; i < 4; ++i)  // T = dn U
+            for (int j = 0; j < 4; ++j) {
+                double v = 0.0;
+                for (int q = 0; q < 4; ++q) v += dn[4 * i + q] * g.U[4 * q + j];
+                T[4 * i + j] = v;
+            }
+        for (int i = 0; i < 4; ++i)  // M = V T
+            for (int j = 0; j < 4; ++j) {
+                double v = 0.0;
+                for (int q = 0; q < 4; ++q) v += g.V[4 * i + q] * T[4 * q + j];
+                M[k][4 * i + j] = v;
+            }
+    }
+    for (int bi = 0; bi < h->n_branch; ++bi) {
+        const int n = h->blist[(size_t)bi * BR_STRIDE];
+        const double t = blx[n], *Ab = &A[(size_t)bi * 16];
+        double P[16], QP[16], At[16], T[16], e[4];
+        gtr_transition(g, t, P);
+        double d = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                double v = 0.0;
+                for (int q = 0; q < 4; ++q) v += g.Q[4 * i + q] * P[4 * q + j];
+                QP[4 * i + j] = v;
+                d += Ab[4 * i + j] * v;
+            }
+        gbl[n] = d;
+        // At = U^T A V^T
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                double v = 0.0;
+                for (int q = 0; q < 4; ++q) v += Ab[4 * i + q] * g.V[4 * j + q];
+                T[4 * i + j] = v;
+            }
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                double v = 0.0;
+                for (int q = 0; q < 4; ++q) v += g.U[4 * q + i] * T[4 * q + j];
+                At[4 * i + j] = v;
+            }
+        for (int i = 0; i < 4; ++i) e[i] = std::exp(g.lam[i] * t);
+        for (int k = 0; k < 5; ++k) {
+            double v = 0.0;
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) {
+                    const double dl = g.lam[i] - g.lam[j];
+                    const double E = std::fabs(dl) < 1e-9 ? t * e[i] : (e[i] - e[j]) / dl;
+                    v += At[4 * i + j] * M[k][4 * i + j] * E;
+                }
+            gth[k] += v;
+        }
+    }
+    if (out_grad_bl) std::copy(gbl.begin(), gbl.end(), out_grad_bl);
+    if (out_grad_theta) std::copy(gth, gth + 5, out_grad_theta);
+    return HY_OK;
 }
 
 int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E, double *V, int32_t *G, double *L,

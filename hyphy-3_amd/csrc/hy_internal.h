@@ -269,6 +269,41 @@ void launch_sink(const SinkArgs &a, int n_slots, hipStream_t s);
 void launch_frechet(const FrechetArgs &a, hipStream_t s);  // chunk sums, spectral transforms, projections
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s);  // one workgroup per gene
 
+// Nucleotide GTR stage (S = 4): one thread per pattern walks the tree.
+// Schedule entry (int32 x GTR_STRIDE), internal nodes in post-order (the root
+// last): [0] node, [1] D slot, [2] #children (<= 3), [3..5] child node ids.
+constexpr int GTR_STRIDE = 8;
+constexpr int GTR_BLOCK = 256;  // patterns per workgroup (P padded to a multiple)
+template <typename R>
+struct GtrArgs {
+    const int32_t *sched;
+    int n_sched;
+    const int32_t *nref;  // [n_ext] reference of a node as a child: (kind << 28) | tip row or D slot
+    const int32_t *nbr;   // [n_ext] branch index of the edge above a node (-1: root, identity edges)
+    const R *Pm;          // [n_ext][16] row-major transition matrices
+    const uint16_t *tips; // [taxa + 1][P_pad] codes: 0..3, TIP_ONES, TIP_AMB | k
+    const uint64_t *amb;  // 4-bit masks of partially ambiguous codes
+    const double *pi;     // [4]
+    const double *weights;  // [P_pad], zero beyond P
+    int P_pad, P, n_waves;
+    R *D;            // [n_int][4][P_pad] inside partials (mantissas)
+    int32_t *E;      // [n_int][P_pad] their power-of-two exponents
+    R *U;            // [n_int][4][P_pad] outside messages at the node's end of its branch
+    int32_t *H;      // [n_int][P_pad]
+    double *Lroot;   // [P_pad]
+    int32_t *Eroot;  // [P_pad]
+    double *site_ll; // [P_pad]
+    double *Apart;   // [n_branch][n_waves][16] per-wave sums of c_p V D^T
+};
+struct GtrReduceArgs {
+    const double *Apart, *site_ll, *weights;
+    int n_branch, n_waves, P;
+    double *A;    // [n_branch][16]
+    double *res;  // [0] lnL
+};
+template <typename R> void launch_gtr(const GtrArgs<R> &a, bool grad, hipStream_t s);
+void launch_gtr_reduce(const GtrReduceArgs &a, bool grad, hipStream_t s);
+
 // Fixed device scratch of the allocation-free entry point
 // (hy_loglik_from_P_noalloc): a file-scope __device__ array in
 // hy_kernels.hip, NOALLOC_SCRATCH_BYTES per device.  noalloc_scratch returns

@@ -2164,6 +2164,224 @@ void launch_final(const FinalArgs &a, int n_genes, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// Nucleotide GTR stage (S = 4; compute_gtr_log_likelihood,
+// likelihood.hpp:181-254).  Four states are no MFMA shape: one thread per
+// pattern walks the tree with 4x4 products in registers; partials and
+// messages go to HBM pattern-minor ([slot][state][pattern], coalesced).  Every
+// node's column is normalised to max in [0.5, 1) by an exact power of two
+// (exponent kept per node and pattern), as in the codon passes.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void gtr_tip(unsigned code, const uint64_t *amb, R *d) {
+    if (code < 4u) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = (unsigned)i == code ? R(1) : R(0);
+    } else if (code == TIP_ONES) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = R(1);
+    } else {
+        const uint64_t m = amb[code & 0x7FFFu];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = ((m >> i) & 1ull) ? R(1) : R(0);
+    }
+}
+
+// The child's partial (mantissa d, exponent e) and X = P_c d.
+template <typename R>
+__device__ __forceinline__ void gtr_child(const GtrArgs<R> &a, int cn, int p, R *d, int &e, R *X) {
+    const int ref = a.nref[cn], kind = ref >> 28, idx = ref & REF_MASK;
+    if (kind == KIND_TIP) {
+        gtr_tip<R>(a.tips[(size_t)idx * a.P_pad + p], a.amb, d);
+        e = 0;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = a.D[((size_t)idx * 4 + i) * a.P_pad + p];
+        e = a.E[(size_t)idx * a.P_pad + p];
+    }
+    const R *P = a.Pm + (size_t)cn * 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) X[i] = P[4 * i] * d[0] + P[4 * i + 1] * d[1] + P[4 * i + 2] * d[2] + P[4 * i + 3] * d[3];
+}
+
+template <typename R>
+__device__ __forceinline__ void gtr_normalise(R *x, int &e) {
+    const R m = fmax(fmax(x[0], x[1]), fmax(x[2], x[3]));
+    if (m > R(0)) {
+        const int k = frexp_e(m);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = ldexp_r(x[i], -k);
+        e += k;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(GTR_BLOCK) void k_gtr_inside(GtrArgs<R> a) {
+    const int p = blockIdx.x * GTR_BLOCK + threadIdx.x;  // < P_pad (grid covers P_pad exactly)
+    R acc[4];
+    int ex = 0;
+    for (int k = 0; k < a.n_sched; ++k) {
+        const int32_t *e = a.sched + (size_t)k * GTR_STRIDE;
+        const int slot = e[1], nc = e[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = R(1);
+        ex = 0;
+        for (int c = 0; c < nc; ++c) {
+            R d[4], X[4];
+            int ec;
+            gtr_child<R>(a, e[3 + c], p, d, ec, X);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] *= X[i];
+            ex += ec;
+        }
+        gtr_normalise<R>(acc, ex);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a.D[((size_t)slot * 4 + i) * a.P_pad + p] = acc[i];
+        a.E[(size_t)slot * a.P_pad + p] = ex;
+    }
+    // the last entry is the root
+    double L = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) L += a.pi[i] * (double)acc[i];
+    a.Lroot[p] = L;
+    a.Eroot[p] = ex;
+    if (p < a.P) a.site_ll[p] = L > 0.0 ? log(L) + (double)ex * 0.69314718055994530942 : -1e20;
+}
+
+template <typename R>
+__global__ __launch_bounds__(GTR_BLOCK) void k_gtr_outside(GtrArgs<R> a) {
+    const int p = blockIdx.x * GTR_BLOCK + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = p >> 6;
+    const double L = a.Lroot[p];
+    const int Er = a.Eroot[p];
+    // c_p = w_p / L_p (zero for padding and for patterns with L <= 0, which
+    // contribute no gradient), the exponents applied per branch
+    const double cw = (p < a.P && L > 0.0) ? a.weights[p] / L : 0.0;
+    for (int k = a.n_sched - 1; k >= 0; --k) {
+        const int32_t *e = a.sched + (size_t)k * GTR_STRIDE;
+        const int slot = e[1], nc = e[2];
+        R Uu[4];
+        int Hu;
+        if (k == a.n_sched - 1) {  // root: the frequencies
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Uu[i] = (R)a.pi[i];
+            Hu = 0;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Uu[i] = a.U[((size_t)slot * 4 + i) * a.P_pad + p];
+            Hu = a.H[(size_t)slot * a.P_pad + p];
+        }
+        R d[MAXC][4], X[MAXC][4];
+        int ec[MAXC];
+        // (loops over children unrolled with guards: register arrays stay in registers)
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            ec[c] = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) X[c][i] = d[c][i] = R(1);
+            if (c < nc) gtr_child<R>(a, e[3 + c], p, d[c], ec[c], X[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            if (c >= nc) continue;
+            R V[4];
+            int G = Hu;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) V[i] = Uu[i];
+#pragma unroll
+            for (int s = 0; s < MAXC; ++s) {
+                if (s == c || s >= nc) continue;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) V[i] *= X[s][i];
+                G += ec[s];
+            }
+            gtr_normalise<R>(V, G);
+            const int cn = e[3 + c], b = a.nbr[cn];
+            if (b >= 0) {
+                // A_b += c_p 2^(G + E_c - E_root) V d_c^T, summed over the wave's 64 patterns
+                const double f = cw != 0.0 ? ldexp(cw, G + ec[c] - Er) : 0.0;
+                double acc16[16];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc16[4 * i + j] = f * (double)V[i] * (double)d[c][j];
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    for (int off = 32; off > 0; off >>= 1) acc16[q] += __shfl_xor(acc16[q], off);
+                if (lane < 16) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 16; ++q)
+                        if (q == lane) v = acc16[q];
+                    a.Apart[((size_t)b * a.n_waves + wave) * 16 + lane] = v;
+                }
+            }
+            const int ref = a.nref[cn];
+            if ((ref >> 28) == KIND_INT) {
+                // message into the child: U_c = P_c^T V
+                const R *P = a.Pm + (size_t)cn * 16;
+                R Uc[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) Uc[j] = P[j] * V[0] + P[4 + j] * V[1] + P[8 + j] * V[2] + P[12 + j] * V[3];
+                gtr_normalise<R>(Uc, G);
+                const int cs = ref & REF_MASK;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a.U[((size_t)cs * 4 + j) * a.P_pad + p] = Uc[j];
+                a.H[(size_t)cs * a.P_pad + p] = G;
+            }
+        }
+    }
+}
+
+// Fixed-order reductions: A_b over the waves (16 interleaved slices, then the
+// slices in order), and the compensated lnL sum (last workgroup).
+__global__ __launch_bounds__(256) void k_gtr_reduce(GtrReduceArgs a) {
+    const int tid = threadIdx.x;
+    __shared__ double part[256], comp[256];
+    if ((int)blockIdx.x < a.n_branch) {
+        const int q = tid & 15, sl = tid >> 4;
+        const double *src = a.Apart + (size_t)blockIdx.x * a.n_waves * 16;
+        double v = 0.0;
+        for (int w = sl; w < a.n_waves; w += 16) v += src[(size_t)w * 16 + q];
+        part[tid] = v;
+        __syncthreads();
+        if (tid < 16) {
+            double t = 0.0;
+            for (int k = 0; k < 16; ++k) t += part[k * 16 + tid];
+            a.A[(size_t)blockIdx.x * 16 + tid] = t;
+        }
+        return;
+    }
+    // lnL: per-thread Neumaier sums over contiguous pattern ranges, then in thread order
+    const int per = (a.P + 255) / 256;
+    double s = 0.0, c = 0.0;
+    for (int p = tid * per; p < min((tid + 1) * per, a.P); ++p) neum(s, c, a.weights[p] * a.site_ll[p]);
+    part[tid] = s;
+    comp[tid] = c;
+    __syncthreads();
+    if (tid == 0) {
+        double S0 = 0.0, C0 = 0.0;
+        for (int k = 0; k < 256; ++k) comp_add(S0, C0, part[k], comp[k]);
+        a.res[0] = S0 + C0;
+    }
+}
+
+template <typename R> void launch_gtr(const GtrArgs<R> &a, bool grad, hipStream_t s) {
+    const dim3 grid(a.P_pad / GTR_BLOCK);
+    hipLaunchKernelGGL(k_gtr_inside<R>, grid, dim3(GTR_BLOCK), 0, s, a);
+    if (grad) hipLaunchKernelGGL(k_gtr_outside<R>, grid, dim3(GTR_BLOCK), 0, s, a);
+}
+
+// workgroups 0 .. n_branch - 1 reduce A_b (with gradients), the last one lnL
+void launch_gtr_reduce(const GtrReduceArgs &a, bool grad, hipStream_t s) {
+    GtrReduceArgs r = a;
+    if (!grad) r.n_branch = 0;
+    hipLaunchKernelGGL(k_gtr_reduce, dim3(r.n_branch + 1), dim3(256), 0, s, r);
+}
+
+template void launch_gtr<float>(const GtrArgs<float> &, bool, hipStream_t);
+template void launch_gtr<double>(const GtrArgs<double> &, bool, hipStream_t);
+
+// ---------------------------------------------------------------------------
 // Fixed scratch of the allocation-free entry point (hy_loglik_from_P_noalloc):
 // one array of NOALLOC_SCRATCH_BYTES per device that loads this library, held
 // until the process ends.  Host code never passes the array's name to a

@@ -78,13 +78,18 @@ py::dict problem_arrays(const Tree &tree, const Alignment &aln) {
     d["states"] = states;
     d["masks"] = masks;
     d["weights"] = weights;
-    d["S"] = aln.code ? aln.code->num_sense_codons : 61;
+    // a nucleotide view (no codons) is a 4-state problem with the nucleotide frequencies
+    const bool nuc = aln.num_codons == 0;
+    d["S"] = nuc ? 4 : (aln.code ? aln.code->num_sense_codons : 61);
     d["code_id"] = aln.code ? aln.code->id : 0;
     py::array_t<double> pos({3, 4});
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 4; ++b) pos.mutable_at(a, b) = aln.pos_nuc_frequencies[a][b];
     d["pos_freqs"] = pos;
-    d["pi"] = py::array_t<double>(py::ssize_t(aln.codon_frequencies_f3x4.size()), aln.codon_frequencies_f3x4.data());
+    if (nuc)
+        d["pi"] = py::array_t<double>(4, aln.nuc_frequencies.data());
+    else
+        d["pi"] = py::array_t<double>(py::ssize_t(aln.codon_frequencies_f3x4.size()), aln.codon_frequencies_f3x4.data());
     return d;
 }
 
@@ -177,6 +182,8 @@ PYBIND11_MODULE(_hyphy3, m) {
         .def_readonly("taxon_names", &Alignment::taxon_names)
         .def_readonly("site_to_pattern", &Alignment::site_to_pattern)
         .def_readonly("nuc_frequencies", &Alignment::nuc_frequencies)
+        .def("to_nucleotide_alignment", &Alignment::to_nucleotide_alignment,
+             "Nucleotide view for the GTR stage (one site per nucleotide column)")
         .def_readonly("pos_nuc_frequencies", &Alignment::pos_nuc_frequencies)
         .def_readonly("codon_frequencies_f1x4", &Alignment::codon_frequencies_f1x4)
         .def_readonly("codon_frequencies_f3x4", &Alignment::codon_frequencies_f3x4)
@@ -326,6 +333,23 @@ PYBIND11_MODULE(_hyphy3, m) {
           py::arg("tree"), py::arg("aln"), py::arg("p_start"), py::arg("p_end"), py::arg("P_branches"), py::arg("pi"),
           "Workspace of compute_inside_outside_batch_in_place (arrays [node][state][pattern])");
 
+    m.def("compute_gtr_log_likelihood",
+          [](const Tree &tree, const Alignment &aln, const std::vector<double> &theta, const std::vector<double> &freqs) {
+              if (theta.size() != 5 || freqs.size() != 4)
+                  throw std::invalid_argument("theta: 5 exchangeabilities (AC, AT, CG, CT, GT); freqs: 4 frequencies");
+              GTRParameters p;
+              p.theta_AC = theta[0];
+              p.theta_AT = theta[1];
+              p.theta_CG = theta[2];
+              p.theta_CT = theta[3];
+              p.theta_GT = theta[4];
+              GTRMatrix g;
+              g.update(p, {freqs[0], freqs[1], freqs[2], freqs[3]});
+              const Alignment nuc = aln.num_codons == 0 ? aln : aln.to_nucleotide_alignment();
+              return LikelihoodEngine::compute_gtr_log_likelihood(tree, nuc, g);
+          },
+          py::arg("tree"), py::arg("aln"), py::arg("theta"), py::arg("freqs"),
+          "GTR nucleotide lnL (compute_gtr_log_likelihood) of the alignment's nucleotide view");
     m.def("problem_arrays", &problem_arrays, py::arg("tree"), py::arg("aln"),
           "Flat tree / pattern arrays in the layout of the C ABI's hy_set_problem and hy_set_model");
 }

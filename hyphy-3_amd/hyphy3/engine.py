@@ -34,7 +34,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
            "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count",
-           "hy_set_memory_budget", "hy_last_checkpoint")
+           "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
     L.hy_loglik_from_P_noalloc.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p]
     L.hy_graph_count.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.hy_set_memory_budget.argtypes = [C.c_void_p, C.c_size_t]
+    L.hy_gtr_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.hy_last_checkpoint.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.hy_set_sink.argtypes = [C.c_void_p, _f64p]
@@ -339,6 +340,28 @@ class Engine:
         self._check(self.L.hy_loglik_from_P(self.h, precision, _p(P, _f64p), _p(pi, _f64p), C.byref(ll),
                                             _p(s, _f64p)))
         return (ll.value, s[:self.num_patterns]) if site else ll.value
+
+    def gtr_eval(self, bl, theta, freqs, precision=HY_PREC_FP64, grad=True, site=False):
+        """Nucleotide GTR stage (hy_gtr_eval) on a problem set with S = 4:
+        lnL and, with grad, d lnL / d bl and d lnL / d theta."""
+        bl = _f64(bl)
+        th = _f64(theta)
+        fr = _f64(freqs)
+        if bl.size != self.num_nodes or th.size != 5 or fr.size != 4:
+            raise ValueError("gtr_eval: need num_nodes branch lengths, 5 exchangeabilities and 4 frequencies")
+        ll = C.c_double()
+        s = np.zeros(max(self.num_patterns, 1)) if site else None
+        gb = np.zeros(self.num_nodes) if grad else None
+        gt = np.zeros(5) if grad else None
+        flags = HY_WANT_LOGL | (HY_WANT_SITE_LL if site else 0) | (HY_WANT_GRAD if grad else 0)
+        self._check(self.L.hy_gtr_eval(self.h, precision, flags, _p(bl, _f64p), _p(th, _f64p), _p(fr, _f64p),
+                                       C.byref(ll), _p(s, _f64p), _p(gb, _f64p), _p(gt, _f64p)))
+        out = {"log_likelihood": ll.value}
+        if site:
+            out["site_ll"] = s[:self.num_patterns]
+        if grad:
+            out.update(grad_bl=gb, grad_theta=gt)
+        return out
 
     def inside_outside_from_P(self, P, pi, precision=HY_PREC_FP64):
         """hy_inside_outside_from_P: lnL from caller-supplied matrices, with

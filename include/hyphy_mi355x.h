@@ -30,6 +30,8 @@
  *                    synonymous-rate weights (busted.hpp:662-1135)
  *   hy_eval_batch    a batch of independent problems (genes), one per handle
  *   hy_loglik_from_P compute_tree_log_likelihood_from_P (likelihood.hpp:766-806)
+ *   hy_gtr_eval      compute_gtr_log_likelihood (likelihood.hpp:181-254), the
+ *                    nucleotide stage of GTRFitter (optimizer.hpp:33-655)
  */
 #ifndef HYPHY_MI355X_H
 #define HYPHY_MI355X_H
@@ -212,6 +214,22 @@ int hy_eval_batch(int n_items, hy_batch_item *items, int precision, unsigned fla
  * No counterpart in the reference. */
 int hy_autotune_batch(int n_items, hy_batch_item *items, int precision, unsigned flags, int reps, int *out_tile,
                       double *out_ms);
+
+/* Nucleotide GTR stage: compute_gtr_log_likelihood (likelihood.hpp:181-254)
+ * on a problem set with S = 4, i.e. the nucleotide view of the alignment
+ * (Alignment::to_nucleotide_alignment, alignment.hpp:832-890: states 0..3 =
+ * A, C, G, T or -1, masks = 4-bit IUPAC masks).  The generator is
+ * GTRMatrix::update's (rate_matrix.hpp:29-99): exchangeabilities theta[5]
+ * (AC, AT, CG, CT, GT; AG = 1), frequencies freqs[4] (also the root
+ * distribution: the alignment's nucleotide frequencies), normalised to one
+ * expected substitution per unit of bl[num_nodes] (root entry ignored).
+ * flags as hy_eval: out_logL (Neumaier sum over patterns), out_site_ll
+ * [num_patterns] with HY_WANT_SITE_LL, and with HY_WANT_GRAD
+ * out_grad_bl[num_nodes] and out_grad_theta[5] (d lnL / d theta of the
+ * normalised model; the reference's GTRFitter is derivative-free,
+ * optimizer.hpp:33-655, so these have no counterpart there). */
+int hy_gtr_eval(hy_engine h, int precision, unsigned flags, const double *bl, const double *theta, const double *freqs,
+                double *out_logL, double *out_site_ll, double *out_grad_bl, double *out_grad_theta);
 
 /* lnL from caller-supplied transition matrices P[num_nodes*S*S] (row-major,
  * entry of the root ignored) and root frequencies pi[S]. */

@@ -199,6 +199,43 @@ class AlignmentRef:
         self.weights = np.array(self.weights, dtype=np.float64)
 
 
+class NucCode:
+    """State space of the nucleotide view: A, C, G, T."""
+    S = 4
+
+
+class NucleotideRef:
+    """Alignment::to_nucleotide_alignment (alignment.hpp:832-890): one site per
+    nucleotide column; state = char_to_nuc (A/C/G/T/U -> 0..3, else -1), mask
+    = IUPAC 4-bit mask (unknown characters: 0b1111); patterns keyed on the
+    column's masks, first-occurrence order; weight = count; the codon
+    alignment's nucleotide frequencies (alignment.hpp:845)."""
+
+    def __init__(self, aln: "AlignmentRef"):
+        self.code = NucCode()
+        self.names = list(aln.names)
+        self.num_taxa = aln.num_taxa
+        self.num_sites = len(aln.seqs[0]) if aln.seqs else 0
+        self.nuc_freqs = np.array(aln.nuc_freqs, dtype=float)
+        seen = {}
+        states, masks, weights, s2p = [], [], [], []
+        for site in range(self.num_sites):
+            col = tuple(iupac_mask(s[site]) for s in aln.seqs)
+            if col not in seen:
+                seen[col] = len(weights)
+                states.append([char_to_nuc(s[site]) for s in aln.seqs])
+                masks.append(list(col))
+                weights.append(0)
+            k = seen[col]
+            weights[k] += 1
+            s2p.append(k)
+        self.num_patterns = len(weights)
+        self.states = np.array(states, dtype=np.int8).reshape(self.num_patterns, self.num_taxa)
+        self.pmasks = np.array(masks, dtype=np.uint64).reshape(self.num_patterns, self.num_taxa)
+        self.weights = np.array(weights, dtype=np.float64)
+        self.site_to_pattern = np.array(s2p, dtype=np.int64)
+
+
 # --------------------------------------------------------------------------
 # Trees
 # --------------------------------------------------------------------------

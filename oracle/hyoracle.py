@@ -489,3 +489,81 @@ def busted_expected_weights(prob, code, base, pos, pi, weights_p, omegas, weight
         return new_w, new_v
     new_eps = float(np.clip(sink_acc / max(1.0, tot), 1e-7, max_eps_sink)) if tot > 0.0 else eps_sink
     return new_w, new_v, new_eps
+
+
+# ---------------------------------------------------------------------------
+# Nucleotide GTR stage (S = 4)
+# ---------------------------------------------------------------------------
+GTR_PAIRS = ((0, 1), (0, 3), (1, 2), (1, 3), (2, 3))  # theta order AC, AT, CG, CT, GT; AG = 1
+
+
+def gtr_Q(theta, pi):
+    """GTRMatrix::update (rate_matrix.hpp:29-90): Q_ij = r_ij pi_j with r_AG = 1,
+    diagonal minus the row sum, divided by mu = -sum_i pi_i Q_ii when mu > 0.
+    Returns (normalised Q, mu)."""
+    r = np.ones((4, 4))
+    for k, (a, b) in enumerate(GTR_PAIRS):
+        r[a, b] = r[b, a] = theta[k]
+    np.fill_diagonal(r, 0.0)
+    pi = np.asarray(pi, dtype=float)
+    Q = r * pi[None, :]
+    Q -= np.diag(Q.sum(axis=1))
+    mu = -float(np.sum(pi * np.diag(Q)))
+    if mu > 0.0:
+        Q = Q / mu
+    return Q, mu
+
+
+def gtr_eval(prob, theta, pi, bl, want_grad=True, threads=1):
+    """compute_gtr_log_likelihood (likelihood.hpp:181-254): P_b = U diag(e^{l t_b})
+    V of the normalised generator (transition_matrix, rate_matrix.hpp:93-99,
+    clamped >= 0), pruning with the alignment's nucleotide frequencies at the
+    root, Neumaier sum of w_p lnL_p.  With gradients (no counterpart in the
+    reference, whose GTRFitter is derivative-free; the same adjoint as the
+    codon joint gradient, likelihood.hpp:1435-1520): from the reference's
+    inside-outside workspace, A_b = sum_p w_p / L_p V_b D_b^T over patterns with
+    L_p > 0; d lnL / d t_b = <A_b, Q P_b>; d lnL / d theta_k = sum_b <U^T A_b V^T,
+    (V dQ_k U) o E(t_b)> with E_ij = (e^{l_i t} - e^{l_j t}) / (l_i - l_j) (t e^{l_i t}
+    when |l_i - l_j| < 1e-9) and dQ_k = (dQ/dtheta_k - Q dmu/dtheta_k) / mu the
+    derivative of the normalised generator."""
+    pi = np.asarray(pi, dtype=float)
+    Q, mu = gtr_Q(theta, pi)
+    ev, U, V = decompose(Q, pi)
+    N = prob.num_nodes
+    bl = np.asarray(bl, dtype=float)
+    P = np.stack([np.eye(4) if n == prob.root else transition(ev, U, V, bl[n]) for n in range(N)])
+    ll, site = loglik_from_P(prob, P, pi, threads=threads, site=True)
+    out = {"log_likelihood": ll, "site_ll": site}
+    if not want_grad:
+        return out
+    ws = workspace(prob, P, pi, 0, prob.num_patterns, outside=True)
+    ok = ws["L"] > 0.0
+    c = np.where(ok, prob.weights / np.where(ok, ws["L"], 1.0), 0.0)
+    A = np.einsum("nip,njp,p->nij", ws["V"], ws["D"], c)
+    gbl = np.zeros(N)
+    gth = np.zeros(5)
+    for k, (a, b) in enumerate(GTR_PAIRS):
+        dQ = np.zeros((4, 4))
+        dQ[a, b], dQ[b, a] = pi[b], pi[a]
+        dQ[a, a], dQ[b, b] = -pi[b], -pi[a]
+        dmu = 2.0 * pi[a] * pi[b]
+        M = V @ ((dQ - Q * dmu) / mu) @ U
+        for n in range(N):
+            if n == prob.root:
+                continue
+            t = bl[n]
+            e = np.exp(ev * t)
+            d = ev[:, None] - ev[None, :]
+            close = np.abs(d) < 1e-9
+            E = np.where(close, t * e[:, None] * np.ones(4)[None, :], (e[:, None] - e[None, :]) / np.where(close, 1.0, d))
+            gth[k] += np.sum((U.T @ A[n] @ V.T) * M * E)
+    for n in range(N):
+        if n != prob.root:
+            gbl[n] = np.sum(A[n] * (Q @ P[n]))
+    out.update(grad_bl=gbl, grad_theta=gth, A=A)
+    return out
+
+
+def nuc_problem(tree, naln):
+    """Problem of the oracle over a nucleotide view (data_ref.NucleotideRef)."""
+    return Problem.from_refs(tree, naln)
