@@ -143,6 +143,11 @@ struct hy_engine_s {
     DevBuf d_gsched, d_gnref, d_gnbr, d_gtips, d_gw, d_gP, d_gpi, d_gD, d_gE, d_gU, d_gH, d_gL, d_gLE, d_gsite,
         d_gApart, d_gA, d_gres;
     std::vector<uint8_t> g_ident;  // extended nodes whose edge is an identity
+    // per-site refits (hy_fel_eval): per-pattern warm-start eigenvectors and
+    // whether a pattern's slot holds a converged solve; candidate buffers
+    DevBuf d_fZ, d_fslot, d_fwarm, d_fpat, d_falpha, d_fparams, d_flam, d_fU, d_fV, d_fmu, d_fstatus, d_fbl, d_fD,
+        d_fE, d_fout, d_fident;
+    std::vector<uint8_t> f_valid;
     std::vector<uint16_t> na_tips;
     std::vector<double> na_weights;
     std::vector<int32_t> na_sched, na_ident;
@@ -1370,7 +1375,10 @@ void hy_destroy(hy_engine h) {
                       &h->d_site, &h->d_Apart, &h->d_fr, &h->d_wl, &h->d_proj, &h->d_wover,
                       &h->d_sink, &h->d_sinkwl, &h->d_blist_ck, &h->d_bsel, &h->d_gsched, &h->d_gnref,
                       &h->d_gnbr, &h->d_gtips, &h->d_gw, &h->d_gP, &h->d_gpi, &h->d_gD, &h->d_gE, &h->d_gU,
-                      &h->d_gH, &h->d_gL, &h->d_gLE, &h->d_gsite, &h->d_gApart, &h->d_gA, &h->d_gres};
+                      &h->d_gH, &h->d_gL, &h->d_gLE, &h->d_gsite, &h->d_gApart, &h->d_gA, &h->d_gres,
+                      &h->d_fZ, &h->d_fslot, &h->d_fwarm, &h->d_fpat, &h->d_falpha, &h->d_fparams, &h->d_flam,
+                      &h->d_fU, &h->d_fV, &h->d_fmu, &h->d_fstatus, &h->d_fbl, &h->d_fD, &h->d_fE, &h->d_fout,
+                      &h->d_fident};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -1684,6 +1692,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
             if (h->ext_ident[n]) h->na_ident.push_back(n);
     }
     h->gtr_ready = false;
+    h->f_valid.clear();
     h->have_problem = true;
     return HY_OK;
 }
@@ -3322,6 +3331,147 @@ int hy_gtr_eval(hy_engine h, int precision, unsigned flags, const double *bl, co
     }
     if (out_grad_bl) std::copy(gbl.begin(), gbl.end(), out_grad_bl);
     if (out_grad_theta) std::copy(gth, gth + 5, out_grad_theta);
+    return HY_OK;
+}
+
+int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *beta,
+                const double *theta, double delta, double psi, const double *bl, double *out_ll, double *out_scale) {
+    // FELAnalyzer::analyze_pattern's eval_pattern_lnl (fel.hpp:367-385):
+    // omega = beta / max(alpha, 1e-12), MG94 with alpha = 1 and that beta,
+    // P_b = transition_matrix(max(alpha, 1e-12) t_b), the pattern's lnL
+    // (compute_pattern_log_likelihood, fel.hpp:276-308)
+    const char *who = "hy_fel_eval";
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "%s before hy_set_problem/hy_set_model", who);
+    if (n_cand < 0 || (n_cand > 0 && (!pattern || !alpha || !beta || !out_ll)) || !theta || !bl)
+        return fail(h, HY_ERR_ARG, "%s: bad candidate arrays", who);
+    if (n_cand == 0) return HY_OK;
+    if (h->f_valid.size() != (size_t)h->P) h->f_valid.assign(h->P, 0);
+    std::vector<uint8_t> seen(h->P, 0), warm(n_cand);
+    std::vector<double> om(n_cand);
+    for (int k = 0; k < n_cand; ++k) {
+        const int p = pattern[k];
+        if (p < 0 || p >= h->P) return fail(h, HY_ERR_ARG, "%s: pattern %d out of range", who, p);
+        if (seen[p]) return fail(h, HY_ERR_ARG, "%s: pattern %d appears twice in one call", who, p);
+        seen[p] = 1;
+        if (!(alpha[k] >= 0.0 && beta[k] >= 0.0 && std::isfinite(alpha[k]) && std::isfinite(beta[k])))
+            return fail(h, HY_ERR_ARG, "%s: candidate %d has alpha %g, beta %g", who, k, alpha[k], beta[k]);
+        om[k] = beta[k] / std::max(alpha[k], 1e-12);
+        warm[k] = h->f_valid[p];
+    }
+    for (int n = 0; n < h->num_nodes; ++n)
+        if (n != h->root && !(bl[n] >= 0.0 && std::isfinite(bl[n])))
+            return fail(h, HY_ERR_ARG, "%s: branch length %d = %g", who, n, bl[n]);
+    for (int i = 0; i < 5; ++i)
+        if (!(theta[i] >= 0.0 && std::isfinite(theta[i])))
+            return fail(h, HY_ERR_ARG, "%s: theta[%d] = %g", who, i, theta[i]);
+    if (!(delta >= 0.0 && psi >= 0.0 && std::isfinite(delta) && std::isfinite(psi)))
+        return fail(h, HY_ERR_ARG, "%s: delta %g, psi %g", who, delta, psi);
+    HIP_TRY(h, hipSetDevice(h->device));
+    TRY(gtr_prepare(h));  // pattern-walk schedule, node tables, padded tip codes
+    const size_t K = (size_t)n_cand;
+    if (!h->d_fZ.p || h->d_fZ.bytes < (size_t)h->P * 4096 * 8) {
+        TRY(ensure(h, h->d_fZ, (size_t)h->P * 4096 * 8));
+        std::fill(h->f_valid.begin(), h->f_valid.end(), 0);
+        std::fill(warm.begin(), warm.end(), 0);
+    }
+    // parameter block of k_model: bl, alpha = 1, one omega per candidate, theta, delta = psi = 0
+    ParamLayout pl{};
+    pl.n_nodes = h->n_ext;
+    pl.n_mat = n_cand;
+    pl.n_comp = 1;
+    pl.off_bl = 0;
+    pl.off_alpha = h->n_ext;
+    pl.off_omega = pl.off_alpha + 1;
+    pl.off_w = pl.off_omega + n_cand;
+    pl.off_theta = pl.off_w + 1;
+    pl.off_delta = pl.off_theta + 5;
+    pl.off_psi = pl.off_delta + 1;
+    pl.total = pl.off_psi + 1;
+    std::vector<double> prm(pl.total, 0.0), blx(h->n_ext, 0.0);
+    for (int n = 0; n < h->num_nodes; ++n) blx[n] = (n == h->root) ? 0.0 : bl[n];
+    std::copy(blx.begin(), blx.end(), prm.begin());
+    prm[pl.off_alpha] = 1.0;
+    std::copy(om.begin(), om.end(), prm.begin() + pl.off_omega);
+    prm[pl.off_w] = 1.0;
+    std::copy(theta, theta + 5, prm.begin() + pl.off_theta);
+    prm[pl.off_delta] = delta;
+    prm[pl.off_psi] = psi;
+    std::vector<int32_t> slots(pattern, pattern + n_cand);
+    std::vector<uint8_t> ident(h->g_ident.begin(), h->g_ident.end());
+    TRY(upload(h, h->d_fparams, prm.data(), prm.size() * 8));
+    TRY(upload(h, h->d_fslot, slots.data(), K * 4));
+    TRY(upload(h, h->d_fwarm, warm.data(), K));
+    TRY(upload(h, h->d_fpat, slots.data(), K * 4));
+    TRY(upload(h, h->d_falpha, alpha, K * 8));
+    TRY(upload(h, h->d_fbl, blx.data(), blx.size() * 8));
+    TRY(upload(h, h->d_fident, ident.data(), ident.size()));
+    TRY(ensure(h, h->d_flam, K * 64 * 8));
+    TRY(ensure(h, h->d_fU, K * 4096 * 8));
+    TRY(ensure(h, h->d_fV, K * 4096 * 8));
+    TRY(ensure(h, h->d_fmu, K * 8));
+    TRY(ensure(h, h->d_fstatus, K * 4));
+    TRY(ensure(h, h->d_fD, K * std::max(h->n_int, 1) * 64 * 8));
+    TRY(ensure(h, h->d_fE, K * std::max(h->n_int, 1) * 4));
+    TRY(ensure(h, h->d_fout, K * 8));
+    hipStream_t s = h->stream;
+    ModelArgs ma{};
+    ma.qdesc = (const int32_t *)h->d_qdesc.p;
+    ma.nd = (const int8_t *)h->d_nd.p;
+    ma.dpos = (const int8_t *)h->d_dpos.p;
+    ma.dfrom = (const int8_t *)h->d_dfrom.p;
+    ma.dto = (const int8_t *)h->d_dto.p;
+    ma.syn = (const uint8_t *)h->d_syn.p;
+    ma.pos = (const double *)h->d_pos.p;
+    ma.pi = (const double *)h->d_pi.p;
+    ma.S = h->S;
+    ma.params = (const double *)h->d_fparams.p;
+    ma.pl = pl;
+    ma.lam = (double *)h->d_flam.p;
+    ma.U = (double *)h->d_fU.p;
+    ma.V = (double *)h->d_fV.p;
+    ma.mu = (double *)h->d_fmu.p;
+    ma.Z = (double *)h->d_fZ.p;
+    ma.warm = 1;
+    ma.zslot = (const int32_t *)h->d_fslot.p;
+    ma.zwarm = (const uint8_t *)h->d_fwarm.p;
+    ma.status = (int *)h->d_fstatus.p;
+    ma.tol = 1e-14;
+    launch_model(ma, n_cand, s);
+    FelArgs fa{};
+    fa.sched = (const int32_t *)h->d_gsched.p;
+    fa.n_sched = h->g_n_sched;
+    fa.nref = (const int32_t *)h->d_gnref.p;
+    fa.ident = (const uint8_t *)h->d_fident.p;
+    fa.tips = (const uint16_t *)h->d_gtips.p;
+    fa.tip_stride = h->g_P_pad;
+    fa.amb = (const uint64_t *)h->d_amb.p;
+    fa.pat = (const int32_t *)h->d_fpat.p;
+    fa.alpha = (const double *)h->d_falpha.p;
+    fa.lam = ma.lam;
+    fa.U = ma.U;
+    fa.V = ma.V;
+    fa.bl = (const double *)h->d_fbl.p;
+    fa.pi = ma.pi;
+    fa.S = h->S;
+    fa.n_int = std::max(h->n_int, 1);
+    fa.D = (double *)h->d_fD.p;
+    fa.E = (int32_t *)h->d_fE.p;
+    fa.out = (double *)h->d_fout.p;
+    launch_fel(fa, n_cand, s);
+    HIP_TRY(h, hipGetLastError());
+    std::vector<int32_t> status(K);
+    HIP_TRY(h, hipMemcpyAsync(out_ll, h->d_fout.p, K * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipMemcpyAsync(status.data(), h->d_fstatus.p, K * 4, hipMemcpyDeviceToHost, s));
+    if (out_scale) HIP_TRY(h, hipMemcpyAsync(out_scale, h->d_fmu.p, K * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipStreamSynchronize(s));
+    for (int k = 0; k < n_cand; ++k) {
+        if (status[k] < 0) {
+            h->f_valid[pattern[k]] = 0;
+            return fail(h, HY_ERR_NUMERIC, "%s: eigensolver did not converge for omega %g", who, om[k]);
+        }
+        h->f_valid[pattern[k]] = 1;
+    }
     return HY_OK;
 }
 

@@ -134,7 +134,36 @@ struct ModelArgs {
     const double *params_host;
     double *params_dev;
     int n_params;
+    // Warm-start state per slot (per-pattern refits): matrix m refines and
+    // stores the eigenvectors of slot zslot[m] (null: slot m), and only when
+    // zwarm[m] is set (null: every matrix, as warm says)
+    const int32_t *zslot;
+    const uint8_t *zwarm;
 };
+
+// Per-site refits (FEL): candidate k is pattern pat[k] at (alpha[k], beta[k]);
+// its spectrum (MG94 with omega = beta / max(alpha, 1e-12)) is matrix k of
+// the spectra buffers, branch times alpha * t_b.  One workgroup per candidate
+// walks the pattern-walk schedule (GTR_STRIDE entries).
+struct FelArgs {
+    const int32_t *sched;
+    int n_sched;
+    const int32_t *nref;     // [n_ext] child references (kind << 28 | tip row or D slot)
+    const uint8_t *ident;    // [n_ext] identity edge
+    const uint16_t *tips;    // [taxa + 1][tip_stride] codon codes
+    int tip_stride;
+    const uint64_t *amb;
+    const int32_t *pat;      // [n_cand]
+    const double *alpha;     // [n_cand]
+    const double *lam, *U, *V;  // spectra of the candidates ([k][64], [k][64][64])
+    const double *bl;        // [n_ext] branch lengths (model time at alpha = 1)
+    const double *pi;        // [64]
+    int S, n_int;
+    double *D;               // [n_cand][n_int][64] partials
+    int32_t *E;              // [n_cand][n_int]
+    double *out;             // [n_cand] pattern lnL
+};
+void launch_fel(const FelArgs &a, int n_cand, hipStream_t s);
 
 template <typename R>
 struct PmatArgs {

@@ -282,8 +282,8 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
 
     int status = 0;
     bool done = false;
-    double *Zg = a.Z + (size_t)m * 4096;
-    if (a.warm) {
+    double *Zg = a.Z + (size_t)(a.zslot ? a.zslot[m] : m) * 4096;
+    if (a.warm && (!a.zwarm || a.zwarm[m])) {
         // ---- Ogita-Aishima refinement of the previous eigenvectors ----
         for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = Zg[idx];
         __syncthreads();
@@ -2380,6 +2380,120 @@ void launch_gtr_reduce(const GtrReduceArgs &a, bool grad, hipStream_t s) {
 
 template void launch_gtr<float>(const GtrArgs<float> &, bool, hipStream_t);
 template void launch_gtr<double>(const GtrArgs<double> &, bool, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// k_fel: per-site refit candidates (FELAnalyzer::compute_pattern_log_likelihood,
+// fel.hpp:276-308, at eval_pattern_lnl's alpha and omega, fel.hpp:367-385).
+// One workgroup per candidate walks its pattern through the tree: per child
+// the clamped P = max(U diag(e^{l a t}) V, 0) is built in LDS on FP64 MFMA
+// and applied to the child's partial; partials are normalised to max in
+// [0.5, 1) by an exact power of two.  FP64 throughout.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fel(FelArgs a) {
+    const int k = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    __shared__ double Us[64 * LDA], Vs[64 * LDA], Pt[64 * LDA];
+    __shared__ double ex[64], dv[64], acc[64], red[4];
+    const int p = a.pat[k];
+    const double at = fmax(a.alpha[k], 1e-12);
+    const double *U = a.U + (size_t)k * 4096, *V = a.V + (size_t)k * 4096, *lam = a.lam + (size_t)k * 64;
+    for (int idx = tid; idx < 4096; idx += 256) {
+        Us[(idx >> 6) * LDA + (idx & 63)] = U[idx];
+        Vs[(idx >> 6) * LDA + (idx & 63)] = V[idx];
+    }
+    const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
+    double *Dk = a.D + (size_t)k * a.n_int * 64;
+    int32_t *Ek = a.E + (size_t)k * a.n_int;
+    int eacc = 0;
+    for (int q = 0; q < a.n_sched; ++q) {
+        const int32_t *e = a.sched + (size_t)q * GTR_STRIDE;
+        const int slot = e[1], nc = e[2];
+        if (tid < 64) acc[tid] = 1.0;
+        eacc = 0;
+        for (int c = 0; c < nc; ++c) {
+            const int cn = e[3 + c], ref = a.nref[cn], kind = ref >> 28, idx = ref & REF_MASK;
+            __syncthreads();  // previous child's reads of dv / Pt are done
+            // the child's partial
+            if (tid < 64) {
+                double d;
+                if (kind == KIND_TIP) {
+                    const unsigned code = a.tips[(size_t)idx * a.tip_stride + p];
+                    if (code == TIP_ONES)
+                        d = tid < a.S ? 1.0 : 0.0;
+                    else if (code & TIP_AMB)
+                        d = ((a.amb[code & 0x7FFFu] & all) >> tid) & 1ull ? 1.0 : 0.0;
+                    else
+                        d = (unsigned)tid == code ? 1.0 : 0.0;
+                } else {
+                    d = a.D[((size_t)k * a.n_int + idx) * 64 + tid];
+                }
+                dv[tid] = d;
+                ex[tid] = exp(lam[tid] * at * a.bl[cn]);
+            }
+            if (kind != KIND_TIP && tid == 0) eacc += Ek[idx];
+            __syncthreads();
+            if (a.ident[cn]) {
+                if (tid < 64) acc[tid] *= dv[tid];
+                continue;
+            }
+            // P = max(U diag(e) V, 0) in Pt (rows of wave w: 16w .. 16w + 15)
+            f64x4 pacc[4];
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) pacc[qq] = splat<f64x4>(0.0);
+#pragma unroll 4
+            for (int kk = 0; kk < 16; ++kk) {
+                const int i = 16 * w + lr, kx = 4 * kk + lg;
+                const double av = Us[i * LDA + kx] * ex[kx];
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) pacc[qq] = MF<double>::mma(av, Vs[kx * LDA + 16 * qq + lr], pacc[qq]);
+            }
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double v = pacc[qq][r];
+                    Pt[(16 * w + MF<double>::crow(lg, r)) * LDA + 16 * qq + lr] = v > 0.0 ? v : 0.0;
+                }
+            __syncthreads();
+            // x = P d (four threads per row), acc *= x
+            {
+                const int row = tid >> 2, part = tid & 3;
+                double sacc = 0.0;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) sacc += Pt[row * LDA + 16 * part + j] * dv[16 * part + j];
+                sacc += __shfl_xor(sacc, 1);
+                sacc += __shfl_xor(sacc, 2);
+                if (part == 0) acc[row] *= sacc;
+            }
+        }
+        __syncthreads();
+        // normalise the node's partial by a power of two (wave 0)
+        if (tid < 64) {
+            double m = acc[tid];
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+            double v = acc[tid];
+            int ke = 0;
+            if (m > 0.0) {
+                ke = frexp_e(m);
+                v = ldexp(v, -ke);
+            }
+            acc[tid] = v;
+            Dk[(size_t)slot * 64 + tid] = v;
+            if (tid == 0) Ek[slot] = eacc + ke;
+            if (tid == 0) red[0] = (double)(eacc + ke);
+        }
+        __syncthreads();
+    }
+    // root (last entry): L = pi . D_root
+    if (tid < 64) {
+        double l = a.pi[tid] * acc[tid];
+        for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off);
+        if (tid == 0) a.out[k] = l > 0.0 ? log(l) + red[0] * 0.69314718055994530942 : -1e20;
+    }
+}
+
+void launch_fel(const FelArgs &a, int n_cand, hipStream_t s) {
+    if (n_cand > 0) hipLaunchKernelGGL(k_fel, dim3(n_cand), dim3(256), 0, s, a);
+}
 
 // ---------------------------------------------------------------------------
 // Fixed scratch of the allocation-free entry point (hy_loglik_from_P_noalloc):

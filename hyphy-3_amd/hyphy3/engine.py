@@ -34,7 +34,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
            "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count",
-           "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval")
+           "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval")
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -83,6 +83,8 @@ def load_library(path: str = LIB_PATH):
     L.hy_graph_count.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.hy_set_memory_budget.argtypes = [C.c_void_p, C.c_size_t]
     L.hy_gtr_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.hy_fel_eval.argtypes = [C.c_void_p, C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double, _f64p, _f64p,
+                              _f64p]
     L.hy_last_checkpoint.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.hy_set_sink.argtypes = [C.c_void_p, _f64p]
@@ -362,6 +364,25 @@ class Engine:
         if grad:
             out.update(grad_bl=gb, grad_theta=gt)
         return out
+
+    def fel_eval(self, pattern, alpha, beta, theta, bl, delta=0.0, psi=0.0, scale=False):
+        """FEL candidate log-likelihoods (hy_fel_eval): pattern[k] at
+        (alpha[k], beta[k]), bl the converted branch lengths.  Returns the
+        array of lnL, and with scale also each candidate generator's mean
+        rate."""
+        pat = np.ascontiguousarray(pattern, dtype=np.int32).ravel()
+        a = _f64(alpha).ravel()
+        b = _f64(beta).ravel()
+        th = _f64(theta)
+        blc = _f64(bl)
+        n = pat.size
+        if a.size != n or b.size != n or th.size != 5 or blc.size != self.num_nodes:
+            raise ValueError("fel_eval: need one alpha and beta per pattern, 5 exchangeabilities and num_nodes lengths")
+        out = np.zeros(max(n, 1))
+        sc = np.zeros(max(n, 1)) if scale else None
+        self._check(self.L.hy_fel_eval(self.h, n, _p(pat, _i32p), _p(a, _f64p), _p(b, _f64p), _p(th, _f64p),
+                                       float(delta), float(psi), _p(blc, _f64p), _p(out, _f64p), _p(sc, _f64p)))
+        return (out[:n], sc[:n]) if scale else out[:n]
 
     def inside_outside_from_P(self, P, pi, precision=HY_PREC_FP64):
         """hy_inside_outside_from_P: lnL from caller-supplied matrices, with

@@ -32,6 +32,8 @@
  *   hy_loglik_from_P compute_tree_log_likelihood_from_P (likelihood.hpp:766-806)
  *   hy_gtr_eval      compute_gtr_log_likelihood (likelihood.hpp:181-254), the
  *                    nucleotide stage of GTRFitter (optimizer.hpp:33-655)
+ *   hy_fel_eval      FEL's per-site candidate likelihoods, eval_pattern_lnl
+ *                    (fel.hpp:367-385) over many (pattern, alpha, beta)
  */
 #ifndef HYPHY_MI355X_H
 #define HYPHY_MI355X_H
@@ -230,6 +232,26 @@ int hy_autotune_batch(int n_items, hy_batch_item *items, int precision, unsigned
  * optimizer.hpp:33-655, so these have no counterpart there). */
 int hy_gtr_eval(hy_engine h, int precision, unsigned flags, const double *bl, const double *theta, const double *freqs,
                 double *out_logL, double *out_site_ll, double *out_grad_bl, double *out_grad_theta);
+
+/* FEL per-site refit candidates, FP64: candidate k is pattern pattern[k] at
+ * synonymous rate alpha[k] and non-synonymous rate beta[k], evaluated as
+ * FELAnalyzer::analyze_pattern's eval_pattern_lnl (fel.hpp:367-385): MG94
+ * with alpha = 1, beta = omega = beta[k] / max(alpha[k], 1e-12), the
+ * exchangeabilities theta[5] (AC, AT, CG, CT, GT) and delta, psi of the base
+ * model (rate_matrix.hpp:170-247, unnormalised), P_b = max(U diag(e^{l a t_b})
+ * V, 0) with a = max(alpha[k], 1e-12) and t_b = bl[b] (the tree's lengths
+ * after the analyzer's conversion, fel.hpp:240-271), then the pattern's
+ * log-likelihood (compute_pattern_log_likelihood, fel.hpp:276-308; -1e20
+ * where the likelihood is not positive) in out_ll[k].  out_scale (may be
+ * NULL) receives each candidate's generator mean rate sum_i pi_i (-Q_ii)
+ * (MG94Matrix::scale_factor).  One workgroup per candidate; the spectra of
+ * all candidates are computed in one launch, refining the eigenvectors kept
+ * from the same pattern's previous candidate.  A pattern may appear at most
+ * once per call (its warm-start slot); out-of-range patterns, negative or
+ * non-finite rates are refused with HY_ERR_ARG, an eigensolver failure with
+ * HY_ERR_NUMERIC.  Requires hy_set_problem and hy_set_model. */
+int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *beta,
+                const double *theta, double delta, double psi, const double *bl, double *out_ll, double *out_scale);
 
 /* lnL from caller-supplied transition matrices P[num_nodes*S*S] (row-major,
  * entry of the root ignored) and root frequencies pi[S]. */
