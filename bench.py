@@ -193,6 +193,9 @@ def main():
     ap.add_argument("--gtr", action="store_true",
                     help="instead of the codon run: the nucleotide GTR stage (lnL + branch and exchangeability "
                          "gradients) on the workload's nucleotide view, against the oracle on the host")
+    ap.add_argument("--fel", action="store_true",
+                    help="instead of the codon run: FEL per-site refits (grid, Nelder-Mead, Brent per pattern)")
+    ap.add_argument("--fel-data", default="cd2", choices=["cd2", "sim"])
     ap.add_argument("--memory-budget", type=float, default=0.0,
                     help="GB allowed for the partials; below what every node's partials need the engine "
                          "checkpoints inside partials (about sqrt(internal nodes) parts recomputed)")
@@ -218,6 +221,9 @@ def main():
         return
     if args.gtr:
         bench_gtr(args, cfg)
+        return
+    if args.fel:
+        bench_fel(args)
         return
     sim, aln, tree, arrays, tau = build_problem(cfg)
     print(f"[bench] {args.config}: taxa={cfg['taxa']} codons={cfg['codons']} patterns={len(arrays['weights'])} "
@@ -495,6 +501,77 @@ def bench_gtr(args, cfg):
                                "sample": f"full problem, {n} lnL + gradient evaluations in {el:.1f} s (oracle: "
                                          "OpenMP pruning and inside-outside workspace, numpy gradient assembly)"}
         res["rel_err_logl_fp64_vs_oracle"] = abs(res["logL"] - ref["log_likelihood"]) / abs(ref["log_likelihood"])
+    print(json.dumps(res), flush=True)
+
+
+def bench_fel(args):
+    """FEL per-site refits (FELAnalyzer's all-branches-tested path,
+    fel.hpp:326-445,778-834) on CD2 with the fitted tree and global MG94 rates
+    of the reference's matched-tree parity test (tests/golden), or on a
+    simulated alignment (``--fel-data sim``).  One step = the whole refit
+    stage over every pattern; the oracle's sequential FEL (one host core) on a
+    bounded sample of patterns is the CPU column, in candidates per second."""
+    import hyphy3
+    from hyphy3 import fel as pf
+    from hyphy3 import simulate
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+    truth = json.load(open(os.path.join(golden, "cd2_fel_mle.json")))
+    if args.fel_data == "cd2":
+        g = truth["global_mg94"]
+        fasta = open(os.path.join(golden, "cd2.fna")).read()
+        newick = open(os.path.join(golden, "cd2_fitted.nwk")).read()
+        theta, alpha, beta = g["theta"], g["alpha"], g["beta"]
+        label = "CD2 (10 taxa x 187 codons), fitted tree and global MG94 rates of the reference's FEL parity test"
+    else:
+        sim = simulate.simulate(32, 1000, 11, omegas=[0.1, 1.0, 4.0], weights=[0.6, 0.3, 0.1])
+        fasta, newick = sim["fasta"], sim["newick"]
+        theta, alpha, beta = list(sim["theta"]), 1.0, 0.5
+        label = "simulated 32 taxa x 1000 codons (omega mixture 0.1/1/4), generating tree"
+    aln = hyphy3.Alignment.from_text(fasta)
+    tree = hyphy3.Tree.from_newick(newick, keep_rooted=True)
+    fel = pf.FEL(tree, aln, theta, alpha, beta)
+    n_pat = int(fel.states.shape[0])
+    res = {"metric": "FEL per-site refit stage", "unit": "ms", "workload": label, "patterns": n_pat,
+           "steps": args.steps, "warmup": args.warmup}
+    for _ in range(args.warmup):
+        fel.run()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = fel.run()
+        times.append(time.perf_counter() - t0)
+    st = fel.stats
+    fel.engine.fel_counters(reset=True)
+    fel.run()
+    res.update(ms_per_stage=1e3 * float(np.median(times)), ms_min=1e3 * float(np.min(times)), calls=st["calls"],
+               candidates=st["candidates"], candidates_per_s=st["candidates"] / float(np.median(times)),
+               spectra=fel.engine.fel_counters())
+    if args.fel_data == "cd2":
+        from scipy.stats import pearsonr
+        lrt = [r["lrt"] for r in out]
+        res["lrt_pearson_vs_hyphy25"] = float(pearsonr(lrt, [m[3] for m in truth["mle"]])[0])
+    fel.engine.close()
+    if not args.no_cpu:
+        from oracle import data_ref as dr
+        from oracle import fel_ref as fr
+        from oracle import hyoracle as ho
+        names, seqs = dr.parse_fasta(fasta)
+        raln = dr.AlignmentRef(names, seqs)
+        rtree = dr.TreeRef.from_newick(newick)
+        prob = ho.Problem.from_refs(rtree, raln)
+        rbl = np.array([0.0 if n.id == rtree.root else n.bl for n in rtree.nodes])
+        n_done, n_cand, t0 = 0, 0, time.perf_counter()
+        for p in range(prob.num_patterns):
+            r = fr.fel(prob, ho.Code(0), theta, raln.pos_freqs, raln.cf3x4, rbl, alpha, beta, patterns=[p])[0]
+            n_done += 1
+            n_cand += r["evaluations"]
+            if time.perf_counter() - t0 >= args.cpu_seconds:
+                break
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n_cand / el, "unit": "candidates/s", "cores": 1, "kind": "port",
+                               "sample": f"first {n_done} of {prob.num_patterns} patterns refitted sequentially "
+                                         f"({n_cand} candidates in {el:.1f} s; FP64 oracle: eigendecomposition "
+                                         "per candidate omega, single-pattern pruning)"}
     print(json.dumps(res), flush=True)
 
 

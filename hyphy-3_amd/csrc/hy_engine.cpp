@@ -15,6 +15,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
+#include <map>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -32,6 +34,36 @@ thread_local std::string g_create_error;
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+};
+
+// Spectrum pool of the per-site refits: slot -> omega (NaN: free) and the
+// tick of its last use; omega -> slot by exact value and in order (nearest
+// cached omega = warm start).  Valid for one model key (exchangeabilities,
+// delta, psi and the problem/model generation).
+struct FelPool {
+    int cap = 0;
+    std::vector<double> omega;
+    std::vector<uint64_t> used;
+    std::unordered_map<uint64_t, int> slot_of;
+    std::map<double, int> by_omega;
+    uint64_t tick = 0;
+    double key[8] = {};
+    bool keyed = false;
+    int64_t n_cand = 0, n_hit = 0, n_warm = 0, n_cold = 0;  // counters (hy_fel_counters)
+    void clear() {
+        std::fill(omega.begin(), omega.end(), std::numeric_limits<double>::quiet_NaN());
+        std::fill(used.begin(), used.end(), 0);
+        slot_of.clear();
+        by_omega.clear();
+    }
+    void drop(int slot) {
+        if (std::isnan(omega[slot])) return;
+        uint64_t bits;
+        std::memcpy(&bits, &omega[slot], 8);
+        slot_of.erase(bits);
+        by_omega.erase(omega[slot]);
+        omega[slot] = std::numeric_limits<double>::quiet_NaN();
+    }
 };
 
 }  // namespace
@@ -143,11 +175,15 @@ struct hy_engine_s {
     DevBuf d_gsched, d_gnref, d_gnbr, d_gtips, d_gw, d_gP, d_gpi, d_gD, d_gE, d_gU, d_gH, d_gL, d_gLE, d_gsite,
         d_gApart, d_gA, d_gres;
     std::vector<uint8_t> g_ident;  // extended nodes whose edge is an identity
-    // per-site refits (hy_fel_eval): per-pattern warm-start eigenvectors and
-    // whether a pattern's slot holds a converged solve; candidate buffers
-    DevBuf d_fZ, d_fslot, d_fwarm, d_fpat, d_falpha, d_fparams, d_flam, d_fU, d_fV, d_fmu, d_fstatus, d_fbl, d_fD,
-        d_fE, d_fout, d_fident;
-    std::vector<uint8_t> f_valid;
+    // per-site refits (hy_fel_eval): a pool of MG94 spectra keyed by omega
+    // (eigenvectors kept for warm starts), per-call staging and scratch
+    FelPool fel;
+    DevBuf d_fZ, d_flam, d_fU, d_fV, d_fmu, d_fstatus, d_fstage, d_fbl, d_fD, d_fE, d_fout, d_fident;
+    void *f_pin = nullptr;  // pinned staging: inputs, then outputs
+    size_t f_pin_bytes = 0;
+    std::vector<double> f_bl;  // branch lengths last uploaded to d_fbl
+    bool f_ident_ready = false;
+    uint64_t model_gen = 0;  // hy_set_problem / hy_set_model calls
     std::vector<uint16_t> na_tips;
     std::vector<double> na_weights;
     std::vector<int32_t> na_sched, na_ident;
@@ -1376,9 +1412,8 @@ void hy_destroy(hy_engine h) {
                       &h->d_sink, &h->d_sinkwl, &h->d_blist_ck, &h->d_bsel, &h->d_gsched, &h->d_gnref,
                       &h->d_gnbr, &h->d_gtips, &h->d_gw, &h->d_gP, &h->d_gpi, &h->d_gD, &h->d_gE, &h->d_gU,
                       &h->d_gH, &h->d_gL, &h->d_gLE, &h->d_gsite, &h->d_gApart, &h->d_gA, &h->d_gres,
-                      &h->d_fZ, &h->d_fslot, &h->d_fwarm, &h->d_fpat, &h->d_falpha, &h->d_fparams, &h->d_flam,
-                      &h->d_fU, &h->d_fV, &h->d_fmu, &h->d_fstatus, &h->d_fbl, &h->d_fD, &h->d_fE, &h->d_fout,
-                      &h->d_fident};
+                      &h->d_fZ, &h->d_flam, &h->d_fU, &h->d_fV, &h->d_fmu, &h->d_fstatus, &h->d_fstage, &h->d_fbl,
+                      &h->d_fD, &h->d_fE, &h->d_fout, &h->d_fident};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -1392,6 +1427,7 @@ void hy_destroy(hy_engine h) {
     if (h->h_res) (void)hipHostFree(h->h_res);
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->h_site) (void)hipHostFree(h->h_site);
+    if (h->f_pin) (void)hipHostFree(h->f_pin);
     if (h->profile)
         for (auto &ev : h->ev) (void)hipEventDestroy(ev);
     for (auto &g : h->graphs) destroy_graph_exec(g.exec);
@@ -1459,6 +1495,7 @@ int hy_set_model(hy_engine h, int code_id, const double *pos_freqs, const double
     h->pi_host = pp;
     h->have_model = true;
     h->warm_valid = false;
+    ++h->model_gen;
     return HY_OK;
 }
 
@@ -1692,7 +1729,8 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
             if (h->ext_ident[n]) h->na_ident.push_back(n);
     }
     h->gtr_ready = false;
-    h->f_valid.clear();
+    h->f_ident_ready = false;
+    ++h->model_gen;
     h->have_problem = true;
     return HY_OK;
 }
@@ -3334,6 +3372,38 @@ int hy_gtr_eval(hy_engine h, int precision, unsigned flags, const double *bl, co
     return HY_OK;
 }
 
+// Grows a device buffer to `bytes`, keeping its first `keep` bytes.
+static int grow_keep(hy_engine h, DevBuf &b, size_t bytes, size_t keep) {
+    if (b.p && b.bytes >= bytes) return HY_OK;
+    DevBuf nb;
+    HIP_TRY(h, hipMalloc(&nb.p, bytes));
+    nb.bytes = bytes;
+    h->device_bytes += bytes;
+    if (b.p && keep) HIP_TRY(h, hipMemcpyAsync(nb.p, b.p, std::min(keep, b.bytes), hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    release(h, b);
+    b = nb;
+    ++h->epoch;
+    return HY_OK;
+}
+
+// The spectrum pool holds `cap` slots (lam, U, V, mu, status, Z per slot).
+static int fel_pool_reserve(hy_engine h, int cap) {
+    FelPool &f = h->fel;
+    if (cap <= f.cap) return HY_OK;
+    const size_t o = (size_t)f.cap, n = (size_t)cap;
+    TRY(grow_keep(h, h->d_flam, n * 64 * 8, o * 64 * 8));
+    TRY(grow_keep(h, h->d_fU, n * 4096 * 8, o * 4096 * 8));
+    TRY(grow_keep(h, h->d_fV, n * 4096 * 8, o * 4096 * 8));
+    TRY(grow_keep(h, h->d_fZ, n * 4096 * 8, o * 4096 * 8));
+    TRY(grow_keep(h, h->d_fmu, n * 8, o * 8));
+    TRY(grow_keep(h, h->d_fstatus, n * 4, o * 4));
+    f.omega.resize(n, std::numeric_limits<double>::quiet_NaN());
+    f.used.resize(n, 0);
+    f.cap = cap;
+    return HY_OK;
+}
+
 int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *beta,
                 const double *theta, double delta, double psi, const double *bl, double *out_ll, double *out_scale) {
     // FELAnalyzer::analyze_pattern's eval_pattern_lnl (fel.hpp:367-385):
@@ -3346,18 +3416,16 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
     if (n_cand < 0 || (n_cand > 0 && (!pattern || !alpha || !beta || !out_ll)) || !theta || !bl)
         return fail(h, HY_ERR_ARG, "%s: bad candidate arrays", who);
     if (n_cand == 0) return HY_OK;
-    if (h->f_valid.size() != (size_t)h->P) h->f_valid.assign(h->P, 0);
-    std::vector<uint8_t> seen(h->P, 0), warm(n_cand);
-    std::vector<double> om(n_cand);
-    for (int k = 0; k < n_cand; ++k) {
+    if (n_cand > (1 << 24)) return fail(h, HY_ERR_ARG, "%s: %d candidates in one call", who, n_cand);
+    const size_t K = (size_t)n_cand;
+    std::vector<double> om(K);
+    for (size_t k = 0; k < K; ++k) {
         const int p = pattern[k];
         if (p < 0 || p >= h->P) return fail(h, HY_ERR_ARG, "%s: pattern %d out of range", who, p);
-        if (seen[p]) return fail(h, HY_ERR_ARG, "%s: pattern %d appears twice in one call", who, p);
-        seen[p] = 1;
         if (!(alpha[k] >= 0.0 && beta[k] >= 0.0 && std::isfinite(alpha[k]) && std::isfinite(beta[k])))
-            return fail(h, HY_ERR_ARG, "%s: candidate %d has alpha %g, beta %g", who, k, alpha[k], beta[k]);
+            return fail(h, HY_ERR_ARG, "%s: candidate %zu has alpha %g, beta %g", who, k, alpha[k], beta[k]);
         om[k] = beta[k] / std::max(alpha[k], 1e-12);
-        warm[k] = h->f_valid[p];
+        if (!std::isfinite(om[k])) return fail(h, HY_ERR_ARG, "%s: candidate %zu has omega %g", who, k, om[k]);
     }
     for (int n = 0; n < h->num_nodes; ++n)
         if (n != h->root && !(bl[n] >= 0.0 && std::isfinite(bl[n])))
@@ -3369,75 +3437,163 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
         return fail(h, HY_ERR_ARG, "%s: delta %g, psi %g", who, delta, psi);
     HIP_TRY(h, hipSetDevice(h->device));
     TRY(gtr_prepare(h));  // pattern-walk schedule, node tables, padded tip codes
-    const size_t K = (size_t)n_cand;
-    if (!h->d_fZ.p || h->d_fZ.bytes < (size_t)h->P * 4096 * 8) {
-        TRY(ensure(h, h->d_fZ, (size_t)h->P * 4096 * 8));
-        std::fill(h->f_valid.begin(), h->f_valid.end(), 0);
-        std::fill(warm.begin(), warm.end(), 0);
+    hipStream_t s = h->stream;
+    // per-problem tables: identity edges, branch lengths (uploaded when they change)
+    if (!h->f_ident_ready) {
+        std::vector<uint8_t> ident(h->g_ident.begin(), h->g_ident.end());
+        TRY(upload(h, h->d_fident, ident.data(), ident.size()));
+        h->f_ident_ready = true;
     }
-    // parameter block of k_model: bl, alpha = 1, one omega per candidate, theta, delta = psi = 0
+    {
+        std::vector<double> blx(h->n_ext, 0.0);
+        for (int n = 0; n < h->num_nodes; ++n) blx[n] = (n == h->root) ? 0.0 : bl[n];
+        if (blx != h->f_bl || !h->d_fbl.p) {
+            TRY(upload(h, h->d_fbl, blx.data(), blx.size() * 8));
+            h->f_bl = blx;
+        }
+    }
+    // spectrum pool: a new model key empties it
+    FelPool &f = h->fel;
+    const double key[8] = {theta[0], theta[1], theta[2], theta[3], theta[4], delta, psi, (double)h->model_gen};
+    if (!f.keyed || std::memcmp(key, f.key, sizeof key) != 0) {
+        f.clear();
+        std::memcpy(f.key, key, sizeof key);
+        f.keyed = true;
+    }
+    const uint64_t tick = ++f.tick;
+    // distinct omegas of the call: pooled ones are hits, the others are solved
+    std::vector<int32_t> spec(K, -1);
+    std::vector<double> miss_om;
+    std::unordered_map<uint64_t, int> call;  // omega bits -> slot, or -(1 + miss index)
+    int n_hit = 0;
+    for (size_t k = 0; k < K; ++k) {
+        uint64_t bits;
+        std::memcpy(&bits, &om[k], 8);
+        auto c = call.find(bits);
+        if (c != call.end()) {
+            spec[k] = c->second;
+            continue;
+        }
+        auto it = f.slot_of.find(bits);
+        if (it != f.slot_of.end()) {
+            f.used[it->second] = tick;
+            call[bits] = spec[k] = it->second;
+            ++n_hit;
+            ++f.n_hit;
+        } else {
+            call[bits] = spec[k] = -1 - (int)miss_om.size();
+            miss_om.push_back(om[k]);
+        }
+    }
+    const int n_miss = (int)miss_om.size();
+    if (n_hit + n_miss > f.cap) TRY(fel_pool_reserve(h, std::max({n_hit + n_miss, 2 * f.cap, 64})));
+    // output slots of the misses: free slots first, then the least recently
+    // used; never a slot this call reads
+    std::vector<int32_t> oslot(n_miss), zsrc(n_miss, -1);
+    if (n_miss > 0) {
+        std::vector<int> cand;
+        cand.reserve(f.cap);
+        for (int sl = 0; sl < f.cap; ++sl)
+            if (f.used[sl] != tick) cand.push_back(sl);
+        if ((int)cand.size() < n_miss) return fail(h, HY_ERR_STATE, "internal: spectrum pool too small");
+        std::partial_sort(cand.begin(), cand.begin() + n_miss, cand.end(), [&](int x, int y) {
+            const bool fx = std::isnan(f.omega[x]), fy = std::isnan(f.omega[y]);
+            if (fx != fy) return fx;
+            return f.used[x] != f.used[y] ? f.used[x] < f.used[y] : x < y;
+        });
+        for (int i = 0; i < n_miss; ++i) {
+            oslot[i] = cand[i];
+            f.drop(cand[i]);
+        }
+    }
+    // warm start from the pooled spectrum nearest in log(omega)
+    auto dist = [](double x, double y) {
+        return std::fabs(std::log(std::max(x, 1e-300)) - std::log(std::max(y, 1e-300)));
+    };
+    for (int i = 0; i < n_miss && !f.by_omega.empty(); ++i) {
+        auto hi = f.by_omega.lower_bound(miss_om[i]);
+        int best = -1;
+        double bd = 0.0;
+        if (hi != f.by_omega.end()) {
+            best = hi->second;
+            bd = dist(hi->first, miss_om[i]);
+        }
+        if (hi != f.by_omega.begin()) {
+            auto lo = std::prev(hi);
+            if (best < 0 || dist(lo->first, miss_om[i]) < bd) best = lo->second;
+        }
+        zsrc[i] = best;
+    }
+    for (size_t k = 0; k < K; ++k)
+        if (spec[k] < 0) spec[k] = oslot[-1 - spec[k]];
+    // staging: [params][alpha] doubles, then [oslot][zsrc][spec][pattern] ints
     ParamLayout pl{};
-    pl.n_nodes = h->n_ext;
-    pl.n_mat = n_cand;
+    pl.n_nodes = 0;
+    pl.n_mat = n_miss;
     pl.n_comp = 1;
     pl.off_bl = 0;
-    pl.off_alpha = h->n_ext;
-    pl.off_omega = pl.off_alpha + 1;
-    pl.off_w = pl.off_omega + n_cand;
+    pl.off_alpha = 0;
+    pl.off_omega = 1;
+    pl.off_w = 1 + n_miss;
     pl.off_theta = pl.off_w + 1;
     pl.off_delta = pl.off_theta + 5;
     pl.off_psi = pl.off_delta + 1;
     pl.total = pl.off_psi + 1;
-    std::vector<double> prm(pl.total, 0.0), blx(h->n_ext, 0.0);
-    for (int n = 0; n < h->num_nodes; ++n) blx[n] = (n == h->root) ? 0.0 : bl[n];
-    std::copy(blx.begin(), blx.end(), prm.begin());
-    prm[pl.off_alpha] = 1.0;
-    std::copy(om.begin(), om.end(), prm.begin() + pl.off_omega);
-    prm[pl.off_w] = 1.0;
-    std::copy(theta, theta + 5, prm.begin() + pl.off_theta);
-    prm[pl.off_delta] = delta;
-    prm[pl.off_psi] = psi;
-    std::vector<int32_t> slots(pattern, pattern + n_cand);
-    std::vector<uint8_t> ident(h->g_ident.begin(), h->g_ident.end());
-    TRY(upload(h, h->d_fparams, prm.data(), prm.size() * 8));
-    TRY(upload(h, h->d_fslot, slots.data(), K * 4));
-    TRY(upload(h, h->d_fwarm, warm.data(), K));
-    TRY(upload(h, h->d_fpat, slots.data(), K * 4));
-    TRY(upload(h, h->d_falpha, alpha, K * 8));
-    TRY(upload(h, h->d_fbl, blx.data(), blx.size() * 8));
-    TRY(upload(h, h->d_fident, ident.data(), ident.size()));
-    TRY(ensure(h, h->d_flam, K * 64 * 8));
-    TRY(ensure(h, h->d_fU, K * 4096 * 8));
-    TRY(ensure(h, h->d_fV, K * 4096 * 8));
-    TRY(ensure(h, h->d_fmu, K * 8));
-    TRY(ensure(h, h->d_fstatus, K * 4));
+    const size_t n_dbl = (size_t)pl.total + K, n_i32 = 2 * (size_t)n_miss + 2 * K;
+    const size_t in_bytes = n_dbl * 8 + n_i32 * 4, out_bytes = 3 * K * 8;
+    const size_t pin_need = std::max(in_bytes, out_bytes);
+    if (h->f_pin_bytes < pin_need) {
+        if (h->f_pin) (void)hipHostFree(h->f_pin);
+        h->f_pin = nullptr;
+        h->f_pin_bytes = 0;
+        HIP_TRY(h, hipHostMalloc(&h->f_pin, pin_need * 2, hipHostMallocDefault));
+        h->f_pin_bytes = pin_need * 2;
+    }
+    double *pd = (double *)h->f_pin;
+    pd[pl.off_alpha] = 1.0;
+    std::copy(miss_om.begin(), miss_om.end(), pd + pl.off_omega);
+    pd[pl.off_w] = 1.0;
+    std::copy(theta, theta + 5, pd + pl.off_theta);
+    pd[pl.off_delta] = delta;
+    pd[pl.off_psi] = psi;
+    std::copy(alpha, alpha + K, pd + pl.total);
+    int32_t *pi32 = (int32_t *)(pd + n_dbl);
+    std::copy(oslot.begin(), oslot.end(), pi32);
+    std::copy(zsrc.begin(), zsrc.end(), pi32 + n_miss);
+    std::copy(spec.begin(), spec.end(), pi32 + 2 * n_miss);
+    std::copy(pattern, pattern + K, pi32 + 2 * n_miss + K);
+    TRY(ensure(h, h->d_fstage, in_bytes));
     TRY(ensure(h, h->d_fD, K * std::max(h->n_int, 1) * 64 * 8));
     TRY(ensure(h, h->d_fE, K * std::max(h->n_int, 1) * 4));
-    TRY(ensure(h, h->d_fout, K * 8));
-    hipStream_t s = h->stream;
-    ModelArgs ma{};
-    ma.qdesc = (const int32_t *)h->d_qdesc.p;
-    ma.nd = (const int8_t *)h->d_nd.p;
-    ma.dpos = (const int8_t *)h->d_dpos.p;
-    ma.dfrom = (const int8_t *)h->d_dfrom.p;
-    ma.dto = (const int8_t *)h->d_dto.p;
-    ma.syn = (const uint8_t *)h->d_syn.p;
-    ma.pos = (const double *)h->d_pos.p;
-    ma.pi = (const double *)h->d_pi.p;
-    ma.S = h->S;
-    ma.params = (const double *)h->d_fparams.p;
-    ma.pl = pl;
-    ma.lam = (double *)h->d_flam.p;
-    ma.U = (double *)h->d_fU.p;
-    ma.V = (double *)h->d_fV.p;
-    ma.mu = (double *)h->d_fmu.p;
-    ma.Z = (double *)h->d_fZ.p;
-    ma.warm = 1;
-    ma.zslot = (const int32_t *)h->d_fslot.p;
-    ma.zwarm = (const uint8_t *)h->d_fwarm.p;
-    ma.status = (int *)h->d_fstatus.p;
-    ma.tol = 1e-14;
-    launch_model(ma, n_cand, s);
+    TRY(ensure(h, h->d_fout, out_bytes));
+    HIP_TRY(h, hipMemcpyAsync(h->d_fstage.p, h->f_pin, in_bytes, hipMemcpyHostToDevice, s));
+    const double *dd = (const double *)h->d_fstage.p;
+    const int32_t *di = (const int32_t *)(dd + n_dbl);
+    if (n_miss > 0) {
+        ModelArgs ma{};
+        ma.qdesc = (const int32_t *)h->d_qdesc.p;
+        ma.nd = (const int8_t *)h->d_nd.p;
+        ma.dpos = (const int8_t *)h->d_dpos.p;
+        ma.dfrom = (const int8_t *)h->d_dfrom.p;
+        ma.dto = (const int8_t *)h->d_dto.p;
+        ma.syn = (const uint8_t *)h->d_syn.p;
+        ma.pos = (const double *)h->d_pos.p;
+        ma.pi = (const double *)h->d_pi.p;
+        ma.S = h->S;
+        ma.params = dd;
+        ma.pl = pl;
+        ma.lam = (double *)h->d_flam.p;
+        ma.U = (double *)h->d_fU.p;
+        ma.V = (double *)h->d_fV.p;
+        ma.mu = (double *)h->d_fmu.p;
+        ma.Z = (double *)h->d_fZ.p;
+        ma.warm = 1;
+        ma.oslot = di;
+        ma.zsrc = di + n_miss;
+        ma.status = (int *)h->d_fstatus.p;
+        ma.tol = 1e-14;
+        launch_model(ma, n_miss, s);
+    }
     FelArgs fa{};
     fa.sched = (const int32_t *)h->d_gsched.p;
     fa.n_sched = h->g_n_sched;
@@ -3446,32 +3602,133 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
     fa.tips = (const uint16_t *)h->d_gtips.p;
     fa.tip_stride = h->g_P_pad;
     fa.amb = (const uint64_t *)h->d_amb.p;
-    fa.pat = (const int32_t *)h->d_fpat.p;
-    fa.alpha = (const double *)h->d_falpha.p;
-    fa.lam = ma.lam;
-    fa.U = ma.U;
-    fa.V = ma.V;
+    fa.pat = di + 2 * n_miss + K;
+    fa.alpha = dd + pl.total;
+    fa.spec = di + 2 * n_miss;
+    fa.lam = (const double *)h->d_flam.p;
+    fa.U = (const double *)h->d_fU.p;
+    fa.V = (const double *)h->d_fV.p;
+    fa.mu = (const double *)h->d_fmu.p;
+    fa.status = (const int *)h->d_fstatus.p;
     fa.bl = (const double *)h->d_fbl.p;
-    fa.pi = ma.pi;
+    fa.pi = (const double *)h->d_pi.p;
     fa.S = h->S;
     fa.n_int = std::max(h->n_int, 1);
+    fa.n_cand = n_cand;
     fa.D = (double *)h->d_fD.p;
     fa.E = (int32_t *)h->d_fE.p;
     fa.out = (double *)h->d_fout.p;
     launch_fel(fa, n_cand, s);
     HIP_TRY(h, hipGetLastError());
-    std::vector<int32_t> status(K);
-    HIP_TRY(h, hipMemcpyAsync(out_ll, h->d_fout.p, K * 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(h, hipMemcpyAsync(status.data(), h->d_fstatus.p, K * 4, hipMemcpyDeviceToHost, s));
-    if (out_scale) HIP_TRY(h, hipMemcpyAsync(out_scale, h->d_fmu.p, K * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipMemcpyAsync(h->f_pin, h->d_fout.p, out_bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(h, hipStreamSynchronize(s));
-    for (int k = 0; k < n_cand; ++k) {
-        if (status[k] < 0) {
-            h->f_valid[pattern[k]] = 0;
-            return fail(h, HY_ERR_NUMERIC, "%s: eigensolver did not converge for omega %g", who, om[k]);
-        }
-        h->f_valid[pattern[k]] = 1;
+    const double *res = (const double *)h->f_pin;
+    for (int i = 0; i < n_miss; ++i) {  // the new spectra join the pool
+        const int sl = oslot[i];
+        f.omega[sl] = miss_om[i];
+        f.used[sl] = tick;
+        uint64_t bits;
+        std::memcpy(&bits, &miss_om[i], 8);
+        f.slot_of[bits] = sl;
+        f.by_omega[miss_om[i]] = sl;
     }
+    f.n_cand += (int64_t)K;
+    {
+        std::unordered_map<int, double> st;  // status of each new spectrum
+        for (size_t k = 0; k < K; ++k) st[spec[k]] = res[2 * K + k];
+        for (int i = 0; i < n_miss; ++i) {
+            const double v = st[oslot[i]];
+            if (v >= 100.0)
+                ++f.n_warm;
+            else
+                ++f.n_cold;
+        }
+    }
+    int bad = -1;
+    for (size_t k = 0; k < K; ++k)
+        if (res[2 * K + k] < 0.0) {
+            f.drop(spec[k]);
+            bad = (int)k;
+        }
+    if (bad >= 0) return fail(h, HY_ERR_NUMERIC, "%s: eigensolver did not converge for omega %g", who, om[bad]);
+    std::copy(res, res + K, out_ll);
+    if (out_scale) std::copy(res + K, res + 2 * K, out_scale);
+    return HY_OK;
+}
+
+namespace {
+struct FelRunCtx {
+    hy_engine h;
+    const double *theta;
+    double delta, psi;
+    const double *bl;
+};
+int fel_run_eval(void *user, int n, const int32_t *pattern, const double *alpha, const double *beta, double *out_ll) {
+    const FelRunCtx *c = (const FelRunCtx *)user;
+    return hy_fel_eval(c->h, n, pattern, alpha, beta, c->theta, c->delta, c->psi, c->bl, out_ll, nullptr);
+}
+}  // namespace
+
+int hy_fel_counters(hy_engine h, int64_t *out4, int reset) {
+    if (!h || !out4) return HY_ERR_ARG;
+    FelPool &f = h->fel;
+    out4[0] = f.n_cand;
+    out4[1] = f.n_hit;
+    out4[2] = f.n_warm;
+    out4[3] = f.n_cold;
+    if (reset) f.n_cand = f.n_hit = f.n_warm = f.n_cold = 0;
+    return HY_OK;
+}
+
+int hy_fel_run(hy_engine h, const double *theta, double alpha, double beta, double delta, double psi,
+               const double *bl, double *out, int *out_calls, int64_t *out_candidates, double *out_q_scale) {
+    const char *who = "hy_fel_run";
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "%s before hy_set_problem/hy_set_model", who);
+    if (!theta || !bl || !out) return fail(h, HY_ERR_ARG, "%s: NULL argument", who);
+    if (!(alpha >= 0.0 && beta >= 0.0 && std::isfinite(alpha) && std::isfinite(beta)))
+        return fail(h, HY_ERR_ARG, "%s: alpha %g, beta %g", who, alpha, beta);
+    if (h->P == 0) {
+        if (out_calls) *out_calls = 0;
+        if (out_candidates) *out_candidates = 0;
+        return HY_OK;
+    }
+    // prepare_branch_lengths (fel.hpp:240-262): Q_scale of the global model;
+    // the generator is linear in (alpha, beta), so Q_scale = a mu(1, beta / a)
+    std::vector<double> bl0(h->num_nodes, 0.0);
+    double tested = 0.0;
+    for (int n = 0; n < h->num_nodes; ++n) {
+        if (n == h->root) continue;
+        if (!(bl[n] >= 0.0 && std::isfinite(bl[n])))
+            return fail(h, HY_ERR_ARG, "%s: branch length %d = %g", who, n, bl[n]);
+        bl0[n] = bl[n];
+        tested += bl[n];
+    }
+    const int32_t p0 = 0;
+    const double a0 = std::max(alpha, 1e-12);
+    double ll0 = 0.0, mu0 = 0.0;
+    TRY(hy_fel_eval(h, 1, &p0, &a0, &beta, theta, delta, psi, bl0.data(), &ll0, &mu0));
+    const double q_scale = a0 * mu0;
+    const double conv = q_scale > 1e-12 ? 3.0 / q_scale : 1.0;
+    std::vector<double> blc(h->num_nodes, 0.0);
+    for (int n = 0; n < h->num_nodes; ++n) blc[n] = (n == h->root) ? 0.0 : bl[n] * conv;
+    // invariant patterns: every resolved codon the same (fel.hpp:333-350)
+    std::vector<uint8_t> inv(h->P, 1);
+    for (int p = 0; p < h->P; ++p) {
+        int first = -1;
+        for (int t = 0; t < h->num_taxa && inv[p]; ++t) {
+            const uint16_t c = h->codes[(size_t)t * h->P + p];
+            if (c >= 64u) continue;
+            if (first < 0)
+                first = c;
+            else if ((int)c != first)
+                inv[p] = 0;
+        }
+    }
+    FelRunCtx ctx{h, theta, delta, psi, blc.data()};
+    const int rc = hy_fel_drive(h->P, inv.data(), tested, fel_run_eval, &ctx, out, out_calls, out_candidates);
+    if (rc != HY_OK) return rc;
+    if (out_q_scale) *out_q_scale = q_scale;
     return HY_OK;
 }
 

@@ -134,17 +134,19 @@ struct ModelArgs {
     const double *params_host;
     double *params_dev;
     int n_params;
-    // Warm-start state per slot (per-pattern refits): matrix m refines and
-    // stores the eigenvectors of slot zslot[m] (null: slot m), and only when
-    // zwarm[m] is set (null: every matrix, as warm says)
-    const int32_t *zslot;
-    const uint8_t *zwarm;
+    // Slot addressing (spectrum pools): matrix m writes lam, U, V, mu, status
+    // and its eigenvectors Z to slot oslot[m] (null: slot m) and, when warm,
+    // refines the eigenvectors held in slot zsrc[m] (null: its own slot;
+    // negative: a cold solve).  The source slot is never an output slot of
+    // the same launch.
+    const int32_t *oslot;
+    const int32_t *zsrc;
 };
 
 // Per-site refits (FEL): candidate k is pattern pat[k] at (alpha[k], beta[k]);
-// its spectrum (MG94 with omega = beta / max(alpha, 1e-12)) is matrix k of
-// the spectra buffers, branch times alpha * t_b.  One workgroup per candidate
-// walks the pattern-walk schedule (GTR_STRIDE entries).
+// its spectrum (MG94 with omega = beta / max(alpha, 1e-12)) is slot spec[k]
+// of the spectrum pool, branch times max(alpha, 1e-12) * t_b.  One workgroup
+// per candidate walks the pattern-walk schedule (GTR_STRIDE entries).
 struct FelArgs {
     const int32_t *sched;
     int n_sched;
@@ -155,13 +157,16 @@ struct FelArgs {
     const uint64_t *amb;
     const int32_t *pat;      // [n_cand]
     const double *alpha;     // [n_cand]
-    const double *lam, *U, *V;  // spectra of the candidates ([k][64], [k][64][64])
+    const int32_t *spec;     // [n_cand] spectrum slot
+    const double *lam, *U, *V;  // spectrum pool ([slot][64], [slot][64][64])
+    const double *mu;        // [slot] generator mean rate
+    const int *status;       // [slot] eigensolver status
     const double *bl;        // [n_ext] branch lengths (model time at alpha = 1)
     const double *pi;        // [64]
-    int S, n_int;
+    int S, n_int, n_cand;
     double *D;               // [n_cand][n_int][64] partials
     int32_t *E;              // [n_cand][n_int]
-    double *out;             // [n_cand] pattern lnL
+    double *out;             // [3][n_cand]: pattern lnL, mean rate, eigensolver status
 };
 void launch_fel(const FelArgs &a, int n_cand, hipStream_t s);
 

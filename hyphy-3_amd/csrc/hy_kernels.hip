@@ -195,6 +195,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
     matrix_gene_view(a, blockIdx.x);
     constexpr int NT = MODEL_NT;
     const int m = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    const int o = a.oslot ? a.oslot[m] : m;  // output slot
     __shared__ double A[64 * LDA];
     __shared__ double Z[64 * LDA];
     __shared__ double T1[64 * LDA];
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
     if (tid < 64) {
         double t = tid < S ? -a.pi[tid] * A[tid * LDA + tid] : 0.0;
         for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
-        if (tid == 0) a.mu[m] = t;
+        if (tid == 0) a.mu[o] = t;
     }
     for (int idx = tid; idx < 4096; idx += NT) {
         const int i = idx >> 6, j = idx & 63;
@@ -282,10 +283,12 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
 
     int status = 0;
     bool done = false;
-    double *Zg = a.Z + (size_t)(a.zslot ? a.zslot[m] : m) * 4096;
-    if (a.warm && (!a.zwarm || a.zwarm[m])) {
+    double *Zg = a.Z + (size_t)o * 4096;
+    const int zs = a.zsrc ? a.zsrc[m] : o;  // warm-start source slot (negative: cold)
+    if (a.warm && zs >= 0) {
         // ---- Ogita-Aishima refinement of the previous eigenvectors ----
-        for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = Zg[idx];
+        const double *Zs = a.Z + (size_t)zs * 4096;
+        for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = Zs[idx];
         __syncthreads();
         double prev_en = 1e300;
         // Wave w owns the 16x16 block (rows 16 rb.., columns 16 cb..) of
@@ -496,7 +499,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
         }
         __syncthreads();
     }
-    double *lo = a.lam + m * 64, *U = a.U + m * 4096, *V = a.V + m * 4096;
+    double *lo = a.lam + (size_t)o * 64, *U = a.U + (size_t)o * 4096, *V = a.V + (size_t)o * 4096;
     if (tid < 64) lo[tid] = lam[tid];
     for (int idx = tid; idx < 4096; idx += NT) {
         const int i = idx >> 6, k = idx & 63;
@@ -505,7 +508,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
         U[idx] = isq[i] * z;                       // U[i][k]
         V[idx] = Z[k * LDA + i] * sq[k];           // V[i][k] = Z[k][i] sqrt(pi_k)
     }
-    if (tid == 0) a.status[m] = status;
+    if (tid == 0) a.status[o] = status;
 }
 
 // Write one branch's P (FP64 in LDS) as row-major with row sums and the MFMA
@@ -2384,18 +2387,25 @@ template void launch_gtr<double>(const GtrArgs<double> &, bool, hipStream_t);
 // ---------------------------------------------------------------------------
 // k_fel: per-site refit candidates (FELAnalyzer::compute_pattern_log_likelihood,
 // fel.hpp:276-308, at eval_pattern_lnl's alpha and omega, fel.hpp:367-385).
-// One workgroup per candidate walks its pattern through the tree: per child
-// the clamped P = max(U diag(e^{l a t}) V, 0) is built in LDS on FP64 MFMA
-// and applied to the child's partial; partials are normalised to max in
+// One workgroup per candidate walks its pattern through the tree.  A child
+// edge contributes x = max(P, 0) d with P = U diag(e^{l a t}) V: for a tip
+// with one or a few possible codons only those columns of P are formed
+// (four threads per row, each column's entries clamped before they are
+// summed); otherwise (internal child, gap, wide ambiguity) the whole P is
+// built in LDS on FP64 MFMA and applied.  Partials are normalised to max in
 // [0.5, 1) by an exact power of two.  FP64 throughout.
 // ---------------------------------------------------------------------------
+constexpr int FEL_MAX_COLUMNS = 12;  // tip columns formed one by one up to this many
+
 __global__ __launch_bounds__(256) void k_fel(FelArgs a) {
     const int k = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     __shared__ double Us[64 * LDA], Vs[64 * LDA], Pt[64 * LDA];
     __shared__ double ex[64], dv[64], acc[64], red[4];
-    const int p = a.pat[k];
+    __shared__ int cols[64];
+    __shared__ int ncol;
+    const int p = a.pat[k], sl = a.spec[k];
     const double at = fmax(a.alpha[k], 1e-12);
-    const double *U = a.U + (size_t)k * 4096, *V = a.V + (size_t)k * 4096, *lam = a.lam + (size_t)k * 64;
+    const double *U = a.U + (size_t)sl * 4096, *V = a.V + (size_t)sl * 4096, *lam = a.lam + (size_t)sl * 64;
     for (int idx = tid; idx < 4096; idx += 256) {
         Us[(idx >> 6) * LDA + (idx & 63)] = U[idx];
         Vs[(idx >> 6) * LDA + (idx & 63)] = V[idx];
@@ -2403,6 +2413,7 @@ __global__ __launch_bounds__(256) void k_fel(FelArgs a) {
     const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
     double *Dk = a.D + (size_t)k * a.n_int * 64;
     int32_t *Ek = a.E + (size_t)k * a.n_int;
+    const int row = tid >> 2, part = tid & 3;
     int eacc = 0;
     for (int q = 0; q < a.n_sched; ++q) {
         const int32_t *e = a.sched + (size_t)q * GTR_STRIDE;
@@ -2411,28 +2422,44 @@ __global__ __launch_bounds__(256) void k_fel(FelArgs a) {
         eacc = 0;
         for (int c = 0; c < nc; ++c) {
             const int cn = e[3 + c], ref = a.nref[cn], kind = ref >> 28, idx = ref & REF_MASK;
-            __syncthreads();  // previous child's reads of dv / Pt are done
-            // the child's partial
+            __syncthreads();  // previous child's reads of dv / Pt / cols are done
+            uint64_t mask = 0;
+            if (kind == KIND_TIP) {
+                const unsigned code = a.tips[(size_t)idx * a.tip_stride + p];
+                mask = code == TIP_ONES ? all : (code & TIP_AMB) ? (a.amb[code & 0x7FFFu] & all) : (1ull << code);
+            }
+            const int pc = kind == KIND_TIP ? __popcll(mask) : 64;
             if (tid < 64) {
-                double d;
-                if (kind == KIND_TIP) {
-                    const unsigned code = a.tips[(size_t)idx * a.tip_stride + p];
-                    if (code == TIP_ONES)
-                        d = tid < a.S ? 1.0 : 0.0;
-                    else if (code & TIP_AMB)
-                        d = ((a.amb[code & 0x7FFFu] & all) >> tid) & 1ull ? 1.0 : 0.0;
-                    else
-                        d = (unsigned)tid == code ? 1.0 : 0.0;
-                } else {
-                    d = a.D[((size_t)k * a.n_int + idx) * 64 + tid];
-                }
-                dv[tid] = d;
+                dv[tid] = kind == KIND_TIP ? (double)((mask >> tid) & 1ull) : a.D[((size_t)k * a.n_int + idx) * 64 + tid];
                 ex[tid] = exp(lam[tid] * at * a.bl[cn]);
             }
-            if (kind != KIND_TIP && tid == 0) eacc += Ek[idx];
+            if (tid == 0) {
+                if (kind != KIND_TIP) eacc += Ek[idx];
+                int n = 0;
+                for (uint64_t mm = mask; mm && n < FEL_MAX_COLUMNS; mm &= mm - 1) cols[n++] = __ffsll((long long)mm) - 1;
+                ncol = n;
+            }
             __syncthreads();
             if (a.ident[cn]) {
                 if (tid < 64) acc[tid] *= dv[tid];
+                continue;
+            }
+            if (kind == KIND_TIP && pc <= FEL_MAX_COLUMNS) {
+                // x_i = sum over the tip's codons s of max(P_is, 0)
+                double x = 0.0;
+                for (int j = 0; j < ncol; ++j) {
+                    const int sc = cols[j];
+                    double v = 0.0;
+#pragma unroll
+                    for (int kk = 0; kk < 16; ++kk) {
+                        const int kx = 16 * part + kk;
+                        v += Us[row * LDA + kx] * ex[kx] * Vs[kx * LDA + sc];
+                    }
+                    v += __shfl_xor(v, 1);
+                    v += __shfl_xor(v, 2);
+                    x += v > 0.0 ? v : 0.0;
+                }
+                if (part == 0) acc[row] *= x;
                 continue;
             }
             // P = max(U diag(e) V, 0) in Pt (rows of wave w: 16w .. 16w + 15)
@@ -2456,7 +2483,6 @@ __global__ __launch_bounds__(256) void k_fel(FelArgs a) {
             __syncthreads();
             // x = P d (four threads per row), acc *= x
             {
-                const int row = tid >> 2, part = tid & 3;
                 double sacc = 0.0;
 #pragma unroll
                 for (int j = 0; j < 16; ++j) sacc += Pt[row * LDA + 16 * part + j] * dv[16 * part + j];
@@ -2487,7 +2513,11 @@ __global__ __launch_bounds__(256) void k_fel(FelArgs a) {
     if (tid < 64) {
         double l = a.pi[tid] * acc[tid];
         for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off);
-        if (tid == 0) a.out[k] = l > 0.0 ? log(l) + red[0] * 0.69314718055994530942 : -1e20;
+        if (tid == 0) {
+            a.out[k] = l > 0.0 ? log(l) + red[0] * 0.69314718055994530942 : -1e20;
+            a.out[a.n_cand + k] = a.mu[sl];
+            a.out[2 * a.n_cand + k] = (double)a.status[sl];
+        }
     }
 }
 

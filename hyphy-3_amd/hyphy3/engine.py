@@ -34,10 +34,19 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_set_warm_start", "hy_last_eigen_status", "hy_last_tile", "hy_last_parts", "hy_eval_srv", "hy_eval_batch", "hy_create_group",
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
            "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count",
-           "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval")
+           "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval", "hy_fel_drive", "hy_fel_run",
+           "hy_fel_counters")
+
+HY_FEL_FIELDS = 10
+FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt",
+                   "log_l_null", "evaluations")
+# int (*)(void *user, int n, const int32_t *pattern, const double *alpha, const double *beta, double *out_ll)
+FEL_EVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                          C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
+_u8p = C.POINTER(C.c_uint8)
 _u64p = C.POINTER(C.c_uint64)
 _f64p = C.POINTER(C.c_double)
 
@@ -85,6 +94,11 @@ def load_library(path: str = LIB_PATH):
     L.hy_gtr_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.hy_fel_eval.argtypes = [C.c_void_p, C.c_int, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double, _f64p, _f64p,
                               _f64p]
+    L.hy_fel_drive.argtypes = [C.c_int, _u8p, C.c_double, FEL_EVAL_FN, C.c_void_p, _f64p, C.POINTER(C.c_int),
+                               C.POINTER(C.c_int64)]
+    L.hy_fel_run.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, _f64p,
+                             C.POINTER(C.c_int), C.POINTER(C.c_int64), _f64p]
+    L.hy_fel_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int]
     L.hy_last_checkpoint.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.hy_set_sink.argtypes = [C.c_void_p, _f64p]
@@ -384,6 +398,25 @@ class Engine:
                                        float(delta), float(psi), _p(blc, _f64p), _p(out, _f64p), _p(sc, _f64p)))
         return (out[:n], sc[:n]) if scale else out[:n]
 
+    def fel_run(self, theta, alpha, beta, bl, delta=0.0, psi=0.0):
+        """FEL refits of every pattern (hy_fel_run): bl the fitted tree's
+        lengths.  Returns (array [num_patterns, HY_FEL_FIELDS], stats)."""
+        th = _f64(theta)
+        blc = _f64(bl)
+        if th.size != 5 or blc.size != self.num_nodes:
+            raise ValueError("fel_run: need 5 exchangeabilities and num_nodes branch lengths")
+        out = np.zeros((max(self.num_patterns, 1), HY_FEL_FIELDS))
+        calls, cands, qs = C.c_int(), C.c_int64(), C.c_double()
+        self._check(self.L.hy_fel_run(self.h, _p(th, _f64p), float(alpha), float(beta), float(delta), float(psi),
+                                      _p(blc, _f64p), _p(out, _f64p), C.byref(calls), C.byref(cands), C.byref(qs)))
+        return out[:self.num_patterns], {"calls": calls.value, "candidates": cands.value, "q_scale": qs.value}
+
+    def fel_counters(self, reset=False):
+        """hy_fel_eval counters: candidates, pool hits, warm and cold solves."""
+        out = (C.c_int64 * 4)()
+        self._check(self.L.hy_fel_counters(self.h, out, 1 if reset else 0))
+        return {"candidates": out[0], "pool_hits": out[1], "warm_solves": out[2], "cold_solves": out[3]}
+
     def inside_outside_from_P(self, P, pi, precision=HY_PREC_FP64):
         """hy_inside_outside_from_P: lnL from caller-supplied matrices, with
         the outside pass run so that the partials can be read back."""
@@ -468,6 +501,39 @@ class Engine:
         v = C.c_size_t()
         self._check(self.L.hy_device_bytes(self.h, C.byref(v)))
         return v.value
+
+
+def fel_drive(invariant, tested_length, evaluate):
+    """FEL refits (hy_fel_drive) over a Python candidate evaluator:
+    ``evaluate(patterns, alphas, betas)`` returns the candidates'
+    log-likelihoods.  Returns (array [n_patterns, HY_FEL_FIELDS], calls,
+    candidates)."""
+    L = load_library()
+    inv = np.ascontiguousarray(invariant, dtype=np.uint8)
+    n = inv.size
+    out = np.zeros((max(n, 1), HY_FEL_FIELDS))
+    failure = []
+
+    def cb(user, k, pat, a, b, ll):
+        try:
+            ps = np.ctypeslib.as_array(pat, shape=(k,)).copy()
+            av = np.ctypeslib.as_array(a, shape=(k,)).copy()
+            bv = np.ctypeslib.as_array(b, shape=(k,)).copy()
+            np.ctypeslib.as_array(ll, shape=(k,))[:] = evaluate(ps, av, bv)
+            return 0
+        except Exception as ex:  # reported after the drive returns
+            failure.append(ex)
+            return -100
+
+    fn = FEL_EVAL_FN(cb)
+    calls, cands = C.c_int(), C.c_int64()
+    rc = L.hy_fel_drive(n, _p(inv, _u8p), float(tested_length), fn, None, _p(out, _f64p), C.byref(calls),
+                        C.byref(cands))
+    if failure:
+        raise failure[0]
+    if rc != 0:
+        raise HyError(rc, "hy_fel_drive failed")
+    return out[:n], calls.value, cands.value
 
 
 def graph_count():

@@ -34,6 +34,10 @@
  *                    nucleotide stage of GTRFitter (optimizer.hpp:33-655)
  *   hy_fel_eval      FEL's per-site candidate likelihoods, eval_pattern_lnl
  *                    (fel.hpp:367-385) over many (pattern, alpha, beta)
+ *   hy_fel_drive     FELAnalyzer::analyze_pattern's refits (fel.hpp:326-445)
+ *                    for every pattern, over any candidate evaluator
+ *   hy_fel_run       FELAnalyzer::prepare_branch_lengths + run's refits
+ *                    (fel.hpp:227-272, 778-834) on the GPU evaluator
  */
 #ifndef HYPHY_MI355X_H
 #define HYPHY_MI355X_H
@@ -244,14 +248,68 @@ int hy_gtr_eval(hy_engine h, int precision, unsigned flags, const double *bl, co
  * log-likelihood (compute_pattern_log_likelihood, fel.hpp:276-308; -1e20
  * where the likelihood is not positive) in out_ll[k].  out_scale (may be
  * NULL) receives each candidate's generator mean rate sum_i pi_i (-Q_ii)
- * (MG94Matrix::scale_factor).  One workgroup per candidate; the spectra of
- * all candidates are computed in one launch, refining the eigenvectors kept
- * from the same pattern's previous candidate.  A pattern may appear at most
- * once per call (its warm-start slot); out-of-range patterns, negative or
- * non-finite rates are refused with HY_ERR_ARG, an eigensolver failure with
- * HY_ERR_NUMERIC.  Requires hy_set_problem and hy_set_model. */
+ * (MG94Matrix::scale_factor).  Patterns may repeat within a call.  The
+ * handle keeps a pool of spectra keyed by omega (for one theta, delta, psi
+ * and problem): a candidate whose omega is pooled reuses its spectrum,
+ * every other distinct omega is solved in one launch, refining the
+ * eigenvectors of the pooled omega nearest in log(omega).  Out-of-range
+ * patterns, negative or non-finite rates are refused with HY_ERR_ARG, an
+ * eigensolver failure with HY_ERR_NUMERIC.  Requires hy_set_problem and
+ * hy_set_model. */
 int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *beta,
                 const double *theta, double delta, double psi, const double *bl, double *out_ll, double *out_scale);
+
+/* Counters of hy_fel_eval since the handle was made (or the last reset):
+ * out4 = {candidates, candidates whose omega was already pooled, spectra
+ * solved by warm refinement, spectra solved cold}.  No reference
+ * counterpart (diagnostics of the spectrum pool). */
+int hy_fel_counters(hy_engine h, int64_t *out4, int reset);
+
+/* One row per pattern of the FEL refit results (SiteResult, fel.hpp:76-89). */
+enum {
+    HY_FEL_ALPHA = 0,        /* synonymous rate of the alternative fit */
+    HY_FEL_BETA = 1,         /* non-synonymous rate of the alternative fit */
+    HY_FEL_BETA_NUISANCE = 2,
+    HY_FEL_ALPHA_NULL = 3,   /* alpha = beta of the null fit */
+    HY_FEL_LRT = 4,
+    HY_FEL_P_VALUE = 5,      /* erfc(sqrt(LRT / 2)) */
+    HY_FEL_LENGTH = 6,       /* alpha_null x tested tree length */
+    HY_FEL_LOGL_ALT = 7,
+    HY_FEL_LOGL_NULL = 8,
+    HY_FEL_EVALS = 9,        /* candidate evaluations of the pattern */
+    HY_FEL_FIELDS = 10
+};
+
+/* Candidate evaluator of hy_fel_drive: n candidates (pattern[k], alpha[k],
+ * beta[k]) -> out_ll[k]; a non-zero return aborts the drive with that code. */
+typedef int (*hy_fel_eval_fn)(void *user, int n, const int32_t *pattern, const double *alpha, const double *beta,
+                              double *out_ll);
+
+/* FELAnalyzer::analyze_pattern with every branch tested (fel.hpp:326-445)
+ * for patterns 0 .. n_patterns-1: invariant patterns (all resolved codons
+ * equal, fel.hpp:333-361) get zeros and p = 1; every other one a 12-point
+ * start grid, NelderMead2D in [1e-4, 1000]^2 (tol 1e-4, 100 iterations,
+ * nelder_mead.hpp:121-204), the null alpha = beta from its start value and a
+ * 7-point grid, Brent1D on [1e-4, 50] (tol 1e-4, 60 iterations,
+ * nelder_mead.hpp:24-101), the LRT and its p-value.  The refits run as
+ * coroutines in lockstep: each call of `evaluate` holds the pending
+ * candidates of every unfinished pattern (the start grid and the null grid
+ * as one batch each), so each pattern sees the reference's own sequence of
+ * evaluations.  out[n_patterns * HY_FEL_FIELDS]; out_calls and
+ * out_candidates (may be NULL) count the evaluator calls and candidates.
+ * Host only (no GPU needed unless the evaluator uses one). */
+int hy_fel_drive(int n_patterns, const uint8_t *invariant, double tested_length, hy_fel_eval_fn evaluate,
+                 void *user, double *out, int *out_calls, int64_t *out_candidates);
+
+/* FEL on the handle's problem and model, every branch tested: the fitted
+ * tree's lengths bl[num_nodes] are converted by 3 / Q_scale of the global
+ * MG94 model (alpha, beta, theta[5], delta, psi; FELAnalyzer::
+ * prepare_branch_lengths, fel.hpp:227-272), then hy_fel_drive refits every
+ * pattern with hy_fel_eval as the evaluator.  out[num_patterns *
+ * HY_FEL_FIELDS] per pattern (sites map to patterns as the alignment's
+ * site_to_pattern); out_q_scale (may be NULL) receives Q_scale. */
+int hy_fel_run(hy_engine h, const double *theta, double alpha, double beta, double delta, double psi,
+               const double *bl, double *out, int *out_calls, int64_t *out_candidates, double *out_q_scale);
 
 /* lnL from caller-supplied transition matrices P[num_nodes*S*S] (row-major,
  * entry of the root ignored) and root frequencies pi[S]. */

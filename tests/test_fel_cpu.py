@@ -2,13 +2,15 @@
 
   * the oracle's optimisers (Brent1D, NelderMead2D restatements) on
     functions whose minima are known, including minima outside the box;
-  * the product's coroutine optimisers ask for exactly the points the
-    oracle's plain-loop ones evaluate, and return the same result;
+  * the native lockstep driver (hy_fel_drive) on synthetic per-pattern
+    objectives asks for exactly the candidates the oracle's plain-loop
+    analyze_pattern evaluates, in the same order, and returns the same
+    results;
   * the oracle's candidate evaluator against the oracle's whole-alignment
     MG94 pruning (site log-likelihoods at the same rates) and against a
     brute-force sum over ancestral codons with Taylor-series P;
-  * the lockstep driver (hyphy3.fel) fed the oracle's candidates reproduces
-    the oracle's sequential FEL bit for bit;
+  * the native driver fed the oracle's candidates reproduces the oracle's
+    sequential FEL on CD2 bit for bit;
   * the oracle's FEL on CD2 against HyPhy 2.5's per-site table at the
     thresholds of the reference's own parity test
     (tests/test_fel_cd2_parity.cpp:67-120).
@@ -25,19 +27,7 @@ from oracle import data_ref as dr
 from oracle import fel_ref as fr
 from oracle import hyoracle as ho
 from hyphy3 import fel as pf
-
-
-def run_coroutine(co, f):
-    """Drives a product optimiser coroutine with objective f; returns
-    (result, points asked)."""
-    pts = []
-    try:
-        pt = next(co)
-        while True:
-            pts.append(pt)
-            pt = co.send(f(*pt) if isinstance(pt, tuple) else f(pt))
-    except StopIteration as stop:
-        return stop.value, pts
+from hyphy3.engine import fel_drive
 
 
 def recorded(f):
@@ -58,14 +48,12 @@ BRENT_CASES = [
 
 
 @pytest.mark.parametrize("case", range(len(BRENT_CASES)))
-def test_brent_minima_and_product_equivalence(case):
+def test_brent_minima(case):
     f, xmin, (ax, bx, cx) = BRENT_CASES[case]
     g, pts = recorded(f)
     x, fx, it, conv = fr.brent_1d(g, ax, bx, cx, 1e-4, 60)
     assert abs(x - xmin) <= 2e-4 * max(1.0, xmin) + 1e-9, (x, xmin)
-    assert fx == f(x)
-    (x2, fx2, it2, conv2), pts2 = run_coroutine(pf.brent_1d(ax, bx, cx, 1e-4, 60), f)
-    assert pts2 == pts and (x2, fx2, it2, conv2) == (x, fx, it, conv)
+    assert fx == f(x) and all(min(ax, cx) <= p <= max(ax, cx) for p in pts)
 
 
 NM_CASES = [
@@ -77,7 +65,7 @@ NM_CASES = [
 
 
 @pytest.mark.parametrize("case", range(len(NM_CASES)))
-def test_nelder_mead_minima_and_product_equivalence(case):
+def test_nelder_mead_minima(case):
     f, (amin, bmin), (a0, b0) = NM_CASES[case]
     g, pts = recorded(f)
     a, b, fv, it, conv = fr.nelder_mead_2d(g, a0, b0, 1e-4, 1000.0, 1e-4, 100)
@@ -85,20 +73,58 @@ def test_nelder_mead_minima_and_product_equivalence(case):
     assert abs(a - amin) < 5e-3 * max(1.0, amin) and abs(b - bmin) < 5e-3 * max(1.0, bmin), (a, b)
     assert all(1e-4 <= p[0] <= 1000.0 and 1e-4 <= p[1] <= 1000.0 for p in pts)
     assert fv == f(a, b)
-    res2, pts2 = run_coroutine(pf.nelder_mead_2d(a0, b0, 1e-4, 1000.0, 1e-4, 100), f)
-    assert pts2 == pts and res2 == (a, b, fv, it, conv)
 
 
-def test_nelder_mead_collapsed_simplex_same_points():
+def test_nelder_mead_collapsed_simplex():
     """A reflection clamped to the box can flatten the simplex onto a bound,
     after which the reference's method stays there (here b = 1e-4, not the
-    minimum's 0.3); the coroutine must do the same."""
+    minimum's 0.3)."""
     f = lambda a, b: 0.01 * (a - 40.0) ** 2 + (b - 0.3) ** 2  # noqa: E731
-    g, pts = recorded(f)
-    res = fr.nelder_mead_2d(g, 10.0, 1.0, 1e-4, 1000.0, 1e-4, 100)
+    res = fr.nelder_mead_2d(f, 10.0, 1.0, 1e-4, 1000.0, 1e-4, 100)
     assert res[1] == 1e-4
-    res2, pts2 = run_coroutine(pf.nelder_mead_2d(10.0, 1.0, 1e-4, 1000.0, 1e-4, 100), f)
-    assert pts2 == pts and res2 == res
+
+
+SYNTH = [  # per-pattern log-likelihood surfaces (alternative optimum, null curvature, floor)
+    lambda a, b: -((math.log(a) - 0.7) ** 2 + 2.0 * (math.log(b) - 1.9) ** 2) - 20.0,
+    lambda a, b: -0.01 * (a - 40.0) ** 2 - (b - 0.3) ** 2 - 5.0,  # collapses onto b = 1e-4
+    lambda a, b: -abs(a - 0.4) - abs(b - 0.9) - 3.0,
+    lambda a, b: -((a - 2000.0) ** 2) * 1e-6 - (b - 3.0) ** 2,  # alpha beyond the box
+    lambda a, b: -1e20 if a > 50.0 else -((a - 0.02) ** 2 + (b - 0.02) ** 2) * 100.0,  # sentinel values
+    lambda a, b: -(math.log(b + 1e-3) - math.log(a + 1e-3)) ** 2 - 0.3 * a - 7.0,
+]
+
+
+def test_native_driver_matches_oracle_sequence():
+    """hy_fel_drive against the oracle's analyze_pattern on synthetic
+    objectives: the same candidates in the same order, the same results."""
+    asked = {p: [] for p in range(len(SYNTH) + 1)}
+
+    def evaluate(ps, a, b):
+        out = np.empty(len(ps))
+        for i, p in enumerate(ps.tolist()):
+            asked[p].append((float(a[i]), float(b[i])))
+            out[i] = SYNTH[p](float(a[i]), float(b[i]))
+        return out
+
+    inv = np.zeros(len(SYNTH) + 1, dtype=np.uint8)
+    inv[-1] = 1  # an invariant pattern: never evaluated
+    rows, calls, cands = fel_drive(inv, 2.5, evaluate)
+    assert asked[len(SYNTH)] == [] and rows[-1][4] == 0.0 and rows[-1][5] == 1.0
+    assert cands == sum(len(v) for v in asked.values())
+    for p, f in enumerate(SYNTH):
+        seq = []
+
+        def g(a, b, f=f, seq=seq):
+            seq.append((a, b))
+            return f(a, b)
+        ref = fr.analyze_pattern(g, np.array([0, 1]), 2.5)
+        assert asked[p] == seq, p
+        r = dict(zip(("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length",
+                      "log_l_alt", "log_l_null"), rows[p][:9]))
+        for k in ("alpha", "beta", "alpha_null", "lrt", "total_branch_length", "log_l_alt", "log_l_null"):
+            assert r[k] == ref[k], (p, k, r[k], ref[k])
+        assert abs(r["p_value"] - ref["p_value"]) <= 1e-15 * max(ref["p_value"], 1e-300), p
+        assert rows[p][9] == len(seq)
 
 
 def test_nelder_mead_converges_on_smooth_bowl():
@@ -191,19 +217,6 @@ def load_cd2_fel():
     return tree, aln, truth
 
 
-class OracleEngine:
-    """Stands in for the engine in hyphy3.fel.FEL on the CPU: the base
-    model's mean rate from the oracle."""
-
-    def __init__(self, code, theta, pos, pi):
-        self.code, self.theta, self.pos, self.pi = code, theta, pos, pi
-
-    def fel_eval(self, pattern, alpha, beta, theta, bl, delta=0.0, psi=0.0, scale=False):
-        mu = [fr.mg94_scale(self.code, theta, self.pos, self.pi, 1.0, b / max(a, 1e-12), delta, psi)
-              for a, b in zip(alpha, beta)]
-        return np.zeros(len(mu)), np.array(mu)
-
-
 def oracle_evaluator(prob, code, theta, pos, pi, blc):
     cache = {}
 
@@ -226,17 +239,20 @@ def test_lockstep_driver_reproduces_sequential_oracle():
     pats = [0, 1, 2, 3, 5, 8, 13, 40, 77, 120]
     seq = fr.fel(prob, code, g["theta"], aln.pos_freqs, aln.cf3x4, bl, g["alpha"], g["beta"], patterns=pats)
     arrays = dict(states=aln.states, site_to_pattern=aln.site_to_pattern, bl=bl, root=tree.root)
-    f = pf.FEL(None, None, g["theta"], g["alpha"], g["beta"], engine=OracleEngine(code, g["theta"], aln.pos_freqs,
-                                                                                  aln.cf3x4), arrays=arrays)
+    f = pf.FEL(None, None, g["theta"], g["alpha"], g["beta"], engine="no engine: candidates come from the oracle",
+               arrays=arrays)
     scale = fr.mg94_scale(code, g["theta"], aln.pos_freqs, aln.cf3x4, g["alpha"], g["beta"])
     blc, tested = fr.converted_lengths(bl, tree.root, scale)
-    np.testing.assert_array_equal(f.bl, blc)
     assert f.baseline_tree_length == tested
-    res = f.run(patterns=pats, evaluate=oracle_evaluator(prob, code, g["theta"], aln.pos_freqs, aln.cf3x4, f.bl))
+    np.testing.assert_array_equal(pf.invariant_patterns(aln.states[pats]),
+                                  [fr.is_invariant(aln.states[p]) for p in pats])
+    res = f.run(patterns=pats, evaluate=oracle_evaluator(prob, code, g["theta"], aln.pos_freqs, aln.cf3x4, blc))
     assert f.stats["candidates"] == sum(r["evaluations"] for r in seq)
     for p, r in zip(pats, seq):
-        for k in ("alpha", "beta", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt", "log_l_null"):
+        for k in ("alpha", "beta", "alpha_null", "lrt", "total_branch_length", "log_l_alt", "log_l_null"):
             assert res[p][k] == r[k], (p, k, res[p][k], r[k])
+        assert abs(res[p]["p_value"] - r["p_value"]) <= 1e-15 * max(r["p_value"], 1e-300)
+        assert res[p]["evaluations"] == r["evaluations"]
 
 
 def correlations(x, y):

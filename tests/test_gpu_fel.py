@@ -9,8 +9,10 @@
     way an optimiser moves them (warm-started spectra), on CD2 and on a random
     tree with polytomies, gaps and ambiguous codons; the generator mean rates
     against the oracle's;
-  * argument checks (a pattern twice in one call, out of range, negative
-    rates);
+  * the spectrum pool: repeated candidates (in one call and across calls)
+    give identical values, and values stay right after the pool grows and
+    evicts;
+  * argument checks (out of range, negative or non-finite rates);
   * the whole FEL on CD2 (the reference's matched-tree parity case) against
     HyPhy 2.5's per-site table at the reference test's thresholds
     (tests/test_fel_cd2_parity.cpp:67-120) and against the oracle's
@@ -80,7 +82,7 @@ def check_candidates(eng, prob, aln, theta, blc, calls, label):
 def candidate_calls(rng, P, n_random):
     ps_all = np.arange(P, dtype=np.int32)
     calls = []
-    for a, b in pf.START_GRID + ((1e-4, 1000.0), (1000.0, 1e-4), (0.01, 50.0), (1e-4, 1e-4)):
+    for a, b in fr.START_GRID + ((1e-4, 1000.0), (1000.0, 1e-4), (0.01, 50.0), (1e-4, 1e-4)):
         calls.append((ps_all, np.full(P, a), np.full(P, b)))
     # optimiser-like walks: each pattern moves by small relative steps
     a = rng.uniform(0.05, 20.0, size=P)
@@ -133,8 +135,6 @@ def test_argument_checks():
         print("[fel] argument checks", flush=True)
         assert eng.fel_eval([], [], [], g["theta"], bl).size == 0
         with pytest.raises(HyError):
-            eng.fel_eval([3, 3], [1.0, 2.0], [1.0, 1.0], g["theta"], bl)
-        with pytest.raises(HyError):
             eng.fel_eval([aln.num_patterns], [1.0], [1.0], g["theta"], bl)
         with pytest.raises(HyError):
             eng.fel_eval([0], [-1.0], [1.0], g["theta"], bl)
@@ -143,6 +143,54 @@ def test_argument_checks():
         # the handle still works after refused calls
         v = eng.fel_eval([0, 1], [1.0, 1.0], [1.0, 1.0], g["theta"], bl)
         assert np.all(np.isfinite(v))
+    finally:
+        eng.close()
+
+
+def test_spectrum_pool_repeats_growth_eviction():
+    tree, aln, truth = load_cd2_fel()
+    g = truth["global_mg94"]
+    prob = ho.Problem.from_refs(tree, aln)
+    bl = branch_lengths(tree) * 3.1
+    eng = make_engine(tree, aln)
+    rng = np.random.default_rng(5)
+    try:
+        print("[fel] spectrum pool", flush=True)
+        # the same candidate twice in one call, and again in the next call (pool hit)
+        ps = np.array([4, 4, 9, 4], dtype=np.int32)
+        a = np.array([0.7, 0.7, 0.7, 2.1])
+        b = np.array([1.3, 1.3, 1.3, 3.9])  # the last one: same omega, other alpha
+        v1 = eng.fel_eval(ps, a, b, g["theta"], bl)
+        v2 = eng.fel_eval(ps, a, b, g["theta"], bl)
+        assert v1[0] == v1[1] and np.array_equal(v1, v2)
+        # three calls of 150 distinct omegas each: the pool grows past its
+        # first size and then evicts; every value against the oracle
+        checks = []
+        for rep in range(3):
+            n = 150
+            pp = rng.integers(0, prob.num_patterns, size=n).astype(np.int32)
+            aa = rng.uniform(0.2, 5.0, size=n)
+            bb = aa * np.exp(rng.uniform(-3.0, 3.0, size=n))
+            checks.append((pp, aa, bb, eng.fel_eval(pp, aa, bb, g["theta"], bl)))
+        pp, aa, bb, _ = checks[0]  # the first call's omegas again, after growth and eviction
+        checks.append((pp, aa, bb, eng.fel_eval(pp, aa, bb, g["theta"], bl)))
+        worst = 0.0
+        for pp, aa, bb, vv in checks:
+            for i in range(len(pp)):
+                ev = fr.CandidateOracle(prob, ho.Code(0), g["theta"], aln.pos_freqs, aln.cf3x4, bl, int(pp[i]))
+                ref = ev(float(aa[i]), float(bb[i]))
+                bound = ev.rounding_bound()
+                err = abs(vv[i] - ref)
+                assert err <= 2.0 * bound + 1e-9 * max(1.0, abs(ref)), (i, vv[i], ref, bound)
+                if bound < 1e-10 * max(1.0, abs(ref)):
+                    worst = max(worst, err / max(1.0, abs(ref)))
+        print(f"[fel] pool: worst relative lnL error {worst:.2e}", flush=True)
+        assert worst < 1e-9
+        # a new theta empties the pool; values follow the new model
+        th2 = list(np.asarray(g["theta"]) * 1.3)
+        v3 = eng.fel_eval(ps[:1], a[:1], b[:1], th2, bl)
+        ref = fr.CandidateOracle(prob, ho.Code(0), th2, aln.pos_freqs, aln.cf3x4, bl, 4)(0.7, 1.3)
+        assert abs(v3[0] - ref) < 1e-9 * max(1.0, abs(ref))
     finally:
         eng.close()
 
