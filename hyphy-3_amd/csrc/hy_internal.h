@@ -125,7 +125,8 @@ struct ModelArgs {
     double *mu;   // [n_mat]
     double *Z;    // [n_mat][64][64] eigenvectors of the symmetrised generator (warm-start state)
     int warm;     // 1: refine Z from the previous call
-    int *status;  // [n_mat] Jacobi sweeps, 100+k after k refinement steps (100: kept), negative = no convergence
+    int *status;  // [n_mat] 100+k: k refinement steps of a warm start (100: kept); 50+k: k steps after a
+                  // tridiagonal solve; 1..40: Jacobi sweeps; negative = no convergence
     const int32_t *mat_gene;  // batched evaluation: gene of each matrix (pi, pos, alpha, theta, delta, psi)
     double tol;               // warm refinement: accepted predicted correction (relative)
     // When set: the parameter block is read from pinned host memory and

@@ -176,6 +176,347 @@ __device__ __forceinline__ void block_sum2_once(double &a, double &b, double *re
 }
 
 constexpr int MODEL_NT = 1024;
+
+// Reciprocal to about one ulp: the hardware estimate and one Newton step.
+__device__ __forceinline__ double rcp_nr(double q) {
+    const double r = __builtin_amdgcn_rcp(q);
+    return fma(r, fma(-q, r, 1.0), r);
+}
+
+// Makes LDS stores of a wave visible to its other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Scratch of the tridiagonal eigensolver (LDS, one per workgroup).
+constexpr int TRI_WAVES = 4;  // waves of the tridiagonalisation
+struct TriScratch {
+    double d[64], e[64], e2[64];     // diagonal, off-diagonal, squares (split entries zeroed)
+    double v[TRI_WAVES][64];         // reflector of the current step (one copy per wave)
+    double w[TRI_WAVES][64];         // its rank-2 partner (one copy per wave)
+    double part[TRI_WAVES][64];      // partial products M v per wave
+    double beta[64];                 // reflector scales (0: no reflection)
+    int blo[64], bhi[64];            // unreduced block (first, last row) of each row
+    double pivmin, tnorm, scale;
+};
+
+// Cold symmetric eigensolver for the n x n leading block of A (n >= 3, zero
+// outside it).
+//  * Householder tridiagonalisation of a copy in M by four waves: lane i of
+//    every wave owns row i, wave w a quarter of the trailing columns; each
+//    wave forms the reflector itself (same arithmetic, same result), so a
+//    step needs two workgroup barriers.  Reflector k is kept in column k of
+//    W, then moved below the diagonal of M.
+//  * Eigenvalues by 17-way multisection on Sturm counts of each unreduced
+//    block (16 lanes per eigenvalue), on T scaled by a power of two to norm
+//    about one: the Sturm sequence p_r = (d_r - x) p_r-1 - e_r-1^2 p_r-2 needs
+//    no division, and a power-of-two rescale every eight steps keeps it in
+//    range.
+//  * Eigenvectors by twisted factorisations of T - lam I (one lane each;
+//    the lane's scratch in column k of W and of Z).
+//  * Back-transformation by the reflectors, 16 lanes per column with the
+//    column in registers.
+//  * Two Loewdin steps Z <- Z (I + (I - Z^T Z) / 2), which restore
+//    orthonormality where close eigenvalues left their vectors tilted.
+// On exit Z holds the eigenvectors (columns; identity outside n) and lam the
+// eigenvalues (0 outside n).  A is not modified.
+template <int NT>
+__device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double *Z, double *M, double *W, double *lam,
+                                                        int n, int tid, TriScratch &ts) {
+    const int lane = tid & 63, w = tid >> 6;
+    for (int idx = tid; idx < 4096; idx += NT) M[(idx >> 6) * LDA + (idx & 63)] = A[(idx >> 6) * LDA + (idx & 63)];
+    __syncthreads();
+    // ---- Householder tridiagonalisation (waves 0 .. TRI_WAVES - 1) ----
+    for (int k = 0; k + 2 < n; ++k) {
+        const int i = lane;
+        const bool mine = i > k && i < n;
+        double vi = 0.0, beta = 0.0;
+        int j0 = 0, j1 = 0;
+        if (w < TRI_WAVES) {
+            const double xi = mine ? M[i * LDA + k] : 0.0;
+            double s = xi * xi;
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            const double x0 = __shfl(xi, k + 1);
+            const double alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
+            const double v0 = x0 - alpha;
+            const double vn2 = s - x0 * x0 + v0 * v0;
+            beta = vn2 > 0.0 ? 2.0 / vn2 : 0.0;
+            vi = mine ? (i == k + 1 ? v0 : xi) : 0.0;
+            ts.v[w][i] = vi;
+            if (w == 0) {
+                if (mine) W[i * LDA + k] = vi;
+                if (i == 0) {
+                    ts.d[k] = M[k * LDA + k];
+                    ts.e[k] = alpha;
+                    ts.beta[k] = beta;
+                }
+            }
+            // this wave's columns of the trailing block
+            const int m = n - k - 1, c = (m + TRI_WAVES - 1) / TRI_WAVES;
+            j0 = k + 1 + w * c;
+            j1 = min(n, j0 + c);
+            wave_sync();
+            double pa[4] = {0.0, 0.0, 0.0, 0.0};
+            if (mine && beta != 0.0) {
+                int j = j0;
+                for (; j + 7 < j1; j += 8) {
+                    double mr[8], vr[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        mr[u] = M[i * LDA + j + u];
+                        vr[u] = ts.v[w][j + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) pa[u & 3] += mr[u] * vr[u];
+                }
+                for (; j < j1; ++j) pa[0] += M[i * LDA + j] * ts.v[w][j];
+            }
+            ts.part[w][i] = (pa[0] + pa[1]) + (pa[2] + pa[3]);
+        }
+        __syncthreads();
+        if (w < TRI_WAVES && beta != 0.0) {
+            const double pi_ = beta * (((ts.part[0][i] + ts.part[1][i]) + ts.part[2][i]) + ts.part[3][i]);
+            double t = vi * pi_;
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+            const double wi = pi_ - 0.5 * beta * t * vi;
+            ts.w[w][i] = mine ? wi : 0.0;
+            wave_sync();
+            if (mine) {  // row i, this wave's columns: M22 -= v w^T + w v^T
+                int j = j0;
+                for (; j + 7 < j1; j += 8) {
+                    double mr[8], wr[8], vr[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        mr[u] = M[i * LDA + j + u];
+                        wr[u] = ts.w[w][j + u];
+                        vr[u] = ts.v[w][j + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) M[i * LDA + j + u] = mr[u] - (vi * wr[u] + wi * vr[u]);
+                }
+                for (; j < j1; ++j) M[i * LDA + j] -= vi * ts.w[w][j] + wi * ts.v[w][j];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        ts.d[n - 2] = M[(n - 2) * LDA + n - 2];
+        ts.e[n - 2] = M[(n - 1) * LDA + n - 2];
+        ts.d[n - 1] = M[(n - 1) * LDA + n - 1];
+        ts.e[n - 1] = 0.0;
+        ts.beta[n - 2] = 0.0;
+        ts.beta[n - 1] = 0.0;
+        // unreduced blocks: off-diagonal entries below eps (|d_i| + |d_i+1|) are dropped
+        double tn = 0.0;
+        for (int r = 0; r + 1 < n; ++r)
+            if (fabs(ts.e[r]) <= 2.220446049250313e-16 * (fabs(ts.d[r]) + fabs(ts.d[r + 1]))) ts.e[r] = 0.0;
+        for (int r = 0; r < n; ++r)
+            tn = fmax(tn, fabs(ts.d[r]) + (r > 0 ? fabs(ts.e[r - 1]) : 0.0) + fabs(ts.e[r]));
+        // the Sturm counts run on T / 2^ceil(log2 tnorm)
+        const double sc = tn > 0.0 ? ldexp(1.0, -frexp_e(tn)) : 1.0;
+        double emax = 0.0;
+        for (int r = 0; r < n; ++r) {
+            ts.e2[r] = (ts.e[r] * sc) * (ts.e[r] * sc);
+            emax = fmax(emax, ts.e2[r]);
+        }
+        ts.scale = sc;
+        ts.tnorm = tn;
+        ts.pivmin = 2.2250738585072014e-308 * fmax(1.0, emax) * 4.0;
+        int s0 = 0;
+        for (int r = 0; r < n; ++r) {
+            if (r + 1 == n || ts.e[r] == 0.0) {
+                for (int c = s0; c <= r; ++c) {
+                    ts.blo[c] = s0;
+                    ts.bhi[c] = r;
+                }
+                s0 = r + 1;
+            }
+        }
+    }
+    __syncthreads();
+    // reflectors below the diagonal of M (column k, rows k+1 .. n-1)
+    for (int idx = tid; idx < 4096; idx += NT) {
+        const int r = idx >> 6, c = idx & 63;
+        if (r > c && r < n && c + 2 < n) M[r * LDA + c] = W[r * LDA + c];
+    }
+    __syncthreads();
+    // ---- eigenvalues: eigenvalue j - blo of block [blo, bhi] for row j, 16 lanes each ----
+    {
+        const int j = tid >> 4, q = tid & 15;
+        double lo = 0.0, hi = 0.0;
+        const double sc = ts.scale;
+        if (j < n) {
+            const int s = ts.blo[j], e = ts.bhi[j], want = j - s;
+            const double pivmin = ts.pivmin;
+            lo = 1e300;
+            hi = -1e300;
+            for (int r = s; r <= e; ++r) {  // Gershgorin interval of the (scaled) block
+                const double rad = ((r > s ? fabs(ts.e[r - 1]) : 0.0) + (r < e ? fabs(ts.e[r]) : 0.0)) * sc;
+                lo = fmin(lo, ts.d[r] * sc - rad);
+                hi = fmax(hi, ts.d[r] * sc + rad);
+            }
+            const double pad = 2.220446049250313e-16 * 4.0 * ts.tnorm * sc + 2.0 * pivmin;
+            lo -= pad;
+            hi += pad;
+            for (int it = 0; it < 24; ++it) {
+                if (hi - lo <= 2.220446049250313e-16 * 2.0 * fmax(fabs(lo), fabs(hi)) ||
+                    hi - lo <= 2.220446049250313e-16 * 0.125 * ts.tnorm * sc)
+                    break;
+                const double x = lo + (hi - lo) * (double)(q + 1) / 17.0;
+                // eigenvalues of the block below x: sign changes of the Sturm
+                // sequence (a zero counts as a change, as a ratio -pivmin would)
+                double pm = 1.0, pc = ts.d[s] * sc - x;
+                if (pc == 0.0) pc = -pivmin;
+                int cnt = pc < 0.0;
+                int r = s + 1;
+                for (; r + 7 <= e; r += 8) {
+                    double dr[8], er[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        dr[u] = ts.d[r + u];
+                        er[u] = ts.e2[r + u - 1];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        double pn = fma(dr[u] * sc - x, pc, -er[u] * pm);
+                        if (pn == 0.0) pn = -0x1p-1000 * pc;
+                        cnt += (pn < 0.0) != (pc < 0.0);
+                        pm = pc;
+                        pc = pn;
+                    }
+                    const int ex = frexp_e(pc);
+                    if (ex > 400 || ex < -400) {
+                        pm = ldexp(pm, -ex);
+                        pc = ldexp(pc, -ex);
+                    }
+                }
+                for (; r <= e; ++r) {
+                    double pn = fma(ts.d[r] * sc - x, pc, -ts.e2[r - 1] * pm);
+                    if (pn == 0.0) pn = -0x1p-1000 * pc;
+                    cnt += (pn < 0.0) != (pc < 0.0);
+                    pm = pc;
+                    pc = pn;
+                }
+                const unsigned long long bal = __ballot(cnt > want);
+                const unsigned bits = (unsigned)((bal >> (16 * ((tid & 63) >> 4))) & 0xFFFFull);
+                const int f = bits ? __ffs(bits) - 1 : 16;  // first point with more than `want` below
+                const double nlo = f > 0 ? lo + (hi - lo) * (double)f / 17.0 : lo;
+                const double nhi = f < 16 ? lo + (hi - lo) * (double)(f + 1) / 17.0 : hi;
+                lo = nlo;
+                hi = nhi;
+            }
+        }
+        if (q == 0 && j < 64) lam[j] = j < n ? 0.5 * (lo + hi) / sc : 0.0;
+    }
+    __syncthreads();
+    // ---- eigenvectors: twisted factorisation of (T - lam I) / 2^.. on the block, one lane each ----
+    if (tid < 64) {
+        const int k = tid;
+        if (k < n) {
+            const int s = ts.blo[k], e = ts.bhi[k];
+            const double sc = ts.scale, l = lam[k] * sc, pivmin = ts.pivmin;
+            double dp = ts.d[s] * sc - l;  // L D+ L^T from the top (in W)
+            if (fabs(dp) < pivmin) dp = -pivmin;
+            W[s * LDA + k] = dp;
+            for (int r = s + 1; r <= e; ++r) {
+                dp = (ts.d[r] * sc - l) - ts.e2[r - 1] * rcp_nr(dp);
+                if (fabs(dp) < pivmin) dp = -pivmin;
+                W[r * LDA + k] = dp;
+            }
+            double dm = ts.d[e] * sc - l;  // U D- U^T from the bottom (in Z); twist at the smallest |gamma|
+            if (fabs(dm) < pivmin) dm = -pivmin;
+            Z[e * LDA + k] = dm;
+            int tw = e;
+            double gbest = fabs(W[e * LDA + k] + dm - (ts.d[e] * sc - l));
+            for (int r = e - 1; r >= s; --r) {
+                dm = (ts.d[r] * sc - l) - ts.e2[r] * rcp_nr(dm);
+                if (fabs(dm) < pivmin) dm = -pivmin;
+                Z[r * LDA + k] = dm;
+                const double g = fabs(W[r * LDA + k] + dm - (ts.d[r] * sc - l));
+                if (g < gbest) {
+                    gbest = g;
+                    tw = r;
+                }
+            }
+            double nrm = 1.0, x = 1.0;
+            for (int r = tw; r < e; ++r) {  // below the twist: x_r+1 = -e_r x_r / D-_r+1
+                x = -(ts.e[r] * sc) * x * rcp_nr(Z[(r + 1) * LDA + k]);
+                Z[(r + 1) * LDA + k] = x;
+                nrm += x * x;
+            }
+            Z[tw * LDA + k] = 1.0;
+            x = 1.0;
+            for (int r = tw - 1; r >= s; --r) {  // above: x_r = -e_r x_r+1 / D+_r
+                x = -(ts.e[r] * sc) * x * rcp_nr(W[r * LDA + k]);
+                Z[r * LDA + k] = x;
+                nrm += x * x;
+            }
+            const double inv = 1.0 / sqrt(nrm);
+#pragma unroll 4
+            for (int r = 0; r < 64; ++r) Z[r * LDA + k] = (r >= s && r <= e) ? Z[r * LDA + k] * inv : 0.0;
+        } else {
+#pragma unroll 4
+            for (int r = 0; r < 64; ++r) Z[r * LDA + k] = r == k ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+    // ---- back-transformation Z <- H_0 ... H_{n-3} Z: column j by 16 lanes, rows q, q + 16, .. in registers ----
+    {
+        const int j = tid >> 4, q = tid & 15;
+        double z[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) z[c] = Z[(q + 16 * c) * LDA + j];
+        if (j < n) {
+#pragma unroll 2
+            for (int k = n - 3; k >= 0; --k) {
+                const double beta = ts.beta[k];  // 0: no reflection, the update vanishes
+                double vr[4], dot = 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int r = q + 16 * c;
+                    vr[c] = (r > k && r < n) ? M[r * LDA + k] : 0.0;
+                    dot += vr[c] * z[c];
+                }
+                for (int off = 8; off > 0; off >>= 1) dot += __shfl_xor(dot, off);
+                dot *= beta;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) z[c] -= vr[c] * dot;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 4; ++c) Z[(q + 16 * c) * LDA + j] = z[c];
+    }
+    __syncthreads();
+    // ---- two Loewdin steps: Z <- Z + Z (I - Z^T Z) / 2 ----
+    const int lr = tid & 15, lg = (tid & 63) >> 4, rb = w & 3, cb = w >> 2;
+    for (int step = 0; step < 2; ++step) {
+        f64x4 aR = splat<f64x4>(0.0);
+#pragma unroll 2
+        for (int kk = 0; kk < 16; ++kk) {
+            const int i = 16 * rb + lr, k = 4 * kk + lg, j = 16 * cb + lr;
+            aR = MF<double>::mma(Z[k * LDA + i], Z[k * LDA + j], aR);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * rb + MF<double>::crow(lg, r), j = 16 * cb + lr;
+            M[i * LDA + j] = (i == j ? 1.0 : 0.0) - aR[r];
+        }
+        __syncthreads();
+        f64x4 aX = splat<f64x4>(0.0);
+#pragma unroll 2
+        for (int kk = 0; kk < 16; ++kk) {
+            const int i = 16 * rb + lr, k = 4 * kk + lg, j = 16 * cb + lr;
+            aX = MF<double>::mma(Z[i * LDA + k], M[k * LDA + j], aX);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Z[(16 * rb + MF<double>::crow(lg, r)) * LDA + 16 * cb + lr] += 0.5 * aX[r];
+        __syncthreads();
+    }
+}
+
 // Batched evaluations: matrix m belongs to gene mat_gene[m], whose
 // frequencies and scalar parameters (alpha, theta, delta, psi) it uses.
 template <typename Args>
@@ -206,6 +547,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
     __shared__ double red[4 * NT / 64];  // block_sum; two alternating buffers of block_sum2_once
     __shared__ int any_rot;
     __shared__ double thl[8], posl[12];
+    __shared__ TriScratch tri;
 
     if (a.params_host)
         for (int i = m * NT + tid; i < a.n_params; i += gridDim.x * NT) a.params_dev[i] = a.params_host[i];
@@ -285,11 +627,19 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
     bool done = false;
     double *Zg = a.Z + (size_t)o * 4096;
     const int zs = a.zsrc ? a.zsrc[m] : o;  // warm-start source slot (negative: cold)
-    if (a.warm && zs >= 0) {
-        // ---- Ogita-Aishima refinement of the previous eigenvectors ----
-        const double *Zs = a.Z + (size_t)zs * 4096;
-        for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = Zs[idx];
+    // Starting eigenvectors: the previous ones (warm start), else those of a
+    // tridiagonal solve; either is refined by Ogita-Aishima steps and, when
+    // that does not converge, the cold Jacobi solve below takes over.
+    for (int att = (a.warm && zs >= 0) ? 0 : 1; att < 2 && !done; ++att) {
+        if (att == 0) {
+            const double *Zs = a.Z + (size_t)zs * 4096;
+            for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = Zs[idx];
+        } else {
+            if (S < 3) break;
+            tridiag_eigen<NT>(A, Z, T1, T2, lam, S, tid, tri);
+        }
         __syncthreads();
+        // ---- Ogita-Aishima refinement ----
         double prev_en = 1e300;
         // Wave w owns the 16x16 block (rows 16 rb.., columns 16 cb..) of
         // every product; S = X^T A X and X E stay in registers.
@@ -401,6 +751,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
             }
         }
         if (done) {
+            if (att == 1) status -= 50;  // 50 + k: tridiagonal start refined k steps
             if (tid < 64) {
                 const double l = lam[tid];
                 lam[tid] = l > 0.0 ? 0.0 : l;
@@ -409,7 +760,7 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
         }
     }
     if (!done) {
-        // ---- cold: parallel cyclic Jacobi ----
+        // ---- last resort: parallel cyclic Jacobi ----
         for (int idx = tid; idx < 4096; idx += NT) Z[(idx >> 6) * LDA + (idx & 63)] = ((idx >> 6) == (idx & 63)) ? 1.0 : 0.0;
         const double thresh = 1e-17 * anorm;
         int sweep = 0;

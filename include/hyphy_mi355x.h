@@ -400,10 +400,13 @@ int hy_last_parts(hy_engine h, int *n_parts);
  * MG94Matrix::update (rate_matrix.hpp:236-248). */
 int hy_set_warm_start(hy_engine h, int enable);
 
-/* Eigensolver status of the last hy_eval per rate matrix: Jacobi sweeps for a
- * cold solve, 100 + k after k refinement steps of a warm start (100: the
- * previous eigenvectors already diagonalised the matrix to 1e-12 and were
- * kept unchanged). */
+/* Eigensolver status of the last hy_eval per rate matrix: 100 + k after k
+ * Ogita-Aishima refinement steps of a warm start (100: the previous
+ * eigenvectors already diagonalised the matrix to 1e-12 and were kept
+ * unchanged); 50 + k after k such steps from a cold tridiagonal solve
+ * (Householder tridiagonalisation, Sturm multisection, twisted
+ * factorisations); 1 .. 40 Jacobi sweeps when neither converged; negative:
+ * no convergence. */
 int hy_last_eigen_status(hy_engine h, int *status, int n);
 
 /* Timing of the last hy_eval's device work, in milliseconds, per stage:
