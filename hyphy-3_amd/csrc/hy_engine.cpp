@@ -1511,7 +1511,6 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
         return fail(h, HY_ERR_ARG, "hy_set_problem: bad sizes (patterns=%d taxa=%d S=%d)", num_patterns, num_taxa, S);
     if (num_patterns > 0 && (!states || !masks || !weights))
         return fail(h, HY_ERR_ARG, "hy_set_problem: NULL pattern arrays");
-    if (h->have_model && S != h->S) return fail(h, HY_ERR_ARG, "hy_set_problem: S=%d differs from the model's %d", S, h->S);
     // --- topology checks ---
     int root = -1;
     for (int n = 0; n < num_nodes; ++n) {
@@ -1663,6 +1662,10 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
         if (!(weights[p] >= 0.0)) return fail(h, HY_ERR_ARG, "hy_set_problem: weight[%d] negative or NaN", p);
     if (amb.empty()) amb.push_back(0);
     // --- commit ---
+    if (h->have_model && S != h->S) {  // a model for another state count no longer applies
+        h->have_model = false;
+        h->sink_set = false;
+    }
     h->num_nodes = num_nodes;
     h->n_ext = n_ext;
     h->root = root;

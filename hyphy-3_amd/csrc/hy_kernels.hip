@@ -183,6 +183,39 @@ __device__ __forceinline__ double rcp_nr(double q) {
     return fma(r, fma(-q, r, 1.0), r);
 }
 
+// DPP lane exchange of a double (both halves with the same control word).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each 16-lane row, in every lane of the row (quad swaps, then the
+// half-row and row mirrors); every lane of the row must be active.
+__device__ __forceinline__ double row_sum16(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return v;
+}
+
+// Value of lane l (uniform) of a double.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wave, in every lane (fixed order); all lanes active.
+__device__ __forceinline__ double wave_sum64(double v) {
+    v = row_sum16(v);
+    return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
+}
+
 // Makes LDS stores of a wave visible to its other lanes.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -190,7 +223,8 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Scratch of the tridiagonal eigensolver (LDS, one per workgroup).
-constexpr int TRI_WAVES = 4;  // waves of the tridiagonalisation
+constexpr int TRI_WAVES = 8;  // waves of the tridiagonalisation
+constexpr int TRI_BIS = 8;    // lanes (points) per eigenvalue in the multisection
 struct TriScratch {
     double d[64], e[64], e2[64];     // diagonal, off-diagonal, squares (split entries zeroed)
     double v[TRI_WAVES][64];         // reflector of the current step (one copy per wave)
@@ -203,13 +237,13 @@ struct TriScratch {
 
 // Cold symmetric eigensolver for the n x n leading block of A (n >= 3, zero
 // outside it).
-//  * Householder tridiagonalisation of a copy in M by four waves: lane i of
-//    every wave owns row i, wave w a quarter of the trailing columns; each
+//  * Householder tridiagonalisation of a copy in M by TRI_WAVES waves: lane i
+//    of every wave owns row i, wave w a share of the trailing columns; each
 //    wave forms the reflector itself (same arithmetic, same result), so a
 //    step needs two workgroup barriers.  Reflector k is kept in column k of
 //    W, then moved below the diagonal of M.
-//  * Eigenvalues by 17-way multisection on Sturm counts of each unreduced
-//    block (16 lanes per eigenvalue), on T scaled by a power of two to norm
+//  * Eigenvalues by (TRI_BIS + 1)-way multisection on Sturm counts of each
+//    unreduced block (TRI_BIS lanes per eigenvalue), on T scaled by a power of two to norm
 //    about one: the Sturm sequence p_r = (d_r - x) p_r-1 - e_r-1^2 p_r-2 needs
 //    no division, and a power-of-two rescale every eight steps keeps it in
 //    range.
@@ -235,9 +269,8 @@ __device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double 
         int j0 = 0, j1 = 0;
         if (w < TRI_WAVES) {
             const double xi = mine ? M[i * LDA + k] : 0.0;
-            double s = xi * xi;
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-            const double x0 = __shfl(xi, k + 1);
+            const double s = wave_sum64(xi * xi);
+            const double x0 = readlane_f64(xi, k + 1);
             const double alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
             const double v0 = x0 - alpha;
             const double vn2 = s - x0 * x0 + v0 * v0;
@@ -276,9 +309,11 @@ __device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double 
         }
         __syncthreads();
         if (w < TRI_WAVES && beta != 0.0) {
-            const double pi_ = beta * (((ts.part[0][i] + ts.part[1][i]) + ts.part[2][i]) + ts.part[3][i]);
-            double t = vi * pi_;
-            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+            double psum = 0.0;
+#pragma unroll
+            for (int u = 0; u < TRI_WAVES; ++u) psum += ts.part[u][i];
+            const double pi_ = beta * psum;
+            const double t = wave_sum64(vi * pi_);
             const double wi = pi_ - 0.5 * beta * t * vi;
             ts.w[w][i] = mine ? wi : 0.0;
             wave_sync();
@@ -341,9 +376,10 @@ __device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double 
         if (r > c && r < n && c + 2 < n) M[r * LDA + c] = W[r * LDA + c];
     }
     __syncthreads();
-    // ---- eigenvalues: eigenvalue j - blo of block [blo, bhi] for row j, 16 lanes each ----
+    // ---- eigenvalues: eigenvalue j - blo of block [blo, bhi] for row j, TRI_BIS lanes each ----
     {
-        const int j = tid >> 4, q = tid & 15;
+        constexpr int L = TRI_BIS;
+        const int j = tid / L, q = tid % L;
         double lo = 0.0, hi = 0.0;
         const double sc = ts.scale;
         if (j < n) {
@@ -359,15 +395,17 @@ __device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double 
             const double pad = 2.220446049250313e-16 * 4.0 * ts.tnorm * sc + 2.0 * pivmin;
             lo -= pad;
             hi += pad;
-            for (int it = 0; it < 24; ++it) {
+            for (int it = 0; it < 80; ++it) {
+                // to full precision: a refinement step would only add
+                // rounding noise divided by the eigenvalue gaps
                 if (hi - lo <= 2.220446049250313e-16 * 2.0 * fmax(fabs(lo), fabs(hi)) ||
                     hi - lo <= 2.220446049250313e-16 * 0.125 * ts.tnorm * sc)
                     break;
-                const double x = lo + (hi - lo) * (double)(q + 1) / 17.0;
+                const double x = lo + (hi - lo) * (double)(q + 1) / (double)(L + 1);
                 // eigenvalues of the block below x: sign changes of the Sturm
-                // sequence (a zero counts as a change, as a ratio -pivmin would)
+                // sequence (a zero p_r and the next term, of opposite sign to
+                // p_r-1, count one change together, as a perturbed zero would)
                 double pm = 1.0, pc = ts.d[s] * sc - x;
-                if (pc == 0.0) pc = -pivmin;
                 int cnt = pc < 0.0;
                 int r = s + 1;
                 for (; r + 7 <= e; r += 8) {
@@ -379,8 +417,7 @@ __device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double 
                     }
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        double pn = fma(dr[u] * sc - x, pc, -er[u] * pm);
-                        if (pn == 0.0) pn = -0x1p-1000 * pc;
+                        const double pn = fma(dr[u] * sc - x, pc, -er[u] * pm);
                         cnt += (pn < 0.0) != (pc < 0.0);
                         pm = pc;
                         pc = pn;
@@ -392,17 +429,16 @@ __device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double 
                     }
                 }
                 for (; r <= e; ++r) {
-                    double pn = fma(ts.d[r] * sc - x, pc, -ts.e2[r - 1] * pm);
-                    if (pn == 0.0) pn = -0x1p-1000 * pc;
+                    const double pn = fma(ts.d[r] * sc - x, pc, -ts.e2[r - 1] * pm);
                     cnt += (pn < 0.0) != (pc < 0.0);
                     pm = pc;
                     pc = pn;
                 }
                 const unsigned long long bal = __ballot(cnt > want);
-                const unsigned bits = (unsigned)((bal >> (16 * ((tid & 63) >> 4))) & 0xFFFFull);
-                const int f = bits ? __ffs(bits) - 1 : 16;  // first point with more than `want` below
-                const double nlo = f > 0 ? lo + (hi - lo) * (double)f / 17.0 : lo;
-                const double nhi = f < 16 ? lo + (hi - lo) * (double)(f + 1) / 17.0 : hi;
+                const unsigned bits = (unsigned)((bal >> (L * ((tid & 63) / L))) & ((1ull << L) - 1ull));
+                const int f = bits ? __ffs(bits) - 1 : L;  // first point with more than `want` below
+                const double nlo = f > 0 ? lo + (hi - lo) * (double)f / (double)(L + 1) : lo;
+                const double nhi = f < L ? lo + (hi - lo) * (double)(f + 1) / (double)(L + 1) : hi;
                 lo = nlo;
                 hi = nhi;
             }
@@ -410,30 +446,44 @@ __device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double 
         if (q == 0 && j < 64) lam[j] = j < n ? 0.5 * (lo + hi) / sc : 0.0;
     }
     __syncthreads();
-    // ---- eigenvectors: twisted factorisation of (T - lam I) / 2^.. on the block, one lane each ----
+    // ---- eigenvectors: twisted factorisation of (T - lam I) / 2^.. on the block, one lane each;
+    //      L D+ L^T from the top by wave 0 (in W), U D- U^T from the bottom by wave 1 (in Z) ----
+    if (tid < 128) {
+        const int k = lane;
+        if (k < n) {
+            const int s = ts.blo[k], e = ts.bhi[k];
+            const double sc = ts.scale, l = lam[k] * sc, pivmin = ts.pivmin;
+            if (w == 0) {
+                double dp = ts.d[s] * sc - l;
+                if (fabs(dp) < pivmin) dp = -pivmin;
+                W[s * LDA + k] = dp;
+                for (int r = s + 1; r <= e; ++r) {
+                    dp = (ts.d[r] * sc - l) - ts.e2[r - 1] * rcp_nr(dp);
+                    if (fabs(dp) < pivmin) dp = -pivmin;
+                    W[r * LDA + k] = dp;
+                }
+            } else {
+                double dm = ts.d[e] * sc - l;
+                if (fabs(dm) < pivmin) dm = -pivmin;
+                Z[e * LDA + k] = dm;
+                for (int r = e - 1; r >= s; --r) {
+                    dm = (ts.d[r] * sc - l) - ts.e2[r] * rcp_nr(dm);
+                    if (fabs(dm) < pivmin) dm = -pivmin;
+                    Z[r * LDA + k] = dm;
+                }
+            }
+        }
+    }
+    __syncthreads();
     if (tid < 64) {
         const int k = tid;
         if (k < n) {
             const int s = ts.blo[k], e = ts.bhi[k];
-            const double sc = ts.scale, l = lam[k] * sc, pivmin = ts.pivmin;
-            double dp = ts.d[s] * sc - l;  // L D+ L^T from the top (in W)
-            if (fabs(dp) < pivmin) dp = -pivmin;
-            W[s * LDA + k] = dp;
-            for (int r = s + 1; r <= e; ++r) {
-                dp = (ts.d[r] * sc - l) - ts.e2[r - 1] * rcp_nr(dp);
-                if (fabs(dp) < pivmin) dp = -pivmin;
-                W[r * LDA + k] = dp;
-            }
-            double dm = ts.d[e] * sc - l;  // U D- U^T from the bottom (in Z); twist at the smallest |gamma|
-            if (fabs(dm) < pivmin) dm = -pivmin;
-            Z[e * LDA + k] = dm;
-            int tw = e;
-            double gbest = fabs(W[e * LDA + k] + dm - (ts.d[e] * sc - l));
-            for (int r = e - 1; r >= s; --r) {
-                dm = (ts.d[r] * sc - l) - ts.e2[r] * rcp_nr(dm);
-                if (fabs(dm) < pivmin) dm = -pivmin;
-                Z[r * LDA + k] = dm;
-                const double g = fabs(W[r * LDA + k] + dm - (ts.d[r] * sc - l));
+            const double sc = ts.scale, l = lam[k] * sc;
+            int tw = e;  // twist at the smallest |gamma_r| = |D+_r + D-_r - (d_r - l)|
+            double gbest = 1e308;
+            for (int r = e; r >= s; --r) {
+                const double g = fabs(W[r * LDA + k] + Z[r * LDA + k] - (ts.d[r] * sc - l));
                 if (g < gbest) {
                     gbest = g;
                     tw = r;
@@ -478,8 +528,7 @@ __device__ __attribute__((noinline)) void tridiag_eigen(const double *A, double 
                     vr[c] = (r > k && r < n) ? M[r * LDA + k] : 0.0;
                     dot += vr[c] * z[c];
                 }
-                for (int off = 8; off > 0; off >>= 1) dot += __shfl_xor(dot, off);
-                dot *= beta;
+                dot = row_sum16(dot) * beta;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) z[c] -= vr[c] * dot;
             }
@@ -685,8 +734,8 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
             const double soff = sqrt(so);
             // Eigenvectors that already diagonalise A to 1e-12 (and are
             // orthonormal to 1e-12) are kept as they are: a step from there
-            // only adds rounding noise to close pairs, which repeated
-            // evaluations at the same parameters would accumulate.
+            // only adds rounding noise divided by the eigenvalue gaps (and
+            // repeated evaluations at the same parameters would accumulate it).
             if (it == 0 && soff <= 1e-12 * anorm && sqrt(rq) <= 1e-12) {
                 done = true;
                 status = 100;

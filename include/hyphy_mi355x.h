@@ -100,8 +100,10 @@ const char *hy_last_error(hy_engine h);
  *   masks[num_patterns*num_taxa]   compatible-codon bit masks
  *                        (alignment.hpp:26-31, genetic_code.hpp:82-109)
  *   weights[num_patterns] pattern multiplicities
- *   S                    number of sense codons of the genetic code (<= 64)
- * Re-calling replaces the problem and frees the previous device buffers. */
+ *   S                    number of sense codons of the genetic code (<= 64;
+ *                        4 for a nucleotide view)
+ * Re-calling replaces the problem and frees the previous device buffers; a
+ * model set for another S is dropped (hy_set_model again before evaluating). */
 int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int32_t *child_ptr,
                    const int32_t *child_idx, const int32_t *leaf_taxon, int num_taxa, int num_patterns,
                    const int8_t *states, const uint64_t *masks, const double *weights, int S);
