@@ -14,8 +14,12 @@
 // P_mix, include/hyphy/analyses/busted.hpp:509-659, 1138-1429) with full
 // omega / weight / GTR gradients.
 //
-// All likelihood arithmetic runs on the GPU; one engine per host thread is
-// created lazily on device 0 and keeps the last problem resident.
+// All likelihood arithmetic runs on the GPU.  One engine per process is
+// created lazily on device 0 and keeps the last problem resident; entries
+// from several host threads (the reference's OpenMP tile loops) are
+// serialised on it, so a tile loop costs one device evaluation and one
+// read-back cache, not one per thread.  MG94Matrix spectra come from a
+// second process-wide engine that holds only the model.
 #pragma once
 
 #include <array>
@@ -24,6 +28,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -36,21 +41,93 @@ namespace hyphy::core {
 
 // Column-major dense matrices with the storage order of the reference's
 // Eigen::MatrixXd / MatrixXi workspace fields (element (i, j) at i + j * rows).
+// The subset of Eigen's dense interface that the reference's callers of
+// these entry points use (busted.hpp:555-620: Zero, +=, scalar products,
+// noalias, element access).
 template <typename T>
 struct DenseMatrix {
     std::vector<T> v;
     long r = 0, c = 0;
     DenseMatrix() = default;
     DenseMatrix(long rows, long cols, T fill = T(0)) : v(size_t(rows) * cols, fill), r(rows), c(cols) {}
+    static DenseMatrix Zero(long rows, long cols) { return DenseMatrix(rows, cols); }
+    static DenseMatrix Constant(long rows, long cols, T x) { return DenseMatrix(rows, cols, x); }
+    static DenseMatrix Identity(long n, long m = -1) {
+        DenseMatrix a(n, m < 0 ? n : m);
+        for (long i = 0; i < std::min(a.r, a.c); ++i) a(i, i) = T(1);
+        return a;
+    }
     long rows() const { return r; }
     long cols() const { return c; }
+    long size() const { return r * c; }
+    void resize(long rows, long cols) {
+        if (rows * cols != r * c) v.assign(size_t(rows) * cols, T(0));
+        r = rows;
+        c = cols;
+    }
+    DenseMatrix &setZero() {
+        std::fill(v.begin(), v.end(), T(0));
+        return *this;
+    }
+    DenseMatrix &setConstant(T x) {
+        std::fill(v.begin(), v.end(), x);
+        return *this;
+    }
+    T *data() { return v.data(); }
+    const T *data() const { return v.data(); }
     T &operator()(long i, long j) { return v[size_t(i) + size_t(j) * r]; }
     T operator()(long i, long j) const { return v[size_t(i) + size_t(j) * r]; }
     T *col(long j) { return v.data() + size_t(j) * r; }
     const T *col(long j) const { return v.data() + size_t(j) * r; }
+    DenseMatrix &noalias() { return *this; }
+    DenseMatrix &operator+=(const DenseMatrix &o) {
+        if (o.r != r || o.c != c) throw std::invalid_argument("DenseMatrix +=: size mismatch");
+        for (size_t k = 0; k < v.size(); ++k) v[k] += o.v[k];
+        return *this;
+    }
+    DenseMatrix &operator-=(const DenseMatrix &o) {
+        if (o.r != r || o.c != c) throw std::invalid_argument("DenseMatrix -=: size mismatch");
+        for (size_t k = 0; k < v.size(); ++k) v[k] -= o.v[k];
+        return *this;
+    }
+    DenseMatrix &operator*=(T x) {
+        for (auto &e : v) e *= x;
+        return *this;
+    }
+    friend DenseMatrix operator*(T x, DenseMatrix a) { return a *= x; }
+    friend DenseMatrix operator*(DenseMatrix a, T x) { return a *= x; }
+    friend DenseMatrix operator+(DenseMatrix a, const DenseMatrix &b) { return a += b; }
+    friend DenseMatrix operator-(DenseMatrix a, const DenseMatrix &b) { return a -= b; }
+};
+template <typename T>
+struct DenseVector {
+    std::vector<T> v;
+    DenseVector() = default;
+    explicit DenseVector(long n, T fill = T(0)) : v(size_t(n), fill) {}
+    DenseVector(std::initializer_list<T> l) : v(l) {}
+    static DenseVector Zero(long n) { return DenseVector(n); }
+    static DenseVector Constant(long n, T x) { return DenseVector(n, x); }
+    long size() const { return long(v.size()); }
+    long rows() const { return long(v.size()); }
+    void resize(long n) { v.resize(size_t(n)); }
+    DenseVector &setZero() {
+        std::fill(v.begin(), v.end(), T(0));
+        return *this;
+    }
+    DenseVector &setConstant(T x) {
+        std::fill(v.begin(), v.end(), x);
+        return *this;
+    }
+    T *data() { return v.data(); }
+    const T *data() const { return v.data(); }
+    T &operator()(long i) { return v[size_t(i)]; }
+    T operator()(long i) const { return v[size_t(i)]; }
+    T &operator[](long i) { return v[size_t(i)]; }
+    T operator[](long i) const { return v[size_t(i)]; }
 };
 using Matrix = DenseMatrix<Scalar>;
 using IntMatrix = DenseMatrix<int>;
+using Vector = DenseVector<Scalar>;
 
 struct MG94Parameters {
     Scalar alpha = 1.0;
@@ -65,29 +142,120 @@ struct MG94Parameters {
     std::array<double, 5> theta() const { return {theta_AC, theta_AT, theta_CG, theta_CT, theta_GT}; }
 };
 
-// Frequencies + parameters of one MG94 generator.  The host builds Q and mu
-// (rate_matrix.hpp:165-234); spectra and transition matrices live on the GPU.
+// Process-wide engine holding only an MG94 model, for MG94Matrix spectra.
+class SpectrumEngine {
+public:
+    static SpectrumEngine &get() {
+        static SpectrumEngine e;
+        return e;
+    }
+    // hy_mg94_spectrum on (code, pos, pi): eigenvalues increasing, U_pi and
+    // V_pi row-major S x S
+    void solve(int code_id, const std::vector<double> &pos12, const std::vector<double> &pi, double alpha, double beta,
+               const std::array<double, 5> &theta, double delta, double psi, std::vector<double> &lam,
+               std::vector<double> &U, std::vector<double> &V) {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (!e_ && hy_create(0, &e_) != HY_OK)
+            throw std::runtime_error(std::string("MI355X engine unavailable: ") + hy_last_error(nullptr));
+        if (code_id != code_ || pos12 != pos_ || pi != pi_) {
+            if (hy_set_model(e_, code_id, pos12.data(), pi.data()) != HY_OK)
+                throw std::runtime_error(std::string("MI355X engine: ") + hy_last_error(e_));
+            code_ = code_id;
+            pos_ = pos12;
+            pi_ = pi;
+        }
+        const size_t S = pi.size();
+        lam.assign(S, 0.0);
+        U.assign(S * S, 0.0);
+        V.assign(S * S, 0.0);
+        if (hy_mg94_spectrum(e_, alpha, beta, theta.data(), delta, psi, lam.data(), U.data(), V.data(), nullptr) != HY_OK)
+            throw std::runtime_error(std::string("MI355X engine: ") + hy_last_error(e_));
+    }
+    ~SpectrumEngine() {
+        if (e_) hy_destroy(e_);
+    }
+
+private:
+    std::mutex mu_;
+    hy_engine e_ = nullptr;
+    int code_ = -1;
+    std::vector<double> pos_, pi_;
+};
+
+// One MG94 generator (rate_matrix.hpp:120-258) with the reference's fields:
+// the host builds Q and scale_factor (rate_matrix.hpp:165-234); the
+// spectrum (eigenvalues clamped to <= 0 in increasing order, eigenvectors of
+// the symmetrised generator, U_pi, V_pi) is solved on the GPU, and
+// transition_matrix(t) = max(U_pi diag(e^{lambda t}) V_pi, 0) is formed on
+// the host.  update(..., false) skips the spectrum (the evaluation entries
+// build their own on the device).
 class MG94Matrix {
 public:
     MG94Parameters params;
-    std::vector<Scalar> Q;  // row-major S x S
-    std::vector<Scalar> pi;
+    Matrix Q;
+    Vector pi, sqrt_pi, inv_sqrt_pi;
+    Vector eigenvalues;
+    Matrix eigenvectors, inv_eigenvectors, U_pi, V_pi;
     std::array<std::array<Scalar, 4>, 3> pos{};
     std::shared_ptr<const GeneticCode> code;
     Scalar scale_factor = 1.0;
 
+    struct SubstitutionRates {
+        Scalar rate_single = 0.0;
+        Scalar rate_double = 0.0;
+        Scalar rate_triple = 0.0;
+        Scalar total_rate = 0.0;
+        Scalar frac_double = 0.0;
+        Scalar frac_triple = 0.0;
+    };
+
+    // Flux of single, double and triple nucleotide substitutions
+    // (compute_substitution_rates, rate_matrix.hpp:137-157).
+    SubstitutionRates compute_substitution_rates(const GeneticCode &gc) const {
+        SubstitutionRates sr;
+        const int S = gc.num_sense_codons;
+        for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j) {
+                if (i == j) continue;
+                const Scalar flux = pi(i) * Q(i, j);
+                const int nd = gc.diff_matrix[i][j].num_diffs;
+                if (nd == 1)
+                    sr.rate_single += flux;
+                else if (nd == 2)
+                    sr.rate_double += flux;
+                else if (nd == 3)
+                    sr.rate_triple += flux;
+            }
+        sr.total_rate = sr.rate_single + sr.rate_double + sr.rate_triple;
+        if (sr.total_rate > 1e-12) {
+            sr.frac_double = sr.rate_double / sr.total_rate;
+            sr.frac_triple = sr.rate_triple / sr.total_rate;
+        }
+        return sr;
+    }
+
     void update(const MG94Parameters &p, const std::array<std::array<Scalar, 4>, 3> &pos_nuc_freqs,
-                const std::vector<Scalar> &codon_freqs, const GeneticCode &gc = *GeneticCode::universal()) {
+                const std::vector<Scalar> &codon_freqs, const GeneticCode &gc = *GeneticCode::universal(),
+                bool with_spectrum = true) {
         params = p;
         pos = pos_nuc_freqs;
-        pi = codon_freqs;
         code = GeneticCode::from_id(gc.id);
         const int S = gc.num_sense_codons;
+        if (int(codon_freqs.size()) != S)
+            throw std::invalid_argument("MG94Matrix::update: codon frequencies must have one entry per sense codon");
+        pi = Vector(S);
+        sqrt_pi = Vector(S);
+        inv_sqrt_pi = Vector(S);
+        for (int i = 0; i < S; ++i) {
+            pi(i) = codon_freqs[i];
+            sqrt_pi(i) = std::sqrt(std::max(pi(i), 1e-12));
+            inv_sqrt_pi(i) = 1.0 / sqrt_pi(i);
+        }
         const double r[4][4] = {{0, p.theta_AC, 1, p.theta_AT},
                                 {p.theta_AC, 0, p.theta_CG, p.theta_CT},
                                 {1, p.theta_CG, 0, p.theta_GT},
                                 {p.theta_AT, p.theta_CT, p.theta_GT, 0}};
-        Q.assign(size_t(S) * S, 0.0);
+        Q = Matrix(S, S);
         for (int i = 0; i < S; ++i) {
             double row = 0.0;
             for (int j = 0; j < S; ++j) {
@@ -105,14 +273,48 @@ public:
                         (r[d.diff_from[0]][d.diff_to[0]] * r[d.diff_from[1]][d.diff_to[1]] * r[d.diff_from[2]][d.diff_to[2]]) *
                         (pos[d.diff_pos[0]][d.diff_to[0]] * pos[d.diff_pos[1]][d.diff_to[1]] * pos[d.diff_pos[2]][d.diff_to[2]]);
                 }
-                Q[size_t(i) * S + j] = q;
+                Q(i, j) = q;
                 row += q;
             }
-            Q[size_t(i) * S + i] = -row;
+            Q(i, i) = -row;
         }
         double mu = 0.0;
-        for (int i = 0; i < S; ++i) mu -= pi[i] * Q[size_t(i) * S + i];
+        for (int i = 0; i < S; ++i) mu -= pi(i) * Q(i, i);
         scale_factor = mu;
+        if (!with_spectrum) return;
+        std::vector<double> pos12(12), lam, U, V;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 4; ++b) pos12[4 * a + b] = pos[a][b];
+        SpectrumEngine::get().solve(gc.id, pos12, codon_freqs, p.alpha, p.beta, p.theta(), p.delta, p.psi, lam, U, V);
+        eigenvalues = Vector(S);
+        eigenvectors = Matrix(S, S);
+        inv_eigenvectors = Matrix(S, S);
+        U_pi = Matrix(S, S);
+        V_pi = Matrix(S, S);
+        for (int k = 0; k < S; ++k) eigenvalues(k) = lam[k];
+        for (int i = 0; i < S; ++i)
+            for (int k = 0; k < S; ++k) {
+                U_pi(i, k) = U[size_t(i) * S + k];
+                V_pi(k, i) = V[size_t(k) * S + i];
+                eigenvectors(i, k) = U_pi(i, k) * sqrt_pi(i);
+                inv_eigenvectors(k, i) = eigenvectors(i, k);
+            }
+    }
+
+    // transition_matrix (rate_matrix.hpp:251-258)
+    Matrix transition_matrix(Scalar branch_length) const {
+        const long S = eigenvalues.size();
+        if (S == 0) throw std::logic_error("MG94Matrix::transition_matrix: update() with the spectrum first");
+        std::vector<double> ex(S);
+        for (long k = 0; k < S; ++k) ex[k] = std::exp(eigenvalues(k) * branch_length);
+        Matrix P(S, S);
+        for (long j = 0; j < S; ++j)
+            for (long i = 0; i < S; ++i) {
+                double acc = 0.0;
+                for (long k = 0; k < S; ++k) acc += U_pi(i, k) * ex[k] * V_pi(k, j);
+                P(i, j) = acc > 0.0 ? acc : 0.0;
+            }
+        return P;
     }
 };
 
@@ -235,9 +437,10 @@ public:
     static Scalar compute_mg94_log_likelihood(const Tree &tree, const Alignment &aln, const MG94Matrix &mg,
                                               const std::vector<size_t> *leaf_to_taxon = nullptr,
                                               Scalar branch_scale = 1.0) {
+        std::lock_guard<std::recursive_mutex> lock_(engine_mutex());
         Handle &h = bind(tree, aln, leaf_to_taxon, mg.code ? mg.code->id : code_id(aln));
         ++h.epoch;  // this evaluation replaces the device partials
-        set_frame(h, aln, mg.pos, mg.pi);
+        set_frame(h, aln, mg.pos, mg.pi.v);
         std::vector<double> bl(tree.num_nodes(), 0.0);
         for (const auto &n : tree.nodes) bl[n.id] = n.branch_length * branch_scale;
         single_mixture(h, HY_TIME_RAW);
@@ -253,6 +456,7 @@ public:
     // nucleotide view (Alignment::to_nucleotide_alignment)
     static Scalar compute_gtr_log_likelihood(const Tree &tree, const Alignment &aln, const GTRMatrix &gtr,
                                              const std::vector<size_t> *leaf_to_taxon = nullptr) {
+        std::lock_guard<std::recursive_mutex> lock_(engine_mutex());
         if (aln.num_codons != 0)
             throw std::invalid_argument("compute_gtr_log_likelihood needs the nucleotide view of the alignment");
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
@@ -270,6 +474,7 @@ public:
                                                      const std::vector<std::vector<Scalar>> &P_branches,
                                                      const std::vector<Scalar> &pi, int S,
                                                      const std::vector<size_t> *leaf_to_taxon = nullptr) {
+        std::lock_guard<std::recursive_mutex> lock_(engine_mutex());
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
         ++h.epoch;  // this evaluation replaces the device partials
         std::vector<double> flat(tree.num_nodes() * size_t(S) * S, 0.0);
@@ -283,6 +488,7 @@ public:
     static std::pair<Scalar, std::vector<Scalar>> compute_branch_length_gradients(
         const Tree &tree, const Alignment &aln, const MG94Parameters &params,
         const std::vector<Scalar> &branch_lengths = {}, const std::vector<size_t> *leaf_to_taxon = nullptr) {
+        std::lock_guard<std::recursive_mutex> lock_(engine_mutex());
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
         ++h.epoch;  // this evaluation replaces the device partials
         set_frame(h, aln, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4);
@@ -303,6 +509,7 @@ public:
                                                               const std::vector<size_t> *leaf_to_taxon = nullptr,
                                                               const std::vector<bool> *is_test_branch = nullptr,
                                                               Scalar omega_bg = -1.0) {
+        std::lock_guard<std::recursive_mutex> lock_(engine_mutex());
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
         ++h.epoch;  // this evaluation replaces the device partials
         set_frame(h, aln, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4);
@@ -342,6 +549,7 @@ public:
                                                       const std::vector<Scalar> &branch_lengths = {},
                                                       int time_convention = HY_TIME_RAW, bool want_grad = true,
                                                       const std::vector<size_t> *leaf_to_taxon = nullptr) {
+        std::lock_guard<std::recursive_mutex> lock_(engine_mutex());
         if (omegas.empty() || omegas.size() != weights.size())
             throw std::invalid_argument("compute_mixture_gradients: omegas and weights must be non-empty and equal length");
         Handle &h = bind(tree, aln, leaf_to_taxon, code_id(aln));
@@ -380,15 +588,43 @@ public:
     // partial child product already fell below 1e-150 may carry its boosts
     // one level differently (same true values).
     struct BatchedInsideWorkspace {
-        std::vector<Matrix> D;
-        Matrix node_log_scale;  // num_nodes x B
-        std::vector<Scalar> root_L, root_log_L;
+        std::vector<Matrix> D;  // [num_nodes] each S x B_cap
+        Matrix M_tmp;           // S x B_cap scratch
+        Matrix node_log_scale;  // num_nodes x B_cap
+        Vector root_L;          // B_cap
+        Vector root_log_L;      // B_cap
+
+        void init(size_t num_nodes, int S, size_t B_cap) {
+            if (D.size() != num_nodes || (num_nodes > 0 && (D[0].rows() != S || size_t(D[0].cols()) < B_cap))) {
+                D.assign(num_nodes, Matrix::Zero(S, long(B_cap)));
+                M_tmp.resize(S, long(B_cap));
+                node_log_scale.resize(long(num_nodes), long(B_cap));
+                root_L.resize(long(B_cap));
+                root_log_L.resize(long(B_cap));
+            }
+        }
     };
     struct BatchedInsideOutsideWorkspace {
-        std::vector<Matrix> D, V, P_D, U;
-        IntMatrix boost_counts;  // num_nodes x B
-        Matrix node_log_scale;   // num_nodes x B
-        std::vector<Scalar> likelihood, log_likelihood;
+        std::vector<Matrix> D, V, P_D, U;  // [num_nodes] each S x B_cap
+        IntMatrix boost_counts;            // num_nodes x B_cap
+        Matrix node_log_scale;             // num_nodes x B_cap
+        std::vector<Matrix> pref, suff;    // scratch of the reference's CPU pass (unused here)
+        IntMatrix pref_boost, suff_boost;
+        Vector likelihood;                 // B_cap
+        Vector log_likelihood;             // B_cap
+
+        void init(size_t num_nodes, int S, size_t B_cap) {
+            if (D.size() != num_nodes || (num_nodes > 0 && (D[0].rows() != S || size_t(D[0].cols()) < B_cap))) {
+                D.assign(num_nodes, Matrix::Zero(S, long(B_cap)));
+                V.assign(num_nodes, Matrix::Zero(S, long(B_cap)));
+                P_D.assign(num_nodes, Matrix::Zero(S, long(B_cap)));
+                U.assign(num_nodes, Matrix::Zero(S, long(B_cap)));
+                boost_counts.resize(long(num_nodes), long(B_cap));
+                node_log_scale.resize(long(num_nodes), long(B_cap));
+                likelihood.resize(long(B_cap));
+                log_likelihood.resize(long(B_cap));
+            }
+        }
     };
 
     // compute_inside_batch_in_place (likelihood.hpp:343-476) for patterns
@@ -397,12 +633,23 @@ public:
                                               const std::vector<size_t> &leaf_to_taxon,
                                               const std::vector<std::vector<Scalar>> &P_branches,
                                               const std::vector<Scalar> &pi, int S, BatchedInsideWorkspace &ws) {
+        std::lock_guard<std::recursive_mutex> lock_(engine_mutex());
         Readback rb;
         read_back(tree, aln, p_start, p_end, leaf_to_taxon, P_branches, pi, S, false, rb);
-        ws.D = std::move(rb.D);
-        ws.node_log_scale = std::move(rb.log_scale);
-        ws.root_L = std::move(rb.L);
-        ws.root_log_L = std::move(rb.logL);
+        fill(ws.D, rb.D);
+        fill(ws.node_log_scale, rb.log_scale);
+        fill(ws.root_L, rb.L);
+        fill(ws.root_log_L, rb.logL);
+    }
+
+    // The reference's form: P_branches[node] an S x S Matrix, bridges unsupported.
+    static void compute_inside_batch_in_place(const Tree &tree, const Alignment &aln, size_t p_start, size_t p_end,
+                                              const std::vector<size_t> &leaf_to_taxon,
+                                              const std::vector<Matrix> &P_branches, const std::vector<Scalar> &pi,
+                                              int S, BatchedInsideWorkspace &ws,
+                                              const std::vector<std::array<Matrix, 3>> *P_bridges = nullptr) {
+        no_bridges(P_bridges);
+        compute_inside_batch_in_place(tree, aln, p_start, p_end, leaf_to_taxon, row_major(P_branches, S), pi, S, ws);
     }
 
     // compute_inside_outside_batch_in_place (likelihood.hpp:479-762).
@@ -411,19 +658,76 @@ public:
                                                       const std::vector<std::vector<Scalar>> &P_branches,
                                                       const std::vector<Scalar> &pi, int S,
                                                       BatchedInsideOutsideWorkspace &ws) {
+        std::lock_guard<std::recursive_mutex> lock_(engine_mutex());
         Readback rb;
         read_back(tree, aln, p_start, p_end, leaf_to_taxon, P_branches, pi, S, true, rb);
-        ws.D = std::move(rb.D);
-        ws.V = std::move(rb.V);
-        ws.P_D = std::move(rb.PD);
-        ws.U = std::move(rb.U);
-        ws.boost_counts = std::move(rb.boost);
-        ws.node_log_scale = std::move(rb.log_scale);
-        ws.likelihood = std::move(rb.L);
-        ws.log_likelihood = std::move(rb.logL);
+        fill(ws.D, rb.D);
+        fill(ws.V, rb.V);
+        fill(ws.P_D, rb.PD);
+        fill(ws.U, rb.U);
+        fill(ws.boost_counts, rb.boost);
+        fill(ws.node_log_scale, rb.log_scale);
+        fill(ws.likelihood, rb.L);
+        fill(ws.log_likelihood, rb.logL);
+    }
+
+    static void compute_inside_outside_batch_in_place(const Tree &tree, const Alignment &aln, size_t p_start,
+                                                      size_t p_end, const std::vector<size_t> &leaf_to_taxon,
+                                                      const std::vector<Matrix> &P_branches,
+                                                      const std::vector<Scalar> &pi, int S,
+                                                      BatchedInsideOutsideWorkspace &ws,
+                                                      const std::vector<std::array<Matrix, 3>> *P_bridges = nullptr) {
+        no_bridges(P_bridges);
+        compute_inside_outside_batch_in_place(tree, aln, p_start, p_end, leaf_to_taxon, row_major(P_branches, S), pi,
+                                              S, ws);
+    }
+
+    static Scalar compute_tree_log_likelihood_from_P(const Tree &tree, const Alignment &aln,
+                                                     const std::vector<Matrix> &P_branches, const Vector &pi, int S,
+                                                     const std::vector<size_t> *leaf_to_taxon = nullptr,
+                                                     const std::vector<std::array<Matrix, 3>> *P_bridges = nullptr) {
+        no_bridges(P_bridges);
+        return compute_tree_log_likelihood_from_P(tree, aln, row_major(P_branches, S), pi.v, S, leaf_to_taxon);
     }
 
 private:
+    static std::recursive_mutex &engine_mutex() {
+        static std::recursive_mutex m;
+        return m;
+    }
+    static void no_bridges(const std::vector<std::array<Matrix, 3>> *P_bridges) {
+        if (P_bridges && !P_bridges->empty())
+            throw std::invalid_argument("Wheatstone-bridged trees are not supported by the MI355X engine");
+    }
+    static std::vector<std::vector<Scalar>> row_major(const std::vector<Matrix> &P, int S) {
+        std::vector<std::vector<Scalar>> out(P.size());
+        for (size_t n = 0; n < P.size(); ++n) {
+            if (P[n].rows() != S || P[n].cols() != S) continue;  // the root's entry may be empty
+            out[n].resize(size_t(S) * S);
+            for (int i = 0; i < S; ++i)
+                for (int j = 0; j < S; ++j) out[n][size_t(i) * S + j] = P[n](i, j);
+        }
+        return out;
+    }
+    // Copies a read-back into a workspace field, into its first columns when
+    // init() gave it a larger capacity.
+    template <typename T>
+    static void fill(DenseMatrix<T> &dst, const DenseMatrix<T> &src) {
+        if (dst.rows() == src.rows() && dst.cols() >= src.cols()) {
+            for (long j = 0; j < src.cols(); ++j) std::copy(src.col(j), src.col(j) + src.rows(), dst.col(j));
+        } else {
+            dst = src;
+        }
+    }
+    static void fill(std::vector<Matrix> &dst, const std::vector<Matrix> &src) {
+        if (dst.size() != src.size()) dst.resize(src.size());
+        for (size_t n = 0; n < src.size(); ++n) fill(dst[n], src[n]);
+    }
+    static void fill(Vector &dst, const std::vector<Scalar> &src) {
+        if (dst.size() < long(src.size())) dst.resize(long(src.size()));
+        std::copy(src.begin(), src.end(), dst.v.begin());
+    }
+
     struct Readback {
         std::vector<Matrix> D, V, PD, U;
         IntMatrix boost;
@@ -604,7 +908,7 @@ private:
     }
 
     static Handle &bind(const Tree &tree, const Alignment &aln, const std::vector<size_t> *leaf_to_taxon, int code) {
-        thread_local Handle h;
+        static Handle h;  // one per process; callers hold engine_mutex()
         if (!h.e) {
             if (hy_create(0, &h.e) != HY_OK)
                 throw std::runtime_error(std::string("MI355X engine unavailable: ") + hy_last_error(nullptr));

@@ -179,6 +179,9 @@ struct hy_engine_s {
     // (eigenvectors kept for warm starts), per-call staging and scratch
     FelPool fel;
     DevBuf d_fZ, d_flam, d_fU, d_fV, d_fmu, d_fstatus, d_fstage, d_fbl, d_fD, d_fE, d_fout, d_fident;
+    // single spectra (hy_mg94_spectrum): last eigenvectors kept for a warm start
+    DevBuf d_sprm, d_slam, d_sU, d_sV, d_smu, d_sZ, d_sstatus;
+    uint64_t s_model_gen = ~uint64_t(0);
     void *f_pin = nullptr;  // pinned staging: inputs, then outputs
     size_t f_pin_bytes = 0;
     std::vector<double> f_bl;  // branch lengths last uploaded to d_fbl
@@ -1413,7 +1416,8 @@ void hy_destroy(hy_engine h) {
                       &h->d_gnbr, &h->d_gtips, &h->d_gw, &h->d_gP, &h->d_gpi, &h->d_gD, &h->d_gE, &h->d_gU,
                       &h->d_gH, &h->d_gL, &h->d_gLE, &h->d_gsite, &h->d_gApart, &h->d_gA, &h->d_gres,
                       &h->d_fZ, &h->d_flam, &h->d_fU, &h->d_fV, &h->d_fmu, &h->d_fstatus, &h->d_fstage, &h->d_fbl,
-                      &h->d_fD, &h->d_fE, &h->d_fout, &h->d_fident};
+                      &h->d_fD, &h->d_fE, &h->d_fout, &h->d_fident, &h->d_sprm, &h->d_slam, &h->d_sU, &h->d_sV,
+                      &h->d_smu, &h->d_sZ, &h->d_sstatus};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -3671,6 +3675,95 @@ int fel_run_eval(void *user, int n, const int32_t *pattern, const double *alpha,
     return hy_fel_eval(c->h, n, pattern, alpha, beta, c->theta, c->delta, c->psi, c->bl, out_ll, nullptr);
 }
 }  // namespace
+
+int hy_mg94_spectrum(hy_engine h, double alpha, double beta, const double *theta, double delta, double psi,
+                     double *lam, double *U, double *V, double *mu) {
+    // MG94Matrix::update's spectrum (rate_matrix.hpp:165-248) for the
+    // handle's model: eigenvalues (clamped to <= 0), U_pi = diag(1/sqrt(pi)) Z,
+    // V_pi = Z^T diag(sqrt(pi)), the mean rate
+    const char *who = "hy_mg94_spectrum";
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_model) return fail(h, HY_ERR_STATE, "%s before hy_set_model", who);
+    if (!theta || !lam || !U || !V) return fail(h, HY_ERR_ARG, "%s: NULL argument", who);
+    const double chk[4] = {alpha, beta, delta, psi};
+    for (double x : chk)
+        if (!(x >= 0.0 && std::isfinite(x))) return fail(h, HY_ERR_ARG, "%s: negative or non-finite rate %g", who, x);
+    for (int i = 0; i < 5; ++i)
+        if (!(theta[i] >= 0.0 && std::isfinite(theta[i])))
+            return fail(h, HY_ERR_ARG, "%s: theta[%d] = %g", who, i, theta[i]);
+    HIP_TRY(h, hipSetDevice(h->device));
+    ParamLayout pl{};
+    pl.n_mat = 1;
+    pl.n_comp = 1;
+    pl.off_alpha = 0;
+    pl.off_omega = 1;
+    pl.off_w = 2;
+    pl.off_theta = 3;
+    pl.off_delta = 8;
+    pl.off_psi = 9;
+    pl.total = 10;
+    const double prm[10] = {alpha, beta, 1.0, theta[0], theta[1], theta[2], theta[3], theta[4], delta, psi};
+    const bool warm = h->d_sZ.p && h->s_model_gen == h->model_gen;
+    TRY(upload(h, h->d_sprm, prm, sizeof prm));
+    TRY(ensure(h, h->d_slam, 64 * 8));
+    TRY(ensure(h, h->d_sU, 4096 * 8));
+    TRY(ensure(h, h->d_sV, 4096 * 8));
+    TRY(ensure(h, h->d_sZ, 4096 * 8));
+    TRY(ensure(h, h->d_smu, 8));
+    TRY(ensure(h, h->d_sstatus, 4));
+    ModelArgs ma{};
+    ma.qdesc = (const int32_t *)h->d_qdesc.p;
+    ma.nd = (const int8_t *)h->d_nd.p;
+    ma.dpos = (const int8_t *)h->d_dpos.p;
+    ma.dfrom = (const int8_t *)h->d_dfrom.p;
+    ma.dto = (const int8_t *)h->d_dto.p;
+    ma.syn = (const uint8_t *)h->d_syn.p;
+    ma.pos = (const double *)h->d_pos.p;
+    ma.pi = (const double *)h->d_pi.p;
+    ma.S = h->S;
+    ma.params = (const double *)h->d_sprm.p;
+    ma.pl = pl;
+    ma.lam = (double *)h->d_slam.p;
+    ma.U = (double *)h->d_sU.p;
+    ma.V = (double *)h->d_sV.p;
+    ma.mu = (double *)h->d_smu.p;
+    ma.Z = (double *)h->d_sZ.p;
+    ma.warm = warm ? 1 : 0;  // refines the previous call's eigenvectors (same model)
+    ma.status = (int *)h->d_sstatus.p;
+    ma.tol = 1e-14;
+    hipStream_t s = h->stream;
+    launch_model(ma, 1, s);
+    HIP_TRY(h, hipGetLastError());
+    std::vector<double> hl(64), hU(4096), hV(4096);
+    double hmu = 0.0;
+    int st = 0;
+    HIP_TRY(h, hipMemcpyAsync(hl.data(), h->d_slam.p, 64 * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipMemcpyAsync(hU.data(), h->d_sU.p, 4096 * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipMemcpyAsync(hV.data(), h->d_sV.p, 4096 * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipMemcpyAsync(&hmu, h->d_smu.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipMemcpyAsync(&st, h->d_sstatus.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipStreamSynchronize(s));
+    if (st < 0) {
+        h->s_model_gen = ~uint64_t(0);
+        return fail(h, HY_ERR_NUMERIC, "%s: eigensolver did not converge", who);
+    }
+    h->s_model_gen = h->model_gen;
+    const int S = h->S;
+    // eigenpairs in increasing eigenvalue order (the order of Eigen's solver)
+    std::vector<int> ord(S);
+    for (int k = 0; k < S; ++k) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return hl[x] < hl[y]; });
+    for (int c = 0; c < S; ++c) {
+        const int k = ord[c];
+        lam[c] = hl[k];
+        for (int i = 0; i < S; ++i) {
+            U[(size_t)i * S + c] = hU[(size_t)i * 64 + k];  // U_pi(i, c)
+            V[(size_t)c * S + i] = hV[(size_t)k * 64 + i];  // V_pi(c, i)
+        }
+    }
+    if (mu) *mu = hmu;
+    return HY_OK;
+}
 
 int hy_fel_counters(hy_engine h, int64_t *out4, int reset) {
     if (!h || !out4) return HY_ERR_ARG;

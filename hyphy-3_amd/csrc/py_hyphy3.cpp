@@ -234,12 +234,44 @@ PYBIND11_MODULE(_hyphy3, m) {
 
     m.def("compute_mg94_log_likelihood",
           [](const Tree &tree, const Alignment &aln, const MG94Parameters &params, double branch_scale) {
-              MG94Matrix mg;
-              mg.update(params, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4, *aln.code);
+              MG94Matrix mg;  // the evaluation builds its own spectra on the device
+              mg.update(params, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4, *aln.code, false);
               py::gil_scoped_release nogil;
               return LikelihoodEngine::compute_mg94_log_likelihood(tree, aln, mg, nullptr, branch_scale);
           },
           py::arg("tree"), py::arg("aln"), py::arg("params"), py::arg("branch_scale") = 1.0);
+
+    m.def("mg94_matrix",
+          [](const Alignment &aln, const MG94Parameters &params) {
+              MG94Matrix mg;
+              {
+                  py::gil_scoped_release nogil;
+                  mg.update(params, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4, *aln.code);
+              }
+              py::dict d;
+              d["Q"] = dense(mg.Q);
+              d["eigenvalues"] = mg.eigenvalues.v;
+              d["eigenvectors"] = dense(mg.eigenvectors);
+              d["U_pi"] = dense(mg.U_pi);
+              d["V_pi"] = dense(mg.V_pi);
+              d["scale_factor"] = mg.scale_factor;
+              return d;
+          },
+          py::arg("aln"), py::arg("params"),
+          "MG94Matrix::update on the alignment's frequencies: Q, spectrum (GPU), U_pi, V_pi, scale_factor");
+
+    m.def("mg94_transition_matrix",
+          [](const Alignment &aln, const MG94Parameters &params, double t) {
+              MG94Matrix mg;
+              Matrix P;
+              {
+                  py::gil_scoped_release nogil;
+                  mg.update(params, aln.pos_nuc_frequencies, aln.codon_frequencies_f3x4, *aln.code);
+                  P = mg.transition_matrix(t);
+              }
+              return dense(P);
+          },
+          py::arg("aln"), py::arg("params"), py::arg("t"), "MG94Matrix::transition_matrix(t)");
 
     m.def("compute_joint_frechet_gradients",
           [](const Tree &tree, const Alignment &aln, const MG94Parameters &params, const std::vector<double> &bl,
@@ -301,8 +333,8 @@ PYBIND11_MODULE(_hyphy3, m) {
               py::dict d;
               d["D"] = node_stack(ws.D);
               d["node_log_scale"] = dense(ws.node_log_scale);
-              d["root_L"] = ws.root_L;
-              d["root_log_L"] = ws.root_log_L;
+              d["root_L"] = ws.root_L.v;
+              d["root_log_L"] = ws.root_log_L.v;
               return d;
           },
           py::arg("tree"), py::arg("aln"), py::arg("p_start"), py::arg("p_end"), py::arg("P_branches"), py::arg("pi"),
@@ -326,8 +358,8 @@ PYBIND11_MODULE(_hyphy3, m) {
               d["U"] = node_stack(ws.U);
               d["boost_counts"] = dense(ws.boost_counts);
               d["node_log_scale"] = dense(ws.node_log_scale);
-              d["likelihood"] = ws.likelihood;
-              d["log_likelihood"] = ws.log_likelihood;
+              d["likelihood"] = ws.likelihood.v;
+              d["log_likelihood"] = ws.log_likelihood.v;
               return d;
           },
           py::arg("tree"), py::arg("aln"), py::arg("p_start"), py::arg("p_end"), py::arg("P_branches"), py::arg("pi"),

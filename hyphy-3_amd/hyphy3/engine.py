@@ -35,7 +35,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
            "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count",
            "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval", "hy_fel_drive", "hy_fel_run",
-           "hy_fel_counters")
+           "hy_fel_counters", "hy_mg94_spectrum")
 
 HY_FEL_FIELDS = 10
 FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt",
@@ -99,6 +99,8 @@ def load_library(path: str = LIB_PATH):
     L.hy_fel_run.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, _f64p,
                              C.POINTER(C.c_int), C.POINTER(C.c_int64), _f64p]
     L.hy_fel_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int]
+    L.hy_mg94_spectrum.argtypes = [C.c_void_p, C.c_double, C.c_double, _f64p, C.c_double, C.c_double, _f64p, _f64p,
+                                   _f64p, _f64p]
     L.hy_last_checkpoint.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
     L.hy_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.hy_set_sink.argtypes = [C.c_void_p, _f64p]
@@ -203,6 +205,8 @@ class Engine:
         pi = _f64(pi)
         self._check_states("set_model", pi)
         self._check(self.L.hy_set_model(self.h, code_id, _p(pos, _f64p), _p(pi, _f64p)))
+        if getattr(self, "S", None) is None:  # a model without a problem (spectra only)
+            self.S = int(pi.size)
 
     def set_mixture(self, n_mat, branch_class, classes, convention=HY_TIME_RAW):
         """classes: list (per class) of lists of matrix indices; HY_SINK names
@@ -410,6 +414,19 @@ class Engine:
         self._check(self.L.hy_fel_run(self.h, _p(th, _f64p), float(alpha), float(beta), float(delta), float(psi),
                                       _p(blc, _f64p), _p(out, _f64p), C.byref(calls), C.byref(cands), C.byref(qs)))
         return out[:self.num_patterns], {"calls": calls.value, "candidates": cands.value, "q_scale": qs.value}
+
+    def mg94_spectrum(self, alpha, beta, theta, delta=0.0, psi=0.0):
+        """Eigenvalues (increasing, clamped to <= 0), U_pi, V_pi and the mean
+        rate of one MG94 generator on the engine's model (hy_mg94_spectrum)."""
+        S = self.S
+        th = _f64(theta)
+        lam = np.zeros(S)
+        U = np.zeros((S, S))
+        V = np.zeros((S, S))
+        mu = C.c_double()
+        self._check(self.L.hy_mg94_spectrum(self.h, float(alpha), float(beta), _p(th, _f64p), float(delta), float(psi),
+                                            _p(lam, _f64p), _p(U, _f64p), _p(V, _f64p), C.byref(mu)))
+        return lam, U, V, mu.value
 
     def fel_counters(self, reset=False):
         """hy_fel_eval counters: candidates, pool hits, warm and cold solves."""

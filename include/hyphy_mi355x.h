@@ -261,6 +261,18 @@ int hy_gtr_eval(hy_engine h, int precision, unsigned flags, const double *bl, co
 int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *beta,
                 const double *theta, double delta, double psi, const double *bl, double *out_ll, double *out_scale);
 
+/* The spectrum of one MG94 generator on the handle's model (hy_set_model's
+ * code and frequencies; no problem needed): MG94Matrix::update
+ * (rate_matrix.hpp:165-248) with rates alpha, beta, theta[5] (AC, AT, CG,
+ * CT, GT), delta, psi.  Outputs, S = the code's sense codons: lam[S] the
+ * eigenvalues of the symmetrised generator clamped to <= 0 in increasing
+ * order, U[S*S] = U_pi = diag(1/sqrt(pi)) Z and V[S*S] = V_pi =
+ * Z^T diag(sqrt(pi)) row-major (P(t) = U_pi diag(e^{lam t}) V_pi), mu (may be
+ * NULL) the mean rate sum_i pi_i (-Q_ii).  Solved on the GPU; a call with
+ * the same model refines the previous call's eigenvectors. */
+int hy_mg94_spectrum(hy_engine h, double alpha, double beta, const double *theta, double delta, double psi,
+                     double *lam, double *U, double *V, double *mu);
+
 /* Counters of hy_fel_eval since the handle was made (or the last reset):
  * out4 = {candidates, candidates whose omega was already pooled, spectra
  * solved by warm refinement, spectra solved cold}.  No reference
