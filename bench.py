@@ -205,6 +205,10 @@ def main():
         print(json.dumps({"error": "this build measures one MI355X; multi-GPU runs are not part of it "
                                    "(the path would shard patterns across GPUs)"}))
         sys.exit(2)
+    # the timed regions are bracketed by torch.cuda.synchronize(): initialise
+    # torch's device context now, so that its start-up work (threads, device
+    # queries) is over before anything is timed
+    sync()
     if args.config == "c3":
         bench_genes(args)
         return

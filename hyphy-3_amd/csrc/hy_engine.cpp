@@ -214,6 +214,12 @@ struct hy_engine_s {
     double *h_res = nullptr;
     size_t h_res_cap = 0;
     int *h_status = nullptr;
+    // completion word: k_final stores the call number of a single evaluation
+    // here after its results (h_res) are visible to the host
+    unsigned long long *h_done = nullptr, *h_done_dev = nullptr;
+    uint64_t seq = 0;
+    bool publish_done = false;  // run_eval: k_final publishes the call number
+    bool wait_done = false;     // the pending evaluation publishes h_done
     // device addresses of the pinned h_params / h_res / h_status: k_model reads
     // the parameters from h_params under graph capture (zc_params) and writes
     // the eigensolver status, k_final writes the results, with no copies
@@ -392,6 +398,7 @@ void build_layout(hy_engine h) {
     o += 5;
     pl.off_delta = o++;
     pl.off_psi = o++;
+    pl.off_seq = o++;
     pl.total = o;
 }
 
@@ -476,11 +483,9 @@ int prepare_buffers(hy_engine h, int prec) {
         std::vector<double> w(h->P_pad, 0.0);
         std::copy(h->weights.begin(), h->weights.end(), w.begin());
         TRY(upload(h, h->d_weights, w.data(), w.size() * sizeof(double)));
-        for (int k = 0; k < 2; ++k) {
-            release(h, h->d_D[k]);
-            release(h, h->d_Vp[k]);
-            release(h, h->d_Uspill[k]);
-        }
+        // the partial buffers of both precisions are kept (ensure() grows
+        // them when the new tiling needs more): switching precision back and
+        // forth between tile widths then allocates nothing
     }
     // Few tiles (fewer than two per CU): the pruning passes split the tree
     // into subtree parts walked by separate workgroups (a shorter dependent
@@ -644,6 +649,11 @@ int prepare_buffers(hy_engine h, int prec) {
     if (!h->h_status) {
         HIP_TRY(h, hipHostMalloc((void **)&h->h_status, sizeof(int) * 64, kZeroCopyFlags));
         HIP_TRY(h, hipHostGetDevicePointer((void **)&h->h_status_dev, h->h_status, 0));
+    }
+    if (!h->h_done) {
+        HIP_TRY(h, hipHostMalloc((void **)&h->h_done, 64, kZeroCopyFlags));
+        HIP_TRY(h, hipHostGetDevicePointer((void **)&h->h_done_dev, h->h_done, 0));
+        *h->h_done = 0;
     }
     return HY_OK;
 }
@@ -1299,6 +1309,8 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
     fn.proj_stride = h->n_grp;
     fn.res = h->h_res_dev;
     fn.genes = (const GeneInfo *)h->d_genes.p;
+    // a single evaluation publishes its call number for eval_complete's wait
+    fn.done = (h->publish_done && h->n_genes <= 1 && !fn.genes && !prof) ? h->h_done_dev : nullptr;
     launch_final(fn, h->n_genes, s);
     HIP_TRY(h, hipGetLastError());
     if (prof) HIP_TRY(h, hipEventRecord(h->ev[7], s));
@@ -1430,6 +1442,7 @@ void hy_destroy(hy_engine h) {
     if (h->h_params) (void)hipHostFree(h->h_params);
     if (h->h_res) (void)hipHostFree(h->h_res);
     if (h->h_status) (void)hipHostFree(h->h_status);
+    if (h->h_done) (void)hipHostFree(h->h_done);
     if (h->h_site) (void)hipHostFree(h->h_site);
     if (h->f_pin) (void)hipHostFree(h->f_pin);
     if (h->profile)
@@ -1856,7 +1869,12 @@ static int launch_graph(hy_engine h, int prec, bool grad) {
     hy_engine_s::GraphEntry *ge = nullptr;
     for (auto &g : h->graphs)
         if (g.prec == prec && g.grad == (int)grad && g.warm == warm) ge = &g;
-    if (ge && ge->epoch != h->epoch) destroy_graph_exec(ge->exec);
+    if (ge && ge->exec && ge->epoch != h->epoch) {
+        // an evaluation awaited on the completion word may still be retiring
+        // on the stream: let it finish before its graph is destroyed
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        destroy_graph_exec(ge->exec);
+    }
     if (!ge || !ge->exec) {
         if (!acquire_graph_slot()) return GRAPH_NO_SLOT;
         hipGraph_t graph = nullptr;
@@ -1883,6 +1901,7 @@ static int launch_graph(hy_engine h, int prec, bool grad) {
             // are warm: release it rather than keep its runtime resources
             for (auto it = h->graphs.begin(); it != h->graphs.end(); ++it)
                 if (warm && it->prec == prec && it->grad == (int)grad && it->warm == 0) {
+                    HIP_TRY(h, hipStreamSynchronize(h->stream));
                     destroy_graph_exec(it->exec);
                     h->graphs.erase(it);
                     break;
@@ -1971,6 +1990,7 @@ int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, do
     for (int k = 0; k < 5; ++k) hp[pl.off_theta + k] = theta[k];
     hp[pl.off_delta] = delta;
     hp[pl.off_psi] = psi;
+    hp[pl.off_seq] = (double)(++h->seq);
     const bool grad = (flags & HY_WANT_GRAD) != 0;
     if (want_site && h->h_site_cap < (size_t)h->P) {
         if (h->h_site) (void)hipHostFree(h->h_site);
@@ -1980,13 +2000,28 @@ int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, do
         h->h_site_cap = h->P;
     }
     int grc = GRAPH_NO_SLOT;
+    h->wait_done = false;
+    h->publish_done = true;
     if (h->use_graphs && !h->profile && !h->wover) grc = launch_graph(h, precision, grad);
-    if (grc < 0) return grc;
-    if (grc == GRAPH_NO_SLOT) {
-        HIP_TRY(h, hipMemcpyAsync(h->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice, h->stream));
-        const int rc = precision == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr) : run_eval<float>(h, flags, nullptr);
-        if (rc) return rc;
+    if (grc < 0) {
+        h->publish_done = false;
+        return grc;
     }
+    if (grc == GRAPH_NO_SLOT) {
+        const hipError_t e = hipMemcpyAsync(h->d_params.p, hp, sizeof(double) * pl.total, hipMemcpyHostToDevice,
+                                            h->stream);
+        const int rc = e != hipSuccess ? fail(h, HY_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e))
+                       : precision == HY_PREC_FP64 ? run_eval<double>(h, flags, nullptr)
+                                                   : run_eval<float>(h, flags, nullptr);
+        if (rc) {
+            h->publish_done = false;
+            return rc;
+        }
+    }
+    h->publish_done = false;
+    // the results can be awaited on the completion word unless a download
+    // (site likelihoods) follows the evaluation on the stream
+    h->wait_done = !want_site && !h->profile;
     if (want_site)
         HIP_TRY(h, hipMemcpyAsync(h->h_site, h->d_site.p, sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
     return HY_OK;
@@ -1995,7 +2030,31 @@ int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, do
 int eval_complete(hy_engine h, bool grad, double *out_logL, double *out_site_ll, double *out_grad_bl,
                   double *out_grad_omega, double *out_grad_w, double *out_grad_theta) {
     HIP_TRY(h, hipSetDevice(h->device));
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->wait_done) {
+        // k_final stores this call's number in the pinned completion word once
+        // every result is visible to the host: spin on it (a stream wait adds
+        // the end-of-dispatch signalling to every evaluation); after 0.2 s the
+        // stream wait takes over, which also reports a failed launch
+        h->wait_done = false;
+        const unsigned long long want = h->seq;
+        bool seen = false;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (unsigned it = 1;; ++it) {
+            if (__atomic_load_n(h->h_done, __ATOMIC_ACQUIRE) == want) {
+                seen = true;
+                break;
+            }
+            __builtin_ia32_pause();
+            if ((it & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) break;
+        }
+        if (!seen) {
+            HIP_TRY(h, hipStreamSynchronize(h->stream));
+            if (__atomic_load_n(h->h_done, __ATOMIC_ACQUIRE) != want)
+                return fail(h, HY_ERR_HIP, "evaluation finished without publishing its results");
+        }
+    } else {
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+    }
     if (out_site_ll) std::memcpy(out_site_ll, h->h_site, sizeof(double) * h->P);
     return finish_host(h, true, out_logL, out_grad_bl, out_grad_omega, out_grad_w, out_grad_theta, grad);
 }

@@ -95,7 +95,9 @@ struct GeneInfo {
 struct ParamLayout {
     int n_nodes;   // extended node count
     int n_mat, n_comp;
-    int off_bl, off_alpha, off_omega, off_w, off_theta, off_delta, off_psi, total;
+    int off_bl, off_alpha, off_omega, off_w, off_theta, off_delta, off_psi;
+    int off_seq;  // call number of the evaluation (k_final publishes it when done)
+    int total;
 };
 
 // Packed description of codon pair (i, j), one int32 per entry of the 64 x 64
@@ -278,6 +280,9 @@ struct FinalArgs {
     int proj_stride;  // groups per matrix in proj ([n_mat][6][proj_stride])
     double *res;  // [0] lnL, [1..n_real] grad_bl, then omega[n_mat], w[n_comp], theta[5]
     const GeneInfo *genes;  // batched evaluation: one workgroup per gene, results at res_off
+    // single evaluations: pinned host word that receives params[off_seq] once
+    // every result is visible to the host (nullptr: none)
+    unsigned long long *done;
 };
 
 // Launchers (hy_kernels.hip).

@@ -911,6 +911,122 @@ __global__ __launch_bounds__(MODEL_NT) void k_model(ModelArgs args) {
     if (tid == 0) a.status[o] = status;
 }
 
+// ---------------------------------------------------------------------------
+// k_pmat: mixed transition matrices per branch, four workgroups per branch,
+// each building one block of 16 rows of P.
+// ---------------------------------------------------------------------------
+// Rows 16 qr .. 16 qr + 15 of one branch's P (FP64 in LDS, row stride 64) in
+// the layouts the pruning passes read: row-major rows with their row sums, the
+// A-operand swizzle entries of P whose row lies in the block (PA[qr]) and of
+// P^T whose column does (PTA entries with kk in [4 qr, 4 qr + 4)).
+template <typename R>
+__device__ __forceinline__ void emit_p_rows(const double *Pq, int qr, int node, int S, R *Prow_all, R *PA_all,
+                                            R *PTA_all, int tid) {
+    R *Prow = Prow_all + (size_t)node * PRS, *PA = PA_all + (size_t)node * 4096, *PTA = PTA_all + (size_t)node * 4096;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int e = tid + 256 * k;  // 0..1023
+        Prow[16 * 64 * qr + e] = (R)Pq[e];
+        // PA[qr][l][kk]: row 16 qr + (l & 15), column 4 kk + (l >> 4)
+        const int l = e >> 4, kk = e & 15;
+        PA[1024 * qr + e] = (R)Pq[(l & 15) * 64 + 4 * kk + (l >> 4)];
+        // PTA[w][l][kk] with kk = 4 qr + (e & 3): P[col][row], col = 4 kk + (l >> 4)
+        // in this block, row = 16 w + (l & 15)
+        const int w = e >> 8, l2 = (e >> 2) & 63, kk2 = 4 * qr + (e & 3);
+        const int col = 4 * kk2 + (l2 >> 4), row = 16 * w + (l2 & 15);
+        PTA[1024 * w + 16 * l2 + kk2] = (R)Pq[(col - 16 * qr) * 64 + row];
+    }
+    if (tid < 16) {
+        double s = 0.0;
+        for (int j = 0; j < S; ++j) s += Pq[tid * 64 + j];
+        Prow[4096 + 16 * qr + tid] = (R)s;
+    }
+}
+
+// Row block qr of P_b = sum_j w_j max(U_j diag(e^{lambda_j tau}) V_j, 0):
+// wave w forms the 16 x 16 block in columns 16 w .. 16 w + 15 on FP64 MFMA.
+// Us: the block's 16 rows of U diag(e^{lambda tau}), then the finished rows
+// (16 x LDA); Vs: V (64 x LDA).
+template <typename R>
+__device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, int qr, double *Us, double *Vs) {
+    const int tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
+    __shared__ double ex[64];
+    const double *prm = a.params;
+    const int j0 = a.comp_ptr[cls], j1 = a.comp_ptr[cls + 1];
+    double conv = 1.0;
+    if (a.convention == 1) {
+        double mb = 0.0;
+        for (int j = j0; j < j1; ++j)
+            if (a.comp_mat[j] >= 0) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];  // sink: no mu
+        if (mb < 1e-12) mb = 1e-12;
+        conv = 3.0 / mb;
+    }
+    const double tau = prm[a.pl.off_bl + node] * conv;
+    if (tid == 0 && qr == 0) a.tau[node] = tau;
+    f64x4 Pacc = splat<f64x4>(0.0);
+    // U rows, V and lambda of the next component are requested before the
+    // current component's product, so only the first component waits on them
+    double ru[4], rv[16], rl = 0.0;
+    auto fetch = [&](int j) {
+        const int m = a.comp_mat[j];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ru[k] = a.U[(size_t)m * 4096 + 1024 * qr + tid + 256 * k];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) rv[k] = a.V[(size_t)m * 4096 + tid + 256 * k];
+        if (tid < 64) rl = a.lam[m * 64 + tid];
+    };
+    // error-sink components (comp_mat < 0) have no spectrum: skipped here,
+    // added as w_j P_sink after the products
+    auto next_mat = [&](int j) {
+        while (j < j1 && a.comp_mat[j] < 0) ++j;
+        return j;
+    };
+    int j = next_mat(j0);
+    if (j < j1) fetch(j);
+    while (j < j1) {
+        const double wj = prm[a.pl.off_w + j];
+        const int jn = next_mat(j + 1);
+        __syncthreads();  // the previous component's operand reads are done
+        if (tid < 64) ex[tid] = exp(rl * tau);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = tid + 256 * k;
+            Us[(e >> 6) * LDA + (e & 63)] = ru[k] * ex[e & 63];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k;
+            Vs[(e >> 6) * LDA + (e & 63)] = rv[k];
+        }
+        if (jn < j1) fetch(jn);
+        __syncthreads();
+        f64x4 acc = splat<f64x4>(0.0);
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            const int k = 4 * kk + lg;
+            acc = MF<double>::mma(Us[lr * LDA + k], Vs[k * LDA + 16 * w + lr], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Pacc[r] += wj * (acc[r] > 0.0 ? acc[r] : 0.0);
+        j = jn;
+    }
+    for (int js = j0; js < j1; ++js)
+        if (a.comp_mat[js] < 0) {
+            const double ws = prm[a.pl.off_w + js];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                Pacc[r] += ws * a.sink[(16 * qr + MF<double>::crow(lg, r)) * 64 + 16 * w + lr];
+        }
+    __syncthreads();
+    double *Pq = Us;  // the block's rows, row stride 64 (16 x 64 fits in 16 x LDA)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Pq[MF<double>::crow(lg, r) * 64 + 16 * w + lr] = Pacc[r];
+    __syncthreads();
+    emit_p_rows<R>(Pq, qr, node, a.S, a.Prow, a.PA, a.PTA, tid);
+}
+
 // Write one branch's P (FP64 in LDS) as row-major with row sums and the MFMA
 // A-operand swizzles of P and P^T in the working type.
 template <typename R>
@@ -929,93 +1045,6 @@ __device__ __forceinline__ void emit_p(const double *P, int node, int S, R *Prow
         for (int j = 0; j < S; ++j) s += P[tid * 64 + j];
         Prow[4096 + tid] = (R)s;
     }
-}
-
-// ---------------------------------------------------------------------------
-// k_pmat: mixed transition matrices per branch.
-// ---------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ void pmat_body(const PmatArgs<R> &a, int bi, double *Us, double *Vs) {
-    // Us: U diag(e^{lambda tau}), the mixed P at the end; Vs: V (64 x LDA each)
-    const int tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
-    const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
-    __shared__ double ex[64];
-    const double *prm = a.params;
-    const int j0 = a.comp_ptr[cls], j1 = a.comp_ptr[cls + 1];
-    double conv = 1.0;
-    if (a.convention == 1) {
-        double mb = 0.0;
-        for (int j = j0; j < j1; ++j)
-            if (a.comp_mat[j] >= 0) mb += prm[a.pl.off_w + j] * a.mu[a.comp_mat[j]];  // sink: no mu
-        if (mb < 1e-12) mb = 1e-12;
-        conv = 3.0 / mb;
-    }
-    const double tau = prm[a.pl.off_bl + node] * conv;
-    if (tid == 0) a.tau[node] = tau;
-    // P = sum_j w_j max(U diag(e^{lambda tau}) V, 0): one 64^3 FP64 MFMA product
-    // per component, wave w owning rows 16w..16w+15
-    f64x4 Pacc[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) Pacc[q] = splat<f64x4>(0.0);
-    // U, V and lambda of the next component are requested before the
-    // current component's product, so only the first component waits on them
-    double ru[16], rv[16], rl = 0.0;
-    auto fetch = [&](int j) {
-        const int m = a.comp_mat[j];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            ru[k] = a.U[(size_t)m * 4096 + tid + 256 * k];
-            rv[k] = a.V[(size_t)m * 4096 + tid + 256 * k];
-        }
-        if (tid < 64) rl = a.lam[m * 64 + tid];
-    };
-    // error-sink components (comp_mat < 0) have no spectrum: skipped here,
-    // added as w_j P_sink after the products
-    auto next_mat = [&](int j) {
-        while (j < j1 && a.comp_mat[j] < 0) ++j;
-        return j;
-    };
-    int j = next_mat(j0);
-    if (j < j1) fetch(j);
-    while (j < j1) {
-        const double wj = prm[a.pl.off_w + j];
-        const int jn = next_mat(j + 1);
-        __syncthreads();  // the previous component's operand reads are done
-        if (tid < 64) ex[tid] = exp(rl * tau);
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int idx = tid + 256 * k;
-            Us[(idx >> 6) * LDA + (idx & 63)] = ru[k] * ex[idx & 63];
-            Vs[(idx >> 6) * LDA + (idx & 63)] = rv[k];
-        }
-        if (jn < j1) fetch(jn);
-        __syncthreads();
-        f64x4 acc[4];
-        gemm64_acc<false, false>(Us, Vs, acc, w, lr, lg);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Pacc[q][r] += wj * (acc[q][r] > 0.0 ? acc[q][r] : 0.0);
-        j = jn;
-    }
-    for (int js = j0; js < j1; ++js)
-        if (a.comp_mat[js] < 0) {
-            const double ws = prm[a.pl.off_w + js];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    Pacc[q][r] += ws * a.sink[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr];
-        }
-    __syncthreads();
-    double *P = Us;  // row-major 64 x 64 (stride 64) in the operand buffer
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) P[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] = Pacc[q][r];
-    __syncthreads();
-    emit_p<R>(P, node, a.S, a.Prow, a.PA, a.PTA, tid);
 }
 
 // Caller-supplied P matrices (row-major S x S per node, FP64).
@@ -2051,8 +2080,11 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs<R> args) {
 // Projections: M_phi = V_pi dQ_phi U_pi and dmu_phi = -sum_i pi_i dQ_phi[i][i]
 // (likelihood.hpp:1497-1562).  phi 0 = omega, 1..5 = theta AC, AT, CG, CT, GT.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void proj_body(const ProjArgs &args, int m, int phi, double *X, double *Y) {
-    // X: dQ_phi, then dQ_phi U; Y: U, then V (64 x LDA each)
+__device__ __forceinline__ void proj_body(const ProjArgs &args, int m, int phi, int qc, double *X, double *Y) {
+    // Column block qc of M_phi = V (dQ_phi U): Y_c = dQ_phi U[:, c] (64 x 16),
+    // then V Y_c; no product is formed twice across the four blocks.
+    // X: dQ_phi, then V (64 x LDA); Y: U[:, c], then Y_c (16 x LDA, transposed:
+    // Y[j * LDA + k] = entry (k, j)).
     ProjArgs a = args;
     matrix_gene_view(a, m);
     const int tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
@@ -2060,13 +2092,28 @@ __device__ __forceinline__ void proj_body(const ProjArgs &args, int m, int phi, 
     const double *prm = a.params;
     const double alpha = prm[a.pl.off_alpha], beta = prm[a.pl.off_omega + m];
     const int S = a.S;
+    // descriptors, U columns and V: every load issued before the first use
+    int rd[16];
+    double ru[4], rv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        rd[k] = a.qdesc[tid + 256 * k];
+        rv[k] = a.V[(size_t)m * 4096 + tid + 256 * k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int e = tid + 256 * k, row = e >> 4, col = 16 * qc + (e & 15);
+        ru[k] = a.U[(size_t)m * 4096 + row * 64 + col];
+    }
     if (tid < 8) thl[tid] = tid < 5 ? prm[a.pl.off_theta + tid] : 1.0;
     if (tid < 12) posl[tid] = a.pos[tid];
     __syncthreads();
     // dQ_phi: single-nucleotide changes only (likelihood.hpp:1529-1562);
     // phi = 0: omega (nonsynonymous pairs), phi = 1..5: theta_AC..theta_GT
-    for (int idx = tid; idx < 4096; idx += 256) {
-        const int i = idx >> 6, j = idx & 63, d = a.qdesc[idx];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int idx = tid + 256 * k;
+        const int i = idx >> 6, j = idx & 63, d = rd[k];
         double q = 0.0;
         if (i < S && j < S && i != j && (d & 3) == 1) {
             const int ti = (d >> 3) & 7;
@@ -2077,7 +2124,11 @@ __device__ __forceinline__ void proj_body(const ProjArgs &args, int m, int phi, 
                 q = ((d & QDESC_SYN) ? alpha : beta) * pf;
         }
         X[i * LDA + j] = q;
-        Y[i * LDA + j] = a.U[(size_t)m * 4096 + idx];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int e = tid + 256 * k;
+        Y[(e & 15) * LDA + (e >> 4)] = ru[k];  // U[row][16 qc + j] at Y[j][row]
     }
     __syncthreads();
     if (tid < 64) {
@@ -2087,33 +2138,48 @@ __device__ __forceinline__ void proj_body(const ProjArgs &args, int m, int phi, 
         X[tid * LDA + tid] = -s;
         double t = tid < S ? a.pi[tid] * s : 0.0;  // d mu / d phi = -sum_i pi_i dQ_ii
         for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
-        if (tid == 0) a.dmu[m * 6 + phi] = t;
+        if (tid == 0 && qc == 0) a.dmu[m * 6 + phi] = t;
     }
     __syncthreads();
-    f64x4 acc[4];
-    gemm64_acc<false, false>(X, Y, acc, w, lr, lg);  // dQ U
+    // Y_c = dQ U[:, c]: wave w forms rows 16 w .. 16 w + 15
+    f64x4 acc = splat<f64x4>(0.0);
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+        const int k = 4 * kk + lg;
+        acc = MF<double>::mma(X[(16 * w + lr) * LDA + k], Y[lr * LDA + k], acc);
+    }
     __syncthreads();
-    gemm64_store(acc, X, w, lr, lg);
-    for (int idx = tid; idx < 4096; idx += 256) Y[(idx >> 6) * LDA + (idx & 63)] = a.V[(size_t)m * 4096 + idx];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Y[lr * LDA + 16 * w + MF<double>::crow(lg, r)] = acc[r];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k;
+        X[(e >> 6) * LDA + (e & 63)] = rv[k];
+    }
     __syncthreads();
-    gemm64_acc<false, false>(Y, X, acc, w, lr, lg);  // V (dQ U)
+    // M[:, c] = V Y_c: wave w forms rows 16 w .. 16 w + 15
+    acc = splat<f64x4>(0.0);
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+        const int k = 4 * kk + lg;
+        acc = MF<double>::mma(X[(16 * w + lr) * LDA + k], Y[lr * LDA + k], acc);
+    }
     double *M = a.M + ((size_t)m * 6 + phi) * 4096;
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) M[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * q + lr] = acc[q][r];
+    for (int r = 0; r < 4; ++r) M[(16 * w + MF<double>::crow(lg, r)) * 64 + 16 * qc + lr] = acc[r];
 }
 
 // P builds and (with gradients) the dQ projections depend only on the
-// spectra: one launch, workgroups [0, n_branch) build P, the rest project.
+// spectra: one launch, four workgroups per branch build P (one row block
+// each), then four per (matrix, parameter) project (one column block each).
 template <typename R>
-__global__ __launch_bounds__(256, 2) void k_pmat(PmatArgs<R> a, ProjArgs pj, int n_proj) {
-    __shared__ double L0[64 * LDA], L1[64 * LDA];  // one pair of tiles, shared by both roles
-    const int b = blockIdx.x;
+__global__ __launch_bounds__(256, 3) void k_pmat(PmatArgs<R> a, ProjArgs pj, int n_proj) {
+    __shared__ double L0[64 * LDA], L1[16 * LDA];  // shared by both roles
+    const int b = blockIdx.x >> 2, q = blockIdx.x & 3;
     if (b < a.n_branch)
-        pmat_body<R>(a, b, L0, L1);
+        pmat_body<R>(a, b, q, L1, L0);
     else if (b - a.n_branch < n_proj)
-        proj_body(pj, (b - a.n_branch) / 6, (b - a.n_branch) % 6, L0, L1);
+        proj_body(pj, (b - a.n_branch) / 6, (b - a.n_branch) % 6, q, L0, L1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2136,9 +2202,23 @@ __global__ __launch_bounds__(256, 2) void k_pmat(PmatArgs<R> a, ProjArgs pj, int
 // GROUPED: the workgroup walks several pairs of one matrix with U and V
 // resident in LDS (three 64 x 64 buffers, one workgroup per CU); otherwise it
 // transforms one pair with two buffers (V, then U, reloaded), two per CU.
+// Grid numbering of the Fréchet stage: workgroups go to the eight XCDs in
+// turn (linear id mod 8), and each XCD has its own L2.  Linear id
+// L = (g / 8) * 8 n_mat + 8 m + g % 8 puts group g of every matrix on XCD
+// g % 8: with a mixture whose classes use every matrix, group g of each
+// matrix's pair list is the same branch, so the A_b those workgroups read
+// (summed over its pattern chunks) comes from HBM once and from L2 after.
+constexpr int FRECHET_XCD = 8;
+inline int frechet_grid(int n_mat, int n_grp) {
+    return (n_grp + FRECHET_XCD - 1) / FRECHET_XCD * FRECHET_XCD * n_mat;
+}
+
 template <bool GROUPED>
 __global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a) {
-    const int m = blockIdx.x, grp = blockIdx.y, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    const int per = FRECHET_XCD * a.n_mat, blk = blockIdx.x / per, rr = blockIdx.x - blk * per;
+    const int m = rr / FRECHET_XCD, grp = blk * FRECHET_XCD + rr % FRECHET_XCD;
+    if (grp >= a.n_grp) return;  // padding of the last block of groups
+    const int tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     __shared__ double Bs[64 * LDA];                  // V (and U unless GROUPED)
     __shared__ double Us[GROUPED ? 64 * LDA : 1];    // U (GROUPED)
     __shared__ double As[64 * LDA];
@@ -2152,32 +2232,67 @@ __global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a)
     f64x4 H[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) H[q] = splat<f64x4>(0.0);
-    // A_b of the next pair (summed over its pattern chunks in chunk order)
-    // is fetched into registers while the current pair is transformed; with
-    // one chunk per branch the fetch only issues loads
+    // A_b of a pair, summed over its pattern chunks in chunk order: the loads
+    // of up to four chunks are issued together (clamped to the last chunk, so
+    // every load is valid and the code has no branches), then summed
     double ra[16];
-    auto fetch_a = [&](int e) {
-        const int bi = a.wl_idx[e] / a.max_comp;
+    auto fetch_a = [&](int bi) {
         const double *src = a.Apart + (size_t)bi * a.n_chunks * 4096 + tid;
+        const int nc = a.n_chunks;
+        double t[4][16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) ra[k] = src[256 * k];
-        for (int c = 1; c < a.n_chunks; ++c)
+        for (int c = 0; c < 4; ++c) {
+            const int cc = c < nc ? c : nc - 1;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) t[c][k] = src[(size_t)cc * 4096 + 256 * k];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ra[k] = t[0][k];
+#pragma unroll
+        for (int c = 1; c < 4; ++c)
+            if (c < nc)
+#pragma unroll
+                for (int k = 0; k < 16; ++k) ra[k] += t[c][k];
+        for (int c = 4; c < nc; ++c)
 #pragma unroll
             for (int k = 0; k < 16; ++k) ra[k] += src[(size_t)c * 4096 + 256 * k];
     };
-    if (e0 < e1) {
-        fetch_a(e0);
-        for (int idx = tid; idx < 4096; idx += 256) {
-            if constexpr (GROUPED) Us[(idx >> 6) * LDA + (idx & 63)] = a.U[(size_t)m * 4096 + idx];
-            Bs[(idx >> 6) * LDA + (idx & 63)] = a.V[(size_t)m * 4096 + idx];
-        }
-        if (tid < 64) lam[tid] = a.lam[m * 64 + tid];
-    }
-    for (int e = e0; e < e1; ++e) {
-        const int slot = a.wl_idx[e], bi = slot / a.max_comp, jj = slot - bi * a.max_comp;
+    // pair metadata; the chain slot -> class -> weight is scalar loads that
+    // do not wait for the operand loads issued before it
+    struct Pair {
+        int bi, jj;
+        double wj, tau;
+    };
+    auto pair_info = [&](int e, int bi) {
+        Pair q;
+        q.bi = bi;
+        q.jj = a.wl_idx[e] - bi * a.max_comp;
         const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3];
-        const int J = a.comp_ptr[cls] + jj;
-        const double wj = a.params[a.pl.off_w + J], tau = a.tau[node];
+        q.wj = a.params[a.pl.off_w + a.comp_ptr[cls] + q.jj];
+        q.tau = a.tau[node];
+        return q;
+    };
+    // operands of the first pair: V (and U when it stays in LDS) of the
+    // matrix and the pair's A_b, all requested before any is used
+    const int bi0 = a.wl_idx[e0] / a.max_comp;
+    double rv[16], ru[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        rv[k] = a.V[(size_t)m * 4096 + tid + 256 * k];
+        if constexpr (GROUPED) ru[k] = a.U[(size_t)m * 4096 + tid + 256 * k];
+    }
+    fetch_a(bi0);
+    Pair cur = pair_info(e0, bi0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int idx = tid + 256 * k;
+        if constexpr (GROUPED) Us[(idx >> 6) * LDA + (idx & 63)] = ru[k];
+        Bs[(idx >> 6) * LDA + (idx & 63)] = rv[k];
+    }
+    if (tid < 64) lam[tid] = a.lam[m * 64 + tid];
+    for (int e = e0; e < e1; ++e) {
+        const int bi = cur.bi, jj = cur.jj;
+        const double wj = cur.wj, tau = cur.tau;
         __syncthreads();  // the previous pair is done with As and ex
         if (tid < 64) ex[tid] = exp(lam[tid] * tau);
 #pragma unroll
@@ -2185,14 +2300,27 @@ __global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a)
             const int idx = tid + 256 * k;
             As[(idx >> 6) * LDA + (idx & 63)] = ra[k];
         }
-        if (e + 1 < e1) fetch_a(e + 1);
+        if (GROUPED && e + 1 < e1) {  // (one pair per workgroup otherwise)
+            const int bn = a.wl_idx[e + 1] / a.max_comp;
+            fetch_a(bn);
+            cur = pair_info(e + 1, bn);
+        }
         __syncthreads();
+        // U over V in the second buffer once A V^T is formed: its loads are
+        // in flight during that product
+        if constexpr (!GROUPED)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) ru[k] = a.U[(size_t)m * 4096 + tid + 256 * k];
         f64x4 acc[4];
         gemm64_acc<false, true>(As, Bs, acc, w, lr, lg);  // A V^T
         __syncthreads();
         gemm64_store(acc, As, w, lr, lg);
         if constexpr (!GROUPED)
-            for (int idx = tid; idx < 4096; idx += 256) Bs[(idx >> 6) * LDA + (idx & 63)] = a.U[(size_t)m * 4096 + idx];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int idx = tid + 256 * k;
+                Bs[(idx >> 6) * LDA + (idx & 63)] = ru[k];
+            }
         __syncthreads();
         gemm64_acc<true, false>(GROUPED ? Us : Bs, As, acc, w, lr, lg);  // A~ = U^T (A V^T), rows of this wave
         double d0 = 0.0, d1 = 0.0;
@@ -2318,13 +2446,14 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
         double s = 0.0, c = 0.0;
         const int pe = min((tid + 1) * per, P);
         for (int p0 = tid * per; p0 < pe; p0 += 8) {
-            // eight independent loads in flight, then the compensated sums in order
+            // eight patterns' loads in flight (unconditional, clamped to the
+            // range), then the compensated sums in order
             double x[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int p = p0 + u;
-                x[u] = 0.0;
-                if (p < pe && !(grad && !(a.Lroot[p] > 0.0))) x[u] = a.weights[p] * a.site_ll[p];
+                const int p = p0 + u, pc = p < pe ? p : pe - 1;
+                const double L = a.Lroot[pc], wv = a.weights[pc], sl = a.site_ll[pc];
+                x[u] = (p < pe && !(grad && !(L > 0.0))) ? wv * sl : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
@@ -2347,9 +2476,18 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
         if (grad && staged)
             for (int bi = gt; bi < a.n_branch; bi += FINAL_NT - NL) {
                 const int node = a.blist[bi * BR_STRIDE], cls = a.blist[bi * BR_STRIDE + 3] - cls0;
+                // the first four components' shares are requested with the
+                // branch's entry (they do not wait on its class), then summed
+                // in component order
+                double f4[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) f4[j] = j < MC ? a.fr[((size_t)bi * MC + j) * 8 + 1] : 0.0;
                 double dt = 0.0;
                 const int nj = a.comp_ptr[cls + 1] - a.comp_ptr[cls];  // components of this branch's class
-                for (int j = 0; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j < nj) dt += f4[j];
+                for (int j = 4; j < nj; ++j) dt += a.fr[((size_t)bi * MC + j) * 8 + 1];
                 s_cls[bi] = cls;
                 s_node[bi] = node - node0;
                 s_dt[bi] = dt;
@@ -2397,47 +2535,57 @@ __global__ __launch_bounds__(FINAL_NT) void k_final(FinalArgs args) {
         for (int w = 0; w < NL / 64; ++w) comp_add(S0, C0, wsum[w], wcmp[w]);
         a.res[0] = S0 + C0;
     }
-    if (!grad || tid < NL) return;
-    const int ntask = NM + 5 + NCP;
-    for (int t = gwave; t < ntask; t += NGW) {
-        double v = 0.0;
-        if (t < NM + 5) {
-            const bool is_om = t < NM;
-            const int phi = is_om ? 0 : (t - NM + 1);
-            // <M_{m,phi}, H_m> partials: one matrix for omega_t, all matrices for theta
-            const int m0 = is_om ? t : 0, m1 = is_om ? t + 1 : NM;
-            const int PS = a.proj_stride;
-            for (int q = lane; q < (m1 - m0) * PS; q += 64) v += a.proj[((size_t)(m0 + q / PS) * 6 + phi) * PS + q % PS];
-            v = wave_sum(v);
-            if (lane == 0) {
-                if (a.convention == 1) {
-                    for (int cl = 0; cl < a.n_class; ++cl) {
-                        const double f = sum_tau[cl] / mubar[cl];
-                        for (int j = a.comp_ptr[cl]; j < a.comp_ptr[cl + 1]; ++j) {
-                            const int mm = a.comp_mat[j];
-                            if (mm < 0 || (is_om && mm != mat0 + t)) continue;
-                            v -= f * prm[a.pl.off_w + j] * a.dmu[mm * 6 + phi];
+    if (grad && tid >= NL) {
+        const int ntask = NM + 5 + NCP;
+        for (int t = gwave; t < ntask; t += NGW) {
+            double v = 0.0;
+            if (t < NM + 5) {
+                const bool is_om = t < NM;
+                const int phi = is_om ? 0 : (t - NM + 1);
+                // <M_{m,phi}, H_m> partials: one matrix for omega_t, all matrices for theta
+                const int m0 = is_om ? t : 0, m1 = is_om ? t + 1 : NM;
+                const int PS = a.proj_stride;
+                for (int q = lane; q < (m1 - m0) * PS; q += 64) v += a.proj[((size_t)(m0 + q / PS) * 6 + phi) * PS + q % PS];
+                v = wave_sum(v);
+                if (lane == 0) {
+                    if (a.convention == 1) {
+                        for (int cl = 0; cl < a.n_class; ++cl) {
+                            const double f = sum_tau[cl] / mubar[cl];
+                            for (int j = a.comp_ptr[cl]; j < a.comp_ptr[cl + 1]; ++j) {
+                                const int mm = a.comp_mat[j];
+                                if (mm < 0 || (is_om && mm != mat0 + t)) continue;
+                                v -= f * prm[a.pl.off_w + j] * a.dmu[mm * 6 + phi];
+                            }
                         }
                     }
+                    if (is_om)
+                        gom[t] = v;
+                    else
+                        gth[phi - 1] = v;
                 }
-                if (is_om)
-                    gom[t] = v;
-                else
-                    gth[phi - 1] = v;
-            }
-        } else {
-            const int J = t - NM - 5, Jg = comp0 + J;
-            int cls = 0;
-            while (cls + 1 < a.n_class && a.comp_ptr[cls + 1] <= Jg) ++cls;
-            const int jj = Jg - a.comp_ptr[cls];
-            for (int bi = lane; bi < a.n_branch; bi += 64)
-                if (a.blist[bi * BR_STRIDE + 3] == cls0 + cls) v += a.fr[((size_t)bi * MC + jj) * 8 + 0];
-            v = wave_sum(v);
-            if (lane == 0) {
-                if (a.convention == 1 && a.comp_mat[Jg] >= 0) v -= sum_tau[cls] / mubar[cls] * a.mu[a.comp_mat[Jg]];
-                gw[J] = v;
+            } else {
+                const int J = t - NM - 5, Jg = comp0 + J;
+                int cls = 0;
+                while (cls + 1 < a.n_class && a.comp_ptr[cls + 1] <= Jg) ++cls;
+                const int jj = Jg - a.comp_ptr[cls];
+                for (int bi = lane; bi < a.n_branch; bi += 64)
+                    if (a.blist[bi * BR_STRIDE + 3] == cls0 + cls) v += a.fr[((size_t)bi * MC + jj) * 8 + 0];
+                v = wave_sum(v);
+                if (lane == 0) {
+                    if (a.convention == 1 && a.comp_mat[Jg] >= 0) v -= sum_tau[cls] / mubar[cls] * a.mu[a.comp_mat[Jg]];
+                    gw[J] = v;
+                }
             }
         }
+    }
+    // single evaluation: every thread's result stores reach the host before
+    // the call number (params[off_seq]) is published in the completion word
+    if (a.done) {
+        __threadfence_system();
+        __syncthreads();
+        if (tid == 0)
+            __hip_atomic_store(a.done, (unsigned long long)prm[a.pl.off_seq], __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -2451,7 +2599,7 @@ void launch_model(const ModelArgs &a, int n_mat, hipStream_t s) {
 template <typename R> void launch_pmat(const PmatArgs<R> &a, const ProjArgs *pj, int n_mat, hipStream_t s) {
     const int n_proj = pj ? 6 * n_mat : 0;
     if (a.n_branch + n_proj > 0)
-        hipLaunchKernelGGL(k_pmat<R>, dim3(a.n_branch + n_proj), dim3(256), 0, s, a, pj ? *pj : ProjArgs{}, n_proj);
+        hipLaunchKernelGGL(k_pmat<R>, dim3(4 * (a.n_branch + n_proj)), dim3(256), 0, s, a, pj ? *pj : ProjArgs{}, n_proj);
 }
 
 template <typename R, int T> void launch_inside_t(const PruneArgs<R> &a, int ny, hipStream_t s) {
@@ -2556,10 +2704,11 @@ void launch_sink(const SinkArgs &a, int n_slots, hipStream_t s) {
 }
 
 void launch_frechet(const FrechetArgs &a, hipStream_t s) {
+    const int n = frechet_grid(a.n_mat, a.n_grp);
     if (a.grp_size > 1)
-        hipLaunchKernelGGL(k_frechet<true>, dim3(a.n_mat, a.n_grp), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_frechet<true>, dim3(n), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(k_frechet<false>, dim3(a.n_mat, a.n_grp), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_frechet<false>, dim3(n), dim3(256), 0, s, a);
 }
 
 void launch_final(const FinalArgs &a, int n_genes, hipStream_t s) {
