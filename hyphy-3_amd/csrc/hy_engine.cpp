@@ -179,6 +179,7 @@ struct hy_engine_s {
     // (eigenvectors kept for warm starts), per-call staging and scratch
     FelPool fel;
     DevBuf d_fZ, d_flam, d_fU, d_fV, d_fmu, d_fstatus, d_fstage, d_fbl, d_fD, d_fE, d_fout, d_fident;
+    DevBuf d_bP, d_bL, d_be, d_blnl;  // hy_branch_eval: candidates, products, exponents, sums
     // single spectra (hy_mg94_spectrum): last eigenvectors kept for a warm start
     DevBuf d_sprm, d_slam, d_sU, d_sV, d_smu, d_sZ, d_sstatus;
     uint64_t s_model_gen = ~uint64_t(0);
@@ -1429,7 +1430,7 @@ void hy_destroy(hy_engine h) {
                       &h->d_gH, &h->d_gL, &h->d_gLE, &h->d_gsite, &h->d_gApart, &h->d_gA, &h->d_gres,
                       &h->d_fZ, &h->d_flam, &h->d_fU, &h->d_fV, &h->d_fmu, &h->d_fstatus, &h->d_fstage, &h->d_fbl,
                       &h->d_fD, &h->d_fE, &h->d_fout, &h->d_fident, &h->d_sprm, &h->d_slam, &h->d_sU, &h->d_sV,
-                      &h->d_smu, &h->d_sZ, &h->d_sstatus};
+                      &h->d_smu, &h->d_sZ, &h->d_sstatus, &h->d_bP, &h->d_bL, &h->d_be, &h->d_blnl};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -2019,6 +2020,10 @@ int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, do
         }
     }
     h->publish_done = false;
+    // partials left resident by this evaluation (a replayed graph does not
+    // pass through run_eval, which records them when it launches eagerly)
+    h->part_ri = precision == HY_PREC_FP64 ? 1 : 0;
+    h->part_outside = grad;
     // the results can be awaited on the completion word unless a download
     // (site likelihoods) follows the evaluation on the stream
     h->wait_done = !want_site && !h->profile;
@@ -3956,6 +3961,88 @@ int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E,
     }
     if (L) std::memcpy(L, lr.data(), B * 8);
     if (LE) std::memcpy(LE, ler.data(), B * 4);
+    return HY_OK;
+}
+
+int hy_branch_eval(hy_engine h, int precision, int node, int n_cand, const double *P_cand, double *out_lnl,
+                   double *out_site_L, int32_t *out_site_e) {
+    if (!h) return HY_ERR_ARG;
+    if (!P_cand || !out_lnl || n_cand < 1) return fail(h, HY_ERR_ARG, "hy_branch_eval: no candidates or NULL output");
+    if (precision != HY_PREC_FP64 && precision != HY_PREC_FP32)
+        return fail(h, HY_ERR_ARG, "hy_branch_eval: unknown precision %d", precision);
+    const int ri = precision == HY_PREC_FP64 ? 1 : 0;
+    if (h->part_ri != ri || !h->part_outside)
+        return fail(h, HY_ERR_STATE,
+                    "hy_branch_eval: the last evaluation on this handle was not a gradient evaluation in this precision");
+    if (h->ck)
+        return fail(h, HY_ERR_UNSUPPORTED, "hy_branch_eval: the last evaluation kept only checkpointed partials");
+    if (h->n_genes > 1) return fail(h, HY_ERR_UNSUPPORTED, "hy_branch_eval: not on a fused gene batch");
+    if (node < 0 || node >= h->num_nodes || node == h->root)
+        return fail(h, HY_ERR_ARG, "hy_branch_eval: node %d has no branch above it", node);
+    int bi = -1;
+    for (int b = 0; b < h->n_branch; ++b)
+        if (h->blist[(size_t)b * BR_STRIDE] == node) bi = b;
+    if (bi < 0) return fail(h, HY_ERR_ARG, "hy_branch_eval: node %d has no branch above it", node);
+    const int32_t *e = &h->blist[(size_t)bi * BR_STRIDE];
+    const int ref = e[2], kind = ref >> 28, idx = ref & REF_MASK;
+    const int S = h->S, T = h->T;
+    HIP_TRY(h, hipSetDevice(h->device));
+    // candidates padded to 64 x 64 (zero rows and columns beyond S)
+    std::vector<double> Pp((size_t)n_cand * 4096, 0.0);
+    for (int c = 0; c < n_cand; ++c)
+        for (int i = 0; i < S; ++i)
+            std::memcpy(&Pp[((size_t)c * 64 + i) * 64], P_cand + ((size_t)c * S + i) * S, sizeof(double) * S);
+    TRY(ensure(h, h->d_bP, Pp.size() * 8));
+    TRY(ensure(h, h->d_bL, (size_t)n_cand * h->P_pad * 8));
+    TRY(ensure(h, h->d_be, (size_t)h->P_pad * 4));
+    TRY(ensure(h, h->d_blnl, (size_t)n_cand * 8));
+    hipStream_t s = h->stream;
+    HIP_TRY(h, hipMemcpyAsync(h->d_bP.p, Pp.data(), Pp.size() * 8, hipMemcpyHostToDevice, s));
+    auto fill = [&](auto &a) {
+        a.P = (const double *)h->d_bP.p;
+        a.E = (const int32_t *)h->d_E.p;
+        a.G = (const int32_t *)h->d_G.p;
+        a.tips = (const uint16_t *)h->d_tips.p;
+        a.amb = (const uint64_t *)h->d_amb.p;
+        a.v_slot = e[1];
+        a.d_slot = kind == KIND_TIP ? 0 : idx;
+        a.tip_row = kind == KIND_TIP ? idx : -1;
+        a.n_tiles = h->n_tiles;
+        a.P_pad = h->P_pad;
+        a.n_patterns = h->P;
+        a.S = S;
+        a.out_L = (double *)h->d_bL.p;
+        a.out_e = (int32_t *)h->d_be.p;
+    };
+    if (ri) {
+        BranchCtxArgs<double> a{};
+        fill(a);
+        a.V = (const double *)h->d_Vp[1].p;
+        a.D = (const double *)h->d_D[1].p;
+        launch_branch_ctx<double>(a, T, n_cand, s);
+    } else {
+        BranchCtxArgs<float> a{};
+        fill(a);
+        a.V = (const float *)h->d_Vp[0].p;
+        a.D = (const float *)h->d_D[0].p;
+        launch_branch_ctx<float>(a, T, n_cand, s);
+    }
+    HIP_TRY(h, hipGetLastError());
+    BranchSumArgs sa{};
+    sa.L = (const double *)h->d_bL.p;
+    sa.e = (const int32_t *)h->d_be.p;
+    sa.weights = (const double *)h->d_weights.p;
+    sa.P_pad = h->P_pad;
+    sa.n_patterns = h->P;
+    sa.out = (double *)h->d_blnl.p;
+    launch_branch_sum(sa, n_cand, s);
+    HIP_TRY(h, hipGetLastError());
+    HIP_TRY(h, hipMemcpyAsync(out_lnl, h->d_blnl.p, (size_t)n_cand * 8, hipMemcpyDeviceToHost, s));
+    if (out_site_L)
+        HIP_TRY(h, hipMemcpy2DAsync(out_site_L, (size_t)h->P * 8, h->d_bL.p, (size_t)h->P_pad * 8, (size_t)h->P * 8,
+                                    n_cand, hipMemcpyDeviceToHost, s));
+    if (out_site_e) HIP_TRY(h, hipMemcpyAsync(out_site_e, h->d_be.p, (size_t)h->P * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipStreamSynchronize(s));
     return HY_OK;
 }
 

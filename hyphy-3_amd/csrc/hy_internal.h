@@ -344,6 +344,31 @@ struct GtrReduceArgs {
 template <typename R> void launch_gtr(const GtrArgs<R> &a, bool grad, hipStream_t s);
 void launch_gtr_reduce(const GtrReduceArgs &a, bool grad, hipStream_t s);
 
+// aBSREL's local projection (hy_branch_eval): candidate transition matrices
+// of one branch scored against the resident partials of the last evaluation.
+template <typename R>
+struct BranchCtxArgs {
+    const double *P;        // [n_cand][64][64] row-major FP64, zero beyond S
+    const R *V, *D;         // the evaluation's outside / inside partial buffers
+    const int32_t *E, *G;   // their exponents ([rows][P_pad])
+    const uint16_t *tips;   // [taxa + 1][P_pad] codes
+    const uint64_t *amb;    // ambiguity masks
+    int v_slot;             // V row of the branch
+    int d_slot, tip_row;    // internal child: D row (tip_row < 0); tip child: its code row
+    int n_tiles, P_pad, n_patterns, S;
+    double *out_L;          // [n_cand][P_pad] V_b^T P_c D_b in the partials' scaling
+    int32_t *out_e;         // [P_pad] G_b + E_b: L = out_L 2^out_e
+};
+struct BranchSumArgs {
+    const double *L;        // [n_cand][P_pad]
+    const int32_t *e;       // [P_pad]
+    const double *weights;  // pattern weights
+    int P_pad, n_patterns;
+    double *out;            // [n_cand] sum_p w_p ln L_p (w_p (-1e20) where L_p <= 0)
+};
+template <typename R> void launch_branch_ctx(const BranchCtxArgs<R> &a, int T, int n_cand, hipStream_t s);
+void launch_branch_sum(const BranchSumArgs &a, int n_cand, hipStream_t s);
+
 // Fixed device scratch of the allocation-free entry point
 // (hy_loglik_from_P_noalloc): a file-scope __device__ array in
 // hy_kernels.hip, NOALLOC_SCRATCH_BYTES per device.  noalloc_scratch returns

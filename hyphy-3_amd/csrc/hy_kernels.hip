@@ -3108,6 +3108,121 @@ hipError_t noalloc_scratch(int device, void **out) {
     return hipSuccess;
 }
 
+// ---------------------------------------------------------------------------
+// Local projection of one branch (aBSREL, absrel.hpp:438-455 over the branch
+// context of :723-780): L_c,p = V_b,p^T P_c D_b,p for candidate matrices P_c,
+// from the outside partials V_b and the inside partials D_b the last
+// evaluation left resident (a tip child's D is the indicator of the pattern's
+// code).  One workgroup per (pattern tile, candidate); wave w forms the
+// products of states 16 w .. 16 w + 15 (P_c D_b on MFMA in the working type,
+// the column dot in FP64), the four waves' shares are summed in wave order.
+// ---------------------------------------------------------------------------
+template <typename R, int T>
+__global__ __launch_bounds__(256) void k_branch_ctx(BranchCtxArgs<R> a) {
+    constexpr int NB = T / 16;
+    typedef typename MF<R>::acc acc_t;
+    __shared__ double part[4][T];
+    const int tile = blockIdx.x, c = blockIdx.y;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
+    const double *P = a.P + (size_t)c * 4096;
+    const int p0 = tile * T;
+    acc_t X[NB];
+    if (a.tip_row < 0) {
+        R av[16], bv[16 * NB];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) av[kk] = (R)P[(16 * w + lr) * 64 + 4 * kk + lg];
+        fetch_b<R, T>(bv, a.D + ((size_t)a.d_slot * a.n_tiles + tile) * (64 * T), lr, lg);
+        mfma_regs<R, T>(av, bv, X);
+    } else {
+        // X[row][p]: P summed over the codons compatible with the code (in
+        // codon order), rounded to the working type like the pruning passes'
+        // tip columns
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const unsigned code = a.tips[(size_t)a.tip_row * a.P_pad + p0 + 16 * nb + lr];
+            const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
+            const uint64_t mask = code < 64u ? 0ull : (code == TIP_ONES ? all : a.amb[code & 0x7FFFu]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double *row = P + (16 * w + MF<R>::crow(lg, r)) * 64;
+                double x = 0.0;
+                if (code < 64u) {
+                    x = row[code];
+                } else {
+                    for (int j = 0; j < 64; ++j)
+                        if ((mask >> j) & 1ull) x += row[j];
+                }
+                X[nb][r] = (R)x;
+            }
+        }
+    }
+    acc_t Vt[NB];
+    load_c<R, T>(Vt, a.V + ((size_t)a.v_slot * a.n_tiles + tile) * (64 * T), w, lr, lg);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s += (double)Vt[nb][r] * (double)X[nb][r];
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        if (lg == 0) part[w][16 * nb + lr] = s;
+    }
+    __syncthreads();
+    if (tid < T) {
+        const int p = p0 + tid;
+        a.out_L[(size_t)c * a.P_pad + p] = ((part[0][tid] + part[1][tid]) + part[2][tid]) + part[3][tid];
+        if (c == 0)
+            a.out_e[p] = a.G[(size_t)a.v_slot * a.P_pad + p] + (a.tip_row < 0 ? a.E[(size_t)a.d_slot * a.P_pad + p] : 0);
+    }
+}
+
+// sum_p w_p ln L_p per candidate (L_p = L 2^e; w_p (-1e20) where L_p <= 0,
+// absrel.hpp:449-452), one workgroup per candidate: compensated per-thread
+// sums over contiguous pattern ranges, combined by error-free additions in a
+// fixed order.
+__global__ __launch_bounds__(1024) void k_branch_sum(BranchSumArgs a) {
+    const int c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ double ws[16], wc[16];
+    const double *L = a.L + (size_t)c * a.P_pad;
+    const int P = a.n_patterns, per = (P + 1023) / 1024;
+    const int pe = min((tid + 1) * per, P);
+    double s = 0.0, cmp = 0.0;
+    for (int p = tid * per; p < pe; ++p) {
+        const double v = L[p];
+        const double term = v > 0.0 ? a.weights[p] * (log(v) + (double)a.e[p] * 0.69314718055994530942)
+                                    : a.weights[p] * -1e20;
+        neum(s, cmp, term);
+    }
+    for (int off = 1; off < 64; off <<= 1) comp_add(s, cmp, __shfl_xor(s, off), __shfl_xor(cmp, off));
+    if (lane == 0) {
+        ws[wave] = s;
+        wc[wave] = cmp;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double S0 = 0.0, C0 = 0.0;
+        for (int k = 0; k < 16; ++k) comp_add(S0, C0, ws[k], wc[k]);
+        a.out[c] = S0 + C0;
+    }
+}
+
+template <typename R> void launch_branch_ctx(const BranchCtxArgs<R> &a, int T, int n_cand, hipStream_t s) {
+    const dim3 g(a.n_tiles, n_cand);
+    if (T == 16) {
+        hipLaunchKernelGGL((k_branch_ctx<R, 16>), g, dim3(256), 0, s, a);
+    } else if (T == 32) {
+        hipLaunchKernelGGL((k_branch_ctx<R, 32>), g, dim3(256), 0, s, a);
+    } else if constexpr (sizeof(R) == 4) {  // FP64 tiles stop at 32 patterns
+        hipLaunchKernelGGL((k_branch_ctx<R, 64>), g, dim3(256), 0, s, a);
+    }
+}
+
+void launch_branch_sum(const BranchSumArgs &a, int n_cand, hipStream_t s) {
+    hipLaunchKernelGGL(k_branch_sum, dim3(n_cand), dim3(1024), 0, s, a);
+}
+
+template void launch_branch_ctx<float>(const BranchCtxArgs<float> &, int, int, hipStream_t);
+template void launch_branch_ctx<double>(const BranchCtxArgs<double> &, int, int, hipStream_t);
 template void launch_pmat<float>(const PmatArgs<float> &, const ProjArgs *, int, hipStream_t);
 template void launch_pmat<double>(const PmatArgs<double> &, const ProjArgs *, int, hipStream_t);
 template void launch_pmat_host<float>(const PmatArgs<float> &, const double *, hipStream_t);

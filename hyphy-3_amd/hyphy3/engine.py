@@ -35,7 +35,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
            "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count",
            "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval", "hy_fel_drive", "hy_fel_run",
-           "hy_fel_counters", "hy_mg94_spectrum")
+           "hy_fel_counters", "hy_mg94_spectrum", "hy_branch_eval")
 
 HY_FEL_FIELDS = 10
 FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt",
@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH):
     L.hy_fel_run.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, _f64p,
                              C.POINTER(C.c_int), C.POINTER(C.c_int64), _f64p]
     L.hy_fel_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int]
+    L.hy_branch_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _i32p]
     L.hy_mg94_spectrum.argtypes = [C.c_void_p, C.c_double, C.c_double, _f64p, C.c_double, C.c_double, _f64p, _f64p,
                                    _f64p, _f64p]
     L.hy_last_checkpoint.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
@@ -427,6 +428,26 @@ class Engine:
         self._check(self.L.hy_mg94_spectrum(self.h, float(alpha), float(beta), _p(th, _f64p), float(delta), float(psi),
                                             _p(lam, _f64p), _p(U, _f64p), _p(V, _f64p), C.byref(mu)))
         return lam, U, V, mu.value
+
+    def branch_eval(self, node, P_cand, precision=HY_PREC_FP64, site=False):
+        """aBSREL's local projection of the branch above ``node`` (hy_branch_eval):
+        lnL for each candidate transition matrix in ``P_cand`` ([n, S, S] or
+        [S, S]) against the partials of the last gradient evaluation in
+        ``precision``.  With ``site``, also the per-pattern products L[n, P]
+        and exponents e[P] (L_c,p = L[c, p] * 2**e[p])."""
+        P = _f64(P_cand)
+        if P.ndim == 2:
+            P = P[None]
+        S = self.S
+        if P.shape[1:] != (S, S):
+            raise ValueError(f"candidate matrices must be {S} x {S}")
+        n = P.shape[0]
+        lnl = np.zeros(n)
+        L = np.zeros((n, self.num_patterns)) if site else None
+        e = np.zeros(self.num_patterns, dtype=np.int32) if site else None
+        self._check(self.L.hy_branch_eval(self.h, precision, int(node), n, _p(P, _f64p), _p(lnl, _f64p),
+                                          _p(L, _f64p) if site else None, _p(e, _i32p) if site else None))
+        return (lnl, L, e) if site else lnl
 
     def fel_counters(self, reset=False):
         """hy_fel_eval counters: candidates, pool hits, warm and cold solves."""

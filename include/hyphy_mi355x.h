@@ -372,6 +372,23 @@ int hy_loglik_from_P_noalloc(hy_engine h, int precision, const double *P, const 
 int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E, double *V, int32_t *G, double *L,
                      int32_t *LE);
 
+/* aBSREL's local projection of one branch: the branch context of
+ * compute_branch_context_matrices (absrel.hpp:723-780; V_mat and D_mat of the
+ * branch above `node` with their log-scale offset) scored by
+ * eval_branch_ll_from_matrices (absrel.hpp:438-455) for n_cand candidate
+ * transition matrices at once, on the GPU and against the partials the last
+ * gradient evaluation on h left resident (hy_eval with HY_WANT_GRAD in
+ * `precision`; not after a checkpointed evaluation).  P_cand[n_cand*S*S]
+ * row-major; out_lnl[n_cand] = sum_p w_p ln(V_p^T P_c D_p) with the partials'
+ * scaling folded in (the reference's offset included; w_p (-1e20) for a
+ * pattern whose product is not positive).  Optional out_site_L[n_cand *
+ * num_patterns] (the products in the partials' scaling) and out_site_e
+ * [num_patterns]: L_c,p = out_site_L[c][p] * 2^out_site_e[p], one exponent
+ * per pattern for every candidate, so candidates can be mixed per pattern
+ * (absrel.hpp:1149-1167). */
+int hy_branch_eval(hy_engine h, int precision, int node, int n_cand, const double *P_cand, double *out_lnl,
+                   double *out_site_L, int32_t *out_site_e);
+
 /* Pattern-tile width used by the pruning kernels (16, 32 or 64 patterns per
  * workgroup; FP64 evaluations run 64 as 32; 0 = automatic, chosen per
  * precision from the pattern count and the device's CU count).  Applies from the next hy_eval; a change of tile

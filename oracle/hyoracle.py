@@ -492,6 +492,56 @@ def busted_expected_weights(prob, code, base, pos, pi, weights_p, omegas, weight
 
 
 # ---------------------------------------------------------------------------
+# aBSREL local projection (one branch, candidate transition matrices)
+# ---------------------------------------------------------------------------
+def _neumaier_acc(s, c, x):
+    """LikelihoodEngine::neumaier_acc (likelihood.hpp:34-43)."""
+    t = s + x
+    if abs(s) >= abs(x):
+        c += (s - t) + x
+    else:
+        c += (x - t) + s
+    return t, c
+
+
+def branch_context(prob, P, pi, node, tile=256):
+    """ABSRELAnalyzer::compute_branch_context_matrices (absrel.hpp:723-780)
+    for the branch above ``node`` (no bridges): the reference's inside-outside
+    workspace per tile of 256 patterns, V_mat = V[node], D_mat = D[node]
+    ([S, P], boosted representation) and log_scale_offset, the sum in pattern
+    order of w_p (logL_p - ln L_p) over patterns with L_p > 0."""
+    S, npat = prob.t.S, prob.num_patterns
+    Vm = np.zeros((S, npat))
+    Dm = np.zeros((S, npat))
+    off = np.zeros(npat)
+    for p0 in range(0, npat, tile):
+        p1 = min(p0 + tile, npat)
+        ws = workspace(prob, P, pi, p0, p1, outside=True)
+        Vm[:, p0:p1] = ws["V"][node]
+        Dm[:, p0:p1] = ws["D"][node]
+        for b in range(p1 - p0):
+            if ws["L"][b] > 0.0:
+                off[p0 + b] = prob.weights[p0 + b] * (ws["logL"][b] - np.log(ws["L"][b]))
+    total = 0.0
+    for p in range(npat):
+        total += off[p]
+    return Vm, Dm, total
+
+
+def branch_ll(prob, P_b, Vm, Dm, log_scale_offset):
+    """ABSRELAnalyzer::eval_branch_ll_from_matrices (absrel.hpp:438-455):
+    offset + sum_p w_p ln(V_p . (P_b D)_p), w_p (-1e20) where the product is
+    not positive, Neumaier-accumulated in pattern order."""
+    PD = np.asarray(P_b, dtype=np.float64) @ Dm
+    ll, c = log_scale_offset, 0.0
+    for p in range(prob.num_patterns):
+        Lp = float(Vm[:, p] @ PD[:, p])
+        term = prob.weights[p] * np.log(Lp) if Lp > 0.0 else prob.weights[p] * -1e20
+        ll, c = _neumaier_acc(ll, c, term)
+    return ll + c
+
+
+# ---------------------------------------------------------------------------
 # Nucleotide GTR stage (S = 4)
 # ---------------------------------------------------------------------------
 GTR_PAIRS = ((0, 1), (0, 3), (1, 2), (1, 3), (2, 3))  # theta order AC, AT, CG, CT, GT; AG = 1
