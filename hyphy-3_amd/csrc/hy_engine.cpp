@@ -159,6 +159,10 @@ struct hy_engine_s {
     // eagerly (graphs bake the weight pointer in).
     DevBuf d_wover;
     const double *wover = nullptr;
+    // caller's pattern weights (hy_set_pattern_weights), in place of the
+    // alignment's for every later evaluation until cleared
+    DevBuf d_wuser;
+    bool wuser = false;
     // hy_loglik_from_P_noalloc: device address of the fixed scratch on this
     // handle's device (looked up at creation; nullptr with the HIP error in
     // scratch_err when that failed) and the host layouts of its fixed
@@ -1424,7 +1428,7 @@ void hy_destroy(hy_engine h) {
                       &h->d_pi_custom, &h->d_Phost, &h->d_sched, &h->d_osched, &h->d_parts, &h->d_blist, &h->d_tips, &h->d_amb, &h->d_weights,
                       &h->d_comp_ptr, &h->d_comp_mat, &h->d_params, &h->d_lam, &h->d_U, &h->d_V, &h->d_mu,
                       &h->d_tau, &h->d_M, &h->d_dmu, &h->d_Z, &h->d_E, &h->d_G, &h->d_Lroot, &h->d_Eroot,
-                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_wl, &h->d_proj, &h->d_wover,
+                      &h->d_site, &h->d_Apart, &h->d_fr, &h->d_wl, &h->d_proj, &h->d_wover, &h->d_wuser,
                       &h->d_sink, &h->d_sinkwl, &h->d_blist_ck, &h->d_bsel, &h->d_gsched, &h->d_gnref,
                       &h->d_gnbr, &h->d_gtips, &h->d_gw, &h->d_gP, &h->d_gpi, &h->d_gD, &h->d_gE, &h->d_gU,
                       &h->d_gH, &h->d_gL, &h->d_gLE, &h->d_gsite, &h->d_gApart, &h->d_gA, &h->d_gres,
@@ -1709,6 +1713,7 @@ int hy_set_problem(hy_engine h, int num_nodes, const int32_t *parent, const int3
     h->codes = std::move(codes);
     h->amb = std::move(amb);
     h->weights.assign(weights, weights + num_patterns);
+    h->wuser = false;  // caller pattern weights belong to the previous problem
     h->T = 0;
     h->branch_class.assign(num_nodes, 0);
     h->n_mat = 1;
@@ -2003,9 +2008,14 @@ int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, do
     int grc = GRAPH_NO_SLOT;
     h->wait_done = false;
     h->publish_done = true;
+    // the caller's pattern weights stand in for the alignment's unless an
+    // internal override (SRV posteriors) is active; both launch eagerly
+    const bool user_w = h->wuser && !h->wover;
+    if (user_w) h->wover = (const double *)h->d_wuser.p;
     if (h->use_graphs && !h->profile && !h->wover) grc = launch_graph(h, precision, grad);
     if (grc < 0) {
         h->publish_done = false;
+        if (user_w) h->wover = nullptr;
         return grc;
     }
     if (grc == GRAPH_NO_SLOT) {
@@ -2016,10 +2026,12 @@ int eval_submit(hy_engine h, int precision, unsigned flags, const double *bl, do
                                                    : run_eval<float>(h, flags, nullptr);
         if (rc) {
             h->publish_done = false;
+            if (user_w) h->wover = nullptr;
             return rc;
         }
     }
     h->publish_done = false;
+    if (user_w) h->wover = nullptr;
     // partials left resident by this evaluation (a replayed graph does not
     // pass through run_eval, which records them when it launches eagerly)
     h->part_ri = precision == HY_PREC_FP64 ? 1 : 0;
@@ -2166,6 +2178,8 @@ int hy_eval_srv(hy_engine h, int precision, unsigned flags, const double *bl, do
     // pattern weights are w_p * post_m(p).
     if (!h) return HY_ERR_ARG;
     TRY(check_eval_args(h, precision, bl, omegas, comp_w, theta, "hy_eval_srv"));
+    if (h->wuser)
+        return fail(h, HY_ERR_STATE, "hy_eval_srv: clear the caller's pattern weights (hy_set_pattern_weights NULL) first");
     if (n_syn < 1 || !syn_rates || !syn_weights) return fail(h, HY_ERR_ARG, "hy_eval_srv: need n_syn >= 1 rates and weights");
     for (int m = 0; m < n_syn; ++m)
         if (!(syn_rates[m] >= 0.0) || !std::isfinite(syn_rates[m]) || !(syn_weights[m] >= 0.0) || !std::isfinite(syn_weights[m]))
@@ -2280,6 +2294,9 @@ int hy_busted_expected_weights(hy_engine h, int precision, const double *bl, dou
     if (!h) return HY_ERR_ARG;
     if (!h->have_problem || !h->have_model)
         return fail(h, HY_ERR_STATE, "hy_busted_expected_weights before hy_set_problem/hy_set_model");
+    if (h->wuser)
+        return fail(h, HY_ERR_STATE,
+                    "hy_busted_expected_weights: clear the caller's pattern weights (hy_set_pattern_weights NULL) first");
     if (h->convention != HY_TIME_RAW)
         return fail(h, HY_ERR_STATE, "hy_busted_expected_weights: the mixture must use HY_TIME_RAW branch times");
     if (test_class < 0 || test_class >= h->n_class)
@@ -2375,8 +2392,8 @@ bool fusable(const std::vector<hy_engine> &hs) {
     if (!h0->group || !h0->group->ctx) return false;
     for (hy_engine h : hs)
         if (!h || h->group != h0->group || !h->have_problem || !h->have_model || h->P == 0 || h->S != h0->S ||
-            h->code_id != h0->code_id || h->convention != h0->convention)
-            return false;
+            h->code_id != h0->code_id || h->convention != h0->convention || h->wuser)
+            return false;  // (caller pattern weights: evaluated engine by engine)
     return true;
 }
 
@@ -3964,6 +3981,26 @@ int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E,
     return HY_OK;
 }
 
+int hy_set_pattern_weights(hy_engine h, const double *weights) {
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem) return fail(h, HY_ERR_STATE, "hy_set_pattern_weights before hy_set_problem");
+    if (!weights) {
+        h->wuser = false;
+        return HY_OK;
+    }
+    for (int p = 0; p < h->P; ++p)
+        if (!(weights[p] >= 0.0) || !std::isfinite(weights[p]))
+            return fail(h, HY_ERR_ARG, "hy_set_pattern_weights: weight %d must be finite and >= 0", p);
+    HIP_TRY(h, hipSetDevice(h->device));
+    // sized for the widest tiling (a later re-tile pads with the same zeros)
+    const size_t cap = ((size_t)h->P + 63) / 64 * 64;
+    std::vector<double> w(cap, 0.0);
+    std::copy(weights, weights + h->P, w.begin());
+    TRY(upload(h, h->d_wuser, w.data(), cap * 8));
+    h->wuser = true;
+    return HY_OK;
+}
+
 int hy_branch_eval(hy_engine h, int precision, int node, int n_cand, const double *P_cand, double *out_lnl,
                    double *out_site_L, int32_t *out_site_e) {
     if (!h) return HY_ERR_ARG;
@@ -4031,7 +4068,7 @@ int hy_branch_eval(hy_engine h, int precision, int node, int n_cand, const doubl
     BranchSumArgs sa{};
     sa.L = (const double *)h->d_bL.p;
     sa.e = (const int32_t *)h->d_be.p;
-    sa.weights = (const double *)h->d_weights.p;
+    sa.weights = (const double *)(h->wuser ? h->d_wuser.p : h->d_weights.p);
     sa.P_pad = h->P_pad;
     sa.n_patterns = h->P;
     sa.out = (double *)h->d_blnl.p;

@@ -35,7 +35,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
            "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count",
            "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval", "hy_fel_drive", "hy_fel_run",
-           "hy_fel_counters", "hy_mg94_spectrum", "hy_branch_eval")
+           "hy_fel_counters", "hy_mg94_spectrum", "hy_branch_eval", "hy_set_pattern_weights")
 
 HY_FEL_FIELDS = 10
 FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt",
@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH):
     L.hy_fel_run.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, _f64p,
                              C.POINTER(C.c_int), C.POINTER(C.c_int64), _f64p]
     L.hy_fel_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int]
+    L.hy_set_pattern_weights.argtypes = [C.c_void_p, _f64p]
     L.hy_branch_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _i32p]
     L.hy_mg94_spectrum.argtypes = [C.c_void_p, C.c_double, C.c_double, _f64p, C.c_double, C.c_double, _f64p, _f64p,
                                    _f64p, _f64p]
@@ -189,6 +190,8 @@ class Engine:
         self.num_nodes, self.num_patterns = n, P
         self.n_mat, self.n_comp = 1, 1
         self._class_sizes = [1]  # hy_set_problem restores the one-component mixture
+        self.weights = w.copy()  # the alignment's pattern weights
+        self.root = int(np.where(self._parent < 0)[0][0])
         self.S = int(S)
 
     def _check_states(self, name, a):
@@ -428,6 +431,17 @@ class Engine:
         self._check(self.L.hy_mg94_spectrum(self.h, float(alpha), float(beta), _p(th, _f64p), float(delta), float(psi),
                                             _p(lam, _f64p), _p(U, _f64p), _p(V, _f64p), C.byref(mu)))
         return lam, U, V, mu.value
+
+    def set_pattern_weights(self, weights=None):
+        """hy_set_pattern_weights: per-pattern weights of every later
+        evaluation (None restores the alignment's)."""
+        if weights is None:
+            self._check(self.L.hy_set_pattern_weights(self.h, None))
+            return
+        w = _f64(weights)
+        if w.shape != (self.num_patterns,):
+            raise ValueError(f"need {self.num_patterns} pattern weights")
+        self._check(self.L.hy_set_pattern_weights(self.h, _p(w, _f64p)))
 
     def branch_eval(self, node, P_cand, precision=HY_PREC_FP64, site=False):
         """aBSREL's local projection of the branch above ``node`` (hy_branch_eval):

@@ -372,6 +372,16 @@ int hy_loglik_from_P_noalloc(hy_engine h, int precision, const double *P, const 
 int hy_read_partials(hy_engine h, int p_start, int p_end, double *D, int32_t *E, double *V, int32_t *G, double *L,
                      int32_t *LE);
 
+/* Pattern weights of every later evaluation on h (lnL and gradients),
+ * weights[num_patterns] >= 0 in place of the alignment's multiplicities, as
+ * the reference's site-level mixtures weight each pattern's gradient by its
+ * class posterior (RELAXAnalyzer::compute_relax_branch_gradients,
+ * relax.hpp:1080-1125; BUSTED's synonymous classes, busted.hpp:1383-1386).
+ * NULL restores the alignment's weights; hy_set_problem clears them.  While
+ * set, hy_eval_srv and hy_busted_expected_weights refuse (they weight
+ * patterns themselves) and gene batches run engine by engine. */
+int hy_set_pattern_weights(hy_engine h, const double *weights);
+
 /* aBSREL's local projection of one branch: the branch context of
  * compute_branch_context_matrices (absrel.hpp:723-780; V_mat and D_mat of the
  * branch above `node` with their log-scale offset) scored by
