@@ -35,7 +35,9 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_inside_outside_from_P", "hy_read_partials", "hy_autotune", "hy_set_tuning", "hy_busted_expected_weights",
            "hy_autotune_batch", "hy_set_sink", "hy_loglik_from_P_noalloc", "hy_graph_count",
            "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval", "hy_fel_drive", "hy_fel_run",
-           "hy_fel_counters", "hy_mg94_spectrum", "hy_branch_eval", "hy_set_pattern_weights")
+           "hy_fel_counters", "hy_mg94_spectrum", "hy_branch_eval", "hy_set_pattern_weights", "hy_network_create",
+           "hy_network_destroy", "hy_network_last_error", "hy_network_set_problem", "hy_network_set_model",
+           "hy_network_set_mixture", "hy_network_set_sink", "hy_network_eval")
 
 HY_FEL_FIELDS = 10
 FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt",

@@ -488,6 +488,54 @@ int hy_graph_count(int *live, int *cap);
 /* Device memory currently held by the engine, in bytes. */
 int hy_device_bytes(hy_engine h, size_t *out);
 
+/* ---------------------------------------------------------------------
+ * Wheatstone-bridged trees (Tree::bridges with WheatstoneBridge hubs,
+ * tree.hpp:32-69; evaluated by LikelihoodEngine's bridge branches,
+ * likelihood.hpp:376-425, 520-569, 623-698, and WheatstoneLikelihood,
+ * wheatstone_likelihood.hpp:21-148, 538-800).  A bridge at hub u with
+ * children {A, B, C} gives u the partial sum_r w_r (P_br(r) (M_x * M_y)) * M_z
+ * over the resolutions (x, y | z) = (A, B | C), (A, C | B), (B, C | A),
+ * M_c = P_c D_c.  The network is evaluated exactly as the per-pattern
+ * mixture of the binary trees that resolve every bridge one way (weights =
+ * products of the chosen w_r; the bridge branch joins the chosen pair to
+ * u), one engine per configuration (3^n_bridges engines).
+ * --------------------------------------------------------------------- */
+#define HY_MAX_BRIDGES 5
+typedef struct hy_network_s *hy_network;
+
+int hy_network_create(int device, hy_network *out);
+void hy_network_destroy(hy_network n);
+const char *hy_network_last_error(hy_network n);
+
+/* The tree and patterns as hy_set_problem; hub[k] is the hub node of
+ * bridge k (not the root) and hub_children[3k..3k+2] its three children
+ * (A, B, C in the reference's order), which must be the hub's children in
+ * the tree.  Resets the mixture to one component. */
+int hy_network_set_problem(hy_network n, int num_nodes, const int32_t *parent, const int32_t *child_ptr,
+                           const int32_t *child_idx, const int32_t *leaf_taxon, int num_taxa, int num_patterns,
+                           const int8_t *states, const uint64_t *masks, const double *weights, int S, int n_bridges,
+                           const int32_t *hub, const int32_t *hub_children);
+int hy_network_set_model(hy_network n, int code_id, const double *pos_freqs, const double *pi);
+/* As hy_set_mixture; bridge_class[n_bridges] (NULL: the hub's class) is the
+ * class of the bridge branches (the reference takes the bridge node's
+ * partition, likelihood.hpp:1283-1287). */
+int hy_network_set_mixture(hy_network n, int n_mat, int n_class, const int32_t *branch_class,
+                           const int32_t *comp_ptr, const int32_t *comp_mat, int time_convention,
+                           const int32_t *bridge_class);
+int hy_network_set_sink(hy_network n, const double *freqs);
+/* As hy_eval, with bridge_weights[3k + r] = w_r and bridge_lengths[3k + r]
+ * = t_r of bridge k (WheatstoneBridge::weights / branch_lengths, in the
+ * time convention of the bridge's class).  Outputs as hy_eval plus
+ * out_grad_bridge_lengths[3 n_bridges] and out_grad_bridge_weights
+ * [3 n_bridges] (JointWheatstoneMG94Gradients::grad_bridge_lengths /
+ * grad_bridge_weights, wheatstone_likelihood.hpp:527-529; each weight
+ * varied on its own).  Any output may be NULL. */
+int hy_network_eval(hy_network n, int precision, unsigned flags, const double *bl, const double *bridge_weights,
+                    const double *bridge_lengths, double alpha, const double *omegas, const double *comp_w,
+                    const double *theta, double delta, double psi, double *out_logL, double *out_site_ll,
+                    double *out_grad_bl, double *out_grad_omega, double *out_grad_w, double *out_grad_theta,
+                    double *out_grad_bridge_lengths, double *out_grad_bridge_weights);
+
 #ifdef __cplusplus
 }
 #endif
