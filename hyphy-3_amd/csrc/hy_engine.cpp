@@ -189,6 +189,9 @@ struct hy_engine_s {
     uint64_t s_model_gen = ~uint64_t(0);
     void *f_pin = nullptr;  // pinned staging: inputs, then outputs
     size_t f_pin_bytes = 0;
+    void *p_pin = nullptr;  // pinned staging of the pool's spectrum solves
+    size_t p_pin_bytes = 0;
+    DevBuf d_pstage;
     std::vector<double> f_bl;  // branch lengths last uploaded to d_fbl
     bool f_ident_ready = false;
     uint64_t model_gen = 0;  // hy_set_problem / hy_set_model calls
@@ -1434,7 +1437,8 @@ void hy_destroy(hy_engine h) {
                       &h->d_gH, &h->d_gL, &h->d_gLE, &h->d_gsite, &h->d_gApart, &h->d_gA, &h->d_gres,
                       &h->d_fZ, &h->d_flam, &h->d_fU, &h->d_fV, &h->d_fmu, &h->d_fstatus, &h->d_fstage, &h->d_fbl,
                       &h->d_fD, &h->d_fE, &h->d_fout, &h->d_fident, &h->d_sprm, &h->d_slam, &h->d_sU, &h->d_sV,
-                      &h->d_smu, &h->d_sZ, &h->d_sstatus, &h->d_bP, &h->d_bL, &h->d_be, &h->d_blnl};
+                      &h->d_smu, &h->d_sZ, &h->d_sstatus, &h->d_bP, &h->d_bL, &h->d_be, &h->d_blnl,
+                      &h->d_pstage};
     for (DevBuf *b : bufs) release(h, *b);
     for (int k = 0; k < 2; ++k) {
         release(h, h->d_Prow[k]);
@@ -1450,6 +1454,7 @@ void hy_destroy(hy_engine h) {
     if (h->h_done) (void)hipHostFree(h->h_done);
     if (h->h_site) (void)hipHostFree(h->h_site);
     if (h->f_pin) (void)hipHostFree(h->f_pin);
+    if (h->p_pin) (void)hipHostFree(h->p_pin);
     if (h->profile)
         for (auto &ev : h->ev) (void)hipEventDestroy(ev);
     for (auto &g : h->graphs) destroy_graph_exec(g.exec);
@@ -3492,29 +3497,21 @@ static int fel_pool_reserve(hy_engine h, int cap) {
     return HY_OK;
 }
 
-int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *beta,
-                const double *theta, double delta, double psi, const double *bl, double *out_ll, double *out_scale) {
-    // FELAnalyzer::analyze_pattern's eval_pattern_lnl (fel.hpp:367-385):
-    // omega = beta / max(alpha, 1e-12), MG94 with alpha = 1 and that beta,
-    // P_b = transition_matrix(max(alpha, 1e-12) t_b), the pattern's lnL
-    // (compute_pattern_log_likelihood, fel.hpp:276-308)
-    const char *who = "hy_fel_eval";
-    if (!h) return HY_ERR_ARG;
-    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "%s before hy_set_problem/hy_set_model", who);
-    if (n_cand < 0 || (n_cand > 0 && (!pattern || !alpha || !beta || !out_ll)) || !theta || !bl)
-        return fail(h, HY_ERR_ARG, "%s: bad candidate arrays", who);
-    if (n_cand == 0) return HY_OK;
-    if (n_cand > (1 << 24)) return fail(h, HY_ERR_ARG, "%s: %d candidates in one call", who, n_cand);
-    const size_t K = (size_t)n_cand;
-    std::vector<double> om(K);
-    for (size_t k = 0; k < K; ++k) {
-        const int p = pattern[k];
-        if (p < 0 || p >= h->P) return fail(h, HY_ERR_ARG, "%s: pattern %d out of range", who, p);
-        if (!(alpha[k] >= 0.0 && beta[k] >= 0.0 && std::isfinite(alpha[k]) && std::isfinite(beta[k])))
-            return fail(h, HY_ERR_ARG, "%s: candidate %zu has alpha %g, beta %g", who, k, alpha[k], beta[k]);
-        om[k] = beta[k] / std::max(alpha[k], 1e-12);
-        if (!std::isfinite(om[k])) return fail(h, HY_ERR_ARG, "%s: candidate %zu has omega %g", who, k, om[k]);
-    }
+// Spectra of a list of omegas (MG94 at alpha = 1 with the given theta,
+// delta, psi) from the handle's pool, shared by the per-site refit entry
+// points: pooled omegas are reused, every other distinct omega is solved in
+// one k_model launch on the stream, refining the eigenvectors of the pooled
+// omega nearest in log(omega).  pool_commit() enters the new spectra once
+// the stream has finished.
+struct PoolSolve {
+    std::vector<int32_t> slot;    // per requested omega
+    std::vector<double> miss_om;  // omegas solved by this call
+    std::vector<int32_t> oslot;   // and their slots
+    uint64_t tick = 0;
+};
+
+static int pool_solve(hy_engine h, const char *who, const double *theta, double delta, double psi, const double *bl,
+                      const std::vector<double> &om, PoolSolve &ps) {
     for (int n = 0; n < h->num_nodes; ++n)
         if (n != h->root && !(bl[n] >= 0.0 && std::isfinite(bl[n])))
             return fail(h, HY_ERR_ARG, "%s: branch length %d = %g", who, n, bl[n]);
@@ -3548,10 +3545,13 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
         std::memcpy(f.key, key, sizeof key);
         f.keyed = true;
     }
-    const uint64_t tick = ++f.tick;
+    const uint64_t tick = ps.tick = ++f.tick;
     // distinct omegas of the call: pooled ones are hits, the others are solved
-    std::vector<int32_t> spec(K, -1);
-    std::vector<double> miss_om;
+    const size_t K = om.size();
+    std::vector<int32_t> &spec = ps.slot;
+    spec.assign(K, -1);
+    std::vector<double> &miss_om = ps.miss_om;
+    miss_om.clear();
     std::unordered_map<uint64_t, int> call;  // omega bits -> slot, or -(1 + miss index)
     int n_hit = 0;
     for (size_t k = 0; k < K; ++k) {
@@ -3577,7 +3577,9 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
     if (n_hit + n_miss > f.cap) TRY(fel_pool_reserve(h, std::max({n_hit + n_miss, 2 * f.cap, 64})));
     // output slots of the misses: free slots first, then the least recently
     // used; never a slot this call reads
-    std::vector<int32_t> oslot(n_miss), zsrc(n_miss, -1);
+    std::vector<int32_t> &oslot = ps.oslot;
+    oslot.assign(n_miss, 0);
+    std::vector<int32_t> zsrc(n_miss, -1);
     if (n_miss > 0) {
         std::vector<int> cand;
         cand.reserve(f.cap);
@@ -3614,7 +3616,8 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
     }
     for (size_t k = 0; k < K; ++k)
         if (spec[k] < 0) spec[k] = oslot[-1 - spec[k]];
-    // staging: [params][alpha] doubles, then [oslot][zsrc][spec][pattern] ints
+    if (n_miss == 0) return HY_OK;
+    // staging of the solve: [params] doubles, then [oslot][zsrc] ints
     ParamLayout pl{};
     pl.n_nodes = 0;
     pl.n_mat = n_miss;
@@ -3626,9 +3629,117 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
     pl.off_theta = pl.off_w + 1;
     pl.off_delta = pl.off_theta + 5;
     pl.off_psi = pl.off_delta + 1;
-    pl.total = pl.off_psi + 1;
-    const size_t n_dbl = (size_t)pl.total + K, n_i32 = 2 * (size_t)n_miss + 2 * K;
-    const size_t in_bytes = n_dbl * 8 + n_i32 * 4, out_bytes = 3 * K * 8;
+    pl.off_seq = pl.off_psi + 1;
+    pl.total = pl.off_seq + 1;
+    const size_t bytes = (size_t)pl.total * 8 + 2 * (size_t)n_miss * 4;
+    if (h->p_pin_bytes < bytes) {
+        if (h->p_pin) (void)hipHostFree(h->p_pin);
+        h->p_pin = nullptr;
+        h->p_pin_bytes = 0;
+        HIP_TRY(h, hipHostMalloc(&h->p_pin, bytes * 2, hipHostMallocDefault));
+        h->p_pin_bytes = bytes * 2;
+    }
+    double *pd = (double *)h->p_pin;
+    std::fill(pd, pd + pl.total, 0.0);
+    pd[pl.off_alpha] = 1.0;
+    std::copy(miss_om.begin(), miss_om.end(), pd + pl.off_omega);
+    pd[pl.off_w] = 1.0;
+    std::copy(theta, theta + 5, pd + pl.off_theta);
+    pd[pl.off_delta] = delta;
+    pd[pl.off_psi] = psi;
+    int32_t *pi32 = (int32_t *)(pd + pl.total);
+    std::copy(oslot.begin(), oslot.end(), pi32);
+    std::copy(zsrc.begin(), zsrc.end(), pi32 + n_miss);
+    TRY(ensure(h, h->d_pstage, bytes));
+    HIP_TRY(h, hipMemcpyAsync(h->d_pstage.p, h->p_pin, bytes, hipMemcpyHostToDevice, s));
+    const double *dd = (const double *)h->d_pstage.p;
+    const int32_t *di = (const int32_t *)(dd + pl.total);
+    ModelArgs ma{};
+    ma.qdesc = (const int32_t *)h->d_qdesc.p;
+    ma.nd = (const int8_t *)h->d_nd.p;
+    ma.dpos = (const int8_t *)h->d_dpos.p;
+    ma.dfrom = (const int8_t *)h->d_dfrom.p;
+    ma.dto = (const int8_t *)h->d_dto.p;
+    ma.syn = (const uint8_t *)h->d_syn.p;
+    ma.pos = (const double *)h->d_pos.p;
+    ma.pi = (const double *)h->d_pi.p;
+    ma.S = h->S;
+    ma.params = dd;
+    ma.pl = pl;
+    ma.lam = (double *)h->d_flam.p;
+    ma.U = (double *)h->d_fU.p;
+    ma.V = (double *)h->d_fV.p;
+    ma.mu = (double *)h->d_fmu.p;
+    ma.Z = (double *)h->d_fZ.p;
+    ma.warm = 1;
+    ma.oslot = di;
+    ma.zsrc = di + n_miss;
+    ma.status = (int *)h->d_fstatus.p;
+    ma.tol = 1e-14;
+    launch_model(ma, n_miss, s);
+    HIP_TRY(h, hipGetLastError());
+    return HY_OK;
+}
+
+// After the stream has finished: the spectra solved by pool_solve join the
+// pool; one whose eigensolver failed is dropped and reported.
+static int pool_commit(hy_engine h, const char *who, const PoolSolve &ps) {
+    FelPool &f = h->fel;
+    const int n_miss = (int)ps.miss_om.size();
+    if (n_miss == 0) return HY_OK;
+    std::vector<int> st(f.cap);
+    HIP_TRY(h, hipMemcpy(st.data(), h->d_fstatus.p, (size_t)f.cap * 4, hipMemcpyDeviceToHost));
+    int bad = -1;
+    for (int i = 0; i < n_miss; ++i) {
+        const int sl = ps.oslot[i];
+        if (st[sl] < 0) {
+            bad = i;
+            continue;
+        }
+        f.omega[sl] = ps.miss_om[i];
+        f.used[sl] = ps.tick;
+        uint64_t bits;
+        std::memcpy(&bits, &ps.miss_om[i], 8);
+        f.slot_of[bits] = sl;
+        f.by_omega[ps.miss_om[i]] = sl;
+        if (st[sl] >= 100)
+            ++f.n_warm;
+        else
+            ++f.n_cold;
+    }
+    if (bad >= 0) return fail(h, HY_ERR_NUMERIC, "%s: eigensolver did not converge for omega %g", who, ps.miss_om[bad]);
+    return HY_OK;
+}
+
+int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *beta,
+                const double *theta, double delta, double psi, const double *bl, double *out_ll, double *out_scale) {
+    // FELAnalyzer::analyze_pattern's eval_pattern_lnl (fel.hpp:367-385):
+    // omega = beta / max(alpha, 1e-12), MG94 with alpha = 1 and that beta,
+    // P_b = transition_matrix(max(alpha, 1e-12) t_b), the pattern's lnL
+    // (compute_pattern_log_likelihood, fel.hpp:276-308)
+    const char *who = "hy_fel_eval";
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "%s before hy_set_problem/hy_set_model", who);
+    if (n_cand < 0 || (n_cand > 0 && (!pattern || !alpha || !beta || !out_ll)) || !theta || !bl)
+        return fail(h, HY_ERR_ARG, "%s: bad candidate arrays", who);
+    if (n_cand == 0) return HY_OK;
+    if (n_cand > (1 << 24)) return fail(h, HY_ERR_ARG, "%s: %d candidates in one call", who, n_cand);
+    const size_t K = (size_t)n_cand;
+    std::vector<double> om(K);
+    for (size_t k = 0; k < K; ++k) {
+        const int p = pattern[k];
+        if (p < 0 || p >= h->P) return fail(h, HY_ERR_ARG, "%s: pattern %d out of range", who, p);
+        if (!(alpha[k] >= 0.0 && beta[k] >= 0.0 && std::isfinite(alpha[k]) && std::isfinite(beta[k])))
+            return fail(h, HY_ERR_ARG, "%s: candidate %zu has alpha %g, beta %g", who, k, alpha[k], beta[k]);
+        om[k] = beta[k] / std::max(alpha[k], 1e-12);
+        if (!std::isfinite(om[k])) return fail(h, HY_ERR_ARG, "%s: candidate %zu has omega %g", who, k, om[k]);
+    }
+    PoolSolve ps;
+    TRY(pool_solve(h, who, theta, delta, psi, bl, om, ps));
+    FelPool &f = h->fel;
+    hipStream_t s = h->stream;
+    // staging: [alpha] doubles, then [spec][pattern] ints
+    const size_t in_bytes = K * 8 + 2 * K * 4, out_bytes = 3 * K * 8;
     const size_t pin_need = std::max(in_bytes, out_bytes);
     if (h->f_pin_bytes < pin_need) {
         if (h->f_pin) (void)hipHostFree(h->f_pin);
@@ -3638,50 +3749,17 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
         h->f_pin_bytes = pin_need * 2;
     }
     double *pd = (double *)h->f_pin;
-    pd[pl.off_alpha] = 1.0;
-    std::copy(miss_om.begin(), miss_om.end(), pd + pl.off_omega);
-    pd[pl.off_w] = 1.0;
-    std::copy(theta, theta + 5, pd + pl.off_theta);
-    pd[pl.off_delta] = delta;
-    pd[pl.off_psi] = psi;
-    std::copy(alpha, alpha + K, pd + pl.total);
-    int32_t *pi32 = (int32_t *)(pd + n_dbl);
-    std::copy(oslot.begin(), oslot.end(), pi32);
-    std::copy(zsrc.begin(), zsrc.end(), pi32 + n_miss);
-    std::copy(spec.begin(), spec.end(), pi32 + 2 * n_miss);
-    std::copy(pattern, pattern + K, pi32 + 2 * n_miss + K);
+    std::copy(alpha, alpha + K, pd);
+    int32_t *pi32 = (int32_t *)(pd + K);
+    std::copy(ps.slot.begin(), ps.slot.end(), pi32);
+    std::copy(pattern, pattern + K, pi32 + K);
     TRY(ensure(h, h->d_fstage, in_bytes));
     TRY(ensure(h, h->d_fD, K * std::max(h->n_int, 1) * 64 * 8));
     TRY(ensure(h, h->d_fE, K * std::max(h->n_int, 1) * 4));
     TRY(ensure(h, h->d_fout, out_bytes));
     HIP_TRY(h, hipMemcpyAsync(h->d_fstage.p, h->f_pin, in_bytes, hipMemcpyHostToDevice, s));
     const double *dd = (const double *)h->d_fstage.p;
-    const int32_t *di = (const int32_t *)(dd + n_dbl);
-    if (n_miss > 0) {
-        ModelArgs ma{};
-        ma.qdesc = (const int32_t *)h->d_qdesc.p;
-        ma.nd = (const int8_t *)h->d_nd.p;
-        ma.dpos = (const int8_t *)h->d_dpos.p;
-        ma.dfrom = (const int8_t *)h->d_dfrom.p;
-        ma.dto = (const int8_t *)h->d_dto.p;
-        ma.syn = (const uint8_t *)h->d_syn.p;
-        ma.pos = (const double *)h->d_pos.p;
-        ma.pi = (const double *)h->d_pi.p;
-        ma.S = h->S;
-        ma.params = dd;
-        ma.pl = pl;
-        ma.lam = (double *)h->d_flam.p;
-        ma.U = (double *)h->d_fU.p;
-        ma.V = (double *)h->d_fV.p;
-        ma.mu = (double *)h->d_fmu.p;
-        ma.Z = (double *)h->d_fZ.p;
-        ma.warm = 1;
-        ma.oslot = di;
-        ma.zsrc = di + n_miss;
-        ma.status = (int *)h->d_fstatus.p;
-        ma.tol = 1e-14;
-        launch_model(ma, n_miss, s);
-    }
+    const int32_t *di = (const int32_t *)(dd + K);
     FelArgs fa{};
     fa.sched = (const int32_t *)h->d_gsched.p;
     fa.n_sched = h->g_n_sched;
@@ -3690,9 +3768,9 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
     fa.tips = (const uint16_t *)h->d_gtips.p;
     fa.tip_stride = h->g_P_pad;
     fa.amb = (const uint64_t *)h->d_amb.p;
-    fa.pat = di + 2 * n_miss + K;
-    fa.alpha = dd + pl.total;
-    fa.spec = di + 2 * n_miss;
+    fa.pat = di + K;
+    fa.alpha = dd;
+    fa.spec = di;
     fa.lam = (const double *)h->d_flam.p;
     fa.U = (const double *)h->d_fU.p;
     fa.V = (const double *)h->d_fV.p;
@@ -3710,37 +3788,122 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
     HIP_TRY(h, hipGetLastError());
     HIP_TRY(h, hipMemcpyAsync(h->f_pin, h->d_fout.p, out_bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(h, hipStreamSynchronize(s));
-    const double *res = (const double *)h->f_pin;
-    for (int i = 0; i < n_miss; ++i) {  // the new spectra join the pool
-        const int sl = oslot[i];
-        f.omega[sl] = miss_om[i];
-        f.used[sl] = tick;
-        uint64_t bits;
-        std::memcpy(&bits, &miss_om[i], 8);
-        f.slot_of[bits] = sl;
-        f.by_omega[miss_om[i]] = sl;
-    }
+    TRY(pool_commit(h, who, ps));
     f.n_cand += (int64_t)K;
-    {
-        std::unordered_map<int, double> st;  // status of each new spectrum
-        for (size_t k = 0; k < K; ++k) st[spec[k]] = res[2 * K + k];
-        for (int i = 0; i < n_miss; ++i) {
-            const double v = st[oslot[i]];
-            if (v >= 100.0)
-                ++f.n_warm;
-            else
-                ++f.n_cold;
-        }
-    }
-    int bad = -1;
-    for (size_t k = 0; k < K; ++k)
-        if (res[2 * K + k] < 0.0) {
-            f.drop(spec[k]);
-            bad = (int)k;
-        }
-    if (bad >= 0) return fail(h, HY_ERR_NUMERIC, "%s: eigensolver did not converge for omega %g", who, om[bad]);
+    const double *res = (const double *)h->f_pin;
     std::copy(res, res + K, out_ll);
     if (out_scale) std::copy(res + K, res + 2 * K, out_scale);
+    return HY_OK;
+}
+
+int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *omega1,
+                 const double *p1, const double *beta_plus, const double *beta_bg, const uint8_t *is_test,
+                 const double *theta, double delta, double psi, const double *bl, double *out_ll) {
+    // MEMEAnalyzer::analyze_pattern's eval_pattern_meme_lnl (meme.hpp:219-277):
+    // with a = max(alpha, 1e-12), MG94(alpha = a, beta = a omega1),
+    // MG94(a, beta_plus), MG94(a, beta_bg) are a times the alpha = 1
+    // generators at omega1, beta_plus / a, beta_bg / a, so their spectra come
+    // from the pool and their transition matrices at a t_b; test branches
+    // P = p1 P_1 + (1 - p1) P_+, background branches P_bg
+    const char *who = "hy_meme_eval";
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "%s before hy_set_problem/hy_set_model", who);
+    if (n_cand < 0 || (n_cand > 0 && (!pattern || !alpha || !omega1 || !p1 || !beta_plus || !out_ll)) || !theta ||
+        !bl)
+        return fail(h, HY_ERR_ARG, "%s: bad candidate arrays", who);
+    if (n_cand == 0) return HY_OK;
+    if (n_cand > (1 << 22)) return fail(h, HY_ERR_ARG, "%s: %d candidates in one call", who, n_cand);
+    bool has_bg = false;
+    if (is_test)
+        for (int n = 0; n < h->num_nodes; ++n)
+            if (n != h->root && !is_test[n]) has_bg = true;
+    if (has_bg && !beta_bg) return fail(h, HY_ERR_ARG, "%s: background branches need beta_bg", who);
+    const size_t K = (size_t)n_cand;
+    const int NS = has_bg ? 3 : 2;
+    std::vector<double> om(K * NS);
+    for (size_t k = 0; k < K; ++k) {
+        const int p = pattern[k];
+        if (p < 0 || p >= h->P) return fail(h, HY_ERR_ARG, "%s: pattern %d out of range", who, p);
+        const double bb = has_bg ? beta_bg[k] : 0.0;
+        if (!(alpha[k] >= 0.0 && omega1[k] >= 0.0 && beta_plus[k] >= 0.0 && bb >= 0.0 && p1[k] >= 0.0 &&
+              p1[k] <= 1.0 && std::isfinite(alpha[k]) && std::isfinite(omega1[k]) && std::isfinite(beta_plus[k]) &&
+              std::isfinite(bb)))
+            return fail(h, HY_ERR_ARG, "%s: candidate %zu out of range", who, k);
+        const double a = std::max(alpha[k], 1e-12);
+        om[k * NS] = omega1[k];
+        om[k * NS + 1] = beta_plus[k] / a;
+        if (has_bg) om[k * NS + 2] = bb / a;
+        for (int j = 0; j < NS; ++j)
+            if (!std::isfinite(om[k * NS + j])) return fail(h, HY_ERR_ARG, "%s: candidate %zu: omega overflow", who, k);
+    }
+    PoolSolve ps;
+    TRY(pool_solve(h, who, theta, delta, psi, bl, om, ps));
+    FelPool &f = h->fel;
+    hipStream_t s = h->stream;
+    // staging: [alpha][p1] doubles, then [spec A][spec B][spec C][pattern] ints, then per-node test flags
+    const size_t n_ext = (size_t)h->n_ext;
+    const size_t in_bytes = 2 * K * 8 + 4 * K * 4 + n_ext, out_bytes = K * 8;
+    const size_t pin_need = std::max(in_bytes, out_bytes);
+    if (h->f_pin_bytes < pin_need) {
+        if (h->f_pin) (void)hipHostFree(h->f_pin);
+        h->f_pin = nullptr;
+        h->f_pin_bytes = 0;
+        HIP_TRY(h, hipHostMalloc(&h->f_pin, pin_need * 2, hipHostMallocDefault));
+        h->f_pin_bytes = pin_need * 2;
+    }
+    double *pd = (double *)h->f_pin;
+    for (size_t k = 0; k < K; ++k) {
+        pd[k] = std::max(alpha[k], 1e-12);
+        pd[K + k] = p1[k];
+    }
+    int32_t *pi32 = (int32_t *)(pd + 2 * K);
+    for (size_t k = 0; k < K; ++k) {
+        pi32[k] = ps.slot[k * NS];
+        pi32[K + k] = ps.slot[k * NS + 1];
+        pi32[2 * K + k] = has_bg ? ps.slot[k * NS + 2] : ps.slot[k * NS];
+        pi32[3 * K + k] = pattern[k];
+    }
+    uint8_t *tf = (uint8_t *)(pi32 + 4 * K);
+    for (size_t n = 0; n < n_ext; ++n)  // extended (identity) nodes are never read as test or background
+        tf[n] = (n < (size_t)h->num_nodes && is_test) ? (is_test[n] ? 1 : 0) : 1;
+    TRY(ensure(h, h->d_fstage, in_bytes));
+    TRY(ensure(h, h->d_fD, K * std::max(h->n_int, 1) * 64 * 8));
+    TRY(ensure(h, h->d_fE, K * std::max(h->n_int, 1) * 4));
+    TRY(ensure(h, h->d_fout, out_bytes));
+    HIP_TRY(h, hipMemcpyAsync(h->d_fstage.p, h->f_pin, in_bytes, hipMemcpyHostToDevice, s));
+    const double *dd = (const double *)h->d_fstage.p;
+    const int32_t *di = (const int32_t *)(dd + 2 * K);
+    MemeArgs ma{};
+    ma.sched = (const int32_t *)h->d_gsched.p;
+    ma.n_sched = h->g_n_sched;
+    ma.nref = (const int32_t *)h->d_gnref.p;
+    ma.ident = (const uint8_t *)h->d_fident.p;
+    ma.tips = (const uint16_t *)h->d_gtips.p;
+    ma.tip_stride = h->g_P_pad;
+    ma.amb = (const uint64_t *)h->d_amb.p;
+    ma.pat = di + 3 * K;
+    ma.alpha = dd;
+    ma.p1 = dd + K;
+    ma.specA = di;
+    ma.specB = di + K;
+    ma.specC = di + 2 * K;
+    ma.is_test = (const uint8_t *)(di + 4 * K);
+    ma.lam = (const double *)h->d_flam.p;
+    ma.Z = (const double *)h->d_fZ.p;
+    ma.bl = (const double *)h->d_fbl.p;
+    ma.pi = (const double *)h->d_pi.p;
+    ma.S = h->S;
+    ma.n_int = std::max(h->n_int, 1);
+    ma.D = (double *)h->d_fD.p;
+    ma.E = (int32_t *)h->d_fE.p;
+    ma.out = (double *)h->d_fout.p;
+    launch_meme(ma, n_cand, s);
+    HIP_TRY(h, hipGetLastError());
+    HIP_TRY(h, hipMemcpyAsync(h->f_pin, h->d_fout.p, out_bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(h, hipStreamSynchronize(s));
+    TRY(pool_commit(h, who, ps));
+    f.n_cand += (int64_t)K;
+    std::copy((const double *)h->f_pin, (const double *)h->f_pin + K, out_ll);
     return HY_OK;
 }
 

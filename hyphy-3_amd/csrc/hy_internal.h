@@ -173,6 +173,27 @@ struct FelArgs {
 };
 void launch_fel(const FelArgs &a, int n_cand, hipStream_t s);
 
+// MEME's per-site candidates (hy_meme_eval): test branches mix the spectra
+// A (weight p1) and B (1 - p1), background branches use C.
+struct MemeArgs {
+    const int32_t *sched;
+    int n_sched;
+    const int32_t *nref, *pat;
+    const uint8_t *ident, *is_test;  // [n_ext]
+    const uint16_t *tips;
+    int tip_stride;
+    const uint64_t *amb;
+    const double *alpha, *p1;        // [n_cand] (alpha already >= 1e-12)
+    const int32_t *specA, *specB, *specC;
+    const double *lam, *Z;           // spectrum pool: [slot][64], [slot][64][64] eigenvectors of the symmetrised generator
+    const double *bl, *pi;
+    int S, n_int;
+    double *D;
+    int32_t *E;
+    double *out;                     // [n_cand] pattern lnL
+};
+void launch_meme(const MemeArgs &a, int n_cand, hipStream_t s);
+
 template <typename R>
 struct PmatArgs {
     const int32_t *blist;  // [n_branch][BR_STRIDE]

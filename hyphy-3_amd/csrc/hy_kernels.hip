@@ -3075,6 +3075,175 @@ void launch_fel(const FelArgs &a, int n_cand, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// k_meme: MEME's per-site candidates (eval_pattern_meme_lnl, meme.hpp:219-277;
+// compute_pattern_log_likelihood, fel.hpp:276-308).  One workgroup per
+// candidate walks its pattern through the tree like k_fel, with up to three
+// spectra: a test branch contributes x = (p1 max(P_A, 0) + (1 - p1)
+// max(P_B, 0)) d, a background branch max(P_C, 0) d, at time a t.  Each
+// spectrum is held in LDS by its eigenvectors Z of the symmetrised generator
+// (P_ij = pi_i^-1/2 pi_j^1/2 sum_k Z_ik e^{l_k a t} Z_jk, k_model's U and V
+// factored), so three fit beside the P tile.  FP64 throughout.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_meme(MemeArgs a) {
+    const int k = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
+    __shared__ double Zs[3][64 * LDA];
+    __shared__ double Pt[64 * LDA];
+    __shared__ double ex[2][64], lamc[3][64], dv[64], acc[64], sq[64], isq[64], red[4];
+    __shared__ int cols[64];
+    __shared__ int ncol;
+    const int p = a.pat[k];
+    const int sl[3] = {a.specA[k], a.specB[k], a.specC[k]};
+    const double at = a.alpha[k], w1 = a.p1[k], w2 = 1.0 - w1;
+    for (int c = 0; c < 3; ++c) {
+        const double *Z = a.Z + (size_t)sl[c] * 4096;
+#pragma unroll 4
+        for (int q = 0; q < 16; ++q) {
+            const int idx = tid + 256 * q;
+            Zs[c][(idx >> 6) * LDA + (idx & 63)] = Z[idx];
+        }
+        if (tid < 64) lamc[c][tid] = a.lam[(size_t)sl[c] * 64 + tid];
+    }
+    if (tid < 64) {
+        // the square roots k_model factored into U and V (rate_matrix.hpp:211-246)
+        const double pv = a.pi[tid];
+        const double v = sqrt(pv > 1e-12 ? pv : 1e-12);
+        sq[tid] = tid < a.S ? v : 0.0;
+        isq[tid] = tid < a.S ? 1.0 / v : 0.0;
+    }
+    const uint64_t all = (a.S >= 64) ? ~0ull : ((1ull << a.S) - 1ull);
+    double *Dk = a.D + (size_t)k * a.n_int * 64;
+    int32_t *Ek = a.E + (size_t)k * a.n_int;
+    const int row = tid >> 2, part = tid & 3;
+    int eacc = 0;
+    for (int q = 0; q < a.n_sched; ++q) {
+        const int32_t *e = a.sched + (size_t)q * GTR_STRIDE;
+        const int slot = e[1], nc = e[2];
+        if (tid < 64) acc[tid] = 1.0;
+        eacc = 0;
+        for (int c = 0; c < nc; ++c) {
+            const int cn = e[3 + c], ref = a.nref[cn], kind = ref >> 28, idx = ref & REF_MASK;
+            const bool test = a.is_test[cn] != 0;
+            // components of this edge: (spectrum, weight)
+            const int n_comp = test ? 2 : 1;
+            const int cs0 = test ? 0 : 2;
+            const double cw0 = test ? w1 : 1.0;
+            __syncthreads();  // previous child's reads of dv / Pt / cols / ex are done
+            uint64_t mask = 0;
+            if (kind == KIND_TIP) {
+                const unsigned code = a.tips[(size_t)idx * a.tip_stride + p];
+                mask = code == TIP_ONES ? all : (code & TIP_AMB) ? (a.amb[code & 0x7FFFu] & all) : (1ull << code);
+            }
+            const int pc = kind == KIND_TIP ? __popcll(mask) : 64;
+            if (tid < 64) {
+                dv[tid] = kind == KIND_TIP ? (double)((mask >> tid) & 1ull) : a.D[((size_t)k * a.n_int + idx) * 64 + tid];
+                ex[0][tid] = exp(lamc[cs0][tid] * at * a.bl[cn]);
+                ex[1][tid] = exp(lamc[1][tid] * at * a.bl[cn]);
+            }
+            if (tid == 0) {
+                if (kind != KIND_TIP) eacc += Ek[idx];
+                int n = 0;
+                for (uint64_t mm = mask; mm && n < FEL_MAX_COLUMNS; mm &= mm - 1) cols[n++] = __ffsll((long long)mm) - 1;
+                ncol = n;
+            }
+            __syncthreads();
+            if (a.ident[cn]) {
+                if (tid < 64) acc[tid] *= dv[tid];
+                continue;
+            }
+            if (kind == KIND_TIP && pc <= FEL_MAX_COLUMNS) {
+                // x_i = sum over the tip's codons s of sum_comp w max(P_is, 0)
+                double x = 0.0;
+                for (int j = 0; j < ncol; ++j) {
+                    const int sc = cols[j];
+                    for (int m = 0; m < n_comp; ++m) {
+                        const int cs = m == 0 ? cs0 : 1;
+                        const double *Zc = Zs[cs];
+                        double v = 0.0;
+#pragma unroll
+                        for (int kk = 0; kk < 16; ++kk) {
+                            const int kx = 16 * part + kk;
+                            v += Zc[row * LDA + kx] * ex[m][kx] * Zc[sc * LDA + kx];
+                        }
+                        v += __shfl_xor(v, 1);
+                        v += __shfl_xor(v, 2);
+                        v *= isq[row] * sq[sc];
+                        x += (m == 0 ? cw0 : w2) * (v > 0.0 ? v : 0.0);
+                    }
+                }
+                if (part == 0) acc[row] *= x;
+                continue;
+            }
+            // P = sum_comp w max(P_comp, 0) in Pt (rows of wave w: 16w .. 16w + 15)
+            f64x4 pmix[4];
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) pmix[qq] = splat<f64x4>(0.0);
+            for (int m = 0; m < n_comp; ++m) {
+                const int cs = m == 0 ? cs0 : 1;
+                const double cw = m == 0 ? cw0 : w2;
+                const double *Zc = Zs[cs];
+                f64x4 pacc[4];
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) pacc[qq] = splat<f64x4>(0.0);
+#pragma unroll 4
+                for (int kk = 0; kk < 16; ++kk) {
+                    const int i = 16 * w + lr, kx = 4 * kk + lg;
+                    const double av = Zc[i * LDA + kx] * ex[m][kx];
+#pragma unroll
+                    for (int qq = 0; qq < 4; ++qq) pacc[qq] = MF<double>::mma(av, Zc[(16 * qq + lr) * LDA + kx], pacc[qq]);
+                }
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = 16 * w + MF<double>::crow(lg, r), j = 16 * qq + lr;
+                        const double v = pacc[qq][r] * isq[i] * sq[j];
+                        pmix[qq][r] += cw * (v > 0.0 ? v : 0.0);
+                    }
+            }
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Pt[(16 * w + MF<double>::crow(lg, r)) * LDA + 16 * qq + lr] = pmix[qq][r];
+            __syncthreads();
+            {
+                double sacc = 0.0;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) sacc += Pt[row * LDA + 16 * part + j] * dv[16 * part + j];
+                sacc += __shfl_xor(sacc, 1);
+                sacc += __shfl_xor(sacc, 2);
+                if (part == 0) acc[row] *= sacc;
+            }
+        }
+        __syncthreads();
+        // normalise the node's partial by a power of two (wave 0)
+        if (tid < 64) {
+            double m = acc[tid];
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+            double v = acc[tid];
+            int ke = 0;
+            if (m > 0.0) {
+                ke = frexp_e(m);
+                v = ldexp(v, -ke);
+            }
+            acc[tid] = v;
+            Dk[(size_t)slot * 64 + tid] = v;
+            if (tid == 0) Ek[slot] = eacc + ke;
+            if (tid == 0) red[0] = (double)(eacc + ke);
+        }
+        __syncthreads();
+    }
+    if (tid < 64) {
+        double l = a.pi[tid] * acc[tid];
+        for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off);
+        if (tid == 0) a.out[k] = l > 0.0 ? log(l) + red[0] * 0.69314718055994530942 : -1e20;
+    }
+}
+
+void launch_meme(const MemeArgs &a, int n_cand, hipStream_t s) {
+    if (n_cand > 0) hipLaunchKernelGGL(k_meme, dim3(n_cand), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
 // Fixed scratch of the allocation-free entry point (hy_loglik_from_P_noalloc):
 // one array of NOALLOC_SCRATCH_BYTES per device that loads this library, held
 // until the process ends.  Host code never passes the array's name to a

@@ -37,7 +37,7 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval", "hy_fel_drive", "hy_fel_run",
            "hy_fel_counters", "hy_mg94_spectrum", "hy_branch_eval", "hy_set_pattern_weights", "hy_network_create",
            "hy_network_destroy", "hy_network_last_error", "hy_network_set_problem", "hy_network_set_model",
-           "hy_network_set_mixture", "hy_network_set_sink", "hy_network_eval")
+           "hy_network_set_mixture", "hy_network_set_sink", "hy_network_eval", "hy_meme_eval")
 
 HY_FEL_FIELDS = 10
 FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt",
@@ -102,6 +102,8 @@ def load_library(path: str = LIB_PATH):
                              C.POINTER(C.c_int), C.POINTER(C.c_int64), _f64p]
     L.hy_fel_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int]
     L.hy_set_pattern_weights.argtypes = [C.c_void_p, _f64p]
+    L.hy_meme_eval.argtypes = [C.c_void_p, C.c_int, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _u8p, _f64p,
+                               C.c_double, C.c_double, _f64p, _f64p]
     L.hy_branch_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _i32p]
     L.hy_mg94_spectrum.argtypes = [C.c_void_p, C.c_double, C.c_double, _f64p, C.c_double, C.c_double, _f64p, _f64p,
                                    _f64p, _f64p]
@@ -464,6 +466,21 @@ class Engine:
         self._check(self.L.hy_branch_eval(self.h, precision, int(node), n, _p(P, _f64p), _p(lnl, _f64p),
                                           _p(L, _f64p) if site else None, _p(e, _i32p) if site else None))
         return (lnl, L, e) if site else lnl
+
+    def meme_eval(self, pattern, alpha, omega1, p1, beta_plus, theta, bl, beta_bg=None, is_test=None, delta=0.0,
+                  psi=0.0):
+        """hy_meme_eval: MEME's per-site lnL for each candidate
+        (pattern[k], alpha[k], omega1[k], p1[k], beta_plus[k][, beta_bg[k]])."""
+        pat = np.ascontiguousarray(pattern, dtype=np.int32)
+        n = pat.size
+        arrs = [_f64(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,))) for x in (alpha, omega1, p1, beta_plus)]
+        bb = None if beta_bg is None else _f64(np.broadcast_to(np.asarray(beta_bg, dtype=np.float64), (n,)))
+        it = None if is_test is None else np.ascontiguousarray(is_test, dtype=np.uint8)
+        out = np.zeros(n)
+        self._check(self.L.hy_meme_eval(self.h, n, _p(pat, _i32p), *[_p(a, _f64p) for a in arrs], _p(bb, _f64p),
+                                        _p(it, _u8p), _p(_f64(theta), _f64p), float(delta), float(psi),
+                                        _p(_f64(bl), _f64p), _p(out, _f64p)))
+        return out
 
     def fel_counters(self, reset=False):
         """hy_fel_eval counters: candidates, pool hits, warm and cold solves."""

@@ -261,6 +261,23 @@ int hy_gtr_eval(hy_engine h, int precision, unsigned flags, const double *bl, co
 int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *beta,
                 const double *theta, double delta, double psi, const double *bl, double *out_ll, double *out_scale);
 
+/* MEME per-site refit candidates, FP64: candidate k is pattern pattern[k]
+ * at MEMEAnalyzer::analyze_pattern's eval_pattern_meme_lnl(a, w1, p1,
+ * b_plus, b_bg) (meme.hpp:219-277; no bridges): with a = max(alpha[k], 1e-12)
+ * the generators MG94(alpha = a, beta = a omega1[k]), MG94(a, beta_plus[k])
+ * and, when a branch is background, MG94(a, beta_bg[k]) (theta[5], delta,
+ * psi of the base model); test branches (is_test[node] != 0; NULL = all)
+ * take P = p1 max(P_1, 0) + (1 - p1) max(P_+, 0) at t_b = bl[b] (the tree's
+ * lengths after the analyzer's conversion), background branches
+ * max(P_bg, 0); out_ll[k] the pattern's lnL (compute_pattern_log_likelihood,
+ * fel.hpp:276-308; -1e20 where not positive).  The spectra are those of the
+ * alpha = 1 generators at omega1, beta_plus / a and beta_bg / a (the
+ * generators are a times those), drawn from the handle's pool like
+ * hy_fel_eval's.  beta_bg may be NULL when every branch is a test branch. */
+int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *omega1,
+                 const double *p1, const double *beta_plus, const double *beta_bg, const uint8_t *is_test,
+                 const double *theta, double delta, double psi, const double *bl, double *out_ll);
+
 /* The spectrum of one MG94 generator on the handle's model (hy_set_model's
  * code and frequencies; no problem needed): MG94Matrix::update
  * (rate_matrix.hpp:165-248) with rates alpha, beta, theta[5] (AC, AT, CG,
