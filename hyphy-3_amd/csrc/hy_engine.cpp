@@ -3798,7 +3798,8 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
 
 int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *omega1,
                  const double *p1, const double *beta_plus, const double *beta_bg, const uint8_t *is_test,
-                 const double *theta, double delta, double psi, const double *bl, double *out_ll) {
+                 const double *theta, double delta, double psi, const double *bl, const int32_t *cond_node,
+                 const int32_t *cond_class, double *out_ll) {
     // MEMEAnalyzer::analyze_pattern's eval_pattern_meme_lnl (meme.hpp:219-277):
     // with a = max(alpha, 1e-12), MG94(alpha = a, beta = a omega1),
     // MG94(a, beta_plus), MG94(a, beta_bg) are a times the alpha = 1
@@ -3829,6 +3830,11 @@ int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *
               p1[k] <= 1.0 && std::isfinite(alpha[k]) && std::isfinite(omega1[k]) && std::isfinite(beta_plus[k]) &&
               std::isfinite(bb)))
             return fail(h, HY_ERR_ARG, "%s: candidate %zu out of range", who, k);
+        if (cond_node && cond_node[k] >= 0) {
+            const int cn = cond_node[k], cc = cond_class ? cond_class[k] : -1;
+            if (cn >= h->num_nodes || cn == h->root || (is_test && !is_test[cn]) || (cc != 0 && cc != 1))
+                return fail(h, HY_ERR_ARG, "%s: candidate %zu conditions node %d on class %d", who, k, cn, cc);
+        }
         const double a = std::max(alpha[k], 1e-12);
         om[k * NS] = omega1[k];
         om[k * NS + 1] = beta_plus[k] / a;
@@ -3840,9 +3846,10 @@ int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *
     TRY(pool_solve(h, who, theta, delta, psi, bl, om, ps));
     FelPool &f = h->fel;
     hipStream_t s = h->stream;
-    // staging: [alpha][p1] doubles, then [spec A][spec B][spec C][pattern] ints, then per-node test flags
+    // staging: [alpha][p1] doubles, then [spec A][spec B][spec C][pattern][condition] ints, then per-node
+    // test flags
     const size_t n_ext = (size_t)h->n_ext;
-    const size_t in_bytes = 2 * K * 8 + 4 * K * 4 + n_ext, out_bytes = K * 8;
+    const size_t in_bytes = 2 * K * 8 + 5 * K * 4 + n_ext, out_bytes = K * 8;
     const size_t pin_need = std::max(in_bytes, out_bytes);
     if (h->f_pin_bytes < pin_need) {
         if (h->f_pin) (void)hipHostFree(h->f_pin);
@@ -3862,8 +3869,9 @@ int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *
         pi32[K + k] = ps.slot[k * NS + 1];
         pi32[2 * K + k] = has_bg ? ps.slot[k * NS + 2] : ps.slot[k * NS];
         pi32[3 * K + k] = pattern[k];
+        pi32[4 * K + k] = (cond_node && cond_node[k] >= 0) ? 2 * cond_node[k] + cond_class[k] : -1;
     }
-    uint8_t *tf = (uint8_t *)(pi32 + 4 * K);
+    uint8_t *tf = (uint8_t *)(pi32 + 5 * K);
     for (size_t n = 0; n < n_ext; ++n)  // extended (identity) nodes are never read as test or background
         tf[n] = (n < (size_t)h->num_nodes && is_test) ? (is_test[n] ? 1 : 0) : 1;
     TRY(ensure(h, h->d_fstage, in_bytes));
@@ -3887,7 +3895,8 @@ int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *
     ma.specA = di;
     ma.specB = di + K;
     ma.specC = di + 2 * K;
-    ma.is_test = (const uint8_t *)(di + 4 * K);
+    ma.cond = di + 4 * K;
+    ma.is_test = (const uint8_t *)(di + 5 * K);
     ma.lam = (const double *)h->d_flam.p;
     ma.Z = (const double *)h->d_fZ.p;
     ma.bl = (const double *)h->d_fbl.p;
@@ -3908,6 +3917,37 @@ int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *
 }
 
 namespace {
+// invariant patterns: every resolved codon the same (fel.hpp:333-350,
+// meme.hpp:186-197)
+void invariant_patterns(hy_engine h, std::vector<uint8_t> &inv) {
+    inv.assign(h->P, 1);
+    for (int p = 0; p < h->P; ++p) {
+        int first = -1;
+        for (int t = 0; t < h->num_taxa && inv[p]; ++t) {
+            const uint16_t c = h->codes[(size_t)t * h->P + p];
+            if (c >= 64u) continue;
+            if (first < 0)
+                first = c;
+            else if ((int)c != first)
+                inv[p] = 0;
+        }
+    }
+}
+
+struct MemeRunCtx {
+    hy_engine h;
+    const double *theta;
+    double delta, psi;
+    const double *bl;
+};
+int meme_run_eval(void *user, int n, const int32_t *pattern, const double *alpha, const double *omega1,
+                  const double *p1, const double *beta_plus, const int32_t *cond_node, const int32_t *cond_class,
+                  double *out_ll) {
+    const MemeRunCtx *c = (const MemeRunCtx *)user;
+    return hy_meme_eval(c->h, n, pattern, alpha, omega1, p1, beta_plus, nullptr, nullptr, c->theta, c->delta, c->psi,
+                        c->bl, cond_node, cond_class, out_ll);
+}
+
 struct FelRunCtx {
     hy_engine h;
     const double *theta;
@@ -4052,23 +4092,68 @@ int hy_fel_run(hy_engine h, const double *theta, double alpha, double beta, doub
     const double conv = q_scale > 1e-12 ? 3.0 / q_scale : 1.0;
     std::vector<double> blc(h->num_nodes, 0.0);
     for (int n = 0; n < h->num_nodes; ++n) blc[n] = (n == h->root) ? 0.0 : bl[n] * conv;
-    // invariant patterns: every resolved codon the same (fel.hpp:333-350)
-    std::vector<uint8_t> inv(h->P, 1);
-    for (int p = 0; p < h->P; ++p) {
-        int first = -1;
-        for (int t = 0; t < h->num_taxa && inv[p]; ++t) {
-            const uint16_t c = h->codes[(size_t)t * h->P + p];
-            if (c >= 64u) continue;
-            if (first < 0)
-                first = c;
-            else if ((int)c != first)
-                inv[p] = 0;
-        }
-    }
+    std::vector<uint8_t> inv;
+    invariant_patterns(h, inv);
     FelRunCtx ctx{h, theta, delta, psi, blc.data()};
     const int rc = hy_fel_drive(h->P, inv.data(), tested, fel_run_eval, &ctx, out, out_calls, out_candidates);
     if (rc != HY_OK) return rc;
     if (out_q_scale) *out_q_scale = q_scale;
+    return HY_OK;
+}
+
+int hy_meme_run(hy_engine h, const double *theta, double alpha, double beta, double delta, double psi,
+                const double *bl, double *out, double *ebf, double *fel, int *out_calls, int64_t *out_candidates) {
+    const char *who = "hy_meme_run";
+    if (!h) return HY_ERR_ARG;
+    if (!h->have_problem || !h->have_model) return fail(h, HY_ERR_STATE, "%s before hy_set_problem/hy_set_model", who);
+    if (!theta || !bl || !out) return fail(h, HY_ERR_ARG, "%s: NULL argument", who);
+    if (out_calls) *out_calls = 0;
+    if (out_candidates) *out_candidates = 0;
+    if (h->P == 0) return HY_OK;
+    // the FEL baseline of every pattern, and the conversion of the lengths
+    // (MEME's own prepare_branch_lengths is FEL's, meme.hpp:150-163)
+    std::vector<double> fr((size_t)h->P * HY_FEL_FIELDS);
+    double q_scale = 0.0;
+    TRY(hy_fel_run(h, theta, alpha, beta, delta, psi, bl, fr.data(), nullptr, nullptr, &q_scale));
+    const double conv = q_scale > 1e-12 ? 3.0 / q_scale : 1.0;
+    std::vector<double> blc(h->num_nodes, 0.0);
+    for (int n = 0; n < h->num_nodes; ++n) blc[n] = (n == h->root) ? 0.0 : bl[n] * conv;
+    std::vector<uint8_t> inv;
+    invariant_patterns(h, inv);
+    MemeRunCtx ctx{h, theta, delta, psi, blc.data()};
+    const int rc = hy_meme_drive(h->P, inv.data(), fr.data(), h->num_nodes, h->root, meme_run_eval, &ctx, out, ebf,
+                                 out_calls, out_candidates);
+    if (rc != HY_OK) return rc;
+    // total branch length at the fitted rates (meme.hpp:614-638): the mean
+    // rates of MG94(alpha, beta1) and MG94(alpha, beta_plus) are alpha times
+    // those of the alpha = 1 generators
+    std::vector<int32_t> pat;
+    std::vector<double> ca, cb;
+    for (int p = 0; p < h->P; ++p) {
+        if (inv[p]) continue;
+        const double *r = out + (size_t)p * HY_MEME_FIELDS;
+        pat.insert(pat.end(), {p, p});
+        ca.insert(ca.end(), {r[HY_MEME_ALPHA], r[HY_MEME_ALPHA]});
+        cb.insert(cb.end(), {r[HY_MEME_BETA1], r[HY_MEME_BETA_PLUS]});
+    }
+    if (!pat.empty()) {
+        std::vector<double> ll(pat.size()), mu(pat.size());
+        TRY(hy_fel_eval(h, (int)pat.size(), pat.data(), ca.data(), cb.data(), theta, delta, psi, blc.data(), ll.data(),
+                        mu.data()));
+        size_t q = 0;
+        for (int p = 0; p < h->P; ++p) {
+            if (inv[p]) continue;
+            double *r = out + (size_t)p * HY_MEME_FIELDS;
+            const double a = std::max(r[HY_MEME_ALPHA], 1e-12);
+            const double rate = (r[HY_MEME_P1] * (a * mu[q]) + r[HY_MEME_P_PLUS] * (a * mu[q + 1])) / 3.0;
+            q += 2;
+            double len = 0.0;
+            for (int n = 0; n < h->num_nodes; ++n)
+                if (n != h->root) len += blc[n] * rate;
+            r[HY_MEME_LENGTH] = len;
+        }
+    }
+    if (fel) std::copy(fr.begin(), fr.end(), fel);
     return HY_OK;
 }
 

@@ -185,6 +185,7 @@ struct MemeArgs {
     const uint64_t *amb;
     const double *alpha, *p1;        // [n_cand] (alpha already >= 1e-12)
     const int32_t *specA, *specB, *specC;
+    const int32_t *cond;             // [n_cand] -1, or 2 node + class: that branch takes class 0 (A) or 1 (B) alone
     const double *lam, *Z;           // spectrum pool: [slot][64], [slot][64][64] eigenvectors of the symmetrised generator
     const double *bl, *pi;
     int S, n_int;

@@ -3079,7 +3079,8 @@ void launch_fel(const FelArgs &a, int n_cand, hipStream_t s) {
 // compute_pattern_log_likelihood, fel.hpp:276-308).  One workgroup per
 // candidate walks its pattern through the tree like k_fel, with up to three
 // spectra: a test branch contributes x = (p1 max(P_A, 0) + (1 - p1)
-// max(P_B, 0)) d, a background branch max(P_C, 0) d, at time a t.  Each
+// max(P_B, 0)) d, a background branch max(P_C, 0) d, at time a t; a
+// candidate may condition one branch on class A or B alone.  Each
 // spectrum is held in LDS by its eigenvectors Z of the symmetrised generator
 // (P_ij = pi_i^-1/2 pi_j^1/2 sum_k Z_ik e^{l_k a t} Z_jk, k_model's U and V
 // factored), so three fit beside the P tile.  FP64 throughout.
@@ -3094,6 +3095,7 @@ __global__ __launch_bounds__(256) void k_meme(MemeArgs a) {
     const int p = a.pat[k];
     const int sl[3] = {a.specA[k], a.specB[k], a.specC[k]};
     const double at = a.alpha[k], w1 = a.p1[k], w2 = 1.0 - w1;
+    const int cnd = a.cond[k], cnode = cnd >= 0 ? cnd >> 1 : -1, ccls = cnd & 1;
     for (int c = 0; c < 3; ++c) {
         const double *Z = a.Z + (size_t)sl[c] * 4096;
 #pragma unroll 4
@@ -3123,10 +3125,12 @@ __global__ __launch_bounds__(256) void k_meme(MemeArgs a) {
         for (int c = 0; c < nc; ++c) {
             const int cn = e[3 + c], ref = a.nref[cn], kind = ref >> 28, idx = ref & REF_MASK;
             const bool test = a.is_test[cn] != 0;
-            // components of this edge: (spectrum, weight)
-            const int n_comp = test ? 2 : 1;
-            const int cs0 = test ? 0 : 2;
-            const double cw0 = test ? w1 : 1.0;
+            // components of this edge: (spectrum, weight); the conditioned
+            // branch takes one class alone (the branch EBFs, meme.hpp:552-580)
+            const bool cbr = cn == cnode;
+            const int n_comp = cbr ? 1 : test ? 2 : 1;
+            const int cs0 = cbr ? ccls : test ? 0 : 2;
+            const double cw0 = cbr ? 1.0 : test ? w1 : 1.0;
             __syncthreads();  // previous child's reads of dv / Pt / cols / ex are done
             uint64_t mask = 0;
             if (kind == KIND_TIP) {

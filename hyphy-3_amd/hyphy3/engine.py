@@ -37,7 +37,8 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_set_memory_budget", "hy_last_checkpoint", "hy_gtr_eval", "hy_fel_eval", "hy_fel_drive", "hy_fel_run",
            "hy_fel_counters", "hy_mg94_spectrum", "hy_branch_eval", "hy_set_pattern_weights", "hy_network_create",
            "hy_network_destroy", "hy_network_last_error", "hy_network_set_problem", "hy_network_set_model",
-           "hy_network_set_mixture", "hy_network_set_sink", "hy_network_eval", "hy_meme_eval")
+           "hy_network_set_mixture", "hy_network_set_sink", "hy_network_eval", "hy_meme_eval", "hy_meme_drive",
+           "hy_meme_run")
 
 HY_FEL_FIELDS = 10
 FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt",
@@ -45,6 +46,16 @@ FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_val
 # int (*)(void *user, int n, const int32_t *pattern, const double *alpha, const double *beta, double *out_ll)
 FEL_EVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                           C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+HY_MEME_FIELDS = 16
+MEME_FIELD_NAMES = ("alpha", "beta1", "p1", "beta_plus", "p_plus", "lrt", "p_value", "branches_under_selection",
+                    "total_branch_length", "log_l_meme", "log_l_fel", "p_value_meme_vs_fel", "fel_alpha", "fel_beta",
+                    "beta_bg", "evaluations")
+# int (*)(void *user, int n, const int32_t *pattern, const double *alpha, const double *omega1, const double *p1,
+#         const double *beta_plus, const int32_t *cond_node, const int32_t *cond_class, double *out_ll)
+MEME_EVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                           C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double))
 
 _i32p = C.POINTER(C.c_int32)
 _i8p = C.POINTER(C.c_int8)
@@ -103,7 +114,11 @@ def load_library(path: str = LIB_PATH):
     L.hy_fel_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int]
     L.hy_set_pattern_weights.argtypes = [C.c_void_p, _f64p]
     L.hy_meme_eval.argtypes = [C.c_void_p, C.c_int, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _u8p, _f64p,
-                               C.c_double, C.c_double, _f64p, _f64p]
+                               C.c_double, C.c_double, _f64p, _i32p, _i32p, _f64p]
+    L.hy_meme_drive.argtypes = [C.c_int, _u8p, _f64p, C.c_int, C.c_int, MEME_EVAL_FN, C.c_void_p, _f64p, _f64p,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+    L.hy_meme_run.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, _f64p,
+                              _f64p, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.hy_branch_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _i32p]
     L.hy_mg94_spectrum.argtypes = [C.c_void_p, C.c_double, C.c_double, _f64p, C.c_double, C.c_double, _f64p, _f64p,
                                    _f64p, _f64p]
@@ -468,19 +483,42 @@ class Engine:
         return (lnl, L, e) if site else lnl
 
     def meme_eval(self, pattern, alpha, omega1, p1, beta_plus, theta, bl, beta_bg=None, is_test=None, delta=0.0,
-                  psi=0.0):
+                  psi=0.0, cond_node=None, cond_class=None):
         """hy_meme_eval: MEME's per-site lnL for each candidate
-        (pattern[k], alpha[k], omega1[k], p1[k], beta_plus[k][, beta_bg[k]])."""
+        (pattern[k], alpha[k], omega1[k], p1[k], beta_plus[k][, beta_bg[k]]);
+        with cond_node[k] >= 0 that branch takes class cond_class[k] alone
+        (0: omega1, 1: beta_plus)."""
         pat = np.ascontiguousarray(pattern, dtype=np.int32)
         n = pat.size
         arrs = [_f64(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,))) for x in (alpha, omega1, p1, beta_plus)]
         bb = None if beta_bg is None else _f64(np.broadcast_to(np.asarray(beta_bg, dtype=np.float64), (n,)))
         it = None if is_test is None else np.ascontiguousarray(is_test, dtype=np.uint8)
+        cn = cc = None
+        if cond_node is not None:
+            cn = np.ascontiguousarray(np.broadcast_to(np.asarray(cond_node), (n,)), dtype=np.int32)
+            cc = np.ascontiguousarray(np.broadcast_to(np.asarray(cond_class if cond_class is not None else 0), (n,)),
+                                      dtype=np.int32)
         out = np.zeros(n)
         self._check(self.L.hy_meme_eval(self.h, n, _p(pat, _i32p), *[_p(a, _f64p) for a in arrs], _p(bb, _f64p),
                                         _p(it, _u8p), _p(_f64(theta), _f64p), float(delta), float(psi),
-                                        _p(_f64(bl), _f64p), _p(out, _f64p)))
+                                        _p(_f64(bl), _f64p), _p(cn, _i32p), _p(cc, _i32p), _p(out, _f64p)))
         return out
+
+    def meme_run(self, theta, alpha, beta, bl, delta=0.0, psi=0.0):
+        """MEME on every pattern, every branch tested (hy_meme_run): bl the
+        fitted tree's (unconverted) lengths, (alpha, beta, theta, delta, psi)
+        the global MG94 fit.  Returns a dict: "sites" [P, HY_MEME_FIELDS]
+        (columns MEME_FIELD_NAMES), "ebf" [P, num_nodes], "fel"
+        [P, HY_FEL_FIELDS], "calls", "candidates"."""
+        P, N = self.num_patterns, self.num_nodes
+        out = np.zeros((max(P, 1), HY_MEME_FIELDS))
+        ebf = np.ones((max(P, 1), N))
+        fel = np.zeros((max(P, 1), HY_FEL_FIELDS))
+        calls, cands = C.c_int(), C.c_int64()
+        self._check(self.L.hy_meme_run(self.h, _p(_f64(theta), _f64p), float(alpha), float(beta), float(delta),
+                                       float(psi), _p(_f64(bl), _f64p), _p(out, _f64p), _p(ebf, _f64p),
+                                       _p(fel, _f64p), C.byref(calls), C.byref(cands)))
+        return {"sites": out[:P], "ebf": ebf[:P], "fel": fel[:P], "calls": calls.value, "candidates": cands.value}
 
     def fel_counters(self, reset=False):
         """hy_fel_eval counters: candidates, pool hits, warm and cold solves."""
@@ -605,6 +643,40 @@ def fel_drive(invariant, tested_length, evaluate):
     if rc != 0:
         raise HyError(rc, "hy_fel_drive failed")
     return out[:n], calls.value, cands.value
+
+
+def meme_drive(invariant, fel_rows, num_nodes, root, evaluate):
+    """MEME refits (hy_meme_drive) over a Python candidate evaluator:
+    ``evaluate(patterns, alphas, omega1s, p1s, beta_pluses, cond_nodes,
+    cond_classes)`` returns the candidates' log-likelihoods.  Returns
+    (sites [n_patterns, HY_MEME_FIELDS], ebf [n_patterns, num_nodes], calls,
+    candidates)."""
+    L = load_library()
+    inv = np.ascontiguousarray(invariant, dtype=np.uint8)
+    n = inv.size
+    fel = np.ascontiguousarray(fel_rows, dtype=np.float64).reshape(n, HY_FEL_FIELDS) if n else np.zeros((1, HY_FEL_FIELDS))
+    out = np.zeros((max(n, 1), HY_MEME_FIELDS))
+    ebf = np.ones((max(n, 1), int(num_nodes)))
+    failure = []
+
+    def cb(user, k, pat, a, w1, p1, bp, cn, cc, ll):
+        try:
+            cols = [np.ctypeslib.as_array(x, shape=(k,)).copy() for x in (pat, a, w1, p1, bp, cn, cc)]
+            np.ctypeslib.as_array(ll, shape=(k,))[:] = evaluate(*cols)
+            return 0
+        except Exception as ex:  # reported after the drive returns
+            failure.append(ex)
+            return -100
+
+    fn = MEME_EVAL_FN(cb)
+    calls, cands = C.c_int(), C.c_int64()
+    rc = L.hy_meme_drive(n, _p(inv, _u8p), _p(fel, _f64p), int(num_nodes), int(root), fn, None, _p(out, _f64p),
+                         _p(ebf, _f64p), C.byref(calls), C.byref(cands))
+    if failure:
+        raise failure[0]
+    if rc != 0:
+        raise HyError(rc, "hy_meme_drive failed")
+    return out[:n], ebf[:n], calls.value, cands.value
 
 
 def graph_count():

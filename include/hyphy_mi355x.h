@@ -273,10 +273,16 @@ int hy_fel_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *a
  * fel.hpp:276-308; -1e20 where not positive).  The spectra are those of the
  * alpha = 1 generators at omega1, beta_plus / a and beta_bg / a (the
  * generators are a times those), drawn from the handle's pool like
- * hy_fel_eval's.  beta_bg may be NULL when every branch is a test branch. */
+ * hy_fel_eval's.  beta_bg may be NULL when every branch is a test branch.
+ * cond_node / cond_class (both NULL, or one entry per candidate): a candidate
+ * with cond_node[k] >= 0 takes max(P_1, 0) (class 0) or max(P_+, 0)
+ * (class 1) alone on that test branch and the mixture elsewhere, the
+ * conditional likelihoods of the branch Empirical Bayes Factors
+ * (meme.hpp:552-600). */
 int hy_meme_eval(hy_engine h, int n_cand, const int32_t *pattern, const double *alpha, const double *omega1,
                  const double *p1, const double *beta_plus, const double *beta_bg, const uint8_t *is_test,
-                 const double *theta, double delta, double psi, const double *bl, double *out_ll);
+                 const double *theta, double delta, double psi, const double *bl, const int32_t *cond_node,
+                 const int32_t *cond_class, double *out_ll);
 
 /* The spectrum of one MG94 generator on the handle's model (hy_set_model's
  * code and frequencies; no problem needed): MG94Matrix::update
@@ -341,6 +347,66 @@ int hy_fel_drive(int n_patterns, const uint8_t *invariant, double tested_length,
  * site_to_pattern); out_q_scale (may be NULL) receives Q_scale. */
 int hy_fel_run(hy_engine h, const double *theta, double alpha, double beta, double delta, double psi,
                const double *bl, double *out, int *out_calls, int64_t *out_candidates, double *out_q_scale);
+
+/* One row per pattern of the MEME refit results (MEMESiteResult,
+ * meme.hpp:31-50). */
+enum {
+    HY_MEME_ALPHA = 0,
+    HY_MEME_BETA1 = 1,          /* alpha x omega1 */
+    HY_MEME_P1 = 2,
+    HY_MEME_BETA_PLUS = 3,
+    HY_MEME_P_PLUS = 4,
+    HY_MEME_LRT = 5,
+    HY_MEME_P_VALUE = 6,        /* 2/3 (0.45 erfc(sqrt(LRT / 2)) + 0.55 e^{-LRT / 2}) */
+    HY_MEME_BRANCHES = 7,       /* branches with EBF >= 100 */
+    HY_MEME_LENGTH = 8,         /* total branch length at the fitted rates */
+    HY_MEME_LOGL = 9,
+    HY_MEME_LOGL_FEL = 10,
+    HY_MEME_P_VS_FEL = 11,      /* e^{-(lnL_MEME - lnL_FEL)} when positive, else 1 */
+    HY_MEME_FEL_ALPHA = 12,
+    HY_MEME_FEL_BETA = 13,
+    HY_MEME_BETA_BG = 14,       /* 0: every branch tested */
+    HY_MEME_EVALS = 15,         /* candidate evaluations of the pattern */
+    HY_MEME_FIELDS = 16
+};
+
+/* Candidate evaluator of hy_meme_drive: n candidates (pattern[k], alpha[k],
+ * omega1[k], p1[k], beta_plus[k], cond_node[k], cond_class[k]) -> out_ll[k],
+ * with hy_meme_eval's meaning (cond_node[k] = -1: no conditioning); a
+ * non-zero return aborts the drive with that code. */
+typedef int (*hy_meme_eval_fn)(void *user, int n, const int32_t *pattern, const double *alpha,
+                               const double *omega1, const double *p1, const double *beta_plus,
+                               const int32_t *cond_node, const int32_t *cond_class, double *out_ll);
+
+/* MEMEAnalyzer::analyze_pattern with every branch tested (meme.hpp:166-648,
+ * no bridges) for patterns 0 .. n_patterns-1, from each pattern's FEL row
+ * fel[p * HY_FEL_FIELDS] (hy_fel_drive's layout): invariant patterns get
+ * zeros, p1 = 1 and p = 1; every other one the regime's start grid, a 4-D
+ * NelderMeadND over (alpha, omega1, p1, beta_plus) in [1e-4, 1e4] x [0, 1] x
+ * [1e-6, 1 - 1e-6] x [1e-4, 1e5] (tol 1e-5, 800 iterations,
+ * nelder_mead.hpp:208-360); when beta_plus > alpha and p_plus > 1e-6 the
+ * null beta_plus = alpha as a 3-D NelderMeadND from its 20-point grid, a
+ * warm alternative refit when the null is better, the LRT and its mixture
+ * p-value, and the branch EBFs (ebf[p * num_nodes + node], may be NULL; 1 for
+ * the root and for patterns without them).  HY_MEME_LENGTH is left 0 (it
+ * needs the generators' mean rates; hy_meme_run fills it).  Lockstep
+ * coroutines as hy_fel_drive: independent points (grids, a simplex's first
+ * vertices, shrink steps, the EBF conditionals) are asked for as one batch.
+ * Host only. */
+int hy_meme_drive(int n_patterns, const uint8_t *invariant, const double *fel, int num_nodes, int root,
+                  hy_meme_eval_fn evaluate, void *user, double *out, double *ebf, int *out_calls,
+                  int64_t *out_candidates);
+
+/* MEME on the handle's problem and model, every branch tested: hy_fel_run
+ * (the same conversion of bl by 3 / Q_scale, MEMEAnalyzer::
+ * prepare_branch_lengths, meme.hpp:150-163), then hy_meme_drive with
+ * hy_meme_eval as the evaluator, then each pattern's total branch length
+ * sum_b t_b (p1 mu(alpha, beta1) + p_plus mu(alpha, beta_plus)) / 3
+ * (meme.hpp:614-638).  out[num_patterns * HY_MEME_FIELDS], ebf[num_patterns *
+ * num_nodes] (may be NULL), fel[num_patterns * HY_FEL_FIELDS] (may be NULL)
+ * the FEL rows it started from. */
+int hy_meme_run(hy_engine h, const double *theta, double alpha, double beta, double delta, double psi,
+                const double *bl, double *out, double *ebf, double *fel, int *out_calls, int64_t *out_candidates);
 
 /* lnL from caller-supplied transition matrices P[num_nodes*S*S] (row-major,
  * entry of the root ignored) and root frequencies pi[S]. */
