@@ -195,7 +195,9 @@ def main():
                          "gradients) on the workload's nucleotide view, against the oracle on the host")
     ap.add_argument("--fel", action="store_true",
                     help="instead of the codon run: FEL per-site refits (grid, Nelder-Mead, Brent per pattern)")
-    ap.add_argument("--fel-data", default="cd2", choices=["cd2", "sim"])
+    ap.add_argument("--meme", action="store_true",
+                    help="instead of the codon run: MEME per-site refits (FEL baseline, 4-D/3-D Nelder-Mead, EBFs)")
+    ap.add_argument("--fel-data", default="cd2", choices=["cd2", "sim"], help="workload of --fel and --meme")
     ap.add_argument("--memory-budget", type=float, default=0.0,
                     help="GB allowed for the partials; below what every node's partials need the engine "
                          "checkpoints inside partials (about sqrt(internal nodes) parts recomputed)")
@@ -228,6 +230,9 @@ def main():
         return
     if args.fel:
         bench_fel(args)
+        return
+    if args.meme:
+        bench_meme(args)
         return
     sim, aln, tree, arrays, tau = build_problem(cfg)
     print(f"[bench] {args.config}: taxa={cfg['taxa']} codons={cfg['codons']} patterns={len(arrays['weights'])} "
@@ -539,24 +544,8 @@ def bench_fel(args):
     simulated alignment (``--fel-data sim``).  One step = the whole refit
     stage over every pattern; the oracle's sequential FEL (one host core) on a
     bounded sample of patterns is the CPU column, in candidates per second."""
-    import hyphy3
     from hyphy3 import fel as pf
-    from hyphy3 import simulate
-    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
-    truth = json.load(open(os.path.join(golden, "cd2_fel_mle.json")))
-    if args.fel_data == "cd2":
-        g = truth["global_mg94"]
-        fasta = open(os.path.join(golden, "cd2.fna")).read()
-        newick = open(os.path.join(golden, "cd2_fitted.nwk")).read()
-        theta, alpha, beta = g["theta"], g["alpha"], g["beta"]
-        label = "CD2 (10 taxa x 187 codons), fitted tree and global MG94 rates of the reference's FEL parity test"
-    else:
-        sim = simulate.simulate(32, 1000, 11, omegas=[0.1, 1.0, 4.0], weights=[0.6, 0.3, 0.1])
-        fasta, newick = sim["fasta"], sim["newick"]
-        theta, alpha, beta = list(sim["theta"]), 1.0, 0.5
-        label = "simulated 32 taxa x 1000 codons (omega mixture 0.1/1/4), generating tree"
-    aln = hyphy3.Alignment.from_text(fasta)
-    tree = hyphy3.Tree.from_newick(newick, keep_rooted=True)
+    fasta, newick, theta, alpha, beta, label, truth, aln, tree = refit_workload(args)
     fel = pf.FEL(tree, aln, theta, alpha, beta)
     n_pat = int(fel.states.shape[0])
     res = {"metric": "FEL per-site refit stage", "unit": "ms", "workload": label, "patterns": n_pat,
@@ -600,6 +589,88 @@ def bench_fel(args):
                                "sample": f"first {n_done} of {prob.num_patterns} patterns refitted sequentially "
                                          f"({n_cand} candidates in {el:.1f} s; FP64 oracle: eigendecomposition "
                                          "per candidate omega, single-pattern pruning)"}
+    print(json.dumps(res), flush=True)
+
+
+def refit_workload(args):
+    """The per-site refit workload: CD2 with the fitted tree and global MG94
+    rates of the reference's matched-tree FEL parity test (tests/golden), or
+    a simulated alignment (``--fel-data sim``)."""
+    import hyphy3
+    from hyphy3 import simulate
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+    truth = json.load(open(os.path.join(golden, "cd2_fel_mle.json")))
+    if args.fel_data == "cd2":
+        g = truth["global_mg94"]
+        fasta = open(os.path.join(golden, "cd2.fna")).read()
+        newick = open(os.path.join(golden, "cd2_fitted.nwk")).read()
+        theta, alpha, beta = g["theta"], g["alpha"], g["beta"]
+        label = "CD2 (10 taxa x 187 codons), fitted tree and global MG94 rates of the reference's FEL parity test"
+    else:
+        sim = simulate.simulate(32, 1000, 11, omegas=[0.1, 1.0, 4.0], weights=[0.6, 0.3, 0.1])
+        fasta, newick = sim["fasta"], sim["newick"]
+        theta, alpha, beta = list(sim["theta"]), 1.0, 0.5
+        label = "simulated 32 taxa x 1000 codons (omega mixture 0.1/1/4), generating tree"
+    aln = hyphy3.Alignment.from_text(fasta)
+    tree = hyphy3.Tree.from_newick(newick, keep_rooted=True)
+    return fasta, newick, theta, alpha, beta, label, truth, aln, tree
+
+
+def bench_meme(args):
+    """MEME per-site refits (MEMEAnalyzer::analyze_pattern with every branch
+    tested, meme.hpp:166-648, after FEL's refits) on the FEL workload.  One
+    step = FEL's and MEME's refit stages over every pattern (at most 10 timed
+    steps); the oracle's sequential MEME (one host core) on a bounded sample
+    of patterns is the CPU column, in candidates per second."""
+    from hyphy3 import meme as pm
+    fasta, newick, theta, alpha, beta, label, truth, aln, tree = refit_workload(args)
+    meme = pm.MEME(tree, aln, theta, alpha, beta)
+    n_pat = int(meme.states.shape[0])
+    steps = max(1, min(args.steps, 10))
+    res = {"metric": "MEME per-site refit stage (FEL baseline included)", "unit": "ms", "workload": label,
+           "patterns": n_pat, "steps": steps, "warmup": min(args.warmup, 2)}
+    for _ in range(res["warmup"]):
+        meme.run()
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        out = meme.run()
+        times.append(time.perf_counter() - t0)
+    st = meme.stats
+    res.update(ms_per_stage=1e3 * float(np.median(times)), ms_min=1e3 * float(np.min(times)),
+               meme_calls=st["calls"], meme_candidates=st["candidates"],
+               meme_candidates_per_s=st["candidates"] / float(np.median(times)),
+               sites_p_below_0_1=int(sum(r["p_value"] <= 0.1 for r in out)))
+    if args.fel_data == "cd2":
+        from scipy.stats import pearsonr, spearmanr
+        mt = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                         "cd2_meme_mle.json")))
+        lrt = [r["lrt"] for r in out]
+        g = [m[5] for m in mt["mle"]]
+        res["lrt_pearson_vs_hyphy25"] = float(pearsonr(lrt, g)[0])
+        res["lrt_spearman_vs_hyphy25"] = float(spearmanr(lrt, g)[0])
+    meme.engine.close()
+    if not args.no_cpu:
+        from oracle import data_ref as dr
+        from oracle import hyoracle as ho
+        from oracle import meme_ref as mr
+        names, seqs = dr.parse_fasta(fasta)
+        raln = dr.AlignmentRef(names, seqs)
+        rtree = dr.TreeRef.from_newick(newick)
+        prob = ho.Problem.from_refs(rtree, raln)
+        rbl = np.array([0.0 if n.id == rtree.root else n.bl for n in rtree.nodes])
+        n_done, n_cand, t0 = 0, 0, time.perf_counter()
+        for p in range(prob.num_patterns):
+            r, _ = mr.meme(prob, ho.Code(0), theta, raln.pos_freqs, raln.cf3x4, rbl, alpha, beta, patterns=[p])[0]
+            n_done += 1
+            n_cand += r.get("evaluations", 0)
+            if time.perf_counter() - t0 >= args.cpu_seconds:
+                break
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n_cand / el, "unit": "MEME candidates/s", "cores": 1, "kind": "port",
+                               "sample": f"first {n_done} of {prob.num_patterns} patterns (FEL then MEME) refitted "
+                                         f"sequentially ({n_cand} MEME candidates in {el:.1f} s; FP64 oracle: "
+                                         "eigendecompositions per candidate, single-pattern pruning)"}
     print(json.dumps(res), flush=True)
 
 

@@ -1272,6 +1272,7 @@ int run_eval(hy_engine h, unsigned flags, const double *custom_pi) {
         fa.wl_ptr = (const int32_t *)h->d_wl.p;
         fa.wl_idx = (const int32_t *)h->d_wl.p + (h->n_mat + 1);
         fa.proj = (double *)h->d_proj.p;
+        fa.mat_gene = (const int32_t *)h->d_mat_gene.p;
         if (h->d_proj.bytes < (size_t)h->n_mat * 6 * h->n_grp * 8 ||
             h->d_Apart.bytes < (size_t)std::max(1, h->n_branch) * h->n_chunks * 4096 * 8)
             return fail(h, HY_ERR_STATE, "internal: Frechet stage buffers smaller than the grouping needs");

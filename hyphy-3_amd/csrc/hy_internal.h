@@ -286,6 +286,7 @@ struct FrechetArgs {
     const int32_t *wl_ptr, *wl_idx;  // per matrix: pair slots (branch * max_comp + component) using it
     double *out;                     // [n_branch][max_comp][8]: [0] <A, P_m>, [1] w_j d lnL / d tau share
     double *proj;                    // [n_mat][6][n_grp] <M_phi, H_m> of each group's partial H_m
+    const int32_t *mat_gene;         // [n_mat] gene of each matrix in a gene batch, or null
 };
 
 struct FinalArgs {

@@ -2204,10 +2204,13 @@ __global__ __launch_bounds__(256, 3) void k_pmat(PmatArgs<R> a, ProjArgs pj, int
 // transforms one pair with two buffers (V, then U, reloaded), two per CU.
 // Grid numbering of the Fréchet stage: workgroups go to the eight XCDs in
 // turn (linear id mod 8), and each XCD has its own L2.  Linear id
-// L = (g / 8) * 8 n_mat + 8 m + g % 8 puts group g of every matrix on XCD
-// g % 8: with a mixture whose classes use every matrix, group g of each
-// matrix's pair list is the same branch, so the A_b those workgroups read
-// (summed over its pattern chunks) comes from HBM once and from L2 after.
+// L = 8 ((g / 8) n_mat + m) + (g + r_m) % 8 puts group g of matrix m on XCD
+// (g + r_m) % 8, with r_m the matrix's gene in a gene batch (0 otherwise).
+// With a mixture whose classes use every matrix, group g of each matrix's
+// pair list is the same branch, so the A_b those workgroups read (summed
+// over its pattern chunks) comes from HBM once and from L2 after; the
+// rotation by gene spreads batches, whose small genes have one group per
+// matrix, over all eight XCDs (without it every gene's group 0 ran on one).
 constexpr int FRECHET_XCD = 8;
 inline int frechet_grid(int n_mat, int n_grp) {
     return (n_grp + FRECHET_XCD - 1) / FRECHET_XCD * FRECHET_XCD * n_mat;
@@ -2215,8 +2218,10 @@ inline int frechet_grid(int n_mat, int n_grp) {
 
 template <bool GROUPED>
 __global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a) {
-    const int per = FRECHET_XCD * a.n_mat, blk = blockIdx.x / per, rr = blockIdx.x - blk * per;
-    const int m = rr / FRECHET_XCD, grp = blk * FRECHET_XCD + rr % FRECHET_XCD;
+    const int slot = blockIdx.x / FRECHET_XCD, x = blockIdx.x % FRECHET_XCD;
+    const int blk = slot / a.n_mat, m = slot - blk * a.n_mat;
+    const int rot = a.mat_gene ? a.mat_gene[m] % FRECHET_XCD : 0;
+    const int grp = blk * FRECHET_XCD + (x + FRECHET_XCD - rot) % FRECHET_XCD;
     if (grp >= a.n_grp) return;  // padding of the last block of groups
     const int tid = threadIdx.x, w = tid >> 6, lr = tid & 15, lg = (tid & 63) >> 4;
     __shared__ double Bs[64 * LDA];                  // V (and U unless GROUPED)
@@ -2239,6 +2244,11 @@ __global__ __launch_bounds__(256, GROUPED ? 1 : 2) void k_frechet(FrechetArgs a)
     auto fetch_a = [&](int bi) {
         const double *src = a.Apart + (size_t)bi * a.n_chunks * 4096 + tid;
         const int nc = a.n_chunks;
+        if (nc == 1) {  // one chunk per branch (gene batches): no clamped re-reads
+#pragma unroll
+            for (int k = 0; k < 16; ++k) ra[k] = src[256 * k];
+            return;
+        }
         double t[4][16];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
