@@ -402,6 +402,30 @@ def bench_fit(args, cfg):
             "mean_ms_per_evaluation_fp64": 1e3 * float(np.mean([c[0] for c in calls if c[2] != 1] or [0.0])),
             "note": "eigensolver status per rate matrix: >= 100 warm start refined, 50+k tridiagonal cold solve, "
                     "1..40 Jacobi"}
+    eng.eval = plain_eval
+    # the same fit with the optimizer on the GPU (hy_fit_lbfgs): FP32 phase,
+    # then FP64, each trial point one replay of a captured evaluation + step
+    # graph; then the FP64 phase again stepped on the host (hy_eval +
+    # hy_lbfgs_step) from the same FP32 result, for the per-iteration cost
+    legs = {}
+    for name, captured in (("captured", True), ("host_stepped", False)):
+        t0 = time.perf_counter()
+        r32 = eng.fit_lbfgs(bl0, om0, cfg["weights"], th0, precision=1, captured=captured, scale=1.0 / n_sites,
+                            max_iter=1000, tol_grad=1e-7, tol_f=1e-9)
+        t1 = time.perf_counter()
+        r64 = eng.fit_lbfgs(r32["bl"], r32["omegas"], cfg["weights"], r32["theta"], precision=0, captured=captured,
+                            scale=1.0 / n_sites, max_iter=1000, tol_grad=1e-7, tol_f=1e-12)
+        t2 = time.perf_counter()
+        legs[name] = {"log_likelihood": r64["logL"], "iterations_fp32": r32["iterations"],
+                      "evaluations_fp32": r32["evaluations"], "seconds_fp32": t1 - t0,
+                      "ms_per_evaluation_fp32": 1e3 * (t1 - t0) / max(1, r32["evaluations"]),
+                      "iterations_fp64": r64["iterations"], "evaluations_fp64": r64["evaluations"],
+                      "seconds_fp64": t2 - t1, "ms_per_evaluation_fp64": 1e3 * (t2 - t1) / max(1, r64["evaluations"]),
+                      "replays": r32["replays"] + r64["replays"], "status": [r32["status"], r64["status"]]}
+        print(f"[bench] fit {name}: {legs[name]}", file=sys.stderr, flush=True)
+    legs["identical_iterates"] = bool(legs["captured"]["log_likelihood"] == legs["host_stepped"]["log_likelihood"]
+                                      and legs["captured"]["evaluations_fp64"] == legs["host_stepped"]["evaluations_fp64"])
+    line["gpu_optimizer"] = legs
     eng.close()
     print(json.dumps(line), flush=True)
     if args.cpu_fit:

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "hy_internal.h"
+#include "hy_lbfgs.h"
 #include "hyphy/core/genetic_code.hpp"
 
 using namespace hy;
@@ -4390,6 +4391,257 @@ int hy_graph_count(int *live, int *cap) {
 int hy_device_bytes(hy_engine h, size_t *out) {
     if (!h || !out) return HY_ERR_ARG;
     *out = h->device_bytes;
+    return HY_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Maximum-likelihood fit with captured optimizer iterations (hy_fit_lbfgs):
+// the joint phase of MG94Fitter::fit (optimizer.hpp:934-1110) with the
+// quasi-Newton step of hy_lbfgs.h.  The start point is evaluated through
+// hy_eval's path and stepped on the host; from there every trial point is
+// one replay of a graph holding the evaluation and k_lbfgs, which writes the
+// next trial's parameters into the pinned parameter block the next replay's
+// k_model reads.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr double LB_LOG_BL_LO = -18.420680744, LB_LOG_BL_HI = 6.91;  // optimizer.hpp:966-967
+constexpr double LB_LOG_RATE_LO = -10.0, LB_LOG_RATE_HI = 6.91;      // optimizer.hpp:971-972, 986-987
+
+struct FitScratch {
+    hy_engine h = nullptr;
+    // optimizer state; the captured graph's parameter block, results and
+    // eigensolver status words (device memory: an iteration touches no host
+    // memory)
+    void *state = nullptr, *hd = nullptr, *maps = nullptr, *params = nullptr, *res = nullptr, *status = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~FitScratch() {
+        if (exec) {
+            (void)hipStreamSynchronize(h->stream);
+            destroy_graph_exec(exec);
+        }
+        if (state) (void)hipFree(state);
+        if (hd) (void)hipFree(hd);
+        if (maps) (void)hipFree(maps);
+        if (params) (void)hipFree(params);
+        if (res) (void)hipFree(res);
+        if (status) (void)hipFree(status);
+    }
+};
+
+// Captures one fit iteration: the evaluation (as hy_eval's graph, without
+// the completion word, with its parameter block, results and status words in
+// the device buffers la names instead of pinned host memory) followed by
+// k_lbfgs.
+template <typename R>
+int capture_fit_iteration(hy_engine h, const LbArgs &la, hipGraphExec_t *out) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    const double *keep_params = h->h_params_dev;
+    double *keep_res = h->h_res_dev;
+    int *keep_status = h->h_status_dev;
+    h->h_params_dev = la.params;
+    h->h_res_dev = const_cast<double *>(la.res);
+    h->h_status_dev = const_cast<int *>(la.eig_status);
+    h->zc_params = true;
+    h->publish_done = false;
+    const int rc = run_eval<R>(h, HY_WANT_LOGL | HY_WANT_GRAD, nullptr);
+    h->zc_params = false;
+    h->h_params_dev = keep_params;
+    h->h_res_dev = keep_res;
+    h->h_status_dev = keep_status;
+    if (rc == HY_OK) launch_lbfgs(la, h->stream);
+    const hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+    if (rc != HY_OK || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != HY_OK ? rc : fail(h, HY_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    }
+    const hipError_t e3 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e3 != hipSuccess) return fail(h, HY_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+    return HY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+double hy_lbfgs_exp(double x) { return lb_exp(x); }
+
+size_t hy_lbfgs_state_doubles(int n, int m) { return (n < 1 || m < 1) ? 0 : lb_state_doubles(n, m); }
+
+int hy_lbfgs_init(int n, int m, double *state, hy_lbfgs_head *hd, const double *x0, const double *lb,
+                  const double *ub, int max_iter, double tol_grad, double tol_f) {
+    if (n < 1 || m < 1 || m > LB_MMAX || !state || !hd || !x0 || !lb || !ub || max_iter < 0) return HY_ERR_ARG;
+    for (int i = 0; i < n; ++i)
+        if (!(lb[i] <= ub[i]) || x0[i] != x0[i]) return HY_ERR_ARG;
+    std::memset(hd, 0, sizeof *hd);
+    hd->n = n;
+    hd->m = m;
+    hd->max_iter = max_iter;
+    hd->max_back = 20;
+    hd->tol_grad = tol_grad;
+    hd->tol_f = tol_f;
+    hd->c1 = 1e-4;
+    std::fill(state, state + lb_state_doubles(n, m), 0.0);
+    const LbVec v = lb_vec(state, n, m);
+    for (int i = 0; i < n; ++i) {
+        v.lb[i] = lb[i];
+        v.ub[i] = ub[i];
+        v.x[i] = std::min(std::max(x0[i], lb[i]), ub[i]);
+        v.xt[i] = v.x[i];
+    }
+    return HY_OK;
+}
+
+int hy_lbfgs_step(double *state, hy_lbfgs_head *hd, double f_trial, const double *g_trial, int ok, double *x_next) {
+    if (!state || !hd || hd->n < 1 || hd->m < 1 || hd->m > LB_MMAX) return HY_ERR_ARG;
+    const LbVec v = lb_vec(state, hd->n, hd->m);
+    if (g_trial) std::memcpy(v.gt, g_trial, sizeof(double) * hd->n);
+    LbHostPar p;
+    lb_step(p, v, *hd, f_trial, ok);
+    if (x_next) std::memcpy(x_next, v.xt, sizeof(double) * hd->n);
+    return HY_OK;
+}
+
+int hy_fit_lbfgs(hy_engine h, int precision, int captured, double *bl, double alpha, double *omegas,
+                 const double *comp_w, double *theta, double delta, double psi, double scale, int m, int max_iter,
+                 double tol_grad, double tol_f, int check_every, double *out_logL, int32_t *out_info) {
+    if (!h) return HY_ERR_ARG;
+    TRY(check_eval_args(h, precision, bl, omegas, comp_w, theta, "hy_fit_lbfgs"));
+    if (m < 1 || m > LB_MMAX) return fail(h, HY_ERR_ARG, "hy_fit_lbfgs: m = %d correction pairs (1..%d)", m, LB_MMAX);
+    if (max_iter < 0 || check_every < 1 || !(scale > 0.0))
+        return fail(h, HY_ERR_ARG, "hy_fit_lbfgs: max_iter >= 0, check_every >= 1 and scale > 0 required");
+    if (h->P == 0) return fail(h, HY_ERR_STATE, "hy_fit_lbfgs: empty alignment");
+    if (h->wuser) return fail(h, HY_ERR_UNSUPPORTED, "hy_fit_lbfgs: caller pattern weights are set");
+    HIP_TRY(h, hipSetDevice(h->device));
+    TRY(prepare_buffers(h, precision));
+    // free variables: log bl of every node with a parent, log omega, log theta
+    std::vector<int32_t> res_map, par_map;
+    std::vector<double> x0, lo, hi;
+    const ParamLayout &pl = h->pl;
+    auto add = [&](double v, int res, int par, double a, double b) -> bool {
+        if (!(v >= 0.0) || !std::isfinite(v)) return false;
+        x0.push_back(v > 0.0 ? std::log(v) : a);
+        lo.push_back(a);
+        hi.push_back(b);
+        res_map.push_back(res);
+        par_map.push_back(par);
+        return true;
+    };
+    for (int n = 0; n < h->num_nodes; ++n)
+        if (h->real_parent[n] >= 0 && !add(bl[n], 1 + n, pl.off_bl + n, LB_LOG_BL_LO, LB_LOG_BL_HI))
+            return fail(h, HY_ERR_ARG, "hy_fit_lbfgs: branch length %d is negative or not finite", n);
+    for (int k = 0; k < h->n_mat; ++k)
+        if (!add(omegas[k], 1 + h->num_nodes + k, pl.off_omega + k, LB_LOG_RATE_LO, LB_LOG_RATE_HI))
+            return fail(h, HY_ERR_ARG, "hy_fit_lbfgs: omega %d is negative or not finite", k);
+    for (int k = 0; k < 5; ++k)
+        if (!add(theta[k], 1 + h->num_nodes + h->n_mat + h->n_comp + k, pl.off_theta + k, LB_LOG_RATE_LO,
+                 LB_LOG_RATE_HI))
+            return fail(h, HY_ERR_ARG, "hy_fit_lbfgs: exchangeability %d is negative or not finite", k);
+    const int n = (int)x0.size();
+    std::vector<double> state(lb_state_doubles(n, m));
+    hy_lbfgs_head hd;
+    TRY(hy_lbfgs_init(n, m, state.data(), &hd, x0.data(), lo.data(), hi.data(), max_iter, tol_grad, tol_f));
+    const LbVec v = lb_vec(state.data(), n, m);
+    std::vector<double> cbl(bl, bl + h->num_nodes), com(omegas, omegas + h->n_mat), cth(theta, theta + 5);
+    // host evaluation of the trial point xt and one host step
+    auto host_iteration = [&]() -> int {
+        for (int i = 0; i < n; ++i) {
+            const double pv = lb_exp(v.xt[i]);
+            const int par = par_map[i];
+            if (par >= pl.off_theta) cth[par - pl.off_theta] = pv;
+            else if (par >= pl.off_omega) com[par - pl.off_omega] = pv;
+            else cbl[par - pl.off_bl] = pv;
+        }
+        const int rc = hy_eval(h, precision, HY_WANT_LOGL | HY_WANT_GRAD, cbl.data(), alpha, com.data(), comp_w,
+                               cth.data(), delta, psi, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (rc != HY_OK && rc != HY_ERR_NUMERIC) return rc;
+        LbHostPar p;
+        const double ft = lb_gather(p, v, n, scale, h->h_res, h->h_params, res_map.data(), par_map.data());
+        lb_step(p, v, hd, ft, rc == HY_OK ? 1 : 0);
+        return HY_OK;
+    };
+    // the start point solves its spectra cold, so a fit does not depend on
+    // what the engine evaluated before it (warm starts refine the previous
+    // spectra, which can move the last bits of an evaluation)
+    h->warm_valid = false;
+    TRY(host_iteration());
+    int replays = 0;
+    if (captured && hd.status == HY_LBFGS_RUNNING) {
+        FitScratch fs;
+        fs.h = h;
+        const size_t sb = state.size() * sizeof(double), mb = sizeof(int32_t) * 2 * n;
+        HIP_TRY(h, hipMalloc(&fs.state, sb));
+        HIP_TRY(h, hipMalloc(&fs.hd, sizeof(hy_lbfgs_head)));
+        HIP_TRY(h, hipMalloc(&fs.maps, mb));
+        const size_t nres = 1 + (size_t)h->num_nodes + h->n_mat + h->n_comp + 5;
+        HIP_TRY(h, hipMalloc(&fs.params, sizeof(double) * pl.total));
+        HIP_TRY(h, hipMalloc(&fs.res, sizeof(double) * nres));
+        HIP_TRY(h, hipMalloc(&fs.status, sizeof(int) * 64));
+        std::vector<int32_t> maps(res_map);
+        maps.insert(maps.end(), par_map.begin(), par_map.end());
+        HIP_TRY(h, hipMemcpy(fs.state, state.data(), sb, hipMemcpyHostToDevice));
+        HIP_TRY(h, hipMemcpy(fs.hd, &hd, sizeof hd, hipMemcpyHostToDevice));
+        HIP_TRY(h, hipMemcpy(fs.maps, maps.data(), mb, hipMemcpyHostToDevice));
+        // the first captured evaluation reads the trial the host step chose
+        std::vector<double> prm(h->h_params, h->h_params + pl.total);
+        for (int i = 0; i < n; ++i) prm[par_map[i]] = lb_exp(v.xt[i]);
+        HIP_TRY(h, hipMemcpy(fs.params, prm.data(), sizeof(double) * pl.total, hipMemcpyHostToDevice));
+        LbArgs la{};
+        la.state = (double *)fs.state;
+        la.hd = (hy_lbfgs_head *)fs.hd;
+        la.res = (const double *)fs.res;
+        la.params = (double *)fs.params;
+        la.eig_status = (const int *)fs.status;
+        la.n_mat = h->n_mat;
+        la.scale = scale;
+        la.res_map = (const int32_t *)fs.maps;
+        la.par_map = (const int32_t *)fs.maps + n;
+        if (!acquire_graph_slot())
+            return fail(h, HY_ERR_NOMEM, "hy_fit_lbfgs: the process-wide cap on captured graphs is reached");
+        const int crc = precision == HY_PREC_FP64 ? capture_fit_iteration<double>(h, la, &fs.exec)
+                                                  : capture_fit_iteration<float>(h, la, &fs.exec);
+        if (crc != HY_OK) {
+            g_live_graphs.fetch_sub(1);
+            fs.exec = nullptr;
+            return crc;
+        }
+        // every trial costs one replay; a line search takes at most max_back + 1
+        const long cap = ((long)max_iter + 1) * (hd.max_back + 2) + 2;
+        hy_lbfgs_head hh = hd;
+        while (hh.status == HY_LBFGS_RUNNING && replays < cap) {
+            for (int j = 0; j < check_every; ++j) HIP_TRY(h, hipGraphLaunch(fs.exec, h->stream));
+            replays += check_every;
+            HIP_TRY(h, hipMemcpyAsync(&hh, fs.hd, sizeof hh, hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(h, hipStreamSynchronize(h->stream));
+        }
+        HIP_TRY(h, hipMemcpy(state.data(), fs.state, sb, hipMemcpyDeviceToHost));
+        // replays past the end re-evaluate x and leave the state unchanged
+        hd = hh;
+        h->part_ri = precision == HY_PREC_FP64 ? 1 : 0;
+        h->part_outside = true;
+        if (hd.status == HY_LBFGS_RUNNING)
+            return fail(h, HY_ERR_STATE, "hy_fit_lbfgs: %d replays without the optimizer finishing", replays);
+    } else {
+        while (hd.status == HY_LBFGS_RUNNING) TRY(host_iteration());
+    }
+    for (int i = 0; i < n; ++i) {
+        const double pv = lb_exp(v.x[i]);
+        const int par = par_map[i];
+        if (par >= pl.off_theta) theta[par - pl.off_theta] = pv;
+        else if (par >= pl.off_omega) omegas[par - pl.off_omega] = pv;
+        else bl[par - pl.off_bl] = pv;
+    }
+    if (out_logL) *out_logL = -hd.f / scale;
+    if (out_info) {
+        out_info[0] = hd.status;
+        out_info[1] = hd.iter;
+        out_info[2] = hd.n_eval;
+        out_info[3] = replays;
+    }
     return HY_OK;
 }
 

@@ -21,6 +21,7 @@
 // holds C[row][l%16] with row = 4*(l/16)+r for f32 and row = 4*r+(l/16) for
 // f64 (MF<R>::crow).
 #include "hy_internal.h"
+#include "hy_lbfgs.h"
 
 #include <cmath>
 #include <cstdio>
@@ -3418,3 +3419,74 @@ template void launch_adjoint<float>(const AdjArgs<float> &, int, hipStream_t);
 template void launch_adjoint<double>(const AdjArgs<double> &, int, hipStream_t);
 
 }  // namespace hy
+
+// ---------------------------------------------------------------------------
+// k_lbfgs: the optimizer step that closes a captured fit iteration (one wave;
+// the shared step logic with wave-wide reductions).
+// ---------------------------------------------------------------------------
+struct LbDevPar {
+    int tid;
+    double *w;  // LDS work area, 2 LB_MMAX^2 + 4 LB_MMAX doubles
+    __device__ double *work() { return w; }
+    template <class F>
+    __device__ void single(F f) {
+        if (tid == 0) f();
+    }
+    __device__ void sync() { __syncthreads(); }
+    template <class F>
+    __device__ void loop(int n, F f) {
+        for (int i = tid; i < n; i += LB_LANES) f(i);
+    }
+    template <class F>
+    __device__ double sum(int n, F f) {
+        LB_NOCONTRACT
+        double s = 0.0;
+        for (int i = tid; i < n; i += LB_LANES) s = s + f(i);
+        // lane t adds lane t + st: the host policy's pairwise tree
+        for (int st = LB_LANES / 2; st > 0; st >>= 1) s = s + __shfl_down(s, st);
+        return __shfl(s, 0);
+    }
+    template <int NQ, class F>
+    __device__ void sums(int n, double *out, F f) {
+        LB_NOCONTRACT
+        double acc[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+        for (int i = tid; i < n; i += LB_LANES) f(i, acc);
+        for (int st = LB_LANES / 2; st > 0; st >>= 1)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[q] = acc[q] + __shfl_down(acc[q], st);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) out[q] = __shfl(acc[q], 0);
+    }
+    template <class F>
+    __device__ double max(int n, F f) {
+        double s = 0.0;
+        for (int i = tid; i < n; i += LB_LANES) {
+            const double x = f(i);
+            s = x > s ? x : s;
+        }
+        for (int st = LB_LANES / 2; st > 0; st >>= 1) {
+            const double o = __shfl_down(s, st);
+            s = o > s ? o : s;
+        }
+        return __shfl(s, 0);
+    }
+};
+
+__global__ __launch_bounds__(LB_LANES) void k_lbfgs(LbArgs a) {
+    __shared__ double lb_work[2 * LB_MMAX * LB_MMAX + 4 * LB_MMAX];
+    LbDevPar p{(int)threadIdx.x, lb_work};
+    hy_lbfgs_head hd = *a.hd;  // every lane steps its own copy of the (uniform) scalars
+    const LbVec v = lb_vec(a.state, hd.n, hd.m);
+    int ok = 1;
+    for (int m = 0; m < a.n_mat; ++m)
+        if (a.eig_status[m] < 0) ok = 0;
+    const double ft = lb_gather(p, v, hd.n, a.scale, a.res, a.params, a.res_map, a.par_map);
+    lb_step(p, v, hd, ft, ok);
+    lb_scatter(p, v, hd.n, a.params, a.par_map);
+    __syncthreads();
+    if (threadIdx.x == 0) *a.hd = hd;
+}
+
+void launch_lbfgs(const LbArgs &a, hipStream_t s) { hipLaunchKernelGGL(k_lbfgs, dim3(1), dim3(LB_LANES), 0, s, a); }

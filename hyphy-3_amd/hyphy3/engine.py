@@ -38,7 +38,10 @@ EXPORTS = ("hy_create", "hy_destroy", "hy_last_error", "hy_set_problem", "hy_set
            "hy_fel_counters", "hy_mg94_spectrum", "hy_branch_eval", "hy_set_pattern_weights", "hy_network_create",
            "hy_network_destroy", "hy_network_last_error", "hy_network_set_problem", "hy_network_set_model",
            "hy_network_set_mixture", "hy_network_set_sink", "hy_network_eval", "hy_meme_eval", "hy_meme_drive",
-           "hy_meme_run")
+           "hy_meme_run", "hy_lbfgs_exp", "hy_lbfgs_state_doubles", "hy_lbfgs_init", "hy_lbfgs_step", "hy_fit_lbfgs")
+
+# optimizer status (hy_lbfgs_head.status)
+HY_LBFGS_RUNNING, HY_LBFGS_CONV_GRAD, HY_LBFGS_CONV_F, HY_LBFGS_MAX_ITER, HY_LBFGS_LS_FAIL, HY_LBFGS_NUMERIC = range(6)
 
 HY_FEL_FIELDS = 10
 FEL_FIELD_NAMES = ("alpha", "beta", "beta_nuisance", "alpha_null", "lrt", "p_value", "total_branch_length", "log_l_alt",
@@ -72,6 +75,15 @@ class BatchItem(C.Structure):
                 ("theta", _f64p), ("delta", C.c_double), ("psi", C.c_double), ("out_logL", _f64p),
                 ("out_site_ll", _f64p), ("out_grad_bl", _f64p), ("out_grad_omega", _f64p), ("out_grad_w", _f64p),
                 ("out_grad_theta", _f64p), ("status", C.c_int)]
+
+
+class LbfgsHead(C.Structure):
+    """hy_lbfgs_head of the C ABI."""
+    _fields_ = [("n", C.c_int32), ("m", C.c_int32), ("status", C.c_int32), ("iter", C.c_int32),
+                ("n_eval", C.c_int32), ("k", C.c_int32), ("head", C.c_int32), ("n_back", C.c_int32),
+                ("max_iter", C.c_int32), ("max_back", C.c_int32), ("pad", C.c_int32 * 2), ("f", C.c_double),
+                ("f_prev", C.c_double), ("alpha", C.c_double), ("gd", C.c_double), ("tol_grad", C.c_double),
+                ("tol_f", C.c_double), ("c1", C.c_double), ("pgnorm", C.c_double)]
 
 
 class HyError(RuntimeError):
@@ -143,6 +155,16 @@ def load_library(path: str = LIB_PATH):
                                              C.c_double, C.c_int, _f64p, _f64p, C.c_int, C.c_double, _f64p, _f64p]
     L.hy_autotune_batch.argtypes = [C.c_int, C.POINTER(BatchItem), C.c_int, C.c_uint, C.c_int, C.POINTER(C.c_int),
                                     _f64p]
+    L.hy_lbfgs_exp.restype = C.c_double
+    L.hy_lbfgs_exp.argtypes = [C.c_double]
+    L.hy_lbfgs_state_doubles.restype = C.c_size_t
+    L.hy_lbfgs_state_doubles.argtypes = [C.c_int, C.c_int]
+    L.hy_lbfgs_init.argtypes = [C.c_int, C.c_int, _f64p, C.POINTER(LbfgsHead), _f64p, _f64p, _f64p, C.c_int,
+                                C.c_double, C.c_double]
+    L.hy_lbfgs_step.argtypes = [_f64p, C.POINTER(LbfgsHead), C.c_double, _f64p, C.c_int, _f64p]
+    L.hy_fit_lbfgs.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_double, _f64p, _f64p, _f64p, C.c_double,
+                               C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _f64p,
+                               _i32p]
     _lib = L
     return L
 
@@ -606,10 +628,70 @@ class Engine:
         self._check(self.L.hy_last_checkpoint(self.h, C.byref(n), C.byref(b)))
         return n.value, b.value
 
+    def fit_lbfgs(self, bl, omegas, comp_w, theta, alpha=1.0, delta=0.0, psi=0.0, precision=HY_PREC_FP64,
+                  captured=True, scale=1.0, m=8, max_iter=500, tol_grad=1e-8, tol_f=1e-12, check_every=4):
+        """Maximum-likelihood fit of log bl (nodes with a parent), log omegas
+        and log theta by hy_fit_lbfgs (objective -scale * lnL).  captured:
+        every trial point after the start is one replay of a graph holding
+        the evaluation and the optimizer step.  Returns a dict with the final
+        bl, omegas, theta, logL, status, iterations, evaluations, replays."""
+        b = _f64(bl).copy()
+        om = _f64(omegas).copy()
+        th = _f64(theta).copy()
+        cw = _f64(comp_w)
+        if b.shape != (self.num_nodes,) or om.shape != (self.n_mat,) or th.shape != (5,) or cw.shape != (self.n_comp,):
+            raise ValueError("fit_lbfgs: bl, omegas, comp_w, theta must have num_nodes, n_mat, n_comp, 5 entries")
+        ll = np.zeros(1)
+        info = np.zeros(4, dtype=np.int32)
+        self._check(self.L.hy_fit_lbfgs(self.h, int(precision), int(bool(captured)), _p(b, _f64p), float(alpha),
+                                        _p(om, _f64p), _p(cw, _f64p), _p(th, _f64p), float(delta), float(psi),
+                                        float(scale), int(m), int(max_iter), float(tol_grad), float(tol_f),
+                                        int(check_every), _p(ll, _f64p), _p(info, _i32p)))
+        return {"bl": b, "omegas": om, "theta": th, "logL": float(ll[0]), "status": int(info[0]),
+                "iterations": int(info[1]), "evaluations": int(info[2]), "replays": int(info[3])}
+
     def device_bytes(self):
         v = C.c_size_t()
         self._check(self.L.hy_device_bytes(self.h, C.byref(v)))
         return v.value
+
+
+class Lbfgs:
+    """The engine's projected L-BFGS step on the host (hy_lbfgs_init /
+    hy_lbfgs_step) for any objective: ``minimize(fun, x0, lb, ub)`` with
+    ``fun(x) -> (f, g)``; the same step the captured fit takes on the GPU."""
+
+    def __init__(self, x0, lb, ub, m=8, max_iter=500, tol_grad=1e-7, tol_f=0.0):
+        self.L = load_library()
+        x0, lb, ub = _f64(x0), _f64(lb), _f64(ub)
+        self.n = x0.size
+        self.state = np.zeros(self.L.hy_lbfgs_state_doubles(self.n, m))
+        self.head = LbfgsHead()
+        rc = self.L.hy_lbfgs_init(self.n, m, _p(self.state, _f64p), C.byref(self.head), _p(x0, _f64p),
+                                  _p(lb, _f64p), _p(ub, _f64p), int(max_iter), float(tol_grad), float(tol_f))
+        if rc != HY_OK:
+            raise HyError(rc, "hy_lbfgs_init: invalid arguments")
+        self.xt = np.clip(x0, lb, ub)
+
+    def step(self, f, g, ok=True):
+        g = _f64(g)
+        xn = np.zeros(self.n)
+        rc = self.L.hy_lbfgs_step(_p(self.state, _f64p), C.byref(self.head), float(f), _p(g, _f64p), int(bool(ok)),
+                                  _p(xn, _f64p))
+        if rc != HY_OK:
+            raise HyError(rc, "hy_lbfgs_step: invalid state")
+        self.xt = xn
+        return xn
+
+    @property
+    def x(self):
+        return self.state[:self.n].copy()
+
+    def minimize(self, fun, max_evals=100000):
+        while self.head.status == HY_LBFGS_RUNNING and self.head.n_eval < max_evals:
+            f, g = fun(self.xt)
+            self.step(f, g, np.isfinite(f))
+        return self.x, self.head.f
 
 
 def fel_drive(invariant, tested_length, evaluate):

@@ -619,6 +619,67 @@ int hy_network_eval(hy_network n, int precision, unsigned flags, const double *b
                     double *out_grad_bl, double *out_grad_omega, double *out_grad_w, double *out_grad_theta,
                     double *out_grad_bridge_lengths, double *out_grad_bridge_weights);
 
+/* ---- Maximum-likelihood fit with captured optimizer iterations ----------
+ * Replaces the joint phase of MG94Fitter::fit (optimizer.hpp:934-1110):
+ * branch lengths, the omegas and the five GTR exchangeabilities in log space
+ * within the reference's box (optimizer.hpp:966-972, 986-987), minimised by
+ * a limited-memory quasi-Newton method (the reference drives
+ * LBFGSpp::LBFGSBSolver with m = 8, optimizer.hpp:1074-1087).  Here the
+ * method is a projected two-loop L-BFGS with a backtracking line search
+ * whose every decision runs on the GPU: one captured graph holds one
+ * evaluation (logL + full gradient) and the kernel that takes the optimizer
+ * step, so an iteration needs no host round trip.  The same step runs on the
+ * host through hy_lbfgs_step, bit for bit. */
+#define HY_LBFGS_RUNNING 0
+#define HY_LBFGS_CONV_GRAD 1 /* largest projected gradient component <= tol_grad */
+#define HY_LBFGS_CONV_F 2    /* |f_prev - f| <= tol_f * max(1, |f|)              */
+#define HY_LBFGS_MAX_ITER 3
+#define HY_LBFGS_LS_FAIL 4 /* no sufficient decrease after max_back halvings   */
+#define HY_LBFGS_NUMERIC 5 /* the start point did not evaluate to finite values */
+
+typedef struct hy_lbfgs_head {
+    int32_t n, m;        /* variables; correction pairs (1..8)                    */
+    int32_t status;      /* HY_LBFGS_*                                            */
+    int32_t iter;        /* accepted steps                                        */
+    int32_t n_eval;      /* evaluations consumed                                  */
+    int32_t k, head;     /* pairs held; ring position of the next pair            */
+    int32_t n_back;      /* backtracks of the current line search                 */
+    int32_t max_iter, max_back;
+    int32_t pad[2];
+    double f, f_prev;    /* objective at x and at the previous accepted point     */
+    double alpha, gd;    /* current step; g.d of the current direction            */
+    double tol_grad, tol_f, c1;
+    double pgnorm;       /* largest projected gradient component at x             */
+} hy_lbfgs_head;
+
+/* The map from a log parameter to the parameter the fits evaluate: exp(x)
+ * computed in the same operations on the host and on the GPU (within 2 ulp
+ * of the C library's exp). */
+double hy_lbfgs_exp(double x);
+/* Doubles of an optimizer state (x, g, d, xt, gt, lb, ub, m pairs). */
+size_t hy_lbfgs_state_doubles(int n, int m);
+/* Initialises a state: xt = x0 clamped to [lb, ub] is the first point to
+ * evaluate.  c1 = 1e-4, max_back = 20. */
+int hy_lbfgs_init(int n, int m, double *state, hy_lbfgs_head *hd, const double *x0, const double *lb,
+                  const double *ub, int max_iter, double tol_grad, double tol_f);
+/* One optimizer step on the host: consumes f_trial and the gradient
+ * g_trial[n] at the trial point (ok = 0: the evaluation failed) and leaves
+ * the next trial point in x_next[n].  Any of the output pointers may be
+ * NULL. */
+int hy_lbfgs_step(double *state, hy_lbfgs_head *hd, double f_trial, const double *g_trial, int ok, double *x_next);
+/* Fit on the engine's problem, mixture and time convention: the free
+ * variables are log bl of every node with a parent, log omega (n_mat), log
+ * theta (5); alpha, comp_w, delta and psi stay fixed.  The objective is
+ * -scale * lnL.  captured = 1: after the start point, every evaluation and
+ * step is one replay of a captured graph, with the host reading the status
+ * every check_every replays; captured = 0: hy_eval and hy_lbfgs_step
+ * alternate on the host (same iterates).  bl[num_nodes], omegas[n_mat] and
+ * theta[5] are read as the start and overwritten with the final point;
+ * out_info[4] = status, iterations, evaluations, replays. */
+int hy_fit_lbfgs(hy_engine h, int precision, int captured, double *bl, double alpha, double *omegas,
+                 const double *comp_w, double *theta, double delta, double psi, double scale, int m, int max_iter,
+                 double tol_grad, double tol_f, int check_every, double *out_logL, int32_t *out_info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "hyphy_mi355x.h")).read()
-    return sorted(set(re.findall(r"^(?:int|void|const char \*)\s*\*?\s*(hy_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|double|size_t|const char \*)\s*\*?\s*(hy_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_expected_entry_points():
