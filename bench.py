@@ -164,9 +164,23 @@ def cpu_baseline(cfg, sim, seconds):
         el = time.perf_counter() - t0
         if el >= seconds or n >= 200:
             break
-    return {"value": n / el, "unit": "evals/s", "cores": threads, "kind": "port",
+    return {"value": n / el, "unit": "evals/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"full {cfg['taxa']} taxa x {cfg['codons']} codons x K={K} problem, {n} logL+gradient evals "
-                      f"in {el:.1f} s (FP64 restatement of the reference algorithm, tile {tile}, OpenMP)"}
+                      f"in {el:.1f} s (FP64 restatement of the reference algorithm, tile {tile}, OpenMP; "
+                      f"register-blocked S x S by S x tile products, -O3 without -march as the reference builds)"}
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
 
 
 def main():

@@ -466,9 +466,28 @@ static void leaf_column(const or_tree *tr, int taxon, int pattern, double *D, in
     }
 }
 
-/* C[S x B] = P[S x S] * X[S x B] */
+/* C[S x B] = P[S x S] * X[S x B].  Four rows of C per pass share each row of
+ * X (register blocking); every element still sums over k in ascending order,
+ * so the result does not depend on the blocking. */
 static void gemm_px(int S, int B, const double *P, const double *X, double *C) {
-    for (int i = 0; i < S; ++i) {
+    int i = 0;
+    for (; i + 4 <= S; i += 4) {
+        double *c0 = C + (size_t)i * B, *c1 = c0 + B, *c2 = c1 + B, *c3 = c2 + B;
+        for (int b = 0; b < B; ++b) c0[b] = c1[b] = c2[b] = c3[b] = 0.0;
+        const double *p0 = P + (size_t)i * S, *p1 = p0 + S, *p2 = p1 + S, *p3 = p2 + S;
+        for (int k = 0; k < S; ++k) {
+            const double q0 = p0[k], q1 = p1[k], q2 = p2[k], q3 = p3[k];
+            const double *x = X + (size_t)k * B;
+            for (int b = 0; b < B; ++b) {
+                const double xv = x[b];
+                c0[b] += q0 * xv;
+                c1[b] += q1 * xv;
+                c2[b] += q2 * xv;
+                c3[b] += q3 * xv;
+            }
+        }
+    }
+    for (; i < S; ++i) {
         double *c = C + (size_t)i * B;
         for (int b = 0; b < B; ++b) c[b] = 0.0;
         const double *pr = P + (size_t)i * S;
@@ -480,15 +499,32 @@ static void gemm_px(int S, int B, const double *P, const double *X, double *C) {
     }
 }
 
-/* C[S x B] = P^T X */
+/* C[S x B] = P^T X, four rows of C per pass (each element sums over i in
+ * ascending order, as unblocked). */
 static void gemm_ptx(int S, int B, const double *P, const double *X, double *C) {
-    for (int j = 0; j < S * B; ++j) C[j] = 0.0;
-    for (int i = 0; i < S; ++i) {
-        const double *x = X + (size_t)i * B;
-        const double *pr = P + (size_t)i * S;
-        for (int j = 0; j < S; ++j) {
-            const double p = pr[j];
-            double *c = C + (size_t)j * B;
+    int j = 0;
+    for (; j + 4 <= S; j += 4) {
+        double *c0 = C + (size_t)j * B, *c1 = c0 + B, *c2 = c1 + B, *c3 = c2 + B;
+        for (int b = 0; b < B; ++b) c0[b] = c1[b] = c2[b] = c3[b] = 0.0;
+        for (int i = 0; i < S; ++i) {
+            const double *pr = P + (size_t)i * S + j;
+            const double q0 = pr[0], q1 = pr[1], q2 = pr[2], q3 = pr[3];
+            const double *x = X + (size_t)i * B;
+            for (int b = 0; b < B; ++b) {
+                const double xv = x[b];
+                c0[b] += q0 * xv;
+                c1[b] += q1 * xv;
+                c2[b] += q2 * xv;
+                c3[b] += q3 * xv;
+            }
+        }
+    }
+    for (; j < S; ++j) {
+        double *c = C + (size_t)j * B;
+        for (int b = 0; b < B; ++b) c[b] = 0.0;
+        for (int i = 0; i < S; ++i) {
+            const double p = P[(size_t)i * S + j];
+            const double *x = X + (size_t)i * B;
             for (int b = 0; b < B; ++b) c[b] += p * x[b];
         }
     }
