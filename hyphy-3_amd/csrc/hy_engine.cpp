@@ -4570,6 +4570,9 @@ int hy_fit_lbfgs(hy_engine h, int precision, int captured, double *bl, double al
     h->warm_valid = false;
     TRY(host_iteration());
     int replays = 0;
+    // engines that launch eagerly (members of a group, per-stage profiling)
+    // step on the host: same iterates, no captured graph
+    if (!h->use_graphs || h->profile) captured = 0;
     if (captured && hd.status == HY_LBFGS_RUNNING) {
         FitScratch fs;
         fs.h = h;

@@ -673,7 +673,8 @@ int hy_lbfgs_step(double *state, hy_lbfgs_head *hd, double f_trial, const double
  * -scale * lnL.  captured = 1: after the start point, every evaluation and
  * step is one replay of a captured graph, with the host reading the status
  * every check_every replays; captured = 0: hy_eval and hy_lbfgs_step
- * alternate on the host (same iterates).  bl[num_nodes], omegas[n_mat] and
+ * alternate on the host (same iterates; also what engines that launch
+ * eagerly do: hy_create_group members, profiled engines).  bl[num_nodes], omegas[n_mat] and
  * theta[5] are read as the start and overwritten with the final point;
  * out_info[4] = status, iterations, evaluations, replays. */
 int hy_fit_lbfgs(hy_engine h, int precision, int captured, double *bl, double alpha, double *omegas,

@@ -91,3 +91,28 @@ def test_captured_fit_argument_checks():
     r = eng.fit_lbfgs(bl, [0.5], [1.0], np.ones(5), max_iter=3, scale=1e-2)
     assert r["status"] in (1, 2, 3) and r["iterations"] <= 3
     eng.close()
+
+
+def test_group_engine_fit_steps_on_the_host():
+    """Members of an engine group launch eagerly: hy_fit_lbfgs steps them on
+    the host (no replays) with the iterates of a standalone engine."""
+    from hyphy3.engine import create_group
+    from hyphy3.fit import make_engine
+    import hyphy3
+    omegas, weights = [0.2, 1.5], [0.7, 0.3]
+    tree, aln, sim = sim_problem(10, 200, 44, omegas=omegas, weights=weights)
+    a = hyphy3.Alignment.from_text(sim["fasta"])
+    t = hyphy3.Tree.from_newick(sim["newick"], keep_rooted=False)
+    grp = create_group(2)
+    eng_g, arrays = make_engine(t, a, 2, engine=grp[0])
+    eng_s, _ = make_engine(t, a, 2)
+    bl0 = branch_lengths(tree) * 1.5
+    kw = dict(scale=1.0 / float(np.sum(arrays["weights"])), max_iter=100, tol_grad=1e-8, tol_f=1e-12, precision=0)
+    rg = eng_g.fit_lbfgs(bl0, [0.5, 2.0], weights, np.ones(5), captured=True, **kw)
+    rs = eng_s.fit_lbfgs(bl0, [0.5, 2.0], weights, np.ones(5), captured=True, **kw)
+    print("group", rg["logL"], rg["iterations"], rg["replays"], "standalone", rs["logL"], rs["iterations"], rs["replays"])
+    assert rg["replays"] == 0 and rs["replays"] > 0
+    assert rg["status"] == rs["status"] and rg["iterations"] == rs["iterations"]
+    assert abs(rg["logL"] - rs["logL"]) <= 1e-9 * abs(rs["logL"])
+    for e in list(grp) + [eng_s]:
+        e.close()
